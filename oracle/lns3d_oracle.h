@@ -1,0 +1,98 @@
+/*
+ * CPU restatement (the oracle) of the LNS3D implicit approximate-factorization
+ * step: block-pentadiagonal line solvers, finite-difference RHS, LHS assembly,
+ * boundary conditions, coefficient generation and time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests, the smoke check and the CPU-baseline
+ * leg of the benchmark may load it, and only as the checker.  The product
+ * library (lns3d_amd) never links or calls it.
+ *
+ * Arrays use the reference's Fortran (column-major) layouts and its 1-based
+ * index meaning, stored 0-based:
+ *   v, r      complex (5, nx, ny)            v[d + 5*(i + nx*j)]
+ *   Ah,Bh,..  real    (5, 5, nx, ny)         a[row + 5*col + 25*(i + nx*j)]
+ *   Vh, ABhi  real    (6, nx, ny)
+ *   mat       complex (5, 5, 5, nx, ny)      mat[k + 5*row + 25*col + 125*p]
+ * Every function states the reference file:line it follows.
+ */
+#ifndef LNS3D_ORACLE_H
+#define LNS3D_ORACLE_H
+#include <complex.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double _Complex zdouble;
+
+/* Scheme, physics and boundary selectors: the module state of src/global.f90
+ * that the step routines read. */
+typedef struct {
+  int nx, ny;
+  int optx, opty;
+  int xper, yper;
+  int wall, wallt, top, left, right;
+  int linear, mattyp, loctime;
+  double Ma, Re, Pr, gamma, kz, omega;
+  double cflmax, dtmax;
+  double alfa;
+  int periodic_exact; /* 1: exact LU for periodic lines; 0: shipped slot order */
+} orc_params;
+
+/* ---- block-pentadiagonal line solvers (src/cpenta1bc.f, cpenta2bc.f,
+ *      cpenta1p.f, cpenta2p.f) ---------------------------------------------- */
+/* layout 1: mat(5,5,5,nsys,n), rhs(5,nsys,n)   (cpenta1bc / cpenta1p)
+ * layout 2: mat(5,5,5,n,nsys), rhs(5,n,nsys)   (cpenta2bc / cpenta2p)
+ * code 0 factor, 1 solve, 2 both.  Returns 0, or -1 on bad arguments. */
+int orc_cpenta_bc(int layout, int nsys, int n, zdouble *mat, zdouble *rhs, int code);
+int orc_cpenta_p(int layout, int nsys, int np, zdouble *mat, zdouble *rhs,
+                 zdouble *per, zdouble *per2, int code, int exact);
+
+/* ---- stencils (src/cgrad.f90, src/cgrad2.f90, src/grad.f90, grad2.f90) ---- */
+void orc_cgrad(const orc_params *p, const zdouble *v, zdouble *g1, zdouble *g2);
+void orc_cgrad2(const orc_params *p, const zdouble *v, const zdouble *g1,
+                zdouble *g11, zdouble *g12, zdouble *g22);
+void orc_grad(const orc_params *p, int ndof, const double *v, double *g1, double *g2);
+void orc_grad2(const orc_params *p, int ndof, const double *v, const double *g1,
+               double *g11, double *g12, double *g22);
+
+/* coefficient set produced by src/genmtrx.f90 */
+typedef struct {
+  double *Ah, *Bh, *Dh, *Dhi;     /* (5,5,nx,ny) */
+  double *Vh11, *Vh12, *Vh22;     /* (6,nx,ny) */
+  double *ABhi;                   /* (6,nx,ny) */
+} orc_coeffs;
+
+/* metrics: m1,m2,n1,n2,m11,m12,m22,n11,n12,n22, each (nx,ny), stacked */
+int orc_genmtrx(const orc_params *p, const double *vm, const double *metric,
+                orc_coeffs *c);
+int orc_set_time_step(const orc_params *p, const double *vm, const double *metric,
+                      double *dtl);
+
+/* ---- one implicit step pieces (src/lrhs3d.f90, impdrv3d.f90, lhs*f3d.f90,
+ *      lhsbt*f3d.f90, lhsbc*f3d.f90, rhsbc3d.f90, itrbc3d.f90) ------------- */
+int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
+               zdouble *r, zdouble *work /* 5 arrays of (5,nx,ny) */);
+void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v,
+               const zdouble *vold, const double *dtl);
+int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl,
+            zdouble *mat);
+int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v);
+int orc_itrbc3d(const orc_params *p, zdouble *v);
+
+/* Factor both directions (dir 1 into the transposed cpenta1bc layout that
+ * src/impdrv3d.f90:257-269 uses, dir 2 in place).  matx is (5,5,5,ny,nx),
+ * maty is (5,5,5,nx,ny); per* arrays are (2,5,5,nx,ny), only for periodic. */
+int orc_factor(const orc_params *p, const orc_coeffs *c, const double *dtl,
+               zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+               zdouble *pery, zdouble *per2y);
+/* One step given the factors: v is updated in place, dv receives r.  work
+ * holds 7 arrays of (5,nx,ny). */
+int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
+             zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+             zdouble *pery, zdouble *per2y, zdouble *v, zdouble *dv,
+             zdouble *work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""ctypes front-end to the oracle: the C restatement (liboracle.so) and, where
+it was built, the compiled reference (_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests, by the smoke check and by the
+benchmark's CPU-baseline leg, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VP = ctypes.c_void_p
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags["F_CONTIGUOUS"] or a.ndim <= 1, "arrays must be Fortran-ordered"
+    return a.ctypes.data_as(VP)
+
+
+class _OrcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("nx", "ny", "optx", "opty", "xper", "yper", "wall", "wallt", "top",
+                 "left", "right", "linear", "mattyp", "loctime")] + \
+               [(n, ctypes.c_double) for n in
+                ("Ma", "Re", "Pr", "gamma", "kz", "omega", "cflmax", "dtmax", "alfa")] + \
+               [("periodic_exact", ctypes.c_int)]
+
+
+class _OrcCoeffs(ctypes.Structure):
+    _fields_ = [(n, VP) for n in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi")]
+
+
+def coeff_arrays(nx, ny):
+    z = lambda *s: np.zeros(s, order="F")
+    return dict(Ah=z(5, 5, nx, ny), Bh=z(5, 5, nx, ny), Dh=z(5, 5, nx, ny),
+                Dhi=z(5, 5, nx, ny), Vh11=z(6, nx, ny), Vh12=z(6, nx, ny),
+                Vh22=z(6, nx, ny), ABhi=z(6, nx, ny))
+
+
+class Oracle:
+    """The C restatement."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        self.lib = ctypes.CDLL(path)
+
+    def params(self, p):
+        s = _OrcParams()
+        for name, _ in _OrcParams._fields_:
+            setattr(s, name, getattr(p, name))
+        return s
+
+    @staticmethod
+    def _coeffs(c):
+        s = _OrcCoeffs()
+        for k in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi"):
+            setattr(s, k, _ptr(c[k]))
+        return s
+
+    def cpenta_bc(self, layout, mat, rhs, code):
+        nsys, n = (mat.shape[3], mat.shape[4]) if layout == 1 else (mat.shape[4], mat.shape[3])
+        rc = self.lib.orc_cpenta_bc(layout, nsys, n, _ptr(mat), _ptr(rhs), code)
+        assert rc == 0
+
+    def cpenta_p(self, layout, mat, rhs, per, per2, code, exact=0):
+        nsys, np_ = (mat.shape[3], mat.shape[4]) if layout == 1 else (mat.shape[4], mat.shape[3])
+        rc = self.lib.orc_cpenta_p(layout, nsys, np_, _ptr(mat), _ptr(rhs), _ptr(per),
+                                   _ptr(per2), code, exact)
+        assert rc == 0
+
+    def genmtrx(self, p, vm, met):
+        c = coeff_arrays(p.nx, p.ny)
+        cs = self._coeffs(c)
+        ps = self.params(p)
+        rc = self.lib.orc_genmtrx(ctypes.byref(ps), _ptr(vm), _ptr(met), ctypes.byref(cs))
+        assert rc == 0, "unsupported options"
+        return c
+
+    def set_time_step(self, p, vm, met):
+        dtl = np.zeros((p.nx, p.ny), order="F")
+        ps = self.params(p)
+        assert self.lib.orc_set_time_step(ctypes.byref(ps), _ptr(vm), _ptr(met), _ptr(dtl)) == 0
+        return dtl
+
+    def lrhs3d(self, p, c, v):
+        r = np.zeros_like(v, order="F")
+        work = np.zeros(v.shape + (5,), complex, order="F")
+        ps, cs = self.params(p), self._coeffs(c)
+        assert self.lib.orc_lrhs3d(ctypes.byref(ps), ctypes.byref(cs), _ptr(v), _ptr(r),
+                                   _ptr(work)) == 0
+        return r, work
+
+    def rbe3d(self, p, r, v, vold, dtl):
+        ps = self.params(p)
+        self.lib.orc_rbe3d(ctypes.byref(ps), _ptr(r), _ptr(v), _ptr(vold), _ptr(dtl))
+
+    def lhs(self, p, direction, c, dtl):
+        mat = np.zeros((5, 5, 5, p.nx, p.ny), complex, order="F")
+        ps, cs = self.params(p), self._coeffs(c)
+        assert self.lib.orc_lhs(ctypes.byref(ps), direction, ctypes.byref(cs), _ptr(dtl),
+                                _ptr(mat)) == 0
+        return mat
+
+    def rhsbc3d(self, p, r, v):
+        ps = self.params(p)
+        assert self.lib.orc_rhsbc3d(ctypes.byref(ps), _ptr(r), _ptr(v)) == 0
+
+    def itrbc3d(self, p, v):
+        ps = self.params(p)
+        assert self.lib.orc_itrbc3d(ctypes.byref(ps), _ptr(v)) == 0
+
+    def factor(self, p, c, dtl):
+        nx, ny = p.nx, p.ny
+        f = dict(matx=np.zeros((5, 5, 5, ny, nx) if not p.xper else (5, 5, 5, nx, ny),
+                               complex, order="F"),
+                 maty=np.zeros((5, 5, 5, nx, ny), complex, order="F"),
+                 perx=np.zeros((2, 5, 5, nx, ny), complex, order="F"),
+                 per2x=np.zeros((2, 5, 5, nx, ny), complex, order="F"),
+                 pery=np.zeros((2, 5, 5, nx, ny), complex, order="F"),
+                 per2y=np.zeros((2, 5, 5, nx, ny), complex, order="F"))
+        ps, cs = self.params(p), self._coeffs(c)
+        rc = self.lib.orc_factor(ctypes.byref(ps), ctypes.byref(cs), _ptr(dtl),
+                                 *(_ptr(f[k]) for k in ("matx", "maty", "perx", "per2x",
+                                                         "pery", "per2y")))
+        assert rc == 0
+        return f
+
+    def step(self, p, c, dtl, f, v):
+        """one step; v updated in place; returns the increment dv"""
+        dv = np.zeros_like(v, order="F")
+        work = np.zeros(v.shape + (7,), complex, order="F")
+        ps, cs = self.params(p), self._coeffs(c)
+        rc = self.lib.orc_step(ctypes.byref(ps), ctypes.byref(cs), _ptr(dtl),
+                               *(_ptr(f[k]) for k in ("matx", "maty", "perx", "per2x",
+                                                       "pery", "per2y")),
+                               _ptr(v), _ptr(dv), _ptr(work))
+        assert rc == 0
+        return dv
+
+
+class Reference:
+    """The reference's own routines, compiled by build_ref.sh (this container
+    only; absent where the reference sources are absent)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_ref", "libref.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+
+    @staticmethod
+    def available(path=None):
+        return os.path.exists(path or os.path.join(HERE, "_ref", "libref.so"))
+
+    def setup(self, p, x, y, met, vm, usecalcd=0):
+        ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, 0, 0, p.wall,
+                       p.wallt, p.top, p.left, p.right, p.linear, 1, p.loctime, p.mattyp,
+                       0, 1, usecalcd], dtype=np.int32)
+        rp = np.array([p.Ma, p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax, p.dtmax,
+                       0.0, 0.0])
+        self.lib.ref_init(ip.ctypes.data_as(VP), rp.ctypes.data_as(VP))
+        self.lib.ref_set_grid(_ptr(x), _ptr(y), _ptr(met))
+        self.lib.ref_set_mean(_ptr(vm))
+        self.p = p
+
+    def genmtrx(self):
+        self.lib.ref_genmtrx()
+        c = coeff_arrays(self.p.nx, self.p.ny)
+        self.lib.ref_get_coeffs(*(_ptr(c[k]) for k in
+                                  ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi")))
+        return c
+
+    def set_coeffs(self, c):
+        self.lib.ref_set_coeffs(*(_ptr(c[k]) for k in
+                                  ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi")))
+
+    def set_time_step(self):
+        dtl = np.zeros((self.p.nx, self.p.ny), order="F")
+        self.lib.ref_set_time_step(_ptr(dtl))
+        return dtl
+
+    def set_dtl(self, dtl):
+        self.lib.ref_set_dtl(_ptr(dtl))
+
+    def lrhs3d(self, v):
+        r = np.zeros_like(v, order="F")
+        g = np.zeros(v.shape + (5,), complex, order="F")
+        self.lib.ref_lrhs3d(_ptr(v), _ptr(r), _ptr(g))
+        return r, g
+
+    def rbe3d(self, r, v, vold):
+        self.lib.ref_rbe3d(_ptr(r), _ptr(v), _ptr(vold))
+
+    def rhsbc3d(self, r, v):
+        self.lib.ref_rhsbc3d(_ptr(r), _ptr(v))
+
+    def itrbc3d(self, v):
+        self.lib.ref_itrbc3d(_ptr(v))
+
+    def lhs(self, direction, alfa=1.0):
+        mat = np.zeros((5, 5, 5, self.p.nx, self.p.ny), complex, order="F")
+        self.lib.ref_lhs(ctypes.c_int(direction), ctypes.c_double(alfa), _ptr(mat))
+        return mat
+
+    def step(self, v, first):
+        dv = np.zeros_like(v, order="F")
+        self.lib.ref_step(_ptr(v), _ptr(dv), ctypes.c_int(1 if first else 0))
+        return dv
+
+    def _call(self, name, *args):
+        cargs = [ctypes.byref(ctypes.c_int(a)) if isinstance(a, int) else _ptr(a) for a in args]
+        getattr(self.lib, name)(*cargs)
+
+    def cpenta1bc(self, mat, rhs, code, shipped=False):
+        self._call("cpenta1bc_shipped_" if shipped else "cpenta1bc_",
+                   mat.shape[3], mat.shape[4], 5, mat, rhs, code)
+
+    def cpenta2bc(self, mat, rhs, code, shipped=False):
+        self._call("cpenta2bc_shipped_" if shipped else "cpenta2bc_",
+                   mat.shape[3], mat.shape[4], 5, mat, rhs, code)
+
+    def cpenta1p(self, mat, rhs, per, per2, code):
+        self._call("cpenta1p_", mat.shape[3], mat.shape[4], 5, mat, rhs, per, per2, code)
+
+    def cpenta2p(self, mat, rhs, per, per2, code):
+        self._call("cpenta2p_", mat.shape[3], mat.shape[4], 5, mat, rhs, per, per2, code)

@@ -1,0 +1,331 @@
+!=============================================================================!
+! Direct-call harness over the reference's own Fortran routines
+! (sscollis/lns3d src/).  Test infrastructure only: it is linked into
+! oracle/_ref/libref.so by oracle/build_ref.sh and called from tests through
+! ctypes.  It replaces the file-reading start-up of src/lns3d.f90:61-149
+! (input, gengrid, genmean) by setters, then calls the reference routines
+! exactly as src/impdrv3d.f90:130-356 does for impl=1, niter=1, iLHS=1.
+!=============================================================================!
+module refdrv_state
+  use iso_c_binding
+  implicit none
+  complex, allocatable :: matx(:,:,:,:,:), maty(:,:,:,:,:)
+  complex, allocatable :: lmatx(:,:,:,:,:), lr(:,:,:)
+  complex, allocatable :: perx(:,:,:,:,:), per2x(:,:,:,:,:)
+  complex, allocatable :: pery(:,:,:,:,:), per2y(:,:,:,:,:)
+  real, allocatable    :: dtlh(:,:)
+end module refdrv_state
+
+!-----------------------------------------------------------------------------
+! ip: 1 nx, 2 ny, 3 optx, 4 opty, 5 carp, 6 xper, 7 yper, 8 lsym, 9 rsym,
+!     10 wall, 11 wallt, 12 top, 13 left, 14 right, 15 linear, 16 impl,
+!     17 loctime, 18 mattyp, 19 ispg, 20 navier, 21 usecalcd
+! rp: 1 Ma, 2 Re, 3 Pr, 4 gamma, 5 kz, 6 omega, 7 cflmax, 8 dtmax, 9 eps_e,
+!     10 x0
+!-----------------------------------------------------------------------------
+subroutine ref_init(ip, rp) bind(C, name="ref_init")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  use refdrv_state
+  implicit none
+  integer(c_int), intent(in) :: ip(21)
+  real(c_double), intent(in) :: rp(10)
+  integer :: i, j
+
+  nx = ip(1); ny = ip(2); nz = 1; ndof = 5
+  optx = ip(3); opty = ip(4)
+  carp = ip(5) /= 0; xper = ip(6) /= 0; yper = ip(7) /= 0
+  lsym = ip(8) /= 0; rsym = ip(9) /= 0; bsym = .false.; tsym = .false.
+  wall = ip(10); wallt = ip(11); top = ip(12); left = ip(13); right = ip(14)
+  linear = ip(15); impl = ip(16); loctime = ip(17); mattyp = ip(18)
+  ispg = ip(19); Navier = ip(20) /= 0; useCalcd = ip(21) /= 0
+  Ma = rp(1); Re = rp(2); Pr = rp(3); gamma = rp(4); gamma1 = gamma - one
+  kz = rp(5); omega = rp(6); cflmax = rp(7); dtmax = rp(8); eps_e = rp(9)
+  x0 = rp(10)
+  iLHS = 1; niter = 1; ires = 0; idamp = 0; general = .true.
+  complex_analysis = (kz /= zero) .or. (linear == 2)
+  is = 1; ie = nx; js = 1; je = ny
+  istep = 0; iter = 1; lstep = 0; alfa = one; time = zero
+
+  dxi = one / float(nx-1)
+  deta = one / float(ny-1)
+
+  if (allocated(x)) then
+    deallocate(x, y, m1, m2, n1, n2, m11, m12, m22, n11, n12, n22)
+    deallocate(m1m1, m1m2, m2m2, n1n1, n1n2, n2n2, m1n1, m1n2, m2n1, m2n2)
+    deallocate(bnl, bnr, bnb, bnt, vm)
+    deallocate(g1v, g2v, g11v, g12v, g22v, Ah, Bh, Dh, Vh11, Vh12, Vh22)
+    deallocate(c1v, c2v, c11v, c12v, c22v, Dhi, ABhi)
+  end if
+  allocate(x(nx,ny), y(nx,ny), m1(nx,ny), m2(nx,ny), n1(nx,ny), n2(nx,ny))
+  allocate(m11(nx,ny), m12(nx,ny), m22(nx,ny), n11(nx,ny), n12(nx,ny), n22(nx,ny))
+  allocate(m1m1(nx,ny), m1m2(nx,ny), m2m2(nx,ny), n1n1(nx,ny), n1n2(nx,ny), n2n2(nx,ny))
+  allocate(m1n1(nx,ny), m1n2(nx,ny), m2n1(nx,ny), m2n2(nx,ny))
+  allocate(bnl(ny,2), bnr(ny,2), bnb(nx,2), bnt(nx,2), vm(ndof,nx,ny))
+  call mlocal2d
+  call mlocal3d
+  x = zero; y = zero; vm = zero
+  if (allocated(dtlh)) deallocate(dtlh)
+  allocate(dtlh(nx,ny))
+  dtlh = zero
+end subroutine ref_init
+
+! grid and metrics (each nx*ny, Fortran order), then the products as in
+! src/gengrid.f90:84-155
+subroutine ref_set_grid(xl, yl, mt) bind(C, name="ref_set_grid")
+  use iso_c_binding
+  use global
+  implicit none
+  real(c_double), intent(in) :: xl(nx,ny), yl(nx,ny), mt(nx,ny,10)
+  integer :: i, j
+  x = xl; y = yl
+  m1 = mt(:,:,1); m2 = mt(:,:,2); n1 = mt(:,:,3); n2 = mt(:,:,4)
+  m11 = mt(:,:,5); m12 = mt(:,:,6); m22 = mt(:,:,7)
+  n11 = mt(:,:,8); n12 = mt(:,:,9); n22 = mt(:,:,10)
+  m1m1 = m1*m1; m1m2 = m1*m2; m2m2 = m2*m2
+  n1n1 = n1*n1; n1n2 = n1*n2; n2n2 = n2*n2
+  m1n1 = m1*n1; m1n2 = m1*n2; m2n1 = m2*n1; m2n2 = m2*n2
+  do i = 1, nx
+    bnb(i,1) = -n1(i,1) / sqrt( n1(i,1)**2 + n2(i,1)**2 )
+    bnb(i,2) = -n2(i,1) / sqrt( n1(i,1)**2 + n2(i,1)**2 )
+    bnt(i,1) = n1(i,ny) / sqrt( n1(i,ny)**2 + n2(i,ny)**2 )
+    bnt(i,2) = n2(i,ny) / sqrt( n1(i,ny)**2 + n2(i,ny)**2 )
+  end do
+  do j = 1, ny
+    bnl(j,1) = -m1(1,j) / sqrt( m1(1,j)**2 + m2(1,j)**2 )
+    bnl(j,2) = -m2(1,j) / sqrt( m1(1,j)**2 + m2(1,j)**2 )
+    bnr(j,1) = m1(nx,j) / sqrt( m1(nx,j)**2 + m2(nx,j)**2 )
+    bnr(j,2) = m2(nx,j) / sqrt( m1(nx,j)**2 + m2(nx,j)**2 )
+  end do
+end subroutine ref_set_grid
+
+subroutine ref_set_mean(vml) bind(C, name="ref_set_mean")
+  use iso_c_binding
+  use global
+  implicit none
+  real(c_double), intent(in) :: vml(ndof,nx,ny)
+  vm = vml
+end subroutine ref_set_mean
+
+! coefficient generation (src/genmtrx.f90) and time step (src/temporal.f90)
+subroutine ref_genmtrx() bind(C, name="ref_genmtrx")
+  use global
+  implicit none
+  real, allocatable :: rl(:,:,:)
+  allocate(rl(ndof,nx,ny))
+  rl = zero
+  call genmtrx(vm, kz, rl)
+  deallocate(rl)
+end subroutine ref_genmtrx
+
+subroutine ref_set_time_step(dtl) bind(C, name="ref_set_time_step")
+  use iso_c_binding
+  use global
+  use temporal
+  use refdrv_state
+  implicit none
+  real(c_double), intent(out) :: dtl(nx,ny)
+  if (allocated(volinv)) deallocate(volinv, sxj, syj, sxi, acoustic, viscous, syi)
+  istep = 0
+  call set_time_step(vm, dtlh)
+  dtl = dtlh
+end subroutine ref_set_time_step
+
+subroutine ref_set_dtl(dtl) bind(C, name="ref_set_dtl")
+  use iso_c_binding
+  use global
+  use refdrv_state
+  implicit none
+  real(c_double), intent(in) :: dtl(nx,ny)
+  dtlh = dtl
+end subroutine ref_set_dtl
+
+! coefficient arrays out: Ah,Bh,Dh,Dhi (25 each), Vh11,Vh12,Vh22,ABhi (6 each)
+subroutine ref_get_coeffs(a, b, d, di, v11, v12, v22, abi) bind(C, name="ref_get_coeffs")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  implicit none
+  real(c_double), intent(out) :: a(ndof,ndof,nx,ny), b(ndof,ndof,nx,ny)
+  real(c_double), intent(out) :: d(ndof,ndof,nx,ny), di(ndof,ndof,nx,ny)
+  real(c_double), intent(out) :: v11(6,nx,ny), v12(6,nx,ny), v22(6,nx,ny), abi(6,nx,ny)
+  a = Ah; b = Bh; d = Dh; di = Dhi
+  v11 = Vh11; v12 = Vh12; v22 = Vh22; abi = ABhi
+end subroutine ref_get_coeffs
+
+subroutine ref_set_coeffs(a, b, d, di, v11, v12, v22, abi) bind(C, name="ref_set_coeffs")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  implicit none
+  real(c_double), intent(in) :: a(ndof,ndof,nx,ny), b(ndof,ndof,nx,ny)
+  real(c_double), intent(in) :: d(ndof,ndof,nx,ny), di(ndof,ndof,nx,ny)
+  real(c_double), intent(in) :: v11(6,nx,ny), v12(6,nx,ny), v22(6,nx,ny), abi(6,nx,ny)
+  Ah = a; Bh = b; Dh = d; Dhi = di
+  Vh11 = v11; Vh12 = v12; Vh22 = v22; ABhi = abi
+end subroutine ref_set_coeffs
+
+! lrhs3D with the derivative arrays it leaves in module local3d
+subroutine ref_lrhs3d(v, r, g) bind(C, name="ref_lrhs3d")
+  use iso_c_binding
+  use global
+  use local3d
+  implicit none
+  complex(c_double_complex), intent(in)  :: v(ndof,nx,ny)
+  complex(c_double_complex), intent(out) :: r(ndof,nx,ny)
+  complex(c_double_complex), intent(out) :: g(ndof,nx,ny,5)
+  complex, allocatable :: vl(:,:,:)
+  allocate(vl(ndof,nx,ny))
+  vl = v
+  call lrhs3D(vl, r, vm, x, y)
+  g(:,:,:,1) = c1v; g(:,:,:,2) = c2v
+  g(:,:,:,3) = c11v; g(:,:,:,4) = c12v; g(:,:,:,5) = c22v
+  deallocate(vl)
+end subroutine ref_lrhs3d
+
+subroutine ref_rbe3d(r, v, vold) bind(C, name="ref_rbe3d")
+  use iso_c_binding
+  use global
+  use refdrv_state
+  implicit none
+  complex(c_double_complex), intent(inout) :: r(ndof,nx,ny)
+  complex(c_double_complex), intent(in) :: v(ndof,nx,ny), vold(ndof,nx,ny)
+  call rbe3D(r, v, vold, dtlh)
+end subroutine ref_rbe3d
+
+subroutine ref_rhsbc3d(r, v) bind(C, name="ref_rhsbc3d")
+  use iso_c_binding
+  use global
+  implicit none
+  complex(c_double_complex), intent(inout) :: r(ndof,nx,ny)
+  complex(c_double_complex), intent(in) :: v(ndof,nx,ny)
+  complex, allocatable :: vl(:,:,:)
+  allocate(vl(ndof,nx,ny))
+  vl = v
+  call rhsBC3D(r, vl, vm)
+  deallocate(vl)
+end subroutine ref_rhsbc3d
+
+subroutine ref_itrbc3d(v) bind(C, name="ref_itrbc3d")
+  use iso_c_binding
+  use global
+  implicit none
+  complex(c_double_complex), intent(inout) :: v(ndof,nx,ny)
+  call itrBC3D(v, vm)
+end subroutine ref_itrbc3d
+
+! assembled (unfactored) LHS blocks, mat(5,ndof,ndof,nx,ny); dir = 1 or 2
+subroutine ref_lhs(dir, alfa_in, mat) bind(C, name="ref_lhs")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  use refdrv_state
+  implicit none
+  integer(c_int), value :: dir
+  real(c_double), value :: alfa_in
+  complex(c_double_complex), intent(out) :: mat(5,ndof,ndof,nx,ny)
+  complex, allocatable :: vl(:,:,:)
+  allocate(vl(ndof,nx,ny))
+  vl = zero
+  alfa = alfa_in
+  if (dir == 1) then
+    call lhs1f3D(mat, Ah, Dh, Dhi, Vh11, ABhi, dtlh, vl)
+  else
+    call lhs2f3D(mat, Bh, Dh, Dhi, Vh22, ABhi, dtlh, vl)
+  end if
+  deallocate(vl)
+end subroutine ref_lhs
+
+! One implicit step following src/impdrv3d.f90:146-356 (impl=1, niter=1,
+! USE_CPENTA1BC path for non-periodic x).  first /= 0 forms and factors the
+! LHS (as at istep=1, iter=1).  On return v holds the new field and dv the
+! solution increment r that was added.
+subroutine ref_step(v, dv, first) bind(C, name="ref_step")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  use refdrv_state
+  implicit none
+  complex(c_double_complex), intent(inout) :: v(ndof,nx,ny)
+  complex(c_double_complex), intent(out) :: dv(ndof,nx,ny)
+  integer(c_int), value :: first
+  complex, allocatable :: vold(:,:,:), r(:,:,:)
+  integer :: i, j, idof, jdof
+
+  allocate(vold(ndof,nx,ny), r(ndof,nx,ny))
+  if (first /= 0) then
+    if (allocated(matx)) deallocate(matx, maty, lmatx, lr)
+    allocate(matx(5,ndof,ndof,nx,ny), maty(5,ndof,ndof,nx,ny))
+    allocate(lmatx(5,ndof,ndof,ny,nx), lr(ndof,ny,nx))
+    if (xper .or. yper) then
+      if (allocated(perx)) deallocate(perx, per2x, pery, per2y)
+      allocate(perx(2,ndof,ndof,nx,ny), per2x(2,ndof,ndof,nx,ny))
+      allocate(pery(2,ndof,ndof,nx,ny), per2y(2,ndof,ndof,nx,ny))
+    end if
+  end if
+  istep = 1; iter = 1
+  vold = v
+  call itrBC3D(v, vm)
+  call lrhs3D(v, r, vm, x, y)
+  call rbe3D(r, v, vold, dtlh)
+  alfa = one
+  call rhsBC3D(r, v, vm)
+
+  if (first /= 0) then
+    call lhs1f3D(matx, Ah, Dh, Dhi, Vh11, ABhi, dtlh, v)
+    if (xper) then
+      call cpenta2p(nx, ny, ndof, matx, r, perx, per2x, 0)
+    else
+      do j = 1, ny
+        do i = 1, nx
+          do idof = 1, ndof
+            do jdof = 1, ndof
+              lmatx(:,idof,jdof,j,i) = matx(:,idof,jdof,i,j)
+            end do
+          end do
+        end do
+      end do
+      call cpenta1bc(ny, nx, ndof, lmatx, lr, 0)
+    end if
+  end if
+  if (xper) then
+    call cpenta2p(nx, ny, ndof, matx, r, perx, per2x, 1)
+  else
+    do j = 1, ny
+      do i = 1, nx
+        lr(:,j,i) = r(:,i,j)
+      end do
+    end do
+    call cpenta1bc(ny, nx, ndof, lmatx, lr, 1)
+    do j = 1, ny
+      do i = 1, nx
+        r(:,i,j) = lr(:,j,i)
+      end do
+    end do
+  end if
+
+  if (first /= 0) then
+    call lhs2f3D(maty, Bh, Dh, Dhi, Vh22, ABhi, dtlh, v)
+    if (yper) then
+      call cpenta1p(nx, ny, ndof, maty, r, pery, per2y, 0)
+    else
+      call cpenta1bc(nx, ny, ndof, maty, r, 0)
+    end if
+  end if
+  if (yper) then
+    call cpenta1p(nx, ny, ndof, maty, r, pery, per2y, 1)
+  else
+    call cpenta1bc(nx, ny, ndof, maty, r, 1)
+  end if
+
+  v = v + r
+  dv = r
+  call itrBC3D(v, vm)
+  deallocate(vold, r)
+end subroutine ref_step

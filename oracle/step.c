@@ -1,0 +1,846 @@
+/*
+ * Oracle: finite-difference derivatives, linearized RHS, LHS assembly with
+ * boundary closures and boundary conditions, coefficient generation, time
+ * step, and the implicit step driver.  TEST INFRASTRUCTURE ONLY (see
+ * lns3d_oracle.h).  Supported options are the ones the benchmark and test
+ * configurations use; anything else returns -1 rather than guessing:
+ *   optx/opty any value; xper/yper; carp = no; lsym/rsym = no;
+ *   wall in {0,1,2}; wallt in {0,1,2}; top = 0; left/right = 0;
+ *   Navier = yes; ispg = 0; eps_e = 0; idamp = 0; useCalcd = no;
+ *   linear = 2 (complex analysis); mattyp 0..3.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "lns3d_oracle.h"
+
+/* src/stencil.f90 coefficient tables (the values are the scheme) */
+static const double ga1 = 8.333333333333333333333E-02, ga2 = -6.666666666666666666667E-01,
+                    ga3 = 6.666666666666666666667E-01, ga4 = -8.333333333333333333333E-02;
+static const double gc1 = -2.083333333333333333333E+00, gc2 = 4.000000000000000000000E+00,
+                    gc3 = -3.000000000000000000000E+00, gc4 = 1.333333333333333333333E+00,
+                    gc5 = -2.500000000000000000000E-01;
+static const double gb1 = -2.500000000000000000000E-01, gb2 = -8.333333333333333333333E-01,
+                    gb3 = 1.500000000000000000000E+00, gb4 = -5.000000000000000000000E-01,
+                    gb5 = 8.333333333333333333333E-02;
+static const double da1 = -8.333333333333333333333E-02, da2 = 1.333333333333333333333E+00,
+                    da3 = -2.500000000000000000000E+00, da4 = 1.333333333333333333333E+00,
+                    da5 = -8.333333333333333333333E-02;
+static const double db1 = 9.166666666666666666667E-01, db2 = -1.666666666666666666667E+00,
+                    db3 = 5.000000000000000000000E-01, db4 = 3.333333333333333333333E-01,
+                    db5 = -8.333333333333333333333E-02;
+static const double dd1 = 2.916666666666666666667E+00, dd2 = -8.666666666666666666667E+00,
+                    dd3 = 9.500000000000000000000E+00, dd4 = -4.666666666666666666667E+00,
+                    dd5 = 9.166666666666666666667E-01;
+static const double pt66 = 6.6666666666666666666e-1;
+
+/* interior 7-point coefficients, src/cgrad.f90:34-78, src/cgrad2.f90:38-133 */
+static void first_coefs(int opt, double dinv, double g[6]) {
+  double c;
+  if (opt == 0) c = 1.0 / 60.0;
+  else if (opt == -1) c = 0.0;
+  else {
+    double w = 2.0 * 3.1415926535897932385e+0 / 12.0;
+    c = (w / 2.0 - 2.0 * sin(w) / 3.0 + sin(2.0 * w) / 12.0) /
+        (5.0 * sin(w) - 4.0 * sin(2.0 * w) + sin(3.0 * w));
+  }
+  double a = (2.0 + 15.0 * c) / 3.0, b = -(1.0 + 48.0 * c) / 12.0;
+  g[0] = -c * dinv; g[1] = -b * dinv; g[2] = -a * dinv;
+  g[3] = a * dinv;  g[4] = b * dinv;  g[5] = c * dinv;
+}
+static void second_coefs(int opt, double dsinv, double d[7]) {
+  double c;
+  if (opt == 0) c = 1.0 / 90.0;
+  else if (opt == -1) c = 0.0;
+  else {
+    double w = 2.0 * 3.1415926535897932385e+0 / 12.0;
+    double s1 = sin(w / 2.0), s2 = sin(w), s3 = sin(3.0 * w / 2.0);
+    c = -(3.0 * w * w - 16.0 * s1 * s1 + s2 * s2) /
+        (12.0 * (-15.0 * s1 * s1 + 6.0 * s2 * s2 - s3 * s3));
+  }
+  double a = (4.0 + 45.0 * c) / 3.0, b = -(1.0 + 72.0 * c) / 12.0;
+  d[0] = c * dsinv; d[1] = b * dsinv; d[2] = a * dsinv;
+  d[3] = -2.0 * (a + b + c) * dsinv;
+  d[4] = a * dsinv; d[5] = b * dsinv; d[6] = c * dsinv;
+}
+
+/* One line of N points: values at x[p*xs + e] for e < ne.  Generic over
+ * real/complex via macros instantiated twice below. */
+#define DEFINE_LINE_OPS(T, SUF)                                                       \
+static void d1_line_##SUF(int N, int per, const double g[6], double dinv, int ne,     \
+                          const T *x, long xs, T *y, long ys) {                       \
+  for (int p = 0; p < N; p++)                                                         \
+    for (int e = 0; e < ne; e++) {                                                    \
+      T r;                                                                            \
+      if (per) {                                                                      \
+        if (p == N - 1) continue;                                                     \
+        int m = N - 1; /* distinct points */                                          \
+        int q[6] = {p - 3, p - 2, p - 1, p + 1, p + 2, p + 3};                        \
+        for (int k = 0; k < 6; k++) q[k] = ((q[k] % m) + m) % m;                       \
+        r = g[0] * x[q[0] * xs + e] + g[1] * x[q[1] * xs + e] + g[2] * x[q[2] * xs + e] \
+          + g[3] * x[q[3] * xs + e] + g[4] * x[q[4] * xs + e] + g[5] * x[q[5] * xs + e]; \
+      } else if (p == 0) {                                                            \
+        r = (gc1 * x[0 * xs + e] + gc2 * x[1 * xs + e] + gc3 * x[2 * xs + e] +        \
+             gc4 * x[3 * xs + e] + gc5 * x[4 * xs + e]) * dinv;                       \
+      } else if (p == 1) {                                                            \
+        r = (gb1 * x[0 * xs + e] + gb2 * x[1 * xs + e] + gb3 * x[2 * xs + e] +        \
+             gb4 * x[3 * xs + e] + gb5 * x[4 * xs + e]) * dinv;                       \
+      } else if (p == 2) {                                                            \
+        r = (ga1 * x[0 * xs + e] + ga2 * x[1 * xs + e] + ga3 * x[3 * xs + e] +        \
+             ga4 * x[4 * xs + e]) * dinv;                                             \
+      } else if (p == N - 3) {                                                        \
+        r = (ga1 * x[(N - 5) * xs + e] + ga2 * x[(N - 4) * xs + e] +                  \
+             ga3 * x[(N - 2) * xs + e] + ga4 * x[(N - 1) * xs + e]) * dinv;           \
+      } else if (p == N - 2) {                                                        \
+        r = -(gb1 * x[(N - 1) * xs + e] + gb2 * x[(N - 2) * xs + e] +                 \
+              gb3 * x[(N - 3) * xs + e] + gb4 * x[(N - 4) * xs + e] +                 \
+              gb5 * x[(N - 5) * xs + e]) * dinv;                                      \
+      } else if (p == N - 1) {                                                        \
+        r = -(gc1 * x[(N - 1) * xs + e] + gc2 * x[(N - 2) * xs + e] +                 \
+              gc3 * x[(N - 3) * xs + e] + gc4 * x[(N - 4) * xs + e] +                 \
+              gc5 * x[(N - 5) * xs + e]) * dinv;                                      \
+      } else {                                                                        \
+        r = g[0] * x[(p - 3) * xs + e] + g[1] * x[(p - 2) * xs + e] +                 \
+            g[2] * x[(p - 1) * xs + e] + g[3] * x[(p + 1) * xs + e] +                 \
+            g[4] * x[(p + 2) * xs + e] + g[5] * x[(p + 3) * xs + e];                  \
+      }                                                                               \
+      y[p * ys + e] = r;                                                              \
+    }                                                                                 \
+  if (per)                                                                            \
+    for (int e = 0; e < ne; e++) y[(N - 1) * ys + e] = y[e];                          \
+}                                                                                     \
+static void d2_line_##SUF(int N, int per, const double d[7], double dsinv, int ne,    \
+                          const T *x, long xs, T *y, long ys) {                       \
+  for (int p = 0; p < N; p++)                                                         \
+    for (int e = 0; e < ne; e++) {                                                    \
+      T r;                                                                            \
+      if (per) {                                                                      \
+        if (p == N - 1) continue;                                                     \
+        int m = N - 1;                                                                \
+        int q[7];                                                                     \
+        for (int k = 0; k < 7; k++) q[k] = (((p - 3 + k) % m) + m) % m;                \
+        r = d[0] * x[q[0] * xs + e] + d[1] * x[q[1] * xs + e] + d[2] * x[q[2] * xs + e] \
+          + d[3] * x[q[3] * xs + e] + d[4] * x[q[4] * xs + e] + d[5] * x[q[5] * xs + e] \
+          + d[6] * x[q[6] * xs + e];                                                  \
+      } else if (p == 0) {                                                            \
+        r = (dd1 * x[0 * xs + e] + dd2 * x[1 * xs + e] + dd3 * x[2 * xs + e] +        \
+             dd4 * x[3 * xs + e] + dd5 * x[4 * xs + e]) * dsinv;                      \
+      } else if (p == 1) {                                                            \
+        r = (db1 * x[0 * xs + e] + db2 * x[1 * xs + e] + db3 * x[2 * xs + e] +        \
+             db4 * x[3 * xs + e] + db5 * x[4 * xs + e]) * dsinv;                      \
+      } else if (p == 2) {                                                            \
+        r = (da1 * x[0 * xs + e] + da2 * x[1 * xs + e] + da3 * x[2 * xs + e] +        \
+             da4 * x[3 * xs + e] + da5 * x[4 * xs + e]) * dsinv;                      \
+      } else if (p == N - 3) {                                                        \
+        r = (da1 * x[(N - 5) * xs + e] + da2 * x[(N - 4) * xs + e] +                  \
+             da3 * x[(N - 3) * xs + e] + da4 * x[(N - 2) * xs + e] +                  \
+             da5 * x[(N - 1) * xs + e]) * dsinv;                                      \
+      } else if (p == N - 2) {                                                        \
+        r = (db1 * x[(N - 1) * xs + e] + db2 * x[(N - 2) * xs + e] +                  \
+             db3 * x[(N - 3) * xs + e] + db4 * x[(N - 4) * xs + e] +                  \
+             db5 * x[(N - 5) * xs + e]) * dsinv;                                      \
+      } else if (p == N - 1) {                                                        \
+        r = (dd1 * x[(N - 1) * xs + e] + dd2 * x[(N - 2) * xs + e] +                  \
+             dd3 * x[(N - 3) * xs + e] + dd4 * x[(N - 4) * xs + e] +                  \
+             dd5 * x[(N - 5) * xs + e]) * dsinv;                                      \
+      } else {                                                                        \
+        r = d[0] * x[(p - 3) * xs + e] + d[1] * x[(p - 2) * xs + e] +                 \
+            d[2] * x[(p - 1) * xs + e] + d[3] * x[p * xs + e] +                       \
+            d[4] * x[(p + 1) * xs + e] + d[5] * x[(p + 2) * xs + e] +                 \
+            d[6] * x[(p + 3) * xs + e];                                               \
+      }                                                                               \
+      y[p * ys + e] = r;                                                              \
+    }                                                                                 \
+  if (per)                                                                            \
+    for (int e = 0; e < ne; e++) y[(N - 1) * ys + e] = y[e];                          \
+}
+
+DEFINE_LINE_OPS(zdouble, z)
+DEFINE_LINE_OPS(double, d)
+
+/* first and second derivatives of an ndof-component field (nx,ny):
+ * src/cgrad.f90:31-461 and src/cgrad2.f90:36-620 (non-Carpenter branches) */
+#define DEFINE_FIELD_OPS(T, SUF)                                                      \
+static void grad_##SUF(const orc_params *p, int nd, const T *v, T *g1, T *g2) {       \
+  int nx = p->nx, ny = p->ny;                                                         \
+  double dxi = 1.0 / (double)(nx - 1), deta = 1.0 / (double)(ny - 1);                 \
+  double dxinv = 1.0 / dxi, dyinv = 1.0 / deta, gx[6], gy[6];                         \
+  first_coefs(p->optx, dxinv, gx);                                                    \
+  first_coefs(p->opty, dyinv, gy);                                                    \
+  for (int j = 0; j < ny; j++)                                                        \
+    d1_line_##SUF(nx, p->xper, gx, dxinv, nd, v + (long)nd * nx * j, nd, g1 + (long)nd * nx * j, nd); \
+  for (int i = 0; i < nx; i++)                                                        \
+    d1_line_##SUF(ny, p->yper, gy, dyinv, nd, v + (long)nd * i, (long)nd * nx, g2 + (long)nd * i, (long)nd * nx); \
+}                                                                                     \
+static void grad2_##SUF(const orc_params *p, int nd, const T *v, const T *g1,         \
+                        T *g11, T *g12, T *g22) {                                     \
+  int nx = p->nx, ny = p->ny;                                                         \
+  double dxi = 1.0 / (double)(nx - 1), deta = 1.0 / (double)(ny - 1);                 \
+  double dxinv = 1.0 / dxi, dyinv = 1.0 / deta;                                       \
+  double dxsinv = 1.0 / (dxi * dxi), dysinv = 1.0 / (deta * deta);                    \
+  double gy[6], dx[7], dy[7];                                                         \
+  first_coefs(p->opty, dyinv, gy);                                                    \
+  second_coefs(p->optx, dxsinv, dx);                                                  \
+  second_coefs(p->opty, dysinv, dy);                                                  \
+  for (int j = 0; j < ny; j++)                                                        \
+    d2_line_##SUF(nx, p->xper, dx, dxsinv, nd, v + (long)nd * nx * j, nd, g11 + (long)nd * nx * j, nd); \
+  for (int i = 0; i < nx; i++) {                                                      \
+    d2_line_##SUF(ny, p->yper, dy, dysinv, nd, v + (long)nd * i, (long)nd * nx, g22 + (long)nd * i, (long)nd * nx); \
+    d1_line_##SUF(ny, p->yper, gy, dyinv, nd, g1 + (long)nd * i, (long)nd * nx, g12 + (long)nd * i, (long)nd * nx); \
+  }                                                                                   \
+}
+
+DEFINE_FIELD_OPS(zdouble, z)
+DEFINE_FIELD_OPS(double, d)
+
+void orc_cgrad(const orc_params *p, const zdouble *v, zdouble *g1, zdouble *g2) {
+  grad_z(p, 5, v, g1, g2);
+}
+void orc_cgrad2(const orc_params *p, const zdouble *v, const zdouble *g1,
+                zdouble *g11, zdouble *g12, zdouble *g22) {
+  grad2_z(p, 5, v, g1, g11, g12, g22);
+}
+void orc_grad(const orc_params *p, int ndof, const double *v, double *g1, double *g2) {
+  grad_d(p, ndof, v, g1, g2);
+}
+void orc_grad2(const orc_params *p, int ndof, const double *v, const double *g1,
+               double *g11, double *g12, double *g22) {
+  grad2_d(p, ndof, v, g1, g11, g12, g22);
+}
+
+static int supported(const orc_params *p) {
+  if (p->nx < 7 || p->ny < 7) return 0;
+  if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
+  if (p->top != 0 || p->left != 0 || p->right != 0) return 0;
+  if (p->linear != 2) return 0;
+  if (p->mattyp < 0 || p->mattyp > 3) return 0;
+  return 1;
+}
+
+/* ---- src/lrhs3d.f90:45-285 (point operator, ispg = 0, eps_e = 0) ---------- */
+int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
+               zdouble *r, zdouble *work) {
+  if (!supported(p)) return -1;
+  int nx = p->nx, ny = p->ny;
+  long P = (long)nx * ny;
+  zdouble *c1 = work, *c2 = work + 5 * P, *c11 = work + 10 * P, *c12 = work + 15 * P,
+          *c22 = work + 20 * P;
+  orc_cgrad(p, v, c1, c2);
+  orc_cgrad2(p, v, c1, c11, c12, c22);
+  if (p->wallt == 1 || p->wallt == 2)                  /* src/cgradbc.f90:15 */
+    for (int i = 0; i < nx; i++) c2[4 + 5 * i] = 0;
+  const zdouble ci = _Complex_I;
+  for (long q = 0; q < P; q++) {
+    const double *A = c->Ah + 25 * q, *B = c->Bh + 25 * q, *D = c->Dh + 25 * q,
+                 *Di = c->Dhi + 25 * q;
+    const double *V11 = c->Vh11 + 6 * q, *V12 = c->Vh12 + 6 * q, *V22 = c->Vh22 + 6 * q,
+                 *ABi = c->ABhi + 6 * q;
+    const zdouble *vq = v + 5 * q, *g1 = c1 + 5 * q, *g2 = c2 + 5 * q, *h11 = c11 + 5 * q,
+                  *h12 = c12 + 5 * q, *h22 = c22 + 5 * q;
+    zdouble rr[5];
+    for (int m = 0; m < 5; m++)
+      rr[m] = A[m] * g1[0] + A[m + 5] * g1[1] + A[m + 10] * g1[2] + A[m + 15] * g1[3] +
+              A[m + 20] * g1[4];
+    for (int m = 0; m < 5; m++)
+      rr[m] = rr[m] + B[m] * g2[0] + B[m + 5] * g2[1] + B[m + 10] * g2[2] +
+              B[m + 15] * g2[3] + B[m + 20] * g2[4];
+    for (int m = 0; m < 5; m++)
+      rr[m] = rr[m] + D[m] * vq[0] + D[m + 5] * vq[1] + D[m + 10] * vq[2] +
+              D[m + 15] * vq[3] + D[m + 20] * vq[4];
+    const double *Vs[3] = {V11, V12, V22};
+    const zdouble *hs[3] = {h11, h12, h22};
+    for (int t = 0; t < 3; t++) {
+      const double *W = Vs[t];
+      const zdouble *h = hs[t];
+      rr[1] = rr[1] - W[0] * h[1] - W[4] * h[2];
+      rr[2] = rr[2] - W[5] * h[1] - W[1] * h[2];
+      rr[3] = rr[3] - W[2] * h[3];
+      rr[4] = rr[4] - W[3] * h[4];
+    }
+    rr[1] = rr[1] - ci * (ABi[0] * g1[3] + ABi[2] * g2[3]);
+    rr[2] = rr[2] - ci * (ABi[1] * g1[3] + ABi[3] * g2[3]);
+    rr[3] = rr[3] - ci * (ABi[0] * g1[1] + ABi[1] * g1[2] + ABi[2] * g2[1] + ABi[3] * g2[2]);
+    for (int m = 0; m < 5; m++)
+      rr[m] = rr[m] + ci * (Di[m] * vq[0] + Di[m + 5] * vq[1] + Di[m + 10] * vq[2] +
+                           Di[m + 15] * vq[3] + Di[m + 20] * vq[4]);
+    for (int m = 0; m < 5; m++) r[5 * q + m] = rr[m];
+  }
+  return 0;
+}
+
+/* src/impdrv3d.f90:444-470 */
+void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v, const zdouble *vold,
+               const double *dtl) {
+  long P = (long)p->nx * p->ny;
+  for (long q = 0; q < P; q++)
+    for (int m = 0; m < 5; m++)
+      r[5 * q + m] = vold[5 * q + m] - v[5 * q + m] - dtl[q] * r[5 * q + m];
+}
+
+/* ---- LHS assembly: src/lhs1f3d.f90 / lhs2f3d.f90 (interior), lhsbt1f3d.f90 /
+ *      lhsbt2f3d.f90 (rows 1,2,n-1,n), lhsbc1f3d.f90 / lhsbc2f3d.f90 (BCs) --- */
+#define MAT(k, r, cc) mat[(k) + 5 * (r) + 25 * (cc) + 125 * pt]
+
+/* row of a boundary-closure line: first-derivative coefficient F[q] and
+ * second-derivative coefficient S[q] act on the column held in slot sl[q];
+ * q = 0 is the diagonal */
+static void closure_row(zdouble *mat, long pt, const double F[5], const double S[5],
+                        const int sl[5], double alfa, double dinv, double dsinv,
+                        double dtl, const double *X, const double *Xi2 /* 2 */,
+                        const double *D, const double *Di, const double *Vh, double rcalcd) {
+  double a[5], cim[5];
+  for (int q = 0; q < 5; q++) a[q] = alfa * F[q] * dinv;
+  for (int r = 0; r < 5; r++)
+    for (int cc = 0; cc < 5; cc++) {
+      for (int q = 1; q < 5; q++) MAT(sl[q], r, cc) = a[q] * dtl * X[r + 5 * cc];
+      double re = dtl * (a[0] * X[r + 5 * cc] + rcalcd * alfa * D[r + 5 * cc]);
+      double im = dtl * (rcalcd * alfa * Di[r + 5 * cc]);
+      MAT(sl[0], r, cc) = re + im * _Complex_I;
+    }
+  for (int q = 0; q < 5; q++) cim[q] = alfa * F[q] * dinv;   /* imaginary coefficient */
+  for (int q = 0; q < 5; q++) {
+    int k = sl[q];
+    MAT(k, 1, 3) -= _Complex_I * (cim[q] * dtl * Xi2[0]);
+    MAT(k, 2, 3) -= _Complex_I * (cim[q] * dtl * Xi2[1]);
+    MAT(k, 3, 1) -= _Complex_I * (cim[q] * dtl * Xi2[0]);
+    MAT(k, 3, 2) -= _Complex_I * (cim[q] * dtl * Xi2[1]);
+  }
+  for (int q = 0; q < 5; q++) {
+    int k = sl[q];
+    double b = alfa * S[q] * dsinv;
+    MAT(k, 1, 1) -= b * dtl * Vh[0];
+    MAT(k, 1, 2) -= b * dtl * Vh[4];
+    MAT(k, 2, 1) -= b * dtl * Vh[5];
+    MAT(k, 2, 2) -= b * dtl * Vh[1];
+    MAT(k, 3, 3) -= b * dtl * Vh[2];
+    MAT(k, 4, 4) -= b * dtl * Vh[3];
+  }
+  for (int r = 0; r < 5; r++) MAT(sl[0], r, r) += 1.0;
+}
+
+int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl,
+            zdouble *mat) {
+  if (!supported(p) || (dir != 1 && dir != 2)) return -1;
+  int nx = p->nx, ny = p->ny;
+  double alfa = p->alfa;
+  double d = dir == 1 ? 1.0 / (double)(nx - 1) : 1.0 / (double)(ny - 1);
+  const double *X = dir == 1 ? c->Ah : c->Bh;
+  const double *Vh = dir == 1 ? c->Vh11 : c->Vh22;
+  int off = dir == 1 ? 0 : 2;                  /* ABhi(1:2) or ABhi(3:4) */
+  int calcd = dir == 1;                        /* D terms ride in the xi factor */
+  double a[5] = {alfa * ga1 / d, alfa * ga2 / d, alfa * 0.0 / d, alfa * ga3 / d, alfa * ga4 / d};
+  double b[5] = {alfa * da1 / (d * d), alfa * da2 / (d * d), alfa * da3 / (d * d),
+                 alfa * da4 / (d * d), alfa * da5 / (d * d)};
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      long pt = i + (long)nx * j;
+      const double *Xp = X + 25 * pt, *Dp = c->Dh + 25 * pt, *Dip = c->Dhi + 25 * pt;
+      const double *Vp = Vh + 6 * pt, *ABp = c->ABhi + 6 * pt + off;
+      double dt = dtl[pt];
+      for (int k = 0; k < 5; k++)
+        for (int r = 0; r < 5; r++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, r, cc) = a[k] * dt * Xp[r + 5 * cc];
+      for (int k = 0; k < 5; k++) {
+        MAT(k, 1, 3) -= _Complex_I * (a[k] * dt * ABp[0]);
+        MAT(k, 2, 3) -= _Complex_I * (a[k] * dt * ABp[1]);
+        MAT(k, 3, 1) -= _Complex_I * (a[k] * dt * ABp[0]);
+        MAT(k, 3, 2) -= _Complex_I * (a[k] * dt * ABp[1]);
+      }
+      if (calcd) {
+        for (int r = 0; r < 5; r++)
+          for (int cc = 0; cc < 5; cc++) MAT(2, r, cc) += alfa * dt * Dp[r + 5 * cc];
+        for (int r = 0; r < 5; r++)
+          for (int cc = 0; cc < 5; cc++)
+            MAT(2, r, cc) += _Complex_I * (alfa * dt * Dip[r + 5 * cc]);
+      }
+      for (int k = 0; k < 5; k++) {
+        MAT(k, 1, 1) -= b[k] * dt * Vp[0];
+        MAT(k, 1, 2) -= b[k] * dt * Vp[4];
+        MAT(k, 2, 1) -= b[k] * dt * Vp[5];
+        MAT(k, 2, 2) -= b[k] * dt * Vp[1];
+        MAT(k, 3, 3) -= b[k] * dt * Vp[2];
+        MAT(k, 4, 4) -= b[k] * dt * Vp[3];
+      }
+      for (int r = 0; r < 5; r++) MAT(2, r, r) += 1.0;
+    }
+
+  int per = dir == 1 ? p->xper : p->yper;
+  if (!per) {
+    int N = dir == 1 ? nx : ny, L = dir == 1 ? ny : nx;
+    double dinv = 1.0 / d, dsinv = 1.0 / (d * d), rc = calcd ? 1.0 : 0.0;
+    const double Fc[5] = {gc1, gc2, gc3, gc4, gc5}, Fb[5] = {gb1, gb2, gb3, gb4, gb5};
+    const double Sd[5] = {dd1, dd2, dd3, dd4, dd5}, Sb[5] = {db1, db2, db3, db4, db5};
+    /* rows 1, 2, n-1, n: diagonal first in each list (Appendix A slots) */
+    const double Fr0[5] = {Fc[0], Fc[1], Fc[2], Fc[3], Fc[4]};
+    const int sl0[5] = {2, 3, 4, 0, 1};
+    const double Fr1[5] = {Fb[1], Fb[0], Fb[2], Fb[3], Fb[4]};
+    const double Sr1[5] = {Sb[1], Sb[0], Sb[2], Sb[3], Sb[4]};
+    const int sl1[5] = {2, 1, 3, 4, 0};
+    const double Frm[5] = {-Fb[1], -Fb[0], -Fb[2], -Fb[3], -Fb[4]};
+    const int slm[5] = {2, 3, 1, 0, 4};
+    const double Frn[5] = {-Fc[0], -Fc[1], -Fc[2], -Fc[3], -Fc[4]};
+    const int sln[5] = {2, 1, 0, 4, 3};
+    for (int l = 0; l < L; l++) {
+      int rows[4] = {0, 1, N - 2, N - 1};
+      const double *Fs[4] = {Fr0, Fr1, Frm, Frn};
+      const double *Ss[4] = {Sd, Sr1, Sr1, Sd};
+      const int *sls[4] = {sl0, sl1, slm, sln};
+      for (int w = 0; w < 4; w++) {
+        int i = dir == 1 ? rows[w] : l, j = dir == 1 ? l : rows[w];
+        long pt = i + (long)nx * j;
+        closure_row(mat, pt, Fs[w], Ss[w], sls[w], alfa, dinv, dsinv, dtl[pt],
+                    X + 25 * pt, c->ABhi + 6 * pt + off, c->Dh + 25 * pt, c->Dhi + 25 * pt,
+                    Vh + 6 * pt, rc);
+      }
+    }
+  }
+
+  /* boundary conditions */
+  if (!p->yper) {
+    for (int i = 0; i < nx; i++) {
+      long pt = i;
+      if (dir == 1 && (p->wall == 1 || p->wall == 2)) {
+        for (int k = 0; k < 5; k++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, 0, cc) = 0;
+        MAT(2, 0, 0) = 1.0;
+      }
+      if (dir == 2 && p->wall == 1) {
+        for (int k = 0; k < 5; k++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, 0, cc) = 0;
+        MAT(2, 0, 0) = -gc1; MAT(3, 0, 0) = -gc2; MAT(4, 0, 0) = -gc3;
+        MAT(0, 0, 0) = -gc4; MAT(1, 0, 0) = -gc5;
+      }
+      for (int r = 1; r <= 3; r++)
+        for (int k = 0; k < 5; k++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, r, cc) = 0;
+      MAT(2, 1, 1) = 1.0; MAT(2, 2, 2) = 1.0; MAT(2, 3, 3) = 1.0;
+      if (p->wallt == 0 || (dir == 1 && p->wallt == 1)) {
+        for (int k = 0; k < 5; k++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, 4, cc) = 0;
+        MAT(2, 4, 4) = 1.0;
+      }
+      if (dir == 2 && p->wallt == 1) {
+        for (int k = 0; k < 5; k++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, 4, cc) = 0;
+        MAT(2, 4, 4) = -gc1; MAT(3, 4, 4) = -gc2; MAT(4, 4, 4) = -gc3;
+        MAT(0, 4, 4) = -gc4; MAT(1, 4, 4) = -gc5;
+      }
+    }
+    for (int i = 0; i < nx; i++) {                        /* top = 0 */
+      long pt = i + (long)nx * (ny - 1);
+      for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
+      for (int r = 0; r < 5; r++) MAT(2, r, r) = 1.0;
+    }
+  }
+  if (!p->xper) {
+    for (int j = 0; j < ny; j++) {                        /* left = right = 0 */
+      long pts[2] = {(long)nx * j, nx - 1 + (long)nx * j};
+      for (int s = 0; s < 2; s++) {
+        long pt = pts[s];
+        for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
+        for (int r = 0; r < 5; r++) MAT(2, r, r) = 1.0;
+      }
+    }
+  }
+  return 0;
+}
+#undef MAT
+
+/* src/rhsbc3d.f90:34-160, 200-215 (wall, top=0, left=right=0) */
+int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
+  if (!supported(p)) return -1;
+  int nx = p->nx, ny = p->ny;
+#define R(d, i, j) r[(d) + 5 * ((i) + (long)nx * (j))]
+#define V(d, i, j) v[(d) + 5 * ((i) + (long)nx * (j))]
+  if (p->yper) {
+    for (int i = 0; i < nx; i++)
+      for (int d = 0; d < 5; d++) R(d, i, ny - 1) = 0;
+  } else {
+    for (int i = 0; i < nx; i++) {
+      for (int d = 1; d <= 3; d++) R(d, i, 0) = -(V(d, i, 0) - 0.0);
+      if (p->wall == 1)
+        R(0, i, 0) = gc1 * V(0, i, 0) + gc2 * V(0, i, 1) + gc3 * V(0, i, 2) +
+                     gc4 * V(0, i, 3) + gc5 * V(0, i, 4);
+      if (p->wall == 2) R(0, i, 0) = V(0, i, 0) - 2.0 * V(0, i, 1) + V(0, i, 2);
+      if (p->wallt == 0) R(4, i, 0) = 0;
+      if (p->wallt == 1)
+        R(4, i, 0) = gc1 * V(4, i, 0) + gc2 * V(4, i, 1) + gc3 * V(4, i, 2) +
+                     gc4 * V(4, i, 3) + gc5 * V(4, i, 4);
+    }
+    for (int i = 0; i < nx; i++)
+      for (int d = 0; d < 5; d++) R(d, i, ny - 1) = 0;
+  }
+  if (p->xper) {
+    for (int j = 0; j < ny; j++)
+      for (int d = 0; d < 5; d++) R(d, nx - 1, j) = 0;
+  } else {
+    for (int j = 0; j < ny; j++)
+      for (int d = 0; d < 5; d++) { R(d, 0, j) = 0; R(d, nx - 1, j) = 0; }
+  }
+  return 0;
+}
+
+/* src/itrbc3d.f90:34-220, 300-310 (same selectors) */
+int orc_itrbc3d(const orc_params *p, zdouble *v) {
+  if (!supported(p)) return -1;
+  int nx = p->nx, ny = p->ny;
+  if (p->yper) {
+    for (int i = 0; i < nx; i++)
+      for (int d = 0; d < 5; d++) V(d, i, ny - 1) = V(d, i, 0);
+  } else {
+    for (int i = 0; i < nx; i++) {
+      V(1, i, 0) = 0; V(2, i, 0) = 0; V(3, i, 0) = 0;
+      if (p->wall == 1)
+        V(0, i, 0) = -(gc2 * V(0, i, 1) + gc3 * V(0, i, 2) + gc4 * V(0, i, 3) +
+                       gc5 * V(0, i, 4)) / gc1;
+      if (p->wall == 2) V(0, i, 0) = 2.0 * V(0, i, 1) - V(0, i, 2);
+      if (p->wallt == 0) V(4, i, 0) = 0;
+      if (p->wallt == 1)
+        V(4, i, 0) = -(gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
+                       gc5 * V(4, i, 4)) / gc1;
+    }
+    for (int i = 0; i < nx; i++)
+      for (int d = 0; d < 5; d++) V(d, i, ny - 1) = 0;
+  }
+  if (p->xper) {
+    for (int j = 0; j < ny; j++)
+      for (int d = 0; d < 5; d++) V(d, nx - 1, j) = V(d, 0, j);
+  } else {
+    for (int j = 0; j < ny; j++)
+      for (int d = 0; d < 5; d++) { V(d, 0, j) = 0; V(d, nx - 1, j) = 0; }
+  }
+  return 0;
+#undef R
+#undef V
+}
+
+/* ---- src/getmat.f90 (scalar form) --------------------------------------- */
+static void getmat(int mattyp, double t, double *mu, double *lm, double *con, double *dmu,
+                   double *d2mu, double *dlm, double *d2lm, double *dcon, double *d2con) {
+  if (mattyp == 0) {
+    *mu = 1.0; *dmu = 0.0; *d2mu = 0.0;
+  } else {  /* 2, 3: linear law (1, Sutherland, is refused by supported()) */
+    *mu = t; *dmu = 1.0; *d2mu = 0.0;
+  }
+  *con = *mu; *dcon = *dmu; *d2con = *d2mu;
+  if (mattyp == 3) { *lm = 0.0; *dlm = 0.0; *d2lm = 0.0; }
+  else { *lm = -pt66 * *mu; *dlm = -pt66 * *dmu; *d2lm = -pt66 * *d2mu; }
+}
+
+/* ---- src/genmtrx.f90:72-1000 (general metrics, linear = 2, idamp = 0) ----- */
+int orc_genmtrx(const orc_params *p, const double *vm, const double *metric, orc_coeffs *c) {
+  if (!supported(p) || p->mattyp == 1) return -1;
+  int nx = p->nx, ny = p->ny;
+  long P = (long)nx * ny;
+  double *g = calloc(25 * P, sizeof(double));
+  if (!g) return -1;
+  double *g1 = g, *g2 = g + 5 * P, *g11 = g + 10 * P, *g12 = g + 15 * P, *g22 = g + 20 * P;
+  orc_grad(p, 5, vm, g1, g2);
+  orc_grad2(p, 5, vm, g1, g11, g12, g22);
+  if (p->wallt == 1 || p->wallt == 2)                    /* src/gradbc.f90:15 */
+    for (int i = 0; i < nx; i++) g2[4 + 5 * i] = 0;
+  const double *m1 = metric, *m2 = metric + P, *n1 = metric + 2 * P, *n2 = metric + 3 * P,
+               *m11 = metric + 4 * P, *m12 = metric + 5 * P, *m22 = metric + 6 * P,
+               *n11 = metric + 7 * P, *n12 = metric + 8 * P, *n22 = metric + 9 * P;
+  double gamma = p->gamma, gamma1 = gamma - 1.0, Ma = p->Ma, Re = p->Re, Pr = p->Pr;
+  double kz = p->kz;
+  for (long q = 0; q < P; q++) {
+    double M1 = m1[q], M2 = m2[q], N1 = n1[q], N2 = n2[q];
+    double m1m1 = M1 * M1, m1m2 = M1 * M2, m2m2 = M2 * M2;
+    double n1n1 = N1 * N1, n1n2 = N1 * N2, n2n2 = N2 * N2;
+    double m1n1 = M1 * N1, m1n2 = M1 * N2, m2n1 = M2 * N1, m2n2 = M2 * N2;
+    double g11l[5], g12l[5], g22l[5], g1l[5], g2l[5];
+    for (int d = 0; d < 5; d++) {
+      double a11 = g11[5 * q + d], a12 = g12[5 * q + d], a22 = g22[5 * q + d];
+      double a1 = g1[5 * q + d], a2 = g2[5 * q + d];
+      g11l[d] = a11 * m1m1 + 2.0 * a12 * m1n1 + a22 * n1n1 + a1 * m11[q] + a2 * n11[q];
+      g12l[d] = a11 * m1m2 + a12 * (m1n2 + m2n1) + a22 * n1n2 + a1 * m12[q] + a2 * n12[q];
+      g22l[d] = a11 * m2m2 + 2.0 * a12 * m2n2 + a22 * n2n2 + a1 * m22[q] + a2 * n22[q];
+      g1l[d] = a1 * M1 + a2 * N1;
+      g2l[d] = a1 * M2 + a2 * N2;
+    }
+    double fact = 1.0 / (gamma * Ma * Ma);
+    double rho = vm[5 * q + 0], u1 = vm[5 * q + 1], u2 = vm[5 * q + 2], u3 = vm[5 * q + 3],
+           t = vm[5 * q + 4];
+    double rhoinv = 1.0 / rho;
+    double grho[3] = {g1l[0], g2l[0], 0.0};
+    double gu[3][3] = {{g1l[1], g2l[1], 0.0}, {g1l[2], g2l[2], 0.0}, {g1l[3], g2l[3], 0.0}};
+    double gt[3] = {g1l[4], g2l[4], 0.0};
+    double g1divu = g11l[1] + g12l[2], g2divu = g12l[1] + g22l[2], g3divu = 0.0;
+    double S1jj = 0.5 * (g11l[1] + g11l[1] + g22l[1] + g12l[2]);
+    double S2jj = 0.5 * (g11l[2] + g12l[1] + g22l[2] + g22l[2]);
+    double S3jj = 0.5 * (g11l[3] + g22l[3]);
+    double Lapt = g11l[4] + g22l[4];
+    double divu = gu[0][0] + gu[1][1] + gu[2][2];
+    double gp[3];
+    for (int k = 0; k < 3; k++) gp[k] = fact * (grho[k] * t + rho * gt[k]);
+    double S[3][3] = {{0}};
+    S[0][0] = gu[0][0];
+    S[1][0] = 0.5 * (gu[1][0] + gu[0][1]);
+    S[2][0] = 0.5 * gu[2][0];
+    S[0][1] = 0.5 * (gu[0][1] + gu[1][0]);
+    S[1][1] = gu[1][1];
+    S[2][1] = 0.5 * gu[2][1];
+    S[0][2] = 0.5 * gu[2][0];
+    S[1][2] = 0.5 * gu[2][1];
+    double mu, lm, con, dmu, d2mu, dlm, d2lm, dcon, d2con;
+    getmat(p->mattyp, t, &mu, &lm, &con, &dmu, &d2mu, &dlm, &d2lm, &dcon, &d2con);
+    double gmu[3], gdmu[3], gcon[3], gdcon[3], glm[3], gdlm[3];
+    for (int k = 0; k < 3; k++) {
+      gmu[k] = dmu * gt[k]; gdmu[k] = d2mu * gt[k];
+      gcon[k] = dcon * gt[k]; gdcon[k] = d2con * gt[k];
+      glm[k] = dlm * gt[k]; gdlm[k] = d2lm * gt[k];
+    }
+    double A[5][5], B[5][5], C[5][5], D[5][5], Vij[4];
+    double f1, f2, gmsinv;
+    /* A, genmtrx.f90:430-480 */
+    A[0][0] = u1; A[0][1] = rho; A[0][2] = 0; A[0][3] = 0; A[0][4] = 0;
+    f1 = rhoinv / Re; gmsinv = 1.0 / (gamma * Ma * Ma);
+    A[1][0] = rhoinv * t * gmsinv;
+    A[1][1] = u1 - f1 * glm[0] - f1 * 2.0 * gmu[0];
+    A[1][2] = -f1 * gmu[1];
+    A[1][3] = -f1 * gmu[2];
+    A[1][4] = gmsinv - f1 * dlm * divu - f1 * dmu * 2.0 * S[0][0];
+    A[2][0] = 0; A[2][1] = -f1 * glm[1]; A[2][2] = u1 - f1 * gmu[0]; A[2][3] = 0;
+    A[2][4] = -f1 * dmu * 2.0 * S[1][0];
+    A[3][0] = 0; A[3][1] = -f1 * glm[2]; A[3][2] = 0; A[3][3] = u1 - f1 * gmu[0];
+    A[3][4] = -f1 * dmu * 2.0 * S[2][0];
+    f1 = gamma * rhoinv / (Pr * Re);
+    f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+    A[4][0] = 0;
+    A[4][1] = gamma1 * t - f2 * 2.0 * lm * divu - 4.0 * f2 * mu * S[0][0];
+    A[4][2] = -4.0 * f2 * mu * S[1][0];
+    A[4][3] = -4.0 * f2 * mu * S[2][0];
+    A[4][4] = u1 - f1 * (gcon[0] + dcon * gt[0]);
+    /* B, :525-570 */
+    B[0][0] = u2; B[0][1] = 0; B[0][2] = rho; B[0][3] = 0; B[0][4] = 0;
+    f1 = rhoinv / Re;
+    B[1][0] = 0; B[1][1] = u2 - f1 * gmu[1]; B[1][2] = -f1 * glm[0]; B[1][3] = 0;
+    B[1][4] = -f1 * dmu * 2.0 * S[0][1];
+    B[2][0] = rhoinv * t * gmsinv; B[2][1] = -f1 * gmu[0];
+    B[2][2] = u2 - f1 * glm[1] - f1 * 2.0 * gmu[1];
+    B[2][3] = -f1 * gmu[2];
+    B[2][4] = gmsinv - f1 * dlm * divu - f1 * dmu * 2.0 * S[1][1];
+    B[3][0] = 0; B[3][1] = 0; B[3][2] = -f1 * glm[2]; B[3][3] = u2 - f1 * gmu[1];
+    B[3][4] = -f1 * dmu * 2.0 * S[2][1];
+    f1 = gamma * rhoinv / (Pr * Re);
+    f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+    B[4][0] = 0;
+    B[4][1] = -4.0 * f2 * mu * S[0][1];
+    B[4][2] = gamma1 * t - f2 * 2.0 * lm * divu - 4.0 * f2 * mu * S[1][1];
+    B[4][3] = -4.0 * f2 * mu * S[2][1];
+    B[4][4] = u2 - f1 * (gcon[1] + dcon * gt[1]);
+    /* Vij, :580-585 */
+    f1 = 1.0 / Re; f2 = gamma / (Pr * Re);
+    Vij[0] = f1 * rhoinv * (lm + 2.0 * mu);
+    Vij[1] = f1 * rhoinv * mu;
+    Vij[2] = f2 * rhoinv * con;
+    Vij[3] = f1 * rhoinv * (lm + mu);
+    /* Ah, Bh, :590-690 */
+    double *Ah = c->Ah + 25 * q, *Bh = c->Bh + 25 * q;
+#define AH(r, cc) Ah[(r) + 5 * (cc)]
+#define BH(r, cc) Bh[(r) + 5 * (cc)]
+    const double *mm[2][5] = {{m1 + q, m2 + q, m11 + q, m12 + q, m22 + q},
+                              {n1 + q, n2 + q, n11 + q, n12 + q, n22 + q}};
+    for (int w = 0; w < 2; w++) {
+      double e1 = *mm[w][0], e2 = *mm[w][1], e11 = *mm[w][2], e12 = *mm[w][3], e22 = *mm[w][4];
+      double *H = w == 0 ? Ah : Bh;
+#define HH(r, cc) H[(r) + 5 * (cc)]
+      HH(0, 0) = A[0][0] * e1 + B[0][0] * e2;
+      HH(0, 1) = A[0][1] * e1;
+      HH(0, 2) = B[0][2] * e2;
+      HH(0, 3) = 0; HH(0, 4) = 0;
+      HH(1, 0) = A[1][0] * e1;
+      HH(1, 1) = A[1][1] * e1 + B[1][1] * e2 - Vij[0] * e11 - Vij[1] * e22;
+      HH(1, 2) = A[1][2] * e1 + B[1][2] * e2 - Vij[3] * e12;
+      HH(1, 3) = A[1][3] * e1;
+      HH(1, 4) = A[1][4] * e1 + B[1][4] * e2;
+      HH(2, 0) = B[2][0] * e2;
+      HH(2, 1) = A[2][1] * e1 + B[2][1] * e2 - Vij[3] * e12;
+      HH(2, 2) = A[2][2] * e1 + B[2][2] * e2 - Vij[1] * e11 - Vij[0] * e22;
+      HH(2, 3) = B[2][3] * e2;
+      HH(2, 4) = A[2][4] * e1 + B[2][4] * e2;
+      HH(3, 0) = 0;
+      HH(3, 1) = A[3][1] * e1;
+      HH(3, 2) = B[3][2] * e2;
+      HH(3, 3) = A[3][3] * e1 + B[3][3] * e2 - Vij[1] * e11 - Vij[1] * e22;
+      HH(3, 4) = A[3][4] * e1 + B[3][4] * e2;
+      for (int cc = 0; cc < 4; cc++) HH(4, cc) = A[4][cc] * e1 + B[4][cc] * e2;
+      HH(4, 4) = A[4][4] * e1 + B[4][4] * e2 - Vij[2] * e11 - Vij[2] * e22;
+#undef HH
+    }
+    if (!p->yper && p->wallt == 2 && q < nx)
+      for (int r = 0; r < 5; r++) BH(r, 4) = 0;
+#undef AH
+#undef BH
+    /* D, :700-820 (linear != 1 form) */
+    f1 = rhoinv / Re;
+    D[0][0] = divu; D[0][1] = grho[0]; D[0][2] = grho[1]; D[0][3] = grho[2]; D[0][4] = 0;
+    double Sjj[3] = {S1jj, S2jj, S3jj}, gdivu[3] = {g1divu, g2divu, g3divu};
+    for (int k = 0; k < 3; k++) {
+      D[1 + k][0] = rhoinv * (gmsinv * gt[k] - rhoinv * gp[k] +
+                              f1 * (glm[k] * divu + lm * gdivu[k]) +
+                              f1 * 2.0 * (gmu[0] * S[k][0] + gmu[1] * S[k][1] +
+                                          gmu[2] * S[k][2] + mu * Sjj[k]));
+      D[1 + k][1] = gu[k][0]; D[1 + k][2] = gu[k][1]; D[1 + k][3] = gu[k][2];
+      D[1 + k][4] = rhoinv * gmsinv * grho[k] - f1 * (gdlm[k] * divu + dlm * gdivu[k]) -
+                    f1 * 2.0 * (gdmu[0] * S[k][0] + gdmu[1] * S[k][1] + gdmu[2] * S[k][2] +
+                                dmu * Sjj[k]);
+    }
+    f1 = gamma * rhoinv / (Pr * Re);
+    f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+    double SS = S[0][0] * S[0][0] + S[0][1] * S[0][1] + S[0][2] * S[0][2] +
+                S[1][0] * S[1][0] + S[1][1] * S[1][1] + S[1][2] * S[1][2] +
+                S[2][0] * S[2][0] + S[2][1] * S[2][1] + S[2][2] * S[2][2];
+    D[4][0] = f1 * rhoinv * (gcon[0] * gt[0] + gcon[1] * gt[1] + gcon[2] * gt[2] + con * Lapt) +
+              f2 * rhoinv * lm * (divu * divu) + 2.0 * f2 * rhoinv * mu * SS;
+    D[4][1] = gt[0]; D[4][2] = gt[1]; D[4][3] = gt[2];
+    D[4][4] = gamma1 * divu;
+    D[4][4] = D[4][4] - f1 * (gdcon[0] * gt[0] + gdcon[1] * gt[1] + gdcon[2] * gt[2] + dcon * Lapt) -
+              f2 * dlm * (divu * divu) - 2.0 * f2 * dmu * SS;
+    double *Dh = c->Dh + 25 * q;
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++) Dh[r + 5 * cc] = D[r][cc];
+    if (kz != 0.0) {
+      Dh[1 + 5 * 1] += kz * kz * Vij[1];
+      Dh[2 + 5 * 2] += kz * kz * Vij[1];
+      Dh[3 + 5 * 3] += kz * kz * Vij[0];
+      Dh[4 + 5 * 4] += kz * kz * Vij[2];
+    }
+    /* compact viscous matrices, :850-913 */
+    double *V11 = c->Vh11 + 6 * q, *V12 = c->Vh12 + 6 * q, *V22 = c->Vh22 + 6 * q;
+    V11[0] = Vij[0] * m1m1 + Vij[1] * m2m2;
+    V11[1] = Vij[1] * m1m1 + Vij[0] * m2m2;
+    V11[2] = Vij[1] * m1m1 + Vij[1] * m2m2;
+    V11[3] = Vij[2] * m1m1 + Vij[2] * m2m2;
+    V11[4] = Vij[3] * m1m2;
+    V11[5] = Vij[3] * m1m2;
+    V12[0] = 2.0 * (Vij[0] * m1n1 + Vij[1] * m2n2);
+    V12[1] = 2.0 * (Vij[1] * m1n1 + Vij[0] * m2n2);
+    V12[2] = 2.0 * (Vij[1] * m1n1 + Vij[1] * m2n2);
+    V12[3] = 2.0 * (Vij[2] * m1n1 + Vij[2] * m2n2);
+    V12[4] = Vij[3] * (m1n2 + m2n1);
+    V12[5] = Vij[3] * (m1n2 + m2n1);
+    V22[0] = Vij[0] * n1n1 + Vij[1] * n2n2;
+    V22[1] = Vij[1] * n1n1 + Vij[0] * n2n2;
+    V22[2] = Vij[1] * n1n1 + Vij[1] * n2n2;
+    V22[3] = Vij[2] * n1n1 + Vij[2] * n2n2;
+    V22[4] = Vij[3] * n1n2;
+    V22[5] = Vij[3] * n1n2;
+    /* 3-d terms, :918-990 */
+    double *ABi = c->ABhi + 6 * q;
+    ABi[0] = kz * Vij[3] * M1;
+    ABi[1] = kz * Vij[3] * M2;
+    ABi[2] = kz * Vij[3] * N1;
+    ABi[3] = kz * Vij[3] * N2;
+    ABi[4] = 0; ABi[5] = 0;
+    f1 = rhoinv / Re;
+    C[0][0] = u3; C[0][1] = 0; C[0][2] = 0; C[0][3] = rho; C[0][4] = 0;
+    C[1][0] = 0; C[1][1] = u3 - f1 * gmu[2]; C[1][2] = 0; C[1][3] = -f1 * glm[0];
+    C[1][4] = -f1 * dmu * 2.0 * S[0][2];
+    C[2][0] = 0; C[2][1] = 0; C[2][2] = u3 - f1 * gmu[2]; C[2][3] = -f1 * glm[1];
+    C[2][4] = -f1 * dmu * 2.0 * S[1][2];
+    C[3][0] = rhoinv * t * gmsinv; C[3][1] = -f1 * gmu[0]; C[3][2] = -f1 * gmu[1];
+    C[3][3] = u3 - f1 * glm[2] - f1 * 2.0 * gmu[2];
+    C[3][4] = gmsinv - f1 * dlm * divu - f1 * dmu * 2.0 * S[2][2];
+    f1 = gamma * rhoinv / (Pr * Re);
+    f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+    C[4][0] = 0;
+    C[4][1] = -4.0 * f2 * mu * S[0][2];
+    C[4][2] = -4.0 * f2 * mu * S[1][2];
+    C[4][3] = gamma1 * t - f2 * 2.0 * lm * divu - 4.0 * f2 * mu * S[2][2];
+    C[4][4] = u3 - f1 * (gcon[2] + dcon * gt[2]);
+    double *Dhi = c->Dhi + 25 * q;
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++) Dhi[r + 5 * cc] = kz * C[r][cc];
+    for (int r = 0; r < 5; r++) Dhi[r + 5 * r] = Dhi[r + 5 * r] - p->omega;
+  }
+  free(g);
+  return 0;
+}
+
+/* src/temporal.f90:40-160 (acoustic CFL, global or local time step) */
+int orc_set_time_step(const orc_params *p, const double *vm, const double *metric,
+                      double *dtl) {
+  if (p->mattyp == 1) return -1;
+  int nx = p->nx, ny = p->ny;
+  long P = (long)nx * ny;
+  double dxi = 1.0 / (double)(nx - 1), deta = 1.0 / (double)(ny - 1);
+  const double *m1 = metric, *m2 = metric + P, *n1 = metric + 2 * P, *n2 = metric + 3 * P;
+  double cfla = -1.0;
+  for (long q = 0; q < P; q++) {
+    double vinv = 1.0 / (m1[q] * n2[q] - m2[q] * n1[q]);
+    double sxj = n2[q] * dxi * vinv, syj = -m2[q] * dxi * vinv;
+    double sxi = -n1[q] * deta * vinv, syi = m1[q] * deta * vinv;
+    double volinv = 1.0 / (dxi * deta * vinv);
+    double acoustic = sqrt(sxi * sxi + 2.0 * sxi * syi + syi * syi + sxj * sxj +
+                           2.0 * sxj * syj + syj * syj) / p->Ma;
+    double u1 = vm[5 * q + 1], u2 = vm[5 * q + 2], t = vm[5 * q + 4];
+    double ct = sqrt(t);
+    dtl[q] = (fabs(u1 * sxi + u2 * syi) + fabs(u1 * sxj + u2 * syj) + ct * acoustic) * volinv;
+    if (dtl[q] > cfla) cfla = dtl[q];
+  }
+  double cfl = cfla, delt;
+  if (cfl * p->dtmax <= p->cflmax) {
+    for (long q = 0; q < P; q++) dtl[q] = p->dtmax;
+  } else {
+    delt = p->cflmax / cfl;
+    if (p->loctime == 0)
+      for (long q = 0; q < P; q++) dtl[q] = delt;
+    else
+      for (long q = 0; q < P; q++) dtl[q] = p->cflmax / dtl[q];
+  }
+  return 0;
+}
+
+/* ---- step driver: src/impdrv3d.f90:146-356 (impl = 1, niter = 1) --------- */
+int orc_factor(const orc_params *p, const orc_coeffs *c, const double *dtl,
+               zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+               zdouble *pery, zdouble *per2y) {
+  int nx = p->nx, ny = p->ny;
+  long P = (long)nx * ny;
+  if (orc_lhs(p, 1, c, dtl, maty) != 0) return -1;     /* maty as scratch */
+  if (p->xper) {
+    memcpy(matx, maty, sizeof(zdouble) * 125 * P);
+    if (orc_cpenta_p(2, ny, nx, matx, NULL, perx, per2x, 0, p->periodic_exact)) return -1;
+  } else {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        memcpy(matx + 125 * (j + (long)ny * i), maty + 125 * (i + (long)nx * j),
+               sizeof(zdouble) * 125);
+    if (orc_cpenta_bc(1, ny, nx, matx, NULL, 0)) return -1;
+  }
+  if (orc_lhs(p, 2, c, dtl, maty) != 0) return -1;
+  if (p->yper) return orc_cpenta_p(1, nx, ny, maty, NULL, pery, per2y, 0, p->periodic_exact);
+  return orc_cpenta_bc(1, nx, ny, maty, NULL, 0);
+}
+
+int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
+             zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+             zdouble *pery, zdouble *per2y, zdouble *v, zdouble *dv, zdouble *work) {
+  int nx = p->nx, ny = p->ny;
+  long P = (long)nx * ny;
+  zdouble *vold = work, *r = work + 5 * P, *g = work + 10 * P;
+  memcpy(vold, v, sizeof(zdouble) * 5 * P);
+  if (orc_itrbc3d(p, v)) return -1;
+  if (orc_lrhs3d(p, c, v, r, g)) return -1;
+  orc_rbe3d(p, r, v, vold, dtl);
+  if (orc_rhsbc3d(p, r, v)) return -1;
+  if (p->xper) {
+    if (orc_cpenta_p(2, ny, nx, matx, r, perx, per2x, 1, p->periodic_exact)) return -1;
+  } else {
+    zdouble *t = dv;
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        for (int d = 0; d < 5; d++) t[d + 5 * (j + (long)ny * i)] = r[d + 5 * (i + (long)nx * j)];
+    if (orc_cpenta_bc(1, ny, nx, matx, t, 1)) return -1;
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        for (int d = 0; d < 5; d++) r[d + 5 * (i + (long)nx * j)] = t[d + 5 * (j + (long)ny * i)];
+  }
+  if (p->yper) {
+    if (orc_cpenta_p(1, nx, ny, maty, r, pery, per2y, 1, p->periodic_exact)) return -1;
+  } else if (orc_cpenta_bc(1, nx, ny, maty, r, 1)) return -1;
+  for (long e = 0; e < 5 * P; e++) { v[e] += r[e]; dv[e] = r[e]; }
+  return orc_itrbc3d(p, v);
+}
