@@ -1,0 +1,77 @@
+"""Build step for the native library: hipcc for gfx950, flang for the
+Fortran drop-in layer.  Outputs live next to the sources (in-tree) so they
+travel with the repository snapshot:
+
+  lns3d_amd/liblns3d_amd.so       HIP kernels + C ABI (include/lns3d_amd.h)
+  lns3d_amd/liblns3d_fortran.so   Fortran drop-in subroutines (iso_c_binding)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "liblns3d_amd.so")
+FLIB = os.path.join(PKG, "liblns3d_fortran.so")
+
+HIP_SOURCES = ["penta.hip", "step.hip", "capi_penta.cpp", "capi_step.cpp"]
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument", "-Wno-unused-result"]
+
+
+def _hipcc():
+    for c in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _flang():
+    for c in (shutil.which("flang"), "/opt/rocm/lib/llvm/bin/flang"):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def _run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("command failed: " + " ".join(cmd))
+
+
+def build(verbose: bool = False) -> str:
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    for s in HIP_SOURCES:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(objdir, s + ".o")
+        lang = ["-x", "hip"] if s.endswith(".hip") else []
+        _run([hipcc] + HIPFLAGS + lang + ["-c", src, "-o", obj], CSRC)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, CSRC)
+    os.replace(tmp, LIB)
+    fl = _flang()
+    fsrc = os.path.join(PKG, "fortran", "lns3d_hip.f90")
+    if fl and os.path.exists(fsrc):
+        ftmp = FLIB + ".tmp"
+        _run([fl, "-O2", "-fPIC", "-shared", "-J", objdir, "-o", ftmp, fsrc,
+              "-L" + PKG, "-llns3d_amd", "-Wl,-rpath,$ORIGIN"], PKG)
+        os.replace(ftmp, FLIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

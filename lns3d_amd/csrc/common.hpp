@@ -1,0 +1,90 @@
+// Shared device helpers for the LNS3D implicit-step kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lns3d {
+
+// complex value with explicit real/imag parts; T = double or float
+template <typename T> struct cx {
+  T re, im;
+};
+
+template <typename T> __host__ __device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
+template <typename T> __host__ __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T> __host__ __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T> __host__ __device__ __forceinline__ cx<T> operator-(cx<T> a) { return {-a.re, -a.im}; }
+template <typename T> __host__ __device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T> __host__ __device__ __forceinline__ cx<T> operator*(T s, cx<T> b) { return {s * b.re, s * b.im}; }
+// acc + a*b
+template <typename T> __host__ __device__ __forceinline__ cx<T> cfma(cx<T> a, cx<T> b, cx<T> acc) {
+  acc.re = fma(a.re, b.re, acc.re);
+  acc.re = fma(-a.im, b.im, acc.re);
+  acc.im = fma(a.re, b.im, acc.im);
+  acc.im = fma(a.im, b.re, acc.im);
+  return acc;
+}
+// a / b, Smith's algorithm (Smith, CACM 5(8), 1962)
+template <typename T> __host__ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    T r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  T r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+template <typename T> __host__ __device__ __forceinline__ cx<T> crecip(cx<T> b) {
+  return cdiv(cx<T>{T(1), T(0)}, b);
+}
+
+// wave-uniform broadcast of a lane's complex value (one line per wave)
+__device__ __forceinline__ cx<double> bcast(cx<double> v, int lane) {
+  union { double d; int i[2]; } a, b;
+  a.d = v.re; b.d = v.im;
+  a.i[0] = __builtin_amdgcn_readlane(a.i[0], lane);
+  a.i[1] = __builtin_amdgcn_readlane(a.i[1], lane);
+  b.i[0] = __builtin_amdgcn_readlane(b.i[0], lane);
+  b.i[1] = __builtin_amdgcn_readlane(b.i[1], lane);
+  return {a.d, b.d};
+}
+__device__ __forceinline__ cx<float> bcast(cx<float> v, int lane) {
+  union { float f; int i; } a, b;
+  a.f = v.re; b.f = v.im;
+  a.i = __builtin_amdgcn_readlane(a.i, lane);
+  b.i = __builtin_amdgcn_readlane(b.i, lane);
+  return {a.f, b.f};
+}
+
+// finite-difference coefficient tables of the scheme (src/stencil.f90)
+namespace st {
+constexpr double ga1 = 8.333333333333333333333E-02, ga2 = -6.666666666666666666667E-01,
+                 ga3 = 6.666666666666666666667E-01, ga4 = -8.333333333333333333333E-02;
+constexpr double gc1 = -2.083333333333333333333E+00, gc2 = 4.000000000000000000000E+00,
+                 gc3 = -3.000000000000000000000E+00, gc4 = 1.333333333333333333333E+00,
+                 gc5 = -2.500000000000000000000E-01;
+constexpr double gb1 = -2.500000000000000000000E-01, gb2 = -8.333333333333333333333E-01,
+                 gb3 = 1.500000000000000000000E+00, gb4 = -5.000000000000000000000E-01,
+                 gb5 = 8.333333333333333333333E-02;
+constexpr double da1 = -8.333333333333333333333E-02, da2 = 1.333333333333333333333E+00,
+                 da3 = -2.500000000000000000000E+00, da4 = 1.333333333333333333333E+00,
+                 da5 = -8.333333333333333333333E-02;
+constexpr double db1 = 9.166666666666666666667E-01, db2 = -1.666666666666666666667E+00,
+                 db3 = 5.000000000000000000000E-01, db4 = 3.333333333333333333333E-01,
+                 db5 = -8.333333333333333333333E-02;
+constexpr double dd1 = 2.916666666666666666667E+00, dd2 = -8.666666666666666666667E+00,
+                 dd3 = 9.500000000000000000000E+00, dd4 = -4.666666666666666666667E+00,
+                 dd5 = 9.166666666666666666667E-01;
+constexpr double pt66 = 6.6666666666666666666e-1;
+}  // namespace st
+
+#define LNS3D_CHECK(expr)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) return lns3d::set_error(__FILE__, __LINE__, hipGetErrorString(e_)); \
+  } while (0)
+
+int set_error(const char *file, int line, const char *msg);
+
+}  // namespace lns3d

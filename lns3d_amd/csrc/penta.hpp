@@ -1,0 +1,33 @@
+// Host interface of the block-pentadiagonal line solvers (penta.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lns3d {
+
+// One set of independent lines of equal length, factored once, solved many
+// times.  Device buffers:
+//   lu, per, per2 : LU working copy, line-interleaved (np rows)
+//   fwd, bwd, tail: solve records of non-periodic lines (prec 0: complex
+//                   double, 1: complex float)
+struct PentaPlan {
+  int periodic = 0, exact = 0;
+  int n = 0, np = 0;   // unknown rows, stored rows (np = n+1 when periodic)
+  long L = 0;          // number of lines
+  int prec = 0;
+  void *lu = nullptr, *per = nullptr, *per2 = nullptr;
+  void *fwd = nullptr, *bwd = nullptr, *tail = nullptr;
+};
+
+int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec);
+void penta_plan_destroy(PentaPlan *pl);
+// copy a device matrix in the reference's layout mat(5,5,5,...) into the
+// working copy; block (line l, row i) starts at complex offset (i*sRm + l*sLm)*125
+int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStream_t s);
+// assembled working copy -> factors (+ records for non-periodic lines)
+int penta_factor(PentaPlan *pl, hipStream_t s);
+// in-place solve of rhs (complex double); component d of (line l, row i) at
+// complex offset (l*sL + i*sR)*5 + d
+int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);
+int penta_solve_reference_order(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);
+
+}  // namespace lns3d
