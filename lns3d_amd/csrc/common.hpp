@@ -57,6 +57,24 @@ __device__ __forceinline__ cx<float> bcast(cx<float> v, int lane) {
   return {a.f, b.f};
 }
 
+// broadcast lane q's complex value to the 16 lanes of its DPP row
+// (v_mov_b64_dpp row_newbcast:q); q must be a compile-time constant
+template <int Q> __device__ __forceinline__ cx<double> row_bcast(cx<double> v) {
+  long a = __builtin_bit_cast(long, v.re), b = __builtin_bit_cast(long, v.im);
+  a = __builtin_amdgcn_update_dpp(0L, a, 0x150 + Q, 0xF, 0xF, false);
+  b = __builtin_amdgcn_update_dpp(0L, b, 0x150 + Q, 0xF, 0xF, false);
+  return {__builtin_bit_cast(double, a), __builtin_bit_cast(double, b)};
+}
+template <int Q> __device__ __forceinline__ cx<float> row_bcast(cx<float> v) {
+  long a = __builtin_bit_cast(long, v);
+  a = __builtin_amdgcn_update_dpp(0L, a, 0x150 + Q, 0xF, 0xF, false);
+  return __builtin_bit_cast(cx<float>, a);
+}
+template <typename T> __device__ __forceinline__ void row_bcast5(cx<T> v, cx<T> (&y)[5]) {
+  y[0] = row_bcast<0>(v); y[1] = row_bcast<1>(v); y[2] = row_bcast<2>(v);
+  y[3] = row_bcast<3>(v); y[4] = row_bcast<4>(v);
+}
+
 // finite-difference coefficient tables of the scheme (src/stencil.f90)
 namespace st {
 constexpr double ga1 = 8.333333333333333333333E-02, ga2 = -6.666666666666666666667E-01,

@@ -221,7 +221,7 @@ __global__ void k_solve_lines(LineGeom g, const cx<T> *mat_, const cx<T> *per_, 
 // stage 2: records for the fast non-periodic solve
 // fwd[(line*n + i)*60 + f]: f = m*10 + q -> N2(m,q); m*10 + 5 + q -> N1(m,q);
 //                           50 + k -> strict lower of Linv, k = l*(l-1)/2 + q
-// bwd[(line*n + i)*65 + f]: f = m*10 + q -> Q4(m,q); m*10 + 5 + q -> Q5(m,q);
+// bwd[(line*n + i)*66 + f] (one complex of padding): f = m*10 + q -> Q4(m,q); m*10 + 5 + q -> Q5(m,q);
 //                           50 + k -> upper of Uinv (row-major, incl. diagonal)
 // tail[line*150 + ...]: 0 row n-2 N(col n-5); 25 row n-1 N(col n-5);
 //                       50 row n-1 N(col n-4); 75 row 0 Q(col 3);
@@ -252,7 +252,7 @@ __global__ void k_records_bc(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR>
         Ui[m][c] = cdiv(acc, D(m, m));
       }
     cx<TR> *F = fwd + ((long)line * n + i) * 60;
-    cx<TR> *B = bwd + ((long)line * n + i) * 65;
+    cx<TR> *B = bwd + ((long)line * n + i) * 66;
     cx<TR> *Tl = tail + line * 150;
     auto Nblk = [&](int col, cx<TR> *dst, int ld) {   // Linv * M(i,col)
       Blk<TF> M;
@@ -293,12 +293,40 @@ __global__ void k_records_bc(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR>
 
 // ---------------------------------------------------------------------------
 // stage 3: streaming solve of non-periodic lines
+//
+// LDS per buffer: nr record rows copied verbatim from HBM (fwd: 60, bwd: 65
+// complex per row) with global_load_lds_dwordx4 (no VGPR round trip), plus
+// the parallel parts c_i = Linv_i b_i / d_i = Uinv_i y_i computed by the
+// loader waves into cb.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <typename T>
+__device__ __forceinline__ void copy_rows_to_lds(const cx<T> *src, long src_elems_left, cx<T> *dst,
+                                                 int nelem, int wave_in, int nwaves, int lane) {
+  // copy nelem complex values src[0..nelem) -> dst[0..nelem) in 1 KB blocks
+  // per wave instruction; lanes past the end read a clamped (valid) address
+  // and write into the buffer's padding
+  constexpr int PER_LANE = 16 / sizeof(cx<T>);          // complex values per 16 B
+  const int nblk = (nelem * (int)sizeof(cx<T>) + 1023) / 1024;
+  for (int b = wave_in; b < nblk; b += nwaves) {
+    long e = (long)b * 64 * PER_LANE + (long)lane * PER_LANE;
+    long ec = e < src_elems_left - PER_LANE ? e : src_elems_left - PER_LANE;
+    if (ec < 0) ec = 0;
+    __builtin_amdgcn_global_load_lds((const void *)(src + ec),
+                                     (lds_void *)(dst + (long)b * 64 * PER_LANE), 16, 0, 0);
+  }
+}
+
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict__ fwd,
                                                  const cx<T> *__restrict__ bwd,
                                                  const cx<T> *__restrict__ tail, double2 *rhs,
-                                                 long sL, long sR) {
-  __shared__ cx<T> buf[2][R][56];
+                                                 long sL, long sR, long total_rows) {
+  constexpr int RECW = 66;                                       // LDS row stride (complex)
+  constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
+  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][BUFE];
+  __shared__ cx<T> cb[2][R][5];
+  __shared__ cx<T> saved[2][5];   // y_{n-5}, y_{n-4} (forward); x_3, x_4 (backward)
   const long line = blockIdx.x;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -306,64 +334,94 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
   const int nchunk = (n + R - 1) / R;
   double2 *X = rhs + line * sL * 5;
   const long rs = sR * 5;
+  const cx<T> *tl = tail + line * 150;
 
-  // ---- forward ----
+  // ---------------- forward: y_i = c_i + N1 y_{i-2} + N2 y_{i-1} ----------
   auto load_fwd = [&](int k) {
     int r0 = k * R, nr = min(R, n - r0);
-    int t = tid - 64;
-    cx<T> (*B)[56] = buf[k & 1];
     const cx<T> *F = fwd + ((long)line * n + r0) * 60;
-    for (int e = t; e < nr * 50; e += 192) {
-      int row = e / 50, f = e - row * 50;
-      B[row][f] = F[(long)row * 60 + f];
-    }
-    for (int e = t; e < nr * 5; e += 192) {
-      int row = e / 5, mm = e - row * 5;
+    copy_rows_to_lds(F, (total_rows - ((long)line * n + r0)) * 60, rec[k & 1], nr * 60, wave - 1, 3,
+                     lane);
+    int t = tid - 64;
+    if (t < nr * 5) {
+      int row = t / 5, mm = t - row * 5;
       const double2 *b = X + (long)(r0 + row) * rs;
-      const cx<T> *Li = F + (long)row * 60 + 50;
+      const cx<T> *Li = F + (long)row * 60 + 50 + mm * (mm - 1) / 2;
       double2 bm = b[mm];
       cx<T> acc{(T)bm.x, (T)bm.y};
-      int k0 = mm * (mm - 1) / 2;
-      for (int q = 0; q < mm; q++) {
-        double2 bq = b[q];
-        acc = cfma(Li[k0 + q], cx<T>{(T)bq.x, (T)bq.y}, acc);
-      }
-      B[row][50 + mm] = acc;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q < mm) {
+          double2 bq = b[q];
+          acc = cfma(Li[q], cx<T>{(T)bq.x, (T)bq.y}, acc);
+        }
+      cb[k & 1][row][mm] = acc;
     }
+    __builtin_amdgcn_s_waitcnt(0);
   };
-  cx<T> y1[5], y2[5], y4s[5], y5s[5];
-  cx<T> X1[5], X2[5], X3[5];
-  for (int q = 0; q < 5; q++) { y1[q] = y2[q] = y4s[q] = y5s[q] = cx<T>{0, 0}; }
-  if (wave == 0)
-    for (int q = 0; q < 5; q++) {
-      X1[q] = tail[line * 150 + 0 + m * 5 + q];
-      X2[q] = tail[line * 150 + 25 + m * 5 + q];
-      X3[q] = tail[line * 150 + 50 + m * 5 + q];
-    }
+  cx<T> y1[5], y2[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) { y1[q] = cx<T>{0, 0}; y2[q] = cx<T>{0, 0}; }
   if (wave != 0) load_fwd(0);
   __syncthreads();
+  // one row of the recurrence on lanes 0..4 (lane m = component m):
+  // out = c + W1 * ym2 + W2 * ym1, then out is broadcast into ym2 (which
+  // held the row two back and is dead afterwards)
+  auto row_step = [&](const cx<T> (&w)[10], cx<T> c, cx<T> (&ym2)[5], const cx<T> (&ym1)[5],
+                      int i, bool fwd_pass) {
+    cx<T> a0 = c, p0{0, 0}, p1{0, 0};
+#pragma unroll
+    for (int q = 0; q < 5; q++) a0 = cfma(w[5 + q], ym2[q], a0);
+    p0 = cfma(w[0], ym1[0], p0); p1 = cfma(w[1], ym1[1], p1);
+    p0 = cfma(w[2], ym1[2], p0); p1 = cfma(w[3], ym1[3], p1);
+    p0 = cfma(w[4], ym1[4], p0);
+    if (fwd_pass) {
+      if (i >= n - 2) {      // rows n-2, n-1: blocks in columns n-5, n-4
+        const cx<T> *Xa = tl + (i == n - 2 ? 0 : 25) + m * 5;
+#pragma unroll
+        for (int q = 0; q < 5; q++) a0 = cfma(Xa[q], saved[0][q], a0);
+        if (i == n - 1)
+#pragma unroll
+          for (int q = 0; q < 5; q++) a0 = cfma(tl[50 + m * 5 + q], saved[1][q], a0);
+      }
+    } else if (i <= 1) {     // rows 1, 0: blocks in columns 3, 4
+      const cx<T> *Qa = tl + (i == 1 ? 125 : 100) + m * 5;
+#pragma unroll
+      for (int q = 0; q < 5; q++) a0 = cfma(Qa[q], saved[1][q], a0);
+      if (i == 0)
+#pragma unroll
+        for (int q = 0; q < 5; q++) a0 = cfma(tl[75 + m * 5 + q], saved[0][q], a0);
+    }
+    cx<T> acc = (a0 + p1) + p0;
+    if (lane < 5) X[(long)i * rs + lane] = make_double2(acc.re, acc.im);
+    if (lane < 5) {
+      if (fwd_pass && (i == n - 5 || i == n - 4)) saved[i - (n - 5)][lane] = acc;
+      if (!fwd_pass && (i == 3 || i == 4)) saved[i - 3][lane] = acc;
+    }
+    row_bcast5(acc, ym2);
+  };
+  auto lds_row = [&](const cx<T> *B, int stride, int row, int kk, cx<T> (&w)[10], cx<T> &c) {
+    const cx<T> *p = B + row * stride + m * 10;
+#pragma unroll
+    for (int q = 0; q < 10; q++) w[q] = p[q];
+    c = cb[kk][row][m];
+  };
+  cx<T> wA[10], wB[10], cA, cB;
   for (int k = 0; k < nchunk; k++) {
     if (wave == 0) {
       int r0 = k * R, nr = min(R, n - r0);
-      cx<T> (*B)[56] = buf[k & 1];
-      for (int row = 0; row < nr; row++) {
-        int i = r0 + row;
-        cx<T> acc = B[row][50 + m];
-        const cx<T> *Nm = &B[row][m * 10];
-        for (int q = 0; q < 5; q++) acc = cfma(Nm[5 + q], y2[q], acc);
-        for (int q = 0; q < 5; q++) acc = cfma(Nm[q], y1[q], acc);
-        if (i == n - 2)
-          for (int q = 0; q < 5; q++) acc = cfma(X1[q], y5s[q], acc);
-        if (i == n - 1) {
-          for (int q = 0; q < 5; q++) acc = cfma(X2[q], y5s[q], acc);
-          for (int q = 0; q < 5; q++) acc = cfma(X3[q], y4s[q], acc);
-        }
-        if (lane < 5) X[(long)i * rs + lane] = make_double2(acc.re, acc.im);
-        for (int q = 0; q < 5; q++) { y2[q] = y1[q]; y1[q] = bcast(acc, q); }
-        if (i == n - 5)
-          for (int q = 0; q < 5; q++) y5s[q] = y1[q];
-        if (i == n - 4)
-          for (int q = 0; q < 5; q++) y4s[q] = y1[q];
+      const cx<T> *B = rec[k & 1];
+      lds_row(B, 60, 0, k & 1, wA, cA);
+      for (int row = 0; row < nr; row += 2) {
+        // prefetches are unconditional (row index clamped into the chunk)
+        lds_row(B, 60, min(row + 1, nr - 1), k & 1, wB, cB);
+        row_step(wA, cA, y2, y1, r0 + row, true);             // -> y2 holds y_i
+        lds_row(B, 60, min(row + 2, nr - 1), k & 1, wA, cA);
+        if (row + 1 < nr) row_step(wB, cB, y1, y2, r0 + row + 1, true);
+      }
+      if (nr & 1) {          // odd tail: restore (older, newer) order
+#pragma unroll
+        for (int q = 0; q < 5; q++) { cx<T> t = y1[q]; y1[q] = y2[q]; y2[q] = t; }
       }
     } else if (k + 1 < nchunk) {
       load_fwd(k + 1);
@@ -371,61 +429,48 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
     __syncthreads();
   }
 
-  // ---- backward ----
+  // ---------------- backward: x_i = d_i + Q4 x_{i+1} + Q5 x_{i+2} ---------
+  // chunk k holds rows rlo..rtop in ascending order in LDS
   auto load_bwd = [&](int k) {
-    int rtop = n - 1 - k * R, nr = min(R, rtop + 1);
+    int rtop = n - 1 - k * R, nr = min(R, rtop + 1), rlo = rtop - nr + 1;
+    const cx<T> *Bw = bwd + ((long)line * n + rlo) * 66;
+    copy_rows_to_lds(Bw, (total_rows - ((long)line * n + rlo)) * 66, rec[k & 1], nr * 66, wave - 1, 3,
+                     lane);
     int t = tid - 64;
-    cx<T> (*B)[56] = buf[k & 1];
-    for (int e = t; e < nr * 50; e += 192) {
-      int row = e / 50, f = e - row * 50;
-      B[row][f] = bwd[((long)line * n + rtop - row) * 65 + f];
-    }
-    for (int e = t; e < nr * 5; e += 192) {
-      int row = e / 5, mm = e - row * 5;
-      int i = rtop - row;
-      const double2 *y = X + (long)i * rs;
-      const cx<T> *Ui = bwd + ((long)line * n + i) * 65 + 50;
-      int k0 = mm * 5 - mm * (mm - 1) / 2;   // start of row mm in packed upper
+    if (t < nr * 5) {
+      int row = t / 5, mm = t - row * 5;     // row counted from rlo
+      const double2 *y = X + (long)(rlo + row) * rs;
+      const cx<T> *Ui = Bw + (long)row * 66 + 50 + mm * 5 - mm * (mm - 1) / 2;
       cx<T> acc{0, 0};
-      for (int q = mm; q < 5; q++) {
-        double2 yq = y[q];
-        acc = cfma(Ui[k0 + q - mm], cx<T>{(T)yq.x, (T)yq.y}, acc);
-      }
-      B[row][50 + mm] = acc;
+#pragma unroll
+      for (int q = 0; q < 5; q++)
+        if (q < 5 - mm) {
+          double2 yq = y[mm + q];
+          acc = cfma(Ui[q], cx<T>{(T)yq.x, (T)yq.y}, acc);
+        }
+      cb[k & 1][row][mm] = acc;
     }
+    __builtin_amdgcn_s_waitcnt(0);
   };
-  cx<T> Q03[5], Q04[5], Q14[5], x3s[5], x4s[5];
-  if (wave == 0)
-    for (int q = 0; q < 5; q++) {
-      Q03[q] = tail[line * 150 + 75 + m * 5 + q];
-      Q04[q] = tail[line * 150 + 100 + m * 5 + q];
-      Q14[q] = tail[line * 150 + 125 + m * 5 + q];
-    }
-  for (int q = 0; q < 5; q++) { y1[q] = y2[q] = x3s[q] = x4s[q] = cx<T>{0, 0}; }
+#pragma unroll
+  for (int q = 0; q < 5; q++) { y1[q] = cx<T>{0, 0}; y2[q] = cx<T>{0, 0}; }
   if (wave != 0) load_bwd(0);
   __syncthreads();
   for (int k = 0; k < nchunk; k++) {
     if (wave == 0) {
-      int rtop = n - 1 - k * R, nr = min(R, rtop + 1);
-      cx<T> (*B)[56] = buf[k & 1];
-      for (int row = 0; row < nr; row++) {
-        int i = rtop - row;
-        cx<T> acc = B[row][50 + m];
-        const cx<T> *Qm = &B[row][m * 10];
-        for (int q = 0; q < 5; q++) acc = cfma(Qm[5 + q], y2[q], acc);
-        for (int q = 0; q < 5; q++) acc = cfma(Qm[q], y1[q], acc);
-        if (i == 1)
-          for (int q = 0; q < 5; q++) acc = cfma(Q14[q], x4s[q], acc);
-        if (i == 0) {
-          for (int q = 0; q < 5; q++) acc = cfma(Q03[q], x3s[q], acc);
-          for (int q = 0; q < 5; q++) acc = cfma(Q04[q], x4s[q], acc);
-        }
-        if (lane < 5) X[(long)i * rs + lane] = make_double2(acc.re, acc.im);
-        for (int q = 0; q < 5; q++) { y2[q] = y1[q]; y1[q] = bcast(acc, q); }
-        if (i == 4)
-          for (int q = 0; q < 5; q++) x4s[q] = y1[q];
-        if (i == 3)
-          for (int q = 0; q < 5; q++) x3s[q] = y1[q];
+      int rtop = n - 1 - k * R, nr = min(R, rtop + 1), rlo = rtop - nr + 1;
+      const cx<T> *B = rec[k & 1];
+      lds_row(B, 66, nr - 1, k & 1, wA, cA);
+      for (int j = 0; j < nr; j += 2) {       // j counts rows from the top of the chunk
+        int row = nr - 1 - j;
+        lds_row(B, 66, max(row - 1, 0), k & 1, wB, cB);
+        row_step(wA, cA, y2, y1, rlo + row, false);
+        lds_row(B, 66, max(row - 2, 0), k & 1, wA, cA);
+        if (j + 1 < nr) row_step(wB, cB, y1, y2, rlo + row - 1, false);
+      }
+      if (nr & 1) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) { cx<T> t = y1[q]; y1[q] = y2[q]; y2[q] = t; }
       }
     } else if (k + 1 < nchunk) {
       load_bwd(k + 1);
@@ -472,7 +517,7 @@ int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long l
   } else {
     size_t es = prec ? sizeof(cx<float>) : sizeof(cx<double>);
     LNS3D_CHECK(hipMalloc(&pl->fwd, (size_t)pl->n * 60 * lines * es));
-    LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 65 * lines * es));
+    LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * es));
     LNS3D_CHECK(hipMalloc(&pl->tail, (size_t)150 * lines * es));
   }
   return 0;
@@ -521,18 +566,20 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
                        (const cx<double> *)pl->lu, (const cx<double> *)pl->per,
                        (const cx<double> *)pl->per2, (double2 *)rhs, sL, sR);
-  } else if (pl->prec) {
-    hipLaunchKernelGGL((k_solve_bc<float, 32>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n,
-                       (const cx<float> *)pl->fwd, (const cx<float> *)pl->bwd,
-                       (const cx<float> *)pl->tail, (double2 *)rhs, sL, sR);
-  } else if (pl->L >= 512) {
-    hipLaunchKernelGGL((k_solve_bc<double, 16>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n,
-                       (const cx<double> *)pl->fwd, (const cx<double> *)pl->bwd,
-                       (const cx<double> *)pl->tail, (double2 *)rhs, sL, sR);
   } else {
-    hipLaunchKernelGGL((k_solve_bc<double, 32>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n,
-                       (const cx<double> *)pl->fwd, (const cx<double> *)pl->bwd,
-                       (const cx<double> *)pl->tail, (double2 *)rhs, sL, sR);
+    long tot = (long)pl->n * pl->L;
+    bool many = pl->L >= 512;    // several lines per CU: smaller LDS ring
+    if (pl->prec) {
+      auto *f = (const cx<float> *)pl->fwd; auto *b = (const cx<float> *)pl->bwd;
+      auto *t = (const cx<float> *)pl->tail;
+      if (many) hipLaunchKernelGGL((k_solve_bc<float, 28>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
+      else hipLaunchKernelGGL((k_solve_bc<float, 56>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
+    } else {
+      auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
+      auto *t = (const cx<double> *)pl->tail;
+      if (many) hipLaunchKernelGGL((k_solve_bc<double, 14>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
+      else hipLaunchKernelGGL((k_solve_bc<double, 28>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
+    }
   }
   LNS3D_CHECK(hipGetLastError());
   return 0;
