@@ -26,6 +26,27 @@ extern "C" {
 
 const char *lns3d_last_error(void);
 
+/* Scheme, physics and boundary selectors: the module state of src/global.f90
+ * that the step routines read (src/input.f90 deck entries of the same names).
+ * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
+ * top = left = right = 0; linear = 2; mattyp in {0,2,3}; loctime in {0,1};
+ * Navier-Stokes (viscous) boundary rows; no sponge; eps_e = 0.  Other values
+ * are rejected with LNS3D_EUNSUPPORTED. */
+typedef struct {
+  int nx, ny;
+  int optx, opty;
+  int xper, yper;
+  int wall, wallt, top, left, right;
+  int linear, mattyp, loctime;
+  double Ma, Re, Pr, gamma;
+  double kz;        /* spanwise wavenumber (beta) */
+  double omega;     /* frequency, enters Dhi as -omega (src/genmtrx.f90:981-989) */
+  double cflmax, dtmax;
+  double alfa;      /* time-scheme factor: 1 for backward Euler */
+  int periodic_exact;
+  int precision;    /* 0: complex-double factors; 1: complex-float factors */
+} lns3d_params;
+
 /* ======================================================================
  * Block-pentadiagonal line solvers, drop-in form (host arrays).
  * Argument meaning, layouts and code (0 factor, 1 solve, 2 both) as in the
@@ -74,6 +95,52 @@ int lns3d_penta_solve(lns3d_penta *h, void *rhs_dev, void *stream);
 /* same result, one lane per line in the reference's operation order */
 int lns3d_penta_solve_reference_order(lns3d_penta *h, void *rhs_dev, void *stream);
 int lns3d_penta_destroy(lns3d_penta *h);
+
+/* ======================================================================
+ * Device-resident implicit step (src/impdrv3d.f90:130-406, impl = 1,
+ * niter = 1).  One context per problem; every array of the step stays in
+ * HBM.  Field arrays v, r, dv are complex (5,nx,ny) in the reference layout.
+ * ====================================================================== */
+typedef struct lns3d_ctx lns3d_ctx;
+int lns3d_ctx_create(const lns3d_params *p, lns3d_ctx **out);
+int lns3d_ctx_destroy(lns3d_ctx *c);
+int lns3d_ctx_set_stream(lns3d_ctx *c, void *stream);
+int lns3d_ctx_synchronize(lns3d_ctx *c);
+/* metric (nx,ny,10: m1,m2,n1,n2,m11,m12,m22,n11,n12,n22; src/gengrid.f90:76-80)
+ * and mean flow vm (5,nx,ny), host arrays; generates on the device the
+ * coefficients of src/genmtrx.f90:2 (Ah,Bh,Dh,Dhi,Vh11,Vh12,Vh22,ABhi) and
+ * the time step dtl of src/temporal.f90:12 (set_time_step). */
+int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm);
+/* alternatively, take the coefficients the caller already has (host arrays
+ * in the module layouts of src/local2d.f90:34-36, src/local3d.f90:14) */
+int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,
+                     const double *Dhi, const double *Vh11, const double *Vh12,
+                     const double *Vh22, const double *ABhi, const double *dtl);
+int lns3d_get_coeffs(lns3d_ctx *c, double *Ah, double *Bh, double *Dh, double *Dhi,
+                     double *Vh11, double *Vh12, double *Vh22, double *ABhi, double *dtl);
+/* replaces src/lhs1f3d.f90:2 lhs1f3D + src/lhs2f3d.f90:2 lhs2f3D (with
+ * lhsbt1f3D/lhsbt2f3d/lhsbc1f3d/lhsbc2f3d) and the factor calls
+ * cpenta1bc/cpenta2p(..,0) of src/impdrv3d.f90:252-269,314-319 */
+int lns3d_factor(lns3d_ctx *c);
+/* assembled LHS of direction dir (1 xi, 2 eta) into mat_dev (5,5,5,nx,ny) */
+int lns3d_assemble(lns3d_ctx *c, int dir, void *mat_dev);
+/* replaces src/lrhs3d.f90:2 lrhs3D(vl,rl,vml,xl,yl): r = L(v) */
+int lns3d_lrhs3d(lns3d_ctx *c, const void *v_dev, void *r_dev);
+/* replaces src/itrbc3d.f90:2 itrBC3D(vl,vml) */
+int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev);
+/* xi then eta line solves of r in place (src/impdrv3d.f90:284-327) */
+int lns3d_solve(lns3d_ctx *c, void *r_dev);
+/* one step: lrhs3D, rbe3D, rhsBC3D, xi and eta solves, v += r, itrBC3D;
+ * v must already satisfy the boundary conditions; dv_dev (may be NULL)
+ * receives the increment r */
+int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev);
+/* the same step with complex-float factors (precision = 1) used as a
+ * defect-correction preconditioner of the complex-double AF system until
+ * |res|_max <= tol |r|_max (or max_iters solves) */
+int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int max_iters,
+                       int *iters_out, double *resid_out);
+/* the step with complex-float factors and no inner refinement */
+int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,322 @@
+// C-ABI of the device-resident implicit step: one context per problem
+// (grid, mean flow, coefficients, factors, work arrays), all in HBM.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/lns3d_amd.h"
+#include "common.hpp"
+#include "step.hpp"
+
+using namespace lns3d;
+
+namespace {
+int fail(int code, const char *msg) {
+  lns3d::set_error("capi_step", 0, msg);
+  return code;
+}
+}  // namespace
+
+struct lns3d_ctx {
+  lns3d_params p{};
+  Stencil st{};
+  Coeffs c{};
+  hipStream_t stream = nullptr;
+  PentaPlan px, py;          // xi lines (ny of length nx), eta lines (nx of length ny)
+  PentaPlan px32, py32;      // complex-float factors for the refined step
+  bool have32 = false;
+  void *work = nullptr;      // 5P complex
+  void *r = nullptr;         // 5P complex
+  void *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;   // 5P complex each (refinement)
+  double *scal = nullptr;    // device scalars
+  double *coef_block = nullptr;
+  bool coeffs_ready = false, factored = false;
+  int last_iters = 0;
+  double last_resid = 0.0;
+};
+
+static int check_params(const lns3d_params &p) {
+  if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
+  if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || p.top != 0 || p.left != 0 ||
+      p.right != 0 || p.linear != 2 || (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) ||
+      (p.loctime != 0 && p.loctime != 1))
+    return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
+  if (p.precision != 0 && p.precision != 1) return fail(LNS3D_EINVAL, "precision must be 0 or 1");
+  if ((p.xper || p.yper) && p.precision == 1)
+    return fail(LNS3D_EUNSUPPORTED, "complex-float factors are for non-periodic lines");
+  return 0;
+}
+
+extern "C" int lns3d_ctx_create(const lns3d_params *p, lns3d_ctx **out) {
+  if (!p || !out) return fail(LNS3D_EINVAL, "lns3d_ctx_create: null argument");
+  int rc = check_params(*p);
+  if (rc) return rc;
+  auto *c = new lns3d_ctx;
+  c->p = *p;
+  c->st = make_stencil(*p);
+  const long P = (long)p->nx * p->ny;
+  size_t ncoef = (size_t)P * (25 * 4 + 6 * 3 + 4 + 1 + 5 + 10);
+  auto bail = [&](int code) {
+    lns3d_ctx_destroy(c);
+    return code;
+  };
+  if (hipMalloc(&c->coef_block, ncoef * sizeof(double)) != hipSuccess) return bail(fail(LNS3D_EHIP, "hipMalloc coefficients"));
+  double *b = c->coef_block;
+  c->c.Ah = b; b += 25 * P;
+  c->c.Bh = b; b += 25 * P;
+  c->c.Dh = b; b += 25 * P;
+  c->c.Dhi = b; b += 25 * P;
+  c->c.V11 = b; b += 6 * P;
+  c->c.V12 = b; b += 6 * P;
+  c->c.V22 = b; b += 6 * P;
+  c->c.ABhi = b; b += 4 * P;
+  c->c.dtl = b; b += P;
+  c->c.vm = b; b += 5 * P;
+  c->c.metric = b;
+  if (hipMalloc(&c->work, (size_t)P * 5 * 16) != hipSuccess ||
+      hipMalloc(&c->r, (size_t)P * 5 * 16) != hipSuccess ||
+      hipMalloc(&c->scal, 64 * sizeof(double)) != hipSuccess)
+    return bail(fail(LNS3D_EHIP, "hipMalloc work"));
+  rc = penta_plan_create(&c->px, p->xper, p->periodic_exact, p->nx, p->ny, 0);
+  if (!rc) rc = penta_plan_create(&c->py, p->yper, p->periodic_exact, p->ny, p->nx, 0);
+  if (rc) return bail(rc);
+  *out = c;
+  return 0;
+}
+
+extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
+  if (!c) return 0;
+  penta_plan_destroy(&c->px);
+  penta_plan_destroy(&c->py);
+  penta_plan_destroy(&c->px32);
+  penta_plan_destroy(&c->py32);
+  if (c->coef_block) (void)hipFree(c->coef_block);
+  if (c->work) (void)hipFree(c->work);
+  if (c->r) (void)hipFree(c->r);
+  if (c->t1) (void)hipFree(c->t1);
+  if (c->t2) (void)hipFree(c->t2);
+  if (c->t3) (void)hipFree(c->t3);
+  if (c->scal) (void)hipFree(c->scal);
+  delete c;
+  return 0;
+}
+
+extern "C" int lns3d_ctx_set_stream(lns3d_ctx *c, void *stream) {
+  if (!c) return fail(LNS3D_EINVAL, "null context");
+  c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+// metric (nx,ny,10) and vm (5,nx,ny), host arrays; coefficients and the time
+// step are then generated on the device (src/genmtrx.f90, src/temporal.f90)
+extern "C" int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm) {
+  if (!c || !metric || !vm) return fail(LNS3D_EINVAL, "lns3d_setup: null argument");
+  const long P = (long)c->p.nx * c->p.ny;
+  LNS3D_CHECK(hipMemcpyAsync(c->c.metric, metric, (size_t)P * 10 * 8, hipMemcpyHostToDevice, c->stream));
+  LNS3D_CHECK(hipMemcpyAsync(c->c.vm, vm, (size_t)P * 5 * 8, hipMemcpyHostToDevice, c->stream));
+  int rc = launch_genmtrx(c->p, c->st, c->c, (double *)c->work, c->stream);
+  if (!rc) rc = launch_timestep(c->p, c->c, c->scal, c->stream);
+  if (rc) return rc;
+  c->coeffs_ready = true;
+  c->factored = false;
+  return 0;
+}
+
+// Coefficients in the reference's layouts (host): Ah,Bh,Dh,Dhi (5,5,nx,ny),
+// Vh11,Vh12,Vh22,ABhi (6,nx,ny), dtl (nx,ny) -- the arrays src/genmtrx.f90
+// and src/temporal.f90 leave in modules local2d/local3d.
+extern "C" int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,
+                                const double *Dhi, const double *Vh11, const double *Vh12,
+                                const double *Vh22, const double *ABhi, const double *dtl) {
+  if (!c || !Ah || !Bh || !Dh || !Dhi || !Vh11 || !Vh12 || !Vh22 || !ABhi || !dtl)
+    return fail(LNS3D_EINVAL, "lns3d_set_coeffs: null argument");
+  const long P = (long)c->p.nx * c->p.ny;
+  std::vector<double> soa((size_t)P * 25);
+  auto put = [&](const double *src, int ne, int stride, double *dst) -> int {
+    for (long q = 0; q < P; q++)
+      for (int e = 0; e < ne; e++) soa[(size_t)e * P + q] = src[q * stride + e];
+    if (hipMemcpy(dst, soa.data(), (size_t)ne * P * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(LNS3D_EHIP, "hipMemcpy coefficients");
+    return 0;
+  };
+  int rc = put(Ah, 25, 25, c->c.Ah);
+  if (!rc) rc = put(Bh, 25, 25, c->c.Bh);
+  if (!rc) rc = put(Dh, 25, 25, c->c.Dh);
+  if (!rc) rc = put(Dhi, 25, 25, c->c.Dhi);
+  if (!rc) rc = put(Vh11, 6, 6, c->c.V11);
+  if (!rc) rc = put(Vh12, 6, 6, c->c.V12);
+  if (!rc) rc = put(Vh22, 6, 6, c->c.V22);
+  if (!rc) rc = put(ABhi, 4, 6, c->c.ABhi);
+  if (!rc) rc = put(dtl, 1, 1, c->c.dtl);
+  if (rc) return rc;
+  c->coeffs_ready = true;
+  c->factored = false;
+  return 0;
+}
+
+extern "C" int lns3d_get_coeffs(lns3d_ctx *c, double *Ah, double *Bh, double *Dh, double *Dhi,
+                                double *Vh11, double *Vh12, double *Vh22, double *ABhi, double *dtl) {
+  if (!c || !c->coeffs_ready) return fail(LNS3D_EINVAL, "lns3d_get_coeffs: no coefficients");
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  const long P = (long)c->p.nx * c->p.ny;
+  std::vector<double> soa((size_t)P * 25);
+  auto get = [&](double *dst, int ne, int stride, const double *src) -> int {
+    if (!dst) return 0;
+    if (hipMemcpy(soa.data(), src, (size_t)ne * P * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(LNS3D_EHIP, "hipMemcpy coefficients");
+    for (long q = 0; q < P; q++)
+      for (int e = 0; e < stride; e++) dst[q * stride + e] = e < ne ? soa[(size_t)e * P + q] : 0.0;
+    return 0;
+  };
+  int rc = get(Ah, 25, 25, c->c.Ah);
+  if (!rc) rc = get(Bh, 25, 25, c->c.Bh);
+  if (!rc) rc = get(Dh, 25, 25, c->c.Dh);
+  if (!rc) rc = get(Dhi, 25, 25, c->c.Dhi);
+  if (!rc) rc = get(Vh11, 6, 6, c->c.V11);
+  if (!rc) rc = get(Vh12, 6, 6, c->c.V12);
+  if (!rc) rc = get(Vh22, 6, 6, c->c.V22);
+  if (!rc) rc = get(ABhi, 4, 6, c->c.ABhi);
+  if (!rc) rc = get(dtl, 1, 1, c->c.dtl);
+  return rc;
+}
+
+// assemble (src/lhs1f3d.f90 / lhs2f3d.f90 with closures and BC rows) and
+// factor both directions; precision 1 also builds complex-float factors
+extern "C" int lns3d_factor(lns3d_ctx *c) {
+  if (!c || !c->coeffs_ready) return fail(LNS3D_EINVAL, "lns3d_factor: no coefficients");
+  int rc = launch_assemble(c->p, c->c, 1, c->px, c->stream);
+  if (!rc) rc = penta_factor(&c->px, c->stream);
+  if (!rc) rc = launch_assemble(c->p, c->c, 2, c->py, c->stream);
+  if (!rc) rc = penta_factor(&c->py, c->stream);
+  if (rc) return rc;
+  if (c->p.precision == 1) {
+    if (!c->have32) {
+      rc = penta_plan_create(&c->px32, 0, 0, c->p.nx, c->p.ny, 1);
+      if (!rc) rc = penta_plan_create(&c->py32, 0, 0, c->p.ny, c->p.nx, 1);
+      const long P = (long)c->p.nx * c->p.ny;
+      if (!rc && (hipMalloc(&c->t1, (size_t)P * 80) != hipSuccess || hipMalloc(&c->t2, (size_t)P * 80) != hipSuccess ||
+                  hipMalloc(&c->t3, (size_t)P * 80) != hipSuccess))
+        rc = fail(LNS3D_EHIP, "hipMalloc refinement buffers");
+      // the complex-float plans borrow the complex-double LU working copy
+      if (!rc) {
+        (void)hipFree(c->px32.lu); c->px32.lu = nullptr;
+        (void)hipFree(c->py32.lu); c->py32.lu = nullptr;
+      }
+      if (rc) return rc;
+      c->have32 = true;
+    }
+    // the complex-float records come from the same complex-double LU
+    std::swap(c->px32.lu, c->px.lu);
+    rc = penta_factor_records_only(&c->px32, c->stream);
+    std::swap(c->px32.lu, c->px.lu);
+    if (!rc) {
+      std::swap(c->py32.lu, c->py.lu);
+      rc = penta_factor_records_only(&c->py32, c->stream);
+      std::swap(c->py32.lu, c->py.lu);
+    }
+    if (rc) return rc;
+  }
+  c->factored = true;
+  return 0;
+}
+
+extern "C" int lns3d_assemble(lns3d_ctx *c, int dir, void *mat_dev) {
+  if (!c || !c->coeffs_ready || (dir != 1 && dir != 2) || !mat_dev)
+    return fail(LNS3D_EINVAL, "lns3d_assemble: bad arguments");
+  return launch_assemble_ref(c->p, c->c, dir, mat_dev, c->stream);
+}
+
+extern "C" int lns3d_lrhs3d(lns3d_ctx *c, const void *v_dev, void *r_dev) {
+  if (!c || !c->coeffs_ready || !v_dev || !r_dev) return fail(LNS3D_EINVAL, "lns3d_lrhs3d: bad arguments");
+  return launch_rhs(c->p, c->st, c->c, v_dev, r_dev, c->work, 0, c->stream);
+}
+
+extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
+  if (!c || !v_dev) return fail(LNS3D_EINVAL, "lns3d_itrbc3d: bad arguments");
+  return launch_itrbc(c->p, v_dev, c->stream);
+}
+
+static int solve_xy(lns3d_ctx *c, PentaPlan &px, PentaPlan &py, void *r) {
+  // xi lines: line j, row i at (i + nx*j); eta lines: line i, row j
+  int rc = penta_solve(&px, r, c->p.nx, 1, c->stream);
+  if (!rc) rc = penta_solve(&py, r, 1, c->p.nx, c->stream);
+  return rc;
+}
+
+extern "C" int lns3d_solve(lns3d_ctx *c, void *r_dev) {
+  if (!c || !c->factored || !r_dev) return fail(LNS3D_ENOTFACTORED, "lns3d_solve: not factored");
+  return solve_xy(c, c->px, c->py, r_dev);
+}
+
+// One implicit step (src/impdrv3d.f90:146-356, impl = 1, niter = 1):
+// r = -dtl * lrhs3D(v) with rhsBC3D rows, xi solve, eta solve, v += r,
+// itrBC3D.  v must already satisfy the boundary conditions (call
+// lns3d_itrbc3d once before the first step, as src/impdrv3d.f90:126 does).
+// dv_dev (optional) receives the increment r.
+extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
+  if (!c || !c->factored || !v_dev) return fail(LNS3D_ENOTFACTORED, "lns3d_step: not factored");
+  void *r = dv_dev ? dv_dev : c->r;
+  int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
+  if (!rc) rc = solve_xy(c, c->px, c->py, r);
+  if (!rc) rc = launch_update_bc(c->p, v_dev, r, c->stream);
+  return rc;
+}
+
+// The same step with the line solves in complex float used as a
+// defect-correction preconditioner for the complex-double AF system
+// M dv = r, M = Lx Ly (M applied with blocks regenerated in double):
+//   dv = 0; repeat { res = r - M dv; dv += M32^{-1} res } until
+//   |res|_max <= tol |r|_max or iters reach max_iters.
+extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int max_iters,
+                                  int *iters_out, double *resid_out) {
+  if (!c || !c->factored || !c->have32 || !v_dev)
+    return fail(LNS3D_ENOTFACTORED, "lns3d_step_refined: needs precision = 1 and lns3d_factor");
+  const long n = 5L * c->p.nx * c->p.ny;
+  void *dv = dv_dev ? dv_dev : c->r;
+  void *rhs = c->t3;            // the residual r of the step
+  int rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
+  if (rc) return rc;
+  LNS3D_CHECK(hipMemcpyAsync(dv, rhs, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  rc = solve_xy(c, c->px32, c->py32, dv);
+  if (rc) return rc;
+  int it = 1;
+  double nr[2] = {0, 0};
+  for (; it < max_iters || max_iters <= 0; it++) {
+    rc = launch_af_matvec(c->p, c->c, dv, c->t2, c->t1, c->stream);       // t2 = M dv
+    if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);   // t2 = rhs - M dv
+    if (rc) return rc;
+    LNS3D_CHECK(hipMemcpyAsync(nr, c->scal + 8, 16, hipMemcpyDeviceToHost, c->stream));
+    LNS3D_CHECK(hipStreamSynchronize(c->stream));
+    if (nr[0] <= tol * nr[1] || it >= max_iters) break;
+    rc = solve_xy(c, c->px32, c->py32, c->t2);
+    if (!rc) rc = launch_axpy(dv, c->t2, n, c->stream);
+    if (rc) return rc;
+  }
+  c->last_iters = it;
+  c->last_resid = nr[1] > 0 ? nr[0] / nr[1] : 0.0;
+  if (iters_out) *iters_out = it;
+  if (resid_out) *resid_out = c->last_resid;
+  if (!(c->last_resid == c->last_resid)) return fail(LNS3D_ENONFINITE, "refinement produced NaN");
+  return launch_update_bc(c->p, v_dev, dv, c->stream);
+}
+
+// the same step, but only the complex-float solve (no refinement): the
+// pseudo-time iteration itself corrects the defect in steady marching
+extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
+  if (!c || !c->factored || !c->have32 || !v_dev)
+    return fail(LNS3D_ENOTFACTORED, "lns3d_step_fp32: needs precision = 1 and lns3d_factor");
+  void *r = dv_dev ? dv_dev : c->r;
+  int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
+  if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
+  if (!rc) rc = launch_update_bc(c->p, v_dev, r, c->stream);
+  return rc;
+}
+
+extern "C" int lns3d_ctx_synchronize(lns3d_ctx *c) {
+  if (!c) return fail(LNS3D_EINVAL, "null context");
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}

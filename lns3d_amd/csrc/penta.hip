@@ -560,6 +560,21 @@ int penta_factor(PentaPlan *pl, hipStream_t s) {
   return 0;
 }
 
+int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
+  if (pl->periodic) return set_error(__FILE__, __LINE__, "records are for non-periodic lines");
+  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
+  if (pl->prec)
+    hipLaunchKernelGGL((k_records_bc<double, float>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
+                       (const cx<double> *)pl->lu, (cx<float> *)pl->fwd, (cx<float> *)pl->bwd,
+                       (cx<float> *)pl->tail);
+  else
+    hipLaunchKernelGGL((k_records_bc<double, double>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
+                       (const cx<double> *)pl->lu, (cx<double> *)pl->fwd, (cx<double> *)pl->bwd,
+                       (cx<double> *)pl->tail);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
   if (pl->periodic) {

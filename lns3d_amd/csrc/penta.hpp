@@ -25,6 +25,8 @@ void penta_plan_destroy(PentaPlan *pl);
 int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStream_t s);
 // assembled working copy -> factors (+ records for non-periodic lines)
 int penta_factor(PentaPlan *pl, hipStream_t s);
+// records only, from an LU already in pl->lu (non-periodic lines)
+int penta_factor_records_only(PentaPlan *pl, hipStream_t s);
 // in-place solve of rhs (complex double); component d of (line l, row i) at
 // complex offset (l*sL + i*sR)*5 + d
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);

@@ -1,0 +1,882 @@
+// Kernels of one implicit approximate-factorization step on gfx950, other
+// than the line solves (penta.hip):
+//   coefficient generation   src/genmtrx.f90 (with grad/grad2/gradbc/getmat)
+//   local time step          src/temporal.f90:40-160
+//   LHS assembly             src/lhs1f3d.f90, lhs2f3d.f90 (interior),
+//                            lhsbt1f3d.f90, lhsbt2f3d.f90 (rows 1,2,n-1,n),
+//                            lhsbc1f3d.f90, lhsbc2f3d.f90 (boundary rows)
+//   RHS                      src/lrhs3d.f90 with cgrad/cgrad2/cgradbc, then
+//                            rbe3D (impdrv3d.f90:444-470) and rhsBC3D
+//   update + BCs             impdrv3d.f90:340-356, src/itrbc3d.f90
+// One thread per grid point.  Coefficients are structure-of-arrays ([e][P])
+// so a wave's loads of one coefficient are contiguous.
+#include "common.hpp"
+#include "step.hpp"
+
+namespace lns3d {
+
+using z = cx<double>;
+
+// --------------------------------------------------------------------------
+// scalar/complex helpers for the stencil templates
+__device__ __forceinline__ double sc(double a, double x) { return a * x; }
+__device__ __forceinline__ z sc(double a, z x) { return {a * x.re, a * x.im}; }
+__device__ __forceinline__ double ld(const double *p) { return *p; }
+__device__ __forceinline__ z ld(const double2 *p) { double2 v = *p; return {v.x, v.y}; }
+
+// first derivative along a line at position p (N points), value stride es
+// between consecutive points; component e selected by base offset.
+// Formulas of src/cgrad.f90:84-283 (periodic and non-periodic branches).
+template <typename S, typename V>
+__device__ __forceinline__ V d1(int N, int per, int p, const double *g, double dinv,
+                                const S *base, long es) {
+  using namespace st;
+  auto x = [&](int q) { return ld(base + (long)q * es); };
+  if (per) {
+    int m = N - 1;
+    if (p == N - 1) p = 0;
+    auto w = [&](int q) { return x(((q % m) + m) % m); };
+    return sc(g[0], w(p - 3)) + sc(g[1], w(p - 2)) + sc(g[2], w(p - 1)) + sc(g[3], w(p + 1)) +
+           sc(g[4], w(p + 2)) + sc(g[5], w(p + 3));
+  }
+  if (p == 0) return sc(dinv, sc(gc1, x(0)) + sc(gc2, x(1)) + sc(gc3, x(2)) + sc(gc4, x(3)) + sc(gc5, x(4)));
+  if (p == 1) return sc(dinv, sc(gb1, x(0)) + sc(gb2, x(1)) + sc(gb3, x(2)) + sc(gb4, x(3)) + sc(gb5, x(4)));
+  if (p == 2) return sc(dinv, sc(ga1, x(0)) + sc(ga2, x(1)) + sc(ga3, x(3)) + sc(ga4, x(4)));
+  if (p == N - 3)
+    return sc(dinv, sc(ga1, x(N - 5)) + sc(ga2, x(N - 4)) + sc(ga3, x(N - 2)) + sc(ga4, x(N - 1)));
+  if (p == N - 2)
+    return sc(-dinv, sc(gb1, x(N - 1)) + sc(gb2, x(N - 2)) + sc(gb3, x(N - 3)) + sc(gb4, x(N - 4)) +
+                         sc(gb5, x(N - 5)));
+  if (p == N - 1)
+    return sc(-dinv, sc(gc1, x(N - 1)) + sc(gc2, x(N - 2)) + sc(gc3, x(N - 3)) + sc(gc4, x(N - 4)) +
+                         sc(gc5, x(N - 5)));
+  return sc(g[0], x(p - 3)) + sc(g[1], x(p - 2)) + sc(g[2], x(p - 1)) + sc(g[3], x(p + 1)) +
+         sc(g[4], x(p + 2)) + sc(g[5], x(p + 3));
+}
+
+// second derivative, src/cgrad2.f90:134-326
+template <typename S, typename V>
+__device__ __forceinline__ V d2(int N, int per, int p, const double *d, double dsinv,
+                                const S *base, long es) {
+  using namespace st;
+  auto x = [&](int q) { return ld(base + (long)q * es); };
+  if (per) {
+    int m = N - 1;
+    if (p == N - 1) p = 0;
+    auto w = [&](int q) { return x(((q % m) + m) % m); };
+    return sc(d[0], w(p - 3)) + sc(d[1], w(p - 2)) + sc(d[2], w(p - 1)) + sc(d[3], w(p)) +
+           sc(d[4], w(p + 1)) + sc(d[5], w(p + 2)) + sc(d[6], w(p + 3));
+  }
+  if (p == 0) return sc(dsinv, sc(dd1, x(0)) + sc(dd2, x(1)) + sc(dd3, x(2)) + sc(dd4, x(3)) + sc(dd5, x(4)));
+  if (p == 1) return sc(dsinv, sc(db1, x(0)) + sc(db2, x(1)) + sc(db3, x(2)) + sc(db4, x(3)) + sc(db5, x(4)));
+  if (p == 2) return sc(dsinv, sc(da1, x(0)) + sc(da2, x(1)) + sc(da3, x(2)) + sc(da4, x(3)) + sc(da5, x(4)));
+  if (p == N - 3)
+    return sc(dsinv, sc(da1, x(N - 5)) + sc(da2, x(N - 4)) + sc(da3, x(N - 3)) + sc(da4, x(N - 2)) +
+                         sc(da5, x(N - 1)));
+  if (p == N - 2)
+    return sc(dsinv, sc(db1, x(N - 1)) + sc(db2, x(N - 2)) + sc(db3, x(N - 3)) + sc(db4, x(N - 4)) +
+                         sc(db5, x(N - 5)));
+  if (p == N - 1)
+    return sc(dsinv, sc(dd1, x(N - 1)) + sc(dd2, x(N - 2)) + sc(dd3, x(N - 3)) + sc(dd4, x(N - 4)) +
+                         sc(dd5, x(N - 5)));
+  return sc(d[0], x(p - 3)) + sc(d[1], x(p - 2)) + sc(d[2], x(p - 1)) + sc(d[3], x(p)) +
+         sc(d[4], x(p + 1)) + sc(d[5], x(p + 2)) + sc(d[6], x(p + 3));
+}
+
+Stencil make_stencil(const lns3d_params &p) {
+  Stencil s;
+  s.dxi = 1.0 / (double)(p.nx - 1);
+  s.deta = 1.0 / (double)(p.ny - 1);
+  s.dxinv = 1.0 / s.dxi;
+  s.dyinv = 1.0 / s.deta;
+  s.dxsinv = 1.0 / (s.dxi * s.dxi);
+  s.dysinv = 1.0 / (s.deta * s.deta);
+  auto first = [](int opt, double dinv, double *g) {
+    double c;
+    if (opt == 0) c = 1.0 / 60.0;
+    else if (opt == -1) c = 0.0;
+    else {
+      double w = 2.0 * 3.1415926535897932385e+0 / 12.0;
+      c = (w / 2.0 - 2.0 * sin(w) / 3.0 + sin(2.0 * w) / 12.0) /
+          (5.0 * sin(w) - 4.0 * sin(2.0 * w) + sin(3.0 * w));
+    }
+    double a = (2.0 + 15.0 * c) / 3.0, b = -(1.0 + 48.0 * c) / 12.0;
+    g[0] = -c * dinv; g[1] = -b * dinv; g[2] = -a * dinv;
+    g[3] = a * dinv; g[4] = b * dinv; g[5] = c * dinv;
+  };
+  auto second = [](int opt, double dsinv, double *d) {
+    double c;
+    if (opt == 0) c = 1.0 / 90.0;
+    else if (opt == -1) c = 0.0;
+    else {
+      double w = 2.0 * 3.1415926535897932385e+0 / 12.0;
+      double s1 = sin(w / 2.0), s2 = sin(w), s3 = sin(3.0 * w / 2.0);
+      c = -(3.0 * w * w - 16.0 * s1 * s1 + s2 * s2) /
+          (12.0 * (-15.0 * s1 * s1 + 6.0 * s2 * s2 - s3 * s3));
+    }
+    double a = (4.0 + 45.0 * c) / 3.0, b = -(1.0 + 72.0 * c) / 12.0;
+    d[0] = c * dsinv; d[1] = b * dsinv; d[2] = a * dsinv;
+    d[3] = -2.0 * (a + b + c) * dsinv;
+    d[4] = a * dsinv; d[5] = b * dsinv; d[6] = c * dsinv;
+  };
+  first(p.optx, s.dxinv, s.gx);
+  first(p.opty, s.dyinv, s.gy);
+  second(p.optx, s.dxsinv, s.dx);
+  second(p.opty, s.dysinv, s.dy);
+  return s;
+}
+
+static inline int nblocks(long n, int b) { return (int)((n + b - 1) / b); }
+
+// ==========================================================================
+// coefficient generation (src/genmtrx.f90:72-1000, general metrics,
+// linear != 1, idamp = 0)
+// first derivative in x of the mean flow into g1 (5,nx,ny)
+__global__ void k_mean_d1x(int nx, int ny, int xper, Stencil st, const double *vm, double *g1) {
+#pragma clang fp contract(off)
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  int i = q % nx, j = q / nx;
+  const double *base = vm + 5L * nx * j;
+  for (int e = 0; e < 5; e++)
+    g1[5 * q + e] = d1<double, double>(nx, xper, i, st.gx, st.dxinv, base + e, 5);
+}
+
+__device__ __forceinline__ void getmat(int mattyp, double t, double &mu, double &lm, double &con,
+                                       double &dmu, double &d2mu, double &dlm, double &d2lm,
+                                       double &dcon, double &d2con) {
+  if (mattyp == 0) { mu = 1.0; dmu = 0.0; d2mu = 0.0; }
+  else { mu = t; dmu = 1.0; d2mu = 0.0; }
+  con = mu; dcon = dmu; d2con = d2mu;
+  if (mattyp == 3) { lm = 0.0; dlm = 0.0; d2lm = 0.0; }
+  else { lm = -st::pt66 * mu; dlm = -st::pt66 * dmu; d2lm = -st::pt66 * d2mu; }
+}
+
+__global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const double *met,
+                          const double *g1a, Coeffs c) {
+  // set-up kernel: rounded operation by operation (no fused multiply-add) so
+  // the coefficients follow the reference's arithmetic; several of them are
+  // differences of nearly equal stencil sums
+#pragma clang fp contract(off)
+  const int nx = p.nx, ny = p.ny;
+  const long P = (long)nx * ny;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  int i = q % nx, j = q / nx;
+  double g1[5], g2[5], g11[5], g12[5], g22[5];
+  const double *colv = vm + 5L * i, *colg = g1a + 5L * i, *rowv = vm + 5L * nx * j;
+  for (int e = 0; e < 5; e++) {
+    g1[e] = g1a[5 * q + e];
+    g2[e] = d1<double, double>(ny, p.yper, j, st.gy, st.dyinv, colv + e, 5L * nx);
+    g11[e] = d2<double, double>(nx, p.xper, i, st.dx, st.dxsinv, rowv + e, 5);
+    g22[e] = d2<double, double>(ny, p.yper, j, st.dy, st.dysinv, colv + e, 5L * nx);
+    g12[e] = d1<double, double>(ny, p.yper, j, st.gy, st.dyinv, colg + e, 5L * nx);
+  }
+  if ((p.wallt == 1 || p.wallt == 2) && j == 0) g2[4] = 0.0;   // src/gradbc.f90:15
+  const double M1 = met[q], M2 = met[P + q], N1 = met[2 * P + q], N2 = met[3 * P + q];
+  const double m11 = met[4 * P + q], m12 = met[5 * P + q], m22 = met[6 * P + q];
+  const double n11 = met[7 * P + q], n12 = met[8 * P + q], n22 = met[9 * P + q];
+  const double m1m1 = M1 * M1, m1m2 = M1 * M2, m2m2 = M2 * M2;
+  const double n1n1 = N1 * N1, n1n2 = N1 * N2, n2n2 = N2 * N2;
+  const double m1n1 = M1 * N1, m1n2 = M1 * N2, m2n1 = M2 * N1, m2n2 = M2 * N2;
+  double g11l[5], g12l[5], g22l[5], g1l[5], g2l[5];
+  for (int d = 0; d < 5; d++) {
+    g11l[d] = g11[d] * m1m1 + 2.0 * g12[d] * m1n1 + g22[d] * n1n1 + g1[d] * m11 + g2[d] * n11;
+    g12l[d] = g11[d] * m1m2 + g12[d] * (m1n2 + m2n1) + g22[d] * n1n2 + g1[d] * m12 + g2[d] * n12;
+    g22l[d] = g11[d] * m2m2 + 2.0 * g12[d] * m2n2 + g22[d] * n2n2 + g1[d] * m22 + g2[d] * n22;
+    g1l[d] = g1[d] * M1 + g2[d] * N1;
+    g2l[d] = g1[d] * M2 + g2[d] * N2;
+  }
+  const double gamma = p.gamma, gamma1 = gamma - 1.0, Ma = p.Ma, Re = p.Re, Pr = p.Pr, kz = p.kz;
+  const double fact = 1.0 / (gamma * Ma * Ma);
+  const double rho = vm[5 * q], u1 = vm[5 * q + 1], u2 = vm[5 * q + 2], u3 = vm[5 * q + 3],
+               t = vm[5 * q + 4];
+  const double rhoinv = 1.0 / rho;
+  const double grho[3] = {g1l[0], g2l[0], 0.0};
+  const double gu[3][3] = {{g1l[1], g2l[1], 0.0}, {g1l[2], g2l[2], 0.0}, {g1l[3], g2l[3], 0.0}};
+  const double gt[3] = {g1l[4], g2l[4], 0.0};
+  const double gdivu[3] = {g11l[1] + g12l[2], g12l[1] + g22l[2], 0.0};
+  const double Sjj[3] = {0.5 * (g11l[1] + g11l[1] + g22l[1] + g12l[2]),
+                         0.5 * (g11l[2] + g12l[1] + g22l[2] + g22l[2]), 0.5 * (g11l[3] + g22l[3])};
+  const double Lapt = g11l[4] + g22l[4];
+  const double divu = gu[0][0] + gu[1][1] + gu[2][2];
+  double gp[3];
+  for (int k = 0; k < 3; k++) gp[k] = fact * (grho[k] * t + rho * gt[k]);
+  double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  S[0][0] = gu[0][0];
+  S[1][0] = 0.5 * (gu[1][0] + gu[0][1]);
+  S[2][0] = 0.5 * gu[2][0];
+  S[0][1] = 0.5 * (gu[0][1] + gu[1][0]);
+  S[1][1] = gu[1][1];
+  S[2][1] = 0.5 * gu[2][1];
+  S[0][2] = 0.5 * gu[2][0];
+  S[1][2] = 0.5 * gu[2][1];
+  double mu, lm, con, dmu, d2mu, dlm, d2lm, dcon, d2con;
+  getmat(p.mattyp, t, mu, lm, con, dmu, d2mu, dlm, d2lm, dcon, d2con);
+  double gmu[3], gdmu[3], gcon[3], gdcon[3], glm[3], gdlm[3];
+  for (int k = 0; k < 3; k++) {
+    gmu[k] = dmu * gt[k]; gdmu[k] = d2mu * gt[k];
+    gcon[k] = dcon * gt[k]; gdcon[k] = d2con * gt[k];
+    glm[k] = dlm * gt[k]; gdlm[k] = d2lm * gt[k];
+  }
+  double A[5][5], B[5][5], D[5][5], Vij[4], f1, f2;
+  const double gmsinv = 1.0 / (gamma * Ma * Ma);
+  // A and B (src/genmtrx.f90:430-570)
+  f1 = rhoinv / Re;
+  A[0][0] = u1; A[0][1] = rho; A[0][2] = 0; A[0][3] = 0; A[0][4] = 0;
+  A[1][0] = rhoinv * t * gmsinv;
+  A[1][1] = u1 - f1 * glm[0] - f1 * 2.0 * gmu[0];
+  A[1][2] = -f1 * gmu[1];
+  A[1][3] = -f1 * gmu[2];
+  A[1][4] = gmsinv - f1 * dlm * divu - f1 * dmu * 2.0 * S[0][0];
+  A[2][0] = 0; A[2][1] = -f1 * glm[1]; A[2][2] = u1 - f1 * gmu[0]; A[2][3] = 0;
+  A[2][4] = -f1 * dmu * 2.0 * S[1][0];
+  A[3][0] = 0; A[3][1] = -f1 * glm[2]; A[3][2] = 0; A[3][3] = u1 - f1 * gmu[0];
+  A[3][4] = -f1 * dmu * 2.0 * S[2][0];
+  B[0][0] = u2; B[0][1] = 0; B[0][2] = rho; B[0][3] = 0; B[0][4] = 0;
+  B[1][0] = 0; B[1][1] = u2 - f1 * gmu[1]; B[1][2] = -f1 * glm[0]; B[1][3] = 0;
+  B[1][4] = -f1 * dmu * 2.0 * S[0][1];
+  B[2][0] = rhoinv * t * gmsinv; B[2][1] = -f1 * gmu[0];
+  B[2][2] = u2 - f1 * glm[1] - f1 * 2.0 * gmu[1];
+  B[2][3] = -f1 * gmu[2];
+  B[2][4] = gmsinv - f1 * dlm * divu - f1 * dmu * 2.0 * S[1][1];
+  B[3][0] = 0; B[3][1] = 0; B[3][2] = -f1 * glm[2]; B[3][3] = u2 - f1 * gmu[1];
+  B[3][4] = -f1 * dmu * 2.0 * S[2][1];
+  f1 = gamma * rhoinv / (Pr * Re);
+  f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+  A[4][0] = 0;
+  A[4][1] = gamma1 * t - f2 * 2.0 * lm * divu - 4.0 * f2 * mu * S[0][0];
+  A[4][2] = -4.0 * f2 * mu * S[1][0];
+  A[4][3] = -4.0 * f2 * mu * S[2][0];
+  A[4][4] = u1 - f1 * (gcon[0] + dcon * gt[0]);
+  B[4][0] = 0;
+  B[4][1] = -4.0 * f2 * mu * S[0][1];
+  B[4][2] = gamma1 * t - f2 * 2.0 * lm * divu - 4.0 * f2 * mu * S[1][1];
+  B[4][3] = -4.0 * f2 * mu * S[2][1];
+  B[4][4] = u2 - f1 * (gcon[1] + dcon * gt[1]);
+  f1 = 1.0 / Re; f2 = gamma / (Pr * Re);
+  Vij[0] = f1 * rhoinv * (lm + 2.0 * mu);
+  Vij[1] = f1 * rhoinv * mu;
+  Vij[2] = f2 * rhoinv * con;
+  Vij[3] = f1 * rhoinv * (lm + mu);
+  // Ah, Bh (src/genmtrx.f90:590-690)
+  for (int w = 0; w < 2; w++) {
+    double e1 = w ? N1 : M1, e2 = w ? N2 : M2, e11 = w ? n11 : m11, e12 = w ? n12 : m12,
+           e22 = w ? n22 : m22;
+    double H[5][5];
+    H[0][0] = A[0][0] * e1 + B[0][0] * e2;
+    H[0][1] = A[0][1] * e1;
+    H[0][2] = B[0][2] * e2;
+    H[0][3] = 0; H[0][4] = 0;
+    H[1][0] = A[1][0] * e1;
+    H[1][1] = A[1][1] * e1 + B[1][1] * e2 - Vij[0] * e11 - Vij[1] * e22;
+    H[1][2] = A[1][2] * e1 + B[1][2] * e2 - Vij[3] * e12;
+    H[1][3] = A[1][3] * e1;
+    H[1][4] = A[1][4] * e1 + B[1][4] * e2;
+    H[2][0] = B[2][0] * e2;
+    H[2][1] = A[2][1] * e1 + B[2][1] * e2 - Vij[3] * e12;
+    H[2][2] = A[2][2] * e1 + B[2][2] * e2 - Vij[1] * e11 - Vij[0] * e22;
+    H[2][3] = B[2][3] * e2;
+    H[2][4] = A[2][4] * e1 + B[2][4] * e2;
+    H[3][0] = 0;
+    H[3][1] = A[3][1] * e1;
+    H[3][2] = B[3][2] * e2;
+    H[3][3] = A[3][3] * e1 + B[3][3] * e2 - Vij[1] * e11 - Vij[1] * e22;
+    H[3][4] = A[3][4] * e1 + B[3][4] * e2;
+    for (int cc = 0; cc < 4; cc++) H[4][cc] = A[4][cc] * e1 + B[4][cc] * e2;
+    H[4][4] = A[4][4] * e1 + B[4][4] * e2 - Vij[2] * e11 - Vij[2] * e22;
+    if (w == 1 && !p.yper && p.wallt == 2 && j == 0)
+      for (int r = 0; r < 5; r++) H[r][4] = 0.0;
+    double *dst = w ? c.Bh : c.Ah;
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++) dst[(long)(r + 5 * cc) * P + q] = H[r][cc];
+  }
+  // D (src/genmtrx.f90:700-820)
+  f1 = rhoinv / Re;
+  D[0][0] = divu; D[0][1] = grho[0]; D[0][2] = grho[1]; D[0][3] = grho[2]; D[0][4] = 0;
+  for (int k = 0; k < 3; k++) {
+    D[1 + k][0] = rhoinv * (gmsinv * gt[k] - rhoinv * gp[k] + f1 * (glm[k] * divu + lm * gdivu[k]) +
+                            f1 * 2.0 * (gmu[0] * S[k][0] + gmu[1] * S[k][1] + gmu[2] * S[k][2] +
+                                        mu * Sjj[k]));
+    D[1 + k][1] = gu[k][0]; D[1 + k][2] = gu[k][1]; D[1 + k][3] = gu[k][2];
+    D[1 + k][4] = rhoinv * gmsinv * grho[k] - f1 * (gdlm[k] * divu + dlm * gdivu[k]) -
+                  f1 * 2.0 * (gdmu[0] * S[k][0] + gdmu[1] * S[k][1] + gdmu[2] * S[k][2] + dmu * Sjj[k]);
+  }
+  f1 = gamma * rhoinv / (Pr * Re);
+  f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+  const double SS = S[0][0] * S[0][0] + S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][0] * S[1][0] +
+                    S[1][1] * S[1][1] + S[1][2] * S[1][2] + S[2][0] * S[2][0] + S[2][1] * S[2][1] +
+                    S[2][2] * S[2][2];
+  D[4][0] = f1 * rhoinv * (gcon[0] * gt[0] + gcon[1] * gt[1] + gcon[2] * gt[2] + con * Lapt) +
+            f2 * rhoinv * lm * (divu * divu) + 2.0 * f2 * rhoinv * mu * SS;
+  D[4][1] = gt[0]; D[4][2] = gt[1]; D[4][3] = gt[2];
+  D[4][4] = gamma1 * divu;
+  D[4][4] = D[4][4] - f1 * (gdcon[0] * gt[0] + gdcon[1] * gt[1] + gdcon[2] * gt[2] + dcon * Lapt) -
+            f2 * dlm * (divu * divu) - 2.0 * f2 * dmu * SS;
+  if (kz != 0.0) {
+    D[1][1] += kz * kz * Vij[1];
+    D[2][2] += kz * kz * Vij[1];
+    D[3][3] += kz * kz * Vij[0];
+    D[4][4] += kz * kz * Vij[2];
+  }
+  for (int r = 0; r < 5; r++)
+    for (int cc = 0; cc < 5; cc++) c.Dh[(long)(r + 5 * cc) * P + q] = D[r][cc];
+  // compact viscous matrices (src/genmtrx.f90:850-913)
+  double V11[6] = {Vij[0] * m1m1 + Vij[1] * m2m2, Vij[1] * m1m1 + Vij[0] * m2m2,
+                   Vij[1] * m1m1 + Vij[1] * m2m2, Vij[2] * m1m1 + Vij[2] * m2m2,
+                   Vij[3] * m1m2, Vij[3] * m1m2};
+  double V12[6] = {2.0 * (Vij[0] * m1n1 + Vij[1] * m2n2), 2.0 * (Vij[1] * m1n1 + Vij[0] * m2n2),
+                   2.0 * (Vij[1] * m1n1 + Vij[1] * m2n2), 2.0 * (Vij[2] * m1n1 + Vij[2] * m2n2),
+                   Vij[3] * (m1n2 + m2n1), Vij[3] * (m1n2 + m2n1)};
+  double V22[6] = {Vij[0] * n1n1 + Vij[1] * n2n2, Vij[1] * n1n1 + Vij[0] * n2n2,
+                   Vij[1] * n1n1 + Vij[1] * n2n2, Vij[2] * n1n1 + Vij[2] * n2n2,
+                   Vij[3] * n1n2, Vij[3] * n1n2};
+  for (int e = 0; e < 6; e++) {
+    c.V11[e * P + q] = V11[e];
+    c.V12[e * P + q] = V12[e];
+    c.V22[e * P + q] = V22[e];
+  }
+  // 3-d terms (src/genmtrx.f90:918-990)
+  c.ABhi[q] = kz * Vij[3] * M1;
+  c.ABhi[P + q] = kz * Vij[3] * M2;
+  c.ABhi[2 * P + q] = kz * Vij[3] * N1;
+  c.ABhi[3 * P + q] = kz * Vij[3] * N2;
+  double C[5][5];
+  f1 = rhoinv / Re;
+  C[0][0] = u3; C[0][1] = 0; C[0][2] = 0; C[0][3] = rho; C[0][4] = 0;
+  C[1][0] = 0; C[1][1] = u3 - f1 * gmu[2]; C[1][2] = 0; C[1][3] = -f1 * glm[0];
+  C[1][4] = -f1 * dmu * 2.0 * S[0][2];
+  C[2][0] = 0; C[2][1] = 0; C[2][2] = u3 - f1 * gmu[2]; C[2][3] = -f1 * glm[1];
+  C[2][4] = -f1 * dmu * 2.0 * S[1][2];
+  C[3][0] = rhoinv * t * gmsinv; C[3][1] = -f1 * gmu[0]; C[3][2] = -f1 * gmu[1];
+  C[3][3] = u3 - f1 * glm[2] - f1 * 2.0 * gmu[2];
+  C[3][4] = gmsinv - f1 * dlm * divu - f1 * dmu * 2.0 * S[2][2];
+  f1 = gamma * rhoinv / (Pr * Re);
+  f2 = gamma * gamma1 * Ma * Ma * rhoinv / Re;
+  C[4][0] = 0;
+  C[4][1] = -4.0 * f2 * mu * S[0][2];
+  C[4][2] = -4.0 * f2 * mu * S[1][2];
+  C[4][3] = gamma1 * t - f2 * 2.0 * lm * divu - 4.0 * f2 * mu * S[2][2];
+  C[4][4] = u3 - f1 * (gcon[2] + dcon * gt[2]);
+  for (int r = 0; r < 5; r++)
+    for (int cc = 0; cc < 5; cc++)
+      c.Dhi[(long)(r + 5 * cc) * P + q] = kz * C[r][cc] - (r == cc ? p.omega : 0.0);
+}
+
+int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s) {
+  long P = (long)p.nx * p.ny;
+  hipLaunchKernelGGL(k_mean_d1x, dim3(nblocks(P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
+                     (const double *)c.vm, work);
+  LNS3D_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_genmtrx, dim3(nblocks(P, 128)), dim3(128), 0, s, p, st, (const double *)c.vm,
+                     (const double *)c.metric, (const double *)work, c);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
+// time step (src/temporal.f90:40-160): acoustic CFL per point, global max
+__global__ void k_dt_raw(lns3d_params p, const double *vm, const double *met, double *dtl,
+                         unsigned long long *maxbits) {
+#pragma clang fp contract(off)
+  const long P = (long)p.nx * p.ny;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (q < P) {
+    double dxi = 1.0 / (double)(p.nx - 1), deta = 1.0 / (double)(p.ny - 1);
+    double m1 = met[q], m2 = met[P + q], n1 = met[2 * P + q], n2 = met[3 * P + q];
+    double vinv = 1.0 / (m1 * n2 - m2 * n1);
+    double sxj = n2 * dxi * vinv, syj = -m2 * dxi * vinv;
+    double sxi = -n1 * deta * vinv, syi = m1 * deta * vinv;
+    double volinv = 1.0 / (dxi * deta * vinv);
+    double acoustic = sqrt(sxi * sxi + 2.0 * sxi * syi + syi * syi + sxj * sxj + 2.0 * sxj * syj +
+                           syj * syj) / p.Ma;
+    double u1 = vm[5 * q + 1], u2 = vm[5 * q + 2], ct = sqrt(vm[5 * q + 4]);
+    v = (fabs(u1 * sxi + u2 * syi) + fabs(u1 * sxj + u2 * syj) + ct * acoustic) * volinv;
+    dtl[q] = v;
+  }
+  // block max of non-negative values, then one atomic per block on the bits
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_dt_final(lns3d_params p, double *dtl, const unsigned long long *maxbits) {
+  const long P = (long)p.nx * p.ny;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  double cfl = __longlong_as_double((long long)*maxbits);
+  if (cfl * p.dtmax <= p.cflmax) dtl[q] = p.dtmax;
+  else if (p.loctime == 0) dtl[q] = p.cflmax / cfl;
+  else dtl[q] = p.cflmax / dtl[q];
+}
+
+__global__ void k_zero_u64(unsigned long long *x) { *x = 0ull; }
+
+int launch_timestep(const lns3d_params &p, Coeffs &c, double *work, hipStream_t s) {
+  long P = (long)p.nx * p.ny;
+  auto *mx = (unsigned long long *)work;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, mx);
+  hipLaunchKernelGGL(k_dt_raw, dim3(nblocks(P, 256)), dim3(256), 0, s, p, (const double *)c.vm,
+                     (const double *)c.metric, c.dtl, mx);
+  hipLaunchKernelGGL(k_dt_final, dim3(nblocks(P, 256)), dim3(256), 0, s, p, c.dtl,
+                     (const unsigned long long *)mx);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
+// LHS assembly: value of entry (slot k, row r, col cc) at point (i, j)
+struct AsmCtx {
+  lns3d_params p;
+  double d, dinv, dsinv;        // spacing along the line
+  double a[5], b[5];            // interior first/second derivative weights x alfa
+};
+
+__device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir, int i, int j, int k,
+                                       int r, int cc) {
+  using namespace st;
+  const lns3d_params &p = A.p;
+  const long P = (long)p.nx * p.ny;
+  const long q = i + (long)p.nx * j;
+  const int N = dir == 1 ? p.nx : p.ny;
+  const int pos = dir == 1 ? i : j;
+  const bool per = dir == 1 ? p.xper : p.yper;
+  const double alfa = p.alfa;
+  // boundary-condition rows (src/lhsbc1f3d.f90, src/lhsbc2f3d.f90); the
+  // left/right rows are applied last, then top, then the wall
+  const z one{(k == 2 && r == cc) ? 1.0 : 0.0, 0.0};
+  if (!p.xper && (i == 0 || i == p.nx - 1)) return one;
+  if (!p.yper) {
+    if (j == p.ny - 1) return one;
+    if (j == 0) {
+      if (r >= 1 && r <= 3) return one;
+      if (r == 0) {
+        if (dir == 1 && (p.wall == 1 || p.wall == 2)) return one;
+        if (dir == 2 && p.wall == 1) {
+          const double w[5] = {-gc4, -gc5, -gc1, -gc2, -gc3};   // slots 1..5
+          return z{cc == 0 ? w[k] : 0.0, 0.0};
+        }
+      }
+      if (r == 4) {
+        if (p.wallt == 0 || (dir == 1 && p.wallt == 1)) return one;
+        if (dir == 2 && p.wallt == 1) {
+          const double w[5] = {-gc4, -gc5, -gc1, -gc2, -gc3};
+          return z{cc == 4 ? w[k] : 0.0, 0.0};
+        }
+      }
+    }
+  }
+  const double dt = c.dtl[q];
+  const double X = (dir == 1 ? c.Ah : c.Bh)[(long)(r + 5 * cc) * P + q];
+  const double *Vh = dir == 1 ? c.V11 : c.V22;
+  const int ab = dir == 1 ? 0 : 2;
+  const bool calcd = dir == 1;
+  // which viscous / spanwise entry (r, cc) is
+  int vi = -1;
+  if (r == 1 && cc == 1) vi = 0;
+  else if (r == 1 && cc == 2) vi = 4;
+  else if (r == 2 && cc == 1) vi = 5;
+  else if (r == 2 && cc == 2) vi = 1;
+  else if (r == 3 && cc == 3) vi = 2;
+  else if (r == 4 && cc == 4) vi = 3;
+  int ai = -1;
+  if ((r == 1 && cc == 3) || (r == 3 && cc == 1)) ai = 0;
+  else if ((r == 2 && cc == 3) || (r == 3 && cc == 2)) ai = 1;
+  const double Vv = vi >= 0 ? Vh[(long)vi * P + q] : 0.0;
+  const double ABv = ai >= 0 ? c.ABhi[(long)(ab + ai) * P + q] : 0.0;
+
+  if (!per && (pos <= 1 || pos >= N - 2)) {
+    // boundary closure rows (src/lhsbt1f3d.f90, lhsbt2f3d.f90): first- and
+    // second-derivative weights of the column held in slot k
+    double F, S;
+    bool diag = k == 2;
+    if (pos == 0) {
+      const double Fs[5] = {gc4, gc5, gc1, gc2, gc3}, Ss[5] = {dd4, dd5, dd1, dd2, dd3};
+      F = Fs[k]; S = Ss[k];
+    } else if (pos == 1) {
+      const double Fs[5] = {gb5, gb1, gb2, gb3, gb4}, Ss[5] = {db5, db1, db2, db3, db4};
+      F = Fs[k]; S = Ss[k];
+    } else if (pos == N - 2) {
+      const double Fs[5] = {-gb4, -gb3, -gb2, -gb1, -gb5}, Ss[5] = {db4, db3, db2, db1, db5};
+      F = Fs[k]; S = Ss[k];
+    } else {
+      const double Fs[5] = {-gc3, -gc2, -gc1, -gc5, -gc4}, Ss[5] = {dd3, dd2, dd1, dd5, dd4};
+      F = Fs[k]; S = Ss[k];
+    }
+    double af = alfa * F * A.dinv;
+    double re, im = 0.0;
+    if (diag) {
+      double rc = calcd ? 1.0 : 0.0;
+      double D = c.Dh[(long)(r + 5 * cc) * P + q], Di = c.Dhi[(long)(r + 5 * cc) * P + q];
+      re = dt * (af * X + rc * alfa * D);
+      im = dt * (rc * alfa * Di);
+    } else {
+      re = af * dt * X;
+    }
+    if (ai >= 0) im = im - af * dt * ABv;
+    if (vi >= 0) re = re - alfa * S * A.dsinv * dt * Vv;
+    if (diag && r == cc) re = re + 1.0;
+    return z{re, im};
+  }
+  // interior (src/lhs1f3d.f90:59-230)
+  double re = A.a[k] * dt * X, im = 0.0;
+  if (ai >= 0) im = im - A.a[k] * dt * ABv;
+  if (calcd && k == 2) {
+    re = re + alfa * dt * c.Dh[(long)(r + 5 * cc) * P + q];
+    im = im + alfa * dt * c.Dhi[(long)(r + 5 * cc) * P + q];
+  }
+  if (vi >= 0) re = re - A.b[k] * dt * Vv;
+  if (k == 2 && r == cc) re = re + 1.0;
+  return z{re, im};
+}
+
+static AsmCtx make_asm(const lns3d_params &p, int dir) {
+  using namespace st;
+  AsmCtx A;
+  A.p = p;
+  A.d = dir == 1 ? 1.0 / (double)(p.nx - 1) : 1.0 / (double)(p.ny - 1);
+  A.dinv = 1.0 / A.d;
+  A.dsinv = 1.0 / (A.d * A.d);
+  double alfa = p.alfa, d = A.d;
+  double ga[5] = {ga1, ga2, 0.0, ga3, ga4}, da[5] = {da1, da2, da3, da4, da5};
+  for (int k = 0; k < 5; k++) {
+    A.a[k] = alfa * ga[k] / d;
+    A.b[k] = alfa * da[k] / (d * d);
+  }
+  return A;
+}
+
+// into the line-interleaved working copy: element ((row*5 + k)*25 + r*5 + cc)*L + line
+__global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, z *lu) {
+  const int nx = A.p.nx, ny = A.p.ny;
+  const long L = dir == 1 ? ny : nx, n = dir == 1 ? nx : ny;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L * n) return;
+  long line = t % L, row = t / L;
+  int i = dir == 1 ? (int)row : (int)line, j = dir == 1 ? (int)line : (int)row;
+  for (int k = 0; k < 5; k++)
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++)
+        lu[((row * 5 + k) * 25 + r * 5 + cc) * L + line] = lhs_entry(A, c, dir, i, j, k, r, cc);
+}
+
+__global__ void k_assemble_ref(AsmCtx A, Coeffs c, int dir, double2 *mat) {
+  const int nx = A.p.nx, ny = A.p.ny;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  int i = q % nx, j = q / nx;
+  for (int k = 0; k < 5; k++)
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++) {
+        z v = lhs_entry(A, c, dir, i, j, k, r, cc);
+        mat[q * 125 + k + 5 * r + 25 * cc] = make_double2(v.re, v.im);
+      }
+}
+
+int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &plan, hipStream_t s) {
+  AsmCtx A = make_asm(p, dir);
+  long tot = (long)p.nx * p.ny;
+  hipLaunchKernelGGL(k_assemble_lines, dim3(nblocks(tot, 128)), dim3(128), 0, s, A, c, dir,
+                     (z *)plan.lu);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *mat, hipStream_t s) {
+  AsmCtx A = make_asm(p, dir);
+  long tot = (long)p.nx * p.ny;
+  hipLaunchKernelGGL(k_assemble_ref, dim3(nblocks(tot, 128)), dim3(128), 0, s, A, c, dir,
+                     (double2 *)mat);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
+// RHS: c1 = d/dxi v first (stored), then the point operator
+__global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v, double2 *c1) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  int i = q % nx, j = q / nx;
+  const double2 *base = v + 5L * nx * j;
+  for (int e = 0; e < 5; e++) {
+    z g = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, base + e, 5);
+    c1[5 * q + e] = make_double2(g.re, g.im);
+  }
+}
+
+__global__ void k_rhs_point(lns3d_params p, Stencil st, Coeffs c, const double2 *v, const double2 *c1a,
+                            double2 *r, int with_rbe) {
+  using namespace st;
+  const int nx = p.nx, ny = p.ny;
+  const long P = (long)nx * ny;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  int i = q % nx, j = q / nx;
+  z vq[5], g1[5], g2[5], g11[5], g12[5], g22[5];
+  const double2 *colv = v + 5L * i, *colg = c1a + 5L * i, *rowv = v + 5L * nx * j;
+  for (int e = 0; e < 5; e++) {
+    vq[e] = ld(v + 5 * q + e);
+    g1[e] = ld(c1a + 5 * q + e);
+    g2[e] = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colv + e, 5L * nx);
+    g11[e] = d2<double2, z>(nx, p.xper, i, st.dx, st.dxsinv, rowv + e, 5);
+    g22[e] = d2<double2, z>(ny, p.yper, j, st.dy, st.dysinv, colv + e, 5L * nx);
+    g12[e] = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colg + e, 5L * nx);
+  }
+  if ((p.wallt == 1 || p.wallt == 2) && j == 0) g2[4] = z{0, 0};   // src/cgradbc.f90:15
+  // point operator (src/lrhs3d.f90:45-220)
+  z rr[5];
+  for (int m = 0; m < 5; m++) {
+    z a{0, 0};
+    for (int e = 0; e < 5; e++) a = a + sc(c.Ah[(long)(m + 5 * e) * P + q], g1[e]);
+    rr[m] = a;
+  }
+  for (int m = 0; m < 5; m++) {
+    z a = rr[m];
+    for (int e = 0; e < 5; e++) a = a + sc(c.Bh[(long)(m + 5 * e) * P + q], g2[e]);
+    rr[m] = a;
+  }
+  for (int m = 0; m < 5; m++) {
+    z a = rr[m];
+    for (int e = 0; e < 5; e++) a = a + sc(c.Dh[(long)(m + 5 * e) * P + q], vq[e]);
+    rr[m] = a;
+  }
+  const double *Vs[3] = {c.V11, c.V12, c.V22};
+  const z *hs[3] = {g11, g12, g22};
+  for (int tt = 0; tt < 3; tt++) {
+    const double *W = Vs[tt];
+    const z *h = hs[tt];
+    double w0 = W[q], w1 = W[P + q], w2 = W[2 * P + q], w3 = W[3 * P + q], w4 = W[4 * P + q],
+           w5 = W[5 * P + q];
+    rr[1] = rr[1] - sc(w0, h[1]) - sc(w4, h[2]);
+    rr[2] = rr[2] - sc(w5, h[1]) - sc(w1, h[2]);
+    rr[3] = rr[3] - sc(w2, h[3]);
+    rr[4] = rr[4] - sc(w3, h[4]);
+  }
+  const double ab0 = c.ABhi[q], ab1 = c.ABhi[P + q], ab2 = c.ABhi[2 * P + q], ab3 = c.ABhi[3 * P + q];
+  auto mi = [](z a) { return z{a.im, -a.re}; };      // -i * a
+  rr[1] = rr[1] + mi(sc(ab0, g1[3]) + sc(ab2, g2[3]));
+  rr[2] = rr[2] + mi(sc(ab1, g1[3]) + sc(ab3, g2[3]));
+  rr[3] = rr[3] + mi(sc(ab0, g1[1]) + sc(ab1, g1[2]) + sc(ab2, g2[1]) + sc(ab3, g2[2]));
+  for (int m = 0; m < 5; m++) {
+    z a{0, 0};
+    for (int e = 0; e < 5; e++) a = a + sc(c.Dhi[(long)(m + 5 * e) * P + q], vq[e]);
+    rr[m] = rr[m] + z{-a.im, a.re};                  // + i * a
+  }
+  // backward Euler: r = vold - v - dtl * r with vold = v (the field already
+  // satisfies the boundary conditions when the step starts)
+  if (with_rbe) {
+    double dt = c.dtl[q];
+    for (int m = 0; m < 5; m++) rr[m] = z{-(dt * rr[m].re), -(dt * rr[m].im)};
+  }
+  // boundary rows of the residual (src/rhsbc3d.f90:34-160, 200-215)
+  if (with_rbe) {
+    if (p.yper) {
+      if (j == ny - 1) for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
+    } else {
+      if (j == 0) {
+        for (int m = 1; m <= 3; m++) rr[m] = z{-vq[m].re, -vq[m].im};
+        if (p.wall == 1) {
+          z a = sc(gc1, ld(v + 5 * q)) + sc(gc2, ld(v + 5 * (q + nx))) + sc(gc3, ld(v + 5 * (q + 2L * nx))) +
+                sc(gc4, ld(v + 5 * (q + 3L * nx))) + sc(gc5, ld(v + 5 * (q + 4L * nx)));
+          rr[0] = a;
+        }
+        if (p.wall == 2)
+          rr[0] = ld(v + 5 * q) - sc(2.0, ld(v + 5 * (q + nx))) + ld(v + 5 * (q + 2L * nx));
+        if (p.wallt == 0) rr[4] = z{0, 0};
+        if (p.wallt == 1) {
+          z a = sc(gc1, ld(v + 5 * q + 4)) + sc(gc2, ld(v + 5 * (q + nx) + 4)) +
+                sc(gc3, ld(v + 5 * (q + 2L * nx) + 4)) + sc(gc4, ld(v + 5 * (q + 3L * nx) + 4)) +
+                sc(gc5, ld(v + 5 * (q + 4L * nx) + 4));
+          rr[4] = a;
+        }
+      }
+      if (j == ny - 1) for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
+    }
+    if (p.xper) {
+      if (i == nx - 1) for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
+    } else if (i == 0 || i == nx - 1) {
+      for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
+    }
+  }
+  for (int m = 0; m < 5; m++) r[5 * q + m] = make_double2(rr[m].re, rr[m].im);
+}
+
+int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
+               void *work, int with_rbe, hipStream_t s) {
+  long P = (long)p.nx * p.ny;
+  hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
+                     (const double2 *)v, (double2 *)work);
+  hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, 128)), dim3(128), 0, s, p, st, c, (const double2 *)v,
+                     (const double2 *)work, (double2 *)r, with_rbe);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
+// update and boundary conditions on v (src/impdrv3d.f90:340-356, itrbc3d.f90)
+__global__ void k_axpy(double2 *x, const double2 *d, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double2 a = x[t], b = d[t];
+  x[t] = make_double2(a.x + b.x, a.y + b.y);
+}
+
+__global__ void k_itrbc_wall_top(lns3d_params p, double2 *v) {
+  using namespace st;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx) return;
+  auto V = [&](int d, int jj) -> double2 & { return v[5 * (i + (long)nx * jj) + d]; };
+  if (p.yper) {
+    for (int d = 0; d < 5; d++) V(d, ny - 1) = V(d, 0);
+    return;
+  }
+  V(1, 0) = make_double2(0, 0); V(2, 0) = make_double2(0, 0); V(3, 0) = make_double2(0, 0);
+  auto lin = [&](int d, double a, double b, double c2, double e) {
+    z s = sc(a, ld(&V(d, 1))) + sc(b, ld(&V(d, 2))) + sc(c2, ld(&V(d, 3))) + sc(e, ld(&V(d, 4)));
+    return s;
+  };
+  if (p.wall == 1) {
+    z s = lin(0, gc2, gc3, gc4, gc5);
+    V(0, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+  }
+  if (p.wall == 2) {
+    z a = ld(&V(0, 1)), b = ld(&V(0, 2));
+    V(0, 0) = make_double2(2.0 * a.re - b.re, 2.0 * a.im - b.im);
+  }
+  if (p.wallt == 0) V(4, 0) = make_double2(0, 0);
+  if (p.wallt == 1) {
+    z s = lin(4, gc2, gc3, gc4, gc5);
+    V(4, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+  }
+  for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(0, 0);
+}
+
+__global__ void k_itrbc_lr(lns3d_params p, double2 *v) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = p.nx, ny = p.ny;
+  if (j >= ny) return;
+  double2 *row = v + 5L * nx * j;
+  for (int d = 0; d < 5; d++) {
+    if (p.xper) {
+      row[5L * (nx - 1) + d] = row[d];
+    } else {
+      row[d] = make_double2(0, 0);
+      row[5L * (nx - 1) + d] = make_double2(0, 0);
+    }
+  }
+}
+
+int launch_itrbc(const lns3d_params &p, void *v, hipStream_t s) {
+  hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (double2 *)v);
+  hipLaunchKernelGGL(k_itrbc_lr, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (double2 *)v);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_axpy(void *x, const void *d, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(nblocks(n, 256)), dim3(256), 0, s, (double2 *)x, (const double2 *)d, n);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_update_bc(const lns3d_params &p, void *v, const void *dv, hipStream_t s) {
+  int rc = launch_axpy(v, dv, 5L * p.nx * p.ny, s);
+  if (rc) return rc;
+  return launch_itrbc(p, v, s);
+}
+
+// ==========================================================================
+// approximate-factorization operator y = Lx (Ly x), blocks regenerated from
+// the coefficients (no assembled matrix is stored)
+__device__ __forceinline__ int slot_col(int N, int per, int pos, int k) {
+  if (per) {
+    int m = N - 1;
+    return (((pos - 2 + k) % m) + m) % m;
+  }
+  if (pos == 0) { const int t[5] = {3, 4, 0, 1, 2}; return t[k]; }
+  if (pos == 1) { const int t[5] = {4, 0, 1, 2, 3}; return t[k]; }
+  if (pos == N - 2) { const int t[5] = {N - 4, N - 3, N - 2, N - 1, N - 5}; return t[k]; }
+  if (pos == N - 1) { const int t[5] = {N - 3, N - 2, N - 1, N - 5, N - 4}; return t[k]; }
+  return pos - 2 + k;
+}
+
+__global__ void k_line_matvec(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
+  const int nx = A.p.nx, ny = A.p.ny;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  int i = q % nx, j = q / nx;
+  const int N = dir == 1 ? nx : ny, pos = dir == 1 ? i : j;
+  const int per = dir == 1 ? A.p.xper : A.p.yper;
+  int posc = (per && pos == N - 1) ? 0 : pos;     // duplicate periodic row
+  int ii = dir == 1 ? posc : i, jj = dir == 1 ? j : posc;
+  z acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int k = 0; k < 5; k++) {
+    int col = slot_col(N, per, posc, k);
+    long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
+    z xc[5];
+    for (int e = 0; e < 5; e++) xc[e] = ld(x + 5 * qc + e);
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++) acc[r] = cfma(lhs_entry(A, c, dir, ii, jj, k, r, cc), xc[cc], acc[r]);
+  }
+  for (int r = 0; r < 5; r++) y[5 * q + r] = make_double2(acc[r].re, acc[r].im);
+}
+
+int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,
+                     hipStream_t s) {
+  long P = (long)p.nx * p.ny;
+  AsmCtx Ax = make_asm(p, 1), Ay = make_asm(p, 2);
+  hipLaunchKernelGGL(k_line_matvec, dim3(nblocks(P, 128)), dim3(128), 0, s, Ay, c, 2,
+                     (const double2 *)x, (double2 *)tmp);
+  hipLaunchKernelGGL(k_line_matvec, dim3(nblocks(P, 128)), dim3(128), 0, s, Ax, c, 1,
+                     (const double2 *)tmp, (double2 *)y);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// y = b - y and max norms |y|, |b| (atomic max on the bit patterns)
+__global__ void k_residual(const double2 *b, double2 *y, long n, unsigned long long *nrm) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double ay = 0.0, abv = 0.0;
+  if (t < n) {
+    double2 bb = b[t], yy = y[t];
+    double2 rr = make_double2(bb.x - yy.x, bb.y - yy.y);
+    y[t] = rr;
+    ay = fmax(fabs(rr.x), fabs(rr.y));
+    abv = fmax(fabs(bb.x), fabs(bb.y));
+  }
+  __shared__ double s1[256], s2[256];
+  s1[threadIdx.x] = ay;
+  s2[threadIdx.x] = abv;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(nrm, (unsigned long long)__double_as_longlong(s1[0]));
+    atomicMax(nrm + 1, (unsigned long long)__double_as_longlong(s2[0]));
+  }
+}
+
+__global__ void k_zero2_u64(unsigned long long *x) { x[0] = 0ull; x[1] = 0ull; }
+
+int launch_residual_norms(const void *b, void *y, long n, double *norms, hipStream_t s) {
+  auto *u = (unsigned long long *)norms;
+  hipLaunchKernelGGL(k_zero2_u64, dim3(1), dim3(1), 0, s, u);
+  hipLaunchKernelGGL(k_residual, dim3(nblocks(n, 256)), dim3(256), 0, s, (const double2 *)b,
+                     (double2 *)y, n, u);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace lns3d

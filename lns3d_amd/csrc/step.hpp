@@ -1,0 +1,49 @@
+// Device-side pieces of one implicit step (step.hip) and their host launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/lns3d_amd.h"
+#include "penta.hpp"
+
+namespace lns3d {
+
+// finite-difference weights derived from the scheme selectors and spacing
+struct Stencil {
+  double gx[6], gy[6];   // interior first derivative (premultiplied by 1/d)
+  double dx[7], dy[7];   // interior second derivative (premultiplied by 1/d^2)
+  double dxinv, dyinv, dxsinv, dysinv;
+  double dxi, deta;
+};
+Stencil make_stencil(const lns3d_params &p);
+
+// coefficients on the device, structure of arrays over the P = nx*ny points
+struct Coeffs {
+  double *Ah, *Bh, *Dh, *Dhi;   // [25][P], entry (r,c) at index r + 5c
+  double *V11, *V12, *V22;      // [6][P]
+  double *ABhi;                 // [4][P]
+  double *dtl;                  // [P]
+  double *vm;                   // [5][P] mean flow (AoS (5,nx,ny) as passed)
+  double *metric;               // [10][P]
+};
+
+int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);
+int launch_timestep(const lns3d_params &p, Coeffs &c, double *work, hipStream_t s);
+// assemble the LHS of direction dir (1: xi, 2: eta) into the line-interleaved
+// working copy of plan (lines: y index for dir 1, x index for dir 2)
+int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &plan, hipStream_t s);
+// assembled LHS in the reference layout mat(5,5,5,nx,ny) (testing)
+int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *mat, hipStream_t s);
+// r = -dtl * L(v) with the boundary rows of rhsBC3D; work holds 5P complex
+int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
+               void *work, int with_rbe, hipStream_t s);
+int launch_update_bc(const lns3d_params &p, void *v, const void *dv, hipStream_t s);
+int launch_itrbc(const lns3d_params &p, void *v, hipStream_t s);
+// y = M x with M = Lx * Ly the approximate-factorization operator (FP64,
+// blocks regenerated from the coefficients); tmp holds 5P complex
+int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,
+                     hipStream_t s);
+// y = b - y (complex, n values) and the max norms of y and b into norms[0..1]
+int launch_residual_norms(const void *b, void *y, long n, double *norms, hipStream_t s);
+int launch_axpy(void *x, const void *d, long n, hipStream_t s);   // x += d
+
+}  // namespace lns3d

@@ -30,6 +30,21 @@ SIGNATURES = {
     "lns3d_penta_solve": ([VP, VP, VP], CI),
     "lns3d_penta_solve_reference_order": ([VP, VP, VP], CI),
     "lns3d_penta_destroy": ([VP], CI),
+    "lns3d_ctx_create": ([VP, ctypes.POINTER(VP)], CI),
+    "lns3d_ctx_destroy": ([VP], CI),
+    "lns3d_ctx_set_stream": ([VP, VP], CI),
+    "lns3d_ctx_synchronize": ([VP], CI),
+    "lns3d_setup": ([VP, VP, VP], CI),
+    "lns3d_set_coeffs": ([VP] * 10, CI),
+    "lns3d_get_coeffs": ([VP] * 10, CI),
+    "lns3d_factor": ([VP], CI),
+    "lns3d_assemble": ([VP, CI, VP], CI),
+    "lns3d_lrhs3d": ([VP, VP, VP], CI),
+    "lns3d_itrbc3d": ([VP, VP], CI),
+    "lns3d_solve": ([VP, VP], CI),
+    "lns3d_step": ([VP, VP, VP], CI),
+    "lns3d_step_refined": ([VP, VP, VP, CD, CI, ctypes.POINTER(CI), ctypes.POINTER(CD)], CI),
+    "lns3d_step_fp32": ([VP, VP, VP], CI),
 }
 
 _lib = None
@@ -91,6 +106,109 @@ class Penta:
     def close(self):
         if self.h:
             lib().lns3d_penta_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CParams(ctypes.Structure):
+    """lns3d_params of include/lns3d_amd.h"""
+    _fields_ = [(n, CI) for n in ("nx", "ny", "optx", "opty", "xper", "yper", "wall", "wallt",
+                                  "top", "left", "right", "linear", "mattyp", "loctime")] + \
+               [(n, CD) for n in ("Ma", "Re", "Pr", "gamma", "kz", "omega", "cflmax", "dtmax",
+                                  "alfa")] + \
+               [("periodic_exact", CI), ("precision", CI)]
+
+    @classmethod
+    def from_params(cls, p, precision=0):
+        s = cls()
+        for name, _ in cls._fields_:
+            if name == "precision":
+                s.precision = precision
+            else:
+                setattr(s, name, getattr(p, name))
+        return s
+
+
+class Step:
+    """Device-resident implicit step context (lns3d_ctx_*)."""
+
+    def __init__(self, p, precision: int = 0, stream=None):
+        self.params = p
+        self.cp = CParams.from_params(p, precision)
+        h = VP()
+        check(lib().lns3d_ctx_create(ctypes.byref(self.cp), ctypes.byref(h)), "lns3d_ctx_create")
+        self.h = h
+        if stream is not None:
+            check(lib().lns3d_ctx_set_stream(self.h, stream), "lns3d_ctx_set_stream")
+
+    def setup(self, metric, vm):
+        import numpy as np
+        met = np.asfortranarray(metric, dtype=np.float64)
+        v = np.asfortranarray(vm, dtype=np.float64)
+        check(lib().lns3d_setup(self.h, met.ctypes.data, v.ctypes.data), "lns3d_setup")
+
+    def set_coeffs(self, c, dtl):
+        import numpy as np
+        arrs = [np.asfortranarray(c[k], dtype=np.float64)
+                for k in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi")]
+        d = np.asfortranarray(dtl, dtype=np.float64)
+        check(lib().lns3d_set_coeffs(self.h, *[a.ctypes.data for a in arrs], d.ctypes.data),
+              "lns3d_set_coeffs")
+
+    def get_coeffs(self):
+        import numpy as np
+        nx, ny = self.params.nx, self.params.ny
+        z = lambda *s: np.zeros(s, order="F")
+        c = dict(Ah=z(5, 5, nx, ny), Bh=z(5, 5, nx, ny), Dh=z(5, 5, nx, ny), Dhi=z(5, 5, nx, ny),
+                 Vh11=z(6, nx, ny), Vh12=z(6, nx, ny), Vh22=z(6, nx, ny), ABhi=z(6, nx, ny))
+        dtl = z(nx, ny)
+        check(lib().lns3d_get_coeffs(self.h, *[c[k].ctypes.data for k in
+                                               ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22",
+                                                "ABhi")], dtl.ctypes.data), "lns3d_get_coeffs")
+        return c, dtl
+
+    def factor(self):
+        check(lib().lns3d_factor(self.h), "lns3d_factor")
+
+    def assemble(self, direction, mat_dev):
+        check(lib().lns3d_assemble(self.h, direction, ptr(mat_dev)), "lns3d_assemble")
+
+    def lrhs3d(self, v_dev, r_dev):
+        check(lib().lns3d_lrhs3d(self.h, ptr(v_dev), ptr(r_dev)), "lns3d_lrhs3d")
+
+    def itrbc3d(self, v_dev):
+        check(lib().lns3d_itrbc3d(self.h, ptr(v_dev)), "lns3d_itrbc3d")
+
+    def solve(self, r_dev):
+        check(lib().lns3d_solve(self.h, ptr(r_dev)), "lns3d_solve")
+
+    def step(self, v_dev, dv_dev=None):
+        check(lib().lns3d_step(self.h, ptr(v_dev), ptr(dv_dev) if dv_dev is not None else None),
+              "lns3d_step")
+
+    def step_fp32(self, v_dev, dv_dev=None):
+        check(lib().lns3d_step_fp32(self.h, ptr(v_dev), ptr(dv_dev) if dv_dev is not None else None),
+              "lns3d_step_fp32")
+
+    def step_refined(self, v_dev, dv_dev=None, tol=1e-10, max_iters=8):
+        it = CI(0)
+        res = CD(0.0)
+        check(lib().lns3d_step_refined(self.h, ptr(v_dev),
+                                       ptr(dv_dev) if dv_dev is not None else None, tol, max_iters,
+                                       ctypes.byref(it), ctypes.byref(res)), "lns3d_step_refined")
+        return it.value, res.value
+
+    def synchronize(self):
+        check(lib().lns3d_ctx_synchronize(self.h), "lns3d_ctx_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lns3d_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
