@@ -130,6 +130,9 @@ int lns3d_lrhs3d(lns3d_ctx *c, const void *v_dev, void *r_dev);
 int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev);
 /* xi then eta line solves of r in place (src/impdrv3d.f90:284-327) */
 int lns3d_solve(lns3d_ctx *c, void *r_dev);
+/* one direction only: dir 1 the xi lines (the cpenta1bc call of
+ * src/impdrv3d.f90:295 on the transposed array), dir 2 the eta lines (:327) */
+int lns3d_solve_dir(lns3d_ctx *c, int dir, void *r_dev);
 /* one step: lrhs3D, rbe3D, rhsBC3D, xi and eta solves, v += r, itrBC3D;
  * v must already satisfy the boundary conditions; dv_dev (may be NULL)
  * receives the increment r */

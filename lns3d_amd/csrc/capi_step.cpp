@@ -251,6 +251,13 @@ extern "C" int lns3d_solve(lns3d_ctx *c, void *r_dev) {
   return solve_xy(c, c->px, c->py, r_dev);
 }
 
+extern "C" int lns3d_solve_dir(lns3d_ctx *c, int dir, void *r_dev) {
+  if (!c || !c->factored || !r_dev || (dir != 1 && dir != 2))
+    return fail(LNS3D_EINVAL, "lns3d_solve_dir: bad arguments or not factored");
+  if (dir == 1) return penta_solve(&c->px, r_dev, c->p.nx, 1, c->stream);
+  return penta_solve(&c->py, r_dev, 1, c->p.nx, c->stream);
+}
+
 // One implicit step (src/impdrv3d.f90:146-356, impl = 1, niter = 1):
 // r = -dtl * lrhs3D(v) with rhsBC3D rows, xi solve, eta solve, v += r,
 // itrBC3D.  v must already satisfy the boundary conditions (call

@@ -342,8 +342,7 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
     const cx<T> *F = fwd + ((long)line * n + r0) * 60;
     copy_rows_to_lds(F, (total_rows - ((long)line * n + r0)) * 60, rec[k & 1], nr * 60, wave - 1, 3,
                      lane);
-    int t = tid - 64;
-    if (t < nr * 5) {
+    for (int t = tid - 64; t < nr * 5; t += 192) {
       int row = t / 5, mm = t - row * 5;
       const double2 *b = X + (long)(r0 + row) * rs;
       const cx<T> *Li = F + (long)row * 60 + 50 + mm * (mm - 1) / 2;
@@ -436,8 +435,7 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
     const cx<T> *Bw = bwd + ((long)line * n + rlo) * 66;
     copy_rows_to_lds(Bw, (total_rows - ((long)line * n + rlo)) * 66, rec[k & 1], nr * 66, wave - 1, 3,
                      lane);
-    int t = tid - 64;
-    if (t < nr * 5) {
+    for (int t = tid - 64; t < nr * 5; t += 192) {
       int row = t / 5, mm = t - row * 5;     // row counted from rlo
       const double2 *y = X + (long)(rlo + row) * rs;
       const cx<T> *Ui = Bw + (long)row * 66 + 50 + mm * 5 - mm * (mm - 1) / 2;

@@ -42,6 +42,7 @@ SIGNATURES = {
     "lns3d_lrhs3d": ([VP, VP, VP], CI),
     "lns3d_itrbc3d": ([VP, VP], CI),
     "lns3d_solve": ([VP, VP], CI),
+    "lns3d_solve_dir": ([VP, CI, VP], CI),
     "lns3d_step": ([VP, VP, VP], CI),
     "lns3d_step_refined": ([VP, VP, VP, CD, CI, ctypes.POINTER(CI), ctypes.POINTER(CD)], CI),
     "lns3d_step_fp32": ([VP, VP, VP], CI),
