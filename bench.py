@@ -201,7 +201,11 @@ class _quiet_stdout:
 
 def cpu_baseline(p, met, vm, v0, nsteps):
     with _quiet_stdout():
-        return _cpu_baseline(p, met, vm, v0, nsteps)
+        out = _cpu_baseline(p, met, vm, v0, nsteps)
+        from oracle.oracle import Reference
+        if Reference.available():
+            Reference().lib.ref_flush()
+    return out
 
 
 def _cpu_baseline(p, met, vm, v0, nsteps):

@@ -610,101 +610,126 @@ __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v
   }
 }
 
-__global__ void k_rhs_point(lns3d_params p, Stencil st, Coeffs c, const double2 *v, const double2 *c1a,
-                            double2 *r, int with_rbe) {
+// The point operator works on tiles of RT consecutive grid points with one
+// workgroup of five waves.  Phase 1 stages the tile's U and U_xi and its 17
+// stencil derivatives (U_eta; U_xixi, U_xieta, U_etaeta of components 1-4) in
+// LDS; phase 2 has wave m form row m of the operator for the tile, so each
+// wave streams one row of every coefficient matrix ([25][P] layout, one
+// contiguous run of RT values per load).  Terms are accumulated in the order
+// of src/lrhs3d.f90:45-220 (U,xi and U,eta; U; U,xixi; U,xieta; U,etaeta;
+// A_i/B_i; D_i), then rbe3D and rhsBC3D (impdrv3d.f90:444-470,
+// src/rhsbc3d.f90:34-160, 200-215).
+constexpr int RT = 64;
+constexpr int NDER = 17;
+
+__global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, Coeffs c,
+                                                  const double2 *v, const double2 *c1a, double2 *r,
+                                                  int with_rbe) {
   using namespace st;
+  __shared__ double2 sv[5 * RT], sg1[5 * RT], sd[NDER * RT], sr[5 * RT];
   const int nx = p.nx, ny = p.ny;
   const long P = (long)nx * ny;
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P) return;
-  int i = q % nx, j = q / nx;
-  z vq[5], g1[5], g2[5], g11[5], g12[5], g22[5];
-  const double2 *colv = v + 5L * i, *colg = c1a + 5L * i, *rowv = v + 5L * nx * j;
-  for (int e = 0; e < 5; e++) {
-    vq[e] = ld(v + 5 * q + e);
-    g1[e] = ld(c1a + 5 * q + e);
-    g2[e] = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colv + e, 5L * nx);
-    g11[e] = d2<double2, z>(nx, p.xper, i, st.dx, st.dxsinv, rowv + e, 5);
-    g22[e] = d2<double2, z>(ny, p.yper, j, st.dy, st.dysinv, colv + e, 5L * nx);
-    g12[e] = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colg + e, 5L * nx);
+  const long q0 = (long)blockIdx.x * RT;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int npt = (int)min((long)RT, P - q0);
+  if (tid < 5 * npt) {
+    sv[tid] = v[5 * q0 + tid];
+    sg1[tid] = c1a[5 * q0 + tid];
   }
-  if ((p.wallt == 1 || p.wallt == 2) && j == 0) g2[4] = z{0, 0};   // src/cgradbc.f90:15
-  // point operator (src/lrhs3d.f90:45-220)
-  z rr[5];
-  for (int m = 0; m < 5; m++) {
+  const long q = q0 + l;
+  const int i = (int)(q % nx), j = (int)(q / nx);
+  if (l < npt) {
+    const double2 *colv = v + 5L * i, *colg = c1a + 5L * i, *rowv = v + 5L * nx * j;
+    for (int sl = w; sl < NDER; sl += 5) {   // slot index is uniform across the wave
+      z h;
+      if (sl < 5) {
+        h = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colv + sl, 5L * nx);
+        if (sl == 4 && (p.wallt == 1 || p.wallt == 2) && j == 0) h = z{0, 0};   // src/cgradbc.f90:15
+      } else if (sl < 9) {
+        h = d2<double2, z>(nx, p.xper, i, st.dx, st.dxsinv, rowv + (sl - 4), 5);
+      } else if (sl < 13) {
+        h = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colg + (sl - 8), 5L * nx);
+      } else {
+        h = d2<double2, z>(ny, p.yper, j, st.dy, st.dysinv, colv + (sl - 12), 5L * nx);
+      }
+      sd[sl * RT + l] = make_double2(h.re, h.im);
+    }
+  }
+  __syncthreads();
+  if (l < npt) {
+    const int m = w;
+    auto S = [](double2 a) { return z{a.x, a.y}; };
+    auto g1 = [&](int e) { return S(sg1[5 * l + e]); };
+    auto g2 = [&](int e) { return S(sd[e * RT + l]); };
+    auto vq = [&](int e) { return S(sv[5 * l + e]); };
     z a{0, 0};
-    for (int e = 0; e < 5; e++) a = a + sc(c.Ah[(long)(m + 5 * e) * P + q], g1[e]);
-    rr[m] = a;
-  }
-  for (int m = 0; m < 5; m++) {
-    z a = rr[m];
-    for (int e = 0; e < 5; e++) a = a + sc(c.Bh[(long)(m + 5 * e) * P + q], g2[e]);
-    rr[m] = a;
-  }
-  for (int m = 0; m < 5; m++) {
-    z a = rr[m];
-    for (int e = 0; e < 5; e++) a = a + sc(c.Dh[(long)(m + 5 * e) * P + q], vq[e]);
-    rr[m] = a;
-  }
-  const double *Vs[3] = {c.V11, c.V12, c.V22};
-  const z *hs[3] = {g11, g12, g22};
-  for (int tt = 0; tt < 3; tt++) {
-    const double *W = Vs[tt];
-    const z *h = hs[tt];
-    double w0 = W[q], w1 = W[P + q], w2 = W[2 * P + q], w3 = W[3 * P + q], w4 = W[4 * P + q],
-           w5 = W[5 * P + q];
-    rr[1] = rr[1] - sc(w0, h[1]) - sc(w4, h[2]);
-    rr[2] = rr[2] - sc(w5, h[1]) - sc(w1, h[2]);
-    rr[3] = rr[3] - sc(w2, h[3]);
-    rr[4] = rr[4] - sc(w3, h[4]);
-  }
-  const double ab0 = c.ABhi[q], ab1 = c.ABhi[P + q], ab2 = c.ABhi[2 * P + q], ab3 = c.ABhi[3 * P + q];
-  auto mi = [](z a) { return z{a.im, -a.re}; };      // -i * a
-  rr[1] = rr[1] + mi(sc(ab0, g1[3]) + sc(ab2, g2[3]));
-  rr[2] = rr[2] + mi(sc(ab1, g1[3]) + sc(ab3, g2[3]));
-  rr[3] = rr[3] + mi(sc(ab0, g1[1]) + sc(ab1, g1[2]) + sc(ab2, g2[1]) + sc(ab3, g2[2]));
-  for (int m = 0; m < 5; m++) {
-    z a{0, 0};
-    for (int e = 0; e < 5; e++) a = a + sc(c.Dhi[(long)(m + 5 * e) * P + q], vq[e]);
-    rr[m] = rr[m] + z{-a.im, a.re};                  // + i * a
-  }
-  // backward Euler: r = vold - v - dtl * r with vold = v (the field already
-  // satisfies the boundary conditions when the step starts)
-  if (with_rbe) {
-    double dt = c.dtl[q];
-    for (int m = 0; m < 5; m++) rr[m] = z{-(dt * rr[m].re), -(dt * rr[m].im)};
-  }
-  // boundary rows of the residual (src/rhsbc3d.f90:34-160, 200-215)
-  if (with_rbe) {
-    if (p.yper) {
-      if (j == ny - 1) for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
-    } else {
-      if (j == 0) {
-        for (int m = 1; m <= 3; m++) rr[m] = z{-vq[m].re, -vq[m].im};
-        if (p.wall == 1) {
-          z a = sc(gc1, ld(v + 5 * q)) + sc(gc2, ld(v + 5 * (q + nx))) + sc(gc3, ld(v + 5 * (q + 2L * nx))) +
+#pragma unroll
+    for (int e = 0; e < 5; e++) a = a + sc(c.Ah[(long)(m + 5 * e) * P + q], g1(e));
+#pragma unroll
+    for (int e = 0; e < 5; e++) a = a + sc(c.Bh[(long)(m + 5 * e) * P + q], g2(e));
+#pragma unroll
+    for (int e = 0; e < 5; e++) a = a + sc(c.Dh[(long)(m + 5 * e) * P + q], vq(e));
+    if (m > 0) {
+      // viscous terms: slots 4+e (xixi), 8+e (xieta), 12+e (etaeta)
+      const double *Ws[3] = {c.V11, c.V12, c.V22};
+#pragma unroll
+      for (int tt = 0; tt < 3; tt++) {
+        const double *W = Ws[tt];
+        auto h = [&](int e) { return S(sd[(4 * tt + 4 + e) * RT + l]); };
+        if (m == 1) a = a - sc(W[q], h(1)) - sc(W[4 * P + q], h(2));
+        else if (m == 2) a = a - sc(W[5 * P + q], h(1)) - sc(W[P + q], h(2));
+        else if (m == 3) a = a - sc(W[2 * P + q], h(3));
+        else a = a - sc(W[3 * P + q], h(4));
+      }
+    }
+    if (m >= 1 && m <= 3) {
+      const double ab0 = c.ABhi[q], ab1 = c.ABhi[P + q], ab2 = c.ABhi[2 * P + q], ab3 = c.ABhi[3 * P + q];
+      z ti;
+      if (m == 1) ti = sc(ab0, g1(3)) + sc(ab2, g2(3));
+      else if (m == 2) ti = sc(ab1, g1(3)) + sc(ab3, g2(3));
+      else ti = sc(ab0, g1(1)) + sc(ab1, g1(2)) + sc(ab2, g2(1)) + sc(ab3, g2(2));
+      a = a + z{ti.im, -ti.re};                       // -i * ti
+    }
+    {
+      z b{0, 0};
+#pragma unroll
+      for (int e = 0; e < 5; e++) b = b + sc(c.Dhi[(long)(m + 5 * e) * P + q], vq(e));
+      a = a + z{-b.im, b.re};                         // + i * b
+    }
+    // backward Euler: r = vold - v - dtl * r with vold = v (the field already
+    // satisfies the boundary conditions when the step starts)
+    if (with_rbe) {
+      const double dt = c.dtl[q];
+      a = z{-(dt * a.re), -(dt * a.im)};
+      // boundary rows of the residual
+      if (p.yper) {
+        if (j == ny - 1) a = z{0, 0};
+      } else {
+        if (j == 0) {
+          if (m >= 1 && m <= 3) a = z{-vq(m).re, -vq(m).im};
+          if (m == 0 && p.wall == 1)
+            a = sc(gc1, ld(v + 5 * q)) + sc(gc2, ld(v + 5 * (q + nx))) + sc(gc3, ld(v + 5 * (q + 2L * nx))) +
                 sc(gc4, ld(v + 5 * (q + 3L * nx))) + sc(gc5, ld(v + 5 * (q + 4L * nx)));
-          rr[0] = a;
-        }
-        if (p.wall == 2)
-          rr[0] = ld(v + 5 * q) - sc(2.0, ld(v + 5 * (q + nx))) + ld(v + 5 * (q + 2L * nx));
-        if (p.wallt == 0) rr[4] = z{0, 0};
-        if (p.wallt == 1) {
-          z a = sc(gc1, ld(v + 5 * q + 4)) + sc(gc2, ld(v + 5 * (q + nx) + 4)) +
+          if (m == 0 && p.wall == 2)
+            a = ld(v + 5 * q) - sc(2.0, ld(v + 5 * (q + nx))) + ld(v + 5 * (q + 2L * nx));
+          if (m == 4 && p.wallt == 0) a = z{0, 0};
+          if (m == 4 && p.wallt == 1)
+            a = sc(gc1, ld(v + 5 * q + 4)) + sc(gc2, ld(v + 5 * (q + nx) + 4)) +
                 sc(gc3, ld(v + 5 * (q + 2L * nx) + 4)) + sc(gc4, ld(v + 5 * (q + 3L * nx) + 4)) +
                 sc(gc5, ld(v + 5 * (q + 4L * nx) + 4));
-          rr[4] = a;
         }
+        if (j == ny - 1) a = z{0, 0};
       }
-      if (j == ny - 1) for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
+      if (p.xper) {
+        if (i == nx - 1) a = z{0, 0};
+      } else if (i == 0 || i == nx - 1) {
+        a = z{0, 0};
+      }
     }
-    if (p.xper) {
-      if (i == nx - 1) for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
-    } else if (i == 0 || i == nx - 1) {
-      for (int m = 0; m < 5; m++) rr[m] = z{0, 0};
-    }
+    sr[5 * l + m] = make_double2(a.re, a.im);
   }
-  for (int m = 0; m < 5; m++) r[5 * q + m] = make_double2(rr[m].re, rr[m].im);
+  __syncthreads();
+  if (tid < 5 * npt) r[5 * q0 + tid] = sr[tid];
 }
 
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
@@ -712,7 +737,7 @@ int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const 
   long P = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
                      (const double2 *)v, (double2 *)work);
-  hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, 128)), dim3(128), 0, s, p, st, c, (const double2 *)v,
+  hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, RT)), dim3(5 * 64), 0, s, p, st, c, (const double2 *)v,
                      (const double2 *)work, (double2 *)r, with_rbe);
   LNS3D_CHECK(hipGetLastError());
   return 0;

@@ -329,3 +329,9 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   call itrBC3D(v, vm)
   deallocate(vold, r)
 end subroutine ref_step
+
+! flush the Fortran unit 6 buffer (the reference's routines write progress
+! lines to it)
+subroutine ref_flush() bind(C, name="ref_flush")
+  flush(6)
+end subroutine ref_flush
