@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace lns3d {
 
@@ -25,6 +26,12 @@ template <typename T> __host__ __device__ __forceinline__ cx<T> cfma(cx<T> a, cx
   acc.im = fma(a.re, b.im, acc.im);
   acc.im = fma(a.im, b.re, acc.im);
   return acc;
+}
+// a*b with one rounding fewer per part than the textbook form
+template <typename T> __host__ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  T re = fma(a.re, b.re, -(a.im * b.im));
+  T im = fma(a.re, b.im, a.im * b.re);
+  return {re, im};
 }
 // a / b, Smith's algorithm (Smith, CACM 5(8), 1962)
 template <typename T> __host__ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
@@ -58,16 +65,17 @@ __device__ __forceinline__ cx<float> bcast(cx<float> v, int lane) {
 }
 
 // broadcast lane q's complex value to the 16 lanes of its DPP row
-// (v_mov_b64_dpp row_newbcast:q); q must be a compile-time constant
+// (v_mov_b64_dpp row_newbcast:q); q must be a compile-time constant.  Every
+// lane receives a value, so bound_ctrl spares the old-value operand.
 template <int Q> __device__ __forceinline__ cx<double> row_bcast(cx<double> v) {
   long a = __builtin_bit_cast(long, v.re), b = __builtin_bit_cast(long, v.im);
-  a = __builtin_amdgcn_update_dpp(0L, a, 0x150 + Q, 0xF, 0xF, false);
-  b = __builtin_amdgcn_update_dpp(0L, b, 0x150 + Q, 0xF, 0xF, false);
+  a = __builtin_amdgcn_update_dpp(0L, a, 0x150 + Q, 0xF, 0xF, true);
+  b = __builtin_amdgcn_update_dpp(0L, b, 0x150 + Q, 0xF, 0xF, true);
   return {__builtin_bit_cast(double, a), __builtin_bit_cast(double, b)};
 }
 template <int Q> __device__ __forceinline__ cx<float> row_bcast(cx<float> v) {
   long a = __builtin_bit_cast(long, v);
-  a = __builtin_amdgcn_update_dpp(0L, a, 0x150 + Q, 0xF, 0xF, false);
+  a = __builtin_amdgcn_update_dpp(0L, a, 0x150 + Q, 0xF, 0xF, true);
   return __builtin_bit_cast(cx<float>, a);
 }
 template <typename T> __device__ __forceinline__ void row_bcast5(cx<T> v, cx<T> (&y)[5]) {

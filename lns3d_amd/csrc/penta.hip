@@ -317,20 +317,27 @@ __device__ __forceinline__ void copy_rows_to_lds(const cx<T> *src, long src_elem
   }
 }
 
-template <typename T, int R>
-__global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict__ fwd,
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__restrict__ fwd,
                                                  const cx<T> *__restrict__ bwd,
                                                  const cx<T> *__restrict__ tail, double2 *rhs,
-                                                 long sL, long sR, long total_rows) {
+                                                 long sL, long sR) {
   constexpr int RECW = 66;                                       // LDS row stride (complex)
   constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
-  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][BUFE];
-  __shared__ cx<T> cb[2][R][5];
-  __shared__ cx<T> saved[2][5];   // y_{n-5}, y_{n-4} (forward); x_3, x_4 (backward)
-  const long line = blockIdx.x;
+  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][G][BUFE];
+  __shared__ cx<T> cb[2][G][R][5];
+  const long total_rows = L * n;
+  const long line0 = (long)blockIdx.x * G;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int m = lane < 5 ? lane : 4;
+  // G lines per workgroup; DPP row r (16 lanes) of the compute wave runs the
+  // recurrence of line g = r mod G (rows past the last line repeat it and
+  // store the same values).  Lane m of a row owns component m (lanes past 4
+  // duplicate component 4).
+  const int g = (lane >> 4) % G;
+  const int m = min(lane & 15, 4);
+  auto line_of = [&](int gg) { return min(line0 + gg, L - 1); };
+  const long line = line_of(g);
   const int nchunk = (n + R - 1) / R;
   double2 *X = rhs + line * sL * 5;
   const long rs = sR * 5;
@@ -339,13 +346,18 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
   // ---------------- forward: y_i = c_i + N1 y_{i-2} + N2 y_{i-1} ----------
   auto load_fwd = [&](int k) {
     int r0 = k * R, nr = min(R, n - r0);
-    const cx<T> *F = fwd + ((long)line * n + r0) * 60;
-    copy_rows_to_lds(F, (total_rows - ((long)line * n + r0)) * 60, rec[k & 1], nr * 60, wave - 1, 3,
-                     lane);
-    for (int t = tid - 64; t < nr * 5; t += 192) {
-      int row = t / 5, mm = t - row * 5;
-      const double2 *b = X + (long)(r0 + row) * rs;
-      const cx<T> *Li = F + (long)row * 60 + 50 + mm * (mm - 1) / 2;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+      const long ln = line_of(gg);
+      copy_rows_to_lds(fwd + (ln * n + r0) * 60, (total_rows - (ln * n + r0)) * 60, rec[k & 1][gg],
+                       nr * 60, wave - 1, 3, lane);
+    }
+    for (int t = tid - 64; t < G * nr * 5; t += 192) {
+      int gg = t / (nr * 5), rem = t - gg * nr * 5;
+      int row = rem / 5, mm = rem - row * 5;
+      const long ln = line_of(gg);
+      const double2 *b = rhs + ln * sL * 5 + (long)(r0 + row) * rs;
+      const cx<T> *Li = fwd + (ln * n + r0 + row) * 60 + 50 + mm * (mm - 1) / 2;
       double2 bm = b[mm];
       cx<T> acc{(T)bm.x, (T)bm.y};
 #pragma unroll
@@ -354,74 +366,145 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
           double2 bq = b[q];
           acc = cfma(Li[q], cx<T>{(T)bq.x, (T)bq.y}, acc);
         }
-      cb[k & 1][row][mm] = acc;
+      cb[k & 1][gg][row][mm] = acc;
     }
     __builtin_amdgcn_s_waitcnt(0);
   };
-  cx<T> y1[5], y2[5];
-#pragma unroll
-  for (int q = 0; q < 5; q++) { y1[q] = cx<T>{0, 0}; y2[q] = cx<T>{0, 0}; }
-  if (wave != 0) load_fwd(0);
-  __syncthreads();
-  // one row of the recurrence on lanes 0..4 (lane m = component m):
-  // out = c + W1 * ym2 + W2 * ym1, then out is broadcast into ym2 (which
-  // held the row two back and is dead afterwards)
-  auto row_step = [&](const cx<T> (&w)[10], cx<T> c, cx<T> (&ym2)[5], const cx<T> (&ym1)[5],
-                      int i, bool fwd_pass) {
-    cx<T> a0 = c, p0{0, 0}, p1{0, 0};
-#pragma unroll
-    for (int q = 0; q < 5; q++) a0 = cfma(w[5 + q], ym2[q], a0);
-    p0 = cfma(w[0], ym1[0], p0); p1 = cfma(w[1], ym1[1], p1);
-    p0 = cfma(w[2], ym1[2], p0); p1 = cfma(w[3], ym1[3], p1);
-    p0 = cfma(w[4], ym1[4], p0);
+  // ---- compute wave: the recurrence on lanes 0..4 (lane m = component m)
+  // Row i is split into the part that needs the newest value,
+  //   y_i = a0_i + W2_i * y_{i-1}          (products summed as a tree)
+  // and the part formed one row ahead, off the critical path,
+  //   a0_i = c_i + W1_i * y_{i-2} (+ boundary-row blocks).
+  // Record rows are read from LDS two rows ahead of use into three rotating
+  // register sets (W2 parts and W1+c parts separately).  The newest two
+  // values live in yA/yB, broadcast to every lane, with roles alternating.
+  cx<T> yA[5], yB[5];
+  cx<T> w2s[3][5], w1s[3][5], cs[3];
+  // rows kept for the boundary-row blocks: y_{n-5}, y_{n-4} (forward),
+  // x_3, x_4 (backward), as broadcast values
+  cx<T> s0[5], s1[5];
+  auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
+    cx<T> t0 = cmul(w[0], yn[0]), t1 = cmul(w[1], yn[1]), t2 = cmul(w[2], yn[2]),
+          t3 = cmul(w[3], yn[3]), t4 = cfma(w[4], yn[4], a0);
+    return (t0 + t1) + ((t2 + t3) + t4);
+  };
+  auto pre0 = [&](const cx<T> (&w)[5], cx<T> c, const cx<T> (&yo)[5]) -> cx<T> {
+    cx<T> a = cfma(w[0], yo[0], c), b = cmul(w[1], yo[1]);
+    a = cfma(w[2], yo[2], a);
+    b = cfma(w[3], yo[3], b);
+    a = cfma(w[4], yo[4], a);
+    return a + b;
+  };
+  // blocks of the boundary rows outside the two-row band
+  auto bnd = [&](int i, bool fwd_pass) -> cx<T> {
+    cx<T> b{0, 0};
     if (fwd_pass) {
       if (i >= n - 2) {      // rows n-2, n-1: blocks in columns n-5, n-4
         const cx<T> *Xa = tl + (i == n - 2 ? 0 : 25) + m * 5;
 #pragma unroll
-        for (int q = 0; q < 5; q++) a0 = cfma(Xa[q], saved[0][q], a0);
+        for (int q = 0; q < 5; q++) b = cfma(Xa[q], s0[q], b);
         if (i == n - 1)
 #pragma unroll
-          for (int q = 0; q < 5; q++) a0 = cfma(tl[50 + m * 5 + q], saved[1][q], a0);
+          for (int q = 0; q < 5; q++) b = cfma(tl[50 + m * 5 + q], s1[q], b);
       }
     } else if (i <= 1) {     // rows 1, 0: blocks in columns 3, 4
       const cx<T> *Qa = tl + (i == 1 ? 125 : 100) + m * 5;
 #pragma unroll
-      for (int q = 0; q < 5; q++) a0 = cfma(Qa[q], saved[1][q], a0);
+      for (int q = 0; q < 5; q++) b = cfma(Qa[q], s1[q], b);
       if (i == 0)
 #pragma unroll
-        for (int q = 0; q < 5; q++) a0 = cfma(tl[75 + m * 5 + q], saved[0][q], a0);
+        for (int q = 0; q < 5; q++) b = cfma(tl[75 + m * 5 + q], s0[q], b);
     }
-    cx<T> acc = (a0 + p1) + p0;
-    if (lane < 5) X[(long)i * rs + lane] = make_double2(acc.re, acc.im);
-    if (lane < 5) {
-      if (fwd_pass && (i == n - 5 || i == n - 4)) saved[i - (n - 5)][lane] = acc;
-      if (!fwd_pass && (i == 3 || i == 4)) saved[i - 3][lane] = acc;
-    }
-    row_bcast5(acc, ym2);
+    return b;
   };
-  auto lds_row = [&](const cx<T> *B, int stride, int row, int kk, cx<T> (&w)[10], cx<T> &c) {
-    const cx<T> *p = B + row * stride + m * 10;
+  auto keep = [&](const cx<T> (&y)[5], int i, bool fwd_pass) {
+    if (i == (fwd_pass ? n - 5 : 3))
 #pragma unroll
-    for (int q = 0; q < 10; q++) w[q] = p[q];
-    c = cb[kk][row][m];
+      for (int q = 0; q < 5; q++) s0[q] = y[q];
+    if (i == (fwd_pass ? n - 4 : 4))
+#pragma unroll
+      for (int q = 0; q < 5; q++) s1[q] = y[q];
   };
-  cx<T> wA[10], wB[10], cA, cB;
+  // one chunk: rows of buffer kk in processing order j = 0..nr-1 (row in
+  // the buffer = j forward, nr-1-j backward; grid row = ibase +/- j)
+  // one chunk: rows of buffer kk in processing order j = 0..nr-1 (row in
+  // the buffer = j forward, nr-1-j backward; grid row = ibase +/- j).  Rows
+  // are processed in branch-free groups of six; the remainder and the last
+  // five rows of a pass (which carry the boundary-row blocks) take a plain
+  // one-row-at-a-time path.
+  auto run_chunk = [&](const cx<T> *B, int stride, int kk, int nr, int ibase, bool fwd_pass) {
+    auto rix = [&](int j) { j = min(j, nr - 1); return fwd_pass ? j : nr - 1 - j; };
+    auto gi = [&](int j) { return fwd_pass ? ibase + j : ibase - j; };
+    auto ldW2 = [&](int j, cx<T> (&w)[5]) {
+      const cx<T> *p = B + rix(j) * stride + m * 10;
+#pragma unroll
+      for (int q = 0; q < 5; q++) w[q] = p[q];
+    };
+    auto ldW1 = [&](int j, cx<T> (&w)[5], cx<T> &c) {
+      const int r = rix(j);
+      const cx<T> *p = B + r * stride + m * 10 + 5;
+#pragma unroll
+      for (int q = 0; q < 5; q++) w[q] = p[q];
+      c = cb[kk][g][r][m];
+    };
+    int before_tail = fwd_pass ? n - 5 - ibase : ibase - 4;
+    before_tail = max(0, min(nr, before_tail));
+    const int nmain = (before_tail / 6) * 6;
+    double2 *xp = X + (long)gi(0) * rs + m;
+    const long xstep = fwd_pass ? rs : -rs;
+    if (nmain > 0) {
+      ldW2(0, w2s[0]);
+      ldW2(1, w2s[1]);
+      ldW1(0, w1s[0], cs[0]);
+      ldW1(1, w1s[1], cs[1]);
+      ldW1(2, w1s[2], cs[2]);
+      cx<T> a0 = pre0(w1s[0], cs[0], yA);
+      auto sub = [&](int j, auto U) {
+        constexpr int u = decltype(U)::value;
+        constexpr int u0 = u % 3, u1 = (u + 1) % 3, u2 = (u + 2) % 3;
+        cx<T>(&yn)[5] = (u % 2 == 0) ? yB : yA;      // newest value, y_{i-1}
+        cx<T>(&yd)[5] = (u % 2 == 0) ? yA : yB;      // receives y_i
+        cx<T> y = crit(w2s[u0], yn, a0);
+        a0 = pre0(w1s[u1], cs[u1], yn);
+        *xp = make_double2(y.re, y.im);    // lanes of one component store the same value
+        xp += xstep;
+        ldW2(j + 2, w2s[u2]);
+        ldW1(j + 3, w1s[u0], cs[u0]);
+        row_bcast5(y, yd);
+      };
+      for (int j = 0; j < nmain; j += 6) {
+        sub(j, std::integral_constant<int, 0>());
+        sub(j + 1, std::integral_constant<int, 1>());
+        sub(j + 2, std::integral_constant<int, 2>());
+        sub(j + 3, std::integral_constant<int, 3>());
+        sub(j + 4, std::integral_constant<int, 4>());
+        sub(j + 5, std::integral_constant<int, 5>());
+      }
+    }
+    for (int j = nmain; j < nr; j++) {
+      const int i = gi(j);
+      cx<T> w1[5], w2[5], c;
+      ldW1(j, w1, c);
+      ldW2(j, w2);
+      cx<T> y = crit(w2, yB, pre0(w1, c, yA) + bnd(i, fwd_pass));
+      *xp = make_double2(y.re, y.im);
+      xp += xstep;
+      row_bcast5(y, yA);                 // y_{i-2} is dead
+#pragma unroll
+      for (int q = 0; q < 5; q++) { cx<T> t = yA[q]; yA[q] = yB[q]; yB[q] = t; }
+      keep(yB, i, fwd_pass);
+    }
+  };
+
+  // ---------------- forward: y_i = c_i + N1 y_{i-2} + N2 y_{i-1} ----------
+#pragma unroll
+  for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
+  if (wave != 0) load_fwd(0);
+  __syncthreads();
   for (int k = 0; k < nchunk; k++) {
     if (wave == 0) {
       int r0 = k * R, nr = min(R, n - r0);
-      const cx<T> *B = rec[k & 1];
-      lds_row(B, 60, 0, k & 1, wA, cA);
-      for (int row = 0; row < nr; row += 2) {
-        // prefetches are unconditional (row index clamped into the chunk)
-        lds_row(B, 60, min(row + 1, nr - 1), k & 1, wB, cB);
-        row_step(wA, cA, y2, y1, r0 + row, true);             // -> y2 holds y_i
-        lds_row(B, 60, min(row + 2, nr - 1), k & 1, wA, cA);
-        if (row + 1 < nr) row_step(wB, cB, y1, y2, r0 + row + 1, true);
-      }
-      if (nr & 1) {          // odd tail: restore (older, newer) order
-#pragma unroll
-        for (int q = 0; q < 5; q++) { cx<T> t = y1[q]; y1[q] = y2[q]; y2[q] = t; }
-      }
+      run_chunk(rec[k & 1][g], 60, k & 1, nr, r0, true);
     } else if (k + 1 < nchunk) {
       load_fwd(k + 1);
     }
@@ -432,13 +515,18 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
   // chunk k holds rows rlo..rtop in ascending order in LDS
   auto load_bwd = [&](int k) {
     int rtop = n - 1 - k * R, nr = min(R, rtop + 1), rlo = rtop - nr + 1;
-    const cx<T> *Bw = bwd + ((long)line * n + rlo) * 66;
-    copy_rows_to_lds(Bw, (total_rows - ((long)line * n + rlo)) * 66, rec[k & 1], nr * 66, wave - 1, 3,
-                     lane);
-    for (int t = tid - 64; t < nr * 5; t += 192) {
-      int row = t / 5, mm = t - row * 5;     // row counted from rlo
-      const double2 *y = X + (long)(rlo + row) * rs;
-      const cx<T> *Ui = Bw + (long)row * 66 + 50 + mm * 5 - mm * (mm - 1) / 2;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+      const long ln = line_of(gg);
+      copy_rows_to_lds(bwd + (ln * n + rlo) * 66, (total_rows - (ln * n + rlo)) * 66, rec[k & 1][gg],
+                       nr * 66, wave - 1, 3, lane);
+    }
+    for (int t = tid - 64; t < G * nr * 5; t += 192) {
+      int gg = t / (nr * 5), rem = t - gg * nr * 5;
+      int row = rem / 5, mm = rem - row * 5;     // row counted from rlo
+      const long ln = line_of(gg);
+      const double2 *y = rhs + ln * sL * 5 + (long)(rlo + row) * rs;
+      const cx<T> *Ui = bwd + (ln * n + rlo + row) * 66 + 50 + mm * 5 - mm * (mm - 1) / 2;
       cx<T> acc{0, 0};
 #pragma unroll
       for (int q = 0; q < 5; q++)
@@ -446,30 +534,18 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, const cx<T> *__restrict
           double2 yq = y[mm + q];
           acc = cfma(Ui[q], cx<T>{(T)yq.x, (T)yq.y}, acc);
         }
-      cb[k & 1][row][mm] = acc;
+      cb[k & 1][gg][row][mm] = acc;
     }
     __builtin_amdgcn_s_waitcnt(0);
   };
 #pragma unroll
-  for (int q = 0; q < 5; q++) { y1[q] = cx<T>{0, 0}; y2[q] = cx<T>{0, 0}; }
+  for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
   if (wave != 0) load_bwd(0);
   __syncthreads();
   for (int k = 0; k < nchunk; k++) {
     if (wave == 0) {
-      int rtop = n - 1 - k * R, nr = min(R, rtop + 1), rlo = rtop - nr + 1;
-      const cx<T> *B = rec[k & 1];
-      lds_row(B, 66, nr - 1, k & 1, wA, cA);
-      for (int j = 0; j < nr; j += 2) {       // j counts rows from the top of the chunk
-        int row = nr - 1 - j;
-        lds_row(B, 66, max(row - 1, 0), k & 1, wB, cB);
-        row_step(wA, cA, y2, y1, rlo + row, false);
-        lds_row(B, 66, max(row - 2, 0), k & 1, wA, cA);
-        if (j + 1 < nr) row_step(wB, cB, y1, y2, rlo + row - 1, false);
-      }
-      if (nr & 1) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) { cx<T> t = y1[q]; y1[q] = y2[q]; y2[q] = t; }
-      }
+      int rtop = n - 1 - k * R, nr = min(R, rtop + 1);
+      run_chunk(rec[k & 1][g], 66, k & 1, nr, rtop, false);
     } else if (k + 1 < nchunk) {
       load_bwd(k + 1);
     }
@@ -580,18 +656,20 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
                        (const cx<double> *)pl->lu, (const cx<double> *)pl->per,
                        (const cx<double> *)pl->per2, (double2 *)rhs, sL, sR);
   } else {
-    long tot = (long)pl->n * pl->L;
-    bool many = pl->L >= 512;    // several lines per CU: smaller LDS ring
+    // many lines: four lines per workgroup (one per DPP row of the compute
+    // wave); few lines: one line per workgroup with a deeper LDS ring
+    const bool many = pl->L >= 512;
+    const unsigned g4 = (unsigned)((pl->L + 3) / 4), g1 = (unsigned)pl->L;
     if (pl->prec) {
       auto *f = (const cx<float> *)pl->fwd; auto *b = (const cx<float> *)pl->bwd;
       auto *t = (const cx<float> *)pl->tail;
-      if (many) hipLaunchKernelGGL((k_solve_bc<float, 28>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
-      else hipLaunchKernelGGL((k_solve_bc<float, 56>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
+      if (many) hipLaunchKernelGGL((k_solve_bc<float, 12, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      else hipLaunchKernelGGL((k_solve_bc<float, 54, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
     } else {
       auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
       auto *t = (const cx<double> *)pl->tail;
-      if (many) hipLaunchKernelGGL((k_solve_bc<double, 14>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
-      else hipLaunchKernelGGL((k_solve_bc<double, 28>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, f, b, t, (double2 *)rhs, sL, sR, tot);
+      if (many) hipLaunchKernelGGL((k_solve_bc<double, 6, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      else hipLaunchKernelGGL((k_solve_bc<double, 30, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
     }
   }
   LNS3D_CHECK(hipGetLastError());

@@ -31,8 +31,15 @@ def _host(t, shape):
     return t.cpu().numpy().reshape(shape, order="F")
 
 
+# sizes cover: the shortest lines; lines whose last rows (the boundary-row
+# blocks) fall in one chunk or straddle two chunks of the streaming solve;
+# many lines (four per workgroup) with a line count that is not a multiple
+# of four
+BC_SIZES = [(1, 7), (3, 8), (5, 16), (65, 129), (2, 33), (3, 35), (513, 14), (515, 17), (516, 9)]
+
+
 @pytest.mark.parametrize("layout", [1, 2])
-@pytest.mark.parametrize("nsys,n", [(1, 7), (3, 8), (5, 16), (65, 129)])
+@pytest.mark.parametrize("nsys,n", BC_SIZES)
 def test_bc_random(torch_cuda, orc, layout, nsys, n):
     from lns3d_amd.native import Penta
     from tests.penta_cases import random_lines, relerr
@@ -70,4 +77,24 @@ def test_periodic_random(torch_cuda, orc, layout, nsys, np_, exact):
     h.factor(dm)
     h.solve(dr)
     assert relerr(_host(dr, rhs.shape), x_ref) < TOL
+    h.close()
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("nsys,n", [(3, 35), (515, 17)])
+def test_bc_complex_float_factors(torch_cuda, orc, layout, nsys, n):
+    """complex-float records (the refinement path's preconditioner): one
+    solve agrees with the complex-double oracle to single-precision accuracy
+    (1e-5 relative for these well-conditioned lines)"""
+    from lns3d_amd.native import Penta
+    from tests.penta_cases import random_lines, relerr
+    print(f"bc-fp32 layout={layout} nsys={nsys} n={n}", flush=True)
+    mat, rhs = random_lines(nsys, n, layout, seed=31 * nsys + n)
+    m_ref, x_ref = mat.copy(order="F"), rhs.copy(order="F")
+    orc.cpenta_bc(layout, m_ref, x_ref, 2)
+    h = Penta(0, layout, nsys, n, 1)
+    dm, dr = _dev(torch_cuda, mat), _dev(torch_cuda, rhs)
+    h.factor(dm)
+    h.solve(dr)
+    assert relerr(_host(dr, rhs.shape), x_ref) < 1e-5
     h.close()
