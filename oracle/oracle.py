@@ -75,6 +75,33 @@ class Oracle:
                                    _ptr(per2), code, exact)
         assert rc == 0
 
+    def cgrad(self, p, v):
+        """first derivatives of a complex (ndof, nx, ny) field (ndof = 5)"""
+        g1, g2 = np.zeros_like(v, order="F"), np.zeros_like(v, order="F")
+        ps = self.params(p)
+        self.lib.orc_cgrad(ctypes.byref(ps), _ptr(v), _ptr(g1), _ptr(g2))
+        return g1, g2
+
+    def cgrad2(self, p, v, g1):
+        g11, g12, g22 = (np.zeros_like(v, order="F") for _ in range(3))
+        ps = self.params(p)
+        self.lib.orc_cgrad2(ctypes.byref(ps), _ptr(v), _ptr(g1), _ptr(g11), _ptr(g12), _ptr(g22))
+        return g11, g12, g22
+
+    def grad(self, p, v):
+        """first derivatives of a real (ndof, nx, ny) field"""
+        g1, g2 = np.zeros_like(v, order="F"), np.zeros_like(v, order="F")
+        ps = self.params(p)
+        self.lib.orc_grad(ctypes.byref(ps), v.shape[0], _ptr(v), _ptr(g1), _ptr(g2))
+        return g1, g2
+
+    def grad2(self, p, v, g1):
+        g11, g12, g22 = (np.zeros_like(v, order="F") for _ in range(3))
+        ps = self.params(p)
+        self.lib.orc_grad2(ctypes.byref(ps), v.shape[0], _ptr(v), _ptr(g1), _ptr(g11), _ptr(g12),
+                           _ptr(g22))
+        return g11, g12, g22
+
     def genmtrx(self, p, vm, met):
         c = coeff_arrays(p.nx, p.ny)
         cs = self._coeffs(c)

@@ -1,0 +1,134 @@
+"""CPU-side checks of the product library and its host layer (no GPU calls):
+the C-ABI library loads and exports every function include/lns3d_amd.h
+declares; the ctypes signature table covers the header; the parameter
+struct has the header's layout; host-side input generation; the benchmark's
+refusal of more than one GPU; the F3 finding against the compiled reference
+where that was built."""
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lns3d_amd.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    return sorted(set(re.findall(r"\b(lns3d_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from lns3d_amd import native
+    if not os.path.exists(native.LIB_PATH):
+        from lns3d_amd.build import build
+        build()
+    return ctypes.CDLL(native.LIB_PATH)
+
+
+def test_library_exports_header(lib):
+    names = header_functions()
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_signature_table_covers_header():
+    from lns3d_amd.native import SIGNATURES
+    names = header_functions()
+    assert sorted(SIGNATURES) == names
+
+
+def test_error_query_without_gpu(lib):
+    lib.lns3d_last_error.restype = ctypes.c_char_p
+    assert isinstance(lib.lns3d_last_error(), bytes)
+
+
+def test_params_struct_layout():
+    """CParams mirrors lns3d_params: 14 ints, 9 doubles, 2 ints"""
+    from lns3d_amd.native import CParams
+    names = [f[0] for f in CParams._fields_]
+    src = open(HEADER).read()
+    body = re.search(r"typedef struct\s*{(.*?)}\s*lns3d_params;", src, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    decl = []
+    for typ, rest in re.findall(r"\b(int|double)\s+([^;]+);", body):
+        decl += [(typ, v.strip()) for v in rest.split(",")]
+    assert [d[1] for d in decl] == names
+    for (typ, name), (fname, ftype) in zip(decl, CParams._fields_):
+        assert (ftype is ctypes.c_int) == (typ == "int"), name
+
+
+def test_grid_and_mean():
+    from lns3d_amd.config import Params, make_grid, make_mean, make_disturbance
+    p = Params(nx=33, ny=17)
+    x, y, met = make_grid(p)
+    assert x.shape == (33, 17) and met.shape == (33, 17, 10)
+    assert x.flags["F_CONTIGUOUS"] and met.flags["F_CONTIGUOUS"]
+    assert y[0, 0] == 0.0 and abs(y[0, -1] - p.ymax) < 1e-12
+    assert np.all(np.diff(y[0]) > 0)
+    # the analytic metric n2 = d(eta)/dy against a centred difference on a
+    # fine grid
+    pf = Params(nx=9, ny=513)
+    _, yf, metf = make_grid(pf)
+    deta = 1.0 / (pf.ny - 1)
+    fd = 2 * deta / (yf[0, 2:] - yf[0, :-2])
+    assert np.abs(fd / metf[0, 1:-1, 3] - 1).max() < 2e-3
+    vm = make_mean(p, y)
+    assert np.allclose(vm[0] * vm[4], 1.0)
+    assert vm[1][0, 0] == 0.0
+    v = make_disturbance(p, x, seed=1)
+    assert v.shape == (5, 33, 17) and np.iscomplexobj(v)
+    assert np.array_equal(v, make_disturbance(p, x, seed=1))
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
+
+
+def test_native_fails_loudly_without_library(tmp_path, monkeypatch):
+    from lns3d_amd import native
+    monkeypatch.setattr(native, "_lib", None)
+    monkeypatch.setattr(native, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(native.LNS3DError):
+        native.lib()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref.so")),
+                    reason="compiled reference not built (reference sources absent)")
+def test_f3_shipped_index_slip():
+    """cpenta1bc as shipped (line 675 uses the wrong row index in the forward
+    sweep of the last row) does not solve the system; with that one index
+    corrected it agrees with a dense solve, and so does the oracle"""
+    from oracle.oracle import Oracle, Reference
+    from tests.penta_cases import random_lines, relerr, dense_bc
+    R, O = Reference(), Oracle()
+    nsys, n = 2, 10
+    mat, rhs = random_lines(nsys, n, 1, seed=77)
+    xs = []
+    for shipped in (False, True):
+        m = mat.copy(order="F")
+        # the shipped routine writes past the rhs array: give it room
+        buf = np.zeros(rhs.size * 8, complex)
+        buf[:rhs.size] = rhs.reshape(-1, order="F")
+        R._call("cpenta1bc_shipped_" if shipped else "cpenta1bc_", nsys, n, 5, m, buf, 2)
+        xs.append(buf[:rhs.size].reshape(rhs.shape, order="F"))
+    mo, xo = mat.copy(order="F"), rhs.copy(order="F")
+    O.cpenta_bc(1, mo, xo, 2)
+    for s in range(nsys):
+        xd = np.linalg.solve(dense_bc(mat[:, :, :, s, :]), rhs[:, s, :].T.reshape(-1)).reshape(n, 5).T
+        assert relerr(xs[0][:, s, :], xd) < 1e-13
+        assert relerr(xo[:, s, :], xd) < 1e-13
+        assert relerr(xs[1][:, s, :], xd) > 1e-6

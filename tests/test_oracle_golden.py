@@ -1,0 +1,136 @@
+"""The oracle (C restatement) against the golden vectors that the reference's
+own routines wrote (tests/golden/gen_golden.py).  These pin, function by
+function: genmtrx (src/genmtrx.f90), set_time_step (src/temporal.f90),
+lrhs3D with cgrad/cgrad2/cgradbc (src/lrhs3d.f90), rhsBC3D
+(src/rhsbc3d.f90), itrBC3D (src/itrbc3d.f90), the assembled LHS factors
+(src/lhs1f3d.f90, lhs2f3d.f90 and their closures), cpenta1bc/cpenta2bc (with
+the index fix at cpenta1bc.f:675 / cpenta2bc.f:676), cpenta1p/cpenta2p (as
+shipped) and the whole implicit step (src/impdrv3d.f90:146-356).
+
+Tolerances: the coefficient, time-step, residual and assembly routines
+perform the same operations in the same order and match to 1e-14 relative
+(max norm); the line solves and steps reorder nothing either but go through
+a different compiler's contraction choices, so 1e-12."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STEP_CASES = ["base", "wall1", "wall2", "xper"]
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def params_of(z):
+    from lns3d_amd.config import Params
+    d = json.loads(str(z["params"]))
+    return Params(**d)
+
+
+@pytest.fixture(scope="module", params=STEP_CASES)
+def case(request, orc):
+    z = load(f"ref_step_{request.param}.npz")
+    p = params_of(z)
+    met = np.asfortranarray(z["met"])
+    vm = np.asfortranarray(z["vm"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    return request.param, z, p, c, dtl
+
+
+def test_genmtrx(case):
+    name, z, p, c, _ = case
+    for k in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22"):
+        assert rel(c[k], z["c_" + k]) < 1e-14, (name, k)
+    # ABhi(1:4) are the entries the step reads (8(a) a16)
+    assert rel(c["ABhi"][:4], z["c_ABhi"][:4]) < 1e-14, name
+
+
+def test_set_time_step(case):
+    name, z, p, _, dtl = case
+    assert rel(dtl, z["dtl"]) < 1e-14, name
+
+
+def test_lrhs3d_and_bcs(case, orc):
+    name, z, p, c, _ = case
+    v0 = np.asfortranarray(z["v0"])
+    r, _ = orc.lrhs3d(p, c, v0)
+    assert rel(r, z["lrhs"]) < 1e-14, name
+    rb = np.asfortranarray(r.copy())
+    orc.rhsbc3d(p, rb, v0)
+    assert rel(rb, z["rhsbc"]) < 1e-14, name
+    v = np.asfortranarray(v0.copy())
+    orc.itrbc3d(p, v)
+    assert rel(v, z["itrbc"]) < 1e-14, name
+
+
+def test_lhs(case, orc):
+    name, z, p, c, dtl = case
+    if "lhs1" not in z.files:
+        pytest.skip("assembly kept for the base and periodic cases only")
+    for d in (1, 2):
+        m = orc.lhs(p, d, c, dtl)
+        assert rel(m, z[f"lhs{d}"]) < 1e-14, (name, d)
+
+
+def test_two_steps(case, orc):
+    name, z, p, c, dtl = case
+    v = np.asfortranarray(z["itrbc"].copy())
+    f = orc.factor(p, c, dtl)
+    dv1 = orc.step(p, c, dtl, f, v)
+    assert rel(dv1, z["dv1"]) < 1e-12, name
+    dv2 = orc.step(p, c, dtl, f, v)
+    assert rel(dv2, z["dv2"]) < 1e-12, name
+    assert rel(v, z["v2"]) < 1e-12, name
+
+
+def test_config0_step(orc):
+    """the 129x65 benchmark deck: two steps against the reference's
+    increments (every 4th point) and norms"""
+    from lns3d_amd.config import make_grid, make_mean, make_disturbance
+    z = load("ref_config0.npz")
+    p = params_of(z)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    v = make_disturbance(p, x, seed=int(z["seed"]))
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    orc.itrbc3d(p, v)
+    f = orc.factor(p, c, dtl)
+    dv1 = orc.step(p, c, dtl, f, v)
+    dv2 = orc.step(p, c, dtl, f, v)
+    assert rel(dv1[:, ::4, ::4], z["dv1_sub"]) < 1e-12
+    assert rel(dv2[:, ::4, ::4], z["dv2_sub"]) < 1e-12
+    n = z["norms"]
+    assert abs(np.abs(dv1).max() - n[0]) <= 1e-12 * n[0]
+    assert abs(np.abs(dv2).max() - n[1]) <= 1e-12 * n[1]
+    assert abs(np.sqrt((np.abs(dv1) ** 2).sum()) - n[2]) <= 1e-12 * n[2]
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+def test_penta_bc_golden(orc, layout):
+    z = load("ref_penta.npz")
+    m, x = np.asfortranarray(z[f"bc{layout}_mat"].copy()), np.asfortranarray(z[f"bc{layout}_rhs"].copy())
+    orc.cpenta_bc(layout, m, x, 2)
+    assert rel(x, z[f"bc{layout}_x"]) < 1e-12
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+def test_penta_periodic_golden(orc, layout):
+    """the periodic solver as shipped (exact=0 reproduces its placement of
+    the row n-5 -> row n-3 update, cpenta1p.f:284/345)"""
+    z = load("ref_penta.npz")
+    m, x = np.asfortranarray(z[f"p{layout}_mat"].copy()), np.asfortranarray(z[f"p{layout}_rhs"].copy())
+    shp = (2, 5, 5) + m.shape[3:]
+    per = np.zeros(shp, complex, order="F")
+    per2 = np.zeros(shp, complex, order="F")
+    orc.cpenta_p(layout, m, x, per, per2, 2, exact=0)
+    assert rel(x, z[f"p{layout}_x"]) < 1e-12
