@@ -145,6 +145,16 @@ int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int 
 /* the step with complex-float factors and no inner refinement */
 int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev);
 
+/* Host-array driving of the time loop of src/impdrv3d.f90:126-406 (for
+ * callers that hold v in host memory, such as the Fortran host layer):
+ * upload v (5,nx,ny) once, advance nsteps steps on the device (the first
+ * advance after an upload applies itrBC3D first, as impdrv3d.f90:126 does),
+ * download v when the caller needs it.  dv_host (may be NULL) receives the
+ * last increment r, the array impdrv3d.f90:340-348 reads for its norms. */
+int lns3d_state_upload(lns3d_ctx *c, const void *v_host);
+int lns3d_state_download(lns3d_ctx *c, void *v_host);
+int lns3d_advance(lns3d_ctx *c, int nsteps, void *dv_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,8 @@ struct lns3d_ctx {
   double *scal = nullptr;    // device scalars
   double *coef_block = nullptr;
   bool coeffs_ready = false, factored = false;
+  void *vstate = nullptr;    // 5P complex: the state of the host-array driver
+  bool state_fresh = false;  // uploaded, boundary conditions not yet applied
   int last_iters = 0;
   double last_resid = 0.0;
 };
@@ -100,6 +102,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->t2) (void)hipFree(c->t2);
   if (c->t3) (void)hipFree(c->t3);
   if (c->scal) (void)hipFree(c->scal);
+  if (c->vstate) (void)hipFree(c->vstate);
   delete c;
   return 0;
 }
@@ -320,6 +323,45 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
   if (!rc) rc = launch_update_bc(c->p, v_dev, r, c->stream);
   return rc;
+}
+
+// Host-array driving of the time loop (src/impdrv3d.f90:126-406): the
+// context keeps the state v in HBM between calls.  The boundary conditions
+// are applied once to a freshly uploaded state before its first step, as
+// src/impdrv3d.f90:126 does before the loop.
+extern "C" int lns3d_state_upload(lns3d_ctx *c, const void *v_host) {
+  if (!c || !v_host) return fail(LNS3D_EINVAL, "lns3d_state_upload: null argument");
+  const size_t bytes = 5ul * c->p.nx * c->p.ny * 16;
+  if (!c->vstate) LNS3D_CHECK(hipMalloc(&c->vstate, bytes));
+  LNS3D_CHECK(hipMemcpyAsync(c->vstate, v_host, bytes, hipMemcpyHostToDevice, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  c->state_fresh = true;
+  return 0;
+}
+
+extern "C" int lns3d_state_download(lns3d_ctx *c, void *v_host) {
+  if (!c || !v_host || !c->vstate) return fail(LNS3D_EINVAL, "lns3d_state_download: no state");
+  const size_t bytes = 5ul * c->p.nx * c->p.ny * 16;
+  LNS3D_CHECK(hipMemcpyAsync(v_host, c->vstate, bytes, hipMemcpyDeviceToHost, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int lns3d_advance(lns3d_ctx *c, int nsteps, void *dv_host) {
+  if (!c || !c->vstate || nsteps < 0) return fail(LNS3D_EINVAL, "lns3d_advance: no state");
+  if (!c->factored) return fail(LNS3D_ENOTFACTORED, "lns3d_advance: call lns3d_factor first");
+  int rc = 0;
+  if (c->state_fresh) {
+    rc = launch_itrbc(c->p, c->vstate, c->stream);
+    c->state_fresh = false;
+  }
+  for (int k = 0; k < nsteps && !rc; k++) rc = lns3d_step(c, c->vstate, c->r);
+  if (rc) return rc;
+  if (dv_host && nsteps > 0)
+    LNS3D_CHECK(hipMemcpyAsync(dv_host, c->r, 5ul * c->p.nx * c->p.ny * 16, hipMemcpyDeviceToHost,
+                               c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 extern "C" int lns3d_ctx_synchronize(lns3d_ctx *c) {

@@ -1,0 +1,231 @@
+!-----------------------------------------------------------------------------
+! Fortran host layer of lns3d_amd (built into liblns3d_fortran.so, linked
+! against liblns3d_amd.so).
+!
+! 1. Drop-in externals with the reference's names and argument lists, so a
+!    Fortran driver links them in place of src/cpenta1bc.f, cpenta2bc.f,
+!    cpenta1p.f and cpenta2p.f:
+!       call cpenta1bc(nsys, n, nblk, mat, rhs, code)      ! cpenta1bc.f:2
+!       call cpenta2bc(n, nsys, nblk, mat, rhs, code)      ! cpenta2bc.f:2
+!       call cpenta1p (nsys, np, nblk, mat, rhs, per, per2, code)
+!       call cpenta2p (np, nsys, nblk, mat, rhs, per, per2, code)
+!    Arguments keep the reference's meaning: mat and rhs in its column-major
+!    layouts, code 0 factor / 1 solve / 2 both.  The factors stay on the GPU
+!    (keyed by the address of mat and the shape), so mat is not overwritten;
+!    per and per2 are not written.  The reference reports no errors; the
+!    status of the last call is kept for lns3d_hip_status().
+! 2. Module lns3d_hip: iso_c_binding interfaces to the C ABI
+!    (include/lns3d_amd.h) and a driver of the implicit step with host
+!    arrays (the time loop of src/impdrv3d.f90:126-406):
+!       call lns3d_hip_init(prm, metric, vm)   ! coefficients, dtl, factors
+!       call lns3d_hip_put(v)                  ! state to the device
+!       call lns3d_hip_advance(nsteps, dv)     ! steps on the device
+!       call lns3d_hip_get(v)                  ! state back
+!       call lns3d_hip_free()
+!-----------------------------------------------------------------------------
+module lns3d_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  ! mirror of lns3d_params (include/lns3d_amd.h)
+  type, bind(C), public :: lns3d_params
+    integer(c_int) :: nx, ny, optx, opty, xper, yper, wall, wallt, top, left, right
+    integer(c_int) :: linear, mattyp, loctime
+    real(c_double) :: Ma, Re, Pr, gamma, kz, omega, cflmax, dtmax, alfa
+    integer(c_int) :: periodic_exact, precision
+  end type lns3d_params
+
+  integer(c_int), public :: lns3d_hip_last = 0
+  type(c_ptr), save :: ctx = c_null_ptr
+
+  public :: lns3d_hip_status, lns3d_hip_message, lns3d_hip_record
+  public :: lns3d_hip_init, lns3d_hip_put, lns3d_hip_get, lns3d_hip_advance, lns3d_hip_free
+  public :: c_cpenta1bc, c_cpenta2bc, c_cpenta1p, c_cpenta2p, c_set_periodic_exact
+
+  interface
+    function c_last_error() bind(C, name="lns3d_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    integer(c_int) function c_cpenta1bc(nsys, n, nblk, mat, rhs, code) bind(C, name="lns3d_cpenta1bc")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nsys, n, nblk, code
+      type(c_ptr), value :: mat, rhs
+    end function
+    integer(c_int) function c_cpenta2bc(n, nsys, nblk, mat, rhs, code) bind(C, name="lns3d_cpenta2bc")
+      import :: c_int, c_ptr
+      integer(c_int), value :: n, nsys, nblk, code
+      type(c_ptr), value :: mat, rhs
+    end function
+    integer(c_int) function c_cpenta1p(nsys, np, nblk, mat, rhs, per, per2, code) &
+        bind(C, name="lns3d_cpenta1p")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nsys, np, nblk, code
+      type(c_ptr), value :: mat, rhs, per, per2
+    end function
+    integer(c_int) function c_cpenta2p(np, nsys, nblk, mat, rhs, per, per2, code) &
+        bind(C, name="lns3d_cpenta2p")
+      import :: c_int, c_ptr
+      integer(c_int), value :: np, nsys, nblk, code
+      type(c_ptr), value :: mat, rhs, per, per2
+    end function
+    integer(c_int) function c_set_periodic_exact(exact) bind(C, name="lns3d_set_periodic_exact")
+      import :: c_int
+      integer(c_int), value :: exact
+    end function
+    integer(c_int) function c_ctx_create(p, out) bind(C, name="lns3d_ctx_create")
+      import :: c_int, c_ptr, lns3d_params
+      type(lns3d_params), intent(in) :: p
+      type(c_ptr), intent(out) :: out
+    end function
+    integer(c_int) function c_ctx_destroy(c) bind(C, name="lns3d_ctx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+    end function
+    integer(c_int) function c_setup(c, metric, vm) bind(C, name="lns3d_setup")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, metric, vm
+    end function
+    integer(c_int) function c_factor(c) bind(C, name="lns3d_factor")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+    end function
+    integer(c_int) function c_state_upload(c, v) bind(C, name="lns3d_state_upload")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, v
+    end function
+    integer(c_int) function c_state_download(c, v) bind(C, name="lns3d_state_download")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, v
+    end function
+    integer(c_int) function c_advance(c, nsteps, dv) bind(C, name="lns3d_advance")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, dv
+      integer(c_int), value :: nsteps
+    end function
+    function c_strlen(s) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function
+  end interface
+
+contains
+
+  subroutine lns3d_hip_record(rc)
+    integer(c_int), intent(in) :: rc
+    lns3d_hip_last = rc
+  end subroutine
+
+  ! status of the last call (0: success; the codes of include/lns3d_amd.h)
+  integer function lns3d_hip_status()
+    lns3d_hip_status = int(lns3d_hip_last)
+  end function
+
+  ! message of the last failure
+  function lns3d_hip_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = c_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [max(n, 1)])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = chars(i)
+    end do
+  end function
+
+  ! metric(nx,ny,10) and vm(5,nx,ny) as src/gengrid.f90 and genmean.f90
+  ! hold them; generates the coefficients and dtl on the device and factors
+  ! both directions (src/impdrv3d.f90:248-269, 310-319)
+  subroutine lns3d_hip_init(prm, metric, vm)
+    type(lns3d_params), intent(in) :: prm
+    real(c_double), intent(in), target :: metric(*), vm(*)
+    integer(c_int) :: rc
+    if (c_associated(ctx)) call lns3d_hip_free()
+    rc = c_ctx_create(prm, ctx)
+    if (rc == 0) rc = c_setup(ctx, c_loc(metric), c_loc(vm))
+    if (rc == 0) rc = c_factor(ctx)
+    call lns3d_hip_record(rc)
+  end subroutine
+
+  subroutine lns3d_hip_put(v)
+    complex(c_double_complex), intent(in), target :: v(*)
+    call lns3d_hip_record(c_state_upload(ctx, c_loc(v)))
+  end subroutine
+
+  subroutine lns3d_hip_get(v)
+    complex(c_double_complex), intent(inout), target :: v(*)
+    call lns3d_hip_record(c_state_download(ctx, c_loc(v)))
+  end subroutine
+
+  subroutine lns3d_hip_advance(nsteps, dv)
+    integer, intent(in) :: nsteps
+    complex(c_double_complex), intent(inout), target :: dv(*)
+    call lns3d_hip_record(c_advance(ctx, int(nsteps, c_int), c_loc(dv)))
+  end subroutine
+
+  subroutine lns3d_hip_free()
+    integer(c_int) :: rc
+    if (c_associated(ctx)) then
+      rc = c_ctx_destroy(ctx)
+      ctx = c_null_ptr
+    end if
+  end subroutine
+
+end module lns3d_hip
+
+!-----------------------------------------------------------------------------
+! drop-in externals (external names cpenta1bc_ ... under flang and gfortran)
+!-----------------------------------------------------------------------------
+subroutine cpenta1bc(nsys, n, nblk, mat, rhs, code)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  integer :: nsys, n, nblk, code
+  complex(c_double_complex), target :: mat(*), rhs(*)
+  call lns3d_hip_record(c_cpenta1bc(int(nsys, c_int), int(n, c_int), int(nblk, c_int), &
+                                    c_loc(mat), c_loc(rhs), int(code, c_int)))
+end subroutine
+
+subroutine cpenta2bc(n, nsys, nblk, mat, rhs, code)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  integer :: n, nsys, nblk, code
+  complex(c_double_complex), target :: mat(*), rhs(*)
+  call lns3d_hip_record(c_cpenta2bc(int(n, c_int), int(nsys, c_int), int(nblk, c_int), &
+                                    c_loc(mat), c_loc(rhs), int(code, c_int)))
+end subroutine
+
+subroutine cpenta1p(nsys, np, nblk, mat, rhs, per, per2, code)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  integer :: nsys, np, nblk, code
+  complex(c_double_complex), target :: mat(*), rhs(*), per(*), per2(*)
+  call lns3d_hip_record(c_cpenta1p(int(nsys, c_int), int(np, c_int), int(nblk, c_int), &
+                                   c_loc(mat), c_loc(rhs), c_loc(per), c_loc(per2), &
+                                   int(code, c_int)))
+end subroutine
+
+subroutine cpenta2p(np, nsys, nblk, mat, rhs, per, per2, code)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  integer :: np, nsys, nblk, code
+  complex(c_double_complex), target :: mat(*), rhs(*), per(*), per2(*)
+  call lns3d_hip_record(c_cpenta2p(int(np, c_int), int(nsys, c_int), int(nblk, c_int), &
+                                   c_loc(mat), c_loc(rhs), c_loc(per), c_loc(per2), &
+                                   int(code, c_int)))
+end subroutine
+
+! status of the last drop-in call, callable from fixed-form code
+integer function lns3d_hip_status_f()
+  use lns3d_hip
+  implicit none
+  lns3d_hip_status_f = lns3d_hip_status()
+end function

@@ -34,6 +34,9 @@ SIGNATURES = {
     "lns3d_ctx_destroy": ([VP], CI),
     "lns3d_ctx_set_stream": ([VP, VP], CI),
     "lns3d_ctx_synchronize": ([VP], CI),
+    "lns3d_state_upload": ([VP, VP], CI),
+    "lns3d_state_download": ([VP, VP], CI),
+    "lns3d_advance": ([VP, CI, VP], CI),
     "lns3d_setup": ([VP, VP, VP], CI),
     "lns3d_set_coeffs": ([VP] * 10, CI),
     "lns3d_get_coeffs": ([VP] * 10, CI),
