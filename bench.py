@@ -41,7 +41,46 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="launch 5 xi solves, 5 eta solves, 5 steps and a calibration copy of "
+                         "one solve's algorithmic bytes, then exit (for rocprofv3 --pmc runs)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_solve_xi_1025x257.json")
+
+
+def measured_traffic():
+    """HBM bytes per xi-solve launch from the committed rocprofv3 --pmc runs
+    (FETCH_SIZE + WRITE_SIZE of k_solve_bc, each scaled by the counter's
+    reading for a copy of known size); None when that file is absent"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def kernels_only(st, P):
+    import torch
+    r = torch.randn(P * 5, dtype=torch.complex128, device="cuda")
+    for which in ("x", "y"):
+        pen = _dir_solver(st, which)
+        for _ in range(5):
+            pen(r)
+    v = r.clone()
+    dv = torch.zeros_like(v)
+    st.itrbc3d(v)
+    for _ in range(5):
+        st.step(v, dv)
+    # calibration: a plain device copy of one xi solve's algorithmic bytes
+    nb = SOLVE_BYTES_PT[0] * P
+    a = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_only": True, "copy_bytes": nb}), flush=True)
 
 
 def main():
@@ -69,6 +108,9 @@ def main():
     st = Step(p, precision=0 if args.no_refine else 1)
     st.setup(met, vm)
     st.factor()
+    if args.kernels_only:
+        kernels_only(st, P)
+        return
     v = dev(v0)
     st.itrbc3d(v)
     dv = torch.zeros_like(v)
@@ -122,7 +164,7 @@ def main():
                    "nx": NX, "ny": NY, "beta": BETA, "Ma": p.Ma, "Re": p.Re, "cfl": p.cflmax},
         "roofline": {"bound": "hbm", "kernel": "k_solve_bc (xi lines: 257 lines x 1025 rows)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "bytes_per_launch": bytes_launch,
                      "avg_launch_ms": round(sol_ms["xi"], 4),
                      "eta_solve_ms": round(sol_ms["eta"], 4)},
