@@ -47,6 +47,8 @@ struct LineGeom {
   int periodic;
   int exact;
   long L;        // lines (interleave stride)
+  int h = -1;    // twisted factor (non-periodic): rows 0..h+1 top-down after
+                 // rows n-1..h+2 bottom-up; -1: rows in order 0..n-1
 };
 
 __device__ __forceinline__ int bc_slot(int n, int i, int c) {
@@ -86,13 +88,28 @@ __device__ __forceinline__ bool blk(const LineGeom &g, cx<T> *mat, cx<T> *per, c
   return false;
 }
 
-__device__ __forceinline__ int partners(const LineGeom &g, int i, int *cand) {
+// rows (and columns) that a pivot on row i eliminates from / updates: the
+// not yet pivoted neighbours in the pivot direction (dir +1: below, -1: above)
+__device__ __forceinline__ int partners(const LineGeom &g, int i, int dir, int *cand) {
   int k = 0;
-  for (int d = 1; d <= 4 && i + d < g.n; d++) cand[k++] = i + d;
+  const int hi = (g.h >= 0 && dir > 0) ? g.h + 1 : g.n - 1;   // twisted top phase stops at h+1
+  for (int d = 1; d <= 4; d++) {
+    int t = i + dir * d;
+    if (t >= 0 && t <= hi) cand[k++] = t;
+  }
   if (g.periodic)
     for (int e = g.n - 2; e < g.n; e++)
       if (e > i + 4) cand[k++] = e;
   return k;
+}
+
+// pivot order: row of elimination step s and its direction
+__device__ __forceinline__ int pivot_row(const LineGeom &g, int s, int *dir) {
+  if (g.h < 0) { *dir = 1; return s; }
+  const int nb = g.n - g.h - 2;                  // bottom rows n-1 .. h+2
+  if (s < nb) { *dir = -1; return g.n - 1 - s; }
+  *dir = 1;
+  return s - nb;
 }
 
 // ---------------------------------------------------------------------------
@@ -120,11 +137,13 @@ __global__ void k_lu_lines(LineGeom g, cx<T> *mat, cx<T> *per, cx<T> *per2) {
     }
   }
   int cand[8];
-  for (int i = 0; i < n; i++) {
+  for (int s = 0; s < n; s++) {
+    int dir;
+    const int i = pivot_row(g, s, &dir);
     Blk<T> D, U[6];
     int ucol[6], nu = 0;
     blk(g, mat, per, per2, line, i, i, D);
-    int nc_ = partners(g, i, cand);
+    int nc_ = partners(g, i, dir, cand);
     for (int a = 0; a < nc_; a++)
       if (blk(g, mat, per, per2, line, i, cand[a], U[nu])) ucol[nu++] = cand[a];
     for (int nc = 0; nc < 4; nc++)
@@ -180,7 +199,7 @@ __global__ void k_solve_lines(LineGeom g, const cx<T> *mat_, const cx<T> *per_, 
     for (int nc = 0; nc < 4; nc++)
       for (int l = nc + 1; l < 5; l++) x[l] = cfma(D(l, nc), x[nc], x[l]);
     for (int d = 0; d < 5; d++) X(i, d) = make_double2(x[d].re, x[d].im);
-    int nc_ = partners(g, i, cand);
+    int nc_ = partners(g, i, 1, cand);
     for (int a = 0; a < nc_; a++) {
       Blk<T> Lb;
       int t = cand[a];
@@ -197,7 +216,7 @@ __global__ void k_solve_lines(LineGeom g, const cx<T> *mat_, const cx<T> *per_, 
     blk(g, mat, per, per2, line, i, i, D);
     cx<T> acc[5];
     for (int d = 0; d < 5; d++) acc[d] = cx<T>{(T)X(i, d).x, (T)X(i, d).y};
-    int nc_ = partners(g, i, cand);
+    int nc_ = partners(g, i, 1, cand);
     for (int a = 0; a < nc_; a++) {
       Blk<T> U;
       int c = cand[a];
@@ -232,62 +251,84 @@ __global__ void k_records_bc(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR>
   if (line >= g.L) return;
   cx<TF> *mat = const_cast<cx<TF> *>(mat_);
   const int n = g.n;
-  for (int i = 0; i < n; i++) {
-    Blk<TF> D;
-    blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, i, D);
-    // Linv: product of the in-block elimination steps applied to I
-    cx<TF> Li[5][5], Ui[5][5];
-    for (int r = 0; r < 5; r++)
-      for (int c = 0; c < 5; c++) Li[r][c] = cx<TF>{TF(r == c), 0};
-    for (int nc = 0; nc < 4; nc++)
-      for (int l = nc + 1; l < 5; l++)
-        for (int c = 0; c <= nc; c++) Li[l][c] = cfma(D(l, nc), Li[nc][c], Li[l][c]);
-    // Uinv by back substitution on identity columns
-    for (int r = 0; r < 5; r++)
-      for (int c = 0; c < 5; c++) Ui[r][c] = cx<TF>{0, 0};
-    for (int c = 0; c < 5; c++)
-      for (int m = c; m >= 0; m--) {
-        cx<TF> acc = cx<TF>{TF(m == c), 0};
-        for (int q = m + 1; q <= c; q++) acc = cfma(-D(m, q), Ui[q][c], acc);
-        Ui[m][c] = cdiv(acc, D(m, m));
+  const bool tw = g.h >= 0;
+  const int nchain = tw ? 2 : 1;
+  for (int t = 0; t < nchain; t++) {
+    // chain t: rows in elimination order; t = 1 runs bottom-up (row n-1-j)
+    const int len = tw ? (t == 0 ? g.h + 2 : n - g.h - 2) : n;
+    const int dir = t == 0 ? 1 : -1;
+    const int off = t == 0 ? 0 : g.h + 2;
+    cx<TR> *Tl = tail + (line * nchain + t) * 150;
+    for (int j = 0; j < len; j++) {
+      const int i = t == 0 ? j : n - 1 - j;
+      Blk<TF> D;
+      blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, i, D);
+      // Linv: product of the in-block elimination steps applied to I
+      cx<TF> Li[5][5], Ui[5][5];
+      for (int r = 0; r < 5; r++)
+        for (int c = 0; c < 5; c++) Li[r][c] = cx<TF>{TF(r == c), 0};
+      for (int nc = 0; nc < 4; nc++)
+        for (int l = nc + 1; l < 5; l++)
+          for (int c = 0; c <= nc; c++) Li[l][c] = cfma(D(l, nc), Li[nc][c], Li[l][c]);
+      // Uinv by back substitution on identity columns
+      for (int r = 0; r < 5; r++)
+        for (int c = 0; c < 5; c++) Ui[r][c] = cx<TF>{0, 0};
+      for (int c = 0; c < 5; c++)
+        for (int m = c; m >= 0; m--) {
+          cx<TF> acc = cx<TF>{TF(m == c), 0};
+          for (int q = m + 1; q <= c; q++) acc = cfma(-D(m, q), Ui[q][c], acc);
+          Ui[m][c] = cdiv(acc, D(m, m));
+        }
+      cx<TR> *F = fwd + ((long)line * n + off + j) * 60;
+      cx<TR> *B = bwd + ((long)line * n + off + j) * 66;
+      auto Nblk = [&](int col, cx<TR> *dst, int ld) {   // Linv * M(i,col) (a multiplier block)
+        Blk<TF> M;
+        bool ok = col >= 0 && col < n && blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, M);
+        for (int m = 0; m < 5; m++)
+          for (int q = 0; q < 5; q++) {
+            cx<TF> acc{0, 0};
+            if (ok)
+              for (int k = 0; k <= m; k++) acc = cfma(Li[m][k], M(k, q), acc);
+            dst[m * ld + q] = cx<TR>{(TR)acc.re, (TR)acc.im};
+          }
+      };
+      auto Qblk = [&](int col, cx<TR> *dst, int ld) {   // -Uinv * U(i,col), col pivoted after i
+        Blk<TF> U;
+        bool later = !(tw && t == 0 && col >= g.h + 2);
+        bool ok = later && col >= 0 && col < n &&
+                  blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, U);
+        for (int m = 0; m < 5; m++)
+          for (int q = 0; q < 5; q++) {
+            cx<TF> acc{0, 0};
+            if (ok)
+              for (int k = m; k < 5; k++) acc = cfma(Ui[m][k], U(k, q), acc);
+            dst[m * ld + q] = cx<TR>{(TR)-acc.re, (TR)-acc.im};
+          }
+      };
+      Nblk(i - dir, F, 10);
+      Nblk(i - 2 * dir, F + 5, 10);
+      for (int l = 1, k = 0; l < 5; l++)
+        for (int q = 0; q < l; q++, k++) F[50 + k] = cx<TR>{(TR)Li[l][q].re, (TR)Li[l][q].im};
+      Qblk(i + dir, B, 10);
+      Qblk(i + 2 * dir, B + 5, 10);
+      for (int m = 0, k = 0; m < 5; m++)
+        for (int q = m; q < 5; q++, k++) B[50 + k] = cx<TR>{(TR)Ui[m][q].re, (TR)Ui[m][q].im};
+      B[65] = cx<TR>{0, 0};
+      // forward blocks of the chain's last two rows outside the band: columns
+      // n-5, n-4 (plain) or the bottom chain's last rows h+2, h+3 (twisted)
+      if (tw && t == 1 && j == 0)
+        for (int e = 0; e < 75; e++) Tl[e] = cx<TR>{0, 0};
+      if (!(tw && t == 1)) {
+        const int x0 = tw ? g.h + 2 : n - 5, x1 = tw ? g.h + 3 : n - 4;
+        if (j == len - 2) Nblk(x0, Tl + 0, 5);
+        if (j == len - 1) { Nblk(x0, Tl + 25, 5); Nblk(x1, Tl + 50, 5); }
       }
-    cx<TR> *F = fwd + ((long)line * n + i) * 60;
-    cx<TR> *B = bwd + ((long)line * n + i) * 66;
-    cx<TR> *Tl = tail + line * 150;
-    auto Nblk = [&](int col, cx<TR> *dst, int ld) {   // Linv * M(i,col)
-      Blk<TF> M;
-      bool ok = blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, M);
-      for (int m = 0; m < 5; m++)
-        for (int q = 0; q < 5; q++) {
-          cx<TF> acc{0, 0};
-          if (ok)
-            for (int k = 0; k <= m; k++) acc = cfma(Li[m][k], M(k, q), acc);
-          dst[m * ld + q] = cx<TR>{(TR)acc.re, (TR)acc.im};
-        }
-    };
-    auto Qblk = [&](int col, cx<TR> *dst, int ld) {   // -Uinv * U(i,col)
-      Blk<TF> U;
-      bool ok = col < n && blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, U);
-      for (int m = 0; m < 5; m++)
-        for (int q = 0; q < 5; q++) {
-          cx<TF> acc{0, 0};
-          if (ok)
-            for (int k = m; k < 5; k++) acc = cfma(Ui[m][k], U(k, q), acc);
-          dst[m * ld + q] = cx<TR>{(TR)-acc.re, (TR)-acc.im};
-        }
-    };
-    Nblk(i - 1, F, 10);
-    Nblk(i - 2, F + 5, 10);
-    for (int l = 1, k = 0; l < 5; l++)
-      for (int q = 0; q < l; q++, k++) F[50 + k] = cx<TR>{(TR)Li[l][q].re, (TR)Li[l][q].im};
-    Qblk(i + 1, B, 10);
-    Qblk(i + 2, B + 5, 10);
-    for (int m = 0, k = 0; m < 5; m++)
-      for (int q = m; q < 5; q++, k++) B[50 + k] = cx<TR>{(TR)Ui[m][q].re, (TR)Ui[m][q].im};
-    if (i == n - 2) Nblk(n - 5, Tl + 0, 5);
-    if (i == n - 1) { Nblk(n - 5, Tl + 25, 5); Nblk(n - 4, Tl + 50, 5); }
-    if (i == 0) { Qblk(3, Tl + 75, 5); Qblk(4, Tl + 100, 5); }
-    if (i == 1) Qblk(4, Tl + 125, 5);
+      // backward blocks of the chain's first two rows: columns 3, 4 counted
+      // along the chain
+      const int c3 = t == 0 ? 3 : n - 4, c4 = t == 0 ? 4 : n - 5;
+      if (j == 0) { Qblk(c3, Tl + 75, 5); Qblk(c4, Tl + 100, 5); }
+      if (j == 1) Qblk(c4, Tl + 125, 5);
+    }
   }
 }
 
@@ -554,6 +595,253 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
 }
 
 // ---------------------------------------------------------------------------
+// stage 3, twisted lines: the factor eliminated rows n-1..h+2 bottom-up and
+// then rows 0..h+1 top-down, so each line is two recurrences of about n/2
+// rows ("chains"): chain 0 runs rows 0..h+1, chain 1 rows n-1..h+2.  They
+// run in lockstep on two DPP rows of the compute wave (G lines per
+// workgroup, 2G chains).  Forward: chain 1 finishes first and hands its last
+// two values (rows h+2, h+3) to chain 0, whose last two rows carry those
+// columns.  Backward: chain 0 starts at row h+1; chain 1 starts two steps
+// later from chain 0's x_{h+1}, x_h.
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
+                                                 const cx<T> *__restrict__ bwd,
+                                                 const cx<T> *__restrict__ tail, double2 *rhs,
+                                                 long sL, long sR) {
+  constexpr int C = 2 * G;
+  constexpr int RECW = 66;
+  constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
+  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][C][BUFE];
+  __shared__ cx<T> cb[2][C][R][5];
+  __shared__ cx<T> xch[G][2][5];     // values handed between the chains of a line
+  __shared__ cx<T> kept[C][2][5];    // x at chain rows 3, 4 (backward boundary blocks)
+  const long total_rows = L * n;
+  const long line0 = (long)blockIdx.x * G;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int ch = (lane >> 4) % C;    // chain of this DPP row (rows past 2G repeat)
+  const int g = ch >> 1, t = ch & 1;
+  const int m = min(lane & 15, 4);
+  auto line_of = [&](int gg) { return min(line0 + gg, L - 1); };
+  const long line = line_of(g);
+  const int len0 = h + 2, len1 = n - h - 2;          // len0 >= len1 + 2
+  auto clen = [&](int tt) { return tt == 0 ? len0 : len1; };
+  auto coff = [&](int tt) { return tt == 0 ? 0 : len0; };
+  auto real_row = [&](int tt, int j) { return tt == 0 ? j : n - 1 - j; };
+  const int len = clen(t);
+  const long rs = sR * 5;
+  double2 *X = rhs + line * sL * 5;
+  const cx<T> *tl = tail + (line * 2 + t) * 150;
+  const int nchunk = (len0 + R - 1) / R;             // steps per pass: len0
+  // chain row at step s of the backward pass
+  auto bj = [&](int tt, int s) { return tt == 0 ? len0 - 1 - s : len1 + 1 - s; };
+  // rows of chunk k held in the LDS buffer for chain tt: [jlo, jlo + nr)
+  auto chunk_rows = [&](int tt, int k, bool fw, int &jlo, int &nr) {
+    const int l = clen(tt);
+    if (fw) {
+      jlo = k * R;
+      nr = max(0, min(R, l - jlo));
+    } else {
+      int jhi = min(l - 1, bj(tt, k * R));
+      jlo = max(0, bj(tt, k * R + R - 1));
+      nr = max(0, jhi - jlo + 1);
+    }
+  };
+
+  auto load = [&](int k, bool fw) {
+    const int stride = fw ? 60 : 66;
+    const cx<T> *base = fw ? fwd : bwd;
+#pragma unroll
+    for (int cc = 0; cc < C; cc++) {
+      int jlo, nr;
+      chunk_rows(cc & 1, k, fw, jlo, nr);
+      if (nr == 0) continue;
+      const long r0 = line_of(cc >> 1) * n + coff(cc & 1) + jlo;
+      copy_rows_to_lds(base + r0 * stride, (total_rows - r0) * stride, rec[k & 1][cc], nr * stride,
+                       wave - 1, 3, lane);
+    }
+    for (int task = tid - 64; task < C * R * 5; task += 192) {
+      const int cc = task / (R * 5), rem = task - cc * R * 5;
+      const int row = rem / 5, mm = rem - row * 5;
+      int jlo, nr;
+      chunk_rows(cc & 1, k, fw, jlo, nr);
+      if (row >= nr) continue;
+      const int j = jlo + row;
+      const long ln = line_of(cc >> 1);
+      const double2 *xr = rhs + ln * sL * 5 + (long)real_row(cc & 1, j) * rs;
+      const cx<T> *recp = base + (ln * n + coff(cc & 1) + j) * stride;
+      cx<T> acc{0, 0};
+      if (fw) {                      // c = Linv b
+        const cx<T> *Li = recp + 50 + mm * (mm - 1) / 2;
+        double2 bm = xr[mm];
+        acc = cx<T>{(T)bm.x, (T)bm.y};
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q < mm) {
+            double2 bq = xr[q];
+            acc = cfma(Li[q], cx<T>{(T)bq.x, (T)bq.y}, acc);
+          }
+      } else {                       // d = Uinv y
+        const cx<T> *Ui = recp + 50 + mm * 5 - mm * (mm - 1) / 2;
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+          if (q < 5 - mm) {
+            double2 yq = xr[mm + q];
+            acc = cfma(Ui[q], cx<T>{(T)yq.x, (T)yq.y}, acc);
+          }
+      }
+      cb[k & 1][cc][row][mm] = acc;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  };
+
+  // ---- compute wave (same recurrence as k_solve_bc, per chain) ----
+  cx<T> yA[5], yB[5];
+  cx<T> w2s[3][5], w1s[3][5], cs[3];
+  cx<T> s0[5], s1[5];
+  auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
+    cx<T> t0 = cmul(w[0], yn[0]), t1 = cmul(w[1], yn[1]), t2 = cmul(w[2], yn[2]),
+          t3 = cmul(w[3], yn[3]), t4 = cfma(w[4], yn[4], a0);
+    return (t0 + t1) + ((t2 + t3) + t4);
+  };
+  auto pre0 = [&](const cx<T> (&w)[5], cx<T> c, const cx<T> (&yo)[5]) -> cx<T> {
+    cx<T> a = cfma(w[0], yo[0], c), b = cmul(w[1], yo[1]);
+    a = cfma(w[2], yo[2], a);
+    b = cfma(w[3], yo[3], b);
+    a = cfma(w[4], yo[4], a);
+    return a + b;
+  };
+  auto run_pass = [&](bool fw) {
+    const int stride = fw ? 60 : 66;
+    // steps where every chain takes a plain row: [main_lo, main_hi)
+    const int main_lo = fw ? 0 : 3;
+    const int main_hi = fw ? len1 - 2 : min(len0 - 5, len1 - 3);
+#pragma unroll
+    for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
+    if (wave != 0) load(0, fw);
+    __syncthreads();
+    for (int k = 0; k < nchunk; k++) {
+      if (wave == 0) {
+        const int kk = k & 1;
+        const cx<T> *B = rec[kk][ch];
+        int jlo, nr;
+        chunk_rows(t, k, fw, jlo, nr);
+        const int s_begin = k * R, s_end = min(k * R + R, len0);
+        // buffer row of step s (clamped into the chunk's rows)
+        auto brow = [&](int s) {
+          int b = fw ? s - s_begin : bj(t, s) - jlo;
+          return max(0, min(max(nr, 1) - 1, b));
+        };
+        auto ldW2 = [&](int s, cx<T> (&w)[5]) {
+          const cx<T> *pp = B + brow(s) * stride + m * 10;
+#pragma unroll
+          for (int q = 0; q < 5; q++) w[q] = pp[q];
+        };
+        auto ldW1 = [&](int s, cx<T> (&w)[5], cx<T> &c) {
+          const int r = brow(s);
+          const cx<T> *pp = B + r * stride + m * 10 + 5;
+#pragma unroll
+          for (int q = 0; q < 5; q++) w[q] = pp[q];
+          c = cb[kk][ch][r][m];
+        };
+        auto jrow = [&](int s) { return fw ? s : bj(t, s); };
+        // one step on the plain path: hand-offs, boundary blocks, inactive chains
+        auto plain = [&](int s) {
+          const int j = jrow(s);
+          const bool act = j >= 0 && j < len;
+          if (!fw && t == 1 && s == 2) {            // chain 1 starts from x_h, x_{h+1}
+#pragma unroll
+            for (int q = 0; q < 5; q++) { yA[q] = xch[g][0][q]; yB[q] = xch[g][1][q]; }
+          }
+          if (fw && t == 0 && j == len0 - 2) {      // columns h+2, h+3 from chain 1
+#pragma unroll
+            for (int q = 0; q < 5; q++) { s0[q] = xch[g][0][q]; s1[q] = xch[g][1][q]; }
+          }
+          if (!fw && j == 1) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) { s0[q] = kept[ch][0][q]; s1[q] = kept[ch][1][q]; }
+          }
+          cx<T> w1[5], w2[5], c;
+          ldW1(s, w1, c);
+          ldW2(s, w2);
+          cx<T> b{0, 0};
+          if (fw) {
+            if (t == 0 && j >= len0 - 2) {
+              const cx<T> *Xa = tl + (j == len0 - 2 ? 0 : 25) + m * 5;
+#pragma unroll
+              for (int q = 0; q < 5; q++) b = cfma(Xa[q], s0[q], b);
+              if (j == len0 - 1)
+#pragma unroll
+                for (int q = 0; q < 5; q++) b = cfma(tl[50 + m * 5 + q], s1[q], b);
+            }
+          } else if (j >= 0 && j <= 1) {
+            const cx<T> *Qa = tl + (j == 1 ? 125 : 100) + m * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) b = cfma(Qa[q], s1[q], b);
+            if (j == 0)
+#pragma unroll
+              for (int q = 0; q < 5; q++) b = cfma(tl[75 + m * 5 + q], s0[q], b);
+          }
+          cx<T> y = crit(w2, yB, pre0(w1, c, yA) + b);
+          if (act) X[(long)real_row(t, j) * rs + m] = make_double2(y.re, y.im);
+          row_bcast5(y, yA);
+#pragma unroll
+          for (int q = 0; q < 5; q++) { cx<T> tt = yA[q]; yA[q] = yB[q]; yB[q] = tt; }
+          if (act) {
+            if (fw && t == 1 && j == len1 - 1) xch[g][0][m] = y;     // z_{h+2}
+            if (fw && t == 1 && j == len1 - 2) xch[g][1][m] = y;     // z_{h+3}
+            if (!fw && t == 0 && j == len0 - 1) xch[g][1][m] = y;    // x_{h+1}
+            if (!fw && t == 0 && j == len0 - 2) xch[g][0][m] = y;    // x_h
+            if (!fw && (j == 3 || j == 4)) kept[ch][j - 3][m] = y;
+          }
+        };
+        const int p_end = max(s_begin, min(main_lo, s_end));
+        for (int s = s_begin; s < p_end; s++) plain(s);
+        const int m_end = p_end + ((max(p_end, min(main_hi, s_end)) - p_end) / 6) * 6;
+        if (m_end > p_end) {
+          int j0 = jrow(p_end);
+          double2 *xp = X + (long)real_row(t, j0) * rs + m;
+          const long xstep = ((t == 0) == fw) ? rs : -rs;
+          ldW2(p_end, w2s[0]);
+          ldW2(p_end + 1, w2s[1]);
+          ldW1(p_end, w1s[0], cs[0]);
+          ldW1(p_end + 1, w1s[1], cs[1]);
+          ldW1(p_end + 2, w1s[2], cs[2]);
+          cx<T> a0 = pre0(w1s[0], cs[0], yA);
+          auto sub = [&](int s, auto U) {
+            constexpr int u = decltype(U)::value;
+            constexpr int u0 = u % 3, u1 = (u + 1) % 3, u2 = (u + 2) % 3;
+            cx<T>(&yn)[5] = (u % 2 == 0) ? yB : yA;
+            cx<T>(&yd)[5] = (u % 2 == 0) ? yA : yB;
+            cx<T> y = crit(w2s[u0], yn, a0);
+            a0 = pre0(w1s[u1], cs[u1], yn);
+            *xp = make_double2(y.re, y.im);
+            xp += xstep;
+            ldW2(s + 2, w2s[u2]);
+            ldW1(s + 3, w1s[u0], cs[u0]);
+            row_bcast5(y, yd);
+          };
+          for (int s = p_end; s < m_end; s += 6) {
+            sub(s, std::integral_constant<int, 0>());
+            sub(s + 1, std::integral_constant<int, 1>());
+            sub(s + 2, std::integral_constant<int, 2>());
+            sub(s + 3, std::integral_constant<int, 3>());
+            sub(s + 4, std::integral_constant<int, 4>());
+            sub(s + 5, std::integral_constant<int, 5>());
+          }
+        }
+        for (int s = m_end; s < s_end; s++) plain(s);
+      } else if (k + 1 < nchunk) {
+        load(k + 1, fw);
+      }
+      __syncthreads();
+    }
+  };
+  run_pass(true);
+  run_pass(false);
+}
+
+// ---------------------------------------------------------------------------
 // layout conversions between the reference's arrays and the working copy
 // ref mat (5,5,5,...) element (slot k, r, c) of (line l, row i) at
 //   ((i*sRm + l*sLm) * 125 + k + 5r + 25c)
@@ -581,6 +869,9 @@ int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long l
   pl->L = lines;
   pl->prec = prec;
   if (pl->n < 7 || lines < 1) return set_error(__FILE__, __LINE__, "line length must be >= 7 unknowns");
+  // few long lines: twisted factor (two chains per line); many lines keep
+  // the plain order (four lines per workgroup already fill the chip)
+  pl->h = (!periodic && pl->n >= 16 && lines < 512) ? (pl->n - 1) / 2 : -1;
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
@@ -592,7 +883,7 @@ int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long l
     size_t es = prec ? sizeof(cx<float>) : sizeof(cx<double>);
     LNS3D_CHECK(hipMalloc(&pl->fwd, (size_t)pl->n * 60 * lines * es));
     LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * es));
-    LNS3D_CHECK(hipMalloc(&pl->tail, (size_t)150 * lines * es));
+    LNS3D_CHECK(hipMalloc(&pl->tail, (size_t)150 * lines * (pl->h >= 0 ? 2 : 1) * es));
   }
   return 0;
 }
@@ -616,7 +907,7 @@ int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStre
 }
 
 int penta_factor(PentaPlan *pl, hipStream_t s) {
-  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
+  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
   hipLaunchKernelGGL(k_lu_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
                      (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2);
   LNS3D_CHECK(hipGetLastError());
@@ -636,7 +927,7 @@ int penta_factor(PentaPlan *pl, hipStream_t s) {
 
 int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
   if (pl->periodic) return set_error(__FILE__, __LINE__, "records are for non-periodic lines");
-  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
+  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
   if (pl->prec)
     hipLaunchKernelGGL((k_records_bc<double, float>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
                        (const cx<double> *)pl->lu, (cx<float> *)pl->fwd, (cx<float> *)pl->bwd,
@@ -656,19 +947,20 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
                        (const cx<double> *)pl->lu, (const cx<double> *)pl->per,
                        (const cx<double> *)pl->per2, (double2 *)rhs, sL, sR);
   } else {
-    // many lines: four lines per workgroup (one per DPP row of the compute
-    // wave); few lines: one line per workgroup with a deeper LDS ring
-    const bool many = pl->L >= 512;
+    // twisted lines: one line (two chains) per workgroup; many lines: four
+    // lines per workgroup (one per DPP row of the compute wave)
     const unsigned g4 = (unsigned)((pl->L + 3) / 4), g1 = (unsigned)pl->L;
     if (pl->prec) {
       auto *f = (const cx<float> *)pl->fwd; auto *b = (const cx<float> *)pl->bwd;
       auto *t = (const cx<float> *)pl->tail;
-      if (many) hipLaunchKernelGGL((k_solve_bc<float, 12, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      if (pl->h >= 0) hipLaunchKernelGGL((k_solve_tw<float, 24, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      else if (pl->L >= 512) hipLaunchKernelGGL((k_solve_bc<float, 12, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
       else hipLaunchKernelGGL((k_solve_bc<float, 54, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
     } else {
       auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
       auto *t = (const cx<double> *)pl->tail;
-      if (many) hipLaunchKernelGGL((k_solve_bc<double, 6, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      if (pl->h >= 0) hipLaunchKernelGGL((k_solve_tw<double, 12, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      else if (pl->L >= 512) hipLaunchKernelGGL((k_solve_bc<double, 6, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
       else hipLaunchKernelGGL((k_solve_bc<double, 30, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
     }
   }
@@ -677,6 +969,8 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
 }
 
 int penta_solve_reference_order(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
+  if (pl->h >= 0)
+    return set_error(__FILE__, __LINE__, "reference-order solve needs factors in row order (this plan is twisted)");
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
   hipLaunchKernelGGL(k_solve_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
                      (const cx<double> *)pl->lu, (const cx<double> *)pl->per,

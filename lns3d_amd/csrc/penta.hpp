@@ -12,6 +12,7 @@ namespace lns3d {
 struct PentaPlan {
   int periodic = 0, exact = 0;
   int n = 0, np = 0;   // unknown rows, stored rows (np = n+1 when periodic)
+  int h = -1;          // twisted factor split (rows 0..h+1 | h+2..n-1), -1: plain
   long L = 0;          // number of lines
   int prec = 0;
   void *lu = nullptr, *per = nullptr, *per2 = nullptr;
