@@ -4,14 +4,16 @@
 // wrap-around storage) and src/cpenta1p.f, src/cpenta2p.f (periodic,
 // bordered).  Three stages:
 //
-//  1. k_lu_lines: non-pivoted block LU of every line, one lane per line, on
-//     a line-interleaved working copy ([row][slot][r][c][line]) so a wave's
-//     accesses to one block element are contiguous across its 64 lines.  The
-//     elimination order and the storage of negated multipliers follow the
-//     reference (cpenta1bc.f:43-495, cpenta1p.f:46-654), including the
-//     periodic factor's slot-4 placement at cpenta1p.f:284/345 unless exact
-//     is requested.
-//  2. k_records_bc: for non-periodic lines, turn row i's factors into the
+//  1. Non-pivoted block LU.  Non-periodic lines: k_factor_chains, one wave
+//     per chain of rows with the rows being updated in an LDS window (lines
+//     of 16+ rows, when there are few lines, use a twisted order: two chains
+//     per line meeting in the middle).  Periodic lines: k_lu_lines, one lane
+//     per line on a line-interleaved copy.  The storage of negated
+//     multipliers and the in-pivot order follow the reference
+//     (cpenta1bc.f:43-495, cpenta1p.f:46-654), including the periodic
+//     factor's slot-4 placement at cpenta1p.f:284/345 unless exact is
+//     requested.
+//  2. k_records_rows: for non-periodic lines, turn row i's factors into the
 //     records the fast solve streams:
 //        forward   y_i = Linv_i b_i + N1_i y_{i-2} + N2_i y_{i-1}
 //        backward  x_i = Uinv_i y_i + Q4_i x_{i+1} + Q5_i x_{i+2}
@@ -21,11 +23,10 @@
 //     (cpenta1bc.f:499-847); the per-row critical path becomes one 5x5
 //     matvec.  Boundary rows 0,1 and n-2,n-1 carry extra blocks, kept in a
 //     per-line tail record.
-//  3. k_solve_bc: one workgroup per line.  Waves 1-3 stream chunks of R rows
-//     of records into a double-buffered LDS ring and compute the parallel
-//     parts (Linv b, Uinv y); wave 0 runs the serial recurrence with lanes
-//     0..4 owning one component each and the previous rows' values broadcast
-//     through scalar registers.
+//  3. k_solve_tw (twisted lines) / k_solve_bc: waves 1-3 stream chunks of
+//     records into a double-buffered LDS ring and compute the parallel parts
+//     (Linv b, Uinv y); wave 0 runs the serial recurrences, one line or one
+//     chain per 16-lane DPP row, lanes 0..4 owning one component each.
 //  Periodic lines are solved by k_solve_lines (one lane per line, the
 //  reference's order) on the LU working copy.
 #include "common.hpp"
@@ -34,8 +35,10 @@
 namespace lns3d {
 
 // ---------------------------------------------------------------------------
-// block addressing in the interleaved LU working copy
-// element (r,c) of a block = p[(r*5 + c) * L]
+// block addressing in the LU working copy.  Periodic lines: line-interleaved
+// ([row][slot][r][c][line], one lane per line in k_lu_lines); non-periodic
+// lines: line-major ([line][row][slot][r][c], one wave per chain in
+// k_factor_chains).  Element (r,c) of a block = p[(r*5 + c) * ldl].
 template <typename T> struct Blk {
   cx<T> *p;
   __device__ __forceinline__ cx<T> &operator()(int r, int c) const { return p[(long)(r * 5 + c) * ldl]; }
@@ -69,7 +72,8 @@ __device__ __forceinline__ bool blk(const LineGeom &g, cx<T> *mat, cx<T> *per, c
   if (!g.periodic) {
     int k = bc_slot(n, i, c);
     if (k < 0) return false;
-    b.p = mat + ((long)(i * 5 + k) * 25) * g.L + line;
+    b.p = mat + ((line * n + i) * 5 + k) * 25;
+    b.ldl = 1;
     return true;
   }
   int d = c - i;
@@ -245,91 +249,258 @@ __global__ void k_solve_lines(LineGeom g, const cx<T> *mat_, const cx<T> *per_, 
 // tail[line*150 + ...]: 0 row n-2 N(col n-5); 25 row n-1 N(col n-5);
 //                       50 row n-1 N(col n-4); 75 row 0 Q(col 3);
 //                       100 row 0 Q(col 4); 125 row 1 Q(col 4)   ([m][q])
+// records of one (line, chain row); a separate function because inlining
+// it into the kernel trips the compiler's CFG simplification (clang 22)
 template <typename TF, typename TR>
-__global__ void k_records_bc(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR> *bwd, cx<TR> *tail) {
-  long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= g.L) return;
-  cx<TF> *mat = const_cast<cx<TF> *>(mat_);
+__device__ __noinline__ void records_row(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR> *bwd,
+                                         cx<TR> *tail, long idx) {
   const int n = g.n;
+  const long line = idx / n;
+  const int jj = (int)(idx - line * n);
+  cx<TF> *mat = const_cast<cx<TF> *>(mat_);
   const bool tw = g.h >= 0;
   const int nchain = tw ? 2 : 1;
-  for (int t = 0; t < nchain; t++) {
-    // chain t: rows in elimination order; t = 1 runs bottom-up (row n-1-j)
-    const int len = tw ? (t == 0 ? g.h + 2 : n - g.h - 2) : n;
-    const int dir = t == 0 ? 1 : -1;
-    const int off = t == 0 ? 0 : g.h + 2;
-    cx<TR> *Tl = tail + (line * nchain + t) * 150;
-    for (int j = 0; j < len; j++) {
-      const int i = t == 0 ? j : n - 1 - j;
-      Blk<TF> D;
-      blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, i, D);
-      // Linv: product of the in-block elimination steps applied to I
-      cx<TF> Li[5][5], Ui[5][5];
-      for (int r = 0; r < 5; r++)
-        for (int c = 0; c < 5; c++) Li[r][c] = cx<TF>{TF(r == c), 0};
-      for (int nc = 0; nc < 4; nc++)
-        for (int l = nc + 1; l < 5; l++)
-          for (int c = 0; c <= nc; c++) Li[l][c] = cfma(D(l, nc), Li[nc][c], Li[l][c]);
-      // Uinv by back substitution on identity columns
-      for (int r = 0; r < 5; r++)
-        for (int c = 0; c < 5; c++) Ui[r][c] = cx<TF>{0, 0};
-      for (int c = 0; c < 5; c++)
-        for (int m = c; m >= 0; m--) {
-          cx<TF> acc = cx<TF>{TF(m == c), 0};
-          for (int q = m + 1; q <= c; q++) acc = cfma(-D(m, q), Ui[q][c], acc);
-          Ui[m][c] = cdiv(acc, D(m, m));
-        }
-      cx<TR> *F = fwd + ((long)line * n + off + j) * 60;
-      cx<TR> *B = bwd + ((long)line * n + off + j) * 66;
-      auto Nblk = [&](int col, cx<TR> *dst, int ld) {   // Linv * M(i,col) (a multiplier block)
-        Blk<TF> M;
-        bool ok = col >= 0 && col < n && blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, M);
-        for (int m = 0; m < 5; m++)
-          for (int q = 0; q < 5; q++) {
-            cx<TF> acc{0, 0};
-            if (ok)
-              for (int k = 0; k <= m; k++) acc = cfma(Li[m][k], M(k, q), acc);
-            dst[m * ld + q] = cx<TR>{(TR)acc.re, (TR)acc.im};
-          }
-      };
-      auto Qblk = [&](int col, cx<TR> *dst, int ld) {   // -Uinv * U(i,col), col pivoted after i
-        Blk<TF> U;
-        bool later = !(tw && t == 0 && col >= g.h + 2);
-        bool ok = later && col >= 0 && col < n &&
-                  blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, U);
-        for (int m = 0; m < 5; m++)
-          for (int q = 0; q < 5; q++) {
-            cx<TF> acc{0, 0};
-            if (ok)
-              for (int k = m; k < 5; k++) acc = cfma(Ui[m][k], U(k, q), acc);
-            dst[m * ld + q] = cx<TR>{(TR)-acc.re, (TR)-acc.im};
-          }
-      };
-      Nblk(i - dir, F, 10);
-      Nblk(i - 2 * dir, F + 5, 10);
-      for (int l = 1, k = 0; l < 5; l++)
-        for (int q = 0; q < l; q++, k++) F[50 + k] = cx<TR>{(TR)Li[l][q].re, (TR)Li[l][q].im};
-      Qblk(i + dir, B, 10);
-      Qblk(i + 2 * dir, B + 5, 10);
-      for (int m = 0, k = 0; m < 5; m++)
-        for (int q = m; q < 5; q++, k++) B[50 + k] = cx<TR>{(TR)Ui[m][q].re, (TR)Ui[m][q].im};
-      B[65] = cx<TR>{0, 0};
-      // forward blocks of the chain's last two rows outside the band: columns
-      // n-5, n-4 (plain) or the bottom chain's last rows h+2, h+3 (twisted)
-      if (tw && t == 1 && j == 0)
-        for (int e = 0; e < 75; e++) Tl[e] = cx<TR>{0, 0};
-      if (!(tw && t == 1)) {
-        const int x0 = tw ? g.h + 2 : n - 5, x1 = tw ? g.h + 3 : n - 4;
-        if (j == len - 2) Nblk(x0, Tl + 0, 5);
-        if (j == len - 1) { Nblk(x0, Tl + 25, 5); Nblk(x1, Tl + 50, 5); }
-      }
-      // backward blocks of the chain's first two rows: columns 3, 4 counted
-      // along the chain
-      const int c3 = t == 0 ? 3 : n - 4, c4 = t == 0 ? 4 : n - 5;
-      if (j == 0) { Qblk(c3, Tl + 75, 5); Qblk(c4, Tl + 100, 5); }
-      if (j == 1) Qblk(c4, Tl + 125, 5);
+  // chain t: rows in elimination order; t = 1 runs bottom-up (row n-1-j)
+  const int t = (tw && jj >= g.h + 2) ? 1 : 0;
+  const int j = t == 0 ? jj : jj - (g.h + 2);
+  const int len = tw ? (t == 0 ? g.h + 2 : n - g.h - 2) : n;
+  const int dir = t == 0 ? 1 : -1;
+  const int i = t == 0 ? j : n - 1 - j;
+  cx<TR> *Tl = tail + (line * nchain + t) * 150;
+  Blk<TF> D;
+  blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, i, D);
+  // Linv: product of the in-block elimination steps applied to I
+  cx<TF> Li[5][5], Ui[5][5];
+  for (int r = 0; r < 5; r++)
+    for (int c = 0; c < 5; c++) Li[r][c] = cx<TF>{TF(r == c), 0};
+  for (int nc = 0; nc < 4; nc++)
+    for (int l = nc + 1; l < 5; l++)
+      for (int c = 0; c <= nc; c++) Li[l][c] = cfma(D(l, nc), Li[nc][c], Li[l][c]);
+  // Uinv by back substitution on identity columns
+  for (int r = 0; r < 5; r++)
+    for (int c = 0; c < 5; c++) Ui[r][c] = cx<TF>{0, 0};
+  for (int c = 0; c < 5; c++)
+    for (int m = c; m >= 0; m--) {
+      cx<TF> acc = cx<TF>{TF(m == c), 0};
+      for (int q = m + 1; q <= c; q++) acc = cfma(-D(m, q), Ui[q][c], acc);
+      Ui[m][c] = cdiv(acc, D(m, m));
     }
+  cx<TR> *F = fwd + ((long)line * n + jj) * 60;
+  cx<TR> *B = bwd + ((long)line * n + jj) * 66;
+  auto Nblk = [&](int col, cx<TR> *dst, int ld) {   // Linv * M(i,col) (a multiplier block)
+    Blk<TF> M;
+    bool ok = col >= 0 && col < n && blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, M);
+    for (int m = 0; m < 5; m++)
+      for (int q = 0; q < 5; q++) {
+        cx<TF> acc{0, 0};
+        if (ok)
+          for (int k = 0; k <= m; k++) acc = cfma(Li[m][k], M(k, q), acc);
+        dst[m * ld + q] = cx<TR>{(TR)acc.re, (TR)acc.im};
+      }
+  };
+  auto Qblk = [&](int col, cx<TR> *dst, int ld) {   // -Uinv * U(i,col), col pivoted after i
+    Blk<TF> U;
+    bool later = !(tw && t == 0 && col >= g.h + 2);
+    bool ok = later && col >= 0 && col < n &&
+              blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, U);
+    for (int m = 0; m < 5; m++)
+      for (int q = 0; q < 5; q++) {
+        cx<TF> acc{0, 0};
+        if (ok)
+          for (int k = m; k < 5; k++) acc = cfma(Ui[m][k], U(k, q), acc);
+        dst[m * ld + q] = cx<TR>{(TR)-acc.re, (TR)-acc.im};
+      }
+  };
+  Nblk(i - dir, F, 10);
+  Nblk(i - 2 * dir, F + 5, 10);
+  for (int l = 1, k = 0; l < 5; l++)
+    for (int q = 0; q < l; q++, k++) F[50 + k] = cx<TR>{(TR)Li[l][q].re, (TR)Li[l][q].im};
+  Qblk(i + dir, B, 10);
+  Qblk(i + 2 * dir, B + 5, 10);
+  for (int m = 0, k = 0; m < 5; m++)
+    for (int q = m; q < 5; q++, k++) B[50 + k] = cx<TR>{(TR)Ui[m][q].re, (TR)Ui[m][q].im};
+  B[65] = cx<TR>{0, 0};
+  // forward blocks of the chain's last two rows outside the band: columns
+  // n-5, n-4 (plain) or the bottom chain's last rows h+2, h+3 (twisted)
+  // (the bottom chain of a twisted factor has none: Nblk of column -1 writes
+  // zeros)
+  const bool fw_tail = !(tw && t == 1);
+  const int x0 = !fw_tail ? -1 : (tw ? g.h + 2 : n - 5), x1 = !fw_tail ? -1 : (tw ? g.h + 3 : n - 4);
+  if (j == (fw_tail ? len - 2 : 0)) Nblk(x0, Tl + 0, 5);
+  if (j == (fw_tail ? len - 1 : 0)) { Nblk(x0, Tl + 25, 5); Nblk(x1, Tl + 50, 5); }
+  // backward blocks of the chain's first two rows: columns 3, 4 counted
+  // along the chain
+  const int c3 = t == 0 ? 3 : n - 4, c4 = t == 0 ? 4 : n - 5;
+  if (j == 0) { Qblk(c3, Tl + 75, 5); Qblk(c4, Tl + 100, 5); }
+  if (j == 1) Qblk(c4, Tl + 125, 5);
+}
+
+// one thread per (line, chain row): rows are independent once factored
+template <typename TF, typename TR>
+__global__ void k_records_rows(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR> *bwd, cx<TR> *tail) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < g.L * g.n) records_row<TF, TR>(g, mat_, fwd, bwd, tail, idx);
+}
+
+// ---------------------------------------------------------------------------
+// stage 1 for non-periodic lines: block LU with one wave per chain, the
+// rows v..v+4 of the chain in an LDS window.  A plain factor is one chain
+// (rows 0..n-1).  The twisted factor runs two chains concurrently, rows
+// 0..h-1 top-down and rows n-1..h+2 bottom-up (the bottom one in reversed
+// indices, where the band looks the same); both update the 2x2 block of
+// rows/columns h, h+1, the top chain in place and the bottom chain into a
+// scratch copy that starts at zero.  Once both are done the two are summed
+// and rows h, h+1 are eliminated.  Inside a pivot the elimination follows
+// cpenta1bc.f:43-495: in-block LU with negated multipliers stored below the
+// diagonal, its row operations applied to the pivot row's right blocks, then
+// each partner row's block in the pivot column turned into multipliers and
+// the partner's blocks updated.
+constexpr int FW = 5;
+
+__global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *lu, cx<double> *mid) {
+  using T = double;
+  __shared__ cx<T> win[2][FW][125];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long line = blockIdx.x;
+  const int n = g.n, h = g.h;
+  const bool tw = h >= 0;
+  cx<T> *lul = lu + line * n * 125;
+  cx<T> *midl = mid ? mid + line * 4 * 25 : nullptr;
+  auto wsync = [] { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); };
+  // one chain on window w, chain direction tt, pivots [v0, v1), rows above
+  // lim never touched
+  auto chain = [&](int w, int tt, int v0, int v1, int lim, bool preloaded) {
+    auto real = [&](int v) { return tt == 0 ? v : n - 1 - v; };
+    auto slot = [&](int vr, int vc) {
+      if (vr < 0 || vc < 0 || vr >= n || vc >= n) return -1;
+      return bc_slot(n, real(vr), real(vc));
+    };
+    auto B = [&](int vr, int sl) -> cx<T> * { return &win[w][vr % FW][sl * 25]; };
+    auto middle = [&](int vr, int vc) {
+      int r = real(vr), c = real(vc);
+      return tw && (r == h || r == h + 1) && (c == h || c == h + 1);
+    };
+    auto load_row = [&](int vr) {
+      const cx<T> *src = lul + (long)real(vr) * 125;
+      for (int e = lane; e < 125; e += 64) {
+        cx<T> x = src[e];
+        if (tt == 1 && tw) {            // bottom chain: middle blocks start at zero
+          int sl = e / 25;
+          for (int vc = vr - 4; vc <= vr + 4; vc++)
+            if (slot(vr, vc) == sl && middle(vr, vc)) x = cx<T>{0, 0};
+        }
+        win[w][vr % FW][e] = x;
+      }
+    };
+    // write back a row: own = all slots (pivot rows) or the chain's share
+    auto store_row = [&](int vr, bool all) {
+      cx<T> *dst = lul + (long)real(vr) * 125;
+      for (int e = lane; e < 125; e += 64) {
+        const int sl = e / 25;
+        int vcol = -1;
+        for (int vc = vr - 4; vc <= vr + 4; vc++)
+          if (slot(vr, vc) == sl) vcol = vc;
+        if (vcol < 0) { if (all) dst[e] = win[w][vr % FW][e]; continue; }
+        const int c = real(vcol);
+        if (all) { dst[e] = win[w][vr % FW][e]; continue; }
+        if (middle(vr, vcol)) {
+          if (tt == 0) dst[e] = win[w][vr % FW][e];
+          else midl[((real(vr) - h) * 2 + (c - h)) * 25 + (e % 25)] = win[w][vr % FW][e];
+        } else if ((tt == 0 && c <= h + 1) || (tt == 1 && c >= h + 2)) {
+          dst[e] = win[w][vr % FW][e];
+        }
+      }
+    };
+    if (!preloaded)
+      for (int vr = v0; vr <= min(v0 + FW - 1, lim); vr++) load_row(vr);
+    wsync();
+    for (int v = v0; v < v1; v++) {
+      const int sd = slot(v, v);
+      cx<T> *D = B(v, sd);
+      int uc[4], us[4], nu = 0, pr[4], nparts = 0;
+      for (int d = 1; d <= 4; d++) {
+        const int vc = v + d;
+        if (vc <= lim && slot(v, vc) >= 0) { uc[nu] = vc; us[nu] = slot(v, vc); nu++; }
+        if (vc <= lim && slot(vc, v) >= 0) pr[nparts++] = vc;
+      }
+      // in-block LU of D, its row operations applied to the right blocks
+      for (int nc = 0; nc < 4; nc++) {
+        if (lane < 4 - nc) {
+          const int l = nc + 1 + lane;
+          D[l * 5 + nc] = -cdiv(D[l * 5 + nc], D[nc * 5 + nc]);
+        }
+        wsync();
+        const int nd = (4 - nc) * (4 - nc), ntask = nd + nu * (4 - nc) * 5;
+        for (int k = lane; k < ntask; k += 64) {
+          if (k < nd) {
+            const int l = nc + 1 + k / (4 - nc), q = nc + 1 + k % (4 - nc);
+            D[l * 5 + q] = cfma(D[l * 5 + nc], D[nc * 5 + q], D[l * 5 + q]);
+          } else {
+            const int kk = k - nd, u = kk / ((4 - nc) * 5), rem = kk % ((4 - nc) * 5);
+            const int l = nc + 1 + rem / 5, q = rem % 5;
+            cx<T> *U = B(v, us[u]);
+            U[l * 5 + q] = cfma(D[l * 5 + nc], U[nc * 5 + q], U[l * 5 + q]);
+          }
+        }
+        wsync();
+      }
+      // partner rows: the block in the pivot column becomes the multipliers
+      for (int nc = 0; nc < 5; nc++) {
+        const cx<T> fact = crecip(D[nc * 5 + nc]);
+        if (lane < nparts * 5) {
+          cx<T> *Lb = B(pr[lane / 5], slot(pr[lane / 5], v));
+          const int mm = lane % 5;
+          Lb[mm * 5 + nc] = -(Lb[mm * 5 + nc] * fact);
+        }
+        wsync();
+        const int nl = nparts * 5 * (4 - nc), ntask = nl + nparts * nu * 25;
+        for (int k = lane; k < ntask; k += 64) {
+          if (k < nl) {
+            const int pp = k / (5 * (4 - nc)), rem = k % (5 * (4 - nc));
+            const int mm = rem / (4 - nc), q = nc + 1 + rem % (4 - nc);
+            cx<T> *Lb = B(pr[pp], slot(pr[pp], v));
+            Lb[mm * 5 + q] = cfma(Lb[mm * 5 + nc], D[nc * 5 + q], Lb[mm * 5 + q]);
+          } else {
+            const int kk = k - nl, pp = kk / (nu * 25), rem = kk % (nu * 25);
+            const int u = rem / 25, mm = (rem % 25) / 5, q = rem % 5;
+            const int sdst = slot(pr[pp], uc[u]);
+            if (sdst < 0) continue;      // no such block (does not occur in the band)
+            cx<T> *Lb = B(pr[pp], slot(pr[pp], v));
+            cx<T> *Dst = B(pr[pp], sdst);
+            const cx<T> *U = B(v, us[u]);
+            Dst[mm * 5 + q] = cfma(Lb[mm * 5 + nc], U[nc * 5 + q], Dst[mm * 5 + q]);
+          }
+        }
+        wsync();
+      }
+      // row v is final: write it back and bring in row v+5
+      store_row(v, true);
+      wsync();
+      if (v + FW <= lim) load_row(v + FW);
+      wsync();
+    }
+    // rows the chain updated but did not pivot
+    for (int vr = v1; vr <= min(v1 + FW - 1, lim); vr++) store_row(vr, !tw);
+  };
+  if (!tw) {
+    chain(0, 0, 0, n, n - 1, false);
+    return;
   }
+  if (wave == 0) chain(0, 0, 0, h, h + 1, false);
+  else chain(1, 1, 0, n - h - 2, n - 1 - h, false);
+  __syncthreads();
+  if (wave != 0) return;
+  // rows h, h+1 with both chains' updates of their 2x2 middle block summed,
+  // then their elimination
+  for (int rr = h; rr <= h + 1; rr++)
+    for (int e = lane; e < 125; e += 64) win[0][rr % FW][e] = lul[(long)rr * 125 + e];
+  wsync();
+  for (int e = lane; e < 100; e += 64) {
+    const int b = e / 25, r = h + b / 2, c = h + b % 2;
+    cx<T> &x = win[0][r % FW][bc_slot(n, r, c) * 25 + e % 25];
+    x = x + midl[e];
+  }
+  chain(0, 0, h, h + 2, h + 1, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -845,7 +1016,8 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
 // layout conversions between the reference's arrays and the working copy
 // ref mat (5,5,5,...) element (slot k, r, c) of (line l, row i) at
 //   ((i*sRm + l*sLm) * 125 + k + 5r + 25c)
-__global__ void k_import_mat(const double2 *ref, cx<double> *lu, int n, long L, long sLm, long sRm) {
+__global__ void k_import_mat(const double2 *ref, cx<double> *lu, int n, long L, long sLm, long sRm,
+                             int line_major) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;   // over (i, k, r, c, line)
   long total = (long)n * 125 * L;
   if (idx >= total) return;
@@ -854,7 +1026,7 @@ __global__ void k_import_mat(const double2 *ref, cx<double> *lu, int n, long L, 
   int c = e % 5, r = (e / 5) % 5, k = (e / 25) % 5;
   long i = e / 125;
   double2 v = ref[(i * sRm + line * sLm) * 125 + k + 5 * r + 25 * c];
-  lu[idx] = cx<double>{v.x, v.y};
+  lu[line_major ? line * n * 125 + e : idx] = cx<double>{v.x, v.y};
 }
 
 // ---------------------------------------------------------------------------
@@ -875,7 +1047,7 @@ int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long l
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
-  pl->fwd = pl->bwd = pl->tail = nullptr;
+  pl->fwd = pl->bwd = pl->tail = pl->mid = nullptr;
   if (periodic) {
     LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * sizeof(cx<double>)));
     LNS3D_CHECK(hipMalloc(&pl->per2, (size_t)pl->np * 50 * lines * sizeof(cx<double>)));
@@ -884,6 +1056,7 @@ int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long l
     LNS3D_CHECK(hipMalloc(&pl->fwd, (size_t)pl->n * 60 * lines * es));
     LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * es));
     LNS3D_CHECK(hipMalloc(&pl->tail, (size_t)150 * lines * (pl->h >= 0 ? 2 : 1) * es));
+    if (pl->h >= 0) LNS3D_CHECK(hipMalloc(&pl->mid, (size_t)100 * lines * sizeof(cx<double>)));
   }
   return 0;
 }
@@ -895,49 +1068,47 @@ void penta_plan_destroy(PentaPlan *pl) {
   if (pl->fwd) (void)hipFree(pl->fwd);
   if (pl->bwd) (void)hipFree(pl->bwd);
   if (pl->tail) (void)hipFree(pl->tail);
-  pl->lu = pl->per = pl->per2 = pl->fwd = pl->bwd = pl->tail = nullptr;
+  if (pl->mid) (void)hipFree(pl->mid);
+  pl->lu = pl->per = pl->per2 = pl->fwd = pl->bwd = pl->tail = pl->mid = nullptr;
 }
 
 int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStream_t s) {
   long total = (long)pl->np * 125 * pl->L;
   hipLaunchKernelGGL(k_import_mat, dim3(grid_for(total, 256)), dim3(256), 0, s,
-                     (const double2 *)ref_mat, (cx<double> *)pl->lu, pl->np, pl->L, sLm, sRm);
+                     (const double2 *)ref_mat, (cx<double> *)pl->lu, pl->np, pl->L, sLm, sRm,
+                     pl->periodic ? 0 : 1);
   LNS3D_CHECK(hipGetLastError());
-  return 0;
-}
-
-int penta_factor(PentaPlan *pl, hipStream_t s) {
-  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
-  hipLaunchKernelGGL(k_lu_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
-                     (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2);
-  LNS3D_CHECK(hipGetLastError());
-  if (!pl->periodic) {
-    if (pl->prec)
-      hipLaunchKernelGGL((k_records_bc<double, float>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
-                         (const cx<double> *)pl->lu, (cx<float> *)pl->fwd, (cx<float> *)pl->bwd,
-                         (cx<float> *)pl->tail);
-    else
-      hipLaunchKernelGGL((k_records_bc<double, double>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
-                         (const cx<double> *)pl->lu, (cx<double> *)pl->fwd, (cx<double> *)pl->bwd,
-                         (cx<double> *)pl->tail);
-    LNS3D_CHECK(hipGetLastError());
-  }
   return 0;
 }
 
 int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
   if (pl->periodic) return set_error(__FILE__, __LINE__, "records are for non-periodic lines");
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
+  const int nb = grid_for(pl->L * (long)pl->n, 128);
   if (pl->prec)
-    hipLaunchKernelGGL((k_records_bc<double, float>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
+    hipLaunchKernelGGL((k_records_rows<double, float>), dim3(nb), dim3(128), 0, s, g,
                        (const cx<double> *)pl->lu, (cx<float> *)pl->fwd, (cx<float> *)pl->bwd,
                        (cx<float> *)pl->tail);
   else
-    hipLaunchKernelGGL((k_records_bc<double, double>), dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
+    hipLaunchKernelGGL((k_records_rows<double, double>), dim3(nb), dim3(128), 0, s, g,
                        (const cx<double> *)pl->lu, (cx<double> *)pl->fwd, (cx<double> *)pl->bwd,
                        (cx<double> *)pl->tail);
   LNS3D_CHECK(hipGetLastError());
   return 0;
+}
+
+int penta_factor(PentaPlan *pl, hipStream_t s) {
+  LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
+  if (pl->periodic) {
+    hipLaunchKernelGGL(k_lu_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
+                       (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2);
+    LNS3D_CHECK(hipGetLastError());
+    return 0;
+  }
+  hipLaunchKernelGGL(k_factor_chains, dim3((unsigned)pl->L), dim3(pl->h >= 0 ? 128 : 64), 0, s, g,
+                     (cx<double> *)pl->lu, (cx<double> *)pl->mid);
+  LNS3D_CHECK(hipGetLastError());
+  return penta_factor_records_only(pl, s);
 }
 
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {

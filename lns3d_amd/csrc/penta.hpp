@@ -6,7 +6,8 @@ namespace lns3d {
 
 // One set of independent lines of equal length, factored once, solved many
 // times.  Device buffers:
-//   lu, per, per2 : LU working copy, line-interleaved (np rows)
+//   lu, per, per2 : LU working copy (np rows): line-interleaved for periodic
+//                   lines, line-major for the others
 //   fwd, bwd, tail: solve records of non-periodic lines (prec 0: complex
 //                   double, 1: complex float)
 struct PentaPlan {
@@ -17,6 +18,7 @@ struct PentaPlan {
   int prec = 0;
   void *lu = nullptr, *per = nullptr, *per2 = nullptr;
   void *fwd = nullptr, *bwd = nullptr, *tail = nullptr;
+  void *mid = nullptr;   // twisted factor: the bottom chain's share of the middle block
 };
 
 int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec);

@@ -552,8 +552,10 @@ static AsmCtx make_asm(const lns3d_params &p, int dir) {
   return A;
 }
 
-// into the line-interleaved working copy: element ((row*5 + k)*25 + r*5 + cc)*L + line
-__global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, z *lu) {
+// into the working copy of the line solver: line-interleaved, element
+// ((row*5 + k)*25 + r*5 + cc)*L + line (periodic lines), or line-major,
+// element ((line*n + row)*5 + k)*25 + r*5 + cc
+__global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, int line_major, z *lu) {
   const int nx = A.p.nx, ny = A.p.ny;
   const long L = dir == 1 ? ny : nx, n = dir == 1 ? nx : ny;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,8 +564,11 @@ __global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, z *lu) {
   int i = dir == 1 ? (int)row : (int)line, j = dir == 1 ? (int)line : (int)row;
   for (int k = 0; k < 5; k++)
     for (int r = 0; r < 5; r++)
-      for (int cc = 0; cc < 5; cc++)
-        lu[((row * 5 + k) * 25 + r * 5 + cc) * L + line] = lhs_entry(A, c, dir, i, j, k, r, cc);
+      for (int cc = 0; cc < 5; cc++) {
+        const long e = line_major ? ((line * n + row) * 5 + k) * 25 + r * 5 + cc
+                                  : ((row * 5 + k) * 25 + r * 5 + cc) * L + line;
+        lu[e] = lhs_entry(A, c, dir, i, j, k, r, cc);
+      }
 }
 
 __global__ void k_assemble_ref(AsmCtx A, Coeffs c, int dir, double2 *mat) {
@@ -583,7 +588,7 @@ int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &
   AsmCtx A = make_asm(p, dir);
   long tot = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_assemble_lines, dim3(nblocks(tot, 128)), dim3(128), 0, s, A, c, dir,
-                     (z *)plan.lu);
+                     plan.periodic ? 0 : 1, (z *)plan.lu);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
