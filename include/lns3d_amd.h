@@ -105,6 +105,10 @@ typedef struct lns3d_ctx lns3d_ctx;
 int lns3d_ctx_create(const lns3d_params *p, lns3d_ctx **out);
 int lns3d_ctx_destroy(lns3d_ctx *c);
 int lns3d_ctx_set_stream(lns3d_ctx *c, void *stream);
+/* new physics (beta, omega, Ma, Re, ... and boundary selectors) for an
+ * existing context of the same grid size, periodicity and precision; form
+ * the coefficients and factors again with lns3d_setup / lns3d_factor */
+int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p);
 int lns3d_ctx_synchronize(lns3d_ctx *c);
 /* metric (nx,ny,10: m1,m2,n1,n2,m11,m12,m22,n11,n12,n22; src/gengrid.f90:76-80)
  * and mean flow vm (5,nx,ny), host arrays; generates on the device the
@@ -144,6 +148,34 @@ int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int 
                        int *iters_out, double *resid_out);
 /* the step with complex-float factors and no inner refinement */
 int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev);
+
+/* The refined step with a fixed number of corrections and no host
+ * synchronisation (graph-capturable); check = 1 adds one operator
+ * application whose relative residual lns3d_last_residual returns.  No
+ * counterpart in the reference (its step is complex double). */
+int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev, int corrections, int check);
+int lns3d_last_residual(lns3d_ctx *c, double *rel);
+/* device bytes a factored context of these parameters holds (admission by
+ * memory budget in the serving loop) */
+int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
+
+/* Graphs of steps: nsteps steps of each of count factored contexts, every
+ * context on its own branch, captured once and replayed with one launch
+ * (batches of uneven instances).  mode 0 complex double (lns3d_step), 1
+ * complex-float factors (lns3d_step_fp32), 2 complex-float factors with
+ * `corrections` defect corrections (lns3d_step_refined_fixed).  dv_dev may
+ * be NULL.  Launch on `stream` (NULL: the graph's own stream). */
+typedef struct lns3d_graph lns3d_graph;
+int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps, int mode,
+                       int corrections, lns3d_graph **out);
+int lns3d_graph_launch(lns3d_graph *g, void *stream);
+int lns3d_graph_synchronize(lns3d_graph *g);
+int lns3d_graph_destroy(lns3d_graph *g);
+/* autotuner of the line solves: times every solve-kernel variant (one line
+ * per workgroup, four lines per workgroup, twisted) on synthetic lines of n
+ * rows x lines and uses the fastest for plans of that shape created later;
+ * times_ms (may be NULL) receives the three timings (-1: not applicable) */
+int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *times_ms);
 
 /* Host-array driving of the time loop of src/impdrv3d.f90:126-406 (for
  * callers that hold v in host memory, such as the Fortran host layer):

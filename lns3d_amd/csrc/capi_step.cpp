@@ -31,7 +31,7 @@ struct lns3d_ctx {
   void *work = nullptr;      // 5P complex
   void *r = nullptr;         // 5P complex
   void *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;   // 5P complex each (refinement)
-  double *scal = nullptr;    // device scalars
+  double *scal = nullptr;    // device scalars (residual norms at +8, partials after)
   double *coef_block = nullptr;
   bool coeffs_ready = false, factored = false;
   void *vstate = nullptr;    // 5P complex: the state of the host-array driver
@@ -80,7 +80,7 @@ extern "C" int lns3d_ctx_create(const lns3d_params *p, lns3d_ctx **out) {
   c->c.metric = b;
   if (hipMalloc(&c->work, (size_t)P * 5 * 16) != hipSuccess ||
       hipMalloc(&c->r, (size_t)P * 5 * 16) != hipSuccess ||
-      hipMalloc(&c->scal, 64 * sizeof(double)) != hipSuccess)
+      hipMalloc(&c->scal, 2048 * sizeof(double)) != hipSuccess)
     return bail(fail(LNS3D_EHIP, "hipMalloc work"));
   rc = penta_plan_create(&c->px, p->xper, p->periodic_exact, p->nx, p->ny, 0);
   if (!rc) rc = penta_plan_create(&c->py, p->yper, p->periodic_exact, p->ny, p->nx, 0);
@@ -104,6 +104,24 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->scal) (void)hipFree(c->scal);
   if (c->vstate) (void)hipFree(c->vstate);
   delete c;
+  return 0;
+}
+
+// new physics for an existing context (same grid size, periodicity and
+// precision): the coefficients and factors must be formed again
+// (lns3d_setup, lns3d_factor)
+extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
+  if (!c || !p) return fail(LNS3D_EINVAL, "lns3d_ctx_set_params: null argument");
+  if (p->nx != c->p.nx || p->ny != c->p.ny || p->xper != c->p.xper || p->yper != c->p.yper ||
+      p->precision != c->p.precision)
+    return fail(LNS3D_EINVAL, "lns3d_ctx_set_params: grid size, periodicity and precision are fixed");
+  int rc = check_params(*p);
+  if (rc) return rc;
+  c->p = *p;
+  c->st = make_stencil(*p);
+  c->px.exact = c->py.exact = p->periodic_exact;
+  c->coeffs_ready = false;
+  c->factored = false;
   return 0;
 }
 
@@ -289,7 +307,8 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
   void *rhs = c->t3;            // the residual r of the step
   int rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
   if (rc) return rc;
-  LNS3D_CHECK(hipMemcpyAsync(dv, rhs, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  rc = launch_copy(dv, rhs, n, c->stream);
+  if (rc) return rc;
   rc = solve_xy(c, c->px32, c->py32, dv);
   if (rc) return rc;
   int it = 1;
@@ -323,6 +342,163 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
   if (!rc) rc = launch_update_bc(c->p, v_dev, r, c->stream);
   return rc;
+}
+
+// The refined step with a fixed number of corrections and no host
+// synchronisation, so that it can be captured in a graph.  With check set,
+// one more operator application leaves |res|_max / |r|_max of the final dv
+// for lns3d_last_residual.
+extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev, int corrections,
+                                        int check) {
+  if (!c || !c->factored || !c->have32 || !v_dev || corrections < 0)
+    return fail(LNS3D_ENOTFACTORED, "lns3d_step_refined_fixed: needs precision = 1 and lns3d_factor");
+  const long n = 5L * c->p.nx * c->p.ny;
+  void *dv = dv_dev ? dv_dev : c->r;
+  void *rhs = c->t3;
+  int rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
+  if (!rc) rc = launch_copy(dv, rhs, n, c->stream);
+  if (!rc) rc = solve_xy(c, c->px32, c->py32, dv);
+  for (int k = 0; k < corrections && !rc; k++) {
+    rc = launch_af_matvec(c->p, c->c, dv, c->t2, c->t1, c->stream);
+    if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);
+    if (!rc) rc = solve_xy(c, c->px32, c->py32, c->t2);
+    if (!rc) rc = launch_axpy(dv, c->t2, n, c->stream);
+  }
+  if (!rc && check) {
+    rc = launch_af_matvec(c->p, c->c, dv, c->t2, c->t1, c->stream);
+    if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);
+  }
+  if (!rc) rc = launch_update_bc(c->p, v_dev, dv, c->stream);
+  return rc;
+}
+
+extern "C" int lns3d_last_residual(lns3d_ctx *c, double *rel) {
+  if (!c || !rel) return fail(LNS3D_EINVAL, "lns3d_last_residual: null argument");
+  double nr[2];
+  LNS3D_CHECK(hipMemcpyAsync(nr, c->scal + 8, 16, hipMemcpyDeviceToHost, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  *rel = nr[1] > 0 ? nr[0] / nr[1] : 0.0;
+  return 0;
+}
+
+// bytes of device memory a context of these parameters holds once factored
+extern "C" int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes) {
+  if (!p || !bytes) return fail(LNS3D_EINVAL, "lns3d_ctx_bytes: null argument");
+  const size_t P = (size_t)p->nx * p->ny;
+  size_t b = P * (25 * 4 + 6 * 3 + 4 + 1 + 5 + 10) * 8 + P * 160 + 512;   // coefficients, work, r
+  auto plan = [&](int periodic, size_t es) {
+    size_t x = P * 125 * 16;                                  // LU working copy
+    if (periodic) x += 2 * P * 50 * 16;
+    else x += P * 126 * es + P * 2 * 150 * es / 32 + P * 100 * 16 / 32;
+    return x;
+  };
+  b += plan(p->xper, 16) + plan(p->yper, 16);
+  if (p->precision == 1) b += 2 * P * 126 * 8 + 3 * P * 80;
+  *bytes = b;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Graphs: nsteps steps of count contexts captured once (each context on its
+// own stream branch, forked from and joined to one capture stream) and
+// replayed with one launch.  mode 0: complex-double step; 1: complex-float
+// factors, no refinement; 2: complex-float factors with `corrections`
+// defect corrections.
+struct lns3d_graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  hipStream_t s = nullptr;
+};
+
+extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps,
+                                  int mode, int corrections, lns3d_graph **out) {
+  if (!ctxs || !v_dev || count < 1 || nsteps < 1 || mode < 0 || mode > 2 || !out)
+    return fail(LNS3D_EINVAL, "lns3d_graph_create: bad arguments");
+  for (int i = 0; i < count; i++)
+    if (!ctxs[i] || !ctxs[i]->factored || !v_dev[i] || (mode > 0 && !ctxs[i]->have32))
+      return fail(LNS3D_EINVAL, "lns3d_graph_create: every context must be factored (precision 1 for modes 1, 2)");
+  auto *gr = new lns3d_graph;
+  std::vector<hipStream_t> st(count, nullptr);
+  std::vector<hipEvent_t> ev(count + 1, nullptr);
+  std::vector<hipStream_t> saved(count);
+  int rc = 0;
+  auto cleanup = [&]() {
+    for (auto s2 : st) if (s2) (void)hipStreamDestroy(s2);
+    for (auto e : ev) if (e) (void)hipEventDestroy(e);
+  };
+  if (hipStreamCreateWithFlags(&gr->s, hipStreamNonBlocking) != hipSuccess) rc = fail(LNS3D_EHIP, "stream");
+  for (int i = 0; i < count && !rc; i++)
+    if (hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) != hipSuccess) rc = fail(LNS3D_EHIP, "stream");
+  for (int i = 0; i <= count && !rc; i++)
+    if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) rc = fail(LNS3D_EHIP, "event");
+  if (!rc && hipStreamBeginCapture(gr->s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    rc = fail(LNS3D_EHIP, "hipStreamBeginCapture");
+  bool capturing = !rc;
+  if (!rc && hipEventRecord(ev[count], gr->s) != hipSuccess) rc = fail(LNS3D_EHIP, "fork event");
+  for (int i = 0; i < count && !rc; i++) {
+    lns3d_ctx *c = ctxs[i];
+    if (hipStreamWaitEvent(st[i], ev[count], 0) != hipSuccess) { rc = fail(LNS3D_EHIP, "fork wait"); break; }
+    saved[i] = c->stream;
+    c->stream = st[i];
+    void *dv = dv_dev ? dv_dev[i] : nullptr;
+    for (int k = 0; k < nsteps && !rc; k++) {
+      if (mode == 0) rc = lns3d_step(c, v_dev[i], dv);
+      else if (mode == 1) rc = lns3d_step_fp32(c, v_dev[i], dv);
+      else rc = lns3d_step_refined_fixed(c, v_dev[i], dv, corrections, 0);
+    }
+    c->stream = saved[i];
+    if (!rc && hipEventRecord(ev[i], st[i]) != hipSuccess) rc = fail(LNS3D_EHIP, "join event");
+    if (!rc && hipStreamWaitEvent(gr->s, ev[i], 0) != hipSuccess) rc = fail(LNS3D_EHIP, "join wait");
+  }
+  if (capturing) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(gr->s, &g);
+    if (!rc && e != hipSuccess) rc = fail(LNS3D_EHIP, "hipStreamEndCapture");
+    gr->g = g;
+  }
+  if (!rc && hipGraphInstantiate(&gr->x, gr->g, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(LNS3D_EHIP, "hipGraphInstantiate");
+  cleanup();
+  if (rc) {
+    if (gr->x) (void)hipGraphExecDestroy(gr->x);
+    if (gr->g) (void)hipGraphDestroy(gr->g);
+    if (gr->s) (void)hipStreamDestroy(gr->s);
+    delete gr;
+    return rc;
+  }
+  *out = gr;
+  return 0;
+}
+
+extern "C" int lns3d_graph_launch(lns3d_graph *gr, void *stream) {
+  if (!gr) return fail(LNS3D_EINVAL, "lns3d_graph_launch: null graph");
+  LNS3D_CHECK(hipGraphLaunch(gr->x, stream ? (hipStream_t)stream : gr->s));
+  return 0;
+}
+
+extern "C" int lns3d_graph_synchronize(lns3d_graph *gr) {
+  if (!gr) return fail(LNS3D_EINVAL, "lns3d_graph_synchronize: null graph");
+  LNS3D_CHECK(hipStreamSynchronize(gr->s));
+  return 0;
+}
+
+extern "C" int lns3d_graph_destroy(lns3d_graph *gr) {
+  if (!gr) return 0;
+  (void)hipStreamSynchronize(gr->s);
+  if (gr->x) (void)hipGraphExecDestroy(gr->x);
+  if (gr->g) (void)hipGraphDestroy(gr->g);
+  if (gr->s) (void)hipStreamDestroy(gr->s);
+  delete gr;
+  return 0;
+}
+
+// autotuner of the line-solve kernels: measures every variant on synthetic
+// lines of n rows x lines and keeps the fastest for plans of that shape
+// created afterwards; times_ms (may be NULL) receives 3 timings (-1: n/a)
+extern "C" int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *times_ms) {
+  if (n < 7 || lines < 1 || (precision != 0 && precision != 1))
+    return fail(LNS3D_EINVAL, "lns3d_tune_lines: bad shape");
+  return penta_tune(n, lines, precision, variant, times_ms);
 }
 
 // Host-array driving of the time loop (src/impdrv3d.f90:126-406): the

@@ -29,6 +29,10 @@
 //     chain per 16-lane DPP row, lanes 0..4 owning one component each.
 //  Periodic lines are solved by k_solve_lines (one lane per line, the
 //  reference's order) on the LU working copy.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "common.hpp"
 #include "penta.hpp"
 
@@ -1033,7 +1037,27 @@ __global__ void k_import_mat(const double2 *ref, cx<double> *lu, int n, long L, 
 // host side
 static int grid_for(long n, int b) { return (int)((n + b - 1) / b); }
 
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, long, int>, int> g_tuned;
+
+int penta_variant_for(int n, long lines, int prec) {
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(n, lines, prec));
+    if (it != g_tuned.end()) return it->second;
+  }
+  if (n >= 16 && lines < 512) return PENTA_TWISTED;
+  return lines >= 512 ? PENTA_PLAIN4 : PENTA_PLAIN1;
+}
+
 int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec) {
+  const int n = periodic ? n_rows - 1 : n_rows;
+  return penta_plan_create_variant(pl, periodic, exact, n_rows, lines, prec,
+                                   periodic ? PENTA_PLAIN1 : penta_variant_for(n, lines, prec));
+}
+
+int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec,
+                              int variant) {
   pl->periodic = periodic;
   pl->exact = exact;
   pl->np = n_rows;
@@ -1041,9 +1065,9 @@ int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long l
   pl->L = lines;
   pl->prec = prec;
   if (pl->n < 7 || lines < 1) return set_error(__FILE__, __LINE__, "line length must be >= 7 unknowns");
-  // few long lines: twisted factor (two chains per line); many lines keep
-  // the plain order (four lines per workgroup already fill the chip)
-  pl->h = (!periodic && pl->n >= 16 && lines < 512) ? (pl->n - 1) / 2 : -1;
+  if (variant == PENTA_TWISTED && pl->n < 16) variant = PENTA_PLAIN1;
+  pl->variant = periodic ? PENTA_PLAIN1 : variant;
+  pl->h = (!periodic && pl->variant == PENTA_TWISTED) ? (pl->n - 1) / 2 : -1;
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
@@ -1118,20 +1142,20 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
                        (const cx<double> *)pl->lu, (const cx<double> *)pl->per,
                        (const cx<double> *)pl->per2, (double2 *)rhs, sL, sR);
   } else {
-    // twisted lines: one line (two chains) per workgroup; many lines: four
+    // twisted lines: one line (two chains) per workgroup; PENTA_PLAIN4: four
     // lines per workgroup (one per DPP row of the compute wave)
     const unsigned g4 = (unsigned)((pl->L + 3) / 4), g1 = (unsigned)pl->L;
     if (pl->prec) {
       auto *f = (const cx<float> *)pl->fwd; auto *b = (const cx<float> *)pl->bwd;
       auto *t = (const cx<float> *)pl->tail;
       if (pl->h >= 0) hipLaunchKernelGGL((k_solve_tw<float, 24, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-      else if (pl->L >= 512) hipLaunchKernelGGL((k_solve_bc<float, 12, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      else if (pl->variant == PENTA_PLAIN4) hipLaunchKernelGGL((k_solve_bc<float, 12, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
       else hipLaunchKernelGGL((k_solve_bc<float, 54, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
     } else {
       auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
       auto *t = (const cx<double> *)pl->tail;
       if (pl->h >= 0) hipLaunchKernelGGL((k_solve_tw<double, 12, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-      else if (pl->L >= 512) hipLaunchKernelGGL((k_solve_bc<double, 6, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
+      else if (pl->variant == PENTA_PLAIN4) hipLaunchKernelGGL((k_solve_bc<double, 6, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
       else hipLaunchKernelGGL((k_solve_bc<double, 30, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
     }
   }
@@ -1147,6 +1171,71 @@ int penta_solve_reference_order(PentaPlan *pl, void *rhs, long sL, long sR, hipS
                      (const cx<double> *)pl->lu, (const cx<double> *)pl->per,
                      (const cx<double> *)pl->per2, (double2 *)rhs, sL, sR);
   LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: synthetic diagonally dominant lines of the requested shape,
+// every solve variant factored and timed, the fastest kept for the shape
+__global__ void k_fill_synthetic(cx<double> *lu, long total, unsigned seed) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  unsigned x = (unsigned)(idx * 2654435761u) ^ seed;
+  x ^= x >> 13; x *= 0x5bd1e995u; x ^= x >> 15;
+  double a = ((x & 0xffff) / 65536.0 - 0.5) * 0.1, b = ((x >> 16) / 65536.0 - 0.5) * 0.1;
+  const int e = (int)(idx % 125), k = e / 25, r = (e / 5) % 5, c = e % 5;
+  if (k == 2 && r == c) a += 3.0;
+  lu[idx] = cx<double>{a, b};
+}
+
+int penta_tune(int n, long lines, int prec, int *variant, float *times_ms) {
+  float best = 1e30f;
+  int bestv = -1;
+  hipStream_t s;
+  LNS3D_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  LNS3D_CHECK(hipEventCreate(&e0));
+  LNS3D_CHECK(hipEventCreate(&e1));
+  void *rhs = nullptr;
+  LNS3D_CHECK(hipMalloc(&rhs, (size_t)n * lines * 5 * sizeof(double2)));
+  LNS3D_CHECK(hipMemsetAsync(rhs, 0, (size_t)n * lines * 5 * sizeof(double2), s));
+  int rc = 0;
+  for (int v = 0; v < PENTA_NVARIANT && !rc; v++) {
+    if (times_ms) times_ms[v] = -1.0f;
+    if (v == PENTA_TWISTED && n < 16) continue;
+    PentaPlan pl;
+    rc = penta_plan_create_variant(&pl, 0, 0, n, lines, prec, v);
+    if (!rc) {
+      long total = (long)n * 125 * lines;
+      hipLaunchKernelGGL(k_fill_synthetic, dim3(grid_for(total, 256)), dim3(256), 0, s,
+                         (cx<double> *)pl.lu, total, 12345u);
+      rc = penta_factor(&pl, s);
+    }
+    if (!rc) rc = penta_solve(&pl, rhs, n, 1, s);      // warm-up
+    if (!rc) {
+      (void)hipEventRecord(e0, s);
+      for (int k = 0; k < 3 && !rc; k++) rc = penta_solve(&pl, rhs, n, 1, s);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= 3;
+      if (times_ms) times_ms[v] = ms;
+      if (ms < best) { best = ms; bestv = v; }
+    }
+    (void)hipStreamSynchronize(s);
+    penta_plan_destroy(&pl);
+  }
+  (void)hipFree(rhs);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(n, lines, prec)] = bestv;
+  }
+  if (variant) *variant = bestv;
   return 0;
 }
 

@@ -14,6 +14,7 @@ struct PentaPlan {
   int periodic = 0, exact = 0;
   int n = 0, np = 0;   // unknown rows, stored rows (np = n+1 when periodic)
   int h = -1;          // twisted factor split (rows 0..h+1 | h+2..n-1), -1: plain
+  int variant = 0;     // solve kernel (non-periodic): see penta_variant_* below
   long L = 0;          // number of lines
   int prec = 0;
   void *lu = nullptr, *per = nullptr, *per2 = nullptr;
@@ -21,7 +22,18 @@ struct PentaPlan {
   void *mid = nullptr;   // twisted factor: the bottom chain's share of the middle block
 };
 
+// solve-kernel variants of non-periodic lines: one line per workgroup with a
+// deep LDS ring, four lines per workgroup, or twisted lines (two chains)
+enum { PENTA_PLAIN1 = 0, PENTA_PLAIN4 = 1, PENTA_TWISTED = 2, PENTA_NVARIANT = 3 };
+// the variant a plan of this shape uses: the autotuned one when
+// penta_tune has measured the shape, else a fixed rule
+int penta_variant_for(int n, long lines, int prec);
+// time every variant on synthetic lines of this shape (n rows, lines,
+// precision) and record the fastest; returns the chosen variant in *variant
+int penta_tune(int n, long lines, int prec, int *variant, float *times_ms);
 int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec);
+int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec,
+                              int variant);
 void penta_plan_destroy(PentaPlan *pl);
 // copy a device matrix in the reference's layout mat(5,5,5,...) into the
 // working copy; block (line l, row i) starts at complex offset (i*sRm + l*sLm)*125

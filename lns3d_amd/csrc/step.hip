@@ -10,6 +10,8 @@
 //   update + BCs             impdrv3d.f90:340-356, src/itrbc3d.f90
 // One thread per grid point.  Coefficients are structure-of-arrays ([e][P])
 // so a wave's loads of one coefficient are contiguous.
+#include <algorithm>
+
 #include "common.hpp"
 #include "step.hpp"
 
@@ -810,6 +812,19 @@ int launch_itrbc(const lns3d_params &p, void *v, hipStream_t s) {
   return 0;
 }
 
+__global__ void k_copy2(double2 *dst, const double2 *src, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+// device copy as a kernel (graph-capture friendly)
+int launch_copy(void *dst, const void *src, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy2, dim3(nblocks(n, 256)), dim3(256), 0, s, (double2 *)dst,
+                     (const double2 *)src, n);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
 int launch_axpy(void *x, const void *d, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(nblocks(n, 256)), dim3(256), 0, s, (double2 *)x, (const double2 *)d, n);
   LNS3D_CHECK(hipGetLastError());
@@ -837,49 +852,151 @@ __device__ __forceinline__ int slot_col(int N, int per, int pos, int k) {
   return pos - 2 + k;
 }
 
-__global__ void k_line_matvec(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
-  const int nx = A.p.nx, ny = A.p.ny;
+// One factor applied as a stencil: with the slot weights w1_k (first
+// derivative) and w2_k (second derivative) of the row (interior a_k, b_k of
+// src/lhs1f3d.f90; closure weights of src/lhsbt1f3d.f90 near the ends),
+//   L x = x_c + dt [ X (d1 x) - V o (d2 x) - i AB o (d1 x) ] + calcd alfa dt (D + i Di) x_c
+// where d1 x = sum_k w1_k x_col(k), d2 x = sum_k w2_k x_col(k), X = Ah or Bh,
+// V the compact viscous entries and AB the spanwise cross entries of the
+// direction; boundary-condition rows as lhs_entry.  Equal to the assembled
+// block rows up to rounding.
+__global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
+  using namespace st;
+  const lns3d_params &p = A.p;
+  const int nx = p.nx, ny = p.ny;
+  const long P = (long)nx * ny;
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)nx * ny) return;
-  int i = q % nx, j = q / nx;
+  if (q >= P) return;
+  const int i = q % nx, j = q / nx;
   const int N = dir == 1 ? nx : ny, pos = dir == 1 ? i : j;
-  const int per = dir == 1 ? A.p.xper : A.p.yper;
-  int posc = (per && pos == N - 1) ? 0 : pos;     // duplicate periodic row
-  int ii = dir == 1 ? posc : i, jj = dir == 1 ? j : posc;
-  z acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  for (int k = 0; k < 5; k++) {
-    int col = slot_col(N, per, posc, k);
-    long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
-    z xc[5];
-    for (int e = 0; e < 5; e++) xc[e] = ld(x + 5 * qc + e);
-    for (int r = 0; r < 5; r++)
-      for (int cc = 0; cc < 5; cc++) acc[r] = cfma(lhs_entry(A, c, dir, ii, jj, k, r, cc), xc[cc], acc[r]);
+  const int per = dir == 1 ? p.xper : p.yper;
+  const int posc = (per && pos == N - 1) ? 0 : pos;       // duplicate periodic row
+  const long qr = dir == 1 ? (posc + (long)nx * j) : (i + (long)nx * posc);   // row point
+  double w1[5], w2[5];
+  if (!per && (posc <= 1 || posc >= N - 2)) {
+    double F[5], S[5];
+    if (posc == 0) {
+      const double f[5] = {gc4, gc5, gc1, gc2, gc3}, sd[5] = {dd4, dd5, dd1, dd2, dd3};
+      for (int k = 0; k < 5; k++) { F[k] = f[k]; S[k] = sd[k]; }
+    } else if (posc == 1) {
+      const double f[5] = {gb5, gb1, gb2, gb3, gb4}, sd[5] = {db5, db1, db2, db3, db4};
+      for (int k = 0; k < 5; k++) { F[k] = f[k]; S[k] = sd[k]; }
+    } else if (posc == N - 2) {
+      const double f[5] = {-gb4, -gb3, -gb2, -gb1, -gb5}, sd[5] = {db4, db3, db2, db1, db5};
+      for (int k = 0; k < 5; k++) { F[k] = f[k]; S[k] = sd[k]; }
+    } else {
+      const double f[5] = {-gc3, -gc2, -gc1, -gc5, -gc4}, sd[5] = {dd3, dd2, dd1, dd5, dd4};
+      for (int k = 0; k < 5; k++) { F[k] = f[k]; S[k] = sd[k]; }
+    }
+    for (int k = 0; k < 5; k++) { w1[k] = p.alfa * F[k] * A.dinv; w2[k] = p.alfa * S[k] * A.dsinv; }
+  } else {
+    for (int k = 0; k < 5; k++) { w1[k] = A.a[k]; w2[k] = A.b[k]; }
   }
-  for (int r = 0; r < 5; r++) y[5 * q + r] = make_double2(acc[r].re, acc[r].im);
+  z d1v[5], d2v[5], xc[5];
+  for (int e = 0; e < 5; e++) { d1v[e] = z{0, 0}; d2v[e] = z{0, 0}; }
+  for (int k = 0; k < 5; k++) {
+    const int col = slot_col(N, per, posc, k);
+    const long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
+    for (int e = 0; e < 5; e++) {
+      const z xv = ld(x + 5 * qc + e);
+      d1v[e] = d1v[e] + sc(w1[k], xv);
+      d2v[e] = d2v[e] + sc(w2[k], xv);
+      if (k == 2) xc[e] = xv;
+    }
+  }
+  const double dt = c.dtl[qr];
+  const double *X = dir == 1 ? c.Ah : c.Bh;
+  const double *V = dir == 1 ? c.V11 : c.V22;
+  const double *AB = c.ABhi + (dir == 1 ? 0 : 2) * P;
+  z out[5];
+  for (int r = 0; r < 5; r++) {
+    z a{0, 0};
+    for (int cc = 0; cc < 5; cc++) a = a + sc(X[(long)(r + 5 * cc) * P + qr], d1v[cc]);
+    out[r] = a;
+  }
+  out[1] = out[1] - sc(V[qr], d2v[1]) - sc(V[4 * P + qr], d2v[2]);
+  out[2] = out[2] - sc(V[5 * P + qr], d2v[1]) - sc(V[P + qr], d2v[2]);
+  out[3] = out[3] - sc(V[2 * P + qr], d2v[3]);
+  out[4] = out[4] - sc(V[3 * P + qr], d2v[4]);
+  {
+    const double ab0 = AB[qr], ab1 = AB[P + qr];
+    z t1 = sc(ab0, d1v[3]), t2 = sc(ab1, d1v[3]), t3 = sc(ab0, d1v[1]) + sc(ab1, d1v[2]);
+    out[1] = out[1] - z{-t1.im, t1.re};     // - i * t
+    out[2] = out[2] - z{-t2.im, t2.re};
+    out[3] = out[3] - z{-t3.im, t3.re};
+  }
+  for (int r = 0; r < 5; r++) out[r] = sc(dt, out[r]);
+  if (dir == 1) {
+    for (int r = 0; r < 5; r++) {
+      z a{0, 0}, b{0, 0};
+      for (int cc = 0; cc < 5; cc++) {
+        a = a + sc(c.Dh[(long)(r + 5 * cc) * P + qr], xc[cc]);
+        b = b + sc(c.Dhi[(long)(r + 5 * cc) * P + qr], xc[cc]);
+      }
+      const double f = p.alfa * dt;
+      out[r] = out[r] + sc(f, a) + z{-f * b.im, f * b.re};
+    }
+  }
+  for (int r = 0; r < 5; r++) out[r] = out[r] + xc[r];
+  // boundary-condition rows (src/lhsbc1f3d.f90, lhsbc2f3d.f90), as lhs_entry
+  bool ident[5] = {false, false, false, false, false};
+  bool wallrow0 = false, wallrow4 = false;
+  const int ir = dir == 1 ? posc : i, jr = dir == 1 ? j : posc;
+  if (!p.xper && (ir == 0 || ir == nx - 1)) for (int r = 0; r < 5; r++) ident[r] = true;
+  if (!p.yper) {
+    if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;
+    if (jr == 0) {
+      ident[1] = ident[2] = ident[3] = true;
+      if (dir == 1 && (p.wall == 1 || p.wall == 2)) ident[0] = true;
+      if (dir == 2 && p.wall == 1) wallrow0 = true;
+      if (p.wallt == 0 || (dir == 1 && p.wallt == 1)) ident[4] = true;
+      if (dir == 2 && p.wallt == 1) wallrow4 = true;
+    }
+  }
+  if (!p.xper && (ir == 0 || ir == nx - 1)) wallrow0 = wallrow4 = false;
+  if (!p.yper && jr == ny - 1) wallrow0 = wallrow4 = false;
+  if (wallrow0 || wallrow4) {
+    const double w[5] = {-gc4, -gc5, -gc1, -gc2, -gc3};
+    z s0{0, 0}, s4{0, 0};
+    for (int k = 0; k < 5; k++) {
+      const int col = slot_col(N, per, posc, k);
+      const long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
+      s0 = s0 + sc(w[k], ld(x + 5 * qc + 0));
+      s4 = s4 + sc(w[k], ld(x + 5 * qc + 4));
+    }
+    if (wallrow0 && !ident[0]) out[0] = s0;
+    if (wallrow4 && !ident[4]) out[4] = s4;
+  }
+  for (int r = 0; r < 5; r++) {
+    if (ident[r]) out[r] = xc[r];
+    y[5 * q + r] = make_double2(out[r].re, out[r].im);
+  }
 }
 
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,
                      hipStream_t s) {
   long P = (long)p.nx * p.ny;
   AsmCtx Ax = make_asm(p, 1), Ay = make_asm(p, 2);
-  hipLaunchKernelGGL(k_line_matvec, dim3(nblocks(P, 128)), dim3(128), 0, s, Ay, c, 2,
+  hipLaunchKernelGGL(k_line_op, dim3(nblocks(P, 256)), dim3(256), 0, s, Ay, c, 2,
                      (const double2 *)x, (double2 *)tmp);
-  hipLaunchKernelGGL(k_line_matvec, dim3(nblocks(P, 128)), dim3(128), 0, s, Ax, c, 1,
+  hipLaunchKernelGGL(k_line_op, dim3(nblocks(P, 256)), dim3(256), 0, s, Ax, c, 1,
                      (const double2 *)tmp, (double2 *)y);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
 
-// y = b - y and max norms |y|, |b| (atomic max on the bit patterns)
-__global__ void k_residual(const double2 *b, double2 *y, long n, unsigned long long *nrm) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// y = b - y and the max norms |y|, |b| (of real and imaginary parts): one
+// partial pair per block, then one block reduces the partials
+constexpr int RES_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void k_residual(const double2 *b, double2 *y, long n, double *part) {
   double ay = 0.0, abv = 0.0;
-  if (t < n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     double2 bb = b[t], yy = y[t];
     double2 rr = make_double2(bb.x - yy.x, bb.y - yy.y);
     y[t] = rr;
-    ay = fmax(fabs(rr.x), fabs(rr.y));
-    abv = fmax(fabs(bb.x), fabs(bb.y));
+    ay = fmax(ay, fmax(fabs(rr.x), fabs(rr.y)));
+    abv = fmax(abv, fmax(fabs(bb.x), fabs(bb.y)));
   }
   __shared__ double s1[256], s2[256];
   s1[threadIdx.x] = ay;
@@ -893,18 +1010,41 @@ __global__ void k_residual(const double2 *b, double2 *y, long n, unsigned long l
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicMax(nrm, (unsigned long long)__double_as_longlong(s1[0]));
-    atomicMax(nrm + 1, (unsigned long long)__double_as_longlong(s2[0]));
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
   }
 }
 
-__global__ void k_zero2_u64(unsigned long long *x) { x[0] = 0ull; x[1] = 0ull; }
+__global__ __launch_bounds__(256) void k_residual_final(const double *part, int nb, double *norms) {
+  __shared__ double s1[256], s2[256];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    a = fmax(a, part[2 * k]);
+    b = fmax(b, part[2 * k + 1]);
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    norms[0] = s1[0];
+    norms[1] = s2[0];
+  }
+}
 
+// norms: 2 doubles of results followed by 2*RES_BLOCKS of partials
 int launch_residual_norms(const void *b, void *y, long n, double *norms, hipStream_t s) {
-  auto *u = (unsigned long long *)norms;
-  hipLaunchKernelGGL(k_zero2_u64, dim3(1), dim3(1), 0, s, u);
-  hipLaunchKernelGGL(k_residual, dim3(nblocks(n, 256)), dim3(256), 0, s, (const double2 *)b,
-                     (double2 *)y, n, u);
+  const int nb = (int)std::min<long>(RES_BLOCKS, (n + 255) / 256);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, (const double2 *)b, (double2 *)y, n,
+                     norms + 2);
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, s, (const double *)(norms + 2), nb,
+                     norms);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }

@@ -42,8 +42,10 @@ int launch_itrbc(const lns3d_params &p, void *v, hipStream_t s);
 // blocks regenerated from the coefficients); tmp holds 5P complex
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,
                      hipStream_t s);
-// y = b - y (complex, n values) and the max norms of y and b into norms[0..1]
+// y = b - y (complex, n values) and the max norms of y and b into norms[0..1];
+// norms holds 2 + 2*512 doubles (the rest is per-block partials)
 int launch_residual_norms(const void *b, void *y, long n, double *norms, hipStream_t s);
 int launch_axpy(void *x, const void *d, long n, hipStream_t s);   // x += d
+int launch_copy(void *dst, const void *src, long n, hipStream_t s);   // complex n values
 
 }  // namespace lns3d

@@ -33,7 +33,16 @@ SIGNATURES = {
     "lns3d_ctx_create": ([VP, ctypes.POINTER(VP)], CI),
     "lns3d_ctx_destroy": ([VP], CI),
     "lns3d_ctx_set_stream": ([VP, VP], CI),
+    "lns3d_ctx_set_params": ([VP, VP], CI),
     "lns3d_ctx_synchronize": ([VP], CI),
+    "lns3d_step_refined_fixed": ([VP, VP, VP, CI, CI], CI),
+    "lns3d_last_residual": ([VP, ctypes.POINTER(CD)], CI),
+    "lns3d_ctx_bytes": ([VP, ctypes.POINTER(ctypes.c_size_t)], CI),
+    "lns3d_graph_create": ([VP, VP, VP, CI, CI, CI, CI, ctypes.POINTER(VP)], CI),
+    "lns3d_graph_launch": ([VP, VP], CI),
+    "lns3d_graph_synchronize": ([VP], CI),
+    "lns3d_graph_destroy": ([VP], CI),
+    "lns3d_tune_lines": ([CI, ctypes.c_long, CI, ctypes.POINTER(CI), ctypes.POINTER(ctypes.c_float)], CI),
     "lns3d_state_upload": ([VP, VP], CI),
     "lns3d_state_download": ([VP, VP], CI),
     "lns3d_advance": ([VP, CI, VP], CI),
@@ -207,12 +216,89 @@ class Step:
                                        ctypes.byref(it), ctypes.byref(res)), "lns3d_step_refined")
         return it.value, res.value
 
+    def step_refined_fixed(self, v_dev, dv_dev=None, corrections=1, check_residual=False):
+        check(lib().lns3d_step_refined_fixed(self.h, ptr(v_dev),
+                                             ptr(dv_dev) if dv_dev is not None else None,
+                                             corrections, 1 if check_residual else 0),
+              "lns3d_step_refined_fixed")
+
+    def last_residual(self):
+        r = CD(0.0)
+        check(lib().lns3d_last_residual(self.h, ctypes.byref(r)), "lns3d_last_residual")
+        return r.value
+
+    def set_params(self, p):
+        """new physics for this context (same grid size and precision)"""
+        cp = CParams.from_params(p, self.cp.precision)
+        check(lib().lns3d_ctx_set_params(self.h, ctypes.byref(cp)), "lns3d_ctx_set_params")
+        self.cp, self.params = cp, p
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream (or raw hipStream_t as int); None = default"""
+        raw = getattr(stream, "cuda_stream", stream)
+        check(lib().lns3d_ctx_set_stream(self.h, raw), "lns3d_ctx_set_stream")
+
     def synchronize(self):
         check(lib().lns3d_ctx_synchronize(self.h), "lns3d_ctx_synchronize")
 
     def close(self):
         if getattr(self, "h", None):
             lib().lns3d_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ctx_bytes(p, precision=0) -> int:
+    """device bytes a factored context of these parameters holds"""
+    cp = CParams.from_params(p, precision)
+    b = ctypes.c_size_t(0)
+    check(lib().lns3d_ctx_bytes(ctypes.byref(cp), ctypes.byref(b)), "lns3d_ctx_bytes")
+    return int(b.value)
+
+
+def tune_lines(n: int, lines: int, precision: int = 0):
+    """time every line-solve variant on synthetic lines of this shape; the
+    fastest is used by plans of that shape created afterwards.  Returns
+    (variant, [ms per variant, -1 where not applicable])"""
+    v = CI(-1)
+    t = (ctypes.c_float * 3)()
+    check(lib().lns3d_tune_lines(n, lines, precision, ctypes.byref(v), t), "lns3d_tune_lines")
+    return v.value, [float(x) for x in t]
+
+
+MODE_FP64, MODE_FP32, MODE_REFINED = 0, 1, 2
+
+
+class Graph:
+    """nsteps steps of several factored contexts captured once (one branch
+    per context) and replayed with one launch (lns3d_graph_*)."""
+
+    def __init__(self, steps, v_devs, dv_devs=None, nsteps=1, mode=MODE_FP64, corrections=1):
+        n = len(steps)
+        ctxs = (VP * n)(*[s.h.value for s in steps])
+        vs = (VP * n)(*[ptr(v) for v in v_devs])
+        dvs = (VP * n)(*[ptr(d) for d in dv_devs]) if dv_devs is not None else None
+        h = VP()
+        check(lib().lns3d_graph_create(ctxs, vs, dvs, n, nsteps, mode, corrections, ctypes.byref(h)),
+              "lns3d_graph_create")
+        self.h = h
+        self._keep = (steps, v_devs, dv_devs)
+
+    def launch(self, stream=None):
+        raw = getattr(stream, "cuda_stream", stream)
+        check(lib().lns3d_graph_launch(self.h, raw), "lns3d_graph_launch")
+
+    def synchronize(self):
+        check(lib().lns3d_graph_synchronize(self.h), "lns3d_graph_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lns3d_graph_destroy(self.h)
             self.h = None
 
     def __del__(self):

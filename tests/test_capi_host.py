@@ -132,3 +132,22 @@ def test_f3_shipped_index_slip():
         assert relerr(xs[0][:, s, :], xd) < 1e-13
         assert relerr(xo[:, s, :], xd) < 1e-13
         assert relerr(xs[1][:, s, :], xd) > 1e-6
+
+
+def test_request_and_instance_draws():
+    """serving requests and batch instances: seeded, in range, Poisson
+    arrivals with the requested mean rate"""
+    from lns3d_amd.serve import draw_requests
+    from lns3d_amd.batch import draw_instances
+    a = draw_requests(400, 50.0, seed=3)
+    b = draw_requests(400, 50.0, seed=3)
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 50.0) < 0.15 / 50.0
+    assert all(0 <= r.beta <= 1 and 0.01 <= r.omega <= 0.2 and r.steps % 10 == 0 for r in a)
+    assert all(r.arrival == 0.0 for r in draw_requests(5, None, seed=1))
+    inst = draw_instances(64, seed=0)
+    assert len(inst) == 64
+    assert all(65 <= p.nx <= 513 and 33 <= p.ny <= 257 for p in inst)
+    assert all(0.3 <= p.Ma <= 0.8 and 500 <= p.Re <= 3000 for p in inst)
+    assert len({(p.nx, p.ny) for p in inst}) > 32
