@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep leg")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch 5 xi solves, 5 eta solves, 5 steps and a calibration copy of "
                          "one solve's algorithmic bytes, then exit (for rocprofv3 --pmc runs)")
@@ -202,10 +204,46 @@ def main():
             st.step_fp32(vf, dvr)
         torch.cuda.synchronize()
         t_f32 = time.perf_counter() - t0
+        # one correction per step, no host synchronisation (the graph form)
+        vx = dev(v0)
+        st.itrbc3d(vx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st.step_refined_fixed(vx, dvr, corrections=1)
+        torch.cuda.synchronize()
+        t_fix = time.perf_counter() - t0
+        st.step_refined_fixed(vx, dvr, corrections=1, check_residual=True)
+        res_fix = st.last_residual()
         out["fp32_refine"] = {"steps_s": round(nref / t_ref, 3), "iters_mean": float(np.mean(iters)),
                               "tol": 1e-10, "dv_rel_err_vs_fp64_step": err,
                               "first_step_iters": its, "first_step_resid": res,
-                              "fp32_no_refine_steps_s": round(args.steps / t_f32, 3)}
+                              "one_correction_steps_s": round(args.steps / t_fix, 3),
+                              "one_correction_resid": res_fix,
+                              "fp32_no_refine_steps_s": round(args.steps / t_f32, 3),
+                              "ratio_to_fp64": round((args.steps / t_fix) / steps_s, 3)}
+    st.close()
+
+    # ---- uneven batch: 64 instances x 20 refined steps in one graph launch ----
+    if not args.no_batch:
+        from lns3d_amd.batch import run_batch
+        b = run_batch(count=64, nsteps=20, seed=0)
+        out["batch"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()}
+
+    # ---- serving sweep: closed-loop peak, then Poisson arrivals at 0.8 x peak ----
+    if not args.no_sweep:
+        from lns3d_amd.serve import sweep, serial_fp64, draw_requests
+        sw = sweep(count=32, seed=0)
+        ser = serial_fp64(draw_requests(32, None, 0))
+        out["sweep"] = {"requests": 32, "mode": "FP32 factors + one FP64 correction per step",
+                        "peak_solves_s": round(sw["peak"]["solves_per_s"], 3),
+                        "arrival_rate_s": round(sw["rate"], 3),
+                        "solves_s": round(sw["open_loop"]["solves_per_s"], 3),
+                        "p50_ms": round(sw["open_loop"]["p50_ms"], 2),
+                        "p99_ms": round(sw["open_loop"]["p99_ms"], 2),
+                        "serial_fp64": {"solves_s": round(ser["solves_per_s"], 3),
+                                        "p50_ms": round(ser["p50_ms"], 2),
+                                        "p99_ms": round(ser["p99_ms"], 2)}}
 
     # ---- CPU baseline ----
     if not args.no_cpu:
