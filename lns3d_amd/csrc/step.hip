@@ -606,15 +606,15 @@ int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *m
 
 // ==========================================================================
 // RHS: c1 = d/dxi v first (stored), then the point operator
+// one lane per (point, component), so a wave's stencil loads are contiguous
 __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v, double2 *c1) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)nx * ny) return;
-  int i = q % nx, j = q / nx;
-  const double2 *base = v + 5L * nx * j;
-  for (int e = 0; e < 5; e++) {
-    z g = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, base + e, 5);
-    c1[5 * q + e] = make_double2(g.re, g.im);
-  }
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 5L * nx * ny) return;
+  const long q = t / 5;
+  const int e = (int)(t - 5 * q);
+  const int i = q % nx, j = q / nx;
+  z g = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, v + 5L * nx * j + e, 5);
+  c1[t] = make_double2(g.re, g.im);
 }
 
 // The point operator works on tiles of RT consecutive grid points with one
@@ -643,25 +643,33 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
     sv[tid] = v[5 * q0 + tid];
     sg1[tid] = c1a[5 * q0 + tid];
   }
+  // stencil derivatives into LDS, one lane per (point, component): slots
+  // 0-4 U_eta, 5-8 U_xixi, 9-12 U_xieta, 13-16 U_etaeta (components 1-4);
+  // the task ranges of the four kinds are multiples of 64, so each wave
+  // works on one kind at a time
+  for (int tsk = tid; tsk < 5 * RT + 3 * 4 * RT; tsk += 5 * 64) {
+    int kind, pt, e;
+    if (tsk < 5 * RT) { kind = 0; pt = tsk / 5; e = tsk - 5 * pt; }
+    else { const int u = tsk - 5 * RT; kind = 1 + u / (4 * RT); const int w4 = u % (4 * RT); pt = w4 / 4; e = 1 + w4 % 4; }
+    if (pt >= npt) continue;
+    const long qq = q0 + pt;
+    const int ii = (int)(qq % nx), jj = (int)(qq / nx);
+    z h;
+    if (kind == 0) {
+      h = d1<double2, z>(ny, p.yper, jj, st.gy, st.dyinv, v + 5L * ii + e, 5L * nx);
+      if (e == 4 && (p.wallt == 1 || p.wallt == 2) && jj == 0) h = z{0, 0};   // src/cgradbc.f90:15
+    } else if (kind == 1) {
+      h = d2<double2, z>(nx, p.xper, ii, st.dx, st.dxsinv, v + 5L * nx * jj + e, 5);
+    } else if (kind == 2) {
+      h = d1<double2, z>(ny, p.yper, jj, st.gy, st.dyinv, c1a + 5L * ii + e, 5L * nx);
+    } else {
+      h = d2<double2, z>(ny, p.yper, jj, st.dy, st.dysinv, v + 5L * ii + e, 5L * nx);
+    }
+    const int sl = kind == 0 ? e : 4 * kind + e;
+    sd[sl * RT + pt] = make_double2(h.re, h.im);
+  }
   const long q = q0 + l;
   const int i = (int)(q % nx), j = (int)(q / nx);
-  if (l < npt) {
-    const double2 *colv = v + 5L * i, *colg = c1a + 5L * i, *rowv = v + 5L * nx * j;
-    for (int sl = w; sl < NDER; sl += 5) {   // slot index is uniform across the wave
-      z h;
-      if (sl < 5) {
-        h = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colv + sl, 5L * nx);
-        if (sl == 4 && (p.wallt == 1 || p.wallt == 2) && j == 0) h = z{0, 0};   // src/cgradbc.f90:15
-      } else if (sl < 9) {
-        h = d2<double2, z>(nx, p.xper, i, st.dx, st.dxsinv, rowv + (sl - 4), 5);
-      } else if (sl < 13) {
-        h = d1<double2, z>(ny, p.yper, j, st.gy, st.dyinv, colg + (sl - 8), 5L * nx);
-      } else {
-        h = d2<double2, z>(ny, p.yper, j, st.dy, st.dysinv, colv + (sl - 12), 5L * nx);
-      }
-      sd[sl * RT + l] = make_double2(h.re, h.im);
-    }
-  }
   __syncthreads();
   if (l < npt) {
     const int m = w;
@@ -742,7 +750,7 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
                void *work, int with_rbe, hipStream_t s) {
   long P = (long)p.nx * p.ny;
-  hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
+  hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(5 * P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
                      (const double2 *)v, (double2 *)work);
   hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, RT)), dim3(5 * 64), 0, s, p, st, c, (const double2 *)v,
                      (const double2 *)work, (double2 *)r, with_rbe);
