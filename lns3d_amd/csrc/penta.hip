@@ -1017,6 +1017,347 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
 }
 
 // ---------------------------------------------------------------------------
+// periodic lines (cpenta1p.f / cpenta2p.f): records from the bordered LU of
+// k_lu_lines and a streaming solve.  The LU's band gives the same recurrence
+// as non-periodic lines; the border (rows n-2, n-1 in every column: per2;
+// columns n-2, n-1 in every row: per) becomes two more per-row records:
+//   p2[row c] = (Linv_{n-2} per2(n-2,c), Linv_{n-1} per2(n-1,c)): the forward
+//     contributions of y_c to the last two rows, summed by the loader waves;
+//   qp[row i] = (-Uinv_i per(i,n-2), -Uinv_i per(i,n-1)): the backward terms
+//     in x_{n-2}, x_{n-1}, which are known first, folded into d_i.
+__device__ __forceinline__ void inblock_inverses(const Blk<double> &D, cx<double> (&Li)[5][5],
+                                                 cx<double> (&Ui)[5][5]) {
+  for (int r = 0; r < 5; r++)
+    for (int c = 0; c < 5; c++) { Li[r][c] = cx<double>{double(r == c), 0}; Ui[r][c] = cx<double>{0, 0}; }
+  for (int nc = 0; nc < 4; nc++)
+    for (int l = nc + 1; l < 5; l++)
+      for (int c = 0; c <= nc; c++) Li[l][c] = cfma(D(l, nc), Li[nc][c], Li[l][c]);
+  for (int c = 0; c < 5; c++)
+    for (int m = c; m >= 0; m--) {
+      cx<double> acc = cx<double>{double(m == c), 0};
+      for (int q = m + 1; q <= c; q++) acc = cfma(-D(m, q), Ui[q][c], acc);
+      Ui[m][c] = cdiv(acc, D(m, m));
+    }
+}
+
+__device__ __noinline__ void records_per_row(LineGeom g, cx<double> *mat, cx<double> *per, cx<double> *per2,
+                                             cx<double> *fwd, cx<double> *bwd, cx<double> *p2,
+                                             cx<double> *qp, long idx) {
+  const int n = g.n;
+  const long line = idx / n;
+  const int i = (int)(idx - line * n);
+  Blk<double> D, M;
+  cx<double> Li[5][5], Ui[5][5];
+  blk(g, mat, per, per2, line, i, i, D);
+  inblock_inverses(D, Li, Ui);
+  cx<double> *F = fwd + idx * 60, *B = bwd + idx * 66, *P2 = p2 + idx * 50, *QP = qp + idx * 50;
+  auto lmul = [&](const cx<double> (&A)[5][5], int row, int col, cx<double> *dst, int ld) {
+    bool ok = col >= 0 && col < n && blk(g, mat, per, per2, line, row, col, M);
+    for (int m = 0; m < 5; m++)
+      for (int q = 0; q < 5; q++) {
+        cx<double> acc{0, 0};
+        if (ok)
+          for (int k = 0; k <= m; k++) acc = cfma(A[m][k], M(k, q), acc);
+        dst[m * ld + q] = acc;
+      }
+  };
+  auto umul = [&](int col, cx<double> *dst, int ld) {
+    bool ok = col >= 0 && col < n && blk(g, mat, per, per2, line, i, col, M);
+    for (int m = 0; m < 5; m++)
+      for (int q = 0; q < 5; q++) {
+        cx<double> acc{0, 0};
+        if (ok)
+          for (int k = m; k < 5; k++) acc = cfma(Ui[m][k], M(k, q), acc);
+        dst[m * ld + q] = cx<double>{-acc.re, -acc.im};
+      }
+  };
+  // band blocks only (columns i-1, i-2 / i+1, i+2 within the line)
+  if (i >= 1) lmul(Li, i, i - 1, F, 10); else lmul(Li, i, -1, F, 10);
+  if (i >= 2) lmul(Li, i, i - 2, F + 5, 10); else lmul(Li, i, -1, F + 5, 10);
+  for (int l = 1, k = 0; l < 5; l++)
+    for (int q = 0; q < l; q++, k++) F[50 + k] = Li[l][q];
+  umul(i + 1 < n ? i + 1 : -1, B, 10);
+  umul(i + 2 < n ? i + 2 : -1, B + 5, 10);
+  for (int m = 0, k = 0; m < 5; m++)
+    for (int q = m; q < 5; q++, k++) B[50 + k] = Ui[m][q];
+  B[65] = cx<double>{0, 0};
+  // border columns of row i (per exists for i < col-2)
+  umul(i < n - 4 ? n - 2 : -1, QP, 5);
+  umul(i < n - 3 ? n - 1 : -1, QP + 25, 5);
+  // border rows in column i (per2 exists for i < row-2), with those rows' Linv
+  cx<double> Lb[5][5], Ub[5][5];
+  blk(g, mat, per, per2, line, n - 2, n - 2, D);
+  inblock_inverses(D, Lb, Ub);
+  lmul(Lb, n - 2, i < n - 4 ? i : -1, P2, 5);
+  blk(g, mat, per, per2, line, n - 1, n - 1, D);
+  inblock_inverses(D, Lb, Ub);
+  lmul(Lb, n - 1, i < n - 3 ? i : -1, P2 + 25, 5);
+}
+
+__global__ void k_records_per(LineGeom g, cx<double> *mat, cx<double> *per, cx<double> *per2,
+                              cx<double> *fwd, cx<double> *bwd, cx<double> *p2, cx<double> *qp) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < g.L * g.n) records_per_row(g, mat, per, per2, fwd, bwd, p2, qp, idx);
+}
+
+// streaming solve of periodic lines: G lines per workgroup, one per DPP row
+// of the compute wave; rows 0..n-3 in chunks of R through the LDS ring.
+//   forward: rows 0..n-3 on the band, then rows n-2, n-1 with the border sums
+//            the loaders formed from p2 and the computed y;
+//   backward: rows n-1, n-2 first, then rows n-3..0 with d_i including
+//            qp_i (x_{n-2}, x_{n-1}); finally x(np) = x(1) (cpenta1p.f:1037).
+template <int R, int G>
+__global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<double> *__restrict__ fwd,
+                                                  const cx<double> *__restrict__ bwd,
+                                                  const cx<double> *__restrict__ p2,
+                                                  const cx<double> *__restrict__ qp, double2 *rhs,
+                                                  long sL, long sR) {
+  using T = double;
+  constexpr int RECW = 66;
+  constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
+  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][G][BUFE];
+  __shared__ cx<T> cb[2][G][R][5];
+  __shared__ cx<T> bsum[G][2][5];     // forward border sums into rows n-2, n-1
+  constexpr int NSUB = 192 / (10 * G);  // loader threads per border sum
+  __shared__ cx<T> bpart[NSUB][10 * G];
+  __shared__ cx<T> xb[G][2][5];       // x_{n-2}, x_{n-1}
+  const long line0 = (long)blockIdx.x * G;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = (lane >> 4) % G;
+  const int m = min(lane & 15, 4);
+  auto line_of = [&](int gg) { return min(line0 + gg, L - 1); };
+  const long line = line_of(g);
+  const long rs = sR * 5;
+  double2 *X = rhs + line * sL * 5;
+  const int nreg = n - 2;                         // rows 0..n-3: band recurrence only
+  const int nck = (nreg + R - 1) / R;
+  auto zx = [](double2 a) { return cx<T>{a.x, a.y}; };
+
+  // ---- loaders: record rows [r0, r0+nr) of pass fw into buffer kk, with
+  // c = Linv b (forward) or d = Uinv y + qp (x_{n-2}, x_{n-1}) (backward)
+  auto load = [&](int kk, int r0, int nr, bool fw, bool border) {
+    const int stride = fw ? 60 : 66;
+    const cx<T> *base = fw ? fwd : bwd;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+      const long r = line_of(gg) * n + r0;
+      copy_rows_to_lds(base + r * stride, (L * n - r) * stride, rec[kk][gg], nr * stride, wave - 1, 3, lane);
+    }
+    for (int t = tid - 64; t < G * nr * 5; t += 192) {
+      const int gg = t / (nr * 5), rem = t - gg * nr * 5, row = rem / 5, mm = rem - row * 5;
+      const long ln = line_of(gg), ri = ln * n + r0 + row;
+      const double2 *xr = rhs + ln * sL * 5 + (long)(r0 + row) * rs;
+      cx<T> acc{0, 0};
+      if (fw) {
+        const cx<T> *Li = fwd + ri * 60 + 50 + mm * (mm - 1) / 2;
+        acc = zx(xr[mm]);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q < mm) acc = cfma(Li[q], zx(xr[q]), acc);
+      } else {
+        const cx<T> *Ui = bwd + ri * 66 + 50 + mm * 5 - mm * (mm - 1) / 2;
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+          if (q < 5 - mm) acc = cfma(Ui[q], zx(xr[mm + q]), acc);
+        if (border) {
+          const cx<T> *Q = qp + ri * 50 + mm * 5;
+#pragma unroll
+          for (int q = 0; q < 5; q++) acc = cfma(Q[q], xb[gg][0][q], acc);
+#pragma unroll
+          for (int q = 0; q < 5; q++) acc = cfma(Q[25 + q], xb[gg][1][q], acc);
+        }
+      }
+      cb[kk][gg][row][mm] = acc;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  };
+  // loader threads keep partial forward border sums of (line gg, border row
+  // bt, component mm) over the rows the compute wave finished: NSUB threads
+  // per sum, each taking every NSUB-th row of a chunk
+  const int at = tid - 64;
+  const bool acc_thread = at >= 0 && at < NSUB * 10 * G;
+  const int aid = acc_thread ? at % (10 * G) : 0, asub = acc_thread ? at / (10 * G) : 0;
+  const int agg = aid / 10, abt = (aid / 5) % 2, amm = aid % 5;
+  cx<T> bacc{0, 0};
+  auto accumulate = [&](int r0, int nr) {
+    if (!acc_thread) return;
+    const long ln = line_of(agg);
+    for (int row = asub; row < nr; row += NSUB) {
+      const long ri = ln * n + r0 + row;
+      const double2 *yr = rhs + ln * sL * 5 + (long)(r0 + row) * rs;
+      const cx<T> *P = p2 + ri * 50 + abt * 25 + amm * 5;
+#pragma unroll
+      for (int q = 0; q < 5; q++) bacc = cfma(P[q], zx(yr[q]), bacc);
+    }
+  };
+
+  // ---- compute wave
+  cx<T> yA[5], yB[5];
+  cx<T> w2s[3][5], w1s[3][5], cs[3];
+  auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
+    cx<T> t0 = cmul(w[0], yn[0]), t1 = cmul(w[1], yn[1]), t2 = cmul(w[2], yn[2]),
+          t3 = cmul(w[3], yn[3]), t4 = cfma(w[4], yn[4], a0);
+    return (t0 + t1) + ((t2 + t3) + t4);
+  };
+  auto pre0 = [&](const cx<T> (&w)[5], cx<T> c, const cx<T> (&yo)[5]) -> cx<T> {
+    cx<T> a = cfma(w[0], yo[0], c), b = cmul(w[1], yo[1]);
+    a = cfma(w[2], yo[2], a);
+    b = cfma(w[3], yo[3], b);
+    a = cfma(w[4], yo[4], a);
+    return a + b;
+  };
+  // rows of buffer kk in processing order j (buffer row = j forward,
+  // nr-1-j backward); extra(j) is added to the row's known part
+  // two rows on the plain path: the border rows of either pass; extra adds
+  // the forward border sums
+  auto border_rows = [&](int kk, int ibase, bool fw, bool extra) {
+    const int stride = fw ? 60 : 66;
+    const cx<T> *B = rec[kk][g];
+    for (int j = 0; j < 2; j++) {
+      const int r = fw ? j : 1 - j;
+      cx<T> w1[5], w2[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) { w2[q] = B[r * stride + m * 10 + q]; w1[q] = B[r * stride + m * 10 + 5 + q]; }
+      cx<T> a = pre0(w1, cb[kk][g][r][m], yA);
+      if (extra) a = a + bsum[g][j][m];
+      cx<T> y = crit(w2, yB, a);
+      X[(long)(fw ? ibase + j : ibase - j) * rs + m] = make_double2(y.re, y.im);
+      row_bcast5(y, yA);
+#pragma unroll
+      for (int q = 0; q < 5; q++) { cx<T> t = yA[q]; yA[q] = yB[q]; yB[q] = t; }
+    }
+  };
+  auto run_chunk = [&](int kk, int nr, int ibase, bool fw) {
+    const int stride = fw ? 60 : 66;
+    const cx<T> *B = rec[kk][g];
+    auto rix = [&](int j) { j = min(j, nr - 1); return fw ? j : nr - 1 - j; };
+    auto ldW2 = [&](int j, cx<T> (&w)[5]) {
+      const cx<T> *pp = B + rix(j) * stride + m * 10;
+#pragma unroll
+      for (int q = 0; q < 5; q++) w[q] = pp[q];
+    };
+    auto ldW1 = [&](int j, cx<T> (&w)[5], cx<T> &c) {
+      const int r = rix(j);
+      const cx<T> *pp = B + r * stride + m * 10 + 5;
+#pragma unroll
+      for (int q = 0; q < 5; q++) w[q] = pp[q];
+      c = cb[kk][g][r][m];
+    };
+    const long xstep = fw ? rs : -rs;
+    double2 *xp = X + (long)(fw ? ibase : ibase) * rs + m;
+    const int nmain = (nr / 6) * 6;
+    if (nmain > 0) {
+      ldW2(0, w2s[0]);
+      ldW2(1, w2s[1]);
+      ldW1(0, w1s[0], cs[0]);
+      ldW1(1, w1s[1], cs[1]);
+      ldW1(2, w1s[2], cs[2]);
+      cx<T> a0 = pre0(w1s[0], cs[0], yA);
+      auto sub = [&](int j, auto U) {
+        constexpr int u = decltype(U)::value;
+        constexpr int u0 = u % 3, u1 = (u + 1) % 3, u2 = (u + 2) % 3;
+        cx<T>(&yn)[5] = (u % 2 == 0) ? yB : yA;
+        cx<T>(&yd)[5] = (u % 2 == 0) ? yA : yB;
+        cx<T> y = crit(w2s[u0], yn, a0);
+        a0 = pre0(w1s[u1], cs[u1], yn);
+        *xp = make_double2(y.re, y.im);
+        xp += xstep;
+        ldW2(j + 2, w2s[u2]);
+        ldW1(j + 3, w1s[u0], cs[u0]);
+        row_bcast5(y, yd);
+      };
+      for (int j = 0; j < nmain; j += 6) {
+        sub(j, std::integral_constant<int, 0>());
+        sub(j + 1, std::integral_constant<int, 1>());
+        sub(j + 2, std::integral_constant<int, 2>());
+        sub(j + 3, std::integral_constant<int, 3>());
+        sub(j + 4, std::integral_constant<int, 4>());
+        sub(j + 5, std::integral_constant<int, 5>());
+      }
+    }
+    for (int j = nmain; j < nr; j++) {
+      cx<T> w1[5], w2[5], c;
+      ldW1(j, w1, c);
+      ldW2(j, w2);
+      cx<T> y = crit(w2, yB, pre0(w1, c, yA));
+      *xp = make_double2(y.re, y.im);
+      xp += xstep;
+      row_bcast5(y, yA);
+#pragma unroll
+      for (int q = 0; q < 5; q++) { cx<T> t = yA[q]; yA[q] = yB[q]; yB[q] = t; }
+    }
+  };
+
+  // ================= forward
+#pragma unroll
+  for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
+  if (wave != 0) load(0, 0, min(R, nreg), true, false);
+  __syncthreads();
+  for (int k = 0; k < nck; k++) {
+    const int r0 = k * R, nr = min(R, nreg - r0);
+    if (wave == 0) {
+      run_chunk(k & 1, nr, r0, true);
+    } else {
+      if (k >= 1) accumulate(r0 - R, R);
+      if (k + 1 < nck) load((k + 1) & 1, r0 + R, min(R, nreg - r0 - R), true, false);
+    }
+    __syncthreads();
+  }
+  // the last regular chunk's sums, the records of rows n-2, n-1
+  if (wave != 0) {
+    const int r0 = (nck - 1) * R;
+    accumulate(r0, nreg - r0);
+    if (acc_thread) bpart[asub][aid] = bacc;
+    load(nck & 1, nreg, 2, true, false);
+  }
+  __syncthreads();
+  if (tid >= 64 && tid < 64 + 10 * G) {
+    const int id = tid - 64;
+    cx<T> t{0, 0};
+    for (int k = 0; k < NSUB; k++) t = t + bpart[k][id];
+    bsum[id / 10][(id / 5) % 2][id % 5] = t;
+  }
+  __syncthreads();
+  if (wave == 0) border_rows(nck & 1, nreg, true, true);
+  __syncthreads();
+
+  // ================= backward
+#pragma unroll
+  for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
+  if (wave != 0) load(0, nreg, 2, false, false);
+  __syncthreads();
+  if (wave == 0) {
+    border_rows(0, n - 1, false, false);
+    // x_{n-1} is yA, x_{n-2} is yB (all components on every lane); pick
+    // component m with constant indices (a runtime index would move the
+    // arrays out of registers)
+    cx<T> xa = yB[0], xc = yA[0];
+#pragma unroll
+    for (int q = 1; q < 5; q++)
+      if (m == q) { xa = yB[q]; xc = yA[q]; }
+    if ((lane & 15) < 5) { xb[g][0][m] = xa; xb[g][1][m] = xc; }
+  }
+  __syncthreads();
+  if (wave != 0) {
+    const int rtop = nreg - 1, nr = min(R, rtop + 1);
+    load(1, rtop - nr + 1, nr, false, true);
+  }
+  __syncthreads();
+  for (int k = 1; k <= nck; k++) {
+    const int rtop = nreg - 1 - (k - 1) * R, nr = min(R, rtop + 1);
+    if (wave == 0) {
+      run_chunk(k & 1, nr, rtop, false);
+    } else if (k < nck) {
+      const int rt2 = rtop - nr, nr2 = min(R, rt2 + 1);
+      load((k + 1) & 1, rt2 - nr2 + 1, nr2, false, true);
+    }
+    __syncthreads();
+  }
+  // x(np) = x(1)
+  if (wave == 0 && (lane & 15) < 5) X[(long)n * rs + m] = X[m];
+}
+
+// ---------------------------------------------------------------------------
 // layout conversions between the reference's arrays and the working copy
 // ref mat (5,5,5,...) element (slot k, r, c) of (line l, row i) at
 //   ((i*sRm + l*sLm) * 125 + k + 5r + 25c)
@@ -1071,10 +1412,15 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
-  pl->fwd = pl->bwd = pl->tail = pl->mid = nullptr;
+  pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = nullptr;
   if (periodic) {
-    LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * sizeof(cx<double>)));
-    LNS3D_CHECK(hipMalloc(&pl->per2, (size_t)pl->np * 50 * lines * sizeof(cx<double>)));
+    const size_t z = sizeof(cx<double>);
+    LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * z));
+    LNS3D_CHECK(hipMalloc(&pl->per2, (size_t)pl->np * 50 * lines * z));
+    LNS3D_CHECK(hipMalloc(&pl->fwd, (size_t)pl->n * 60 * lines * z));
+    LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * z));
+    LNS3D_CHECK(hipMalloc(&pl->p2, (size_t)pl->n * 50 * lines * z));
+    LNS3D_CHECK(hipMalloc(&pl->qp, (size_t)pl->n * 50 * lines * z));
   } else {
     size_t es = prec ? sizeof(cx<float>) : sizeof(cx<double>);
     LNS3D_CHECK(hipMalloc(&pl->fwd, (size_t)pl->n * 60 * lines * es));
@@ -1093,7 +1439,9 @@ void penta_plan_destroy(PentaPlan *pl) {
   if (pl->bwd) (void)hipFree(pl->bwd);
   if (pl->tail) (void)hipFree(pl->tail);
   if (pl->mid) (void)hipFree(pl->mid);
-  pl->lu = pl->per = pl->per2 = pl->fwd = pl->bwd = pl->tail = pl->mid = nullptr;
+  if (pl->p2) (void)hipFree(pl->p2);
+  if (pl->qp) (void)hipFree(pl->qp);
+  pl->lu = pl->per = pl->per2 = pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = nullptr;
 }
 
 int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStream_t s) {
@@ -1127,6 +1475,11 @@ int penta_factor(PentaPlan *pl, hipStream_t s) {
     hipLaunchKernelGGL(k_lu_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
                        (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2);
     LNS3D_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_records_per, dim3(grid_for(pl->L * (long)pl->n, 128)), dim3(128), 0, s, g,
+                       (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2,
+                       (cx<double> *)pl->fwd, (cx<double> *)pl->bwd, (cx<double> *)pl->p2,
+                       (cx<double> *)pl->qp);
+    LNS3D_CHECK(hipGetLastError());
     return 0;
   }
   hipLaunchKernelGGL(k_factor_chains, dim3((unsigned)pl->L), dim3(pl->h >= 0 ? 128 : 64), 0, s, g,
@@ -1138,9 +1491,14 @@ int penta_factor(PentaPlan *pl, hipStream_t s) {
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
   if (pl->periodic) {
-    hipLaunchKernelGGL(k_solve_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
-                       (const cx<double> *)pl->lu, (const cx<double> *)pl->per,
-                       (const cx<double> *)pl->per2, (double2 *)rhs, sL, sR);
+    (void)g;
+    auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
+    auto *p2 = (const cx<double> *)pl->p2; auto *qp = (const cx<double> *)pl->qp;
+    // one line per workgroup while two workgroups per CU hold every line
+    if (pl->L > 512)
+      hipLaunchKernelGGL((k_solve_per<6, 4>), dim3((unsigned)((pl->L + 3) / 4)), dim3(256), 0, s, pl->n, pl->L, f, b, p2, qp, (double2 *)rhs, sL, sR);
+    else
+      hipLaunchKernelGGL((k_solve_per<30, 1>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, pl->L, f, b, p2, qp, (double2 *)rhs, sL, sR);
   } else {
     // twisted lines: one line (two chains) per workgroup; PENTA_PLAIN4: four
     // lines per workgroup (one per DPP row of the compute wave)

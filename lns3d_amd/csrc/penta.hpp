@@ -20,6 +20,7 @@ struct PentaPlan {
   void *lu = nullptr, *per = nullptr, *per2 = nullptr;
   void *fwd = nullptr, *bwd = nullptr, *tail = nullptr;
   void *mid = nullptr;   // twisted factor: the bottom chain's share of the middle block
+  void *p2 = nullptr, *qp = nullptr;   // periodic lines: border records (complex double)
 };
 
 // solve-kernel variants of non-periodic lines: one line per workgroup with a
