@@ -27,8 +27,12 @@
 //     records into a double-buffered LDS ring and compute the parallel parts
 //     (Linv b, Uinv y); wave 0 runs the serial recurrences, one line or one
 //     chain per 16-lane DPP row, lanes 0..4 owning one component each.
-//  Periodic lines are solved by k_solve_lines (one lane per line, the
-//  reference's order) on the LU working copy.
+//  Periodic lines: k_records_per forms the same records from the bordered
+//  LU plus two border blocks per row (p2: rows n-2, n-1 in column i; qp:
+//  row i in columns n-2, n-1), and k_solve_per streams them like k_solve_bc
+//  with the border sums formed by the loader waves.  k_solve_lines (one lane
+//  per line, the reference's substitution order on the LU working copy) is
+//  kept as the order-exact solve behind lns3d_penta_solve_reference_order.
 #include <map>
 #include <mutex>
 #include <tuple>
