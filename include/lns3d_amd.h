@@ -171,10 +171,13 @@ int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count,
 int lns3d_graph_launch(lns3d_graph *g, void *stream);
 int lns3d_graph_synchronize(lns3d_graph *g);
 int lns3d_graph_destroy(lns3d_graph *g);
-/* autotuner of the line solves: times every solve-kernel variant (one line
- * per workgroup, four lines per workgroup, twisted) on synthetic lines of n
- * rows x lines and uses the fastest for plans of that shape created later;
- * times_ms (may be NULL) receives the three timings (-1: not applicable) */
+/* autotuner of the line solves: times every solve-kernel variant on
+ * synthetic lines of n rows x lines and uses the fastest for plans of that
+ * shape created later.  Variants: 0 one line per workgroup, 1 four lines,
+ * 2 twisted (two chains per line, n >= 16), 3 two lines, 4 three lines.
+ * times_ms (may be NULL) receives LNS3D_TUNE_VARIANTS timings in ms (-1: not
+ * applicable).  No counterpart in the reference. */
+#define LNS3D_TUNE_VARIANTS 5
 int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *times_ms);
 
 /* Host-array driving of the time loop of src/impdrv3d.f90:126-406 (for

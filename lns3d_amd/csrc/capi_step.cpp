@@ -494,7 +494,8 @@ extern "C" int lns3d_graph_destroy(lns3d_graph *gr) {
 
 // autotuner of the line-solve kernels: measures every variant on synthetic
 // lines of n rows x lines and keeps the fastest for plans of that shape
-// created afterwards; times_ms (may be NULL) receives 3 timings (-1: n/a)
+// created afterwards; times_ms (may be NULL) receives LNS3D_TUNE_VARIANTS
+// timings (-1: n/a)
 extern "C" int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *times_ms) {
   if (n < 7 || lines < 1 || (precision != 0 && precision != 1))
     return fail(LNS3D_EINVAL, "lns3d_tune_lines: bad shape");

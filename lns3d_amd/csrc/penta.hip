@@ -514,10 +514,18 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
 // ---------------------------------------------------------------------------
 // stage 3: streaming solve of non-periodic lines
 //
-// LDS per buffer: nr record rows copied verbatim from HBM (fwd: 60, bwd: 65
-// complex per row) with global_load_lds_dwordx4 (no VGPR round trip), plus
-// the parallel parts c_i = Linv_i b_i / d_i = Uinv_i y_i computed by the
-// loader waves into cb.
+// A workgroup of four waves.  Wave 0 runs the recurrences; waves 1-3 stream
+// the records through a ring of NBUF = 4 LDS buffers.  Chunk k (R rows of
+// each line or chain of the workgroup) belongs to loader wave 1 + k % 3,
+// which
+//   * in iteration k - 3 starts its copies: the record rows verbatim
+//     (global_load_lds_dwordx4, no VGPR round trip; fwd 60, bwd 66 complex
+//     per row) and the right-hand-side rows gathered into a staging buffer;
+//   * in iteration k - 1 waits for them (the only copies it has in flight)
+//     and replaces each staged row by its parallel part c_i = Linv_i b_i
+//     (forward) or d_i = Uinv_i y_i (backward).
+// A chunk's copies so have two iterations to arrive while the compute wave
+// works through the chunks before it.
 typedef __attribute__((address_space(3))) void lds_void;
 
 template <typename T>
@@ -537,16 +545,102 @@ __device__ __forceinline__ void copy_rows_to_lds(const cx<T> *src, long src_elem
   }
 }
 
+// one wave copies nelem complex values src[0..nelem) into LDS at dst, 1 KB
+// (16 B per lane) per instruction; lanes past the end are masked off (nelem
+// is even for complex float, so no lane copies past the end)
+template <typename T>
+__device__ __forceinline__ void dma_rows(const cx<T> *src, cx<T> *dst, int nelem, int lane) {
+  constexpr int PER_LANE = 16 / sizeof(cx<T>);
+  for (int b = 0; b * 64 * PER_LANE < nelem; b++) {
+    const int e = (b * 64 + lane) * PER_LANE;
+    if (e < nelem)
+      __builtin_amdgcn_global_load_lds((const void *)(src + e), (lds_void *)(dst + b * 64 * PER_LANE),
+                                       16, 0, 0);
+  }
+}
+
+// the parallel part of one row, in place: bp[0..5) holds the row's
+// right-hand side (forward) or forward result (backward) in complex double,
+// rp the row's record.  Lane mm of the row overwrites bp[mm] with component
+// mm of c = Linv b (forward) or d = Uinv y (backward), as cx<T>.  The five
+// lanes of a row sit in one wave, so all of them have read the row before
+// any writes to it.
+template <typename T>
+__device__ __forceinline__ void parallel_part(double2 *bp, const cx<T> *rp, int mm, bool fw) {
+  cx<T> acc{0, 0};
+  if (fw) {
+    const cx<T> *Li = rp + 50 + mm * (mm - 1) / 2;
+    const double2 bm = bp[mm];
+    acc = cx<T>{(T)bm.x, (T)bm.y};
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q < mm) {
+        const double2 bq = bp[q];
+        acc = cfma(Li[q], cx<T>{(T)bq.x, (T)bq.y}, acc);
+      }
+  } else {
+    const cx<T> *Ui = rp + 50 + mm * 5 - mm * (mm - 1) / 2;
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+      if (q < 5 - mm) {
+        const double2 yq = bp[mm + q];
+        acc = cfma(Ui[q], cx<T>{(T)yq.x, (T)yq.y}, acc);
+      }
+  }
+  *reinterpret_cast<cx<T> *>(bp + mm) = acc;
+}
+
+// The workgroup barrier of the loader waves: their LDS writes are complete
+// (lgkmcnt 0) but the copies they have in flight are not waited for, as a
+// __syncthreads fence would (it treats global_load_lds as pending LDS
+// writes).  The empty asm statements keep the compiler from moving memory
+// accesses across the barrier.
+__device__ __forceinline__ void loader_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);     // lgkmcnt(0); vmcnt, expcnt not waited for
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// the loader side of one pass over nchunk chunks (loader wave w = 0, 1, 2):
+// one barrier after the prologue and one per chunk, as on the compute side
+// (compute_pass)
+template <typename Issue, typename Prep>
+__device__ __forceinline__ void stream_pass(int nchunk, int w, Issue issue, Prep prep) {
+  if (w < nchunk) issue(w);
+  if (w == 0) {
+    __builtin_amdgcn_s_waitcnt(0);
+    prep(0);
+  }
+  loader_barrier();
+  for (int k = 0; k < nchunk; k++) {
+    if (k + 1 < nchunk && (k + 1) % 3 == w) {
+      __builtin_amdgcn_s_waitcnt(0);      // this wave's copies of chunk k + 1
+      prep(k + 1);
+    }
+    if (k + 3 < nchunk && k % 3 == w) issue(k + 3);   // into the buffer chunk k - 1 left
+    loader_barrier();
+  }
+}
+
+template <typename Body>
+__device__ __forceinline__ void compute_pass(int nchunk, Body body) {
+  __syncthreads();
+  for (int k = 0; k < nchunk; k++) {
+    body(k);
+    __syncthreads();
+  }
+}
+
 template <typename T, int R, int G>
 __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__restrict__ fwd,
                                                  const cx<T> *__restrict__ bwd,
                                                  const cx<T> *__restrict__ tail, double2 *rhs,
                                                  long sL, long sR) {
-  constexpr int RECW = 66;                                       // LDS row stride (complex)
-  constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
-  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][G][BUFE];
-  __shared__ cx<T> cb[2][G][R][5];
-  const long total_rows = L * n;
+  constexpr int NBUF = 4, BUFE = R * 66;                        // LDS record rows of one line
+  static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
+  __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][G][BUFE];
+  __shared__ double2 bst[NBUF][G][R][5];    // rhs rows, then c (or d) in place
   const long line0 = (long)blockIdx.x * G;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -562,34 +656,56 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
   double2 *X = rhs + line * sL * 5;
   const long rs = sR * 5;
   const cx<T> *tl = tail + line * 150;
-
-  // ---------------- forward: y_i = c_i + N1 y_{i-2} + N2 y_{i-1} ----------
-  auto load_fwd = [&](int k) {
-    int r0 = k * R, nr = min(R, n - r0);
-#pragma unroll
-    for (int gg = 0; gg < G; gg++) {
-      const long ln = line_of(gg);
-      copy_rows_to_lds(fwd + (ln * n + r0) * 60, (total_rows - (ln * n + r0)) * 60, rec[k & 1][gg],
-                       nr * 60, wave - 1, 3, lane);
+  // rows of chunk k, ascending as they sit in LDS: [lo, lo + nr); forward
+  // chunks run up from row 0, backward chunks down from row n-1
+  auto chunk = [&](int k, bool fw, int &lo, int &nr) {
+    if (fw) {
+      lo = k * R;
+      nr = min(R, n - lo);
+    } else {
+      const int top = n - 1 - k * R;
+      nr = min(R, top + 1);
+      lo = top - nr + 1;
     }
-    for (int t = tid - 64; t < G * nr * 5; t += 192) {
-      int gg = t / (nr * 5), rem = t - gg * nr * 5;
-      int row = rem / 5, mm = rem - row * 5;
-      const long ln = line_of(gg);
-      const double2 *b = rhs + ln * sL * 5 + (long)(r0 + row) * rs;
-      const cx<T> *Li = fwd + (ln * n + r0 + row) * 60 + 50 + mm * (mm - 1) / 2;
-      double2 bm = b[mm];
-      cx<T> acc{(T)bm.x, (T)bm.y};
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (q < mm) {
-          double2 bq = b[q];
-          acc = cfma(Li[q], cx<T>{(T)bq.x, (T)bq.y}, acc);
-        }
-      cb[k & 1][gg][row][mm] = acc;
-    }
-    __builtin_amdgcn_s_waitcnt(0);
   };
+
+  if (wave != 0) {
+    // ---- loader waves ----
+    auto issue = [&](int k, bool fw) {
+      const int buf = k % NBUF;
+      int lo, nr;
+      chunk(k, fw, lo, nr);
+#pragma unroll
+      for (int gg = 0; gg < G; gg++) {
+        const long r0 = line_of(gg) * n + lo;
+        if (fw) dma_rows(fwd + r0 * 60, rec[buf][gg], nr * 60, lane);
+        else dma_rows(bwd + r0 * 66, rec[buf][gg], nr * 66, lane);
+      }
+      double2 *dst = &bst[buf][0][0][0];
+#pragma unroll
+      for (int b = 0; b * 64 < G * R * 5; b++) {
+        const int e = b * 64 + lane, gg = e / (R * 5), j = (e / 5) % R, q = e % 5;
+        if (e < G * R * 5 && j < nr)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(rhs + line_of(gg) * sL * 5 + (long)(lo + j) * rs + q),
+              (lds_void *)(dst + b * 64), 16, 0, 0);
+      }
+    };
+    auto prep = [&](int k, bool fw) {
+      const int buf = k % NBUF;
+      int lo, nr;
+      chunk(k, fw, lo, nr);
+      for (int r0 = 0; r0 < G * R; r0 += 12) {     // 12 rows (60 lanes) at a time
+        const int rr = r0 + lane / 5, gg = rr / R, j = rr % R;
+        if (lane < 60 && rr < G * R && j < nr)
+          parallel_part(bst[buf][gg][j], rec[buf][gg] + j * (fw ? 60 : 66), lane % 5, fw);
+      }
+    };
+    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
+    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, false); }, [&](int k) { prep(k, false); });
+    return;
+  }
+
   // ---- compute wave: the recurrence on lanes 0..4 (lane m = component m)
   // Row i is split into the part that needs the newest value,
   //   y_i = a0_i + W2_i * y_{i-1}          (products summed as a tree)
@@ -646,8 +762,6 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
       for (int q = 0; q < 5; q++) s1[q] = y[q];
   };
   // one chunk: rows of buffer kk in processing order j = 0..nr-1 (row in
-  // the buffer = j forward, nr-1-j backward; grid row = ibase +/- j)
-  // one chunk: rows of buffer kk in processing order j = 0..nr-1 (row in
   // the buffer = j forward, nr-1-j backward; grid row = ibase +/- j).  Rows
   // are processed in branch-free groups of six; the remainder and the last
   // five rows of a pass (which carry the boundary-row blocks) take a plain
@@ -665,7 +779,7 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
       const cx<T> *p = B + r * stride + m * 10 + 5;
 #pragma unroll
       for (int q = 0; q < 5; q++) w[q] = p[q];
-      c = cb[kk][g][r][m];
+      c = *reinterpret_cast<const cx<T> *>(&bst[kk][g][r][m]);
     };
     int before_tail = fwd_pass ? n - 5 - ibase : ibase - 4;
     before_tail = max(0, min(nr, before_tail));
@@ -716,61 +830,22 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
     }
   };
 
-  // ---------------- forward: y_i = c_i + N1 y_{i-2} + N2 y_{i-1} ----------
+  // forward: y_i = c_i + N1 y_{i-2} + N2 y_{i-1}; backward: x_i = d_i +
+  // Q4 x_{i+1} + Q5 x_{i+2}
 #pragma unroll
   for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
-  if (wave != 0) load_fwd(0);
-  __syncthreads();
-  for (int k = 0; k < nchunk; k++) {
-    if (wave == 0) {
-      int r0 = k * R, nr = min(R, n - r0);
-      run_chunk(rec[k & 1][g], 60, k & 1, nr, r0, true);
-    } else if (k + 1 < nchunk) {
-      load_fwd(k + 1);
-    }
-    __syncthreads();
-  }
-
-  // ---------------- backward: x_i = d_i + Q4 x_{i+1} + Q5 x_{i+2} ---------
-  // chunk k holds rows rlo..rtop in ascending order in LDS
-  auto load_bwd = [&](int k) {
-    int rtop = n - 1 - k * R, nr = min(R, rtop + 1), rlo = rtop - nr + 1;
-#pragma unroll
-    for (int gg = 0; gg < G; gg++) {
-      const long ln = line_of(gg);
-      copy_rows_to_lds(bwd + (ln * n + rlo) * 66, (total_rows - (ln * n + rlo)) * 66, rec[k & 1][gg],
-                       nr * 66, wave - 1, 3, lane);
-    }
-    for (int t = tid - 64; t < G * nr * 5; t += 192) {
-      int gg = t / (nr * 5), rem = t - gg * nr * 5;
-      int row = rem / 5, mm = rem - row * 5;     // row counted from rlo
-      const long ln = line_of(gg);
-      const double2 *y = rhs + ln * sL * 5 + (long)(rlo + row) * rs;
-      const cx<T> *Ui = bwd + (ln * n + rlo + row) * 66 + 50 + mm * 5 - mm * (mm - 1) / 2;
-      cx<T> acc{0, 0};
-#pragma unroll
-      for (int q = 0; q < 5; q++)
-        if (q < 5 - mm) {
-          double2 yq = y[mm + q];
-          acc = cfma(Ui[q], cx<T>{(T)yq.x, (T)yq.y}, acc);
-        }
-      cb[k & 1][gg][row][mm] = acc;
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-  };
+  compute_pass(nchunk, [&](int k) {
+    int lo, nr;
+    chunk(k, true, lo, nr);
+    run_chunk(rec[k % NBUF][g], 60, k % NBUF, nr, lo, true);
+  });
 #pragma unroll
   for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
-  if (wave != 0) load_bwd(0);
-  __syncthreads();
-  for (int k = 0; k < nchunk; k++) {
-    if (wave == 0) {
-      int rtop = n - 1 - k * R, nr = min(R, rtop + 1);
-      run_chunk(rec[k & 1][g], 66, k & 1, nr, rtop, false);
-    } else if (k + 1 < nchunk) {
-      load_bwd(k + 1);
-    }
-    __syncthreads();
-  }
+  compute_pass(nchunk, [&](int k) {
+    int lo, nr;
+    chunk(k, false, lo, nr);
+    run_chunk(rec[k % NBUF][g], 66, k % NBUF, nr, lo + nr - 1, false);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -788,13 +863,12 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
                                                  const cx<T> *__restrict__ tail, double2 *rhs,
                                                  long sL, long sR) {
   constexpr int C = 2 * G;
-  constexpr int RECW = 66;
-  constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
-  __shared__ __attribute__((aligned(1024))) cx<T> rec[2][C][BUFE];
-  __shared__ cx<T> cb[2][C][R][5];
+  constexpr int NBUF = 4, BUFE = R * 66;
+  static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
+  __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][C][BUFE];
+  __shared__ double2 bst[NBUF][C][R][5];   // rhs rows, then c (or d) in place
   __shared__ cx<T> xch[G][2][5];     // values handed between the chains of a line
   __shared__ cx<T> kept[C][2][5];    // x at chain rows 3, 4 (backward boundary blocks)
-  const long total_rows = L * n;
   const long line0 = (long)blockIdx.x * G;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -804,16 +878,18 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
   auto line_of = [&](int gg) { return min(line0 + gg, L - 1); };
   const long line = line_of(g);
   const int len0 = h + 2, len1 = n - h - 2;          // len0 >= len1 + 2
-  auto clen = [&](int tt) { return tt == 0 ? len0 : len1; };
-  auto coff = [&](int tt) { return tt == 0 ? 0 : len0; };
-  auto real_row = [&](int tt, int j) { return tt == 0 ? j : n - 1 - j; };
+  // chain tt (0 or 1) in arithmetic form: a select here becomes a per-lane
+  // table in scratch, whose loads would wait on the loaders' copies
+  auto clen = [&](int tt) { return len0 + tt * (len1 - len0); };
+  auto coff = [&](int tt) { return tt * len0; };
+  auto real_row = [&](int tt, int j) { return j + tt * (n - 1 - 2 * j); };
   const int len = clen(t);
   const long rs = sR * 5;
   double2 *X = rhs + line * sL * 5;
   const cx<T> *tl = tail + (line * 2 + t) * 150;
   const int nchunk = (len0 + R - 1) / R;             // steps per pass: len0
   // chain row at step s of the backward pass
-  auto bj = [&](int tt, int s) { return tt == 0 ? len0 - 1 - s : len1 + 1 - s; };
+  auto bj = [&](int tt, int s) { return len0 - 1 - s + tt * (len1 + 2 - len0); };
   // rows of chunk k held in the LDS buffer for chain tt: [jlo, jlo + nr)
   auto chunk_rows = [&](int tt, int k, bool fw, int &jlo, int &nr) {
     const int l = clen(tt);
@@ -827,52 +903,44 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
     }
   };
 
-  auto load = [&](int k, bool fw) {
-    const int stride = fw ? 60 : 66;
-    const cx<T> *base = fw ? fwd : bwd;
+  if (wave != 0) {
+    // ---- loader waves ----
+    auto issue = [&](int k, bool fw) {
+      const int buf = k % NBUF;
 #pragma unroll
-    for (int cc = 0; cc < C; cc++) {
-      int jlo, nr;
-      chunk_rows(cc & 1, k, fw, jlo, nr);
-      if (nr == 0) continue;
-      const long r0 = line_of(cc >> 1) * n + coff(cc & 1) + jlo;
-      copy_rows_to_lds(base + r0 * stride, (total_rows - r0) * stride, rec[k & 1][cc], nr * stride,
-                       wave - 1, 3, lane);
-    }
-    for (int task = tid - 64; task < C * R * 5; task += 192) {
-      const int cc = task / (R * 5), rem = task - cc * R * 5;
-      const int row = rem / 5, mm = rem - row * 5;
-      int jlo, nr;
-      chunk_rows(cc & 1, k, fw, jlo, nr);
-      if (row >= nr) continue;
-      const int j = jlo + row;
-      const long ln = line_of(cc >> 1);
-      const double2 *xr = rhs + ln * sL * 5 + (long)real_row(cc & 1, j) * rs;
-      const cx<T> *recp = base + (ln * n + coff(cc & 1) + j) * stride;
-      cx<T> acc{0, 0};
-      if (fw) {                      // c = Linv b
-        const cx<T> *Li = recp + 50 + mm * (mm - 1) / 2;
-        double2 bm = xr[mm];
-        acc = cx<T>{(T)bm.x, (T)bm.y};
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (q < mm) {
-            double2 bq = xr[q];
-            acc = cfma(Li[q], cx<T>{(T)bq.x, (T)bq.y}, acc);
-          }
-      } else {                       // d = Uinv y
-        const cx<T> *Ui = recp + 50 + mm * 5 - mm * (mm - 1) / 2;
-#pragma unroll
-        for (int q = 0; q < 5; q++)
-          if (q < 5 - mm) {
-            double2 yq = xr[mm + q];
-            acc = cfma(Ui[q], cx<T>{(T)yq.x, (T)yq.y}, acc);
-          }
+      for (int cc = 0; cc < C; cc++) {
+        int jlo, nr;
+        chunk_rows(cc & 1, k, fw, jlo, nr);
+        const long r0 = line_of(cc >> 1) * n + coff(cc & 1) + jlo;
+        if (fw) dma_rows(fwd + r0 * 60, rec[buf][cc], nr * 60, lane);
+        else dma_rows(bwd + r0 * 66, rec[buf][cc], nr * 66, lane);
       }
-      cb[k & 1][cc][row][mm] = acc;
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-  };
+      double2 *dst = &bst[buf][0][0][0];
+#pragma unroll
+      for (int b = 0; b * 64 < C * R * 5; b++) {
+        const int e = b * 64 + lane, cc = e / (R * 5), j = (e / 5) % R, q = e % 5;
+        int jlo, nr;
+        chunk_rows(cc & 1, k, fw, jlo, nr);
+        if (e < C * R * 5 && j < nr)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(rhs + line_of(cc >> 1) * sL * 5 + (long)real_row(cc & 1, jlo + j) * rs + q),
+              (lds_void *)(dst + b * 64), 16, 0, 0);
+      }
+    };
+    auto prep = [&](int k, bool fw) {
+      const int buf = k % NBUF;
+      for (int r0 = 0; r0 < C * R; r0 += 12) {     // 12 rows (60 lanes) at a time
+        const int rr = r0 + lane / 5, cc = rr / R, j = rr % R;
+        int jlo, nr;
+        chunk_rows(cc & 1, k, fw, jlo, nr);
+        if (lane < 60 && rr < C * R && j < nr)
+          parallel_part(bst[buf][cc][j], rec[buf][cc] + j * (fw ? 60 : 66), lane % 5, fw);
+      }
+    };
+    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
+    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, false); }, [&](int k) { prep(k, false); });
+    return;
+  }
 
   // ---- compute wave (same recurrence as k_solve_bc, per chain) ----
   cx<T> yA[5], yB[5];
@@ -897,11 +965,9 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
     const int main_hi = fw ? len1 - 2 : min(len0 - 5, len1 - 3);
 #pragma unroll
     for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
-    if (wave != 0) load(0, fw);
-    __syncthreads();
-    for (int k = 0; k < nchunk; k++) {
-      if (wave == 0) {
-        const int kk = k & 1;
+    compute_pass(nchunk, [&](int k) {
+      {
+        const int kk = k % NBUF;
         const cx<T> *B = rec[kk][ch];
         int jlo, nr;
         chunk_rows(t, k, fw, jlo, nr);
@@ -921,7 +987,7 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
           const cx<T> *pp = B + r * stride + m * 10 + 5;
 #pragma unroll
           for (int q = 0; q < 5; q++) w[q] = pp[q];
-          c = cb[kk][ch][r][m];
+          c = *reinterpret_cast<const cx<T> *>(&bst[kk][ch][r][m]);
         };
         auto jrow = [&](int s) { return fw ? s : bj(t, s); };
         // one step on the plain path: hand-offs, boundary blocks, inactive chains
@@ -1010,11 +1076,8 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
           }
         }
         for (int s = m_end; s < s_end; s++) plain(s);
-      } else if (k + 1 < nchunk) {
-        load(k + 1, fw);
       }
-      __syncthreads();
-    }
+    });
   };
   run_pass(true);
   run_pass(false);
@@ -1392,7 +1455,7 @@ int penta_variant_for(int n, long lines, int prec) {
     if (it != g_tuned.end()) return it->second;
   }
   if (n >= 16 && lines < 512) return PENTA_TWISTED;
-  return lines >= 512 ? PENTA_PLAIN4 : PENTA_PLAIN1;
+  return lines >= 512 ? PENTA_PLAIN3 : PENTA_PLAIN1;
 }
 
 int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec) {
@@ -1492,6 +1555,36 @@ int penta_factor(PentaPlan *pl, hipStream_t s) {
   return penta_factor_records_only(pl, s);
 }
 
+// non-periodic solve kernels by variant.  Chunks of R rows, four LDS
+// buffers: twisted lines and one line per workgroup use longer chunks (one
+// workgroup per line holds more rows); two, three or four lines per
+// workgroup take chunks of six rows so that two or more workgroups fit a CU.
+template <typename T>
+static void launch_solve_bc(const PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
+  constexpr bool D = sizeof(T) == 8;
+  const auto *f = (const cx<T> *)pl->fwd;
+  const auto *b = (const cx<T> *)pl->bwd;
+  const auto *t = (const cx<T> *)pl->tail;
+  auto *x = (double2 *)rhs;
+  auto wgs = [&](int G) { return dim3((unsigned)((pl->L + G - 1) / G)); };
+  switch (pl->variant) {
+    case PENTA_TWISTED:
+      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, x, sL, sR);
+      break;
+    case PENTA_PLAIN2:
+      hipLaunchKernelGGL((k_solve_bc<T, D ? 6 : 12, 2>), wgs(2), dim3(256), 0, s, pl->n, pl->L, f, b, t, x, sL, sR);
+      break;
+    case PENTA_PLAIN3:
+      hipLaunchKernelGGL((k_solve_bc<T, 6, 3>), wgs(3), dim3(256), 0, s, pl->n, pl->L, f, b, t, x, sL, sR);
+      break;
+    case PENTA_PLAIN4:
+      hipLaunchKernelGGL((k_solve_bc<T, 6, 4>), wgs(4), dim3(256), 0, s, pl->n, pl->L, f, b, t, x, sL, sR);
+      break;
+    default:
+      hipLaunchKernelGGL((k_solve_bc<T, D ? 12 : 24, 1>), wgs(1), dim3(256), 0, s, pl->n, pl->L, f, b, t, x, sL, sR);
+  }
+}
+
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
   if (pl->periodic) {
@@ -1504,22 +1597,8 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
     else
       hipLaunchKernelGGL((k_solve_per<30, 1>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, pl->L, f, b, p2, qp, (double2 *)rhs, sL, sR);
   } else {
-    // twisted lines: one line (two chains) per workgroup; PENTA_PLAIN4: four
-    // lines per workgroup (one per DPP row of the compute wave)
-    const unsigned g4 = (unsigned)((pl->L + 3) / 4), g1 = (unsigned)pl->L;
-    if (pl->prec) {
-      auto *f = (const cx<float> *)pl->fwd; auto *b = (const cx<float> *)pl->bwd;
-      auto *t = (const cx<float> *)pl->tail;
-      if (pl->h >= 0) hipLaunchKernelGGL((k_solve_tw<float, 24, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-      else if (pl->variant == PENTA_PLAIN4) hipLaunchKernelGGL((k_solve_bc<float, 12, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-      else hipLaunchKernelGGL((k_solve_bc<float, 54, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-    } else {
-      auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
-      auto *t = (const cx<double> *)pl->tail;
-      if (pl->h >= 0) hipLaunchKernelGGL((k_solve_tw<double, 12, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-      else if (pl->variant == PENTA_PLAIN4) hipLaunchKernelGGL((k_solve_bc<double, 6, 4>), dim3(g4), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-      else hipLaunchKernelGGL((k_solve_bc<double, 30, 1>), dim3(g1), dim3(256), 0, s, pl->n, pl->L, f, b, t, (double2 *)rhs, sL, sR);
-    }
+    if (pl->prec) launch_solve_bc<float>(pl, rhs, sL, sR, s);
+    else launch_solve_bc<double>(pl, rhs, sL, sR, s);
   }
   LNS3D_CHECK(hipGetLastError());
   return 0;

@@ -261,12 +261,17 @@ def ctx_bytes(p, precision=0) -> int:
     return int(b.value)
 
 
+# line-solve variants the autotuner times (LNS3D_TUNE_VARIANTS)
+TUNE_VARIANTS = 5
+VARIANT_NAMES = ("1 line/WG", "4 lines/WG", "twisted", "2 lines/WG", "3 lines/WG")
+
+
 def tune_lines(n: int, lines: int, precision: int = 0):
     """time every line-solve variant on synthetic lines of this shape; the
     fastest is used by plans of that shape created afterwards.  Returns
     (variant, [ms per variant, -1 where not applicable])"""
     v = CI(-1)
-    t = (ctypes.c_float * 3)()
+    t = (ctypes.c_float * TUNE_VARIANTS)()
     check(lib().lns3d_tune_lines(n, lines, precision, ctypes.byref(v), t), "lns3d_tune_lines")
     return v.value, [float(x) for x in t]
 
