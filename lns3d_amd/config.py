@@ -110,3 +110,24 @@ def make_disturbance(p: Params, x: np.ndarray, seed: int = 0, amp: float = 1.0e-
     z = rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny))
     bump = np.exp(-0.5 * ((x - x_c) / sigma) ** 2)
     return np.asfortranarray(amp * z * bump[None, :, :])
+
+
+def disturbance_envelope(x: np.ndarray, x_c: float = 30.0, sigma: float = 5.0) -> np.ndarray:
+    """the Gaussian bump in x of make_disturbance, (nx, ny)"""
+    return np.exp(-0.5 * ((x - x_c) / sigma) ** 2)
+
+
+def make_disturbance_device(envelope_dev, seed: int, amp: float = 1.0e-6):
+    """The same kind of initial disturbance drawn on the device (torch's
+    generator, seeded; the current stream): complex N(0, amp^2) in each of
+    the real and imaginary parts times the envelope, as the flat (5, nx, ny)
+    state in the reference's order.  envelope_dev: the envelope flattened in
+    Fortran order (nx, ny), complex double, on the device.  Reproducible for
+    a seed, but not the same numbers as make_disturbance."""
+    import torch
+    gen = torch.Generator(device=envelope_dev.device)
+    gen.manual_seed(int(seed))
+    P = envelope_dev.numel()
+    # torch's complex normal has variance 1/2 in each part
+    z = torch.randn((P, 5), dtype=torch.complex128, generator=gen, device=envelope_dev.device)
+    return (z * (amp * math.sqrt(2.0)) * envelope_dev[:, None]).reshape(-1)

@@ -82,6 +82,13 @@ template <typename T> __device__ __forceinline__ void row_bcast5(cx<T> v, cx<T> 
   y[0] = row_bcast<0>(v); y[1] = row_bcast<1>(v); y[2] = row_bcast<2>(v);
   y[3] = row_bcast<3>(v); y[4] = row_bcast<4>(v);
 }
+// orders the LDS accesses of one wave before / after it (lanes of the wave
+// exchanging values through LDS)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // finite-difference coefficient tables of the scheme (src/stencil.f90)
 namespace st {

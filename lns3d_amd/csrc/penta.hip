@@ -257,15 +257,22 @@ __global__ void k_solve_lines(LineGeom g, const cx<T> *mat_, const cx<T> *per_, 
 // tail[line*150 + ...]: 0 row n-2 N(col n-5); 25 row n-1 N(col n-5);
 //                       50 row n-1 N(col n-4); 75 row 0 Q(col 3);
 //                       100 row 0 Q(col 4); 125 row 1 Q(col 4)   ([m][q])
-// records of one (line, chain row); a separate function because inlining
-// it into the kernel trips the compiler's CFG simplification (clang 22)
+// Records of (line, chain row)s: eight per workgroup, 32 lanes each (a row
+// and its inverses in LDS; lane (m, q) = (l / 5, l % 5) of lanes 0..24 forms
+// entry (m, q) of a block).  The row's five LU blocks are read with
+// consecutive lanes on consecutive entries.
 template <typename TF, typename TR>
-__device__ __noinline__ void records_row(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR> *bwd,
-                                         cx<TR> *tail, long idx) {
+__global__ __launch_bounds__(256) void k_records_rows(LineGeom g, const cx<TF> *mat, cx<TR> *fwd, cx<TR> *bwd,
+                                                      cx<TR> *tail) {
+  __shared__ cx<TF> rowb[8][125];
+  __shared__ cx<TF> Lis[8][25], Uis[8][25];
   const int n = g.n;
+  const int grp = threadIdx.x >> 5, gl = threadIdx.x & 31;
+  const long idx0 = (long)blockIdx.x * 8 + grp;
+  const bool valid = idx0 < g.L * n;
+  const long idx = valid ? idx0 : g.L * n - 1;        // idle groups repeat the last row, store nothing
   const long line = idx / n;
   const int jj = (int)(idx - line * n);
-  cx<TF> *mat = const_cast<cx<TF> *>(mat_);
   const bool tw = g.h >= 0;
   const int nchain = tw ? 2 : 1;
   // chain t: rows in elimination order; t = 1 runs bottom-up (row n-1-j)
@@ -274,80 +281,91 @@ __device__ __noinline__ void records_row(LineGeom g, const cx<TF> *mat_, cx<TR> 
   const int len = tw ? (t == 0 ? g.h + 2 : n - g.h - 2) : n;
   const int dir = t == 0 ? 1 : -1;
   const int i = t == 0 ? j : n - 1 - j;
+  cx<TF> *rb = rowb[grp];
+  const cx<TF> *src = mat + ((line * n + i) * 125);
+  for (int e = gl; e < 125; e += 32) rb[e] = src[e];
+  const int m = gl / 5, q = gl % 5;
+  const bool act = gl < 25;
+  const int sd = bc_slot(n, i, i) * 25;
+  if (act) Lis[grp][gl] = cx<TF>{TF(m == q), 0};
+  wave_sync();
+  // Linv: the in-block elimination steps applied to I
+#pragma unroll
+  for (int nc = 0; nc < 4; nc++) {
+    if (act && m > nc && q <= nc) Lis[grp][gl] = cfma(rb[sd + m * 5 + nc], Lis[grp][nc * 5 + q], Lis[grp][gl]);
+    wave_sync();
+  }
+  // Uinv by back substitution on identity columns (lane c < 5: column c)
+  if (gl < 5) {
+    const int c = gl;
+    cx<TF> u[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) u[r] = cx<TF>{0, 0};
+#pragma unroll
+    for (int mm = 4; mm >= 0; mm--)
+      if (mm <= c) {
+        cx<TF> acc = cx<TF>{TF(mm == c), 0};
+#pragma unroll
+        for (int k = mm + 1; k < 5; k++)
+          if (k <= c) acc = cfma(-rb[sd + mm * 5 + k], u[k], acc);
+        u[mm] = cdiv(acc, rb[sd + mm * 5 + mm]);
+      }
+#pragma unroll
+    for (int r = 0; r < 5; r++) Uis[grp][r * 5 + c] = u[r];
+  }
+  wave_sync();
+  const cx<TF> *Li = Lis[grp], *Ui = Uis[grp];
+  // entry (m, q) of Linv * M(i, col) (a multiplier block), zero if col has none
+  auto Nent = [&](int col) {
+    cx<TF> acc{0, 0};
+    const int sl = (col >= 0 && col < n) ? bc_slot(n, i, col) : -1;
+    if (sl >= 0)
+      for (int k = 0; k <= m; k++) acc = cfma(Li[m * 5 + k], rb[sl * 25 + k * 5 + q], acc);
+    return cx<TR>{(TR)acc.re, (TR)acc.im};
+  };
+  // entry (m, q) of -Uinv * U(i, col), col pivoted after i
+  auto Qent = [&](int col) {
+    cx<TF> acc{0, 0};
+    const bool later = !(tw && t == 0 && col >= g.h + 2);
+    const int sl = (later && col >= 0 && col < n) ? bc_slot(n, i, col) : -1;
+    if (sl >= 0)
+      for (int k = m; k < 5; k++) acc = cfma(Ui[m * 5 + k], rb[sl * 25 + k * 5 + q], acc);
+    return cx<TR>{(TR)-acc.re, (TR)-acc.im};
+  };
+  cx<TR> *F = fwd + idx * 60;
+  cx<TR> *B = bwd + idx * 66;
   cx<TR> *Tl = tail + (line * nchain + t) * 150;
-  Blk<TF> D;
-  blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, i, D);
-  // Linv: product of the in-block elimination steps applied to I
-  cx<TF> Li[5][5], Ui[5][5];
-  for (int r = 0; r < 5; r++)
-    for (int c = 0; c < 5; c++) Li[r][c] = cx<TF>{TF(r == c), 0};
-  for (int nc = 0; nc < 4; nc++)
-    for (int l = nc + 1; l < 5; l++)
-      for (int c = 0; c <= nc; c++) Li[l][c] = cfma(D(l, nc), Li[nc][c], Li[l][c]);
-  // Uinv by back substitution on identity columns
-  for (int r = 0; r < 5; r++)
-    for (int c = 0; c < 5; c++) Ui[r][c] = cx<TF>{0, 0};
-  for (int c = 0; c < 5; c++)
-    for (int m = c; m >= 0; m--) {
-      cx<TF> acc = cx<TF>{TF(m == c), 0};
-      for (int q = m + 1; q <= c; q++) acc = cfma(-D(m, q), Ui[q][c], acc);
-      Ui[m][c] = cdiv(acc, D(m, m));
+  if (valid && act) {
+    F[m * 10 + q] = Nent(i - dir);
+    F[m * 10 + 5 + q] = Nent(i - 2 * dir);
+    B[m * 10 + q] = Qent(i + dir);
+    B[m * 10 + 5 + q] = Qent(i + 2 * dir);
+    // forward blocks of the chain's last two rows outside the band: columns
+    // n-5, n-4 (plain) or the bottom chain's last rows h+2, h+3 (twisted);
+    // the bottom chain of a twisted factor has none (zeros)
+    const bool fw_tail = !(tw && t == 1);
+    const int x0 = !fw_tail ? -1 : (tw ? g.h + 2 : n - 5), x1 = !fw_tail ? -1 : (tw ? g.h + 3 : n - 4);
+    if (j == (fw_tail ? len - 2 : 0)) Tl[m * 5 + q] = Nent(x0);
+    if (j == (fw_tail ? len - 1 : 0)) { Tl[25 + m * 5 + q] = Nent(x0); Tl[50 + m * 5 + q] = Nent(x1); }
+    // backward blocks of the chain's first two rows: columns 3, 4 counted
+    // along the chain
+    const int c3 = t == 0 ? 3 : n - 4, c4 = t == 0 ? 4 : n - 5;
+    if (j == 0) { Tl[75 + m * 5 + q] = Qent(c3); Tl[100 + m * 5 + q] = Qent(c4); }
+    if (j == 1) Tl[125 + m * 5 + q] = Qent(c4);
+  }
+  if (valid && gl < 10) {            // strict lower of Linv, row-major
+    const int l = gl < 1 ? 1 : gl < 3 ? 2 : gl < 6 ? 3 : 4, c = gl - l * (l - 1) / 2;
+    F[50 + gl] = cx<TR>{(TR)Li[l * 5 + c].re, (TR)Li[l * 5 + c].im};
+  }
+  if (valid && gl < 16) {            // upper of Uinv incl. diagonal, row-major; then the pad
+    if (gl < 15) {
+      const int r = gl < 5 ? 0 : gl < 9 ? 1 : gl < 12 ? 2 : gl < 14 ? 3 : 4;
+      const int c = r + gl - (5 * r - r * (r - 1) / 2);
+      B[50 + gl] = cx<TR>{(TR)Ui[r * 5 + c].re, (TR)Ui[r * 5 + c].im};
+    } else {
+      B[65] = cx<TR>{0, 0};
     }
-  cx<TR> *F = fwd + ((long)line * n + jj) * 60;
-  cx<TR> *B = bwd + ((long)line * n + jj) * 66;
-  auto Nblk = [&](int col, cx<TR> *dst, int ld) {   // Linv * M(i,col) (a multiplier block)
-    Blk<TF> M;
-    bool ok = col >= 0 && col < n && blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, M);
-    for (int m = 0; m < 5; m++)
-      for (int q = 0; q < 5; q++) {
-        cx<TF> acc{0, 0};
-        if (ok)
-          for (int k = 0; k <= m; k++) acc = cfma(Li[m][k], M(k, q), acc);
-        dst[m * ld + q] = cx<TR>{(TR)acc.re, (TR)acc.im};
-      }
-  };
-  auto Qblk = [&](int col, cx<TR> *dst, int ld) {   // -Uinv * U(i,col), col pivoted after i
-    Blk<TF> U;
-    bool later = !(tw && t == 0 && col >= g.h + 2);
-    bool ok = later && col >= 0 && col < n &&
-              blk(g, mat, (cx<TF> *)nullptr, (cx<TF> *)nullptr, line, i, col, U);
-    for (int m = 0; m < 5; m++)
-      for (int q = 0; q < 5; q++) {
-        cx<TF> acc{0, 0};
-        if (ok)
-          for (int k = m; k < 5; k++) acc = cfma(Ui[m][k], U(k, q), acc);
-        dst[m * ld + q] = cx<TR>{(TR)-acc.re, (TR)-acc.im};
-      }
-  };
-  Nblk(i - dir, F, 10);
-  Nblk(i - 2 * dir, F + 5, 10);
-  for (int l = 1, k = 0; l < 5; l++)
-    for (int q = 0; q < l; q++, k++) F[50 + k] = cx<TR>{(TR)Li[l][q].re, (TR)Li[l][q].im};
-  Qblk(i + dir, B, 10);
-  Qblk(i + 2 * dir, B + 5, 10);
-  for (int m = 0, k = 0; m < 5; m++)
-    for (int q = m; q < 5; q++, k++) B[50 + k] = cx<TR>{(TR)Ui[m][q].re, (TR)Ui[m][q].im};
-  B[65] = cx<TR>{0, 0};
-  // forward blocks of the chain's last two rows outside the band: columns
-  // n-5, n-4 (plain) or the bottom chain's last rows h+2, h+3 (twisted)
-  // (the bottom chain of a twisted factor has none: Nblk of column -1 writes
-  // zeros)
-  const bool fw_tail = !(tw && t == 1);
-  const int x0 = !fw_tail ? -1 : (tw ? g.h + 2 : n - 5), x1 = !fw_tail ? -1 : (tw ? g.h + 3 : n - 4);
-  if (j == (fw_tail ? len - 2 : 0)) Nblk(x0, Tl + 0, 5);
-  if (j == (fw_tail ? len - 1 : 0)) { Nblk(x0, Tl + 25, 5); Nblk(x1, Tl + 50, 5); }
-  // backward blocks of the chain's first two rows: columns 3, 4 counted
-  // along the chain
-  const int c3 = t == 0 ? 3 : n - 4, c4 = t == 0 ? 4 : n - 5;
-  if (j == 0) { Qblk(c3, Tl + 75, 5); Qblk(c4, Tl + 100, 5); }
-  if (j == 1) Qblk(c4, Tl + 125, 5);
-}
-
-// one thread per (line, chain row): rows are independent once factored
-template <typename TF, typename TR>
-__global__ void k_records_rows(LineGeom g, const cx<TF> *mat_, cx<TR> *fwd, cx<TR> *bwd, cx<TR> *tail) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < g.L * g.n) records_row<TF, TR>(g, mat_, fwd, bwd, tail, idx);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -425,13 +443,27 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
     for (int v = v0; v < v1; v++) {
       const int sd = slot(v, v);
       cx<T> *D = B(v, sd);
-      int uc[4], us[4], nu = 0, pr[4], nparts = 0;
+      // blocks right of the pivot (columns uc, slots us) and partner rows
+      // (pr); filled and read with constant indices only, so that they stay
+      // in registers (a run-time index puts a local array in scratch)
+      int uc[4] = {0, 0, 0, 0}, us[4] = {0, 0, 0, 0}, pr[4] = {0, 0, 0, 0}, nu = 0, nparts = 0;
+#pragma unroll
       for (int d = 1; d <= 4; d++) {
         const int vc = v + d;
-        if (vc <= lim && slot(v, vc) >= 0) { uc[nu] = vc; us[nu] = slot(v, vc); nu++; }
-        if (vc <= lim && slot(vc, v) >= 0) pr[nparts++] = vc;
+        const int su = vc <= lim ? slot(v, vc) : -1;
+        const bool hp = vc <= lim && slot(vc, v) >= 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          if (su >= 0 && nu == i) { uc[i] = vc; us[i] = su; }
+          if (hp && nparts == i) pr[i] = vc;
+        }
+        nu += su >= 0;
+        nparts += hp;
       }
+      auto pick = [](const int (&a)[4], int i) { return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3]; };
       // in-block LU of D, its row operations applied to the right blocks
+      // (loops over nc unrolled: the task decodes divide by constants)
+#pragma unroll
       for (int nc = 0; nc < 4; nc++) {
         if (lane < 4 - nc) {
           const int l = nc + 1 + lane;
@@ -446,17 +478,18 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
           } else {
             const int kk = k - nd, u = kk / ((4 - nc) * 5), rem = kk % ((4 - nc) * 5);
             const int l = nc + 1 + rem / 5, q = rem % 5;
-            cx<T> *U = B(v, us[u]);
+            cx<T> *U = B(v, pick(us, u));
             U[l * 5 + q] = cfma(D[l * 5 + nc], U[nc * 5 + q], U[l * 5 + q]);
           }
         }
         wsync();
       }
       // partner rows: the block in the pivot column becomes the multipliers
+#pragma unroll
       for (int nc = 0; nc < 5; nc++) {
         const cx<T> fact = crecip(D[nc * 5 + nc]);
         if (lane < nparts * 5) {
-          cx<T> *Lb = B(pr[lane / 5], slot(pr[lane / 5], v));
+          cx<T> *Lb = B(pick(pr, lane / 5), slot(pick(pr, lane / 5), v));
           const int mm = lane % 5;
           Lb[mm * 5 + nc] = -(Lb[mm * 5 + nc] * fact);
         }
@@ -466,16 +499,16 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
           if (k < nl) {
             const int pp = k / (5 * (4 - nc)), rem = k % (5 * (4 - nc));
             const int mm = rem / (4 - nc), q = nc + 1 + rem % (4 - nc);
-            cx<T> *Lb = B(pr[pp], slot(pr[pp], v));
+            cx<T> *Lb = B(pick(pr, pp), slot(pick(pr, pp), v));
             Lb[mm * 5 + q] = cfma(Lb[mm * 5 + nc], D[nc * 5 + q], Lb[mm * 5 + q]);
           } else {
             const int kk = k - nl, pp = kk / (nu * 25), rem = kk % (nu * 25);
             const int u = rem / 25, mm = (rem % 25) / 5, q = rem % 5;
-            const int sdst = slot(pr[pp], uc[u]);
+            const int sdst = slot(pick(pr, pp), pick(uc, u));
             if (sdst < 0) continue;      // no such block (does not occur in the band)
-            cx<T> *Lb = B(pr[pp], slot(pr[pp], v));
-            cx<T> *Dst = B(pr[pp], sdst);
-            const cx<T> *U = B(v, us[u]);
+            cx<T> *Lb = B(pick(pr, pp), slot(pick(pr, pp), v));
+            cx<T> *Dst = B(pick(pr, pp), sdst);
+            const cx<T> *U = B(v, pick(us, u));
             Dst[mm * 5 + q] = cfma(Lb[mm * 5 + nc], U[nc * 5 + q], Dst[mm * 5 + q]);
           }
         }
@@ -1523,13 +1556,13 @@ int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStre
 int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
   if (pl->periodic) return set_error(__FILE__, __LINE__, "records are for non-periodic lines");
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
-  const int nb = grid_for(pl->L * (long)pl->n, 128);
+  const int nb = grid_for(pl->L * (long)pl->n, 8);
   if (pl->prec)
-    hipLaunchKernelGGL((k_records_rows<double, float>), dim3(nb), dim3(128), 0, s, g,
+    hipLaunchKernelGGL((k_records_rows<double, float>), dim3(nb), dim3(256), 0, s, g,
                        (const cx<double> *)pl->lu, (cx<float> *)pl->fwd, (cx<float> *)pl->bwd,
                        (cx<float> *)pl->tail);
   else
-    hipLaunchKernelGGL((k_records_rows<double, double>), dim3(nb), dim3(128), 0, s, g,
+    hipLaunchKernelGGL((k_records_rows<double, double>), dim3(nb), dim3(256), 0, s, g,
                        (const cx<double> *)pl->lu, (cx<double> *)pl->fwd, (cx<double> *)pl->bwd,
                        (cx<double> *)pl->tail);
   LNS3D_CHECK(hipGetLastError());
