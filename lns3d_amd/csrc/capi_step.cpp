@@ -389,7 +389,7 @@ extern "C" int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes) {
   auto plan = [&](int periodic, size_t es) {
     size_t x = P * 125 * 16;                                  // LU working copy
     if (periodic) x += 2 * P * 50 * 16;
-    else x += P * 126 * es + P * 2 * 150 * es / 32 + P * 100 * 16 / 32;
+    else x += P * 126 * es + P * 2 * 150 * es / 32 + P * 100 * 16 / 32 + P * 26 * es;   // + pair side blocks
     return x;
   };
   b += plan(p->xper, 16) + plan(p->yper, 16);

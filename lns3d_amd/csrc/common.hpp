@@ -82,6 +82,11 @@ template <typename T> __device__ __forceinline__ void row_bcast5(cx<T> v, cx<T> 
   y[0] = row_bcast<0>(v); y[1] = row_bcast<1>(v); y[2] = row_bcast<2>(v);
   y[3] = row_bcast<3>(v); y[4] = row_bcast<4>(v);
 }
+// y[q] = v of lane OFF + q of the 16-lane row
+template <int OFF, typename T> __device__ __forceinline__ void row_bcast5_at(cx<T> v, cx<T> (&y)[5]) {
+  y[0] = row_bcast<OFF>(v); y[1] = row_bcast<OFF + 1>(v); y[2] = row_bcast<OFF + 2>(v);
+  y[3] = row_bcast<OFF + 3>(v); y[4] = row_bcast<OFF + 4>(v);
+}
 // orders the LDS accesses of one wave before / after it (lanes of the wave
 // exchanging values through LDS)
 __device__ __forceinline__ void wave_sync() {

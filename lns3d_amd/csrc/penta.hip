@@ -369,6 +369,93 @@ __global__ __launch_bounds__(256) void k_records_rows(LineGeom g, const cx<TF> *
 }
 
 // ---------------------------------------------------------------------------
+// Twisted lines take their chain steps in pairs where no hand-off or boundary
+// block is involved.  In processing order a step is u_s = c_s + W2_s u_{s-1}
+// + W1_s u_{s-2} (forward W2 = N2, W1 = N1; backward W2 = Q4, W1 = Q5).  The
+// second step of a pair written in the same inputs as the first,
+//   u_{s+1} = c' + W2' u_{s-1} + W1' u_{s-2},
+//   W2' = W1_{s+1} + W2_{s+1} W2_s,  W1' = W2_{s+1} W1_s,  c' = c_{s+1} + W2_{s+1} c_s,
+// lets the solve form both at once (lanes 0-4 and 5-9 of a DPP row).  The
+// factor rewrites the second step's record with W2', W1' and keeps
+// M = W2_{s+1} Linv_s (forward) / W2_{s+1} Uinv_s (backward) in a side array
+// (xf / xb, 25 complex per pair), so that the solve's loader waves form
+// c' = Linv_{s+1} b_{s+1} + M b_s (forward; backward with Uinv and the
+// forward result) from the staged right-hand sides in one pass.
+//
+// Pairs (s, s+1), s even, of chunk k (steps [kR, min(kR+R, len0))) lie in
+// [p_end, m_end); the same rule on both sides, as it depends on the solve's
+// chunk size R.
+__host__ __device__ inline void tw_pair_range(int R, int k, bool fw, int len0, int len1, int &p_end,
+                                              int &m_end) {
+  const int p0 = fw ? 0 : 4;                                   // even, after the backward hand-off
+  const int main_hi = fw ? len1 - 2 : (len0 - 5 < len1 - 3 ? len0 - 5 : len1 - 3);
+  const int s_begin = k * R, s_end = (k * R + R < len0) ? k * R + R : len0;
+  p_end = s_begin > (p0 < s_end ? p0 : s_end) ? s_begin : (p0 < s_end ? p0 : s_end);
+  const int hi = main_hi < s_end ? main_hi : s_end;
+  m_end = p_end + ((hi > p_end ? hi - p_end : 0) / 6) * 6;
+}
+// chunk size of the twisted solve for a record precision
+constexpr int tw_chunk(bool single) { return single ? 12 : 6; }
+
+// the pair rewrite at factor time: one 32-lane group per (line, chain, pass,
+// pair), lane (m, q) of lanes 0..24 forms entry (m, q) of W2', W1' and M
+template <typename TR>
+__global__ __launch_bounds__(256) void k_pair_records(int n, int h, long L, int R, cx<TR> *fwd, cx<TR> *bwd,
+                                                      cx<TR> *xf, cx<TR> *xb) {
+  const int len0 = h + 2, len1 = n - h - 2, npair = (len0 + 1) / 2;
+  const int grp = threadIdx.x >> 5, gl = threadIdx.x & 31;
+  const long id = (long)blockIdx.x * 8 + grp;     // ((line * 2 + chain) * 2 + pass) * npair + pair
+  const long total = L * 4 * npair;
+  const long idc = id < total ? id : total - 1;
+  const int q2 = (int)(idc % npair), pass = (int)((idc / npair) % 2), cc = (int)((idc / npair / 2) % 2);
+  const long line = idc / npair / 4;
+  const bool fw = pass == 0;
+  const int s = 2 * q2;
+  int p_end, m_end;
+  tw_pair_range(R, s / R, fw, len0, len1, p_end, m_end);
+  const bool is_pair = id < total && s >= p_end && s + 1 < m_end;
+  // chain rows of steps s, s+1 and their records
+  const int coff = cc * len0;
+  const int j0 = fw ? s : len0 - 1 - s + cc * (len1 + 2 - len0);
+  const int j1 = fw ? j0 + 1 : j0 - 1;
+  const int stride = fw ? 60 : 66;
+  cx<TR> *base = fw ? fwd : bwd;
+  cx<TR> *r0 = base + (line * n + coff + (is_pair ? j0 : 0)) * stride;
+  cx<TR> *r1 = base + (line * n + coff + (is_pair ? j1 : 0)) * stride;
+  cx<TR> *x = (fw ? xf : xb) + ((line * 2 + cc) * npair + q2) * 25;
+  const int m = gl / 5, q = gl % 5;
+  cx<double> w2n{0, 0}, w1n{0, 0}, mx{0, 0};
+  if (is_pair && gl < 25) {
+    // M(m, q) = (W2_{s+1} Linv_s)(m, q) forward (Linv: unit lower, strict
+    // part at 50 + l(l-1)/2 + c), (W2_{s+1} Uinv_s)(m, q) backward (Uinv:
+    // upper incl. diagonal at 50 + 5k - k(k-1)/2 + (c - k))
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const cx<double> a{(double)r1[m * 10 + k].re, (double)r1[m * 10 + k].im};
+      cx<TR> lk{0, 0};
+      if (fw) lk = k == q ? cx<TR>{1, 0} : (k > q ? r0[50 + k * (k - 1) / 2 + q] : cx<TR>{0, 0});
+      else if (k <= q) lk = r0[50 + 5 * k - k * (k - 1) / 2 + (q - k)];
+      mx = cfma(a, cx<double>{(double)lk.re, (double)lk.im}, mx);
+    }
+    w2n = cx<double>{(double)r1[m * 10 + 5 + q].re, (double)r1[m * 10 + 5 + q].im};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const cx<double> a{(double)r1[m * 10 + k].re, (double)r1[m * 10 + k].im};
+      const cx<double> b2{(double)r0[k * 10 + q].re, (double)r0[k * 10 + q].im};
+      const cx<double> b1{(double)r0[k * 10 + 5 + q].re, (double)r0[k * 10 + 5 + q].im};
+      w2n = cfma(a, b2, w2n);
+      w1n = cfma(a, b1, w1n);
+    }
+  }
+  wave_sync();
+  if (is_pair && gl < 25) {
+    x[m * 5 + q] = cx<TR>{(TR)mx.re, (TR)mx.im};
+    r1[m * 10 + q] = cx<TR>{(TR)w2n.re, (TR)w2n.im};
+    r1[m * 10 + 5 + q] = cx<TR>{(TR)w1n.re, (TR)w1n.im};
+  }
+}
+
+// ---------------------------------------------------------------------------
 // stage 1 for non-periodic lines: block LU with one wave per chain, the
 // rows v..v+4 of the chain in an LDS window.  A plain factor is one chain
 // (rows 0..n-1).  The twisted factor runs two chains concurrently, rows
@@ -635,6 +722,57 @@ __device__ __forceinline__ void loader_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// the parallel part of a row, plus M b_prev for the second step of a pair
+// (M: the pair's side block, b_prev: the staged right-hand side of the first
+// step).  One straight-line path for every lane of a round, so that all of
+// them read before any writes.
+template <typename T>
+__device__ __forceinline__ void parallel_part_x(double2 *bp, const cx<T> *rp, const cx<T> *M,
+                                                const double2 *bprev, int mm, bool fw, bool second) {
+  cx<T> acc{0, 0};
+  double2 bz[5];
+  cx<T> mz[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    bz[q] = bprev[q];
+    mz[q] = second ? M[mm * 5 + q] : cx<T>{0, 0};
+  }
+  if (fw) {
+    const int base = 50 + mm * (mm - 1) / 2;
+    const double2 bm = bp[mm];
+    double2 bq[4];
+    cx<T> l[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      bq[q] = bp[q];
+      l[q] = rp[min(base + q, 59)];
+    }
+    acc = cx<T>{(T)bm.x, (T)bm.y};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const cx<T> w = q < mm ? l[q] : cx<T>{0, 0};
+      acc = cfma(w, cx<T>{(T)bq[q].x, (T)bq[q].y}, acc);
+    }
+  } else {
+    const int base = 50 + mm * 5 - mm * (mm - 1) / 2;
+    double2 yq[5];
+    cx<T> u[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      yq[q] = bp[min(mm + q, 4)];
+      u[q] = rp[min(base + q, 64)];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const cx<T> w = q < 5 - mm ? u[q] : cx<T>{0, 0};
+      acc = cfma(w, cx<T>{(T)yq[q].x, (T)yq[q].y}, acc);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 5; q++) acc = cfma(mz[q], cx<T>{(T)bz[q].x, (T)bz[q].y}, acc);
+  *reinterpret_cast<cx<T> *>(bp + mm) = acc;
+}
+
 // the loader side of one pass over nchunk chunks (loader wave w = 0, 1, 2):
 // one barrier after the prologue and one per chunk, as on the compute side
 // (compute_pass)
@@ -890,16 +1028,20 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
 // two values (rows h+2, h+3) to chain 0, whose last two rows carry those
 // columns.  Backward: chain 0 starts at row h+1; chain 1 starts two steps
 // later from chain 0's x_{h+1}, x_h.
-template <typename T, int R, int G>
+template <typename T, int R, int G, bool PAIRS>
 __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
                                                  const cx<T> *__restrict__ bwd,
-                                                 const cx<T> *__restrict__ tail, double2 *rhs,
+                                                 const cx<T> *__restrict__ tail,
+                                                 const cx<T> *__restrict__ xf,
+                                                 const cx<T> *__restrict__ xb, double2 *rhs,
                                                  long sL, long sR) {
   constexpr int C = 2 * G;
   constexpr int NBUF = 4, BUFE = R * 66;
   static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
   __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][C][BUFE];
   __shared__ double2 bst[NBUF][C][R][5];   // rhs rows, then c (or d) in place
+  constexpr int XE = PAIRS ? (R / 2) * 25 + 1 : 1;   // side blocks of a chunk's pairs (+1: copy padding)
+  __shared__ __attribute__((aligned(16))) cx<T> xtr[NBUF][C][XE];
   __shared__ cx<T> xch[G][2][5];     // values handed between the chains of a line
   __shared__ cx<T> kept[C][2][5];    // x at chain rows 3, 4 (backward boundary blocks)
   const long line0 = (long)blockIdx.x * G;
@@ -921,6 +1063,7 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
   double2 *X = rhs + line * sL * 5;
   const cx<T> *tl = tail + (line * 2 + t) * 150;
   const int nchunk = (len0 + R - 1) / R;             // steps per pass: len0
+  const int npair = (len0 + 1) / 2;                  // pair slots per chain in xf / xb
   // chain row at step s of the backward pass
   auto bj = [&](int tt, int s) { return len0 - 1 - s + tt * (len1 + 2 - len0); };
   // rows of chunk k held in the LDS buffer for chain tt: [jlo, jlo + nr)
@@ -948,6 +1091,15 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
         if (fw) dma_rows(fwd + r0 * 60, rec[buf][cc], nr * 60, lane);
         else dma_rows(bwd + r0 * 66, rec[buf][cc], nr * 66, lane);
       }
+      int p_end, m_end;
+      tw_pair_range(R, k, fw, len0, len1, p_end, m_end);
+      if (PAIRS && m_end > p_end) {
+        const int ne = (m_end - p_end) / 2 * 25;
+#pragma unroll
+        for (int cc = 0; cc < C; cc++)
+          dma_rows((fw ? xf : xb) + ((line_of(cc >> 1) * 2 + (cc & 1)) * npair + p_end / 2) * 25, xtr[buf][cc],
+                   (ne + 1) & ~1, lane);
+      }
       double2 *dst = &bst[buf][0][0][0];
 #pragma unroll
       for (int b = 0; b * 64 < C * R * 5; b++) {
@@ -962,12 +1114,24 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
     };
     auto prep = [&](int k, bool fw) {
       const int buf = k % NBUF;
-      for (int r0 = 0; r0 < C * R; r0 += 12) {     // 12 rows (60 lanes) at a time
-        const int rr = r0 + lane / 5, cc = rr / R, j = rr % R;
+      int p_end, m_end;
+      tw_pair_range(R, k, fw, len0, len1, p_end, m_end);
+      for (int r0 = 0; r0 < C * R; r0 += 12) {     // 12 rows (60 lanes) at a time (a chain's rows
+        const int rr = r0 + lane / 5, cc = rr / R, j = rr % R;   // never straddle two rounds)
         int jlo, nr;
         chunk_rows(cc & 1, k, fw, jlo, nr);
-        if (lane < 60 && rr < C * R && j < nr)
-          parallel_part(bst[buf][cc][j], rec[buf][cc] + j * (fw ? 60 : 66), lane % 5, fw);
+        // the step of LDS row j, and whether it is the second of a pair
+        const int st = fw ? k * R + j : bj(cc & 1, 0) - (jlo + j);
+        const bool second = PAIRS && (st & 1) && st - 1 >= p_end && st < m_end;
+        const int jp = second ? (fw ? j - 1 : j + 1) : j;   // LDS row of step st - 1
+        if (lane < 60 && rr < C * R && j < nr) {
+          if constexpr (PAIRS)
+            parallel_part_x(bst[buf][cc][j], rec[buf][cc] + j * (fw ? 60 : 66),
+                            xtr[buf][cc] + (second ? (st - 1 - p_end) / 2 : 0) * 25, bst[buf][cc][jp], lane % 5,
+                            fw, second);
+          else
+            parallel_part(bst[buf][cc][j], rec[buf][cc] + j * (fw ? 60 : 66), lane % 5, fw);
+        }
       }
     };
     stream_pass(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
@@ -991,11 +1155,19 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
     a = cfma(w[4], yo[4], a);
     return a + b;
   };
+  // one step of a pair: c + W2 yn + W1 yo, in four independent FMA chains
+  auto pair_eval = [&](const cx<T> (&w2)[5], const cx<T> (&w1)[5], cx<T> c, const cx<T> (&yn)[5],
+                       const cx<T> (&yo)[5]) -> cx<T> {
+    cx<T> a = cfma(w1[0], yo[0], c), b = cmul(w2[0], yn[0]);
+#pragma unroll
+    for (int q = 1; q < 5; q++) {
+      a = cfma(w1[q], yo[q], a);
+      b = cfma(w2[q], yn[q], b);
+    }
+    return a + b;
+  };
   auto run_pass = [&](bool fw) {
     const int stride = fw ? 60 : 66;
-    // steps where every chain takes a plain row: [main_lo, main_hi)
-    const int main_lo = fw ? 0 : 3;
-    const int main_hi = fw ? len1 - 2 : min(len0 - 5, len1 - 3);
 #pragma unroll
     for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
     compute_pass(nchunk, [&](int k) {
@@ -1073,10 +1245,43 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
             if (!fw && (j == 3 || j == 4)) kept[ch][j - 3][m] = y;
           }
         };
-        const int p_end = max(s_begin, min(main_lo, s_end));
+        int p_end, m_end;
+        tw_pair_range(R, k, fw, len0, len1, p_end, m_end);
         for (int s = s_begin; s < p_end; s++) plain(s);
-        const int m_end = p_end + ((max(p_end, min(main_hi, s_end)) - p_end) / 6) * 6;
-        if (m_end > p_end) {
+        if (PAIRS && m_end > p_end) {
+          // pairs of steps (s, s+1): lanes 0-4 step s, lanes 5-9 step s+1
+          // (its record and parallel part rewritten for the pair), lanes
+          // 10-15 repeat lane 9; both from y_{s-2} (yA), y_{s-1} (yB).
+          // Coefficients two pairs ahead in three rotating register sets.
+          const int pl = min(lane & 15, 9), prow = pl >= 5 ? 1 : 0, pm = pl - 5 * prow;
+          auto ldP = [&](int st, cx<T> (&w2)[5], cx<T> (&w1)[5], cx<T> &c) {
+            const int r = brow(st + prow);
+            const cx<T> *pp = B + r * stride + pm * 10;
+#pragma unroll
+            for (int q = 0; q < 5; q++) { w2[q] = pp[q]; w1[q] = pp[5 + q]; }
+            c = *reinterpret_cast<const cx<T> *>(&bst[kk][ch][r][pm]);
+          };
+          const long xstep = ((t == 0) == fw) ? rs : -rs;
+          double2 *xq = X + (long)real_row(t, jrow(p_end + prow)) * rs + pm;
+          ldP(p_end, w2s[0], w1s[0], cs[0]);
+          ldP(p_end + 2, w2s[1], w1s[1], cs[1]);
+          auto psub = [&](int st, auto U) {
+            constexpr int u = decltype(U)::value, u2 = (u + 2) % 3;
+            const cx<T> y = pair_eval(w2s[u], w1s[u], cs[u], yB, yA);
+            *xq = make_double2(y.re, y.im);
+            xq += 2 * xstep;
+            ldP(st + 4, w2s[u2], w1s[u2], cs[u2]);
+            row_bcast5_at<0>(y, yA);
+            row_bcast5_at<5>(y, yB);
+          };
+          for (int st = p_end; st < m_end; st += 6) {
+            psub(st, std::integral_constant<int, 0>());
+            psub(st + 2, std::integral_constant<int, 1>());
+            psub(st + 4, std::integral_constant<int, 2>());
+          }
+        }
+        if (!PAIRS && m_end > p_end) {
+          // one step at a time in groups of six (no pair rewrite)
           int j0 = jrow(p_end);
           double2 *xp = X + (long)real_row(t, j0) * rs + m;
           const long xstep = ((t == 0) == fw) ? rs : -rs;
@@ -1512,7 +1717,7 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
-  pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = nullptr;
+  pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = pl->xf = pl->xb = nullptr;
   if (periodic) {
     const size_t z = sizeof(cx<double>);
     LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * z));
@@ -1527,6 +1732,13 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
     LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * es));
     LNS3D_CHECK(hipMalloc(&pl->tail, (size_t)150 * lines * (pl->h >= 0 ? 2 : 1) * es));
     if (pl->h >= 0) LNS3D_CHECK(hipMalloc(&pl->mid, (size_t)100 * lines * sizeof(cx<double>)));
+    if (pl->h >= 0 && prec) {
+      // 25 per (line, chain, pair), plus one complex past the end (a copy of
+      // an odd count of complex float moves 16 B at a time)
+      const size_t nx = ((size_t)lines * 2 * ((pl->h + 3) / 2) * 25 + 2) * es;
+      LNS3D_CHECK(hipMalloc(&pl->xf, nx));
+      LNS3D_CHECK(hipMalloc(&pl->xb, nx));
+    }
   }
   return 0;
 }
@@ -1541,7 +1753,10 @@ void penta_plan_destroy(PentaPlan *pl) {
   if (pl->mid) (void)hipFree(pl->mid);
   if (pl->p2) (void)hipFree(pl->p2);
   if (pl->qp) (void)hipFree(pl->qp);
+  if (pl->xf) (void)hipFree(pl->xf);
+  if (pl->xb) (void)hipFree(pl->xb);
   pl->lu = pl->per = pl->per2 = pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = nullptr;
+  pl->xf = pl->xb = nullptr;
 }
 
 int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStream_t s) {
@@ -1566,6 +1781,13 @@ int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
                        (const cx<double> *)pl->lu, (cx<double> *)pl->fwd, (cx<double> *)pl->bwd,
                        (cx<double> *)pl->tail);
   LNS3D_CHECK(hipGetLastError());
+  if (pl->h >= 0 && pl->prec) {  // twisted, complex float: the pair rewrite for the solve's chunk size
+    const long groups = pl->L * 4 * ((pl->h + 3) / 2);
+    hipLaunchKernelGGL((k_pair_records<float>), dim3(grid_for(groups, 8)), dim3(256), 0, s, pl->n, pl->h, pl->L,
+                       tw_chunk(true), (cx<float> *)pl->fwd, (cx<float> *)pl->bwd, (cx<float> *)pl->xf,
+                       (cx<float> *)pl->xb);
+    LNS3D_CHECK(hipGetLastError());
+  }
   return 0;
 }
 
@@ -1602,7 +1824,12 @@ static void launch_solve_bc(const PentaPlan *pl, void *rhs, long sL, long sR, hi
   auto wgs = [&](int G) { return dim3((unsigned)((pl->L + G - 1) / G)); };
   switch (pl->variant) {
     case PENTA_TWISTED:
-      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t, x, sL, sR);
+      // complex float: pairs of steps (factor-time rewrite); complex double:
+      // one step at a time (its streaming, not its recurrence, bounds the
+      // paired form)
+      static_assert(tw_chunk(!D) == (D ? 6 : 12), "the pair rewrite assumes the solve's chunk size");
+      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1, !D>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t,
+                         (const cx<T> *)pl->xf, (const cx<T> *)pl->xb, x, sL, sR);
       break;
     case PENTA_PLAIN2:
       hipLaunchKernelGGL((k_solve_bc<T, D ? 6 : 12, 2>), wgs(2), dim3(256), 0, s, pl->n, pl->L, f, b, t, x, sL, sR);

@@ -21,6 +21,8 @@ struct PentaPlan {
   void *fwd = nullptr, *bwd = nullptr, *tail = nullptr;
   void *mid = nullptr;   // twisted factor: the bottom chain's share of the middle block
   void *p2 = nullptr, *qp = nullptr;   // periodic lines: border records (complex double)
+  void *xf = nullptr, *xb = nullptr;   // twisted lines: per step pair, the second row's
+                                       // original N2 (forward) / Q4 (backward) block
 };
 
 // solve-kernel variants of non-periodic lines: one line per workgroup with a
