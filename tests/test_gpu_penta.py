@@ -93,8 +93,15 @@ def test_periodic_random(torch_cuda, orc, layout, nsys, np_, exact):
     h.close()
 
 
+# complex-float twisted lines take chain steps in pairs (12-row chunks): the
+# sizes include the shortest twisted lines (no pairs), pair regions ending
+# inside a chunk, several chunks, and many lines (no pairs: plain variants)
+FP32_SIZES = [(3, 35), (2, 61), (515, 17), (2, 16), (3, 17), (2, 18), (2, 29), (3, 40), (3, 63),
+              (1, 101), (65, 129)]
+
+
 @pytest.mark.parametrize("layout", [1, 2])
-@pytest.mark.parametrize("nsys,n", [(3, 35), (2, 61), (515, 17)])
+@pytest.mark.parametrize("nsys,n", FP32_SIZES)
 def test_bc_complex_float_factors(torch_cuda, orc, layout, nsys, n):
     """complex-float records (the refinement path's preconditioner): one
     solve agrees with the complex-double oracle to single-precision accuracy
