@@ -191,6 +191,157 @@ __global__ void k_lu_lines(LineGeom g, cx<T> *mat, cx<T> *per, cx<T> *per2) {
 }
 
 // ---------------------------------------------------------------------------
+// stage 1 for periodic lines, one wave per line: the elimination of
+// k_lu_lines (the same pivots, partners, storage and fill-in placement, and
+// the same operations on every element, in the same order per element) with
+// its block loops spread over the lanes.  Per pivot, the blocks it touches
+// -- the pivot row's diagonal and right blocks, and each partner row's
+// blocks in the pivot column and in the right blocks' columns (at most 49)
+// -- are brought into LDS, eliminated there (the in-block steps and the
+// partner steps each in two phases: multipliers, then one element per
+// task), and written back.
+constexpr int LUP_SLOTS = 49;    // 0 D, 1..6 right blocks, 7 + 7a + b partner a (b 0: pivot column)
+__global__ __launch_bounds__(64) void k_lu_per(LineGeom g, cx<double> *mat, cx<double> *per, cx<double> *per2) {
+  using T = double;
+  __shared__ cx<T> blkv[LUP_SLOTS][25];
+  __shared__ cx<T> mlt[6][5];
+  const long line = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int n = g.n, np = n + 1;
+  const long L = g.L;
+  // bordered fill-in storage (cpenta1p.f:53-78)
+  for (long e = lane; e < (long)np * 50; e += 64) {
+    per[e * L + line] = cx<T>{0, 0};
+    per2[e * L + line] = cx<T>{0, 0};
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync();
+  if (lane < 25) {
+    const int e = lane;
+    per[((long)(0 * 2 + 0) * 25 + e) * L + line] = mat[((long)(0 * 5 + 0) * 25 + e) * L + line];
+    per[((long)(0 * 2 + 1) * 25 + e) * L + line] = mat[((long)(0 * 5 + 1) * 25 + e) * L + line];
+    per[((long)(1 * 2 + 1) * 25 + e) * L + line] = mat[((long)(1 * 5 + 0) * 25 + e) * L + line];
+    per2[((long)(0 * 2 + 0) * 25 + e) * L + line] = mat[((long)((n - 2) * 5 + 4) * 25 + e) * L + line];
+    per2[((long)(0 * 2 + 1) * 25 + e) * L + line] = mat[((long)((n - 1) * 5 + 3) * 25 + e) * L + line];
+    per2[((long)(1 * 2 + 1) * 25 + e) * L + line] = mat[((long)((n - 1) * 5 + 4) * 25 + e) * L + line];
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync();
+  for (int i = 0; i < n; i++) {
+    // candidates a = 0..5: rows / columns i+1..i+4 in the line, then the
+    // border rows / columns n-2, n-1 beyond i+4 (partners(), dir +1)
+    auto cand = [&](int a) {
+      const int t = a < 4 ? i + 1 + a : n - 6 + a;
+      const bool ok = a < 4 ? t <= n - 1 : t > i + 4;
+      return ok ? t : -1;
+    };
+    // where slot k lives in HBM (false: no such block); uniform over the wave
+    auto where = [&](int k, Blk<T> &B) -> bool {
+      if (k == 0) return blk(g, mat, per, per2, line, i, i, B);
+      if (k <= 6) {
+        const int c = cand(k - 1);
+        return c >= 0 && blk(g, mat, per, per2, line, i, c, B);
+      }
+      const int a = (k - 7) / 7, bb = (k - 7) % 7, t = cand(a);
+      Blk<T> Lb;
+      if (t < 0 || !blk(g, mat, per, per2, line, t, i, Lb)) return false;   // not a partner
+      if (bb == 0) { B = Lb; return true; }
+      Blk<T> U;
+      const int c = cand(bb - 1);
+      if (c < 0 || !blk(g, mat, per, per2, line, i, c, U)) return false;    // no right block there
+      if (!blk(g, mat, per, per2, line, t, c, B)) return false;
+      if (!g.exact && i == n - 6 && t == n - 4 && c == n - 2)             // cpenta1p.f:284, 345
+        B.p = mat + ((long)(t * 5 + 3) * 25) * L + line;
+      return true;
+    };
+    // bring the blocks into LDS (all loads in flight at once)
+    constexpr int PER_LANE = (LUP_SLOTS * 25 + 63) / 64;
+    cx<T> tmp[PER_LANE];
+#pragma unroll
+    for (int r = 0; r < PER_LANE; r++) {
+      const int e = r * 64 + lane, k = e / 25, el = e % 25;
+      Blk<T> B;
+      tmp[r] = (e < LUP_SLOTS * 25 && where(k, B)) ? B(el / 5, el % 5) : cx<T>{0, 0};
+    }
+#pragma unroll
+    for (int r = 0; r < PER_LANE; r++) {
+      const int e = r * 64 + lane;
+      if (e < LUP_SLOTS * 25) blkv[e / 25][e % 25] = tmp[r];
+    }
+    wave_sync();
+    // slot existence, uniform: right blocks and partners
+    unsigned uok = 0, pok = 0;
+#pragma unroll
+    for (int u = 0; u < 6; u++) {
+      Blk<T> B;
+      if (where(1 + u, B)) uok |= 1u << u;
+      if (where(7 + 7 * u, B)) pok |= 1u << u;
+    }
+    cx<T> *D = blkv[0];
+    // in-block LU of D, its row operations applied to the right blocks
+    for (int nc = 0; nc < 4; nc++) {
+      if (lane < 4 - nc) {
+        const int l = nc + 1 + lane;
+        const cx<T> mult = -cdiv(D[l * 5 + nc], D[nc * 5 + nc]);
+        mlt[0][l] = mult;
+        D[l * 5 + nc] = mult;
+      }
+      wave_sync();
+      const int nt = (4 - nc) * 5 * 7;          // tasks (l, q, b): b 0 D (q > nc), 1 + u right block u
+      for (int tk = lane; tk < nt; tk += 64) {
+        const int l = nc + 1 + tk / 35, q = (tk / 7) % 5, b = tk % 7;
+        const cx<T> mult = mlt[0][l];
+        if (b == 0) {
+          if (q > nc) D[l * 5 + q] = cfma(mult, D[nc * 5 + q], D[l * 5 + q]);
+        } else if (uok >> (b - 1) & 1) {
+          cx<T> *U = blkv[b];
+          U[l * 5 + q] = cfma(mult, U[nc * 5 + q], U[l * 5 + q]);
+        }
+      }
+      wave_sync();
+    }
+    // partner rows: the block in the pivot column becomes the multipliers,
+    // the partner's blocks in the right blocks' columns are updated
+    for (int nc = 0; nc < 5; nc++) {
+      if (lane < 30) {
+        const int a = lane / 5, m = lane % 5;
+        if (pok >> a & 1) {
+          cx<T> *Lb = blkv[7 + 7 * a];
+          const cx<T> fact = crecip(D[nc * 5 + nc]);
+          const cx<T> mult = -(Lb[m * 5 + nc] * fact);
+          mlt[a][m] = mult;
+          Lb[m * 5 + nc] = mult;
+        }
+      }
+      wave_sync();
+      const int nt = 6 * 5 * 5 * 7;             // tasks (a, m, q, b)
+      for (int tk = lane; tk < nt; tk += 64) {
+        const int a = tk / 175, m = (tk / 35) % 5, q = (tk / 7) % 5, b = tk % 7;
+        if (!(pok >> a & 1)) continue;
+        const cx<T> mult = mlt[a][m];
+        cx<T> *Lb = blkv[7 + 7 * a];
+        if (b == 0) {
+          if (q > nc) Lb[m * 5 + q] = cfma(mult, D[nc * 5 + q], Lb[m * 5 + q]);
+        } else if (uok >> (b - 1) & 1) {
+          cx<T> *Dst = blkv[7 + 7 * a + b];
+          Dst[m * 5 + q] = cfma(mult, blkv[b][nc * 5 + q], Dst[m * 5 + q]);
+        }
+      }
+      wave_sync();
+    }
+    // write the blocks back; the next pivot reads them after the stores
+#pragma unroll
+    for (int r = 0; r < PER_LANE; r++) {
+      const int e = r * 64 + lane, k = e / 25, el = e % 25;
+      Blk<T> B;
+      if (e < LUP_SLOTS * 25 && where(k, B)) B(el / 5, el % 5) = blkv[k][el];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // periodic (and reference-order) solve, one lane per line, in place on the
 // rhs array: element (row i, comp m) of line l at rhs[(l*sL + i*sR)*5 + m]
 template <typename T>
@@ -1794,7 +1945,7 @@ int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
 int penta_factor(PentaPlan *pl, hipStream_t s) {
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
   if (pl->periodic) {
-    hipLaunchKernelGGL(k_lu_lines<double>, dim3(grid_for(pl->L, 64)), dim3(64), 0, s, g,
+    hipLaunchKernelGGL(k_lu_per, dim3((unsigned)pl->L), dim3(64), 0, s, g,
                        (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2);
     LNS3D_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_records_per, dim3(grid_for(pl->L * (long)pl->n, 128)), dim3(128), 0, s, g,
