@@ -7,7 +7,7 @@
 //  1. Non-pivoted block LU.  Non-periodic lines: k_factor_chains, one wave
 //     per chain of rows with the rows being updated in an LDS window (lines
 //     of 16+ rows, when there are few lines, use a twisted order: two chains
-//     per line meeting in the middle).  Periodic lines: k_lu_lines, one lane
+//     per line meeting in the middle).  Periodic lines: k_lu_per, one wave
 //     per line on a line-interleaved copy.  The storage of negated
 //     multipliers and the in-pivot order follow the reference
 //     (cpenta1bc.f:43-495, cpenta1p.f:46-654), including the periodic
@@ -44,7 +44,7 @@ namespace lns3d {
 
 // ---------------------------------------------------------------------------
 // block addressing in the LU working copy.  Periodic lines: line-interleaved
-// ([row][slot][r][c][line], one lane per line in k_lu_lines); non-periodic
+// ([row][slot][r][c][line], one wave per line in k_lu_per); non-periodic
 // lines: line-major ([line][row][slot][r][c], one wave per chain in
 // k_factor_chains).  Element (r,c) of a block = p[(r*5 + c) * ldl].
 template <typename T> struct Blk {
@@ -125,76 +125,12 @@ __device__ __forceinline__ int pivot_row(const LineGeom &g, int s, int *dir) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 1: LU of each line (one lane per line)
-template <typename T>
-__global__ void k_lu_lines(LineGeom g, cx<T> *mat, cx<T> *per, cx<T> *per2) {
-  long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= g.L) return;
-  const int n = g.n;
-  if (g.periodic) {
-    // bordered fill-in storage (cpenta1p.f:53-78)
-    int np = n + 1;
-    for (int i = 0; i < np; i++)
-      for (int e = 0; e < 50; e++) {
-        per[((long)i * 50 + e) * g.L + line] = cx<T>{0, 0};
-        per2[((long)i * 50 + e) * g.L + line] = cx<T>{0, 0};
-      }
-    for (int e = 0; e < 25; e++) {
-      per[((long)(0 * 2 + 0) * 25 + e) * g.L + line] = mat[((long)(0 * 5 + 0) * 25 + e) * g.L + line];
-      per[((long)(0 * 2 + 1) * 25 + e) * g.L + line] = mat[((long)(0 * 5 + 1) * 25 + e) * g.L + line];
-      per[((long)(1 * 2 + 1) * 25 + e) * g.L + line] = mat[((long)(1 * 5 + 0) * 25 + e) * g.L + line];
-      per2[((long)(0 * 2 + 0) * 25 + e) * g.L + line] = mat[((long)((n - 2) * 5 + 4) * 25 + e) * g.L + line];
-      per2[((long)(0 * 2 + 1) * 25 + e) * g.L + line] = mat[((long)((n - 1) * 5 + 3) * 25 + e) * g.L + line];
-      per2[((long)(1 * 2 + 1) * 25 + e) * g.L + line] = mat[((long)((n - 1) * 5 + 4) * 25 + e) * g.L + line];
-    }
-  }
-  int cand[8];
-  for (int s = 0; s < n; s++) {
-    int dir;
-    const int i = pivot_row(g, s, &dir);
-    Blk<T> D, U[6];
-    int ucol[6], nu = 0;
-    blk(g, mat, per, per2, line, i, i, D);
-    int nc_ = partners(g, i, dir, cand);
-    for (int a = 0; a < nc_; a++)
-      if (blk(g, mat, per, per2, line, i, cand[a], U[nu])) ucol[nu++] = cand[a];
-    for (int nc = 0; nc < 4; nc++)
-      for (int l = nc + 1; l < 5; l++) {
-        cx<T> mult = -cdiv(D(l, nc), D(nc, nc));
-        D(l, nc) = mult;
-        for (int q = nc + 1; q < 5; q++) D(l, q) = cfma(mult, D(nc, q), D(l, q));
-        for (int u = 0; u < nu; u++)
-          for (int q = 0; q < 5; q++) U[u](l, q) = cfma(mult, U[u](nc, q), U[u](l, q));
-      }
-    for (int a = 0; a < nc_; a++) {
-      int t = cand[a];
-      Blk<T> Lb;
-      if (!blk(g, mat, per, per2, line, t, i, Lb)) continue;
-      Blk<T> Dst[6];
-      for (int u = 0; u < nu; u++) {
-        blk(g, mat, per, per2, line, t, ucol[u], Dst[u]);
-        if (g.periodic && !g.exact && i == n - 6 && t == n - 4 && ucol[u] == n - 2)
-          Dst[u].p = mat + ((long)(t * 5 + 3) * 25) * g.L + line;
-      }
-      for (int nc = 0; nc < 5; nc++) {
-        cx<T> fact = crecip(D(nc, nc));
-        for (int m = 0; m < 5; m++) {
-          cx<T> mult = -(Lb(m, nc) * fact);
-          Lb(m, nc) = mult;
-          for (int q = nc + 1; q < 5; q++) Lb(m, q) = cfma(mult, D(nc, q), Lb(m, q));
-          for (int u = 0; u < nu; u++)
-            for (int q = 0; q < 5; q++) Dst[u](m, q) = cfma(mult, U[u](nc, q), Dst[u](m, q));
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// stage 1 for periodic lines, one wave per line: the elimination of
-// k_lu_lines (the same pivots, partners, storage and fill-in placement, and
-// the same operations on every element, in the same order per element) with
-// its block loops spread over the lanes.  Per pivot, the blocks it touches
+// stage 1 for periodic lines, one wave per line: the bordered block LU in
+// the reference's order (cpenta1p.f:46-654): pivots 0..n-1, each with its
+// partner rows i+1..i+4 and the border rows n-2, n-1 (partners()), the
+// storage of blk() and the fill-in placement of the shipped factor unless
+// exact is requested; every element receives its operations in the
+// reference's order.  Per pivot, the blocks it touches
 // -- the pivot row's diagonal and right blocks, and each partner row's
 // blocks in the pivot column and in the right blocks' columns (at most 49)
 // -- are brought into LDS, eliminated there (the in-block steps and the
@@ -1474,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
 
 // ---------------------------------------------------------------------------
 // periodic lines (cpenta1p.f / cpenta2p.f): records from the bordered LU of
-// k_lu_lines and a streaming solve.  The LU's band gives the same recurrence
+// k_lu_per and a streaming solve.  The LU's band gives the same recurrence
 // as non-periodic lines; the border (rows n-2, n-1 in every column: per2;
 // columns n-2, n-1 in every row: per) becomes two more per-row records:
 //   p2[row c] = (Linv_{n-2} per2(n-2,c), Linv_{n-1} per2(n-1,c)): the forward
