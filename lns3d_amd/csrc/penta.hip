@@ -977,10 +977,14 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
   // rows kept for the boundary-row blocks: y_{n-5}, y_{n-4} (forward),
   // x_3, x_4 (backward), as broadcast values
   cx<T> s0[5], s1[5];
+  // the term in the newest value: two FMA accumulators (four independent
+  // chains over re and im), 22 instructions
   auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
-    cx<T> t0 = cmul(w[0], yn[0]), t1 = cmul(w[1], yn[1]), t2 = cmul(w[2], yn[2]),
-          t3 = cmul(w[3], yn[3]), t4 = cfma(w[4], yn[4], a0);
-    return (t0 + t1) + ((t2 + t3) + t4);
+    cx<T> a = cfma(w[0], yn[0], a0), b = cmul(w[1], yn[1]);
+    a = cfma(w[2], yn[2], a);
+    b = cfma(w[3], yn[3], b);
+    a = cfma(w[4], yn[4], a);
+    return a + b;
   };
   auto pre0 = [&](const cx<T> (&w)[5], cx<T> c, const cx<T> (&yo)[5]) -> cx<T> {
     cx<T> a = cfma(w[0], yo[0], c), b = cmul(w[1], yo[1]);
@@ -1230,10 +1234,14 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
   cx<T> yA[5], yB[5];
   cx<T> w2s[3][5], w1s[3][5], cs[3];
   cx<T> s0[5], s1[5];
+  // the term in the newest value: two FMA accumulators (four independent
+  // chains over re and im), 22 instructions
   auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
-    cx<T> t0 = cmul(w[0], yn[0]), t1 = cmul(w[1], yn[1]), t2 = cmul(w[2], yn[2]),
-          t3 = cmul(w[3], yn[3]), t4 = cfma(w[4], yn[4], a0);
-    return (t0 + t1) + ((t2 + t3) + t4);
+    cx<T> a = cfma(w[0], yn[0], a0), b = cmul(w[1], yn[1]);
+    a = cfma(w[2], yn[2], a);
+    b = cfma(w[3], yn[3], b);
+    a = cfma(w[4], yn[4], a);
+    return a + b;
   };
   auto pre0 = [&](const cx<T> (&w)[5], cx<T> c, const cx<T> (&yo)[5]) -> cx<T> {
     cx<T> a = cfma(w[0], yo[0], c), b = cmul(w[1], yo[1]);
@@ -1586,10 +1594,14 @@ __global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<doubl
   // ---- compute wave
   cx<T> yA[5], yB[5];
   cx<T> w2s[3][5], w1s[3][5], cs[3];
+  // the term in the newest value: two FMA accumulators (four independent
+  // chains over re and im), 22 instructions
   auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
-    cx<T> t0 = cmul(w[0], yn[0]), t1 = cmul(w[1], yn[1]), t2 = cmul(w[2], yn[2]),
-          t3 = cmul(w[3], yn[3]), t4 = cfma(w[4], yn[4], a0);
-    return (t0 + t1) + ((t2 + t3) + t4);
+    cx<T> a = cfma(w[0], yn[0], a0), b = cmul(w[1], yn[1]);
+    a = cfma(w[2], yn[2], a);
+    b = cfma(w[3], yn[3], b);
+    a = cfma(w[4], yn[4], a);
+    return a + b;
   };
   auto pre0 = [&](const cx<T> (&w)[5], cx<T> c, const cx<T> (&yo)[5]) -> cx<T> {
     cx<T> a = cfma(w[0], yo[0], c), b = cmul(w[1], yo[1]);
