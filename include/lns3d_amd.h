@@ -174,10 +174,11 @@ int lns3d_graph_destroy(lns3d_graph *g);
 /* autotuner of the line solves: times every solve-kernel variant on
  * synthetic lines of n rows x lines and uses the fastest for plans of that
  * shape created later.  Variants: 0 one line per workgroup, 1 four lines,
- * 2 twisted (two chains per line, n >= 16), 3 two lines, 4 three lines.
- * times_ms (may be NULL) receives LNS3D_TUNE_VARIANTS timings in ms (-1: not
- * applicable).  No counterpart in the reference. */
-#define LNS3D_TUNE_VARIANTS 5
+ * 2 twisted (two chains per line, n >= 16), 3 two lines, 4 three lines,
+ * 5 twisted with chain steps taken in pairs.  times_ms (may be NULL)
+ * receives LNS3D_TUNE_VARIANTS timings in ms (-1: not applicable).  No
+ * counterpart in the reference. */
+#define LNS3D_TUNE_VARIANTS 6
 int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *times_ms);
 
 /* Host-array driving of the time loop of src/impdrv3d.f90:126-406 (for

@@ -492,6 +492,8 @@ extern "C" int lns3d_graph_destroy(lns3d_graph *gr) {
   return 0;
 }
 
+static_assert(PENTA_NVARIANT == LNS3D_TUNE_VARIANTS, "the header's variant count");
+
 // autotuner of the line-solve kernels: measures every variant on synthetic
 // lines of n rows x lines and keeps the fastest for plans of that shape
 // created afterwards; times_ms (may be NULL) receives LNS3D_TUNE_VARIANTS

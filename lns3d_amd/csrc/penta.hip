@@ -860,12 +860,19 @@ __device__ __forceinline__ void parallel_part_x(double2 *bp, const cx<T> *rp, co
   *reinterpret_cast<cx<T> *>(bp + mm) = acc;
 }
 
-// the loader side of one pass over nchunk chunks (loader wave w = 0, 1, 2):
-// one barrier after the prologue and one per chunk, as on the compute side
-// (compute_pass)
-template <typename Issue, typename Prep>
+// The loader side of one pass over nchunk chunks (loader wave w = 0, 1, 2)
+// with a ring of NBUF (4 or 5) buffers.  Chunk k belongs to wave k % 3,
+// which starts its copies in iteration k - (NBUF - 1) and, in iteration
+// k - 1, waits for them and prepares the chunk before it starts any further
+// copies.  So a wave has one chunk's copies in flight when it waits, and
+// vmcnt(0) waits for exactly those (with NBUF = 5 the wave preparing chunk
+// k + 1 is the one that then starts chunk k + 4; in the prologue wave 0
+// waits for chunks 0 and 3).  One barrier after the prologue and one per
+// chunk, as in compute_pass.
+template <int NBUF, typename Issue, typename Prep>
 __device__ __forceinline__ void stream_pass(int nchunk, int w, Issue issue, Prep prep) {
-  if (w < nchunk) issue(w);
+  static_assert(NBUF == 4 || NBUF == 5, "one chunk in flight per wave when it waits");
+  for (int c = w; c < NBUF - 1 && c < nchunk; c += 3) issue(c);
   if (w == 0) {
     __builtin_amdgcn_s_waitcnt(0);
     prep(0);
@@ -876,7 +883,7 @@ __device__ __forceinline__ void stream_pass(int nchunk, int w, Issue issue, Prep
       __builtin_amdgcn_s_waitcnt(0);      // this wave's copies of chunk k + 1
       prep(k + 1);
     }
-    if (k + 3 < nchunk && k % 3 == w) issue(k + 3);   // into the buffer chunk k - 1 left
+    if (k + NBUF - 1 < nchunk && (k + NBUF - 1) % 3 == w) issue(k + NBUF - 1);   // buffer of chunk k - 1
     loader_barrier();
   }
 }
@@ -959,8 +966,8 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
           parallel_part(bst[buf][gg][j], rec[buf][gg] + j * (fw ? 60 : 66), lane % 5, fw);
       }
     };
-    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
-    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, false); }, [&](int k) { prep(k, false); });
+    stream_pass<NBUF>(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
+    stream_pass<NBUF>(nchunk, wave - 1, [&](int k) { issue(k, false); }, [&](int k) { prep(k, false); });
     return;
   }
 
@@ -1119,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
 // two values (rows h+2, h+3) to chain 0, whose last two rows carry those
 // columns.  Backward: chain 0 starts at row h+1; chain 1 starts two steps
 // later from chain 0's x_{h+1}, x_h.
-template <typename T, int R, int G, bool PAIRS>
+template <typename T, int R, int G, bool PAIRS, int NBUF>
 __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
                                                  const cx<T> *__restrict__ bwd,
                                                  const cx<T> *__restrict__ tail,
@@ -1127,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
                                                  const cx<T> *__restrict__ xb, double2 *rhs,
                                                  long sL, long sR) {
   constexpr int C = 2 * G;
-  constexpr int NBUF = 4, BUFE = R * 66;
+  constexpr int BUFE = R * 66;
   static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
   __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][C][BUFE];
   __shared__ double2 bst[NBUF][C][R][5];   // rhs rows, then c (or d) in place
@@ -1225,8 +1232,8 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
         }
       }
     };
-    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
-    stream_pass(nchunk, wave - 1, [&](int k) { issue(k, false); }, [&](int k) { prep(k, false); });
+    stream_pass<NBUF>(nchunk, wave - 1, [&](int k) { issue(k, true); }, [&](int k) { prep(k, true); });
+    stream_pass<NBUF>(nchunk, wave - 1, [&](int k) { issue(k, false); }, [&](int k) { prep(k, false); });
     return;
   }
 
@@ -1791,7 +1798,7 @@ int penta_variant_for(int n, long lines, int prec) {
     auto it = g_tuned.find(std::make_tuple(n, lines, prec));
     if (it != g_tuned.end()) return it->second;
   }
-  if (n >= 16 && lines < 512) return PENTA_TWISTED;
+  if (n >= 16 && lines < 512) return prec ? PENTA_TWISTED_PAIRS : PENTA_TWISTED;
   return lines >= 512 ? PENTA_PLAIN3 : PENTA_PLAIN1;
 }
 
@@ -1810,9 +1817,10 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
   pl->L = lines;
   pl->prec = prec;
   if (pl->n < 7 || lines < 1) return set_error(__FILE__, __LINE__, "line length must be >= 7 unknowns");
-  if (variant == PENTA_TWISTED && pl->n < 16) variant = PENTA_PLAIN1;
+  const bool twisted = variant == PENTA_TWISTED || variant == PENTA_TWISTED_PAIRS;
+  if (twisted && pl->n < 16) variant = PENTA_PLAIN1;
   pl->variant = periodic ? PENTA_PLAIN1 : variant;
-  pl->h = (!periodic && pl->variant == PENTA_TWISTED) ? (pl->n - 1) / 2 : -1;
+  pl->h = (!periodic && (pl->variant == PENTA_TWISTED || pl->variant == PENTA_TWISTED_PAIRS)) ? (pl->n - 1) / 2 : -1;
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
@@ -1831,7 +1839,7 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
     LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * es));
     LNS3D_CHECK(hipMalloc(&pl->tail, (size_t)150 * lines * (pl->h >= 0 ? 2 : 1) * es));
     if (pl->h >= 0) LNS3D_CHECK(hipMalloc(&pl->mid, (size_t)100 * lines * sizeof(cx<double>)));
-    if (pl->h >= 0 && prec) {
+    if (pl->variant == PENTA_TWISTED_PAIRS) {
       // 25 per (line, chain, pair), plus one complex past the end (a copy of
       // an odd count of complex float moves 16 B at a time)
       const size_t nx = ((size_t)lines * 2 * ((pl->h + 3) / 2) * 25 + 2) * es;
@@ -1880,11 +1888,16 @@ int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
                        (const cx<double> *)pl->lu, (cx<double> *)pl->fwd, (cx<double> *)pl->bwd,
                        (cx<double> *)pl->tail);
   LNS3D_CHECK(hipGetLastError());
-  if (pl->h >= 0 && pl->prec) {  // twisted, complex float: the pair rewrite for the solve's chunk size
+  if (pl->variant == PENTA_TWISTED_PAIRS) {   // the pair rewrite for the solve's chunk size
     const long groups = pl->L * 4 * ((pl->h + 3) / 2);
-    hipLaunchKernelGGL((k_pair_records<float>), dim3(grid_for(groups, 8)), dim3(256), 0, s, pl->n, pl->h, pl->L,
-                       tw_chunk(true), (cx<float> *)pl->fwd, (cx<float> *)pl->bwd, (cx<float> *)pl->xf,
-                       (cx<float> *)pl->xb);
+    if (pl->prec)
+      hipLaunchKernelGGL((k_pair_records<float>), dim3(grid_for(groups, 8)), dim3(256), 0, s, pl->n, pl->h, pl->L,
+                         tw_chunk(true), (cx<float> *)pl->fwd, (cx<float> *)pl->bwd, (cx<float> *)pl->xf,
+                         (cx<float> *)pl->xb);
+    else
+      hipLaunchKernelGGL((k_pair_records<double>), dim3(grid_for(groups, 8)), dim3(256), 0, s, pl->n, pl->h, pl->L,
+                         tw_chunk(false), (cx<double> *)pl->fwd, (cx<double> *)pl->bwd, (cx<double> *)pl->xf,
+                         (cx<double> *)pl->xb);
     LNS3D_CHECK(hipGetLastError());
   }
   return 0;
@@ -1922,13 +1935,16 @@ static void launch_solve_bc(const PentaPlan *pl, void *rhs, long sL, long sR, hi
   auto *x = (double2 *)rhs;
   auto wgs = [&](int G) { return dim3((unsigned)((pl->L + G - 1) / G)); };
   switch (pl->variant) {
-    case PENTA_TWISTED:
-      // complex float: pairs of steps (factor-time rewrite); complex double:
-      // one step at a time (its streaming, not its recurrence, bounds the
-      // paired form)
+    // twisted lines, a ring of four buffers (five measured slower: the wave
+    // preparing a chunk then also starts the next copies)
+    case PENTA_TWISTED:        // one step at a time
+      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1, false, 4>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f,
+                         b, t, (const cx<T> *)pl->xf, (const cx<T> *)pl->xb, x, sL, sR);
+      break;
+    case PENTA_TWISTED_PAIRS:  // pairs of steps (factor-time rewrite)
       static_assert(tw_chunk(!D) == (D ? 6 : 12), "the pair rewrite assumes the solve's chunk size");
-      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1, !D>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b, t,
-                         (const cx<T> *)pl->xf, (const cx<T> *)pl->xb, x, sL, sR);
+      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1, true, 4>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f, b,
+                         t, (const cx<T> *)pl->xf, (const cx<T> *)pl->xb, x, sL, sR);
       break;
     case PENTA_PLAIN2:
       hipLaunchKernelGGL((k_solve_bc<T, D ? 6 : 12, 2>), wgs(2), dim3(256), 0, s, pl->n, pl->L, f, b, t, x, sL, sR);
@@ -2002,7 +2018,7 @@ int penta_tune(int n, long lines, int prec, int *variant, float *times_ms) {
   int rc = 0;
   for (int v = 0; v < PENTA_NVARIANT && !rc; v++) {
     if (times_ms) times_ms[v] = -1.0f;
-    if (v == PENTA_TWISTED && n < 16) continue;
+    if ((v == PENTA_TWISTED || v == PENTA_TWISTED_PAIRS) && n < 16) continue;
     PentaPlan pl;
     rc = penta_plan_create_variant(&pl, 0, 0, n, lines, prec, v);
     if (!rc) {

@@ -28,7 +28,7 @@ struct PentaPlan {
 // solve-kernel variants of non-periodic lines: one line per workgroup with a
 // deep LDS ring, four lines per workgroup, or twisted lines (two chains)
 enum { PENTA_PLAIN1 = 0, PENTA_PLAIN4 = 1, PENTA_TWISTED = 2, PENTA_PLAIN2 = 3, PENTA_PLAIN3 = 4,
-       PENTA_NVARIANT = 5 };
+       PENTA_TWISTED_PAIRS = 5, PENTA_NVARIANT = 6 };
 // the variant a plan of this shape uses: the autotuned one when
 // penta_tune has measured the shape, else a fixed rule
 int penta_variant_for(int n, long lines, int prec);

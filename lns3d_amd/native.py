@@ -262,8 +262,8 @@ def ctx_bytes(p, precision=0) -> int:
 
 
 # line-solve variants the autotuner times (LNS3D_TUNE_VARIANTS)
-TUNE_VARIANTS = 5
-VARIANT_NAMES = ("1 line/WG", "4 lines/WG", "twisted", "2 lines/WG", "3 lines/WG")
+TUNE_VARIANTS = 6
+VARIANT_NAMES = ("1 line/WG", "4 lines/WG", "twisted", "2 lines/WG", "3 lines/WG", "twisted pairs")
 
 
 def tune_lines(n: int, lines: int, precision: int = 0):

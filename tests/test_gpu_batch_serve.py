@@ -88,13 +88,14 @@ def test_refined_fixed_matches_fp64(torch_cuda):
 
 
 def test_tuner_choice_solves(torch_cuda, orc):
-    from lns3d_amd.native import Penta, tune_lines
+    from lns3d_amd.native import Penta, tune_lines, TUNE_VARIANTS
     from tests.penta_cases import random_lines, relerr
     torch = torch_cuda
     nsys, n = 70, 40
     var, times = tune_lines(n, nsys, 0)
     print("tuned", var, times, flush=True)
-    assert var in (0, 1, 2) and times[var] > 0 and times[var] == min(t for t in times if t > 0)
+    assert len(times) == TUNE_VARIANTS and all(t > 0 for t in times)   # every variant applies at n = 40
+    assert 0 <= var < TUNE_VARIANTS and times[var] == min(times)
     mat, rhs = random_lines(nsys, n, 2, seed=5)
     m, xr = mat.copy(order="F"), rhs.copy(order="F")
     orc.cpenta_bc(2, m, xr, 2)
