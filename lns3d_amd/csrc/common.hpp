@@ -87,6 +87,14 @@ template <int OFF, typename T> __device__ __forceinline__ void row_bcast5_at(cx<
   y[0] = row_bcast<OFF>(v); y[1] = row_bcast<OFF + 1>(v); y[2] = row_bcast<OFF + 2>(v);
   y[3] = row_bcast<OFF + 3>(v); y[4] = row_bcast<OFF + 4>(v);
 }
+// The chip places workgroup b of a launch on XCD b % 8 (round robin over its
+// eight XCDs, each with its own L2).  xcd_tile(b, nb) numbers the tiles so
+// that each XCD works through a contiguous range of them (a bijection of
+// [0, nb)): neighbouring tiles, which share stencil rows, share an L2.
+__device__ __forceinline__ long xcd_tile(long b, long nb) {
+  const long q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+  return x * q + (x < r ? x : r) + k;
+}
 // orders the LDS accesses of one wave before / after it (lanes of the wave
 // exchanging values through LDS)
 __device__ __forceinline__ void wave_sync() {

@@ -115,14 +115,6 @@ __device__ __forceinline__ int partners(const LineGeom &g, int i, int dir, int *
   return k;
 }
 
-// pivot order: row of elimination step s and its direction
-__device__ __forceinline__ int pivot_row(const LineGeom &g, int s, int *dir) {
-  if (g.h < 0) { *dir = 1; return s; }
-  const int nb = g.n - g.h - 2;                  // bottom rows n-1 .. h+2
-  if (s < nb) { *dir = -1; return g.n - 1 - s; }
-  *dir = 1;
-  return s - nb;
-}
 
 // ---------------------------------------------------------------------------
 // stage 1 for periodic lines, one wave per line: the bordered block LU in

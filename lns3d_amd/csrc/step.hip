@@ -608,7 +608,7 @@ int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *m
 // RHS: c1 = d/dxi v first (stored), then the point operator
 // one lane per (point, component), so a wave's stencil loads are contiguous
 __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v, double2 *c1) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= 5L * nx * ny) return;
   const long q = t / 5;
   const int e = (int)(t - 5 * q);
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
   __shared__ double2 sv[5 * RT], sg1[5 * RT], sd[NDER * RT], sr[5 * RT];
   const int nx = p.nx, ny = p.ny;
   const long P = (long)nx * ny;
-  const long q0 = (long)blockIdx.x * RT;
+  const long q0 = xcd_tile(blockIdx.x, gridDim.x) * RT;   // tiles of one XCD: consecutive rows
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int npt = (int)min((long)RT, P - q0);
   if (tid < 5 * npt) {
@@ -873,7 +873,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   const lns3d_params &p = A.p;
   const int nx = p.nx, ny = p.ny;
   const long P = (long)nx * ny;
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= P) return;
   const int i = q % nx, j = q / nx;
   const int N = dir == 1 ? nx : ny, pos = dir == 1 ? i : j;
