@@ -572,10 +572,13 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
       int r = real(vr), c = real(vc);
       return tw && (r == h || r == h + 1) && (c == h || c == h + 1);
     };
-    auto load_row = [&](int vr) {
-      const cx<T> *src = lul + (long)real(vr) * 125;
-      for (int e = lane; e < 125; e += 64) {
-        cx<T> x = src[e];
+    // a row into the window: entries e = lane, lane + 64 (x0, x1) of row vr
+    auto put_row = [&](int vr, cx<T> x0, cx<T> x1) {
+#pragma unroll
+      for (int h2 = 0; h2 < 2; h2++) {
+        const int e = lane + 64 * h2;
+        if (e >= 125) continue;
+        cx<T> x = h2 == 0 ? x0 : x1;
         if (tt == 1 && tw) {            // bottom chain: middle blocks start at zero
           int sl = e / 25;
           for (int vc = vr - 4; vc <= vr + 4; vc++)
@@ -583,6 +586,10 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
         }
         win[w][vr % FW][e] = x;
       }
+    };
+    auto load_row = [&](int vr) {
+      const cx<T> *src = lul + (long)real(vr) * 125;
+      put_row(vr, src[lane], lane + 64 < 125 ? src[lane + 64] : cx<T>{0, 0});
     };
     // write back a row: own = all slots (pivot rows) or the chain's share
     auto store_row = [&](int vr, bool all) {
@@ -607,6 +614,15 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
       for (int vr = v0; vr <= min(v0 + FW - 1, lim); vr++) load_row(vr);
     wsync();
     for (int v = v0; v < v1; v++) {
+      // row v+5 comes into the window when row v leaves it; its loads are
+      // issued now, so that they arrive while row v is eliminated
+      const bool pf_ok = v + FW <= lim;
+      cx<T> pf0, pf1;
+      {
+        const cx<T> *src = lul + (long)real(pf_ok ? v + FW : v) * 125;
+        pf0 = src[lane];
+        pf1 = lane + 64 < 125 ? src[lane + 64] : cx<T>{0, 0};
+      }
       const int sd = slot(v, v);
       cx<T> *D = B(v, sd);
       // blocks right of the pivot (columns uc, slots us) and partner rows
@@ -683,7 +699,7 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
       // row v is final: write it back and bring in row v+5
       store_row(v, true);
       wsync();
-      if (v + FW <= lim) load_row(v + FW);
+      if (pf_ok) put_row(v + FW, pf0, pf1);
       wsync();
     }
     // rows the chain updated but did not pivot
