@@ -579,7 +579,7 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
         const int e = lane + 64 * h2;
         if (e >= 125) continue;
         cx<T> x = h2 == 0 ? x0 : x1;
-        if (tt == 1 && tw) {            // bottom chain: middle blocks start at zero
+        if (tt == 1 && tw && (real(vr) == h || real(vr) == h + 1)) {   // bottom chain: middle blocks start at zero
           int sl = e / 25;
           for (int vc = vr - 4; vc <= vr + 4; vc++)
             if (slot(vr, vc) == sl && middle(vr, vc)) x = cx<T>{0, 0};
@@ -594,6 +594,10 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
     // write back a row: own = all slots (pivot rows) or the chain's share
     auto store_row = [&](int vr, bool all) {
       cx<T> *dst = lul + (long)real(vr) * 125;
+      if (all) {
+        for (int e = lane; e < 125; e += 64) dst[e] = win[w][vr % FW][e];
+        return;
+      }
       for (int e = lane; e < 125; e += 64) {
         const int sl = e / 25;
         int vcol = -1;
@@ -625,76 +629,136 @@ __global__ __launch_bounds__(128) void k_factor_chains(LineGeom g, cx<double> *l
       }
       const int sd = slot(v, v);
       cx<T> *D = B(v, sd);
-      // blocks right of the pivot (columns uc, slots us) and partner rows
-      // (pr); filled and read with constant indices only, so that they stay
-      // in registers (a run-time index puts a local array in scratch)
-      int uc[4] = {0, 0, 0, 0}, us[4] = {0, 0, 0, 0}, pr[4] = {0, 0, 0, 0}, nu = 0, nparts = 0;
+      // Interior pivot: rows v..v+4 (those up to lim) are interior rows of
+      // the line, and v+2 <= lim.  Then the right blocks are columns v+1,
+      // v+2 and the partners rows v+1, v+2, at slots fixed by the chain
+      // direction -- the generic path below with its lists in constants
+      // (the same operations on every element, in the same order).
+      const int sg = tt == 0 ? 1 : -1;
+      const bool fast = v >= 2 && v + 2 <= lim && min(v + 4, lim) <= n - 3;
+      if (fast) {
+        cx<T> *U0 = B(v, 2 + sg), *U1 = B(v, 2 + 2 * sg);
 #pragma unroll
-      for (int d = 1; d <= 4; d++) {
-        const int vc = v + d;
-        const int su = vc <= lim ? slot(v, vc) : -1;
-        const bool hp = vc <= lim && slot(vc, v) >= 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          if (su >= 0 && nu == i) { uc[i] = vc; us[i] = su; }
-          if (hp && nparts == i) pr[i] = vc;
-        }
-        nu += su >= 0;
-        nparts += hp;
-      }
-      auto pick = [](const int (&a)[4], int i) { return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3]; };
-      // in-block LU of D, its row operations applied to the right blocks
-      // (loops over nc unrolled: the task decodes divide by constants)
-#pragma unroll
-      for (int nc = 0; nc < 4; nc++) {
-        if (lane < 4 - nc) {
-          const int l = nc + 1 + lane;
-          D[l * 5 + nc] = -cdiv(D[l * 5 + nc], D[nc * 5 + nc]);
-        }
-        wsync();
-        const int nd = (4 - nc) * (4 - nc), ntask = nd + nu * (4 - nc) * 5;
-        for (int k = lane; k < ntask; k += 64) {
-          if (k < nd) {
-            const int l = nc + 1 + k / (4 - nc), q = nc + 1 + k % (4 - nc);
-            D[l * 5 + q] = cfma(D[l * 5 + nc], D[nc * 5 + q], D[l * 5 + q]);
-          } else {
-            const int kk = k - nd, u = kk / ((4 - nc) * 5), rem = kk % ((4 - nc) * 5);
-            const int l = nc + 1 + rem / 5, q = rem % 5;
-            cx<T> *U = B(v, pick(us, u));
-            U[l * 5 + q] = cfma(D[l * 5 + nc], U[nc * 5 + q], U[l * 5 + q]);
+        for (int nc = 0; nc < 4; nc++) {
+          if (lane < 4 - nc) {
+            const int l = nc + 1 + lane;
+            D[l * 5 + nc] = -cdiv(D[l * 5 + nc], D[nc * 5 + nc]);
           }
-        }
-        wsync();
-      }
-      // partner rows: the block in the pivot column becomes the multipliers
-#pragma unroll
-      for (int nc = 0; nc < 5; nc++) {
-        const cx<T> fact = crecip(D[nc * 5 + nc]);
-        if (lane < nparts * 5) {
-          cx<T> *Lb = B(pick(pr, lane / 5), slot(pick(pr, lane / 5), v));
-          const int mm = lane % 5;
-          Lb[mm * 5 + nc] = -(Lb[mm * 5 + nc] * fact);
-        }
-        wsync();
-        const int nl = nparts * 5 * (4 - nc), ntask = nl + nparts * nu * 25;
-        for (int k = lane; k < ntask; k += 64) {
-          if (k < nl) {
-            const int pp = k / (5 * (4 - nc)), rem = k % (5 * (4 - nc));
-            const int mm = rem / (4 - nc), q = nc + 1 + rem % (4 - nc);
-            cx<T> *Lb = B(pick(pr, pp), slot(pick(pr, pp), v));
-            Lb[mm * 5 + q] = cfma(Lb[mm * 5 + nc], D[nc * 5 + q], Lb[mm * 5 + q]);
-          } else {
-            const int kk = k - nl, pp = kk / (nu * 25), rem = kk % (nu * 25);
-            const int u = rem / 25, mm = (rem % 25) / 5, q = rem % 5;
-            const int sdst = slot(pick(pr, pp), pick(uc, u));
-            if (sdst < 0) continue;      // no such block (does not occur in the band)
-            cx<T> *Lb = B(pick(pr, pp), slot(pick(pr, pp), v));
-            cx<T> *Dst = B(pick(pr, pp), sdst);
-            const cx<T> *U = B(v, pick(us, u));
-            Dst[mm * 5 + q] = cfma(Lb[mm * 5 + nc], U[nc * 5 + q], Dst[mm * 5 + q]);
+          wsync();
+          const int nd = (4 - nc) * (4 - nc), ntask = nd + 2 * (4 - nc) * 5;
+          for (int k = lane; k < ntask; k += 64) {
+            if (k < nd) {
+              const int l = nc + 1 + k / (4 - nc), q = nc + 1 + k % (4 - nc);
+              D[l * 5 + q] = cfma(D[l * 5 + nc], D[nc * 5 + q], D[l * 5 + q]);
+            } else {
+              const int kk = k - nd, u = kk / ((4 - nc) * 5), rem = kk % ((4 - nc) * 5);
+              const int l = nc + 1 + rem / 5, q = rem % 5;
+              cx<T> *U = u ? U1 : U0;
+              U[l * 5 + q] = cfma(D[l * 5 + nc], U[nc * 5 + q], U[l * 5 + q]);
+            }
           }
+          wsync();
         }
-        wsync();
+        cx<T> *L0 = B(v + 1, 2 - sg), *L1 = B(v + 2, 2 - 2 * sg);
+#pragma unroll
+        for (int nc = 0; nc < 5; nc++) {
+          const cx<T> fact = crecip(D[nc * 5 + nc]);
+          if (lane < 10) {
+            cx<T> *Lb = lane < 5 ? L0 : L1;
+            const int mm = lane % 5;
+            Lb[mm * 5 + nc] = -(Lb[mm * 5 + nc] * fact);
+          }
+          wsync();
+          const int nl = 2 * 5 * (4 - nc), ntask = nl + 2 * 2 * 25;
+          for (int k = lane; k < ntask; k += 64) {
+            if (k < nl) {
+              const int pp = k / (5 * (4 - nc)), rem = k % (5 * (4 - nc));
+              const int mm = rem / (4 - nc), q = nc + 1 + rem % (4 - nc);
+              cx<T> *Lb = pp ? L1 : L0;
+              Lb[mm * 5 + q] = cfma(Lb[mm * 5 + nc], D[nc * 5 + q], Lb[mm * 5 + q]);
+            } else {
+              const int kk = k - nl, pp = kk / 50, rem = kk % 50;
+              const int u = rem / 25, mm = (rem % 25) / 5, q = rem % 5;
+              cx<T> *Lb = pp ? L1 : L0;
+              cx<T> *Dst = B(v + 1 + pp, 2 + sg * (u - pp));
+              const cx<T> *U = u ? U1 : U0;
+              Dst[mm * 5 + q] = cfma(Lb[mm * 5 + nc], U[nc * 5 + q], Dst[mm * 5 + q]);
+            }
+          }
+          wsync();
+        }
+      } else {
+        // blocks right of the pivot (columns uc, slots us) and partner rows
+        // (pr); filled and read with constant indices only, so that they stay
+        // in registers (a run-time index puts a local array in scratch)
+        int uc[4] = {0, 0, 0, 0}, us[4] = {0, 0, 0, 0}, pr[4] = {0, 0, 0, 0}, nu = 0, nparts = 0;
+#pragma unroll
+        for (int d = 1; d <= 4; d++) {
+          const int vc = v + d;
+          const int su = vc <= lim ? slot(v, vc) : -1;
+          const bool hp = vc <= lim && slot(vc, v) >= 0;
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            if (su >= 0 && nu == i) { uc[i] = vc; us[i] = su; }
+            if (hp && nparts == i) pr[i] = vc;
+          }
+          nu += su >= 0;
+          nparts += hp;
+        }
+        auto pick = [](const int (&a)[4], int i) { return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3]; };
+        // in-block LU of D, its row operations applied to the right blocks
+        // (loops over nc unrolled: the task decodes divide by constants)
+#pragma unroll
+        for (int nc = 0; nc < 4; nc++) {
+          if (lane < 4 - nc) {
+            const int l = nc + 1 + lane;
+            D[l * 5 + nc] = -cdiv(D[l * 5 + nc], D[nc * 5 + nc]);
+          }
+          wsync();
+          const int nd = (4 - nc) * (4 - nc), ntask = nd + nu * (4 - nc) * 5;
+          for (int k = lane; k < ntask; k += 64) {
+            if (k < nd) {
+              const int l = nc + 1 + k / (4 - nc), q = nc + 1 + k % (4 - nc);
+              D[l * 5 + q] = cfma(D[l * 5 + nc], D[nc * 5 + q], D[l * 5 + q]);
+            } else {
+              const int kk = k - nd, u = kk / ((4 - nc) * 5), rem = kk % ((4 - nc) * 5);
+              const int l = nc + 1 + rem / 5, q = rem % 5;
+              cx<T> *U = B(v, pick(us, u));
+              U[l * 5 + q] = cfma(D[l * 5 + nc], U[nc * 5 + q], U[l * 5 + q]);
+            }
+          }
+          wsync();
+        }
+        // partner rows: the block in the pivot column becomes the multipliers
+#pragma unroll
+        for (int nc = 0; nc < 5; nc++) {
+          const cx<T> fact = crecip(D[nc * 5 + nc]);
+          if (lane < nparts * 5) {
+            cx<T> *Lb = B(pick(pr, lane / 5), slot(pick(pr, lane / 5), v));
+            const int mm = lane % 5;
+            Lb[mm * 5 + nc] = -(Lb[mm * 5 + nc] * fact);
+          }
+          wsync();
+          const int nl = nparts * 5 * (4 - nc), ntask = nl + nparts * nu * 25;
+          for (int k = lane; k < ntask; k += 64) {
+            if (k < nl) {
+              const int pp = k / (5 * (4 - nc)), rem = k % (5 * (4 - nc));
+              const int mm = rem / (4 - nc), q = nc + 1 + rem % (4 - nc);
+              cx<T> *Lb = B(pick(pr, pp), slot(pick(pr, pp), v));
+              Lb[mm * 5 + q] = cfma(Lb[mm * 5 + nc], D[nc * 5 + q], Lb[mm * 5 + q]);
+            } else {
+              const int kk = k - nl, pp = kk / (nu * 25), rem = kk % (nu * 25);
+              const int u = rem / 25, mm = (rem % 25) / 5, q = rem % 5;
+              const int sdst = slot(pick(pr, pp), pick(uc, u));
+              if (sdst < 0) continue;      // no such block (does not occur in the band)
+              cx<T> *Lb = B(pick(pr, pp), slot(pick(pr, pp), v));
+              cx<T> *Dst = B(pick(pr, pp), sdst);
+              const cx<T> *U = B(v, pick(us, u));
+              Dst[mm * 5 + q] = cfma(Lb[mm * 5 + nc], U[nc * 5 + q], Dst[mm * 5 + q]);
+            }
+          }
+          wsync();
+        }
       }
       // row v is final: write it back and bring in row v+5
       store_row(v, true);
