@@ -132,6 +132,7 @@ constexpr int LUP_SLOTS = 49;    // 0 D, 1..6 right blocks, 7 + 7a + b partner a
 __global__ __launch_bounds__(64) void k_lu_per(LineGeom g, cx<double> *mat, cx<double> *per, cx<double> *per2) {
   using T = double;
   __shared__ cx<T> blkv[LUP_SLOTS][25];
+  __shared__ cx<T> *bptr[LUP_SLOTS];      // the slots' blocks in HBM (null: no such block)
   __shared__ cx<T> mlt[6][5];
   const long line = blockIdx.x;
   const int lane = threadIdx.x;
@@ -182,14 +183,20 @@ __global__ __launch_bounds__(64) void k_lu_per(LineGeom g, cx<double> *mat, cx<d
         B.p = mat + ((long)(t * 5 + 3) * 25) * L + line;
       return true;
     };
-    // bring the blocks into LDS (all loads in flight at once)
+    // the slots' addresses, one lane per slot, then the blocks into LDS (all
+    // loads in flight at once)
+    if (lane < LUP_SLOTS) {
+      Blk<T> B;
+      bptr[lane] = where(lane, B) ? B.p : nullptr;
+    }
+    wave_sync();
     constexpr int PER_LANE = (LUP_SLOTS * 25 + 63) / 64;
     cx<T> tmp[PER_LANE];
 #pragma unroll
     for (int r = 0; r < PER_LANE; r++) {
       const int e = r * 64 + lane, k = e / 25, el = e % 25;
-      Blk<T> B;
-      tmp[r] = (e < LUP_SLOTS * 25 && where(k, B)) ? B(el / 5, el % 5) : cx<T>{0, 0};
+      const cx<T> *bp = e < LUP_SLOTS * 25 ? bptr[k] : nullptr;
+      tmp[r] = bp ? bp[(long)el * L] : cx<T>{0, 0};
     }
 #pragma unroll
     for (int r = 0; r < PER_LANE; r++) {
@@ -201,9 +208,8 @@ __global__ __launch_bounds__(64) void k_lu_per(LineGeom g, cx<double> *mat, cx<d
     unsigned uok = 0, pok = 0;
 #pragma unroll
     for (int u = 0; u < 6; u++) {
-      Blk<T> B;
-      if (where(1 + u, B)) uok |= 1u << u;
-      if (where(7 + 7 * u, B)) pok |= 1u << u;
+      if (bptr[1 + u]) uok |= 1u << u;
+      if (bptr[7 + 7 * u]) pok |= 1u << u;
     }
     cx<T> *D = blkv[0];
     // in-block LU of D, its row operations applied to the right blocks
@@ -261,8 +267,8 @@ __global__ __launch_bounds__(64) void k_lu_per(LineGeom g, cx<double> *mat, cx<d
 #pragma unroll
     for (int r = 0; r < PER_LANE; r++) {
       const int e = r * 64 + lane, k = e / 25, el = e % 25;
-      Blk<T> B;
-      if (e < LUP_SLOTS * 25 && where(k, B)) B(el / 5, el % 5) = blkv[k][el];
+      cx<T> *bp = e < LUP_SLOTS * 25 ? bptr[k] : nullptr;
+      if (bp) bp[(long)el * L] = blkv[k][el];
     }
     __builtin_amdgcn_s_waitcnt(0);
     wave_sync();
