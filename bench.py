@@ -164,7 +164,7 @@ def main():
         "config": {"workload": "LNS3D implicit AF step: lrhs3D + xi/eta block-penta line solves + "
                                "update, complex double, factors formed once",
                    "nx": NX, "ny": NY, "beta": BETA, "Ma": p.Ma, "Re": p.Re, "cfl": p.cflmax},
-        "roofline": {"bound": "hbm", "kernel": "k_solve_tw<double,6,1,false> (xi lines: 257 lines x 1025 rows, twisted)",
+        "roofline": {"bound": "hbm", "kernel": "k_solve_tw<double,6,1,false,4,true> (xi lines: 257 lines x 1025 rows, twisted, column halves)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "bytes_per_launch": bytes_launch,

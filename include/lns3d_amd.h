@@ -175,11 +175,17 @@ int lns3d_graph_destroy(lns3d_graph *g);
  * synthetic lines of n rows x lines and uses the fastest for plans of that
  * shape created later.  Variants: 0 one line per workgroup, 1 four lines,
  * 2 twisted (two chains per line, n >= 16), 3 two lines, 4 three lines,
- * 5 twisted with chain steps taken in pairs.  times_ms (may be NULL)
+ * 5 twisted with chain steps taken in pairs, 6 twisted with each row's
+ * columns split over the two halves of a wave.  times_ms (may be NULL)
  * receives LNS3D_TUNE_VARIANTS timings in ms (-1: not applicable).  No
  * counterpart in the reference. */
-#define LNS3D_TUNE_VARIANTS 6
+#define LNS3D_TUNE_VARIANTS 7
 int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *times_ms);
+/* record a solve variant for the shape (n rows x lines, precision) without
+ * timing it, for plans created later (a twisted variant on lines shorter
+ * than 16 rows falls back to variant 0); variant -1 drops the record, so
+ * that the built-in choice applies again.  No counterpart in the reference. */
+int lns3d_tune_set(int n, long lines, int precision, int variant);
 
 /* Host-array driving of the time loop of src/impdrv3d.f90:126-406 (for
  * callers that hold v in host memory, such as the Fortran host layer):

@@ -504,6 +504,12 @@ extern "C" int lns3d_tune_lines(int n, long lines, int precision, int *variant, 
   return penta_tune(n, lines, precision, variant, times_ms);
 }
 
+extern "C" int lns3d_tune_set(int n, long lines, int precision, int variant) {
+  if (n < 7 || lines < 1 || (precision != 0 && precision != 1) || variant < -1 || variant >= LNS3D_TUNE_VARIANTS)
+    return fail(LNS3D_EINVAL, "lns3d_tune_set: bad shape or variant");
+  return penta_tune_set(n, lines, precision, variant);
+}
+
 // Host-array driving of the time loop (src/impdrv3d.f90:126-406): the
 // context keeps the state v in HBM between calls.  The boundary conditions
 // are applied once to a freshly uploaded state before its first step, as

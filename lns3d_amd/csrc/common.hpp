@@ -87,6 +87,46 @@ template <int OFF, typename T> __device__ __forceinline__ void row_bcast5_at(cx<
   y[0] = row_bcast<OFF>(v); y[1] = row_bcast<OFF + 1>(v); y[2] = row_bcast<OFF + 2>(v);
   y[3] = row_bcast<OFF + 3>(v); y[4] = row_bcast<OFF + 4>(v);
 }
+// lane Q's value of each DPP row written to the rows in ROWS (bit r: row r);
+// the other rows keep old
+template <int Q, int ROWS> __device__ __forceinline__ cx<double> row_bcast_rows(cx<double> old, cx<double> v) {
+  long a = __builtin_bit_cast(long, v.re), b = __builtin_bit_cast(long, v.im);
+  a = __builtin_amdgcn_update_dpp(__builtin_bit_cast(long, old.re), a, 0x150 + Q, ROWS, 0xF, false);
+  b = __builtin_amdgcn_update_dpp(__builtin_bit_cast(long, old.im), b, 0x150 + Q, ROWS, 0xF, false);
+  return {__builtin_bit_cast(double, a), __builtin_bit_cast(double, b)};
+}
+template <int Q, int ROWS> __device__ __forceinline__ cx<float> row_bcast_rows(cx<float> old, cx<float> v) {
+  long a = __builtin_bit_cast(long, v);
+  a = __builtin_amdgcn_update_dpp(__builtin_bit_cast(long, old), a, 0x150 + Q, ROWS, 0xF, false);
+  return __builtin_bit_cast(cx<float>, a);
+}
+// the sum of a value over the two 32-lane halves of the wave (lane l and lane
+// l ^ 32), the same bits in both halves: v_permlane32_swap of the value with a
+// copy of itself leaves the lower half's value in one register and the upper
+// half's in the other, in every lane.  The copy and the two results pass
+// through empty asm statements: without them the compiler has taken the
+// swap's two results for one value (the float form added a result to itself).
+__device__ __forceinline__ unsigned opaque_copy(unsigned u) {
+  asm volatile("" : "+v"(u));
+  return u;
+}
+__device__ __forceinline__ double half_sum(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, opaque_copy(lo), false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, opaque_copy(hi), false, false);
+  const double a = __builtin_bit_cast(double, ((unsigned long long)opaque_copy(h[0]) << 32) | opaque_copy(l[0]));
+  const double b = __builtin_bit_cast(double, ((unsigned long long)opaque_copy(h[1]) << 32) | opaque_copy(l[1]));
+  return a + b;
+}
+__device__ __forceinline__ float half_sum(float v) {
+  const unsigned u = __builtin_bit_cast(unsigned, v);
+  const auto r = __builtin_amdgcn_permlane32_swap(u, opaque_copy(u), false, false);
+  return __builtin_bit_cast(float, opaque_copy(r[0])) + __builtin_bit_cast(float, opaque_copy(r[1]));
+}
+template <typename T> __device__ __forceinline__ cx<T> half_sum(cx<T> v) {
+  return {half_sum(v.re), half_sum(v.im)};
+}
 // The chip places workgroup b of a launch on XCD b % 8 (round robin over its
 // eight XCDs, each with its own L2).  xcd_tile(b, nb) numbers the tiles so
 // that each XCD works through a contiguous range of them (a bijection of

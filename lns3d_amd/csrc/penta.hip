@@ -1114,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
   // five rows of a pass (which carry the boundary-row blocks) take a plain
   // one-row-at-a-time path.
   auto run_chunk = [&](const cx<T> *B, int stride, int kk, int nr, int ibase, bool fwd_pass) {
-    auto rix = [&](int j) { j = min(j, nr - 1); return fwd_pass ? j : nr - 1 - j; };
+    auto rix = [&](int j) { return fwd_pass ? j : nr - 1 - j; };     // j < nr at every call
     auto gi = [&](int j) { return fwd_pass ? ibase + j : ibase - j; };
     auto ldW2 = [&](int j, cx<T> (&w)[5]) {
       const cx<T> *p = B + rix(j) * stride + m * 10;
@@ -1140,7 +1140,9 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
       ldW1(1, w1s[1], cs[1]);
       ldW1(2, w1s[2], cs[2]);
       cx<T> a0 = pre0(w1s[0], cs[0], yA);
-      auto sub = [&](int j, auto U) {
+      // rows two and three ahead are read only while they lie in this
+      // chunk's groups (the next chunk starts by reading its own first rows)
+      auto sub = [&](int j, bool more, auto U) {
         constexpr int u = decltype(U)::value;
         constexpr int u0 = u % 3, u1 = (u + 1) % 3, u2 = (u + 2) % 3;
         cx<T>(&yn)[5] = (u % 2 == 0) ? yB : yA;      // newest value, y_{i-1}
@@ -1149,17 +1151,18 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
         a0 = pre0(w1s[u1], cs[u1], yn);
         *xp = make_double2(y.re, y.im);    // lanes of one component store the same value
         xp += xstep;
-        ldW2(j + 2, w2s[u2]);
-        ldW1(j + 3, w1s[u0], cs[u0]);
+        if (u + 2 < 6 || more) ldW2(j + 2, w2s[u2]);
+        if (u + 3 < 6 || more) ldW1(j + 3, w1s[u0], cs[u0]);
         row_bcast5(y, yd);
       };
       for (int j = 0; j < nmain; j += 6) {
-        sub(j, std::integral_constant<int, 0>());
-        sub(j + 1, std::integral_constant<int, 1>());
-        sub(j + 2, std::integral_constant<int, 2>());
-        sub(j + 3, std::integral_constant<int, 3>());
-        sub(j + 4, std::integral_constant<int, 4>());
-        sub(j + 5, std::integral_constant<int, 5>());
+        const bool more = j + 6 < nmain;
+        sub(j, more, std::integral_constant<int, 0>());
+        sub(j + 1, more, std::integral_constant<int, 1>());
+        sub(j + 2, more, std::integral_constant<int, 2>());
+        sub(j + 3, more, std::integral_constant<int, 3>());
+        sub(j + 4, more, std::integral_constant<int, 4>());
+        sub(j + 5, more, std::integral_constant<int, 5>());
       }
     }
     for (int j = nmain; j < nr; j++) {
@@ -1204,8 +1207,10 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
 // two values (rows h+2, h+3) to chain 0, whose last two rows carry those
 // columns.  Backward: chain 0 starts at row h+1; chain 1 starts two steps
 // later from chain 0's x_{h+1}, x_h.
-template <typename T, int R, int G, bool PAIRS, int NBUF>
-__global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
+// HALF (one line per workgroup, single steps): the DPP rows that would
+// repeat the two chains instead take half of each row's columns (below).
+template <typename T, int R, int G, bool PAIRS, int NBUF, bool HALF = false>
+__global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
                                                  const cx<T> *__restrict__ bwd,
                                                  const cx<T> *__restrict__ tail,
                                                  const cx<T> *__restrict__ xf,
@@ -1214,18 +1219,22 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
   constexpr int C = 2 * G;
   constexpr int BUFE = R * 66;
   static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
+  static_assert(!HALF || (G == 1 && !PAIRS), "column halves: one line per workgroup, single steps");
   __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][C][BUFE];
   __shared__ double2 bst[NBUF][C][R][5];   // rhs rows, then c (or d) in place
   constexpr int XE = PAIRS ? (R / 2) * 25 + 1 : 1;   // side blocks of a chunk's pairs (+1: copy padding)
   __shared__ __attribute__((aligned(16))) cx<T> xtr[NBUF][C][XE];
   __shared__ cx<T> xch[G][2][5];     // values handed between the chains of a line
   __shared__ cx<T> kept[C][2][5];    // x at chain rows 3, 4 (backward boundary blocks)
+  // zero rows standing in for the staged c rows in the column half without c
+  __shared__ double2 zrows[HALF ? R : 1][5];
   const long line0 = (long)blockIdx.x * G;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int ch = (lane >> 4) % C;    // chain of this DPP row (rows past 2G repeat)
   const int g = ch >> 1, t = ch & 1;
   const int m = min(lane & 15, 4);
+  const int hq = lane >> 5;          // column half (HALF): DPP rows 0, 1 and rows 2, 3
   auto line_of = [&](int gg) { return min(line0 + gg, L - 1); };
   const long line = line_of(g);
   const int len0 = h + 2, len1 = n - h - 2;          // len0 >= len1 + 2
@@ -1319,6 +1328,21 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
   cx<T> yA[5], yB[5];
   cx<T> w2s[3][5], w1s[3][5], cs[3];
   cx<T> s0[5], s1[5];
+  // HALF: the two newest values (lane m: component m) and their split
+  // broadcasts, current while `split` holds (else yA, yB are)
+  cx<T> vA{0, 0}, vB{0, 0}, SA[3], SB[3];
+  bool split = false;
+  if (HALF) {
+    for (int e = lane; e < R * 5; e += 64) zrows[e / 5][e % 5] = make_double2(0.0, 0.0);
+    wave_sync();
+  }
+  // split broadcast of v into S, in place: slot k = component 3 hq + k of
+  // the chain; slot 2 of the upper half is never written and holds zero
+  auto split_bcast = [&](cx<T> v, cx<T> (&S)[3]) {
+    S[0] = row_bcast_rows<3, 0xC>(row_bcast_rows<0, 0x3>(S[0], v), v);
+    S[1] = row_bcast_rows<4, 0xC>(row_bcast_rows<1, 0x3>(S[1], v), v);
+    S[2] = row_bcast_rows<2, 0x3>(S[2], v);
+  };
   // the term in the newest value: two FMA accumulators (four independent
   // chains over re and im), 22 instructions
   auto crit = [&](const cx<T> (&w)[5], const cx<T> (&yn)[5], cx<T> a0) -> cx<T> {
@@ -1350,6 +1374,8 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
     const int stride = fw ? 60 : 66;
 #pragma unroll
     for (int q = 0; q < 5; q++) { yA[q] = cx<T>{0, 0}; yB[q] = cx<T>{0, 0}; }
+    vA = vB = cx<T>{0, 0};
+    split = false;
     compute_pass(nchunk, [&](int k) {
       {
         const int kk = k % NBUF;
@@ -1362,6 +1388,9 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
           int b = fw ? s - s_begin : bj(t, s) - jlo;
           return max(0, min(max(nr, 1) - 1, b));
         };
+        // the same inside the main range, where every step's row is in the
+        // buffer for both chains
+        auto brow_m = [&](int s) { return fw ? s - s_begin : bj(t, s) - jlo; };
         auto ldW2 = [&](int s, cx<T> (&w)[5]) {
           const cx<T> *pp = B + brow(s) * stride + m * 10;
 #pragma unroll
@@ -1379,9 +1408,15 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
         auto plain = [&](int s) {
           const int j = jrow(s);
           const bool act = j >= 0 && j < len;
+          if (HALF && split) {
+            row_bcast5(vA, yA);
+            row_bcast5(vB, yB);
+            split = false;
+          }
           if (!fw && t == 1 && s == 2) {            // chain 1 starts from x_h, x_{h+1}
 #pragma unroll
             for (int q = 0; q < 5; q++) { yA[q] = xch[g][0][q]; yB[q] = xch[g][1][q]; }
+            if (HALF) { vA = xch[g][0][m]; vB = xch[g][1][m]; }
           }
           if (fw && t == 0 && j == len0 - 2) {      // columns h+2, h+3 from chain 1
 #pragma unroll
@@ -1417,6 +1452,7 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
           row_bcast5(y, yA);
 #pragma unroll
           for (int q = 0; q < 5; q++) { cx<T> tt = yA[q]; yA[q] = yB[q]; yB[q] = tt; }
+          if (HALF) { vA = vB; vB = y; }
           if (act) {
             if (fw && t == 1 && j == len1 - 1) xch[g][0][m] = y;     // z_{h+2}
             if (fw && t == 1 && j == len1 - 2) xch[g][1][m] = y;     // z_{h+3}
@@ -1460,18 +1496,105 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
             psub(st + 4, std::integral_constant<int, 2>());
           }
         }
-        if (!PAIRS && m_end > p_end) {
-          // one step at a time in groups of six (no pair rewrite)
+        if (HALF && m_end > p_end) {
+          // Columns split over the wave's halves: half hq of a chain sums
+          // columns 3 hq + k (k = 0, 1, 2) of each block row, slot 2 of the
+          // upper half meeting a zero broadcast; the upper half adds c.  The
+          // halves' partial rows are added across the wave (half_sum), so both
+          // halves hold y and broadcast it.  Steps two and three ahead are
+          // read only while they lie in the main range.
+          if (!split) {
+            SA[2] = SB[2] = cx<T>{0, 0};
+            split_bcast(vA, SA);
+            split_bcast(vB, SB);
+            split = true;
+          }
+          const int qb = 3 * hq;
+          const double2 *cb = hq ? &bst[kk][ch][0][m] : &zrows[0][m];   // row r: cb + 5 r
+          auto ldW2h = [&](int s, cx<T> (&w)[3]) {
+            const cx<T> *pp = B + brow_m(s) * stride + m * 10 + qb;
+#pragma unroll
+            for (int k = 0; k < 3; k++) w[k] = pp[k];
+          };
+          auto ldW1h = [&](int s, cx<T> (&w)[3], cx<T> &c) {
+            const int r = brow_m(s);
+            const cx<T> *pp = B + r * stride + m * 10 + 5 + qb;
+#pragma unroll
+            for (int k = 0; k < 3; k++) w[k] = pp[k];
+            c = *reinterpret_cast<const cx<T> *>(cb + 5 * r);
+          };
+          // a + b: the half's part of c + W1 y_{i-2}, formed a step ahead
+          auto pre0h = [&](const cx<T> (&w)[3], cx<T> c, const cx<T> (&S)[3], cx<T> &a, cx<T> &b) {
+            a = cfma(w[0], S[0], c);
+            b = cmul(w[1], S[1]);
+            a = cfma(w[2], S[2], a);
+          };
+          auto crith = [&](const cx<T> (&w)[3], const cx<T> (&S)[3], cx<T> a, cx<T> b) -> cx<T> {
+            a = cfma(w[0], S[0], a);
+            b = cfma(w[1], S[1], b);
+            a = cfma(w[2], S[2], a);
+            return half_sum(a + b);
+          };
+          cx<T> w2h[3][3], w1h[3][3], chh[3];
+          double2 *xp = X + (long)real_row(t, jrow(p_end)) * rs + m;
+          const long xstep = ((t == 0) == fw) ? rs : -rs;
+          ldW2h(p_end, w2h[0]);
+          ldW2h(p_end + 1, w2h[1]);
+          ldW1h(p_end, w1h[0], chh[0]);
+          ldW1h(p_end + 1, w1h[1], chh[1]);
+          ldW1h(p_end + 2, w1h[2], chh[2]);
+          cx<T> pa, pb;
+          pre0h(w1h[0], chh[0], SA, pa, pb);
+          auto subh = [&](int s, bool more, auto U) {
+            constexpr int u = decltype(U)::value;
+            constexpr int u0 = u % 3, u1 = (u + 1) % 3, u2 = (u + 2) % 3;
+            cx<T>(&Sn)[3] = (u % 2 == 0) ? SB : SA;      // y_{i-1}
+            cx<T>(&Sd)[3] = (u % 2 == 0) ? SA : SB;      // receives y_i
+            const cx<T> y = crith(w2h[u0], Sn, pa, pb);
+            pre0h(w1h[u1], chh[u1], Sn, pa, pb);
+            *xp = make_double2(y.re, y.im);
+            xp += xstep;
+            if (u + 2 < 6 || more) ldW2h(s + 2, w2h[u2]);
+            if (u + 3 < 6 || more) ldW1h(s + 3, w1h[u0], chh[u0]);
+            split_bcast(y, Sd);
+            if constexpr (u == 4) vA = y;
+            if constexpr (u == 5) vB = y;
+          };
+          for (int s = p_end; s < m_end; s += 6) {
+            const bool more = s + 6 < m_end;
+            subh(s, more, std::integral_constant<int, 0>());
+            subh(s + 1, more, std::integral_constant<int, 1>());
+            subh(s + 2, more, std::integral_constant<int, 2>());
+            subh(s + 3, more, std::integral_constant<int, 3>());
+            subh(s + 4, more, std::integral_constant<int, 4>());
+            subh(s + 5, more, std::integral_constant<int, 5>());
+          }
+        }
+        if (!PAIRS && !HALF && m_end > p_end) {
+          // one step at a time in groups of six (no pair rewrite); steps two
+          // and three ahead are read only while they lie in the main range
+          auto ldW2m = [&](int s, cx<T> (&w)[5]) {
+            const cx<T> *pp = B + brow_m(s) * stride + m * 10;
+#pragma unroll
+            for (int q = 0; q < 5; q++) w[q] = pp[q];
+          };
+          auto ldW1m = [&](int s, cx<T> (&w)[5], cx<T> &c) {
+            const int r = brow_m(s);
+            const cx<T> *pp = B + r * stride + m * 10 + 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) w[q] = pp[q];
+            c = *reinterpret_cast<const cx<T> *>(&bst[kk][ch][r][m]);
+          };
           int j0 = jrow(p_end);
           double2 *xp = X + (long)real_row(t, j0) * rs + m;
           const long xstep = ((t == 0) == fw) ? rs : -rs;
-          ldW2(p_end, w2s[0]);
-          ldW2(p_end + 1, w2s[1]);
-          ldW1(p_end, w1s[0], cs[0]);
-          ldW1(p_end + 1, w1s[1], cs[1]);
-          ldW1(p_end + 2, w1s[2], cs[2]);
+          ldW2m(p_end, w2s[0]);
+          ldW2m(p_end + 1, w2s[1]);
+          ldW1m(p_end, w1s[0], cs[0]);
+          ldW1m(p_end + 1, w1s[1], cs[1]);
+          ldW1m(p_end + 2, w1s[2], cs[2]);
           cx<T> a0 = pre0(w1s[0], cs[0], yA);
-          auto sub = [&](int s, auto U) {
+          auto sub = [&](int s, bool more, auto U) {
             constexpr int u = decltype(U)::value;
             constexpr int u0 = u % 3, u1 = (u + 1) % 3, u2 = (u + 2) % 3;
             cx<T>(&yn)[5] = (u % 2 == 0) ? yB : yA;
@@ -1480,17 +1603,18 @@ __global__ __launch_bounds__(256) void k_solve_tw(int n, int h, long L, const cx
             a0 = pre0(w1s[u1], cs[u1], yn);
             *xp = make_double2(y.re, y.im);
             xp += xstep;
-            ldW2(s + 2, w2s[u2]);
-            ldW1(s + 3, w1s[u0], cs[u0]);
+            if (u + 2 < 6 || more) ldW2m(s + 2, w2s[u2]);
+            if (u + 3 < 6 || more) ldW1m(s + 3, w1s[u0], cs[u0]);
             row_bcast5(y, yd);
           };
           for (int s = p_end; s < m_end; s += 6) {
-            sub(s, std::integral_constant<int, 0>());
-            sub(s + 1, std::integral_constant<int, 1>());
-            sub(s + 2, std::integral_constant<int, 2>());
-            sub(s + 3, std::integral_constant<int, 3>());
-            sub(s + 4, std::integral_constant<int, 4>());
-            sub(s + 5, std::integral_constant<int, 5>());
+            const bool more = s + 6 < m_end;
+            sub(s, more, std::integral_constant<int, 0>());
+            sub(s + 1, more, std::integral_constant<int, 1>());
+            sub(s + 2, more, std::integral_constant<int, 2>());
+            sub(s + 3, more, std::integral_constant<int, 3>());
+            sub(s + 4, more, std::integral_constant<int, 4>());
+            sub(s + 5, more, std::integral_constant<int, 5>());
           }
         }
         for (int s = m_end; s < s_end; s++) plain(s);
@@ -1876,8 +2000,17 @@ int penta_variant_for(int n, long lines, int prec) {
     auto it = g_tuned.find(std::make_tuple(n, lines, prec));
     if (it != g_tuned.end()) return it->second;
   }
-  if (n >= 16 && lines < 512) return prec ? PENTA_TWISTED_PAIRS : PENTA_TWISTED;
+  if (n >= 16 && lines < 512) return prec ? PENTA_TWISTED_PAIRS : PENTA_TWISTED_HALF;
   return lines >= 512 ? PENTA_PLAIN3 : PENTA_PLAIN1;
+}
+
+int penta_tune_set(int n, long lines, int prec, int variant) {
+  if (variant < -1 || variant >= PENTA_NVARIANT)
+    return set_error(__FILE__, __LINE__, "line-solve variant out of range");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (variant < 0) g_tuned.erase(std::make_tuple(n, lines, prec));
+  else g_tuned[std::make_tuple(n, lines, prec)] = variant;
+  return 0;
 }
 
 int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec) {
@@ -1895,10 +2028,9 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
   pl->L = lines;
   pl->prec = prec;
   if (pl->n < 7 || lines < 1) return set_error(__FILE__, __LINE__, "line length must be >= 7 unknowns");
-  const bool twisted = variant == PENTA_TWISTED || variant == PENTA_TWISTED_PAIRS;
-  if (twisted && pl->n < 16) variant = PENTA_PLAIN1;
+  if (penta_twisted(variant) && pl->n < 16) variant = PENTA_PLAIN1;
   pl->variant = periodic ? PENTA_PLAIN1 : variant;
-  pl->h = (!periodic && (pl->variant == PENTA_TWISTED || pl->variant == PENTA_TWISTED_PAIRS)) ? (pl->n - 1) / 2 : -1;
+  pl->h = (!periodic && penta_twisted(pl->variant)) ? (pl->n - 1) / 2 : -1;
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
@@ -2014,10 +2146,16 @@ static void launch_solve_bc(const PentaPlan *pl, void *rhs, long sL, long sR, hi
   auto wgs = [&](int G) { return dim3((unsigned)((pl->L + G - 1) / G)); };
   switch (pl->variant) {
     // twisted lines, a ring of four buffers (five measured slower: the wave
-    // preparing a chunk then also starts the next copies)
+    // preparing a chunk then also starts the next copies).  The kernel is
+    // bounded to 256 VGPRs per wave: above that only one workgroup fits a CU,
+    // and the 257th line of a 257-line solve would run alone after the rest.
     case PENTA_TWISTED:        // one step at a time
       hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1, false, 4>), wgs(1), dim3(256), 0, s, pl->n, pl->h, pl->L, f,
                          b, t, (const cx<T> *)pl->xf, (const cx<T> *)pl->xb, x, sL, sR);
+      break;
+    case PENTA_TWISTED_HALF:   // one step at a time, columns split over the wave's halves
+      hipLaunchKernelGGL((k_solve_tw<T, D ? 6 : 12, 1, false, 4, true>), wgs(1), dim3(256), 0, s, pl->n, pl->h,
+                         pl->L, f, b, t, (const cx<T> *)pl->xf, (const cx<T> *)pl->xb, x, sL, sR);
       break;
     case PENTA_TWISTED_PAIRS:  // pairs of steps (factor-time rewrite)
       static_assert(tw_chunk(!D) == (D ? 6 : 12), "the pair rewrite assumes the solve's chunk size");
@@ -2096,7 +2234,7 @@ int penta_tune(int n, long lines, int prec, int *variant, float *times_ms) {
   int rc = 0;
   for (int v = 0; v < PENTA_NVARIANT && !rc; v++) {
     if (times_ms) times_ms[v] = -1.0f;
-    if ((v == PENTA_TWISTED || v == PENTA_TWISTED_PAIRS) && n < 16) continue;
+    if (penta_twisted(v) && n < 16) continue;
     PentaPlan pl;
     rc = penta_plan_create_variant(&pl, 0, 0, n, lines, prec, v);
     if (!rc) {

@@ -26,15 +26,23 @@ struct PentaPlan {
 };
 
 // solve-kernel variants of non-periodic lines: one line per workgroup with a
-// deep LDS ring, four lines per workgroup, or twisted lines (two chains)
+// deep LDS ring, two, three or four lines per workgroup, or twisted lines
+// (two chains): single steps, steps in pairs, or single steps with each
+// row's columns split over the two halves of the wave
 enum { PENTA_PLAIN1 = 0, PENTA_PLAIN4 = 1, PENTA_TWISTED = 2, PENTA_PLAIN2 = 3, PENTA_PLAIN3 = 4,
-       PENTA_TWISTED_PAIRS = 5, PENTA_NVARIANT = 6 };
+       PENTA_TWISTED_PAIRS = 5, PENTA_TWISTED_HALF = 6, PENTA_NVARIANT = 7 };
+inline bool penta_twisted(int v) {
+  return v == PENTA_TWISTED || v == PENTA_TWISTED_PAIRS || v == PENTA_TWISTED_HALF;
+}
 // the variant a plan of this shape uses: the autotuned one when
 // penta_tune has measured the shape, else a fixed rule
 int penta_variant_for(int n, long lines, int prec);
 // time every variant on synthetic lines of this shape (n rows, lines,
 // precision) and record the fastest; returns the chosen variant in *variant
 int penta_tune(int n, long lines, int prec, int *variant, float *times_ms);
+// record a variant for the shape without timing (later plans use it);
+// variant -1 drops the record (back to the fixed rule)
+int penta_tune_set(int n, long lines, int prec, int variant);
 int penta_plan_create(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec);
 int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows, long lines, int prec,
                               int variant);

@@ -43,6 +43,7 @@ SIGNATURES = {
     "lns3d_graph_synchronize": ([VP], CI),
     "lns3d_graph_destroy": ([VP], CI),
     "lns3d_tune_lines": ([CI, ctypes.c_long, CI, ctypes.POINTER(CI), ctypes.POINTER(ctypes.c_float)], CI),
+    "lns3d_tune_set": ([CI, ctypes.c_long, CI, CI], CI),
     "lns3d_state_upload": ([VP, VP], CI),
     "lns3d_state_download": ([VP, VP], CI),
     "lns3d_advance": ([VP, CI, VP], CI),
@@ -262,8 +263,9 @@ def ctx_bytes(p, precision=0) -> int:
 
 
 # line-solve variants the autotuner times (LNS3D_TUNE_VARIANTS)
-TUNE_VARIANTS = 6
-VARIANT_NAMES = ("1 line/WG", "4 lines/WG", "twisted", "2 lines/WG", "3 lines/WG", "twisted pairs")
+TUNE_VARIANTS = 7
+VARIANT_NAMES = ("1 line/WG", "4 lines/WG", "twisted", "2 lines/WG", "3 lines/WG", "twisted pairs",
+                 "twisted half")
 
 
 def tune_lines(n: int, lines: int, precision: int = 0):
@@ -274,6 +276,12 @@ def tune_lines(n: int, lines: int, precision: int = 0):
     t = (ctypes.c_float * TUNE_VARIANTS)()
     check(lib().lns3d_tune_lines(n, lines, precision, ctypes.byref(v), t), "lns3d_tune_lines")
     return v.value, [float(x) for x in t]
+
+
+def set_line_variant(n: int, lines: int, precision: int, variant: int) -> None:
+    """use this solve variant for plans of the shape created afterwards
+    (-1: the built-in choice again)"""
+    check(lib().lns3d_tune_set(n, lines, precision, variant), "lns3d_tune_set")
 
 
 MODE_FP64, MODE_FP32, MODE_REFINED = 0, 1, 2

@@ -118,3 +118,33 @@ def test_bc_complex_float_factors(torch_cuda, orc, layout, nsys, n):
     h.solve(dr)
     assert relerr(_host(dr, rhs.shape), x_ref) < 1e-5
     h.close()
+
+
+# every solve variant pinned in turn (lns3d_tune_set) on shapes no other test
+# uses: the shortest twisted lines, odd and even lengths, chunk boundaries in
+# both chunk sizes, and many lines
+VARIANT_SIZES = [(2, 16), (3, 37), (1, 103), (5, 64), (33, 97)]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("nsys,n", VARIANT_SIZES)
+def test_every_variant(torch_cuda, orc, nsys, n, prec):
+    from lns3d_amd.native import Penta, set_line_variant, TUNE_VARIANTS, VARIANT_NAMES
+    from tests.penta_cases import random_lines, relerr
+    layout = 1
+    mat, rhs = random_lines(nsys, n, layout, seed=17 * nsys + n)
+    m_ref, x_ref = mat.copy(order="F"), rhs.copy(order="F")
+    orc.cpenta_bc(layout, m_ref, x_ref, 2)
+    tol = TOL if prec == 0 else 1e-5
+    try:
+        for v in range(TUNE_VARIANTS):
+            print(f"variant {v} ({VARIANT_NAMES[v]}) nsys={nsys} n={n} prec={prec}", flush=True)
+            set_line_variant(n, nsys, prec, v)
+            h = Penta(0, layout, nsys, n, prec)
+            dm, dr = _dev(torch_cuda, mat), _dev(torch_cuda, rhs)
+            h.factor(dm)
+            h.solve(dr)
+            assert relerr(_host(dr, rhs.shape), x_ref) < tol, VARIANT_NAMES[v]
+            h.close()
+    finally:
+        set_line_variant(n, nsys, prec, -1)
