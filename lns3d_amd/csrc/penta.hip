@@ -1328,8 +1328,8 @@ __global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const
   cx<T> yA[5], yB[5];
   cx<T> w2s[3][5], w1s[3][5], cs[3];
   cx<T> s0[5], s1[5];
-  // HALF: the two newest values (lane m: component m) and their split
-  // broadcasts, current while `split` holds (else yA, yB are)
+  // HALF: the two newest values (lane m: component m), always current, and
+  // their split broadcasts, current while `split` holds
   cx<T> vA{0, 0}, vB{0, 0}, SA[3], SB[3];
   bool split = false;
   if (HALF) {
@@ -1408,7 +1408,7 @@ __global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const
         auto plain = [&](int s) {
           const int j = jrow(s);
           const bool act = j >= 0 && j < len;
-          if (HALF && split) {
+          if (HALF) {          // vA, vB are the state; yA, yB live only within a plain step
             row_bcast5(vA, yA);
             row_bcast5(vB, yB);
             split = false;
