@@ -187,6 +187,38 @@ int lns3d_tune_lines(int n, long lines, int precision, int *variant, float *time
  * that the built-in choice applies again.  No counterpart in the reference. */
 int lns3d_tune_set(int n, long lines, int precision, int variant);
 
+/* ======================================================================
+ * Reference file formats (Fortran sequential unformatted: 4-byte record
+ * length markers, 4-byte integers, 8-byte reals, 16-byte complex), host
+ * arrays in the reference layouts.  Readers check every record length.
+ * ====================================================================== */
+/* first record of restart and mean files (src/impdrv3d.f90:396-397) */
+typedef struct {
+  int lstep;
+  double time;
+  int nx, ny, nz, ndof;
+  double Re, Ma, Pr, gamma, cv;
+} lns3d_field_header;
+int lns3d_read_header(const char *path, lns3d_field_header *h);
+/* restart file: header, v complex(ndof,nx,ny), and vold when impl = 2
+ * (written by src/impdrv3d.f90:393-399, read by src/genini3d.f90:26-57).
+ * v must hold ndof*nx*ny*nz complex values (read the header first); vold may
+ * be NULL; *got_vold (may be NULL) tells whether the file held vold. */
+int lns3d_read_restart(const char *path, lns3d_field_header *h, void *v, void *vold, int *got_vold);
+int lns3d_write_restart(const char *path, const lns3d_field_header *h, const void *v, const void *vold);
+/* mean-field file: header and vm real(ndof,nx,ny) (src/genmean.f90:28-33) */
+int lns3d_read_mean(const char *path, lns3d_field_header *h, double *vm);
+int lns3d_write_mean(const char *path, const lns3d_field_header *h, const double *vm);
+/* grid file: (nx, ny, nz) then x, y, z of (nx,ny,nz) reals in one record
+ * (src/gengrid.f90:17-32).  With x and y NULL only the sizes are read; the
+ * writer stores z = 0. */
+int lns3d_read_grid(const char *path, int *nx, int *ny, int *nz, double *x, double *y);
+int lns3d_write_grid(const char *path, int nx, int ny, int nz, const double *x, const double *y);
+/* metric file: m1, m2, n1, n2, m11, m12, m22, n11, n12, n22 of (nx,ny) reals
+ * in one record (src/gengrid.f90:76-80), the metric layout of lns3d_setup */
+int lns3d_read_metric(const char *path, int nx, int ny, double *metric);
+int lns3d_write_metric(const char *path, int nx, int ny, const double *metric);
+
 /* Host-array driving of the time loop of src/impdrv3d.f90:126-406 (for
  * callers that hold v in host memory, such as the Fortran host layer):
  * upload v (5,nx,ny) once, advance nsteps steps on the device (the first

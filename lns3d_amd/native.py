@@ -59,6 +59,15 @@ SIGNATURES = {
     "lns3d_step": ([VP, VP, VP], CI),
     "lns3d_step_refined": ([VP, VP, VP, CD, CI, ctypes.POINTER(CI), ctypes.POINTER(CD)], CI),
     "lns3d_step_fp32": ([VP, VP, VP], CI),
+    "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
+    "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
+    "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
+    "lns3d_read_mean": ([ctypes.c_char_p, VP, VP], CI),
+    "lns3d_write_mean": ([ctypes.c_char_p, VP, VP], CI),
+    "lns3d_read_grid": ([ctypes.c_char_p, ctypes.POINTER(CI), ctypes.POINTER(CI), ctypes.POINTER(CI), VP, VP], CI),
+    "lns3d_write_grid": ([ctypes.c_char_p, CI, CI, CI, VP, VP], CI),
+    "lns3d_read_metric": ([ctypes.c_char_p, CI, CI, VP], CI),
+    "lns3d_write_metric": ([ctypes.c_char_p, CI, CI, VP], CI),
 }
 
 _lib = None
