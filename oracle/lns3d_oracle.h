@@ -70,6 +70,18 @@ int orc_set_time_step(const orc_params *p, const double *vm, const double *metri
 
 /* ---- one implicit step pieces (src/lrhs3d.f90, impdrv3d.f90, lhs*f3d.f90,
  *      lhsbt*f3d.f90, lhsbc*f3d.f90, rhsbc3d.f90, itrbc3d.f90) ------------- */
+/* sponge state read by orc_lrhs3d and orc_lhs, as the reference's module
+ * state (src/global.f90:132-141, src/sponge.f90): ispg 0 (off), 1, 2, 3 or 5
+ * (4, the iterative form of src/sponge.f90:536, is not restated); spg, spg2
+ * (nx,ny); x (nx,ny) physical x and vm (5,nx,ny) mean flow for the inflow
+ * forcing of ispg 2, 3; wamp (nx) for ispg 3.  The arrays must outlive their
+ * use.  Returns -1 on a bad combination. */
+typedef struct {
+  int ispg;
+  const double *spg, *spg2, *x, *wamp, *vm;
+  double x0;
+} orc_sponge;
+int orc_set_sponge(const orc_sponge *s);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
                zdouble *r, zdouble *work /* 5 arrays of (5,nx,ny) */);
 void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v,

@@ -31,6 +31,11 @@ class _OrcParams(ctypes.Structure):
                [("periodic_exact", ctypes.c_int)]
 
 
+class _OrcSponge(ctypes.Structure):
+    _fields_ = [("ispg", ctypes.c_int)] + [(n, VP) for n in ("spg", "spg2", "x", "wamp", "vm")] + \
+               [("x0", ctypes.c_double)]
+
+
 class _OrcCoeffs(ctypes.Structure):
     _fields_ = [(n, VP) for n in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi")]
 
@@ -56,6 +61,16 @@ class Oracle:
         for name, _ in _OrcParams._fields_:
             setattr(s, name, getattr(p, name))
         return s
+
+    def set_sponge(self, ispg, spg=None, spg2=None, x=None, wamp=None, vm=None, x0=0.0):
+        """sponge state for later lrhs3d / lhs / step calls (ispg 0: off); the
+        arrays are kept referenced here while they are in use"""
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64)
+                for a in (spg, spg2, x, wamp, vm)]
+        s = _OrcSponge(ispg, *(_ptr(a) for a in arrs), x0)
+        rc = self.lib.orc_set_sponge(ctypes.byref(s))
+        assert rc == 0, "bad sponge arguments"
+        self._sponge_keep = arrs
 
     @staticmethod
     def _coeffs(c):
@@ -186,16 +201,22 @@ class Reference:
     def available(path=None):
         return os.path.exists(path or os.path.join(HERE, "_ref", "libref.so"))
 
-    def setup(self, p, x, y, met, vm, usecalcd=0):
+    def setup(self, p, x, y, met, vm, usecalcd=0, ispg=0, x0=0.0):
         ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, 0, 0, p.wall,
                        p.wallt, p.top, p.left, p.right, p.linear, 1, p.loctime, p.mattyp,
-                       0, 1, usecalcd], dtype=np.int32)
+                       ispg, 1, usecalcd], dtype=np.int32)
         rp = np.array([p.Ma, p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax, p.dtmax,
-                       0.0, 0.0])
+                       0.0, x0])
         self.lib.ref_init(ip.ctypes.data_as(VP), rp.ctypes.data_as(VP))
         self.lib.ref_set_grid(_ptr(x), _ptr(y), _ptr(met))
         self.lib.ref_set_mean(_ptr(vm))
         self.p = p
+
+    def set_sponge(self, spg, spg2, wamp):
+        """the reference's spg, spg2 (nx,ny) and wamp (nx) module arrays"""
+        self.lib.ref_set_sponge(_ptr(np.asfortranarray(spg, dtype=np.float64)),
+                                _ptr(np.asfortranarray(spg2, dtype=np.float64)),
+                                _ptr(np.ascontiguousarray(wamp, dtype=np.float64)))
 
     def genmtrx(self):
         self.lib.ref_genmtrx()

@@ -109,6 +109,21 @@ subroutine ref_set_mean(vml) bind(C, name="ref_set_mean")
   vm = vml
 end subroutine ref_set_mean
 
+! sponge profiles spg, spg2 (nx*ny) and the inflow amplitude wamp (nx), the
+! module arrays that src/sponge.f90:20 (init_sponge) and the forcing input
+! fill; ispg and x0 come from ref_init
+subroutine ref_set_sponge(spgl, spg2l, wampl) bind(C, name="ref_set_sponge")
+  use iso_c_binding
+  use global
+  implicit none
+  real(c_double), intent(in) :: spgl(nx,ny), spg2l(nx,ny), wampl(nx)
+  if (allocated(spg)) deallocate(spg)
+  if (allocated(spg2)) deallocate(spg2)
+  if (allocated(wamp)) deallocate(wamp)
+  allocate(spg(nx,ny), spg2(nx,ny), wamp(nx))
+  spg = spgl; spg2 = spg2l; wamp = wampl
+end subroutine ref_set_sponge
+
 ! coefficient generation (src/genmtrx.f90) and time step (src/temporal.f90)
 subroutine ref_genmtrx() bind(C, name="ref_genmtrx")
   use global

@@ -217,7 +217,55 @@ static int supported(const orc_params *p) {
   return 1;
 }
 
-/* ---- src/lrhs3d.f90:45-285 (point operator, ispg = 0, eps_e = 0) ---------- */
+/* ---- sponge: the module state src/lrhs3d.f90:222-289 and the LHS routines
+ *      read (spg, spg2 of src/sponge.f90:20, wamp, x0, omega) ----------------- */
+static orc_sponge g_spg;          /* ispg = 0: no sponge */
+
+int orc_set_sponge(const orc_sponge *s) {
+  if (!s || s->ispg == 0) { memset(&g_spg, 0, sizeof g_spg); return 0; }
+  if (s->ispg < 0 || s->ispg == 4 || s->ispg > 5 || !s->spg) return -1;
+  if (s->ispg >= 2 && !s->spg2) return -1;
+  if ((s->ispg == 2 || s->ispg == 3) && (!s->x || !s->vm)) return -1;
+  if (s->ispg == 3 && !s->wamp) return -1;
+  g_spg = *s;
+  return 0;
+}
+
+/* LHS diagonal sponge of the factor without the D terms (src/lhs1f3d.f90:207-225,
+ * lhsbt1f3d.f90:156-174): spg (ispg = 1) or spg + spg2 (ispg >= 2) */
+static double lhs_sponge(long q) {
+  if (g_spg.ispg == 0) return 0.0;
+  return g_spg.ispg == 1 ? g_spg.spg[q] : g_spg.spg[q] + g_spg.spg2[q];
+}
+
+/* src/lrhs3d.f90:222-282: the sponge terms of point q (i = q mod nx) */
+static void rhs_sponge(const orc_params *p, long q, int i, const zdouble *vq, zdouble rr[5]) {
+  const int ispg = g_spg.ispg;
+  if (ispg == 1) {
+    for (int m = 0; m < 5; m++) rr[m] = rr[m] + g_spg.spg[q] * vq[m];
+  } else if (ispg > 4) {
+    for (int m = 0; m < 5; m++) rr[m] = rr[m] + (g_spg.spg[q] + g_spg.spg2[q]) * vq[m];
+  } else if (ispg == 2 || ispg == 3) {
+    for (int m = 0; m < 5; m++) rr[m] = rr[m] + g_spg.spg[q] * vq[m];       /* outflow */
+    const double *vml = g_spg.vm + 5 * q;                                   /* inflow */
+    const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
+    const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / p->Re;
+    const double kk = p->omega / (cm + um);
+    const double a = p->omega * p->omega * d / ((cm + um) * (cm + um) * (cm + um));
+    const double xl = g_spg.x[q];
+    zdouble c3 = exp(-a * (xl - g_spg.x0)) * cexp(_Complex_I * (kk * xl));
+    if (ispg == 3) c3 = g_spg.wamp[i] * c3;
+    const double s2 = g_spg.spg2[q], rho = vml[0], T = vml[4];
+    const double gM2 = p->gamma * p->Ma * p->Ma;
+    rr[0] = rr[0] + s2 * (vq[0] - 0.5 * c3 / (cm * cm));
+    rr[1] = rr[1] + s2 * (vq[1] - c3 * 0.5 / (rho * cm));
+    rr[2] = rr[2] + s2 * vq[2];
+    rr[3] = rr[3] + s2 * vq[3];
+    rr[4] = rr[4] + s2 * (vq[4] - (gM2 * c3 * 0.5 - T * 0.5 * c3 / (cm * cm)) / rho);
+  }
+}
+
+/* ---- src/lrhs3d.f90:45-285 (point operator, eps_e = 0) -------------------- */
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
                zdouble *r, zdouble *work) {
   if (!supported(p)) return -1;
@@ -263,6 +311,7 @@ int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
     for (int m = 0; m < 5; m++)
       rr[m] = rr[m] + ci * (Di[m] * vq[0] + Di[m + 5] * vq[1] + Di[m + 10] * vq[2] +
                            Di[m + 15] * vq[3] + Di[m + 20] * vq[4]);
+    rhs_sponge(p, q, (int)(q % nx), vq, rr);
     for (int m = 0; m < 5; m++) r[5 * q + m] = rr[m];
   }
   return 0;
@@ -287,7 +336,8 @@ void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v, const zdouble 
 static void closure_row(zdouble *mat, long pt, const double F[5], const double S[5],
                         const int sl[5], double alfa, double dinv, double dsinv,
                         double dtl, const double *X, const double *Xi2 /* 2 */,
-                        const double *D, const double *Di, const double *Vh, double rcalcd) {
+                        const double *D, const double *Di, const double *Vh, double rcalcd,
+                        double spgd /* alfa dtl (spg [+ spg2]) or 0 */) {
   double a[5], cim[5];
   for (int q = 0; q < 5; q++) a[q] = alfa * F[q] * dinv;
   for (int r = 0; r < 5; r++)
@@ -316,6 +366,7 @@ static void closure_row(zdouble *mat, long pt, const double F[5], const double S
     MAT(k, 4, 4) -= b * dtl * Vh[3];
   }
   for (int r = 0; r < 5; r++) MAT(sl[0], r, r) += 1.0;
+  for (int r = 0; r < 5; r++) MAT(sl[0], r, r) += spgd;
 }
 
 int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl,
@@ -362,6 +413,8 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
         MAT(k, 4, 4) -= b[k] * dt * Vp[3];
       }
       for (int r = 0; r < 5; r++) MAT(2, r, r) += 1.0;
+      if (!calcd && g_spg.ispg)                  /* src/lhs2f3d.f90:207-225 */
+        for (int r = 0; r < 5; r++) MAT(2, r, r) += alfa * dt * lhs_sponge(pt);
     }
 
   int per = dir == 1 ? p->xper : p->yper;
@@ -390,7 +443,7 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
         long pt = i + (long)nx * j;
         closure_row(mat, pt, Fs[w], Ss[w], sls[w], alfa, dinv, dsinv, dtl[pt],
                     X + 25 * pt, c->ABhi + 6 * pt + off, c->Dh + 25 * pt, c->Dhi + 25 * pt,
-                    Vh + 6 * pt, rc);
+                    Vh + 6 * pt, rc, (!calcd && g_spg.ispg) ? alfa * dtl[pt] * lhs_sponge(pt) : 0.0);
       }
     }
   }

@@ -18,6 +18,12 @@ reference's outputs:
                         and cpenta1p/cpenta2p (as shipped) on seeded lines
   ref_config0.npz       the 129x65 benchmark step: norms of the first two
                         increments and the increment on every 4th point
+  ref_sponge_<ispg>.npz sponge cases ispg = 1, 2, 3, 5: profiles shaped as in
+                        src/sponge.f90:20 (init_sponge), inflow amplitude
+                        wamp and x0; lrhs3D, both assembled LHS factors and
+                        one implicit step with the sponge
+
+    python tests/golden/gen_golden.py sponge     (the sponge cases only)
 """
 from __future__ import annotations
 
@@ -127,14 +133,68 @@ def config0(R):
                                         np.sqrt((np.abs(dv2) ** 2).sum())]))
 
 
+SPONGE_CASES = (1, 2, 3, 5)
+
+
+def sponge_profiles(p, x, ispg):
+    """outflow spg on the computational xi and an inflow / far-field spg2 on
+    xi (ispg 2: left side) or eta (ispg 3, 5), as src/sponge.f90:56-210 forms
+    them with compSpg = compSpg2 = .true."""
+    xi = np.linspace(0.0, 1.0, p.nx)[:, None] * np.ones((1, p.ny))
+    eta = np.ones((p.nx, 1)) * np.linspace(0.0, 1.0, p.ny)[None, :]
+    As, Ns, xs, xt = 2.0, 3, 0.7, 1.0
+    spg = np.where(xi >= xs, As * ((xi - xs) / (xt - xs)) ** Ns, 0.0)
+    As2, Ns2 = 1.5, 2
+    if ispg == 2:
+        xs2, xt2 = 0.3, 0.0
+        spg2 = np.where(xi <= xs2, As2 * ((xi - xs2) / (xt2 - xs2)) ** Ns2, 0.0)
+    else:
+        xs2, xt2 = 0.5, 1.0
+        spg2 = np.where((eta >= xs2) & (eta <= xt2), As2 * ((eta - xs2) / (xt2 - xs2)) ** Ns2, 0.0)
+    wamp = 1.0 + 0.25 * np.cos(np.linspace(0.0, 3.0, p.nx))
+    return np.asfortranarray(spg), np.asfortranarray(spg2), wamp
+
+
+def sponge_case(R, ispg):
+    p = Params(nx=13, ny=9, omega=0.3, kz=0.4)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(17)      # a field over the whole grid, sponge zones included
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    x0 = float(x.min()) + 0.5
+    spg, spg2, wamp = sponge_profiles(p, x, ispg)
+    R.setup(p, x, y, met, vm, ispg=ispg, x0=x0)
+    R.set_sponge(spg, spg2, wamp)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), ispg=np.array(ispg), x0=np.array(x0), x=x, y=y, met=met,
+               vm=vm, v0=v0, spg=spg, spg2=spg2, wamp=wamp)
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    out["lrhs"] = r
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    out["dv1"] = R.step(v, True)
+    np.savez_compressed(os.path.join(HERE, f"ref_sponge_{ispg}.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
     R = Reference()
+    if sys.argv[1:] == ["sponge"]:
+        for ispg in SPONGE_CASES:
+            sponge_case(R, ispg)
+        R.lib.ref_flush()
+        return
     for name, kw in STEP_CASES.items():
         step_case(R, name, kw, keep_lhs=(name in ("base", "xper")))
     penta_case(R)
     config0(R)
+    for ispg in SPONGE_CASES:
+        sponge_case(R, ispg)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

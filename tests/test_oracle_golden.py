@@ -134,3 +134,33 @@ def test_penta_periodic_golden(orc, layout):
     per2 = np.zeros(shp, complex, order="F")
     orc.cpenta_p(layout, m, x, per, per2, 2, exact=0)
     assert rel(x, z[f"p{layout}_x"]) < 1e-12
+
+
+@pytest.mark.parametrize("ispg", [1, 2, 3, 5])
+def test_sponge(orc, ispg):
+    """sponge terms (src/lrhs3d.f90:222-282; src/lhs2f3d.f90:207-225 and the
+    closure rows of src/lhsbt2f3d.f90, the factor without the D terms) against
+    the reference's lrhs3D, assembled factors and implicit step with its
+    spg, spg2, wamp, x0 module state"""
+    z = load(f"ref_sponge_{ispg}.npz")
+    p = params_of(z)
+    met, vm, x = (np.asfortranarray(z[k]) for k in ("met", "vm", "x"))
+    v0 = np.asfortranarray(z["v0"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    r0, _ = orc.lrhs3d(p, c, v0)              # without the sponge: far from the reference
+    assert rel(r0, z["lrhs"]) > 1e-10
+    # the sponge rides in the eta factor only (the xi factor carries the D terms)
+    assert rel(orc.lhs(p, 2, c, dtl), z["lhs2"]) > 1e-10
+    assert rel(orc.lhs(p, 1, c, dtl), z["lhs1"]) < 1e-14
+    try:
+        orc.set_sponge(ispg, z["spg"], z["spg2"], x, z["wamp"], vm, float(z["x0"]))
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        for d in (1, 2):
+            assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        v = np.asfortranarray(z["itrbc"].copy())
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+    finally:
+        orc.set_sponge(0)
