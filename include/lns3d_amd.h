@@ -30,8 +30,9 @@ const char *lns3d_last_error(void);
  * that the step routines read (src/input.f90 deck entries of the same names).
  * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
  * top = left = right = 0; linear = 2; mattyp in {0,2,3}; loctime in {0,1};
- * Navier-Stokes (viscous) boundary rows; no sponge; eps_e = 0.  Other values
- * are rejected with LNS3D_EUNSUPPORTED. */
+ * Navier-Stokes (viscous) boundary rows; eps_e = 0; sponges through
+ * lns3d_set_sponge (ispg 1, 2, 3, 5).  Other values are rejected with
+ * LNS3D_EUNSUPPORTED. */
 typedef struct {
   int nx, ny;
   int optx, opty;
@@ -115,6 +116,19 @@ int lns3d_ctx_synchronize(lns3d_ctx *c);
  * coefficients of src/genmtrx.f90:2 (Ah,Bh,Dh,Dhi,Vh11,Vh12,Vh22,ABhi) and
  * the time step dtl of src/temporal.f90:12 (set_time_step). */
 int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm);
+/* Sponge (src/global.f90:132-141 ispg, spg, spg2, wamp, x0): the terms of
+ * src/lrhs3d.f90:222-282 in the residual and the diagonal of
+ * src/lhs2f3d.f90:207-225 (and its closure rows) in the eta factor, the
+ * factor without the D terms.  ispg 0 (none), 1 (spg), 2 and 3 (spg, and
+ * spg2 toward the incoming acoustic wave of frequency omega: x physical
+ * coordinate, x0, and for 3 the amplitude wamp(nx)) or 5 (spg + spg2).  spg,
+ * spg2, x are host (nx,ny) arrays as src/sponge.f90:20 (init_sponge) forms
+ * them.  Call after lns3d_setup (the forcing uses the mean flow and the
+ * context's omega, Ma, Re, Pr, gamma) and before lns3d_factor; again after
+ * lns3d_ctx_set_params.  ispg = 4 (src/sponge.f90:536) returns
+ * LNS3D_EUNSUPPORTED. */
+int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const double *spg2, const double *x,
+                     const double *wamp, double x0);
 /* alternatively, take the coefficients the caller already has (host arrays
  * in the module layouts of src/local2d.f90:34-36, src/local3d.f90:14) */
 int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,

@@ -33,7 +33,9 @@ struct lns3d_ctx {
   void *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;   // 5P complex each (refinement)
   double *scal = nullptr;    // device scalars (residual norms at +8, partials after)
   double *coef_block = nullptr;
+  double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
   bool coeffs_ready = false, factored = false;
+  bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
   void *vstate = nullptr;    // 5P complex: the state of the host-array driver
   bool state_fresh = false;  // uploaded, boundary conditions not yet applied
   int last_iters = 0;
@@ -103,6 +105,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->t3) (void)hipFree(c->t3);
   if (c->scal) (void)hipFree(c->scal);
   if (c->vstate) (void)hipFree(c->vstate);
+  if (c->sponge_block) (void)hipFree(c->sponge_block);
   delete c;
   return 0;
 }
@@ -142,7 +145,59 @@ extern "C" int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm)
   if (!rc) rc = launch_timestep(c->p, c->c, c->scal, c->stream);
   if (rc) return rc;
   c->coeffs_ready = true;
+  c->vm_ready = true;
   c->factored = false;
+  return 0;
+}
+
+// Sponge (src/global.f90:132-141 module state: ispg, spg, spg2, wamp, x0;
+// terms of src/lrhs3d.f90:222-282 and src/lhs2f3d.f90:207-225).  The
+// residual's sponge terms and the eta factor's diagonal are formed from the
+// profiles here; the inflow forcing (ispg 2, 3) from the mean flow, x, wamp,
+// x0 and the context's omega, Ma, Re, Pr, gamma.
+extern "C" int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const double *spg2,
+                                const double *x, const double *wamp, double x0) {
+  if (!c) return fail(LNS3D_EINVAL, "lns3d_set_sponge: null context");
+  if (ispg < 0 || ispg > 5) return fail(LNS3D_EINVAL, "lns3d_set_sponge: ispg must be 0..5");
+  if (ispg == 4)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_set_sponge: ispg = 4 (iterative sponge, src/sponge.f90:536) "
+                                    "is not implemented");
+  c->factored = false;
+  if (ispg == 0) {
+    c->c.sp1 = c->c.sp2 = c->c.spf = c->c.spl = nullptr;
+    return 0;
+  }
+  const bool inflow = ispg == 2 || ispg == 3;
+  if (!spg || (ispg >= 2 && !spg2) || (inflow && !x) || (ispg == 3 && !wamp))
+    return fail(LNS3D_EINVAL, "lns3d_set_sponge: a profile this ispg needs is null");
+  if (inflow && !c->vm_ready)
+    return fail(LNS3D_EINVAL, "lns3d_set_sponge: the inflow forcing needs the mean flow (lns3d_setup first)");
+  const long P = (long)c->p.nx * c->p.ny;
+  if (!c->sponge_block)
+    LNS3D_CHECK(hipMalloc(&c->sponge_block, (size_t)(14 * P + c->p.nx) * sizeof(double)));
+  double *sp1 = c->sponge_block, *sp2 = sp1 + P, *spl = sp2 + P, *spf = spl + P, *xd = spf + 10 * P,
+         *wd = xd + P;
+  std::vector<double> h1((size_t)P), hl((size_t)P);
+  for (long q = 0; q < P; q++) {
+    h1[q] = ispg == 5 ? spg[q] + spg2[q] : spg[q];
+    hl[q] = ispg == 1 ? spg[q] : spg[q] + spg2[q];
+  }
+  LNS3D_CHECK(hipMemcpyAsync(sp1, h1.data(), (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+  LNS3D_CHECK(hipMemcpyAsync(spl, hl.data(), (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+  c->c.sp1 = sp1;
+  c->c.spl = spl;
+  c->c.sp2 = c->c.spf = nullptr;
+  if (inflow) {
+    LNS3D_CHECK(hipMemcpyAsync(sp2, spg2, (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    LNS3D_CHECK(hipMemcpyAsync(xd, x, (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    if (ispg == 3)
+      LNS3D_CHECK(hipMemcpyAsync(wd, wamp, (size_t)c->p.nx * 8, hipMemcpyHostToDevice, c->stream));
+    c->c.sp2 = sp2;
+    c->c.spf = spf;
+    int rc = launch_sponge_forcing(c->p, ispg, c->c, xd, wd, x0, c->stream);
+    if (rc) return rc;
+  }
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));     // the host staging vectors end here
   return 0;
 }
 

@@ -433,6 +433,49 @@ int launch_timestep(const lns3d_params &p, Coeffs &c, double *work, hipStream_t 
 }
 
 // ==========================================================================
+// inflow forcing of the sponge, src/lrhs3d.f90:248-282: per point the
+// complex amplitude c3 of the incoming acoustic wave and the five terms
+// subtracted from v under spg2 (components 3 and 4 have none)
+__global__ void k_sponge_forcing(lns3d_params p, int ispg, const double *vm, const double *x,
+                                 const double *wamp, double x0, double *spf) {
+  const long P = (long)p.nx * p.ny;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  const int i = (int)(q % p.nx);
+  const double *vml = vm + 5 * q;
+  const double cm = sqrt(vml[4]) / p.Ma, um = vml[1];
+  const double d = 0.5 * (1.3333333333333333333 + (p.gamma - 1.0) / p.Pr) / p.Re;
+  const double kk = p.omega / (cm + um);
+  const double a = p.omega * p.omega * d / ((cm + um) * (cm + um) * (cm + um));
+  const double xl = x[q];
+  double sn, cs;
+  sincos(kk * xl, &sn, &cs);
+  const double e = exp(-a * (xl - x0));
+  z c3{e * cs, e * sn};
+  if (ispg == 3) c3 = sc(wamp[i], c3);
+  const double rho = vml[0], T = vml[4], cm2 = cm * cm, gM2 = p.gamma * p.Ma * p.Ma;
+  z f[5];
+  f[0] = sc(0.5 / cm2, c3);
+  f[1] = sc(0.5 / (rho * cm), c3);
+  f[2] = z{0, 0};
+  f[3] = z{0, 0};
+  f[4] = sc(1.0 / rho, sc(gM2 * 0.5, c3) - sc(T * 0.5 / cm2, c3));
+  for (int m = 0; m < 5; m++) {
+    spf[2 * (m * P + q)] = f[m].re;
+    spf[2 * (m * P + q) + 1] = f[m].im;
+  }
+}
+
+int launch_sponge_forcing(const lns3d_params &p, int ispg, const Coeffs &c, const double *x,
+                          const double *wamp, double x0, hipStream_t s) {
+  const long P = (long)p.nx * p.ny;
+  hipLaunchKernelGGL(k_sponge_forcing, dim3(nblocks(P, 256)), dim3(256), 0, s, p, ispg,
+                     (const double *)c.vm, x, wamp, x0, c.spf);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
 // LHS assembly: value of entry (slot k, row r, col cc) at point (i, j)
 struct AsmCtx {
   lns3d_params p;
@@ -524,6 +567,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
     if (ai >= 0) im = im - af * dt * ABv;
     if (vi >= 0) re = re - alfa * S * A.dsinv * dt * Vv;
     if (diag && r == cc) re = re + 1.0;
+    if (!calcd && diag && r == cc && c.spl) re = re + alfa * dt * c.spl[q];   // lhsbt2f3d.f90:156-174
     return z{re, im};
   }
   // interior (src/lhs1f3d.f90:59-230)
@@ -535,6 +579,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   }
   if (vi >= 0) re = re - A.b[k] * dt * Vv;
   if (k == 2 && r == cc) re = re + 1.0;
+  if (!calcd && k == 2 && r == cc && c.spl) re = re + alfa * dt * c.spl[q];  // lhs2f3d.f90:207-225
   return z{re, im};
 }
 
@@ -710,6 +755,12 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
 #pragma unroll
       for (int e = 0; e < 5; e++) b = b + sc(c.Dhi[(long)(m + 5 * e) * P + q], vq(e));
       a = a + z{-b.im, b.re};                         // + i * b
+    }
+    // sponge (src/lrhs3d.f90:222-282): sp1 v, then the inflow part sp2 (v - f)
+    if (c.sp1) a = a + sc(c.sp1[q], vq(m));
+    if (c.sp2) {
+      const z f{c.spf[2 * ((long)m * P + q)], c.spf[2 * ((long)m * P + q) + 1]};
+      a = a + sc(c.sp2[q], vq(m) - f);
     }
     // backward Euler: r = vold - v - dtl * r with vold = v (the field already
     // satisfies the boundary conditions when the step starts)
@@ -944,6 +995,9 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
       const double f = p.alfa * dt;
       out[r] = out[r] + sc(f, a) + z{-f * b.im, f * b.re};
     }
+  } else if (c.spl) {                  // sponge of the factor without the D terms
+    const double f = p.alfa * dt * c.spl[qr];
+    for (int r = 0; r < 5; r++) out[r] = out[r] + sc(f, xc[r]);
   }
   for (int r = 0; r < 5; r++) out[r] = out[r] + xc[r];
   // boundary-condition rows (src/lhsbc1f3d.f90, lhsbc2f3d.f90), as lhs_entry

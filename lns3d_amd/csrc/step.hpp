@@ -24,10 +24,19 @@ struct Coeffs {
   double *dtl;                  // [P]
   double *vm;                   // [5][P] mean flow (AoS (5,nx,ny) as passed)
   double *metric;               // [10][P]
+  // sponge (src/lrhs3d.f90:222-282, src/lhs2f3d.f90:207-225); null: none
+  double *sp1;                  // [P] on v in the residual: spg (ispg 1, 2, 3) or spg + spg2 (5)
+  double *sp2;                  // [P] inflow sponge spg2 on (v - forcing) (ispg 2, 3)
+  double *spf;                  // [5][P] complex inflow forcing, (re, im) pairs (ispg 2, 3)
+  double *spl;                  // [P] eta-factor diagonal: spg (ispg 1) or spg + spg2
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);
 int launch_timestep(const lns3d_params &p, Coeffs &c, double *work, hipStream_t s);
+// inflow forcing of the sponge (ispg 2 or 3) into c.spf from the mean flow c.vm,
+// the physical x (P, device) and wamp (nx, device; ispg 3)
+int launch_sponge_forcing(const lns3d_params &p, int ispg, const Coeffs &c, const double *x,
+                          const double *wamp, double x0, hipStream_t s);
 // assemble the LHS of direction dir (1: xi, 2: eta) into the line-interleaved
 // working copy of plan (lines: y index for dir 1, x index for dir 2)
 int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &plan, hipStream_t s);

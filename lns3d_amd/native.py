@@ -59,6 +59,7 @@ SIGNATURES = {
     "lns3d_step": ([VP, VP, VP], CI),
     "lns3d_step_refined": ([VP, VP, VP, CD, CI, ctypes.POINTER(CI), ctypes.POINTER(CD)], CI),
     "lns3d_step_fp32": ([VP, VP, VP], CI),
+    "lns3d_set_sponge": ([VP, CI, VP, VP, VP, VP, CD], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -174,6 +175,14 @@ class Step:
         met = np.asfortranarray(metric, dtype=np.float64)
         v = np.asfortranarray(vm, dtype=np.float64)
         check(lib().lns3d_setup(self.h, met.ctypes.data, v.ctypes.data), "lns3d_setup")
+
+    def set_sponge(self, ispg, spg=None, spg2=None, x=None, wamp=None, x0=0.0):
+        """sponge profiles (nx,ny), physical x (nx,ny) and wamp (nx); after
+        setup, before factor (lns3d_set_sponge)"""
+        import numpy as np
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (spg, spg2, x, wamp)]
+        check(lib().lns3d_set_sponge(self.h, ispg, *[None if a is None else a.ctypes.data for a in arrs],
+                                     float(x0)), "lns3d_set_sponge")
 
     def set_coeffs(self, c, dtl):
         import numpy as np

@@ -1,0 +1,91 @@
+"""GPU parity of the sponge (lns3d_set_sponge: src/lrhs3d.f90:222-282,
+src/lhs2f3d.f90:207-225 and the closure rows of src/lhsbt2f3d.f90) against
+the reference's own lrhs3D, assembled factors and implicit step with its
+sponge module state (tests/golden/ref_sponge_<ispg>.npz, written by
+tests/golden/gen_golden.py from the compiled reference), ispg = 1, 2, 3, 5.
+Tolerance 1e-12 relative, max norm.  The refined step (complex-float factors,
+whose residual operator carries the sponge too) reaches the complex-double
+increment; ispg = 4 is rejected."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _case(ispg):
+    import json
+    from lns3d_amd.config import Params
+    z = np.load(os.path.join(GOLD, f"ref_sponge_{ispg}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    return z, p
+
+
+@pytest.mark.parametrize("ispg", [1, 2, 3, 5])
+def test_sponge_against_reference(ispg):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.native import Step
+    print(f"sponge ispg={ispg}", flush=True)
+    z, p = _case(ispg)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t, shape):
+        return t.cpu().numpy().reshape(shape, order="F")
+
+    st = Step(p)
+    st.setup(met, vm)
+    st.set_sponge(ispg, z["spg"], z["spg2"], z["x"], z["wamp"], float(z["x0"]))
+    v0 = z["v0"]
+    vd = dev(v0)
+    rd = torch.zeros_like(vd)
+    st.lrhs3d(vd, rd)
+    assert rel(host(rd, v0.shape), z["lrhs"]) < TOL
+    for d in (1, 2):
+        m_dev = torch.zeros(125 * p.nx * p.ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        assert rel(host(m_dev, z[f"lhs{d}"].shape), z[f"lhs{d}"]) < TOL, d
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    st.step(vd, dvd)
+    assert rel(host(dvd, v0.shape), z["dv1"]) < TOL
+    st.close()
+
+    # complex-float factors with refinement reach the same increment
+    st32 = Step(p, precision=1)
+    st32.setup(met, vm)
+    st32.set_sponge(ispg, z["spg"], z["spg2"], z["x"], z["wamp"], float(z["x0"]))
+    st32.factor()
+    vb = dev(z["itrbc"])
+    dvb = torch.zeros_like(vb)
+    its, res = st32.step_refined(vb, dvb, tol=1e-12, max_iters=10)
+    e = rel(host(dvb, v0.shape), z["dv1"])
+    print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
+    assert res <= 1e-12 and e < 1e-10
+    st32.close()
+
+
+def test_sponge_rejects_iterative_form():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.native import Step, LNS3DError
+    z, p = _case(1)
+    st = Step(p)
+    st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+    with pytest.raises(LNS3DError):
+        st.set_sponge(4, z["spg"], z["spg2"])
+    st.close()
