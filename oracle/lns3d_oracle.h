@@ -82,6 +82,9 @@ typedef struct {
   double x0;
 } orc_sponge;
 int orc_set_sponge(const orc_sponge *s);
+/* top = 1 (src/rhsbc3d.f90:129-161, itrbc3d.f90:145-195): physical x
+ * (nx,ny), amplitude wamp (nx; NULL for 1), mean flow vm (5,nx,ny), x0 */
+int orc_set_top(const double *x, const double *wamp, const double *vm, double x0);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
                zdouble *r, zdouble *work /* 5 arrays of (5,nx,ny) */);
 void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v,

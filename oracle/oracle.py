@@ -72,6 +72,13 @@ class Oracle:
         assert rc == 0, "bad sponge arguments"
         self._sponge_keep = arrs
 
+    def set_top(self, x=None, wamp=None, vm=None, x0=0.0):
+        """top = 1 forcing data (x, vm None: cleared)"""
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (x, wamp, vm)]
+        self.lib.orc_set_top.argtypes = [VP, VP, VP, ctypes.c_double]
+        self.lib.orc_set_top(*(_ptr(a) for a in arrs), x0)
+        self._top_keep = arrs
+
     @staticmethod
     def _coeffs(c):
         s = _OrcCoeffs()

@@ -70,6 +70,11 @@ subroutine ref_init(ip, rp) bind(C, name="ref_init")
   if (allocated(dtlh)) deallocate(dtlh)
   allocate(dtlh(nx,ny))
   dtlh = zero
+  ! forcing state of an earlier case: itrBC3D (top = 1) allocates wamp itself
+  ! at istep = 0 (src/itrbc3d.f90:145-160)
+  if (allocated(spg)) deallocate(spg)
+  if (allocated(spg2)) deallocate(spg2)
+  if (allocated(wamp)) deallocate(wamp)
 end subroutine ref_init
 
 ! grid and metrics (each nx*ny, Fortran order), then the products as in

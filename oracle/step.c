@@ -211,7 +211,7 @@ void orc_grad2(const orc_params *p, int ndof, const double *v, const double *g1,
 static int supported(const orc_params *p) {
   if (p->nx < 7 || p->ny < 7) return 0;
   if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
-  if (p->top != 0 || p->left != 0 || p->right != 0) return 0;
+  if ((p->top != 0 && p->top != 1) || p->left != 0 || p->right != 0) return 0;
   if (p->linear != 2) return 0;
   if (p->mattyp < 0 || p->mattyp > 3) return 0;
   return 1;
@@ -238,6 +238,41 @@ static double lhs_sponge(long q) {
   return g_spg.ispg == 1 ? g_spg.spg[q] : g_spg.spg[q] + g_spg.spg2[q];
 }
 
+/* the incoming acoustic wave of src/lrhs3d.f90:248-282 and
+ * src/rhsbc3d.f90:131-159: amplitude amp, mean state vml at physical x xl;
+ * f[m] is the wave's disturbance in component m */
+static void acoustic_wave(const orc_params *p, const double *vml, double xl, double x0, double amp,
+                          zdouble f[5]) {
+  const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
+  const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / p->Re;
+  const double kk = p->omega / (cm + um);
+  const double a = p->omega * p->omega * d / ((cm + um) * (cm + um) * (cm + um));
+  const zdouble c3 = amp * (exp(-a * (xl - x0)) * cexp(_Complex_I * (kk * xl)));
+  const double rho = vml[0], T = vml[4];
+  const double gM2 = p->gamma * p->Ma * p->Ma;
+  f[0] = 0.5 * c3 / (cm * cm);
+  f[1] = c3 * 0.5 / (rho * cm);
+  f[2] = 0;
+  f[3] = 0;
+  f[4] = (gM2 * c3 * 0.5 - T * 0.5 * c3 / (cm * cm)) / rho;
+}
+
+/* top = 1 forcing state (src/global.f90 x, wamp, x0): wamp NULL means 1 */
+static struct { const double *x, *wamp, *vm; double x0; } g_top;
+
+int orc_set_top(const double *x, const double *wamp, const double *vm, double x0) {
+  g_top.x = x; g_top.wamp = wamp; g_top.vm = vm; g_top.x0 = x0;
+  return 0;
+}
+
+/* the wave on the top boundary at column i (src/rhsbc3d.f90:131-150) */
+static int top_wave(const orc_params *p, int i, zdouble f[5]) {
+  if (!g_top.x || !g_top.vm) return -1;
+  const long q = i + (long)p->nx * (p->ny - 1);
+  acoustic_wave(p, g_top.vm + 5 * q, g_top.x[q], g_top.x0, g_top.wamp ? g_top.wamp[i] : 1.0, f);
+  return 0;
+}
+
 /* src/lrhs3d.f90:222-282: the sponge terms of point q (i = q mod nx) */
 static void rhs_sponge(const orc_params *p, long q, int i, const zdouble *vq, zdouble rr[5]) {
   const int ispg = g_spg.ispg;
@@ -247,21 +282,10 @@ static void rhs_sponge(const orc_params *p, long q, int i, const zdouble *vq, zd
     for (int m = 0; m < 5; m++) rr[m] = rr[m] + (g_spg.spg[q] + g_spg.spg2[q]) * vq[m];
   } else if (ispg == 2 || ispg == 3) {
     for (int m = 0; m < 5; m++) rr[m] = rr[m] + g_spg.spg[q] * vq[m];       /* outflow */
-    const double *vml = g_spg.vm + 5 * q;                                   /* inflow */
-    const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
-    const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / p->Re;
-    const double kk = p->omega / (cm + um);
-    const double a = p->omega * p->omega * d / ((cm + um) * (cm + um) * (cm + um));
-    const double xl = g_spg.x[q];
-    zdouble c3 = exp(-a * (xl - g_spg.x0)) * cexp(_Complex_I * (kk * xl));
-    if (ispg == 3) c3 = g_spg.wamp[i] * c3;
-    const double s2 = g_spg.spg2[q], rho = vml[0], T = vml[4];
-    const double gM2 = p->gamma * p->Ma * p->Ma;
-    rr[0] = rr[0] + s2 * (vq[0] - 0.5 * c3 / (cm * cm));
-    rr[1] = rr[1] + s2 * (vq[1] - c3 * 0.5 / (rho * cm));
-    rr[2] = rr[2] + s2 * vq[2];
-    rr[3] = rr[3] + s2 * vq[3];
-    rr[4] = rr[4] + s2 * (vq[4] - (gM2 * c3 * 0.5 - T * 0.5 * c3 / (cm * cm)) / rho);
+    zdouble f[5];                                                           /* inflow */
+    acoustic_wave(p, g_spg.vm + 5 * q, g_spg.x[q], g_spg.x0, ispg == 3 ? g_spg.wamp[i] : 1.0, f);
+    const double s2 = g_spg.spg2[q];
+    for (int m = 0; m < 5; m++) rr[m] = rr[m] + s2 * (vq[m] - f[m]);
   }
 }
 
@@ -520,8 +544,15 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
         R(4, i, 0) = gc1 * V(4, i, 0) + gc2 * V(4, i, 1) + gc3 * V(4, i, 2) +
                      gc4 * V(4, i, 3) + gc5 * V(4, i, 4);
     }
-    for (int i = 0; i < nx; i++)
-      for (int d = 0; d < 5; d++) R(d, i, ny - 1) = 0;
+    for (int i = 0; i < nx; i++) {
+      if (p->top == 1) {                           /* src/rhsbc3d.f90:129-161 */
+        zdouble f[5];
+        if (top_wave(p, i, f)) return -1;
+        for (int d = 0; d < 5; d++) R(d, i, ny - 1) = -(V(d, i, ny - 1) - f[d]);
+      } else {
+        for (int d = 0; d < 5; d++) R(d, i, ny - 1) = 0;
+      }
+    }
   }
   if (p->xper) {
     for (int j = 0; j < ny; j++)
@@ -552,8 +583,15 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
         V(4, i, 0) = -(gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
                        gc5 * V(4, i, 4)) / gc1;
     }
-    for (int i = 0; i < nx; i++)
-      for (int d = 0; d < 5; d++) V(d, i, ny - 1) = 0;
+    for (int i = 0; i < nx; i++) {
+      if (p->top == 1) {                           /* src/itrbc3d.f90:145-195 */
+        zdouble f[5];
+        if (top_wave(p, i, f)) return -1;
+        for (int d = 0; d < 5; d++) V(d, i, ny - 1) = f[d];
+      } else {
+        for (int d = 0; d < 5; d++) V(d, i, ny - 1) = 0;
+      }
+    }
   }
   if (p->xper) {
     for (int j = 0; j < ny; j++)

@@ -18,12 +18,16 @@ reference's outputs:
                         and cpenta1p/cpenta2p (as shipped) on seeded lines
   ref_config0.npz       the 129x65 benchmark step: norms of the first two
                         increments and the increment on every 4th point
+  ref_top1.npz          top = 1 (incoming acoustic wave on the top boundary,
+                        src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195;
+                        wamp = 1 as itrBC3D allocates it): itrBC3D, lrhs3D +
+                        rhsBC3D, both assembled factors, one implicit step
   ref_sponge_<ispg>.npz sponge cases ispg = 1, 2, 3, 5: profiles shaped as in
                         src/sponge.f90:20 (init_sponge), inflow amplitude
                         wamp and x0; lrhs3D, both assembled LHS factors and
                         one implicit step with the sponge
 
-    python tests/golden/gen_golden.py sponge     (the sponge cases only)
+    python tests/golden/gen_golden.py sponge     (the sponge and top = 1 cases only)
 """
 from __future__ import annotations
 
@@ -180,6 +184,31 @@ def sponge_case(R, ispg):
     np.savez_compressed(os.path.join(HERE, f"ref_sponge_{ispg}.npz"), **out)
 
 
+def top1_case(R):
+    p = Params(nx=13, ny=9, omega=0.3, kz=0.4, top=1)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(23)
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    x0 = float(x.min()) + 0.5
+    R.setup(p, x, y, met, vm, x0=x0)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), x0=np.array(x0), x=x, y=y, met=met, vm=vm, v0=v0)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)                          # allocates wamp = 1 (istep = 0)
+    out["itrbc"] = v.copy()
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    out["lrhs"] = r
+    rb = np.asfortranarray(r.copy())
+    R.rhsbc3d(rb, np.asfortranarray(v0))
+    out["rhsbc"] = rb
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    out["dv1"] = R.step(v, True)
+    np.savez_compressed(os.path.join(HERE, "ref_top1.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -187,6 +216,7 @@ def main():
     if sys.argv[1:] == ["sponge"]:
         for ispg in SPONGE_CASES:
             sponge_case(R, ispg)
+        top1_case(R)
         R.lib.ref_flush()
         return
     for name, kw in STEP_CASES.items():
@@ -195,6 +225,7 @@ def main():
     config0(R)
     for ispg in SPONGE_CASES:
         sponge_case(R, ispg)
+    top1_case(R)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -164,3 +164,31 @@ def test_sponge(orc, ispg):
         assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
     finally:
         orc.set_sponge(0)
+
+
+def test_top1(orc):
+    """top = 1, the incoming acoustic wave on the top boundary
+    (src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195), against the
+    reference's itrBC3D, lrhs3D + rhsBC3D, assembled factors and step"""
+    z = load("ref_top1.npz")
+    p = params_of(z)
+    met, vm, x = (np.asfortranarray(z[k]) for k in ("met", "vm", "x"))
+    v0 = np.asfortranarray(z["v0"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    try:
+        orc.set_top(x, None, vm, float(z["x0"]))
+        v = np.asfortranarray(v0.copy())
+        orc.itrbc3d(p, v)
+        assert rel(v, z["itrbc"]) < 1e-14
+        assert np.abs(v[:, 1:-1, -1]).max() > 0.01          # the wave is on the boundary
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        orc.rhsbc3d(p, r, v0)
+        assert rel(r, z["rhsbc"]) < 1e-14
+        for d in (1, 2):
+            assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+    finally:
+        orc.set_top()
