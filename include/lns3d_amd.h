@@ -29,7 +29,7 @@ const char *lns3d_last_error(void);
 /* Scheme, physics and boundary selectors: the module state of src/global.f90
  * that the step routines read (src/input.f90 deck entries of the same names).
  * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
- * top = left = right = 0; linear = 2; mattyp in {0,2,3}; loctime in {0,1};
+ * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left = right = 0; linear = 2; mattyp in {0,2,3}; loctime in {0,1};
  * Navier-Stokes (viscous) boundary rows; eps_e = 0; sponges through
  * lns3d_set_sponge (ispg 1, 2, 3, 5).  Other values are rejected with
  * LNS3D_EUNSUPPORTED. */
@@ -129,6 +129,12 @@ int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm);
  * LNS3D_EUNSUPPORTED. */
 int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const double *spg2, const double *x,
                      const double *wamp, double x0);
+/* top = 1 (src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195): the incoming
+ * acoustic wave of frequency omega on the top boundary, from the mean flow,
+ * the physical x (host (nx,ny); its top row is used), the amplitude wamp (nx;
+ * NULL for 1) and x0.  After lns3d_setup; lns3d_factor refuses top = 1
+ * without it. */
+int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double *wamp, double x0);
 /* alternatively, take the coefficients the caller already has (host arrays
  * in the module layouts of src/local2d.f90:34-36, src/local3d.f90:14) */
 int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,

@@ -34,6 +34,7 @@ struct lns3d_ctx {
   double *scal = nullptr;    // device scalars (residual norms at +8, partials after)
   double *coef_block = nullptr;
   double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
+  double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
   bool coeffs_ready = false, factored = false;
   bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
   void *vstate = nullptr;    // 5P complex: the state of the host-array driver
@@ -44,7 +45,7 @@ struct lns3d_ctx {
 
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
-  if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || p.top != 0 || p.left != 0 ||
+  if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) || p.left != 0 ||
       p.right != 0 || p.linear != 2 || (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) ||
       (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
@@ -106,6 +107,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->scal) (void)hipFree(c->scal);
   if (c->vstate) (void)hipFree(c->vstate);
   if (c->sponge_block) (void)hipFree(c->sponge_block);
+  if (c->top_block) (void)hipFree(c->top_block);
   delete c;
   return 0;
 }
@@ -201,6 +203,28 @@ extern "C" int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const
   return 0;
 }
 
+// top = 1: the incoming acoustic wave on the top boundary
+// (src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195) from the mean flow, the
+// top row of the physical x, wamp (NULL: 1, as src/itrbc3d.f90:154-158) and x0
+extern "C" int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double *wamp, double x0) {
+  if (!c || !x) return fail(LNS3D_EINVAL, "lns3d_set_top_forcing: null argument");
+  if (c->p.top != 1) return fail(LNS3D_EINVAL, "lns3d_set_top_forcing: the context has top != 1");
+  if (!c->vm_ready)
+    return fail(LNS3D_EINVAL, "lns3d_set_top_forcing: the wave needs the mean flow (lns3d_setup first)");
+  const int nx = c->p.nx, ny = c->p.ny;
+  if (!c->top_block) LNS3D_CHECK(hipMalloc(&c->top_block, (size_t)12 * nx * sizeof(double)));
+  double *ftop = c->top_block, *xd = ftop + 10 * nx, *wd = xd + nx;
+  std::vector<double> xt((size_t)nx);
+  for (int i = 0; i < nx; i++) xt[i] = x[i + (long)nx * (ny - 1)];
+  LNS3D_CHECK(hipMemcpyAsync(xd, xt.data(), (size_t)nx * 8, hipMemcpyHostToDevice, c->stream));
+  if (wamp) LNS3D_CHECK(hipMemcpyAsync(wd, wamp, (size_t)nx * 8, hipMemcpyHostToDevice, c->stream));
+  c->c.ftop = ftop;
+  int rc = launch_top_forcing(c->p, c->c, xd, wamp ? wd : nullptr, x0, c->stream);
+  if (rc) return rc;
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 // Coefficients in the reference's layouts (host): Ah,Bh,Dh,Dhi (5,5,nx,ny),
 // Vh11,Vh12,Vh22,ABhi (6,nx,ny), dtl (nx,ny) -- the arrays src/genmtrx.f90
 // and src/temporal.f90 leave in modules local2d/local3d.
@@ -263,6 +287,8 @@ extern "C" int lns3d_get_coeffs(lns3d_ctx *c, double *Ah, double *Bh, double *Dh
 // factor both directions; precision 1 also builds complex-float factors
 extern "C" int lns3d_factor(lns3d_ctx *c) {
   if (!c || !c->coeffs_ready) return fail(LNS3D_EINVAL, "lns3d_factor: no coefficients");
+  if (c->p.top == 1 && !c->c.ftop)
+    return fail(LNS3D_EINVAL, "lns3d_factor: top = 1 needs its incoming wave (lns3d_set_top_forcing)");
   int rc = launch_assemble(c->p, c->c, 1, c->px, c->stream);
   if (!rc) rc = penta_factor(&c->px, c->stream);
   if (!rc) rc = launch_assemble(c->p, c->c, 2, c->py, c->stream);
@@ -312,7 +338,7 @@ extern "C" int lns3d_lrhs3d(lns3d_ctx *c, const void *v_dev, void *r_dev) {
 
 extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
   if (!c || !v_dev) return fail(LNS3D_EINVAL, "lns3d_itrbc3d: bad arguments");
-  return launch_itrbc(c->p, v_dev, c->stream);
+  return launch_itrbc(c->p, c->c.ftop, v_dev, c->stream);
 }
 
 static int solve_xy(lns3d_ctx *c, PentaPlan &px, PentaPlan &py, void *r) {
@@ -344,7 +370,7 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   void *r = dv_dev ? dv_dev : c->r;
   int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
   if (!rc) rc = solve_xy(c, c->px, c->py, r);
-  if (!rc) rc = launch_update_bc(c->p, v_dev, r, c->stream);
+  if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
   return rc;
 }
 
@@ -384,7 +410,7 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
   if (iters_out) *iters_out = it;
   if (resid_out) *resid_out = c->last_resid;
   if (!(c->last_resid == c->last_resid)) return fail(LNS3D_ENONFINITE, "refinement produced NaN");
-  return launch_update_bc(c->p, v_dev, dv, c->stream);
+  return launch_update_bc(c->p, c->c.ftop, v_dev, dv, c->stream);
 }
 
 // the same step, but only the complex-float solve (no refinement): the
@@ -395,7 +421,7 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   void *r = dv_dev ? dv_dev : c->r;
   int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
   if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
-  if (!rc) rc = launch_update_bc(c->p, v_dev, r, c->stream);
+  if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
   return rc;
 }
 
@@ -423,7 +449,7 @@ extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev,
     rc = launch_af_matvec(c->p, c->c, dv, c->t2, c->t1, c->stream);
     if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);
   }
-  if (!rc) rc = launch_update_bc(c->p, v_dev, dv, c->stream);
+  if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, dv, c->stream);
   return rc;
 }
 
@@ -592,7 +618,7 @@ extern "C" int lns3d_advance(lns3d_ctx *c, int nsteps, void *dv_host) {
   if (!c->factored) return fail(LNS3D_ENOTFACTORED, "lns3d_advance: call lns3d_factor first");
   int rc = 0;
   if (c->state_fresh) {
-    rc = launch_itrbc(c->p, c->vstate, c->stream);
+    rc = launch_itrbc(c->p, c->c.ftop, c->vstate, c->stream);
     c->state_fresh = false;
   }
   for (int k = 0; k < nsteps && !rc; k++) rc = lns3d_step(c, c->vstate, c->r);

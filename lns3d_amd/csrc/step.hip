@@ -436,34 +436,58 @@ int launch_timestep(const lns3d_params &p, Coeffs &c, double *work, hipStream_t 
 // inflow forcing of the sponge, src/lrhs3d.f90:248-282: per point the
 // complex amplitude c3 of the incoming acoustic wave and the five terms
 // subtracted from v under spg2 (components 3 and 4 have none)
+__device__ void acoustic_wave(const lns3d_params &p, const double *vml, double xl, double x0, double amp,
+                              z (&f)[5]) {
+  const double cm = sqrt(vml[4]) / p.Ma, um = vml[1];
+  const double d = 0.5 * (1.3333333333333333333 + (p.gamma - 1.0) / p.Pr) / p.Re;
+  const double kk = p.omega / (cm + um);
+  const double a = p.omega * p.omega * d / ((cm + um) * (cm + um) * (cm + um));
+  double sn, cs;
+  sincos(kk * xl, &sn, &cs);
+  const double e = exp(-a * (xl - x0));
+  const z c3 = sc(amp, z{e * cs, e * sn});
+  const double rho = vml[0], T = vml[4], cm2 = cm * cm, gM2 = p.gamma * p.Ma * p.Ma;
+  f[0] = sc(0.5 / cm2, c3);
+  f[1] = sc(0.5 / (rho * cm), c3);
+  f[2] = z{0, 0};
+  f[3] = z{0, 0};
+  f[4] = sc(1.0 / rho, sc(gM2 * 0.5, c3) - sc(T * 0.5 / cm2, c3));
+}
+
 __global__ void k_sponge_forcing(lns3d_params p, int ispg, const double *vm, const double *x,
                                  const double *wamp, double x0, double *spf) {
   const long P = (long)p.nx * p.ny;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= P) return;
   const int i = (int)(q % p.nx);
-  const double *vml = vm + 5 * q;
-  const double cm = sqrt(vml[4]) / p.Ma, um = vml[1];
-  const double d = 0.5 * (1.3333333333333333333 + (p.gamma - 1.0) / p.Pr) / p.Re;
-  const double kk = p.omega / (cm + um);
-  const double a = p.omega * p.omega * d / ((cm + um) * (cm + um) * (cm + um));
-  const double xl = x[q];
-  double sn, cs;
-  sincos(kk * xl, &sn, &cs);
-  const double e = exp(-a * (xl - x0));
-  z c3{e * cs, e * sn};
-  if (ispg == 3) c3 = sc(wamp[i], c3);
-  const double rho = vml[0], T = vml[4], cm2 = cm * cm, gM2 = p.gamma * p.Ma * p.Ma;
   z f[5];
-  f[0] = sc(0.5 / cm2, c3);
-  f[1] = sc(0.5 / (rho * cm), c3);
-  f[2] = z{0, 0};
-  f[3] = z{0, 0};
-  f[4] = sc(1.0 / rho, sc(gM2 * 0.5, c3) - sc(T * 0.5 / cm2, c3));
+  acoustic_wave(p, vm + 5 * q, x[q], x0, ispg == 3 ? wamp[i] : 1.0, f);
   for (int m = 0; m < 5; m++) {
     spf[2 * (m * P + q)] = f[m].re;
     spf[2 * (m * P + q) + 1] = f[m].im;
   }
+}
+
+// the top = 1 wave at the top row, src/rhsbc3d.f90:131-150
+__global__ void k_top_forcing(lns3d_params p, const double *vm, const double *xtop, const double *wamp,
+                              double x0, double *ftop) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.nx) return;
+  const long q = i + (long)p.nx * (p.ny - 1);
+  z f[5];
+  acoustic_wave(p, vm + 5 * q, xtop[i], x0, wamp ? wamp[i] : 1.0, f);
+  for (int m = 0; m < 5; m++) {
+    ftop[2 * (m * p.nx + i)] = f[m].re;
+    ftop[2 * (m * p.nx + i) + 1] = f[m].im;
+  }
+}
+
+int launch_top_forcing(const lns3d_params &p, const Coeffs &c, const double *xtop, const double *wamp,
+                       double x0, hipStream_t s) {
+  hipLaunchKernelGGL(k_top_forcing, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (const double *)c.vm,
+                     xtop, wamp, x0, c.ftop);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
 }
 
 int launch_sponge_forcing(const lns3d_params &p, int ispg, const Coeffs &c, const double *x,
@@ -784,7 +808,15 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
                 sc(gc3, ld(v + 5 * (q + 2L * nx) + 4)) + sc(gc4, ld(v + 5 * (q + 3L * nx) + 4)) +
                 sc(gc5, ld(v + 5 * (q + 4L * nx) + 4));
         }
-        if (j == ny - 1) a = z{0, 0};
+        if (j == ny - 1) {
+          if (p.top == 1) {        // src/rhsbc3d.f90:152-157: -(v - wave)
+            const z f{c.ftop[2 * (m * nx + i)], c.ftop[2 * (m * nx + i) + 1]};
+            const z d = vq(m) - f;
+            a = z{-d.re, -d.im};
+          } else {
+            a = z{0, 0};
+          }
+        }
       }
       if (p.xper) {
         if (i == nx - 1) a = z{0, 0};
@@ -818,7 +850,7 @@ __global__ void k_axpy(double2 *x, const double2 *d, long n) {
   x[t] = make_double2(a.x + b.x, a.y + b.y);
 }
 
-__global__ void k_itrbc_wall_top(lns3d_params p, double2 *v) {
+__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, double2 *v) {
   using namespace st;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = p.nx, ny = p.ny;
@@ -846,7 +878,10 @@ __global__ void k_itrbc_wall_top(lns3d_params p, double2 *v) {
     z s = lin(4, gc2, gc3, gc4, gc5);
     V(4, 0) = make_double2(-s.re / gc1, -s.im / gc1);
   }
-  for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(0, 0);
+  if (p.top == 1)        // src/itrbc3d.f90:187-191: the incoming wave
+    for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(ftop[2 * (d * nx + i)], ftop[2 * (d * nx + i) + 1]);
+  else
+    for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(0, 0);
 }
 
 __global__ void k_itrbc_lr(lns3d_params p, double2 *v) {
@@ -864,8 +899,8 @@ __global__ void k_itrbc_lr(lns3d_params p, double2 *v) {
   }
 }
 
-int launch_itrbc(const lns3d_params &p, void *v, hipStream_t s) {
-  hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (double2 *)v);
+int launch_itrbc(const lns3d_params &p, const double *ftop, void *v, hipStream_t s) {
+  hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, ftop, (double2 *)v);
   hipLaunchKernelGGL(k_itrbc_lr, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (double2 *)v);
   LNS3D_CHECK(hipGetLastError());
   return 0;
@@ -890,10 +925,10 @@ int launch_axpy(void *x, const void *d, long n, hipStream_t s) {
   return 0;
 }
 
-int launch_update_bc(const lns3d_params &p, void *v, const void *dv, hipStream_t s) {
+int launch_update_bc(const lns3d_params &p, const double *ftop, void *v, const void *dv, hipStream_t s) {
   int rc = launch_axpy(v, dv, 5L * p.nx * p.ny, s);
   if (rc) return rc;
-  return launch_itrbc(p, v, s);
+  return launch_itrbc(p, ftop, v, s);
 }
 
 // ==========================================================================

@@ -29,6 +29,9 @@ struct Coeffs {
   double *sp2;                  // [P] inflow sponge spg2 on (v - forcing) (ispg 2, 3)
   double *spf;                  // [5][P] complex inflow forcing, (re, im) pairs (ispg 2, 3)
   double *spl;                  // [P] eta-factor diagonal: spg (ispg 1) or spg + spg2
+  // top = 1 (src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195): the incoming
+  // wave on the top boundary, [5][nx] complex as (re, im) pairs
+  double *ftop;
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);
@@ -37,6 +40,10 @@ int launch_timestep(const lns3d_params &p, Coeffs &c, double *work, hipStream_t 
 // the physical x (P, device) and wamp (nx, device; ispg 3)
 int launch_sponge_forcing(const lns3d_params &p, int ispg, const Coeffs &c, const double *x,
                           const double *wamp, double x0, hipStream_t s);
+// the top = 1 wave into c.ftop from the mean flow c.vm, the top row of the
+// physical x (nx, device) and wamp (nx, device; null for 1)
+int launch_top_forcing(const lns3d_params &p, const Coeffs &c, const double *xtop, const double *wamp,
+                       double x0, hipStream_t s);
 // assemble the LHS of direction dir (1: xi, 2: eta) into the line-interleaved
 // working copy of plan (lines: y index for dir 1, x index for dir 2)
 int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &plan, hipStream_t s);
@@ -45,8 +52,9 @@ int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *m
 // r = -dtl * L(v) with the boundary rows of rhsBC3D; work holds 5P complex
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
                void *work, int with_rbe, hipStream_t s);
-int launch_update_bc(const lns3d_params &p, void *v, const void *dv, hipStream_t s);
-int launch_itrbc(const lns3d_params &p, void *v, hipStream_t s);
+// ftop: the top = 1 wave (Coeffs::ftop), null for top = 0
+int launch_update_bc(const lns3d_params &p, const double *ftop, void *v, const void *dv, hipStream_t s);
+int launch_itrbc(const lns3d_params &p, const double *ftop, void *v, hipStream_t s);
 // y = M x with M = Lx * Ly the approximate-factorization operator (FP64,
 // blocks regenerated from the coefficients); tmp holds 5P complex
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,

@@ -60,6 +60,7 @@ SIGNATURES = {
     "lns3d_step_refined": ([VP, VP, VP, CD, CI, ctypes.POINTER(CI), ctypes.POINTER(CD)], CI),
     "lns3d_step_fp32": ([VP, VP, VP], CI),
     "lns3d_set_sponge": ([VP, CI, VP, VP, VP, VP, CD], CI),
+    "lns3d_set_top_forcing": ([VP, VP, VP, CD], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -175,6 +176,14 @@ class Step:
         met = np.asfortranarray(metric, dtype=np.float64)
         v = np.asfortranarray(vm, dtype=np.float64)
         check(lib().lns3d_setup(self.h, met.ctypes.data, v.ctypes.data), "lns3d_setup")
+
+    def set_top_forcing(self, x, wamp=None, x0=0.0):
+        """the top = 1 incoming wave (lns3d_set_top_forcing); x (nx,ny), wamp (nx)"""
+        import numpy as np
+        xa = np.asfortranarray(x, dtype=np.float64)
+        wa = None if wamp is None else np.ascontiguousarray(wamp, dtype=np.float64)
+        check(lib().lns3d_set_top_forcing(self.h, xa.ctypes.data, None if wa is None else wa.ctypes.data,
+                                          float(x0)), "lns3d_set_top_forcing")
 
     def set_sponge(self, ispg, spg=None, spg2=None, x=None, wamp=None, x0=0.0):
         """sponge profiles (nx,ny), physical x (nx,ny) and wamp (nx); after

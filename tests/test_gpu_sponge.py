@@ -5,7 +5,9 @@ sponge module state (tests/golden/ref_sponge_<ispg>.npz, written by
 tests/golden/gen_golden.py from the compiled reference), ispg = 1, 2, 3, 5.
 Tolerance 1e-12 relative, max norm.  The refined step (complex-float factors,
 whose residual operator carries the sponge too) reaches the complex-double
-increment; ispg = 4 is rejected."""
+increment; ispg = 4 is rejected.  top = 1 (lns3d_set_top_forcing: the same
+incoming wave on the top boundary, src/rhsbc3d.f90:129-161,
+src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz."""
 import os
 
 import numpy as np
@@ -88,4 +90,46 @@ def test_sponge_rejects_iterative_form():
     st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
     with pytest.raises(LNS3DError):
         st.set_sponge(4, z["spg"], z["spg2"])
+    st.close()
+
+
+def test_top1_against_reference():
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, "ref_top1.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"top=1 {p.nx}x{p.ny}", flush=True)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t, shape):
+        return t.cpu().numpy().reshape(shape, order="F")
+
+    st = Step(p)
+    st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+    with pytest.raises(LNS3DError):
+        st.factor()                          # top = 1 without its wave
+    st.set_top_forcing(z["x"], None, float(z["x0"]))
+    v0 = z["v0"]
+    vd = dev(v0)
+    st.itrbc3d(vd)
+    assert rel(host(vd, v0.shape), z["itrbc"]) < TOL
+    vd = dev(v0)
+    rd = torch.zeros_like(vd)
+    st.lrhs3d(vd, rd)
+    assert rel(host(rd, v0.shape), z["lrhs"]) < TOL
+    for d in (1, 2):
+        m_dev = torch.zeros(125 * p.nx * p.ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        assert rel(host(m_dev, z[f"lhs{d}"].shape), z[f"lhs{d}"]) < TOL, d
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    st.step(vd, dvd)
+    assert rel(host(dvd, v0.shape), z["dv1"]) < TOL
     st.close()
