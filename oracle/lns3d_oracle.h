@@ -106,6 +106,15 @@ int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
              zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
              zdouble *pery, zdouble *per2y, zdouble *v, zdouble *dv,
              zdouble *work);
+/* src/impdrv3d.f90:473-501 rtb3D, and the impl = 2 step after the first
+ * (two-step backward residual with the field v2old of the previous step's
+ * start; factors formed with alfa = 2/3) */
+void orc_rtb3d(const orc_params *p, zdouble *r, const zdouble *v, const zdouble *vold,
+               const zdouble *v2old, const double *dtl);
+int orc_step_bdf2(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                  zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+                  zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
+                  zdouble *dv, zdouble *work);
 
 #ifdef __cplusplus
 }

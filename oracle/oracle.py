@@ -193,6 +193,19 @@ class Oracle:
         assert rc == 0
         return dv
 
+    def step_bdf2(self, p, c, dtl, f, v, v2old):
+        """an impl = 2 step after the first (v2old: the field at the start of
+        the previous step); v updated in place; returns dv"""
+        dv = np.zeros_like(v, order="F")
+        work = np.zeros(v.shape + (7,), complex, order="F")
+        ps, cs = self.params(p), self._coeffs(c)
+        rc = self.lib.orc_step_bdf2(ctypes.byref(ps), ctypes.byref(cs), _ptr(dtl),
+                                    *(_ptr(f[k]) for k in ("matx", "maty", "perx", "per2x",
+                                                            "pery", "per2y")),
+                                    _ptr(v), _ptr(np.asfortranarray(v2old)), _ptr(dv), _ptr(work))
+        assert rc == 0
+        return dv
+
 
 class Reference:
     """The reference's own routines, compiled by build_ref.sh (this container
@@ -208,9 +221,9 @@ class Reference:
     def available(path=None):
         return os.path.exists(path or os.path.join(HERE, "_ref", "libref.so"))
 
-    def setup(self, p, x, y, met, vm, usecalcd=0, ispg=0, x0=0.0):
+    def setup(self, p, x, y, met, vm, usecalcd=0, ispg=0, x0=0.0, impl=1):
         ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, 0, 0, p.wall,
-                       p.wallt, p.top, p.left, p.right, p.linear, 1, p.loctime, p.mattyp,
+                       p.wallt, p.top, p.left, p.right, p.linear, impl, p.loctime, p.mattyp,
                        ispg, 1, usecalcd], dtype=np.int32)
         rp = np.array([p.Ma, p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax, p.dtmax,
                        0.0, x0])

@@ -14,6 +14,9 @@ module refdrv_state
   complex, allocatable :: perx(:,:,:,:,:), per2x(:,:,:,:,:)
   complex, allocatable :: pery(:,:,:,:,:), per2y(:,:,:,:,:)
   real, allocatable    :: dtlh(:,:)
+  ! impl = 2: the field at the start of the previous step, and the steps taken
+  complex, allocatable :: vhist(:,:,:), v2old(:,:,:)
+  integer :: lstep_h = 0
 end module refdrv_state
 
 !-----------------------------------------------------------------------------
@@ -277,8 +280,23 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   integer(c_int), value :: first
   complex, allocatable :: vold(:,:,:), r(:,:,:)
   integer :: i, j, idof, jdof
+  logical :: form
 
   allocate(vold(ndof,nx,ny), r(ndof,nx,ny))
+  if (first /= 0) lstep_h = 0
+  ! src/impdrv3d.f90:134-153: the time levels of impl = 2
+  if (impl == 2) then
+    if (.not. allocated(vhist)) allocate(vhist(ndof,nx,ny), v2old(ndof,nx,ny))
+    if (size(vhist,2) /= nx .or. size(vhist,3) /= ny) then
+      deallocate(vhist, v2old)
+      allocate(vhist(ndof,nx,ny), v2old(ndof,nx,ny))
+    end if
+    if (lstep_h > 0) v2old = vhist
+    vhist = v
+  end if
+  ! src/impdrv3d.f90:229-231: factors formed at the first step, and again
+  ! at the second for impl = 2 (alfa 2/3)
+  form = first /= 0 .or. (impl == 2 .and. lstep_h == 1)
   if (first /= 0) then
     if (allocated(matx)) deallocate(matx, maty, lmatx, lr)
     allocate(matx(5,ndof,ndof,nx,ny), maty(5,ndof,ndof,nx,ny))
@@ -292,12 +310,23 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   istep = 1; iter = 1
   vold = v
   call itrBC3D(v, vm)
-  call lrhs3D(v, r, vm, x, y)
-  call rbe3D(r, v, vold, dtlh)
-  alfa = one
+  ! src/impdrv3d.f90:179-199
+  if (impl == 3) then
+    call lrhs3D(pt5*(v+vold), r, vm, x, y)
+    call rbe3D(r, v, vold, dtlh)
+    alfa = pt5
+  else if (impl == 2 .and. lstep_h > 0) then
+    call lrhs3D(v, r, vm, x, y)
+    call rtb3D(r, v, vold, v2old, dtlh)
+    alfa = pt66
+  else
+    call lrhs3D(v, r, vm, x, y)
+    call rbe3D(r, v, vold, dtlh)
+    alfa = one
+  end if
   call rhsBC3D(r, v, vm)
 
-  if (first /= 0) then
+  if (form) then
     call lhs1f3D(matx, Ah, Dh, Dhi, Vh11, ABhi, dtlh, v)
     if (xper) then
       call cpenta2p(nx, ny, ndof, matx, r, perx, per2x, 0)
@@ -330,7 +359,7 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
     end do
   end if
 
-  if (first /= 0) then
+  if (form) then
     call lhs2f3D(maty, Bh, Dh, Dhi, Vh22, ABhi, dtlh, v)
     if (yper) then
       call cpenta1p(nx, ny, ndof, maty, r, pery, per2y, 0)
@@ -347,6 +376,7 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   v = v + r
   dv = r
   call itrBC3D(v, vm)
+  lstep_h = lstep_h + 1
   deallocate(vold, r)
 end subroutine ref_step
 

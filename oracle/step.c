@@ -906,16 +906,32 @@ int orc_factor(const orc_params *p, const orc_coeffs *c, const double *dtl,
   return orc_cpenta_bc(1, nx, ny, maty, NULL, 0);
 }
 
-int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
-             zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
-             zdouble *pery, zdouble *per2y, zdouble *v, zdouble *dv, zdouble *work) {
+/* src/impdrv3d.f90:473-501 (rtb3D): r = 4/3 vold - 1/3 v2old - v - 2/3 dtl r */
+void orc_rtb3d(const orc_params *p, zdouble *r, const zdouble *v, const zdouble *vold,
+               const zdouble *v2old, const double *dtl) {
+  const double onept33 = 1.3333333333333333333, pt33 = 3.3333333333333333333e-1,
+               pt66 = 6.6666666666666666666e-1;
+  long P = (long)p->nx * p->ny;
+  for (long q = 0; q < P; q++)
+    for (int m = 0; m < 5; m++)
+      r[5 * q + m] = onept33 * vold[5 * q + m] - pt33 * v2old[5 * q + m] - v[5 * q + m] -
+                     pt66 * dtl[q] * r[5 * q + m];
+}
+
+/* one step; v2old NULL: backward Euler (rbe3D), else the two-step backward
+ * residual of impl = 2 (rtb3D, src/impdrv3d.f90:187-190) */
+static int step_core(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                     zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+                     zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
+                     zdouble *dv, zdouble *work) {
   int nx = p->nx, ny = p->ny;
   long P = (long)nx * ny;
   zdouble *vold = work, *r = work + 5 * P, *g = work + 10 * P;
   memcpy(vold, v, sizeof(zdouble) * 5 * P);
   if (orc_itrbc3d(p, v)) return -1;
   if (orc_lrhs3d(p, c, v, r, g)) return -1;
-  orc_rbe3d(p, r, v, vold, dtl);
+  if (v2old) orc_rtb3d(p, r, v, vold, v2old, dtl);
+  else orc_rbe3d(p, r, v, vold, dtl);
   if (orc_rhsbc3d(p, r, v)) return -1;
   if (p->xper) {
     if (orc_cpenta_p(2, ny, nx, matx, r, perx, per2x, 1, p->periodic_exact)) return -1;
@@ -934,4 +950,17 @@ int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
   } else if (orc_cpenta_bc(1, nx, ny, maty, r, 1)) return -1;
   for (long e = 0; e < 5 * P; e++) { v[e] += r[e]; dv[e] = r[e]; }
   return orc_itrbc3d(p, v);
+}
+
+int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
+             zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+             zdouble *pery, zdouble *per2y, zdouble *v, zdouble *dv, zdouble *work) {
+  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, NULL, dv, work);
+}
+
+int orc_step_bdf2(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                  zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+                  zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
+                  zdouble *dv, zdouble *work) {
+  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, v2old, dv, work);
 }

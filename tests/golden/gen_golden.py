@@ -22,12 +22,16 @@ reference's outputs:
                         src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195;
                         wamp = 1 as itrBC3D allocates it): itrBC3D, lrhs3D +
                         rhsBC3D, both assembled factors, one implicit step
+  ref_impl<k>.npz       time schemes impl = 2 (two-step backward, src/impdrv3d.f90
+                        :134-153, 187-190, rtb3D; factors again at the second
+                        step with alfa = 2/3) and impl = 3 (alfa = 1/2):
+                        three steps
   ref_sponge_<ispg>.npz sponge cases ispg = 1, 2, 3, 5: profiles shaped as in
                         src/sponge.f90:20 (init_sponge), inflow amplitude
                         wamp and x0; lrhs3D, both assembled LHS factors and
                         one implicit step with the sponge
 
-    python tests/golden/gen_golden.py sponge     (the sponge and top = 1 cases only)
+    python tests/golden/gen_golden.py sponge     (the sponge, top = 1 and impl cases only)
 """
 from __future__ import annotations
 
@@ -209,6 +213,24 @@ def top1_case(R):
     np.savez_compressed(os.path.join(HERE, "ref_top1.npz"), **out)
 
 
+def impl_case(R, impl):
+    p = Params(nx=13, ny=9, kz=0.4)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    v0 = make_disturbance(p, x, seed=29, amp=1.0)
+    R.setup(p, x, y, met, vm, impl=impl)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), impl=np.array(impl), met=met, vm=vm, v0=v0)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    for k in range(1, 4):
+        out[f"dv{k}"] = R.step(v, k == 1)
+    out["v3"] = v.copy()
+    np.savez_compressed(os.path.join(HERE, f"ref_impl{impl}.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -217,6 +239,8 @@ def main():
         for ispg in SPONGE_CASES:
             sponge_case(R, ispg)
         top1_case(R)
+        for impl in (2, 3):
+            impl_case(R, impl)
         R.lib.ref_flush()
         return
     for name, kw in STEP_CASES.items():
@@ -226,6 +250,8 @@ def main():
     for ispg in SPONGE_CASES:
         sponge_case(R, ispg)
     top1_case(R)
+    for impl in (2, 3):
+        impl_case(R, impl)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -192,3 +192,37 @@ def test_top1(orc):
         assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
     finally:
         orc.set_top()
+
+
+@pytest.mark.parametrize("impl", [2, 3])
+def test_time_schemes(orc, impl):
+    """impl = 2 (two-step backward: backward Euler first, then rtb3D with the
+    factors formed again at alfa = 2/3, src/impdrv3d.f90:134-153, 179-199,
+    229-231) and impl = 3 (alfa = 1/2) against three of the reference's steps"""
+    import dataclasses
+    z = load(f"ref_impl{impl}.npz")
+    p = params_of(z)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    v = np.asfortranarray(z["itrbc"].copy())
+    if impl == 3:
+        p3 = dataclasses.replace(p, alfa=0.5)
+        f = orc.factor(p3, c, dtl)
+        for k in (1, 2, 3):
+            assert rel(orc.step(p3, c, dtl, f, v), z[f"dv{k}"]) < 1e-12, k
+    else:
+        f = orc.factor(p, c, dtl)
+        v2old = v.copy(order="F")
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+        p2 = dataclasses.replace(p, alfa=6.6666666666666666666e-1)
+        f2 = orc.factor(p2, c, dtl)
+        for k in (2, 3):
+            vstart = v.copy(order="F")
+            assert rel(orc.step_bdf2(p2, c, dtl, f2, v, v2old), z[f"dv{k}"]) < 1e-12, k
+            v2old = vstart
+        # the backward Euler step misses the second increment
+        vb = np.asfortranarray(z["itrbc"].copy())
+        orc.step(p, c, dtl, f, vb)
+        assert rel(orc.step(p, c, dtl, f, vb), z["dv2"]) > 1e-6
+    assert rel(v, z["v3"]) < 1e-12
