@@ -108,7 +108,9 @@ int lns3d_ctx_destroy(lns3d_ctx *c);
 int lns3d_ctx_set_stream(lns3d_ctx *c, void *stream);
 /* new physics (beta, omega, Ma, Re, ... and boundary selectors) for an
  * existing context of the same grid size, periodicity and precision; form
- * the coefficients and factors again with lns3d_setup / lns3d_factor */
+ * the coefficients and factors again with lns3d_setup / lns3d_factor.  The
+ * time scheme restarts (its alfa applies); the sponge and the top wave are
+ * cleared (set them again). */
 int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p);
 int lns3d_ctx_synchronize(lns3d_ctx *c);
 /* metric (nx,ny,10: m1,m2,n1,n2,m11,m12,m22,n11,n12,n22; src/gengrid.f90:76-80)
@@ -135,6 +137,18 @@ int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const double *sp
  * NULL for 1) and x0.  After lns3d_setup; lns3d_factor refuses top = 1
  * without it. */
 int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double *wamp, double x0);
+/* Time scheme, the deck's impl (src/impdrv3d.f90:179-199): 1 backward Euler
+ * (default), 2 two-step backward (rtb3D, src/impdrv3d.f90:473-501: the first
+ * step backward Euler, then the second-order residual, the factors formed
+ * again at the second step with alfa = 2/3), 3 alfa = 1/2.  Starts a new
+ * run; lns3d_factor afterwards.  With impl = 2 lns3d_step and lns3d_advance
+ * keep the field of the previous step on the device; the complex-float and
+ * refined steps and graphs return LNS3D_EUNSUPPORTED. */
+int lns3d_set_time_scheme(lns3d_ctx *c, int impl);
+/* impl = 2 resumed from a restart holding vold (src/genini3d.f90:55-57):
+ * vold_dev is the field at the start of the last step taken (device), lstep
+ * the number of steps taken. */
+int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep);
 /* alternatively, take the coefficients the caller already has (host arrays
  * in the module layouts of src/local2d.f90:34-36, src/local3d.f90:14) */
 int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,

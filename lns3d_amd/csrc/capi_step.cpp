@@ -37,6 +37,9 @@ struct lns3d_ctx {
   double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
   bool coeffs_ready = false, factored = false;
   bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
+  int impl = 1;                     // time scheme (src/impdrv3d.f90:179-199)
+  long lstep = 0;                   // steps taken with this scheme
+  void *vhist = nullptr;            // impl = 2: 5P complex, v at the start of the last step
   void *vstate = nullptr;    // 5P complex: the state of the host-array driver
   bool state_fresh = false;  // uploaded, boundary conditions not yet applied
   int last_iters = 0;
@@ -108,6 +111,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->vstate) (void)hipFree(c->vstate);
   if (c->sponge_block) (void)hipFree(c->sponge_block);
   if (c->top_block) (void)hipFree(c->top_block);
+  if (c->vhist) (void)hipFree(c->vhist);
   delete c;
   return 0;
 }
@@ -126,7 +130,14 @@ extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
   c->st = make_stencil(*p);
   c->px.exact = c->py.exact = p->periodic_exact;
   c->coeffs_ready = false;
+  c->vm_ready = false;
   c->factored = false;
+  // a new run: the time scheme starts again; the sponge and the top wave,
+  // formed from the old physics, are set again by the caller
+  c->lstep = 0;
+  if (c->impl != 1) c->p.alfa = c->impl == 3 ? 0.5 : 1.0;
+  c->c.sp1 = c->c.sp2 = c->c.spf = c->c.spl = nullptr;
+  c->c.ftop = nullptr;
   return 0;
 }
 
@@ -365,13 +376,54 @@ extern "C" int lns3d_solve_dir(lns3d_ctx *c, int dir, void *r_dev) {
 // itrBC3D.  v must already satisfy the boundary conditions (call
 // lns3d_itrbc3d once before the first step, as src/impdrv3d.f90:126 does).
 // dv_dev (optional) receives the increment r.
+//
+// impl = 2 (two-step backward, src/impdrv3d.f90:134-153, 187-190): the first
+// step is backward Euler; from the second on the residual is rtb3D with the
+// field at the start of the previous step, and at the second the factors are
+// formed again with alfa = 2/3 (src/impdrv3d.f90:229-231).
 extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   if (!c || !c->factored || !v_dev) return fail(LNS3D_ENOTFACTORED, "lns3d_step: not factored");
   void *r = dv_dev ? dv_dev : c->r;
-  int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
+  const bool two = c->impl == 2 && c->lstep >= 1;
+  int rc = 0;
+  if (two && c->p.alfa != 6.6666666666666666666e-1) {
+    c->p.alfa = 6.6666666666666666666e-1;
+    rc = lns3d_factor(c);
+    if (rc) return rc;
+  }
+  rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
+  if (!rc && c->impl == 2) rc = launch_copy(c->vhist, v_dev, 5L * c->p.nx * c->p.ny, c->stream);
   if (!rc) rc = solve_xy(c, c->px, c->py, r);
   if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+  if (!rc) c->lstep++;
   return rc;
+}
+
+// Time scheme (the deck's impl, src/impdrv3d.f90:179-199): 1 backward Euler
+// (alfa 1), 2 two-step backward (alfa 1 for the first step, then 2/3), 3
+// alfa 1/2 (its residual lrhs3D(0.5 (v + vold)) equals lrhs3D(v) at the
+// start of a step, where vold = v).  Starts a new run (no earlier step);
+// factor (again) afterwards.
+extern "C" int lns3d_set_time_scheme(lns3d_ctx *c, int impl) {
+  if (!c || impl < 1 || impl > 3) return fail(LNS3D_EINVAL, "lns3d_set_time_scheme: impl must be 1, 2 or 3");
+  c->impl = impl;
+  c->lstep = 0;
+  c->p.alfa = impl == 3 ? 0.5 : 1.0;
+  c->factored = false;
+  if (impl == 2 && !c->vhist) LNS3D_CHECK(hipMalloc(&c->vhist, 5ul * c->p.nx * c->p.ny * 16));
+  return 0;
+}
+
+// impl = 2 runs resumed from a restart (src/genini3d.f90:55-57 reads vold):
+// vold_dev (device, complex (5,nx,ny)) is the field at the start of the last
+// step taken, lstep the number of steps taken
+extern "C" int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep) {
+  if (!c || c->impl != 2 || !vold_dev || lstep < 0)
+    return fail(LNS3D_EINVAL, "lns3d_set_time_history: needs impl = 2, a field and lstep >= 0");
+  int rc = launch_copy(c->vhist, vold_dev, 5L * c->p.nx * c->p.ny, c->stream);
+  if (rc) return rc;
+  c->lstep = lstep;
+  return 0;
 }
 
 // The same step with the line solves in complex float used as a
@@ -381,6 +433,8 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
 //   |res|_max <= tol |r|_max or iters reach max_iters.
 extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int max_iters,
                                   int *iters_out, double *resid_out) {
+  if (c && c->impl == 2)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_refined: impl = 2 takes lns3d_step");
   if (!c || !c->factored || !c->have32 || !v_dev)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_refined: needs precision = 1 and lns3d_factor");
   const long n = 5L * c->p.nx * c->p.ny;
@@ -416,6 +470,8 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
 // the same step, but only the complex-float solve (no refinement): the
 // pseudo-time iteration itself corrects the defect in steady marching
 extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
+  if (c && c->impl == 2)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_fp32: impl = 2 takes lns3d_step");
   if (!c || !c->factored || !c->have32 || !v_dev)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_fp32: needs precision = 1 and lns3d_factor");
   void *r = dv_dev ? dv_dev : c->r;
@@ -431,6 +487,8 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
 // for lns3d_last_residual.
 extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev, int corrections,
                                         int check) {
+  if (c && c->impl == 2)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_refined_fixed: impl = 2 takes lns3d_step");
   if (!c || !c->factored || !c->have32 || !v_dev || corrections < 0)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_refined_fixed: needs precision = 1 and lns3d_factor");
   const long n = 5L * c->p.nx * c->p.ny;
@@ -495,9 +553,12 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
                                   int mode, int corrections, lns3d_graph **out) {
   if (!ctxs || !v_dev || count < 1 || nsteps < 1 || mode < 0 || mode > 2 || !out)
     return fail(LNS3D_EINVAL, "lns3d_graph_create: bad arguments");
-  for (int i = 0; i < count; i++)
+  for (int i = 0; i < count; i++) {
     if (!ctxs[i] || !ctxs[i]->factored || !v_dev[i] || (mode > 0 && !ctxs[i]->have32))
       return fail(LNS3D_EINVAL, "lns3d_graph_create: every context must be factored (precision 1 for modes 1, 2)");
+    if (ctxs[i]->impl == 2)
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2 contexts form their factors again at the second step");
+  }
   auto *gr = new lns3d_graph;
   std::vector<hipStream_t> st(count, nullptr);
   std::vector<hipEvent_t> ev(count + 1, nullptr);

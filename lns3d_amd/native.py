@@ -61,6 +61,8 @@ SIGNATURES = {
     "lns3d_step_fp32": ([VP, VP, VP], CI),
     "lns3d_set_sponge": ([VP, CI, VP, VP, VP, VP, CD], CI),
     "lns3d_set_top_forcing": ([VP, VP, VP, CD], CI),
+    "lns3d_set_time_scheme": ([VP, CI], CI),
+    "lns3d_set_time_history": ([VP, VP, ctypes.c_long], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -176,6 +178,13 @@ class Step:
         met = np.asfortranarray(metric, dtype=np.float64)
         v = np.asfortranarray(vm, dtype=np.float64)
         check(lib().lns3d_setup(self.h, met.ctypes.data, v.ctypes.data), "lns3d_setup")
+
+    def set_time_scheme(self, impl):
+        """the deck's impl: 1 backward Euler, 2 two-step backward, 3 alfa 1/2"""
+        check(lib().lns3d_set_time_scheme(self.h, impl), "lns3d_set_time_scheme")
+
+    def set_time_history(self, vold_dev, lstep):
+        check(lib().lns3d_set_time_history(self.h, ptr(vold_dev), lstep), "lns3d_set_time_history")
 
     def set_top_forcing(self, x, wamp=None, x0=0.0):
         """the top = 1 incoming wave (lns3d_set_top_forcing); x (nx,ny), wamp (nx)"""

@@ -1,4 +1,8 @@
-"""GPU parity of the sponge (lns3d_set_sponge: src/lrhs3d.f90:222-282,
+"""GPU parity of the deck branches beyond the base step against the
+reference's own vectors (tests/golden/*.npz, written by
+tests/golden/gen_golden.py from the compiled reference).
+
+The sponge (lns3d_set_sponge: src/lrhs3d.f90:222-282,
 src/lhs2f3d.f90:207-225 and the closure rows of src/lhsbt2f3d.f90) against
 the reference's own lrhs3D, assembled factors and implicit step with its
 sponge module state (tests/golden/ref_sponge_<ispg>.npz, written by
@@ -7,7 +11,9 @@ Tolerance 1e-12 relative, max norm.  The refined step (complex-float factors,
 whose residual operator carries the sponge too) reaches the complex-double
 increment; ispg = 4 is rejected.  top = 1 (lns3d_set_top_forcing: the same
 incoming wave on the top boundary, src/rhsbc3d.f90:129-161,
-src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz."""
+src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz.
+The time schemes impl = 2 and 3 (lns3d_set_time_scheme) over three of the
+reference's steps."""
 import os
 
 import numpy as np
@@ -132,4 +138,56 @@ def test_top1_against_reference():
     dvd = torch.zeros_like(vd)
     st.step(vd, dvd)
     assert rel(host(dvd, v0.shape), z["dv1"]) < TOL
+    st.close()
+
+
+@pytest.mark.parametrize("impl", [2, 3])
+def test_time_scheme_against_reference(impl):
+    """lns3d_set_time_scheme: impl = 2 (two-step backward, the factors formed
+    again at the second step) and impl = 3 (alfa 1/2) over three of the
+    reference's steps (tests/golden/ref_impl<k>.npz), 1e-12; lns3d_advance
+    continues the same run; impl = 2 restarts from a history"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, f"ref_impl{impl}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"impl={impl} {p.nx}x{p.ny}", flush=True)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t):
+        return t.cpu().numpy().reshape(z["v0"].shape, order="F")
+
+    st = Step(p)
+    st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+    st.set_time_scheme(impl)
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    starts = []
+    for k in (1, 2, 3):
+        starts.append(vd.clone())
+        st.step(vd, dvd)
+        e = rel(host(dvd), z[f"dv{k}"])
+        print(f"  step {k}: dv rel {e:.2e}", flush=True)
+        assert e < TOL, k
+    assert rel(host(vd), z["v3"]) < TOL
+    if impl == 2:
+        with pytest.raises(LNS3DError):
+            st.step_fp32(vd, dvd)
+        # a second context resumed after two steps from the history
+        st2 = Step(p)
+        st2.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+        st2.set_time_scheme(2)
+        st2.factor()
+        v2 = starts[2].clone()
+        st2.set_time_history(starts[1], 2)
+        st2.step(v2, dvd)
+        assert rel(host(dvd), z["dv3"]) < TOL
+        st2.close()
     st.close()
