@@ -115,6 +115,13 @@ int orc_step_bdf2(const orc_params *p, const orc_coeffs *c, const double *dtl,
                   zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
                   zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
                   zdouble *dv, zdouble *work);
+/* a step of niter corrector iterations (src/impdrv3d.f90:173-358) with vold
+ * the field at its start: v2old NULL for backward Euler, else rtb3D; half:
+ * the residual at 0.5 (v + vold) (impl = 3) */
+int orc_step_iter(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                  zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+                  zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old, int niter,
+                  int half, zdouble *dv, zdouble *work);
 
 #ifdef __cplusplus
 }

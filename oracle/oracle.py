@@ -207,6 +207,22 @@ class Oracle:
         return dv
 
 
+    def step_iter(self, p, c, dtl, f, v, niter, v2old=None, half=False):
+        """a step of niter corrector iterations (impl = 2 with v2old, impl = 3
+        with half); v updated in place; returns the last increment"""
+        dv = np.zeros_like(v, order="F")
+        work = np.zeros(v.shape + (7,), complex, order="F")
+        ps, cs = self.params(p), self._coeffs(c)
+        self.lib.orc_step_iter.argtypes = [VP] * 10 + [VP, ctypes.c_int, ctypes.c_int, VP, VP]
+        rc = self.lib.orc_step_iter(ctypes.byref(ps), ctypes.byref(cs), _ptr(dtl),
+                                    *(_ptr(f[k]) for k in ("matx", "maty", "perx", "per2x",
+                                                            "pery", "per2y")),
+                                    _ptr(v), None if v2old is None else _ptr(np.asfortranarray(v2old)),
+                                    niter, 1 if half else 0, _ptr(dv), _ptr(work))
+        assert rc == 0
+        return dv
+
+
 class Reference:
     """The reference's own routines, compiled by build_ref.sh (this container
     only; absent where the reference sources are absent)."""
@@ -231,6 +247,10 @@ class Reference:
         self.lib.ref_set_grid(_ptr(x), _ptr(y), _ptr(met))
         self.lib.ref_set_mean(_ptr(vm))
         self.p = p
+
+    def set_niter(self, n):
+        """corrector iterations per step (the deck's niter)"""
+        self.lib.ref_set_niter(ctypes.c_int(n))
 
     def set_sponge(self, spg, spg2, wamp):
         """the reference's spg, spg2 (nx,ny) and wamp (nx) module arrays"""

@@ -279,7 +279,7 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   complex(c_double_complex), intent(out) :: dv(ndof,nx,ny)
   integer(c_int), value :: first
   complex, allocatable :: vold(:,:,:), r(:,:,:)
-  integer :: i, j, idof, jdof
+  integer :: i, j, idof, jdof, it
   logical :: form
 
   allocate(vold(ndof,nx,ny), r(ndof,nx,ny))
@@ -310,6 +310,10 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   istep = 1; iter = 1
   vold = v
   call itrBC3D(v, vm)
+  ! the corrector iterations, src/impdrv3d.f90:173-358 (factors at iter 1)
+  do it = 1, niter
+  iter = it
+  if (it > 1) form = .false.
   ! src/impdrv3d.f90:179-199
   if (impl == 3) then
     call lrhs3D(pt5*(v+vold), r, vm, x, y)
@@ -376,9 +380,19 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   v = v + r
   dv = r
   call itrBC3D(v, vm)
+  end do
   lstep_h = lstep_h + 1
   deallocate(vold, r)
 end subroutine ref_step
+
+! corrector iterations per step (the deck's niter, src/impdrv3d.f90:173)
+subroutine ref_set_niter(n) bind(C, name="ref_set_niter")
+  use iso_c_binding
+  use global
+  implicit none
+  integer(c_int), value :: n
+  niter = n
+end subroutine ref_set_niter
 
 ! flush the Fortran unit 6 buffer (the reference's routines write progress
 ! lines to it)

@@ -918,20 +918,12 @@ void orc_rtb3d(const orc_params *p, zdouble *r, const zdouble *v, const zdouble 
                      pt66 * dtl[q] * r[5 * q + m];
 }
 
-/* one step; v2old NULL: backward Euler (rbe3D), else the two-step backward
- * residual of impl = 2 (rtb3D, src/impdrv3d.f90:187-190) */
-static int step_core(const orc_params *p, const orc_coeffs *c, const double *dtl,
-                     zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
-                     zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
-                     zdouble *dv, zdouble *work) {
+/* rhsBC3D rows, the xi and eta line solves of r, v += r, itrBC3D */
+static int solve_update(const orc_params *p, zdouble *matx, zdouble *maty, zdouble *perx,
+                        zdouble *per2x, zdouble *pery, zdouble *per2y, zdouble *v, zdouble *r,
+                        zdouble *dv) {
   int nx = p->nx, ny = p->ny;
   long P = (long)nx * ny;
-  zdouble *vold = work, *r = work + 5 * P, *g = work + 10 * P;
-  memcpy(vold, v, sizeof(zdouble) * 5 * P);
-  if (orc_itrbc3d(p, v)) return -1;
-  if (orc_lrhs3d(p, c, v, r, g)) return -1;
-  if (v2old) orc_rtb3d(p, r, v, vold, v2old, dtl);
-  else orc_rbe3d(p, r, v, vold, dtl);
   if (orc_rhsbc3d(p, r, v)) return -1;
   if (p->xper) {
     if (orc_cpenta_p(2, ny, nx, matx, r, perx, per2x, 1, p->periodic_exact)) return -1;
@@ -952,15 +944,54 @@ static int step_core(const orc_params *p, const orc_coeffs *c, const double *dtl
   return orc_itrbc3d(p, v);
 }
 
+/* one step of niter corrector iterations (src/impdrv3d.f90:173-358; vold is
+ * the field at the start of the step); v2old NULL: backward Euler (rbe3D),
+ * else the two-step backward residual of impl = 2 (rtb3D,
+ * src/impdrv3d.f90:187-190); half: the residual at 0.5 (v + vold) (impl = 3) */
+static int step_core(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                     zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+                     zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
+                     int niter, int half, zdouble *dv, zdouble *work) {
+  int nx = p->nx, ny = p->ny;
+  long P = (long)nx * ny;
+  zdouble *vold = work, *r = work + 5 * P, *g = work + 10 * P;
+  zdouble *w = half ? (zdouble *)malloc(sizeof(zdouble) * 5 * P) : NULL;
+  if (half && !w) return -1;
+  memcpy(vold, v, sizeof(zdouble) * 5 * P);
+  int rc = orc_itrbc3d(p, v);
+  for (int it = 0; it < niter && !rc; it++) {
+    if (half) {
+      for (long e = 0; e < 5 * P; e++) w[e] = 0.5 * (v[e] + vold[e]);
+      rc = orc_lrhs3d(p, c, w, r, g);
+    } else {
+      rc = orc_lrhs3d(p, c, v, r, g);
+    }
+    if (rc) break;
+    if (v2old) orc_rtb3d(p, r, v, vold, v2old, dtl);
+    else orc_rbe3d(p, r, v, vold, dtl);
+    rc = solve_update(p, matx, maty, perx, per2x, pery, per2y, v, r, dv);
+  }
+  free(w);
+  return rc;
+}
+
 int orc_step(const orc_params *p, const orc_coeffs *c, const double *dtl,
              zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
              zdouble *pery, zdouble *per2y, zdouble *v, zdouble *dv, zdouble *work) {
-  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, NULL, dv, work);
+  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, NULL, 1, 0, dv, work);
 }
 
 int orc_step_bdf2(const orc_params *p, const orc_coeffs *c, const double *dtl,
                   zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
                   zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
                   zdouble *dv, zdouble *work) {
-  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, v2old, dv, work);
+  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, v2old, 1, 0, dv, work);
+}
+
+int orc_step_iter(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                  zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
+                  zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old, int niter,
+                  int half, zdouble *dv, zdouble *work) {
+  if (niter < 1) return -1;
+  return step_core(p, c, dtl, matx, maty, perx, per2x, pery, per2y, v, v2old, niter, half, dv, work);
 }

@@ -26,6 +26,8 @@ reference's outputs:
                         :134-153, 187-190, rtb3D; factors again at the second
                         step with alfa = 2/3) and impl = 3 (alfa = 1/2):
                         three steps
+  ref_niter<k>.npz      corrector iterations (niter = 3 with impl = 1, niter =
+                        2 with impl = 2, src/impdrv3d.f90:173-358): three steps
   ref_sponge_<ispg>.npz sponge cases ispg = 1, 2, 3, 5: profiles shaped as in
                         src/sponge.f90:20 (init_sponge), inflow amplitude
                         wamp and x0; lrhs3D, both assembled LHS factors and
@@ -213,22 +215,25 @@ def top1_case(R):
     np.savez_compressed(os.path.join(HERE, "ref_top1.npz"), **out)
 
 
-def impl_case(R, impl):
+def impl_case(R, impl, niter=1, name=None):
     p = Params(nx=13, ny=9, kz=0.4)
     x, y, met = make_grid(p)
     vm = make_mean(p, y)
     v0 = make_disturbance(p, x, seed=29, amp=1.0)
     R.setup(p, x, y, met, vm, impl=impl)
+    R.set_niter(niter)
     R.genmtrx()
     R.set_time_step()
-    out = dict(params=_params_json(p), impl=np.array(impl), met=met, vm=vm, v0=v0)
+    out = dict(params=_params_json(p), impl=np.array(impl), niter=np.array(niter), met=met, vm=vm,
+               v0=v0)
     v = np.asfortranarray(v0.copy())
     R.itrbc3d(v)
     out["itrbc"] = v.copy()
     for k in range(1, 4):
         out[f"dv{k}"] = R.step(v, k == 1)
     out["v3"] = v.copy()
-    np.savez_compressed(os.path.join(HERE, f"ref_impl{impl}.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, name or f"ref_impl{impl}.npz"), **out)
+    R.set_niter(1)
 
 
 def main():
@@ -241,6 +246,8 @@ def main():
         top1_case(R)
         for impl in (2, 3):
             impl_case(R, impl)
+        impl_case(R, 1, 3, "ref_niter1.npz")
+        impl_case(R, 2, 2, "ref_niter2.npz")
         R.lib.ref_flush()
         return
     for name, kw in STEP_CASES.items():
@@ -252,6 +259,8 @@ def main():
     top1_case(R)
     for impl in (2, 3):
         impl_case(R, impl)
+    impl_case(R, 1, 3, "ref_niter1.npz")
+    impl_case(R, 2, 2, "ref_niter2.npz")
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -226,3 +226,38 @@ def test_time_schemes(orc, impl):
         orc.step(p, c, dtl, f, vb)
         assert rel(orc.step(p, c, dtl, f, vb), z["dv2"]) > 1e-6
     assert rel(v, z["v3"]) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["niter1", "niter2"])
+def test_corrector_iterations(orc, case):
+    """niter corrector iterations per step (src/impdrv3d.f90:173-358; factors
+    from the first iteration): niter = 3 with impl = 1 and niter = 2 with
+    impl = 2, three steps.  The last increments of a converging corrector are
+    small, so they are compared relative to the field; the fields at 1e-12."""
+    import dataclasses
+    z = load(f"ref_{case}.npz")
+    p = params_of(z)
+    impl, niter = int(z["impl"]), int(z["niter"])
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    v = np.asfortranarray(z["itrbc"].copy())
+    scale = np.abs(v).max()
+    f = orc.factor(p, c, dtl)
+    v2old = None
+    for k in (1, 2, 3):
+        vstart = v.copy(order="F")
+        if impl == 2 and k == 2:
+            p = dataclasses.replace(p, alfa=6.6666666666666666666e-1)
+            f = orc.factor(p, c, dtl)
+        dv = orc.step_iter(p, c, dtl, f, v, niter, v2old if (impl == 2 and k > 1) else None)
+        assert np.abs(dv - z[f"dv{k}"]).max() < 1e-12 * scale, k
+        v2old = vstart
+    assert rel(v, z["v3"]) < 1e-12
+    # one iteration per step ends elsewhere
+    p1 = params_of(z)
+    vb = np.asfortranarray(z["itrbc"].copy())
+    fb = orc.factor(p1, c, dtl)
+    for k in (1, 2, 3):
+        orc.step(p1, c, dtl, fb, vb)
+    assert rel(vb, z["v3"]) > 1e-8
