@@ -149,6 +149,11 @@ int lns3d_set_time_scheme(lns3d_ctx *c, int impl);
  * vold_dev is the field at the start of the last step taken (device), lstep
  * the number of steps taken. */
 int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep);
+/* corrector iterations per step, the deck's niter (src/impdrv3d.f90:173-358):
+ * the residual with vold held at the start of the step, solved with the
+ * factors of the first iteration, niter times.  lns3d_step only (impl 1 or
+ * 2); dv_dev receives the last iteration's increment. */
+int lns3d_set_corrector_iterations(lns3d_ctx *c, int niter);
 /* alternatively, take the coefficients the caller already has (host arrays
  * in the module layouts of src/local2d.f90:34-36, src/local3d.f90:14) */
 int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,

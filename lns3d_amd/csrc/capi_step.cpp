@@ -40,6 +40,8 @@ struct lns3d_ctx {
   int impl = 1;                     // time scheme (src/impdrv3d.f90:179-199)
   long lstep = 0;                   // steps taken with this scheme
   void *vhist = nullptr;            // impl = 2: 5P complex, v at the start of the last step
+  int niter = 1;                    // corrector iterations per step (src/impdrv3d.f90:173)
+  void *vstart = nullptr;           // niter > 1: 5P complex, v at the start of this step
   void *vstate = nullptr;    // 5P complex: the state of the host-array driver
   bool state_fresh = false;  // uploaded, boundary conditions not yet applied
   int last_iters = 0;
@@ -112,6 +114,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->sponge_block) (void)hipFree(c->sponge_block);
   if (c->top_block) (void)hipFree(c->top_block);
   if (c->vhist) (void)hipFree(c->vhist);
+  if (c->vstart) (void)hipFree(c->vstart);
   delete c;
   return 0;
 }
@@ -383,7 +386,10 @@ extern "C" int lns3d_solve_dir(lns3d_ctx *c, int dir, void *r_dev) {
 // formed again with alfa = 2/3 (src/impdrv3d.f90:229-231).
 extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   if (!c || !c->factored || !v_dev) return fail(LNS3D_ENOTFACTORED, "lns3d_step: not factored");
+  if (c->impl == 3 && c->niter > 1)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step: corrector iterations with impl = 3 are not implemented");
   void *r = dv_dev ? dv_dev : c->r;
+  const long n = 5L * c->p.nx * c->p.ny;
   const bool two = c->impl == 2 && c->lstep >= 1;
   int rc = 0;
   if (two && c->p.alfa != 6.6666666666666666666e-1) {
@@ -391,12 +397,33 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     rc = lns3d_factor(c);
     if (rc) return rc;
   }
-  rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
-  if (!rc && c->impl == 2) rc = launch_copy(c->vhist, v_dev, 5L * c->p.nx * c->p.ny, c->stream);
-  if (!rc) rc = solve_xy(c, c->px, c->py, r);
-  if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+  if (c->niter == 1) {
+    rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
+    if (!rc && c->impl == 2) rc = launch_copy(c->vhist, v_dev, n, c->stream);
+    if (!rc) rc = solve_xy(c, c->px, c->py, r);
+    if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+  } else {
+    // corrector iterations (src/impdrv3d.f90:173-358): vold stays the field at
+    // the start of the step; the factors are those of the first iteration
+    rc = launch_copy(c->vstart, v_dev, n, c->stream);
+    for (int it = 0; it < c->niter && !rc; it++) {
+      const int mode = it == 0 ? (two ? 2 : 1) : (two ? 4 : 3);
+      rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, mode, c->stream, c->vhist, c->vstart);
+      if (!rc) rc = solve_xy(c, c->px, c->py, r);
+      if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+    }
+    if (!rc && c->impl == 2) std::swap(c->vhist, c->vstart);   // this step's start is the next v2old
+  }
   if (!rc) c->lstep++;
   return rc;
+}
+
+// corrector iterations per step, the deck's niter (src/impdrv3d.f90:173)
+extern "C" int lns3d_set_corrector_iterations(lns3d_ctx *c, int niter) {
+  if (!c || niter < 1) return fail(LNS3D_EINVAL, "lns3d_set_corrector_iterations: niter must be >= 1");
+  c->niter = niter;
+  if (niter > 1 && !c->vstart) LNS3D_CHECK(hipMalloc(&c->vstart, 5ul * c->p.nx * c->p.ny * 16));
+  return 0;
 }
 
 // Time scheme (the deck's impl, src/impdrv3d.f90:179-199): 1 backward Euler
@@ -433,8 +460,8 @@ extern "C" int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long l
 //   |res|_max <= tol |r|_max or iters reach max_iters.
 extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int max_iters,
                                   int *iters_out, double *resid_out) {
-  if (c && c->impl == 2)
-    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_refined: impl = 2 takes lns3d_step");
+  if (c && (c->impl == 2 || c->niter > 1))
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_refined: impl = 2 and corrector iterations take lns3d_step");
   if (!c || !c->factored || !c->have32 || !v_dev)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_refined: needs precision = 1 and lns3d_factor");
   const long n = 5L * c->p.nx * c->p.ny;
@@ -470,8 +497,8 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
 // the same step, but only the complex-float solve (no refinement): the
 // pseudo-time iteration itself corrects the defect in steady marching
 extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
-  if (c && c->impl == 2)
-    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_fp32: impl = 2 takes lns3d_step");
+  if (c && (c->impl == 2 || c->niter > 1))
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_fp32: impl = 2 and corrector iterations take lns3d_step");
   if (!c || !c->factored || !c->have32 || !v_dev)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_fp32: needs precision = 1 and lns3d_factor");
   void *r = dv_dev ? dv_dev : c->r;
@@ -487,8 +514,8 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
 // for lns3d_last_residual.
 extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev, int corrections,
                                         int check) {
-  if (c && c->impl == 2)
-    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_refined_fixed: impl = 2 takes lns3d_step");
+  if (c && (c->impl == 2 || c->niter > 1))
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_refined_fixed: impl = 2 and corrector iterations take lns3d_step");
   if (!c || !c->factored || !c->have32 || !v_dev || corrections < 0)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_refined_fixed: needs precision = 1 and lns3d_factor");
   const long n = 5L * c->p.nx * c->p.ny;
@@ -556,8 +583,8 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
   for (int i = 0; i < count; i++) {
     if (!ctxs[i] || !ctxs[i]->factored || !v_dev[i] || (mode > 0 && !ctxs[i]->have32))
       return fail(LNS3D_EINVAL, "lns3d_graph_create: every context must be factored (precision 1 for modes 1, 2)");
-    if (ctxs[i]->impl == 2)
-      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2 contexts form their factors again at the second step");
+    if (ctxs[i]->impl == 2 || ctxs[i]->niter > 1)
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2 and corrector iterations take lns3d_step");
   }
   auto *gr = new lns3d_graph;
   std::vector<hipStream_t> st(count, nullptr);

@@ -700,7 +700,8 @@ constexpr int NDER = 17;
 
 __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, Coeffs c,
                                                   const double2 *v, const double2 *c1a, double2 *r,
-                                                  int with_rbe, const double2 *v2old) {
+                                                  int with_rbe, const double2 *v2old,
+                                                  const double2 *vold) {
   using namespace st;
   __shared__ double2 sv[5 * RT], sg1[5 * RT], sd[NDER * RT], sr[5 * RT];
   const int nx = p.nx, ny = p.ny;
@@ -790,12 +791,20 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
     // satisfies the boundary conditions when the step starts)
     if (with_rbe) {
       const double dt = c.dtl[q];
+      constexpr double onept33 = 1.3333333333333333333, pt33 = 3.3333333333333333333e-1;
       if (with_rbe == 2) {
         // two-step backward (src/impdrv3d.f90:473-501, rtb3D) with vold = v:
         // r = 4/3 vold - 1/3 v2old - v - 2/3 dtl r
-        constexpr double onept33 = 1.3333333333333333333, pt33 = 3.3333333333333333333e-1;
         const z vv = vq(m), vo = ld(v2old + 5 * q + m);
         a = sc(onept33, vv) - sc(pt33, vo) - vv - sc(pt66 * dt, a);
+      } else if (with_rbe == 3) {
+        // backward Euler with vold given (later corrector iterations):
+        // r = vold - v - dtl r (src/impdrv3d.f90:444-470)
+        const z vv = vq(m), vb = ld(vold + 5 * q + m);
+        a = vb - vv - sc(dt, a);
+      } else if (with_rbe == 4) {
+        const z vv = vq(m), vb = ld(vold + 5 * q + m), vo = ld(v2old + 5 * q + m);
+        a = sc(onept33, vb) - sc(pt33, vo) - vv - sc(pt66 * dt, a);
       } else {
         a = z{-(dt * a.re), -(dt * a.im)};
       }
@@ -839,12 +848,13 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
 }
 
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
-               void *work, int with_rbe, hipStream_t s, const void *v2old) {
+               void *work, int with_rbe, hipStream_t s, const void *v2old, const void *vold) {
   long P = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(5 * P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
                      (const double2 *)v, (double2 *)work);
   hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, RT)), dim3(5 * 64), 0, s, p, st, c, (const double2 *)v,
-                     (const double2 *)work, (double2 *)r, with_rbe, (const double2 *)v2old);
+                     (const double2 *)work, (double2 *)r, with_rbe, (const double2 *)v2old,
+                     (const double2 *)vold);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }

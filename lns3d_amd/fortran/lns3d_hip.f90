@@ -18,6 +18,7 @@
 !    (include/lns3d_amd.h) and a driver of the implicit step with host
 !    arrays (the time loop of src/impdrv3d.f90:126-406):
 !       call lns3d_hip_init(prm, metric, vm)   ! coefficients, dtl, factors
+!       call lns3d_hip_scheme(impl, niter)     ! optional: the deck's impl, niter
 !       call lns3d_hip_put(v)                  ! state to the device
 !       call lns3d_hip_advance(nsteps, dv)     ! steps on the device
 !       call lns3d_hip_get(v)                  ! state back
@@ -104,6 +105,17 @@ module lns3d_hip
       type(c_ptr), value :: c, dv
       integer(c_int), value :: nsteps
     end function
+    integer(c_int) function c_set_time_scheme(c, impl) bind(C, name="lns3d_set_time_scheme")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: impl
+    end function
+    integer(c_int) function c_set_corrector_iterations(c, niter) &
+        bind(C, name="lns3d_set_corrector_iterations")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: niter
+    end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
@@ -148,6 +160,18 @@ contains
     if (c_associated(ctx)) call lns3d_hip_free()
     rc = c_ctx_create(prm, ctx)
     if (rc == 0) rc = c_setup(ctx, c_loc(metric), c_loc(vm))
+    if (rc == 0) rc = c_factor(ctx)
+    call lns3d_hip_record(rc)
+  end subroutine
+
+  ! the deck's time scheme and corrector iterations (src/impdrv3d.f90:173-199):
+  ! impl 1 backward Euler, 2 two-step backward, 3 alfa = 1/2; niter >= 1
+  ! (impl 1 or 2).  Forms the factors again; a new run starts at lstep 0.
+  subroutine lns3d_hip_scheme(impl, niter)
+    integer, intent(in) :: impl, niter
+    integer(c_int) :: rc
+    rc = c_set_time_scheme(ctx, int(impl, c_int))
+    if (rc == 0) rc = c_set_corrector_iterations(ctx, int(niter, c_int))
     if (rc == 0) rc = c_factor(ctx)
     call lns3d_hip_record(rc)
   end subroutine

@@ -63,6 +63,7 @@ SIGNATURES = {
     "lns3d_set_top_forcing": ([VP, VP, VP, CD], CI),
     "lns3d_set_time_scheme": ([VP, CI], CI),
     "lns3d_set_time_history": ([VP, VP, ctypes.c_long], CI),
+    "lns3d_set_corrector_iterations": ([VP, CI], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -182,6 +183,9 @@ class Step:
     def set_time_scheme(self, impl):
         """the deck's impl: 1 backward Euler, 2 two-step backward, 3 alfa 1/2"""
         check(lib().lns3d_set_time_scheme(self.h, impl), "lns3d_set_time_scheme")
+
+    def set_corrector_iterations(self, niter):
+        check(lib().lns3d_set_corrector_iterations(self.h, niter), "lns3d_set_corrector_iterations")
 
     def set_time_history(self, vold_dev, lstep):
         check(lib().lns3d_set_time_history(self.h, ptr(vold_dev), lstep), "lns3d_set_time_history")

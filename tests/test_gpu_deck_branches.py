@@ -12,8 +12,9 @@ whose residual operator carries the sponge too) reaches the complex-double
 increment; ispg = 4 is rejected.  top = 1 (lns3d_set_top_forcing: the same
 incoming wave on the top boundary, src/rhsbc3d.f90:129-161,
 src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz.
-The time schemes impl = 2 and 3 (lns3d_set_time_scheme) over three of the
-reference's steps."""
+The time schemes impl = 2 and 3 (lns3d_set_time_scheme) and the corrector
+iterations (lns3d_set_corrector_iterations) over three of the reference's
+steps."""
 import os
 
 import numpy as np
@@ -190,4 +191,60 @@ def test_time_scheme_against_reference(impl):
         st2.step(v2, dvd)
         assert rel(host(dvd), z["dv3"]) < TOL
         st2.close()
+    st.close()
+
+
+@pytest.mark.parametrize("case", ["niter1", "niter2"])
+def test_corrector_iterations_against_reference(case):
+    """lns3d_set_corrector_iterations: niter = 3 with impl = 1 and niter = 2
+    with impl = 2 (src/impdrv3d.f90:173-358) over three of the reference's
+    steps (tests/golden/ref_niter<k>.npz).  The last increments of a
+    converging corrector are small, so they are compared relative to the
+    field; the fields at 1e-12.  impl = 3 with niter > 1 and the other step
+    forms are rejected."""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, f"ref_{case}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    impl, niter = int(z["impl"]), int(z["niter"])
+    print(f"{case}: impl={impl} niter={niter} {p.nx}x{p.ny}", flush=True)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t):
+        return t.cpu().numpy().reshape(z["v0"].shape, order="F")
+
+    st = Step(p)
+    st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+    st.set_time_scheme(impl)
+    st.set_corrector_iterations(niter)
+    st.factor()
+    vd = dev(z["itrbc"])
+    scale = float(np.abs(z["itrbc"]).max())
+    dvd = torch.zeros_like(vd)
+    for k in (1, 2, 3):
+        st.step(vd, dvd)
+        e = float(np.abs(host(dvd) - z[f"dv{k}"]).max()) / scale
+        print(f"  step {k}: dv err/max|v| {e:.2e}", flush=True)
+        assert e < TOL, k
+    assert rel(host(vd), z["v3"]) < TOL
+    with pytest.raises(LNS3DError):
+        st.step_fp32(vd, dvd)
+    st.set_time_scheme(3)
+    st.factor()
+    with pytest.raises(LNS3DError):
+        st.step(vd, dvd)
+    # one iteration per step ends elsewhere
+    st.set_time_scheme(impl)
+    st.set_corrector_iterations(1)
+    st.factor()
+    vb = dev(z["itrbc"])
+    for k in (1, 2, 3):
+        st.step(vb, dvd)
+    assert rel(host(vb), z["v3"]) > 1e-8
     st.close()

@@ -134,3 +134,50 @@ def test_module_driver_steps(flib, orc):
         dvo = orc.step(p, c, dtl, f, vo)
     assert relerr(dv, dvo) < TOL
     assert relerr(v, vo) < TOL
+
+
+def test_module_driver_scheme(flib, orc):
+    """lns3d_hip_scheme(impl = 2, niter = 2) then three steps through the
+    module driver, against the oracle's two-step backward corrector"""
+    import dataclasses
+    from lns3d_amd.config import Params, make_grid, make_mean, make_disturbance
+    from tests.penta_cases import relerr
+    p = Params(nx=33, ny=17, wall=1, wallt=1)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    v0 = make_disturbance(p, x, seed=6, amp=1.0)
+    fp = FParams()
+    for name, _ in FParams._fields_:
+        setattr(fp, name, 0 if name == "precision" else getattr(p, name))
+    fn = lambda s: getattr(flib, "_QMlns3d_hipPlns3d_hip_" + s)
+    print("fortran module driver 33x17 impl 2 niter 2", flush=True)
+    fn("init")(ctypes.byref(fp), P(met), P(vm))
+    assert status(flib) == 0
+    fn("scheme")(I(2), I(2))
+    assert status(flib) == 0
+    v = v0.copy(order="F")
+    fn("put")(P(v))
+    dv = np.zeros_like(v, order="F")
+    fn("advance")(I(3), P(dv))
+    assert status(flib) == 0
+    fn("get")(P(v))
+    fn("scheme")(I(3), I(2))            # impl 3 takes one iteration: refused at the step
+    assert status(flib) == 0
+    fn("advance")(I(1), P(dv))
+    assert status(flib) != 0
+    fn("free")()
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    vo = v0.copy(order="F")
+    orc.itrbc3d(p, vo)
+    f = orc.factor(p, c, dtl)
+    v2old = None
+    for k in range(3):
+        if k == 1:
+            f = orc.factor(dataclasses.replace(p, alfa=6.6666666666666666666e-1), c, dtl)
+        start = vo.copy(order="F")
+        pk = p if k == 0 else dataclasses.replace(p, alfa=6.6666666666666666666e-1)
+        dvo = orc.step_iter(pk, c, dtl, f, vo, 2, v2old)
+        v2old = start
+    assert relerr(v, vo) < TOL
+    assert np.abs(dv - dvo).max() < TOL * np.abs(vo).max()
