@@ -238,7 +238,9 @@ class Reference:
         return os.path.exists(path or os.path.join(HERE, "_ref", "libref.so"))
 
     def setup(self, p, x, y, met, vm, usecalcd=0, ispg=0, x0=0.0, impl=1):
-        ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, 0, 0, p.wall,
+        lsym = int(p.left == 7 and not p.xper)           # src/input.f90:235-243
+        rsym = int(p.right == 7 and not p.xper)
+        ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, lsym, rsym, p.wall,
                        p.wallt, p.top, p.left, p.right, p.linear, impl, p.loctime, p.mattyp,
                        ispg, 1, usecalcd], dtype=np.int32)
         rp = np.array([p.Ma, p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax, p.dtmax,

@@ -4,8 +4,8 @@
  * step, and the implicit step driver.  TEST INFRASTRUCTURE ONLY (see
  * lns3d_oracle.h).  Supported options are the ones the benchmark and test
  * configurations use; anything else returns -1 rather than guessing:
- *   optx/opty any value; xper/yper; carp = no; lsym/rsym = no;
- *   wall in {0,1,2}; wallt in {0,1,2}; top = 0; left/right = 0;
+ *   optx/opty any value; xper/yper; carp = no; lsym/rsym (left/right = 7);
+ *   wall in {0,1,2}; wallt in {0,1,2}; top in {0,1}; left/right in {0,7};
  *   Navier = yes; ispg = 0; eps_e = 0; idamp = 0; useCalcd = no;
  *   linear = 2 (complex analysis); mattyp 0..3.
  */
@@ -64,11 +64,30 @@ static void second_coefs(int opt, double dsinv, double d[7]) {
   d[4] = a * dsinv; d[5] = b * dsinv; d[6] = c * dsinv;
 }
 
+/* Symmetry boundaries (left/right = 7: src/cgrad.f90:190-262,
+ * src/cgrad2.f90:229-310, src/grad.f90:155-290, src/grad2.f90:250-330): the
+ * three points next to a symmetric end take the interior stencil, the points
+ * beyond the end reflected onto the line (index q -> -q on the left,
+ * 2(N-1) - q on the right), component 3 (index 2) odd.  sym: bit 0 left,
+ * bit 1 right. */
+static int sym_row(int sym, int p, int N) {
+  return ((sym & 1) && p <= 2) || ((sym & 2) && p >= N - 3);
+}
+static double reflect(int *q, int N, int e) {
+  double s = e == 2 ? -1.0 : 1.0;
+  if (*q < 0) { *q = -*q; return s; }
+  if (*q > N - 1) { *q = 2 * (N - 1) - *q; return s; }
+  return 1.0;
+}
+static int xsym(const orc_params *p) {
+  return p->xper ? 0 : ((p->left == 7 ? 1 : 0) | (p->right == 7 ? 2 : 0));
+}
+
 /* One line of N points: values at x[p*xs + e] for e < ne.  Generic over
  * real/complex via macros instantiated twice below. */
 #define DEFINE_LINE_OPS(T, SUF)                                                       \
-static void d1_line_##SUF(int N, int per, const double g[6], double dinv, int ne,     \
-                          const T *x, long xs, T *y, long ys) {                       \
+static void d1_line_##SUF(int N, int per, int sym, const double g[6], double dinv,   \
+                          int ne, const T *x, long xs, T *y, long ys) {               \
   for (int p = 0; p < N; p++)                                                         \
     for (int e = 0; e < ne; e++) {                                                    \
       T r;                                                                            \
@@ -79,6 +98,13 @@ static void d1_line_##SUF(int N, int per, const double g[6], double dinv, int ne
         for (int k = 0; k < 6; k++) q[k] = ((q[k] % m) + m) % m;                       \
         r = g[0] * x[q[0] * xs + e] + g[1] * x[q[1] * xs + e] + g[2] * x[q[2] * xs + e] \
           + g[3] * x[q[3] * xs + e] + g[4] * x[q[4] * xs + e] + g[5] * x[q[5] * xs + e]; \
+      } else if (sym_row(sym, p, N)) {                                                \
+        int q[6] = {p - 3, p - 2, p - 1, p + 1, p + 2, p + 3};                        \
+        double sg[6];                                                                 \
+        for (int k = 0; k < 6; k++) sg[k] = reflect(&q[k], N, e);                      \
+        r = g[0] * (sg[0] * x[q[0] * xs + e]) + g[1] * (sg[1] * x[q[1] * xs + e])     \
+          + g[2] * (sg[2] * x[q[2] * xs + e]) + g[3] * (sg[3] * x[q[3] * xs + e])     \
+          + g[4] * (sg[4] * x[q[4] * xs + e]) + g[5] * (sg[5] * x[q[5] * xs + e]);    \
       } else if (p == 0) {                                                            \
         r = (gc1 * x[0 * xs + e] + gc2 * x[1 * xs + e] + gc3 * x[2 * xs + e] +        \
              gc4 * x[3 * xs + e] + gc5 * x[4 * xs + e]) * dinv;                       \
@@ -109,8 +135,8 @@ static void d1_line_##SUF(int N, int per, const double g[6], double dinv, int ne
   if (per)                                                                            \
     for (int e = 0; e < ne; e++) y[(N - 1) * ys + e] = y[e];                          \
 }                                                                                     \
-static void d2_line_##SUF(int N, int per, const double d[7], double dsinv, int ne,    \
-                          const T *x, long xs, T *y, long ys) {                       \
+static void d2_line_##SUF(int N, int per, int sym, const double d[7], double dsinv,   \
+                          int ne, const T *x, long xs, T *y, long ys) {               \
   for (int p = 0; p < N; p++)                                                         \
     for (int e = 0; e < ne; e++) {                                                    \
       T r;                                                                            \
@@ -122,6 +148,14 @@ static void d2_line_##SUF(int N, int per, const double d[7], double dsinv, int n
         r = d[0] * x[q[0] * xs + e] + d[1] * x[q[1] * xs + e] + d[2] * x[q[2] * xs + e] \
           + d[3] * x[q[3] * xs + e] + d[4] * x[q[4] * xs + e] + d[5] * x[q[5] * xs + e] \
           + d[6] * x[q[6] * xs + e];                                                  \
+      } else if (sym_row(sym, p, N)) {                                                \
+        int q[7];                                                                     \
+        double sg[7];                                                                 \
+        for (int k = 0; k < 7; k++) { q[k] = p - 3 + k; sg[k] = reflect(&q[k], N, e); } \
+        r = d[0] * (sg[0] * x[q[0] * xs + e]) + d[1] * (sg[1] * x[q[1] * xs + e])     \
+          + d[2] * (sg[2] * x[q[2] * xs + e]) + d[3] * (sg[3] * x[q[3] * xs + e])     \
+          + d[4] * (sg[4] * x[q[4] * xs + e]) + d[5] * (sg[5] * x[q[5] * xs + e])     \
+          + d[6] * (sg[6] * x[q[6] * xs + e]);                                        \
       } else if (p == 0) {                                                            \
         r = (dd1 * x[0 * xs + e] + dd2 * x[1 * xs + e] + dd3 * x[2 * xs + e] +        \
              dd4 * x[3 * xs + e] + dd5 * x[4 * xs + e]) * dsinv;                      \
@@ -168,9 +202,9 @@ static void grad_##SUF(const orc_params *p, int nd, const T *v, T *g1, T *g2) { 
   first_coefs(p->optx, dxinv, gx);                                                    \
   first_coefs(p->opty, dyinv, gy);                                                    \
   for (int j = 0; j < ny; j++)                                                        \
-    d1_line_##SUF(nx, p->xper, gx, dxinv, nd, v + (long)nd * nx * j, nd, g1 + (long)nd * nx * j, nd); \
+    d1_line_##SUF(nx, p->xper, xsym(p), gx, dxinv, nd, v + (long)nd * nx * j, nd, g1 + (long)nd * nx * j, nd); \
   for (int i = 0; i < nx; i++)                                                        \
-    d1_line_##SUF(ny, p->yper, gy, dyinv, nd, v + (long)nd * i, (long)nd * nx, g2 + (long)nd * i, (long)nd * nx); \
+    d1_line_##SUF(ny, p->yper, 0, gy, dyinv, nd, v + (long)nd * i, (long)nd * nx, g2 + (long)nd * i, (long)nd * nx); \
 }                                                                                     \
 static void grad2_##SUF(const orc_params *p, int nd, const T *v, const T *g1,         \
                         T *g11, T *g12, T *g22) {                                     \
@@ -183,10 +217,10 @@ static void grad2_##SUF(const orc_params *p, int nd, const T *v, const T *g1,   
   second_coefs(p->optx, dxsinv, dx);                                                  \
   second_coefs(p->opty, dysinv, dy);                                                  \
   for (int j = 0; j < ny; j++)                                                        \
-    d2_line_##SUF(nx, p->xper, dx, dxsinv, nd, v + (long)nd * nx * j, nd, g11 + (long)nd * nx * j, nd); \
+    d2_line_##SUF(nx, p->xper, xsym(p), dx, dxsinv, nd, v + (long)nd * nx * j, nd, g11 + (long)nd * nx * j, nd); \
   for (int i = 0; i < nx; i++) {                                                      \
-    d2_line_##SUF(ny, p->yper, dy, dysinv, nd, v + (long)nd * i, (long)nd * nx, g22 + (long)nd * i, (long)nd * nx); \
-    d1_line_##SUF(ny, p->yper, gy, dyinv, nd, g1 + (long)nd * i, (long)nd * nx, g12 + (long)nd * i, (long)nd * nx); \
+    d2_line_##SUF(ny, p->yper, 0, dy, dysinv, nd, v + (long)nd * i, (long)nd * nx, g22 + (long)nd * i, (long)nd * nx); \
+    d1_line_##SUF(ny, p->yper, 0, gy, dyinv, nd, g1 + (long)nd * i, (long)nd * nx, g12 + (long)nd * i, (long)nd * nx); \
   }                                                                                   \
 }
 
@@ -211,7 +245,15 @@ void orc_grad2(const orc_params *p, int ndof, const double *v, const double *g1,
 static int supported(const orc_params *p) {
   if (p->nx < 7 || p->ny < 7) return 0;
   if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
-  if ((p->top != 0 && p->top != 1) || p->left != 0 || p->right != 0) return 0;
+  if ((p->top != 0 && p->top != 1) || (p->left != 0 && p->left != 7) ||
+      (p->right != 0 && p->right != 7))
+    return 0;
+  /* symmetric rows read the spanwise cross entries through a transposed
+   * shape (src/lhsbs1f3d.f90:17, Ahi(6,ny,nx)): point i + ny j, which must
+   * lie on the grid (it does for nx >= ny) */
+  if (!p->xper && p->left == 7 && 1 + (long)p->ny * (p->ny - 1) >= (long)p->nx * p->ny) return 0;
+  if (!p->xper && p->right == 7 && p->nx - 1 + (long)p->ny * (p->ny - 1) >= (long)p->nx * p->ny)
+    return 0;
   if (p->linear != 2) return 0;
   if (p->mattyp < 0 || p->mattyp > 3) return 0;
   return 1;
@@ -393,6 +435,52 @@ static void closure_row(zdouble *mat, long pt, const double F[5], const double S
   for (int r = 0; r < 5; r++) MAT(sl[0], r, r) += spgd;
 }
 
+/* a row of the xi factor next to a symmetric end (src/lhsbs1f3d.f90:36-329
+ * left, :331-624 right; calcd, so no sponge; Xi2 read where that routine
+ * reads it, see orc_lhs): the interior row of
+ * src/lhs1f3d.f90 with the weight of each point beyond the end added to the
+ * point it reflects onto, negated for column component 3 (index 2); slots
+ * keep their interior meaning (slot k: point i + k - 2), the slots that then
+ * hold nothing are zero.  a[], b[]: interior first/second-derivative weights
+ * times alfa. */
+static void sym_lhs_row(zdouble *mat, long pt, int i, int N, const double a[5], const double b[5],
+                        double dt, const double *X, const double *Xi2, const double *D,
+                        const double *Di, const double *Vh, double alfa) {
+  double aw[5][5], bw[5][5];                   /* [slot][column component] */
+  for (int k = 0; k < 5; k++)
+    for (int cc = 0; cc < 5; cc++) aw[k][cc] = bw[k][cc] = 0.0;
+  for (int o = 0; o < 5; o++) {
+    int col = i + o - 2;
+    for (int cc = 0; cc < 5; cc++) {
+      int q = col;
+      double s = reflect(&q, N, cc);
+      int t = q - i + 2;
+      aw[t][cc] += s * a[o];
+      bw[t][cc] += s * b[o];
+    }
+  }
+  for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
+  for (int k = 0; k < 5; k++)
+    for (int r = 0; r < 5; r++)
+      for (int cc = 0; cc < 5; cc++) MAT(k, r, cc) = dt * aw[k][cc] * X[r + 5 * cc];
+  for (int r = 0; r < 5; r++)
+    for (int cc = 0; cc < 5; cc++)
+      MAT(2, r, cc) += alfa * dt * D[r + 5 * cc] + _Complex_I * (alfa * dt * Di[r + 5 * cc]);
+  for (int k = 0; k < 5; k++) {
+    MAT(k, 1, 3) -= _Complex_I * (aw[k][3] * dt * Xi2[0]);
+    MAT(k, 2, 3) -= _Complex_I * (aw[k][3] * dt * Xi2[1]);
+    MAT(k, 3, 1) -= _Complex_I * (aw[k][1] * dt * Xi2[0]);
+    MAT(k, 3, 2) -= _Complex_I * (aw[k][2] * dt * Xi2[1]);
+    MAT(k, 1, 1) -= dt * bw[k][1] * Vh[0];
+    MAT(k, 1, 2) -= dt * bw[k][2] * Vh[4];
+    MAT(k, 2, 1) -= dt * bw[k][1] * Vh[5];
+    MAT(k, 2, 2) -= dt * bw[k][2] * Vh[1];
+    MAT(k, 3, 3) -= dt * bw[k][3] * Vh[2];
+    MAT(k, 4, 4) -= dt * bw[k][4] * Vh[3];
+  }
+  for (int r = 0; r < 5; r++) MAT(2, r, r) += 1.0;
+}
+
 int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl,
             zdouble *mat) {
   if (!supported(p) || (dir != 1 && dir != 2)) return -1;
@@ -471,6 +559,19 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
       }
     }
   }
+  if (dir == 1 && xsym(p)) {                   /* src/lhsbs1f3d.f90 (xi factor only) */
+    const int sym = xsym(p);
+    for (int j = 0; j < ny; j++) {
+      const int rows[4] = {0, 1, nx - 2, nx - 1};
+      for (int w = 0; w < 4; w++) {
+        if (!((w < 2 && (sym & 1)) || (w >= 2 && (sym & 2)))) continue;
+        long pt = rows[w] + (long)nx * j;
+        const long pa = rows[w] + (long)ny * j;     /* Ahi(6,ny,nx) of src/lhsbs1f3d.f90:17 */
+        sym_lhs_row(mat, pt, rows[w], nx, a, b, dtl[pt], X + 25 * pt, c->ABhi + 6 * pa,
+                    c->Dh + 25 * pt, c->Dhi + 25 * pt, Vh + 6 * pt, alfa);
+      }
+    }
+  }
 
   /* boundary conditions */
   if (!p->yper) {
@@ -510,9 +611,10 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
     }
   }
   if (!p->xper) {
-    for (int j = 0; j < ny; j++) {                        /* left = right = 0 */
+    for (int j = 0; j < ny; j++) {                        /* left, right = 0 */
       long pts[2] = {(long)nx * j, nx - 1 + (long)nx * j};
       for (int s = 0; s < 2; s++) {
+        if ((s == 0 ? p->left : p->right) != 0) continue;
         long pt = pts[s];
         for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
         for (int r = 0; r < 5; r++) MAT(2, r, r) = 1.0;
@@ -558,8 +660,11 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
     for (int j = 0; j < ny; j++)
       for (int d = 0; d < 5; d++) R(d, nx - 1, j) = 0;
   } else {
-    for (int j = 0; j < ny; j++)
-      for (int d = 0; d < 5; d++) { R(d, 0, j) = 0; R(d, nx - 1, j) = 0; }
+    for (int j = 0; j < ny; j++)              /* left/right = 7: the operator's row */
+      for (int d = 0; d < 5; d++) {
+        if (p->left == 0) R(d, 0, j) = 0;
+        if (p->right == 0) R(d, nx - 1, j) = 0;
+      }
   }
   return 0;
 }
@@ -597,8 +702,11 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
     for (int j = 0; j < ny; j++)
       for (int d = 0; d < 5; d++) V(d, nx - 1, j) = V(d, 0, j);
   } else {
-    for (int j = 0; j < ny; j++)
-      for (int d = 0; d < 5; d++) { V(d, 0, j) = 0; V(d, nx - 1, j) = 0; }
+    for (int j = 0; j < ny; j++)              /* left/right = 7: left as they are */
+      for (int d = 0; d < 5; d++) {
+        if (p->left == 0) V(d, 0, j) = 0;
+        if (p->right == 0) V(d, nx - 1, j) = 0;
+      }
   }
   return 0;
 #undef R

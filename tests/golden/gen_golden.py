@@ -33,7 +33,14 @@ reference's outputs:
                         wamp and x0; lrhs3D, both assembled LHS factors and
                         one implicit step with the sponge
 
+  ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
+                        (with wall = 1, wallt = 1) and mse (the boundary set
+                        of test/mse-recep/complex.inp: top 1, left 7); the
+                        coefficients, dtl, itrBC3D, lrhs3D, rhsBC3D, both
+                        assembled factors and two steps
+
     python tests/golden/gen_golden.py sponge     (the sponge, top = 1 and impl cases only)
+    python tests/golden/gen_golden.py sym        (the symmetry cases only)
 """
 from __future__ import annotations
 
@@ -236,6 +243,54 @@ def impl_case(R, impl, niter=1, name=None):
     R.set_niter(1)
 
 
+SYM_CASES = {
+    "left7": dict(nx=13, ny=9, kz=0.4, omega=0.2, left=7),
+    "both7": dict(nx=14, ny=9, optx=0, opty=0, wall=1, wallt=1, left=7, right=7, kz=0.3,
+                  loctime=1),
+    # the boundary set of test/mse-recep/complex.inp (top 1, wall 0, left 7,
+    # right 0, wallt 0; Ma 0.1, Re 2400, Pr 1, optx = opty = -1, its omega),
+    # small and without its sponge
+    "mse": dict(nx=16, ny=9, Ma=0.1, Re=2400.0, Pr=1.0, top=1, left=7, kz=0.0,
+                omega=0.5529203070318036),
+}
+
+
+def sym_case(R, name, kw):
+    """symmetry boundaries (left/right = 7: the symmetric closures of
+    src/cgrad.f90, cgrad2.f90, grad.f90, grad2.f90 and src/lhsbs1f3d.f90).
+    The mean flow varies along x so its xi derivatives next to the ends are
+    not zero; the disturbance covers the whole grid."""
+    p = Params(**kw)
+    x, y, met = make_grid(p)
+    xs = x / x.max()
+    vm = make_mean(p, y)
+    vm = np.asfortranarray(vm * (1.0 + 0.1 * np.sin(3.0 * xs + 0.3))[None] +
+                           0.05 * np.cos(2.0 * xs)[None])
+    rng = np.random.default_rng(31)
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    x0 = float(x.min()) + 0.5
+    R.setup(p, x, y, met, vm, x0=x0)
+    c = R.genmtrx()
+    c["ABhi"][4:] = 0.0                   # never set by genmtrx (unused, not initialised)
+    dtl = R.set_time_step()
+    out = dict(params=_params_json(p), x0=np.array(x0), x=x, y=y, met=met, vm=vm, v0=v0, dtl=dtl,
+               **{"c_" + k: v for k, v in c.items()})
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)                          # top = 1: allocates wamp = 1 (istep = 0)
+    out["itrbc"] = v.copy()
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    out["lrhs"] = r
+    rb = np.asfortranarray(r.copy())
+    R.rhsbc3d(rb, np.asfortranarray(v0))
+    out["rhsbc"] = rb
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    out["dv1"] = R.step(v, True)
+    out["dv2"] = R.step(v, False)
+    out["v2"] = v.copy()
+    np.savez_compressed(os.path.join(HERE, f"ref_sym_{name}.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -250,6 +305,11 @@ def main():
         impl_case(R, 2, 2, "ref_niter2.npz")
         R.lib.ref_flush()
         return
+    if sys.argv[1:] == ["sym"]:
+        for name, kw in SYM_CASES.items():
+            sym_case(R, name, kw)
+        R.lib.ref_flush()
+        return
     for name, kw in STEP_CASES.items():
         step_case(R, name, kw, keep_lhs=(name in ("base", "xper")))
     penta_case(R)
@@ -261,6 +321,8 @@ def main():
         impl_case(R, impl)
     impl_case(R, 1, 3, "ref_niter1.npz")
     impl_case(R, 2, 2, "ref_niter2.npz")
+    for name, kw in SYM_CASES.items():
+        sym_case(R, name, kw)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
