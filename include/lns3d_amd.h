@@ -29,7 +29,11 @@ const char *lns3d_last_error(void);
 /* Scheme, physics and boundary selectors: the module state of src/global.f90
  * that the step routines read (src/input.f90 deck entries of the same names).
  * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
- * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left = right = 0; linear = 2; mattyp in {0,2,3}; loctime in {0,1};
+ * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left and
+ * right in {0,7} (7: symmetry, src/lhsbs1f3d.f90 and the symmetric closures
+ * of src/cgrad.f90, cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
+ * the reference's symmetric rows read point i + ny j); linear = 2; mattyp in
+ * {0,2,3}; loctime in {0,1};
  * Navier-Stokes (viscous) boundary rows; eps_e = 0; sponges through
  * lns3d_set_sponge (ispg 1, 2, 3, 5).  Other values are rejected with
  * LNS3D_EUNSUPPORTED. */

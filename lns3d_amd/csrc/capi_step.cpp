@@ -50,10 +50,17 @@ struct lns3d_ctx {
 
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
-  if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) || p.left != 0 ||
-      p.right != 0 || p.linear != 2 || (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) ||
-      (p.loctime != 0 && p.loctime != 1))
+  if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
+      (p.left != 0 && p.left != 7) || (p.right != 0 && p.right != 7) || p.linear != 2 ||
+      (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) || (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
+  // symmetric ends: src/lhsbs1f3d.f90:17 reads the spanwise cross entries of
+  // row (i, j) at point i + ny j (its Ahi(6,ny,nx) shape), which lies on the
+  // grid only when nx >= ny (as in the reference's own symmetric case)
+  const long P = (long)p.nx * p.ny;
+  if (!p.xper && ((p.left == 7 && 1 + (long)p.ny * (p.ny - 1) >= P) ||
+                  (p.right == 7 && p.nx - 1 + (long)p.ny * (p.ny - 1) >= P)))
+    return fail(LNS3D_EUNSUPPORTED, "left/right = 7 needs nx >= ny (the reference reads off the grid otherwise)");
   if (p.precision != 0 && p.precision != 1) return fail(LNS3D_EINVAL, "precision must be 0 or 1");
   if ((p.xper || p.yper) && p.precision == 1)
     return fail(LNS3D_EUNSUPPORTED, "complex-float factors are for non-periodic lines");

@@ -29,15 +29,28 @@ __device__ __forceinline__ z ld(const double2 *p) { double2 v = *p; return {v.x,
 // first derivative along a line at position p (N points), value stride es
 // between consecutive points; component e selected by base offset.
 // Formulas of src/cgrad.f90:84-283 (periodic and non-periodic branches).
+// Symmetric ends (left/right = 7; sym bit 0 left, bit 1 right): the three
+// points next to the end take the interior stencil with the points beyond it
+// reflected onto the line, negated by sgn (-1 for component 3), as
+// src/cgrad.f90:190-262, cgrad2.f90:229-310, grad.f90:155-290, grad2.f90:250-330.
+__device__ __forceinline__ bool sym_near(int sym, int p, int N) {
+  return ((sym & 1) && p <= 2) || ((sym & 2) && p >= N - 3);
+}
+
 template <typename S, typename V>
 __device__ __forceinline__ V d1(int N, int per, int p, const double *g, double dinv,
-                                const S *base, long es) {
+                                const S *base, long es, int sym = 0, double sgn = 1.0) {
   using namespace st;
   auto x = [&](int q) { return ld(base + (long)q * es); };
   if (per) {
     int m = N - 1;
     if (p == N - 1) p = 0;
     auto w = [&](int q) { return x(((q % m) + m) % m); };
+    return sc(g[0], w(p - 3)) + sc(g[1], w(p - 2)) + sc(g[2], w(p - 1)) + sc(g[3], w(p + 1)) +
+           sc(g[4], w(p + 2)) + sc(g[5], w(p + 3));
+  }
+  if (sym && sym_near(sym, p, N)) {
+    auto w = [&](int q) { return q < 0 ? sc(sgn, x(-q)) : q > N - 1 ? sc(sgn, x(2 * (N - 1) - q)) : x(q); };
     return sc(g[0], w(p - 3)) + sc(g[1], w(p - 2)) + sc(g[2], w(p - 1)) + sc(g[3], w(p + 1)) +
            sc(g[4], w(p + 2)) + sc(g[5], w(p + 3));
   }
@@ -59,13 +72,18 @@ __device__ __forceinline__ V d1(int N, int per, int p, const double *g, double d
 // second derivative, src/cgrad2.f90:134-326
 template <typename S, typename V>
 __device__ __forceinline__ V d2(int N, int per, int p, const double *d, double dsinv,
-                                const S *base, long es) {
+                                const S *base, long es, int sym = 0, double sgn = 1.0) {
   using namespace st;
   auto x = [&](int q) { return ld(base + (long)q * es); };
   if (per) {
     int m = N - 1;
     if (p == N - 1) p = 0;
     auto w = [&](int q) { return x(((q % m) + m) % m); };
+    return sc(d[0], w(p - 3)) + sc(d[1], w(p - 2)) + sc(d[2], w(p - 1)) + sc(d[3], w(p)) +
+           sc(d[4], w(p + 1)) + sc(d[5], w(p + 2)) + sc(d[6], w(p + 3));
+  }
+  if (sym && sym_near(sym, p, N)) {
+    auto w = [&](int q) { return q < 0 ? sc(sgn, x(-q)) : q > N - 1 ? sc(sgn, x(2 * (N - 1) - q)) : x(q); };
     return sc(d[0], w(p - 3)) + sc(d[1], w(p - 2)) + sc(d[2], w(p - 1)) + sc(d[3], w(p)) +
            sc(d[4], w(p + 1)) + sc(d[5], w(p + 2)) + sc(d[6], w(p + 3));
   }
@@ -93,6 +111,7 @@ Stencil make_stencil(const lns3d_params &p) {
   s.dyinv = 1.0 / s.deta;
   s.dxsinv = 1.0 / (s.dxi * s.dxi);
   s.dysinv = 1.0 / (s.deta * s.deta);
+  s.sym = p.xper ? 0 : ((p.left == 7 ? 1 : 0) | (p.right == 7 ? 2 : 0));   // src/input.f90:235-243
   auto first = [](int opt, double dinv, double *g) {
     double c;
     if (opt == 0) c = 1.0 / 60.0;
@@ -141,7 +160,7 @@ __global__ void k_mean_d1x(int nx, int ny, int xper, Stencil st, const double *v
   int i = q % nx, j = q / nx;
   const double *base = vm + 5L * nx * j;
   for (int e = 0; e < 5; e++)
-    g1[5 * q + e] = d1<double, double>(nx, xper, i, st.gx, st.dxinv, base + e, 5);
+    g1[5 * q + e] = d1<double, double>(nx, xper, i, st.gx, st.dxinv, base + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
 }
 
 __device__ __forceinline__ void getmat(int mattyp, double t, double &mu, double &lm, double &con,
@@ -170,7 +189,7 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
   for (int e = 0; e < 5; e++) {
     g1[e] = g1a[5 * q + e];
     g2[e] = d1<double, double>(ny, p.yper, j, st.gy, st.dyinv, colv + e, 5L * nx);
-    g11[e] = d2<double, double>(nx, p.xper, i, st.dx, st.dxsinv, rowv + e, 5);
+    g11[e] = d2<double, double>(nx, p.xper, i, st.dx, st.dxsinv, rowv + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
     g22[e] = d2<double, double>(ny, p.yper, j, st.dy, st.dysinv, colv + e, 5L * nx);
     g12[e] = d1<double, double>(ny, p.yper, j, st.gy, st.dyinv, colg + e, 5L * nx);
   }
@@ -505,7 +524,26 @@ struct AsmCtx {
   lns3d_params p;
   double d, dinv, dsinv;        // spacing along the line
   double a[5], b[5];            // interior first/second derivative weights x alfa
+  int sym;                      // symmetric ends of the xi lines (bit 0 left, bit 1 right)
 };
+
+// weights of slot k (point pos + k - 2) for column component cc in a row next
+// to a symmetric end: interior weights, those of points beyond the end moved
+// onto their reflections (-q on the left, 2(N-1) - q on the right), negated
+// for component 3 (src/lhsbs1f3d.f90:44-120, 178-270)
+__device__ __forceinline__ void sym_weights(const AsmCtx &A, int pos, int N, int k, int cc,
+                                            double &aw, double &bw) {
+  aw = 0.0;
+  bw = 0.0;
+  const double s = cc == 2 ? -1.0 : 1.0;
+  for (int o = 0; o < 5; o++) {
+    int col = pos + o - 2;
+    double sg = 1.0;
+    if (col < 0) { col = -col; sg = s; }
+    else if (col > N - 1) { col = 2 * (N - 1) - col; sg = s; }
+    if (col - pos + 2 == k) { aw += sg * A.a[o]; bw += sg * A.b[o]; }
+  }
+}
 
 __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir, int i, int j, int k,
                                        int r, int cc) {
@@ -520,7 +558,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   // boundary-condition rows (src/lhsbc1f3d.f90, src/lhsbc2f3d.f90); the
   // left/right rows are applied last, then top, then the wall
   const z one{(k == 2 && r == cc) ? 1.0 : 0.0, 0.0};
-  if (!p.xper && (i == 0 || i == p.nx - 1)) return one;
+  if (!p.xper && ((i == 0 && p.left == 0) || (i == p.nx - 1 && p.right == 0))) return one;
   if (!p.yper) {
     if (j == p.ny - 1) return one;
     if (j == 0) {
@@ -560,6 +598,26 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   const double Vv = vi >= 0 ? Vh[(long)vi * P + q] : 0.0;
   const double ABv = ai >= 0 ? c.ABhi[(long)(ab + ai) * P + q] : 0.0;
 
+  const int sym = dir == 1 ? A.sym : 0;
+  if (sym && (((sym & 1) && pos <= 1) || ((sym & 2) && pos >= N - 2))) {
+    // rows next to a symmetric end (src/lhsbs1f3d.f90, xi factor, calcd):
+    // the interior row with the weight of each point beyond the end added to
+    // the point it reflects onto, negated for column component 3; the
+    // spanwise cross entries are read through that routine's Ahi(6,ny,nx)
+    // shape (point i + ny j; check_params keeps it on the grid)
+    double aw, bw;
+    sym_weights(A, pos, N, k, cc, aw, bw);
+    const double ABs = ai >= 0 ? c.ABhi[(long)(ab + ai) * P + pos + (long)p.ny * j] : 0.0;
+    double re = dt * aw * X, im = 0.0;
+    if (k == 2) {
+      re = re + alfa * dt * c.Dh[(long)(r + 5 * cc) * P + q];
+      im = alfa * dt * c.Dhi[(long)(r + 5 * cc) * P + q];
+    }
+    if (ai >= 0) im = im - aw * dt * ABs;
+    if (vi >= 0) re = re - dt * bw * Vv;
+    if (k == 2 && r == cc) re = re + 1.0;
+    return z{re, im};
+  }
   if (!per && (pos <= 1 || pos >= N - 2)) {
     // boundary closure rows (src/lhsbt1f3d.f90, lhsbt2f3d.f90): first- and
     // second-derivative weights of the column held in slot k
@@ -614,6 +672,7 @@ static AsmCtx make_asm(const lns3d_params &p, int dir) {
   A.d = dir == 1 ? 1.0 / (double)(p.nx - 1) : 1.0 / (double)(p.ny - 1);
   A.dinv = 1.0 / A.d;
   A.dsinv = 1.0 / (A.d * A.d);
+  A.sym = (dir == 1 && !p.xper) ? ((p.left == 7 ? 1 : 0) | (p.right == 7 ? 2 : 0)) : 0;
   double alfa = p.alfa, d = A.d;
   double ga[5] = {ga1, ga2, 0.0, ga3, ga4}, da[5] = {da1, da2, da3, da4, da5};
   for (int k = 0; k < 5; k++) {
@@ -682,7 +741,7 @@ __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v
   const long q = t / 5;
   const int e = (int)(t - 5 * q);
   const int i = q % nx, j = q / nx;
-  z g = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, v + 5L * nx * j + e, 5);
+  z g = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, v + 5L * nx * j + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
   c1[t] = make_double2(g.re, g.im);
 }
 
@@ -729,7 +788,8 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
       h = d1<double2, z>(ny, p.yper, jj, st.gy, st.dyinv, v + 5L * ii + e, 5L * nx);
       if (e == 4 && (p.wallt == 1 || p.wallt == 2) && jj == 0) h = z{0, 0};   // src/cgradbc.f90:15
     } else if (kind == 1) {
-      h = d2<double2, z>(nx, p.xper, ii, st.dx, st.dxsinv, v + 5L * nx * jj + e, 5);
+      h = d2<double2, z>(nx, p.xper, ii, st.dx, st.dxsinv, v + 5L * nx * jj + e, 5, st.sym,
+                         e == 2 ? -1.0 : 1.0);
     } else if (kind == 2) {
       h = d1<double2, z>(ny, p.yper, jj, st.gy, st.dyinv, c1a + 5L * ii + e, 5L * nx);
     } else {
@@ -837,8 +897,8 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
       }
       if (p.xper) {
         if (i == nx - 1) a = z{0, 0};
-      } else if (i == 0 || i == nx - 1) {
-        a = z{0, 0};
+      } else if ((i == 0 && p.left == 0) || (i == nx - 1 && p.right == 0)) {
+        a = z{0, 0};              // left/right = 7: the operator's row (rhsbc3d.f90:271)
       }
     }
     sr[5 * l + m] = make_double2(a.re, a.im);
@@ -910,9 +970,9 @@ __global__ void k_itrbc_lr(lns3d_params p, double2 *v) {
   for (int d = 0; d < 5; d++) {
     if (p.xper) {
       row[5L * (nx - 1) + d] = row[d];
-    } else {
-      row[d] = make_double2(0, 0);
-      row[5L * (nx - 1) + d] = make_double2(0, 0);
+    } else {                 // left/right = 7 leave the row alone (itrbc3d.f90:342)
+      if (p.left == 0) row[d] = make_double2(0, 0);
+      if (p.right == 0) row[5L * (nx - 1) + d] = make_double2(0, 0);
     }
   }
 }
@@ -984,8 +1044,14 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   const int per = dir == 1 ? p.xper : p.yper;
   const int posc = (per && pos == N - 1) ? 0 : pos;       // duplicate periodic row
   const long qr = dir == 1 ? (posc + (long)nx * j) : (i + (long)nx * posc);   // row point
-  double w1[5], w2[5];
-  if (!per && (posc <= 1 || posc >= N - 2)) {
+  double w1[5], w2[5], w1m[5], w2m[5];      // w1m, w2m: component 3 (symmetric rows)
+  const bool symrow = A.sym && (((A.sym & 1) && posc <= 1) || ((A.sym & 2) && posc >= N - 2));
+  if (symrow) {
+    for (int k = 0; k < 5; k++) {
+      sym_weights(A, posc, N, k, 0, w1[k], w2[k]);
+      sym_weights(A, posc, N, k, 2, w1m[k], w2m[k]);
+    }
+  } else if (!per && (posc <= 1 || posc >= N - 2)) {
     double F[5], S[5];
     if (posc == 0) {
       const double f[5] = {gc4, gc5, gc1, gc2, gc3}, sd[5] = {dd4, dd5, dd1, dd2, dd3};
@@ -1004,15 +1070,20 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   } else {
     for (int k = 0; k < 5; k++) { w1[k] = A.a[k]; w2[k] = A.b[k]; }
   }
+  if (!symrow)
+    for (int k = 0; k < 5; k++) { w1m[k] = w1[k]; w2m[k] = w2[k]; }
   z d1v[5], d2v[5], xc[5];
   for (int e = 0; e < 5; e++) { d1v[e] = z{0, 0}; d2v[e] = z{0, 0}; }
   for (int k = 0; k < 5; k++) {
-    const int col = slot_col(N, per, posc, k);
+    // symmetric rows: slot k is point posc + k - 2; the slots beyond the end
+    // carry no weight and are skipped
+    const int col = symrow ? posc - 2 + k : slot_col(N, per, posc, k);
+    if (symrow && (col < 0 || col > N - 1)) continue;
     const long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
     for (int e = 0; e < 5; e++) {
       const z xv = ld(x + 5 * qc + e);
-      d1v[e] = d1v[e] + sc(w1[k], xv);
-      d2v[e] = d2v[e] + sc(w2[k], xv);
+      d1v[e] = d1v[e] + sc(e == 2 ? w1m[k] : w1[k], xv);
+      d2v[e] = d2v[e] + sc(e == 2 ? w2m[k] : w2[k], xv);
       if (k == 2) xc[e] = xv;
     }
   }
@@ -1031,7 +1102,10 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   out[3] = out[3] - sc(V[2 * P + qr], d2v[3]);
   out[4] = out[4] - sc(V[3 * P + qr], d2v[4]);
   {
-    const double ab0 = AB[qr], ab1 = AB[P + qr];
+    // symmetric rows read the cross entries through src/lhsbs1f3d.f90's
+    // Ahi(6,ny,nx) shape: point posc + ny j
+    const long qa = symrow ? posc + (long)ny * j : qr;
+    const double ab0 = AB[qa], ab1 = AB[P + qa];
     z t1 = sc(ab0, d1v[3]), t2 = sc(ab1, d1v[3]), t3 = sc(ab0, d1v[1]) + sc(ab1, d1v[2]);
     out[1] = out[1] - z{-t1.im, t1.re};     // - i * t
     out[2] = out[2] - z{-t2.im, t2.re};
@@ -1057,7 +1131,8 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   bool ident[5] = {false, false, false, false, false};
   bool wallrow0 = false, wallrow4 = false;
   const int ir = dir == 1 ? posc : i, jr = dir == 1 ? j : posc;
-  if (!p.xper && (ir == 0 || ir == nx - 1)) for (int r = 0; r < 5; r++) ident[r] = true;
+  const bool lrrow = !p.xper && ((ir == 0 && p.left == 0) || (ir == nx - 1 && p.right == 0));
+  if (lrrow) for (int r = 0; r < 5; r++) ident[r] = true;
   if (!p.yper) {
     if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;
     if (jr == 0) {
@@ -1068,7 +1143,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
       if (dir == 2 && p.wallt == 1) wallrow4 = true;
     }
   }
-  if (!p.xper && (ir == 0 || ir == nx - 1)) wallrow0 = wallrow4 = false;
+  if (lrrow) wallrow0 = wallrow4 = false;
   if (!p.yper && jr == ny - 1) wallrow0 = wallrow4 = false;
   if (wallrow0 || wallrow4) {
     const double w[5] = {-gc4, -gc5, -gc1, -gc2, -gc3};

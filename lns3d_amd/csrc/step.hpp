@@ -13,6 +13,7 @@ struct Stencil {
   double dx[7], dy[7];   // interior second derivative (premultiplied by 1/d^2)
   double dxinv, dyinv, dxsinv, dysinv;
   double dxi, deta;
+  int sym;               // symmetric xi ends (left/right = 7): bit 0 left, bit 1 right
 };
 Stencil make_stencil(const lns3d_params &p);
 
