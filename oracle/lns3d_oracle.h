@@ -85,6 +85,11 @@ int orc_set_sponge(const orc_sponge *s);
 /* top = 1 (src/rhsbc3d.f90:129-161, itrbc3d.f90:145-195): physical x
  * (nx,ny), amplitude wamp (nx; NULL for 1), mean flow vm (5,nx,ny), x0 */
 int orc_set_top(const double *x, const double *wamp, const double *vm, double x0);
+/* left = 1 / left = 4 / right = 4 inflow state: x (nx,ny), vm (5,nx,ny), x0;
+ * eig (10,ny) the eigenfunction profiles rhor, rhoi, ur, ui, vr, vi, wr, wi,
+ * tr, ti; lalpha (re, im) for right = 4 (NULL: 0) */
+int orc_set_inflow(const double *x, const double *vm, double x0, const double *eig,
+                   const double *lalpha);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
                zdouble *r, zdouble *work /* 5 arrays of (5,nx,ny) */);
 void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v,

@@ -79,6 +79,15 @@ class Oracle:
         self.lib.orc_set_top(*(_ptr(a) for a in arrs), x0)
         self._top_keep = arrs
 
+    def set_inflow(self, x=None, vm=None, x0=0.0, eig=None, lalpha=0j):
+        """left = 1 / left = 4 / right = 4 inflow data (None: cleared); eig
+        (10, ny): rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti"""
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (x, vm, eig)]
+        la = np.array([complex(lalpha).real, complex(lalpha).imag])
+        self.lib.orc_set_inflow.argtypes = [VP, VP, ctypes.c_double, VP, VP]
+        self.lib.orc_set_inflow(_ptr(arrs[0]), _ptr(arrs[1]), x0, _ptr(arrs[2]), _ptr(la))
+        self._inf_keep = arrs + [la]
+
     @staticmethod
     def _coeffs(c):
         s = _OrcCoeffs()
@@ -249,6 +258,13 @@ class Reference:
         self.lib.ref_set_grid(_ptr(x), _ptr(y), _ptr(met))
         self.lib.ref_set_mean(_ptr(vm))
         self.p = p
+
+    def set_inflow(self, eig, lalpha=0j, lomega=0j):
+        """the eigenfunction profiles of src/gendist.f90 (eig (10, ny): rhor,
+        rhoi, ur, ui, vr, vi, wr, wi, tr, ti) and the namelist's lalpha, lomega"""
+        la = np.array([complex(lalpha).real, complex(lalpha).imag])
+        lo = np.array([complex(lomega).real, complex(lomega).imag])
+        self.lib.ref_set_inflow(_ptr(np.asfortranarray(eig, dtype=np.float64)), _ptr(la), _ptr(lo))
 
     def set_niter(self, n):
         """corrector iterations per step (the deck's niter)"""

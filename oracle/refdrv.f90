@@ -132,6 +132,21 @@ subroutine ref_set_sponge(spgl, spg2l, wampl) bind(C, name="ref_set_sponge")
   spg = spgl; spg2 = spg2l; wamp = wampl
 end subroutine ref_set_sponge
 
+! eigenfunction inflow (left = 4, right = 4): the profiles src/gendist.f90
+! reads from inflow_file (columns rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti)
+! and the inflow namelist's lalpha, lomega
+subroutine ref_set_inflow(eig, la, lo) bind(C, name="ref_set_inflow")
+  use iso_c_binding
+  use global
+  implicit none
+  real(c_double), intent(in) :: eig(10,ny), la(2), lo(2)
+  if (allocated(rhor)) deallocate(rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti)
+  allocate(rhor(ny), rhoi(ny), ur(ny), ui(ny), vr(ny), vi(ny), wr(ny), wi(ny), tr(ny), ti(ny))
+  rhor = eig(1,:); rhoi = eig(2,:); ur = eig(3,:); ui = eig(4,:); vr = eig(5,:)
+  vi = eig(6,:); wr = eig(7,:); wi = eig(8,:); tr = eig(9,:); ti = eig(10,:)
+  lalpha = cmplx(la(1), la(2)); lomega = cmplx(lo(1), lo(2))
+end subroutine ref_set_inflow
+
 ! coefficient generation (src/genmtrx.f90) and time step (src/temporal.f90)
 subroutine ref_genmtrx() bind(C, name="ref_genmtrx")
   use global

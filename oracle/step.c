@@ -245,9 +245,11 @@ void orc_grad2(const orc_params *p, int ndof, const double *v, const double *g1,
 static int supported(const orc_params *p) {
   if (p->nx < 7 || p->ny < 7) return 0;
   if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
-  if ((p->top != 0 && p->top != 1) || (p->left != 0 && p->left != 7) ||
-      (p->right != 0 && p->right != 7))
+  if ((p->top != 0 && p->top != 1) ||
+      (p->left != 0 && p->left != 1 && p->left != 4 && p->left != 7) ||
+      (p->right != 0 && p->right != 4 && p->right != 7))
     return 0;
+  if (p->right == 4 && p->left != 4) return 0;   /* the profiles exist for left = 4 only */
   /* symmetric rows read the spanwise cross entries through a transposed
    * shape (src/lhsbs1f3d.f90:17, Ahi(6,ny,nx)): point i + ny j, which must
    * lie on the grid (it does for nx >= ny) */
@@ -297,6 +299,45 @@ static void acoustic_wave(const orc_params *p, const double *vml, double xl, dou
   f[2] = 0;
   f[3] = 0;
   f[4] = (gM2 * c3 * 0.5 - T * 0.5 * c3 / (cm * cm)) / rho;
+}
+
+/* left/right inflow state: left = 1 (the acoustic wave on x(1,j), from the
+ * mean flow, x and x0: src/rhsbc3d.f90:176-205, src/itrbc3d.f90:219-245),
+ * left = 4 (the eigenfunction profiles rhor, rhoi, ..., ti of
+ * src/gendist.f90, eig[10 j + 2 d] + i eig[10 j + 2 d + 1]: src/rhsbc3d.f90
+ * :207-246, src/itrbc3d.f90:247-318, omega != 0 so no transient factor) and
+ * right = 4 (the same profiles times exp(i lalpha x(nx,j)):
+ * src/itrbc3d.f90:352-376) */
+static struct { const double *x, *vm, *eig; double x0; zdouble lalpha; } g_inf;
+
+int orc_set_inflow(const double *x, const double *vm, double x0, const double *eig,
+                   const double *lalpha) {
+  g_inf.x = x; g_inf.vm = vm; g_inf.x0 = x0; g_inf.eig = eig;
+  g_inf.lalpha = lalpha ? lalpha[0] + _Complex_I * lalpha[1] : 0;
+  return 0;
+}
+
+/* the values the left column (side 0) or the right column (side 1) is held
+ * at on line j; -1 when the state they need is missing */
+static int side_values(const orc_params *p, int side, int j, zdouble f[5]) {
+  const long q = (side ? p->nx - 1 : 0) + (long)p->nx * j;
+  const int sel = side ? p->right : p->left;
+  if (sel == 1) {
+    if (!g_inf.x || !g_inf.vm) return -1;
+    acoustic_wave(p, g_inf.vm + 5 * q, g_inf.x[q], g_inf.x0, 1.0, f);
+    return 0;
+  }
+  if (sel == 4) {
+    if (!g_inf.eig || p->omega == 0.0) return -1;
+    for (int d = 0; d < 5; d++) f[d] = g_inf.eig[10 * j + 2 * d] + _Complex_I * g_inf.eig[10 * j + 2 * d + 1];
+    if (side) {
+      if (!g_inf.x) return -1;
+      const zdouble e = cexp(_Complex_I * g_inf.lalpha * g_inf.x[q]);
+      for (int d = 0; d < 5; d++) f[d] = f[d] * e;
+    }
+    return 0;
+  }
+  return -1;
 }
 
 /* top = 1 forcing state (src/global.f90 x, wamp, x0): wamp NULL means 1 */
@@ -614,7 +655,8 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
     for (int j = 0; j < ny; j++) {                        /* left, right = 0 */
       long pts[2] = {(long)nx * j, nx - 1 + (long)nx * j};
       for (int s = 0; s < 2; s++) {
-        if ((s == 0 ? p->left : p->right) != 0) continue;
+        const int sel = s == 0 ? p->left : p->right;   /* src/lhsbc1f3d.f90:414-436 */
+        if (sel != 0 && sel != 4) continue;
         long pt = pts[s];
         for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
         for (int r = 0; r < 5; r++) MAT(2, r, r) = 1.0;
@@ -625,7 +667,8 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
 }
 #undef MAT
 
-/* src/rhsbc3d.f90:34-160, 200-215 (wall, top=0, left=right=0) */
+/* src/rhsbc3d.f90:34-215, 271, 281-284 (wall; top 0, 1; left 0, 1, 4, 7;
+ * right 0, 4, 7) */
 int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
   if (!supported(p)) return -1;
   int nx = p->nx, ny = p->ny;
@@ -660,16 +703,22 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
     for (int j = 0; j < ny; j++)
       for (int d = 0; d < 5; d++) R(d, nx - 1, j) = 0;
   } else {
-    for (int j = 0; j < ny; j++)              /* left/right = 7: the operator's row */
+    for (int j = 0; j < ny; j++) {            /* left/right = 7: the operator's row */
+      zdouble f[5];
+      if (p->left == 1 || p->left == 4) {
+        if (side_values(p, 0, j, f)) return -1;
+        for (int d = 0; d < 5; d++) R(d, 0, j) = -(V(d, 0, j) - f[d]);
+      }
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) R(d, 0, j) = 0;
-        if (p->right == 0) R(d, nx - 1, j) = 0;
+        if (p->right == 0 || p->right == 4) R(d, nx - 1, j) = 0;
       }
+    }
   }
   return 0;
 }
 
-/* src/itrbc3d.f90:34-220, 300-310 (same selectors) */
+/* src/itrbc3d.f90:34-376 (same selectors) */
 int orc_itrbc3d(const orc_params *p, zdouble *v) {
   if (!supported(p)) return -1;
   int nx = p->nx, ny = p->ny;
@@ -702,11 +751,21 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
     for (int j = 0; j < ny; j++)
       for (int d = 0; d < 5; d++) V(d, nx - 1, j) = V(d, 0, j);
   } else {
-    for (int j = 0; j < ny; j++)              /* left/right = 7: left as they are */
+    for (int j = 0; j < ny; j++) {            /* left/right = 7: left as they are */
+      zdouble f[5];
+      if (p->left == 1 || p->left == 4) {
+        if (side_values(p, 0, j, f)) return -1;
+        for (int d = 0; d < 5; d++) V(d, 0, j) = f[d];
+      }
+      if (p->right == 4) {
+        if (side_values(p, 1, j, f)) return -1;
+        for (int d = 0; d < 5; d++) V(d, nx - 1, j) = f[d];
+      }
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) V(d, 0, j) = 0;
         if (p->right == 0) V(d, nx - 1, j) = 0;
       }
+    }
   }
   return 0;
 #undef R

@@ -33,6 +33,10 @@ reference's outputs:
                         wamp and x0; lrhs3D, both assembled LHS factors and
                         one implicit step with the sponge
 
+  ref_inflow_<case>.npz left = 1 (acoustic wave), left = 4 with right = 4
+                        (eigenfunction inflow, seeded profiles, complex
+                        lalpha) and left = 4 with wall = wallt = 1, top = 1:
+                        itrBC3D, lrhs3D, rhsBC3D, both factors, two steps
   ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
                         (with wall = 1, wallt = 1) and mse (the boundary set
                         of test/mse-recep/complex.inp: top 1, left 7); the
@@ -41,6 +45,7 @@ reference's outputs:
 
     python tests/golden/gen_golden.py sponge     (the sponge, top = 1 and impl cases only)
     python tests/golden/gen_golden.py sym        (the symmetry cases only)
+    python tests/golden/gen_golden.py inflow     (the left/right inflow cases only)
 """
 from __future__ import annotations
 
@@ -291,6 +296,47 @@ def sym_case(R, name, kw):
     np.savez_compressed(os.path.join(HERE, f"ref_sym_{name}.npz"), **out)
 
 
+INFLOW_CASES = {
+    "left1": dict(left=1),
+    "left4": dict(left=4, right=4),
+    "left4top": dict(left=4, wall=1, wallt=1, top=1),
+}
+
+
+def inflow_case(R, name, kw):
+    """left = 1 (acoustic wave on the left boundary), left = 4 / right = 4
+    (eigenfunction inflow: seeded profiles in the columns of src/gendist.f90's
+    inflow file, a complex lalpha; omega != 0)"""
+    p = Params(nx=13, ny=9, kz=0.4, omega=0.3, **kw)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(37)
+    eig = np.asfortranarray(rng.standard_normal((10, p.ny)))
+    lalpha = 0.31 - 0.02j
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    x0 = float(x.min()) + 0.5
+    R.setup(p, x, y, met, vm, x0=x0)
+    R.set_inflow(eig, lalpha, 0.0)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), x0=np.array(x0), x=x, y=y, met=met, vm=vm, v0=v0, eig=eig,
+               lalpha=np.array(lalpha))
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    rb = np.asfortranarray(r.copy())
+    R.rhsbc3d(rb, np.asfortranarray(v0))
+    out["lrhs"] = r
+    out["rhsbc"] = rb
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    out["dv1"] = R.step(v, True)
+    out["dv2"] = R.step(v, False)
+    out["v2"] = v.copy()
+    np.savez_compressed(os.path.join(HERE, f"ref_inflow_{name}.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -310,6 +356,11 @@ def main():
             sym_case(R, name, kw)
         R.lib.ref_flush()
         return
+    if sys.argv[1:] == ["inflow"]:
+        for name, kw in INFLOW_CASES.items():
+            inflow_case(R, name, kw)
+        R.lib.ref_flush()
+        return
     for name, kw in STEP_CASES.items():
         step_case(R, name, kw, keep_lhs=(name in ("base", "xper")))
     penta_case(R)
@@ -323,6 +374,8 @@ def main():
     impl_case(R, 2, 2, "ref_niter2.npz")
     for name, kw in SYM_CASES.items():
         sym_case(R, name, kw)
+    for name, kw in INFLOW_CASES.items():
+        inflow_case(R, name, kw)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -317,3 +317,43 @@ def test_symmetry_transposed_read_bounds(orc):
     vm = make_mean(p, y)
     with pytest.raises(AssertionError):
         orc.genmtrx(p, vm, met)
+
+
+@pytest.mark.parametrize("case", ["left1", "left4", "left4top"])
+def test_inflow(orc, case):
+    """left = 1 (the acoustic wave on x(1,j): src/rhsbc3d.f90:176-205,
+    src/itrbc3d.f90:219-245, the xi factor's closure row kept), left = 4 and
+    right = 4 (eigenfunction inflow: src/rhsbc3d.f90:207-246,
+    src/itrbc3d.f90:247-318, 352-376, identity rows of src/lhsbc1f3d.f90
+    :414-436) against the reference's routines (ref_inflow_<case>.npz); the
+    boundary values are not those of left = right = 0"""
+    import dataclasses
+    z = load(f"ref_inflow_{case}.npz")
+    p = params_of(z)
+    met, vm, x = (np.asfortranarray(z[k]) for k in ("met", "vm", "x"))
+    v0 = np.asfortranarray(z["v0"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    try:
+        orc.set_inflow(x, vm, float(z["x0"]), z["eig"], complex(z["lalpha"]))
+        if p.top == 1:
+            orc.set_top(x, None, vm, float(z["x0"]))
+        v = np.asfortranarray(v0.copy())
+        orc.itrbc3d(p, v)
+        assert rel(v, z["itrbc"]) < 1e-14
+        assert np.abs(v[:, 0, 1:-1]).max() > 0.1          # the inflow is on the boundary
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        orc.rhsbc3d(p, r, v0)
+        assert rel(r, z["rhsbc"]) < 1e-14
+        for d in (1, 2):
+            assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        p0 = dataclasses.replace(p, left=0, right=0)
+        assert rel(orc.lhs(p0, 1, c, dtl), z["lhs1"]) > 1e-6 or p.left != 1
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+        assert rel(orc.step(p, c, dtl, f, v), z["dv2"]) < 1e-12
+        assert rel(v, z["v2"]) < 1e-12
+    finally:
+        orc.set_inflow()
+        orc.set_top()
