@@ -4,6 +4,7 @@
 #include <math.h>
 #include <string.h>
 #include <string>
+#include <complex>
 #include <utility>
 #include <vector>
 
@@ -35,6 +36,7 @@ struct lns3d_ctx {
   double *coef_block = nullptr;
   double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
   double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
+  double *side_block = nullptr;     // fside (20 ny), x left column staging (ny)
   bool coeffs_ready = false, factored = false;
   bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
   int impl = 1;                     // time scheme (src/impdrv3d.f90:179-199)
@@ -51,9 +53,14 @@ struct lns3d_ctx {
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
   if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
-      (p.left != 0 && p.left != 7) || (p.right != 0 && p.right != 7) || p.linear != 2 ||
+      (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 7) ||
+      (p.right != 0 && p.right != 4 && p.right != 7) || p.linear != 2 ||
       (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) || (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
+  // right = 4 reads the eigenfunction profiles, which the reference reads
+  // only for left = 4 (src/gendist.f90:17)
+  if (!p.xper && p.right == 4 && p.left != 4)
+    return fail(LNS3D_EUNSUPPORTED, "right = 4 needs left = 4 (the eigenfunction inflow)");
   // symmetric ends: src/lhsbs1f3d.f90:17 reads the spanwise cross entries of
   // row (i, j) at point i + ny j (its Ahi(6,ny,nx) shape), which lies on the
   // grid only when nx >= ny (as in the reference's own symmetric case)
@@ -120,6 +127,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->vstate) (void)hipFree(c->vstate);
   if (c->sponge_block) (void)hipFree(c->sponge_block);
   if (c->top_block) (void)hipFree(c->top_block);
+  if (c->side_block) (void)hipFree(c->side_block);
   if (c->vhist) (void)hipFree(c->vhist);
   if (c->vstart) (void)hipFree(c->vstart);
   delete c;
@@ -148,6 +156,7 @@ extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
   if (c->impl != 1) c->p.alfa = c->impl == 3 ? 0.5 : 1.0;
   c->c.sp1 = c->c.sp2 = c->c.spf = c->c.spl = nullptr;
   c->c.ftop = nullptr;
+  c->c.fside = nullptr;
   return 0;
 }
 
@@ -246,6 +255,59 @@ extern "C" int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double
   return 0;
 }
 
+// Left and right boundary values (src/rhsbc3d.f90:176-246, src/itrbc3d.f90
+// :219-376): left = 1 the acoustic wave on the left column (from the mean flow,
+// x (host (nx,ny)) and x0, as the top wave with amplitude 1); left = 4 the
+// eigenfunction profiles eig (host (10,ny): rhor, rhoi, ur, ui, vr, vi, wr,
+// wi, tr, ti, the columns of src/gendist.f90's inflow file); right = 4 the
+// same profiles times exp(i lalpha x(nx,j)), lalpha = (re, im).
+extern "C" int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const double *eig,
+                                const double *lalpha) {
+  if (!c) return fail(LNS3D_EINVAL, "lns3d_set_inflow: null context");
+  const lns3d_params &p = c->p;
+  const bool l1 = !p.xper && p.left == 1, l4 = !p.xper && p.left == 4, r4 = !p.xper && p.right == 4;
+  if (!l1 && !l4 && !r4) return fail(LNS3D_EINVAL, "lns3d_set_inflow: the context has no left = 1, 4 or right = 4");
+  if (((l1 || r4) && !x) || ((l4 || r4) && !eig)) return fail(LNS3D_EINVAL, "lns3d_set_inflow: null argument");
+  if (l4 && p.omega == 0.0)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_set_inflow: left = 4 with omega = 0 varies in time in the reference "
+                                    "(its transient eigenfunction), not implemented");
+  if (l1 && !c->vm_ready)
+    return fail(LNS3D_EINVAL, "lns3d_set_inflow: the left wave needs the mean flow (lns3d_setup first)");
+  const int nx = p.nx, ny = p.ny;
+  if (!c->side_block) LNS3D_CHECK(hipMalloc(&c->side_block, (size_t)21 * ny * sizeof(double)));
+  std::vector<double> hs((size_t)20 * ny, 0.0);
+  if (l4)
+    for (int j = 0; j < ny; j++)
+      for (int d = 0; d < 5; d++) {
+        hs[2 * ((size_t)d * ny + j)] = eig[10 * (size_t)j + 2 * d];
+        hs[2 * ((size_t)d * ny + j) + 1] = eig[10 * (size_t)j + 2 * d + 1];
+      }
+  if (r4) {
+    const std::complex<double> la(lalpha ? lalpha[0] : 0.0, lalpha ? lalpha[1] : 0.0);
+    for (int j = 0; j < ny; j++) {
+      const std::complex<double> e = std::exp(std::complex<double>(0.0, 1.0) * la * x[nx - 1 + (long)nx * j]);
+      for (int d = 0; d < 5; d++) {
+        const std::complex<double> f =
+            std::complex<double>(eig[10 * (size_t)j + 2 * d], eig[10 * (size_t)j + 2 * d + 1]) * e;
+        hs[2 * ((size_t)(5 + d) * ny + j)] = f.real();
+        hs[2 * ((size_t)(5 + d) * ny + j) + 1] = f.imag();
+      }
+    }
+  }
+  LNS3D_CHECK(hipMemcpyAsync(c->side_block, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, c->stream));
+  c->c.fside = c->side_block;
+  if (l1) {
+    std::vector<double> xl((size_t)ny);
+    for (int j = 0; j < ny; j++) xl[j] = x[(long)nx * j];
+    double *xd = c->side_block + 20 * ny;
+    LNS3D_CHECK(hipMemcpyAsync(xd, xl.data(), (size_t)ny * 8, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_left_wave(p, c->c, xd, x0, c->stream);
+    if (rc) return rc;
+  }
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 // Coefficients in the reference's layouts (host): Ah,Bh,Dh,Dhi (5,5,nx,ny),
 // Vh11,Vh12,Vh22,ABhi (6,nx,ny), dtl (nx,ny) -- the arrays src/genmtrx.f90
 // and src/temporal.f90 leave in modules local2d/local3d.
@@ -310,6 +372,8 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
   if (!c || !c->coeffs_ready) return fail(LNS3D_EINVAL, "lns3d_factor: no coefficients");
   if (c->p.top == 1 && !c->c.ftop)
     return fail(LNS3D_EINVAL, "lns3d_factor: top = 1 needs its incoming wave (lns3d_set_top_forcing)");
+  if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.right == 4) && !c->c.fside)
+    return fail(LNS3D_EINVAL, "lns3d_factor: left = 1, 4 and right = 4 need their values (lns3d_set_inflow)");
   int rc = launch_assemble(c->p, c->c, 1, c->px, c->stream);
   if (!rc) rc = penta_factor(&c->px, c->stream);
   if (!rc) rc = launch_assemble(c->p, c->c, 2, c->py, c->stream);
@@ -359,7 +423,12 @@ extern "C" int lns3d_lrhs3d(lns3d_ctx *c, const void *v_dev, void *r_dev) {
 
 extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
   if (!c || !v_dev) return fail(LNS3D_EINVAL, "lns3d_itrbc3d: bad arguments");
-  return launch_itrbc(c->p, c->c.ftop, v_dev, c->stream);
+  // the boundary values these selectors hold the field at must be formed first
+  if (!c->p.yper && c->p.top == 1 && !c->c.ftop)
+    return fail(LNS3D_EINVAL, "lns3d_itrbc3d: top = 1 needs its incoming wave (lns3d_set_top_forcing)");
+  if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.right == 4) && !c->c.fside)
+    return fail(LNS3D_EINVAL, "lns3d_itrbc3d: left = 1, 4 and right = 4 need their values (lns3d_set_inflow)");
+  return launch_itrbc(c->p, c->c, v_dev, c->stream);
 }
 
 static int solve_xy(lns3d_ctx *c, PentaPlan &px, PentaPlan &py, void *r) {
@@ -408,7 +477,7 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
     if (!rc && c->impl == 2) rc = launch_copy(c->vhist, v_dev, n, c->stream);
     if (!rc) rc = solve_xy(c, c->px, c->py, r);
-    if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+    if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
   } else {
     // corrector iterations (src/impdrv3d.f90:173-358): vold stays the field at
     // the start of the step; the factors are those of the first iteration
@@ -417,7 +486,7 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
       const int mode = it == 0 ? (two ? 2 : 1) : (two ? 4 : 3);
       rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, mode, c->stream, c->vhist, c->vstart);
       if (!rc) rc = solve_xy(c, c->px, c->py, r);
-      if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+      if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
     }
     if (!rc && c->impl == 2) std::swap(c->vhist, c->vstart);   // this step's start is the next v2old
   }
@@ -498,7 +567,7 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
   if (iters_out) *iters_out = it;
   if (resid_out) *resid_out = c->last_resid;
   if (!(c->last_resid == c->last_resid)) return fail(LNS3D_ENONFINITE, "refinement produced NaN");
-  return launch_update_bc(c->p, c->c.ftop, v_dev, dv, c->stream);
+  return launch_update_bc(c->p, c->c, v_dev, dv, c->stream);
 }
 
 // the same step, but only the complex-float solve (no refinement): the
@@ -511,7 +580,7 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   void *r = dv_dev ? dv_dev : c->r;
   int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
   if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
-  if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, r, c->stream);
+  if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
   return rc;
 }
 
@@ -541,7 +610,7 @@ extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev,
     rc = launch_af_matvec(c->p, c->c, dv, c->t2, c->t1, c->stream);
     if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);
   }
-  if (!rc) rc = launch_update_bc(c->p, c->c.ftop, v_dev, dv, c->stream);
+  if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, dv, c->stream);
   return rc;
 }
 
@@ -713,7 +782,7 @@ extern "C" int lns3d_advance(lns3d_ctx *c, int nsteps, void *dv_host) {
   if (!c->factored) return fail(LNS3D_ENOTFACTORED, "lns3d_advance: call lns3d_factor first");
   int rc = 0;
   if (c->state_fresh) {
-    rc = launch_itrbc(c->p, c->c.ftop, c->vstate, c->stream);
+    rc = launch_itrbc(c->p, c->c, c->vstate, c->stream);
     c->state_fresh = false;
   }
   for (int k = 0; k < nsteps && !rc; k++) rc = lns3d_step(c, c->vstate, c->r);

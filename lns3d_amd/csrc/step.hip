@@ -501,6 +501,29 @@ __global__ void k_top_forcing(lns3d_params p, const double *vm, const double *xt
   }
 }
 
+// left = 1: the wave on the left column (src/rhsbc3d.f90:176-205,
+// src/itrbc3d.f90:219-245; the top wave's form with amplitude 1)
+__global__ void k_left_wave(lns3d_params p, const double *vm, const double *xleft, double x0,
+                            double *fside) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p.ny) return;
+  const long q = (long)p.nx * j;
+  z f[5];
+  acoustic_wave(p, vm + 5 * q, xleft[j], x0, 1.0, f);
+  for (int m = 0; m < 5; m++) {
+    fside[2 * (m * p.ny + j)] = f[m].re;
+    fside[2 * (m * p.ny + j) + 1] = f[m].im;
+  }
+}
+
+int launch_left_wave(const lns3d_params &p, const Coeffs &c, const double *xleft, double x0,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_left_wave, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (const double *)c.vm,
+                     xleft, x0, c.fside);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
 int launch_top_forcing(const lns3d_params &p, const Coeffs &c, const double *xtop, const double *wamp,
                        double x0, hipStream_t s) {
   hipLaunchKernelGGL(k_top_forcing, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (const double *)c.vm,
@@ -558,7 +581,11 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   // boundary-condition rows (src/lhsbc1f3d.f90, src/lhsbc2f3d.f90); the
   // left/right rows are applied last, then top, then the wall
   const z one{(k == 2 && r == cc) ? 1.0 : 0.0, 0.0};
-  if (!p.xper && ((i == 0 && p.left == 0) || (i == p.nx - 1 && p.right == 0))) return one;
+  // left/right 0 and 4: identity rows (src/lhsbc1f3d.f90:414-436); left 1
+  // keeps the closure row, 7 the symmetric row
+  if (!p.xper && ((i == 0 && (p.left == 0 || p.left == 4)) ||
+                  (i == p.nx - 1 && (p.right == 0 || p.right == 4))))
+    return one;
   if (!p.yper) {
     if (j == p.ny - 1) return one;
     if (j == 0) {
@@ -897,8 +924,15 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
       }
       if (p.xper) {
         if (i == nx - 1) a = z{0, 0};
-      } else if ((i == 0 && p.left == 0) || (i == nx - 1 && p.right == 0)) {
-        a = z{0, 0};              // left/right = 7: the operator's row (rhsbc3d.f90:271)
+      } else {
+        // src/rhsbc3d.f90:172-300: left 0 and right 0, 4 zero; left 1, 4
+        // -(v - the held value); left/right 7 the operator's row
+        if (i == 0 && (p.left == 1 || p.left == 4)) {
+          const z f{c.fside[2 * (m * ny + j)], c.fside[2 * (m * ny + j) + 1]};
+          const z d = vq(m) - f;
+          a = z{-d.re, -d.im};
+        }
+        if ((i == 0 && p.left == 0) || (i == nx - 1 && (p.right == 0 || p.right == 4))) a = z{0, 0};
       }
     }
     sr[5 * l + m] = make_double2(a.re, a.im);
@@ -962,7 +996,9 @@ __global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, double2 *v)
     for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(0, 0);
 }
 
-__global__ void k_itrbc_lr(lns3d_params p, double2 *v) {
+// left and right columns (src/itrbc3d.f90:200-376): periodic copy; 0 zero;
+// left 1 and 4, right 4 the values in fside; 7 (symmetry) left alone
+__global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = p.nx, ny = p.ny;
   if (j >= ny) return;
@@ -970,16 +1006,23 @@ __global__ void k_itrbc_lr(lns3d_params p, double2 *v) {
   for (int d = 0; d < 5; d++) {
     if (p.xper) {
       row[5L * (nx - 1) + d] = row[d];
-    } else {                 // left/right = 7 leave the row alone (itrbc3d.f90:342)
+    } else {
       if (p.left == 0) row[d] = make_double2(0, 0);
+      if (p.left == 1 || p.left == 4)
+        row[d] = make_double2(fside[2 * (d * ny + j)], fside[2 * (d * ny + j) + 1]);
       if (p.right == 0) row[5L * (nx - 1) + d] = make_double2(0, 0);
+      if (p.right == 4)
+        row[5L * (nx - 1) + d] =
+            make_double2(fside[2 * ((5 + d) * ny + j)], fside[2 * ((5 + d) * ny + j) + 1]);
     }
   }
 }
 
-int launch_itrbc(const lns3d_params &p, const double *ftop, void *v, hipStream_t s) {
-  hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, ftop, (double2 *)v);
-  hipLaunchKernelGGL(k_itrbc_lr, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (double2 *)v);
+int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s) {
+  hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (const double *)c.ftop,
+                     (double2 *)v);
+  hipLaunchKernelGGL(k_itrbc_lr, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (const double *)c.fside,
+                     (double2 *)v);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
@@ -1003,10 +1046,10 @@ int launch_axpy(void *x, const void *d, long n, hipStream_t s) {
   return 0;
 }
 
-int launch_update_bc(const lns3d_params &p, const double *ftop, void *v, const void *dv, hipStream_t s) {
+int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, const void *dv, hipStream_t s) {
   int rc = launch_axpy(v, dv, 5L * p.nx * p.ny, s);
   if (rc) return rc;
-  return launch_itrbc(p, ftop, v, s);
+  return launch_itrbc(p, c, v, s);
 }
 
 // ==========================================================================
@@ -1131,7 +1174,8 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   bool ident[5] = {false, false, false, false, false};
   bool wallrow0 = false, wallrow4 = false;
   const int ir = dir == 1 ? posc : i, jr = dir == 1 ? j : posc;
-  const bool lrrow = !p.xper && ((ir == 0 && p.left == 0) || (ir == nx - 1 && p.right == 0));
+  const bool lrrow = !p.xper && ((ir == 0 && (p.left == 0 || p.left == 4)) ||
+                                  (ir == nx - 1 && (p.right == 0 || p.right == 4)));
   if (lrrow) for (int r = 0; r < 5; r++) ident[r] = true;
   if (!p.yper) {
     if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;

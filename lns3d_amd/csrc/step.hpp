@@ -33,6 +33,10 @@ struct Coeffs {
   // top = 1 (src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195): the incoming
   // wave on the top boundary, [5][nx] complex as (re, im) pairs
   double *ftop;
+  // left = 1, 4 and right = 4 (src/rhsbc3d.f90:176-246, src/itrbc3d.f90
+  // :219-376): the values the left and right columns are held at, [2][5][ny]
+  // complex as (re, im) pairs (side 0 left, side 1 right)
+  double *fside;
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);
@@ -43,6 +47,10 @@ int launch_sponge_forcing(const lns3d_params &p, int ispg, const Coeffs &c, cons
                           const double *wamp, double x0, hipStream_t s);
 // the top = 1 wave into c.ftop from the mean flow c.vm, the top row of the
 // physical x (nx, device) and wamp (nx, device; null for 1)
+// the left column's acoustic wave (left = 1) into c.fside side 0, from the
+// mean flow and the physical x of the left column (ny, device)
+int launch_left_wave(const lns3d_params &p, const Coeffs &c, const double *xleft, double x0,
+                     hipStream_t s);
 int launch_top_forcing(const lns3d_params &p, const Coeffs &c, const double *xtop, const double *wamp,
                        double x0, hipStream_t s);
 // assemble the LHS of direction dir (1: xi, 2: eta) into the line-interleaved
@@ -58,8 +66,8 @@ int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const 
                void *work, int with_rbe, hipStream_t s, const void *v2old = nullptr,
                const void *vold = nullptr);
 // ftop: the top = 1 wave (Coeffs::ftop), null for top = 0
-int launch_update_bc(const lns3d_params &p, const double *ftop, void *v, const void *dv, hipStream_t s);
-int launch_itrbc(const lns3d_params &p, const double *ftop, void *v, hipStream_t s);
+int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, const void *dv, hipStream_t s);
+int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s);
 // y = M x with M = Lx * Ly the approximate-factorization operator (FP64,
 // blocks regenerated from the coefficients); tmp holds 5P complex
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,

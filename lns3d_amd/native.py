@@ -64,6 +64,7 @@ SIGNATURES = {
     "lns3d_set_time_scheme": ([VP, CI], CI),
     "lns3d_set_time_history": ([VP, VP, ctypes.c_long], CI),
     "lns3d_set_corrector_iterations": ([VP, CI], CI),
+    "lns3d_set_inflow": ([VP, VP, ctypes.c_double, VP, VP], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -183,6 +184,16 @@ class Step:
     def set_time_scheme(self, impl):
         """the deck's impl: 1 backward Euler, 2 two-step backward, 3 alfa 1/2"""
         check(lib().lns3d_set_time_scheme(self.h, impl), "lns3d_set_time_scheme")
+
+    def set_inflow(self, x=None, x0=0.0, eig=None, lalpha=0j):
+        """left = 1 / left = 4 / right = 4 boundary values: x (nx, ny), eig
+        (10, ny) (rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti), lalpha complex"""
+        import numpy as np
+        xa = None if x is None else np.asfortranarray(x, dtype=np.float64)
+        ea = None if eig is None else np.asfortranarray(eig, dtype=np.float64)
+        la = np.array([complex(lalpha).real, complex(lalpha).imag])
+        hp = lambda a: None if a is None else a.ctypes.data
+        check(lib().lns3d_set_inflow(self.h, hp(xa), float(x0), hp(ea), hp(la)), "lns3d_set_inflow")
 
     def set_corrector_iterations(self, niter):
         check(lib().lns3d_set_corrector_iterations(self.h, niter), "lns3d_set_corrector_iterations")

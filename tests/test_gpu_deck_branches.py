@@ -14,7 +14,8 @@ incoming wave on the top boundary, src/rhsbc3d.f90:129-161,
 src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz.
 The time schemes impl = 2 and 3 (lns3d_set_time_scheme) and the corrector
 iterations (lns3d_set_corrector_iterations) over three of the reference's
-steps."""
+steps.  The left/right inflow branches (lns3d_set_inflow: left = 1, left = 4,
+right = 4) against tests/golden/ref_inflow_<case>.npz."""
 import os
 
 import numpy as np
@@ -247,4 +248,100 @@ def test_corrector_iterations_against_reference(case):
     for k in (1, 2, 3):
         st.step(vb, dvd)
     assert rel(host(vb), z["v3"]) > 1e-8
+    st.close()
+
+
+@pytest.mark.parametrize("case", ["left1", "left4", "left4top"])
+def test_inflow_against_reference(case):
+    """lns3d_set_inflow: left = 1 (the acoustic wave on the left column), left =
+    4 with right = 4 (eigenfunction inflow) and left = 4 with wall = wallt =
+    1, top = 1, against the reference's itrBC3D, lrhs3D, both assembled
+    factors and two steps (tests/golden/ref_inflow_<case>.npz), 1e-12; the
+    refined step reaches the same increment; missing values are refused
+    before any kernel reads them"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, f"ref_inflow_{case}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"inflow {case}: left={p.left} right={p.right} top={p.top} {p.nx}x{p.ny}", flush=True)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    x0, la = float(z["x0"]), complex(z["lalpha"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t, shape):
+        return t.cpu().numpy().reshape(shape, order="F")
+
+    def make(precision=0):
+        s = Step(p, precision=precision)
+        s.setup(met, vm)
+        if p.top == 1:
+            s.set_top_forcing(z["x"], None, x0)
+        return s
+
+    st = make()
+    v0 = z["v0"]
+    vd = dev(v0)
+    with pytest.raises(LNS3DError):
+        st.itrbc3d(vd)                       # the left values are not there yet
+    with pytest.raises(LNS3DError):
+        st.factor()
+    st.set_inflow(z["x"], x0, z["eig"], la)
+    st.itrbc3d(vd)
+    assert rel(host(vd, v0.shape), z["itrbc"]) < TOL
+    vd = dev(v0)
+    rd = torch.zeros_like(vd)
+    st.lrhs3d(vd, rd)
+    assert rel(host(rd, v0.shape), z["lrhs"]) < TOL
+    for d in (1, 2):
+        m_dev = torch.zeros(125 * p.nx * p.ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        assert rel(host(m_dev, z[f"lhs{d}"].shape), z[f"lhs{d}"]) < TOL, d
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    for k in (1, 2):
+        st.step(vd, dvd)
+        e = rel(host(dvd, v0.shape), z[f"dv{k}"])
+        print(f"  step {k}: dv rel {e:.2e}", flush=True)
+        assert e < TOL, k
+    assert rel(host(vd, v0.shape), z["v2"]) < TOL
+    st.close()
+
+    st32 = make(precision=1)
+    st32.set_inflow(z["x"], x0, z["eig"], la)
+    st32.factor()
+    vb = dev(z["itrbc"])
+    dvb = torch.zeros_like(vb)
+    its, res = st32.step_refined(vb, dvb, tol=1e-12, max_iters=10)
+    e = rel(host(dvb, v0.shape), z["dv1"])
+    print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
+    assert res <= 1e-12 and e < 1e-10
+    st32.close()
+
+
+def test_inflow_refusals():
+    import dataclasses
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params, make_grid, make_mean
+    from lns3d_amd.native import Step, LNS3DError
+    with pytest.raises(LNS3DError):
+        Step(Params(nx=13, ny=9, right=4))          # right = 4 reads the left = 4 profiles
+    p = Params(nx=13, ny=9, left=4, omega=0.0)
+    x, y, met = make_grid(p)
+    st = Step(p)
+    st.setup(met, make_mean(p, y))
+    with pytest.raises(LNS3DError):                 # omega = 0: a time-dependent inflow
+        st.set_inflow(x, 0.0, np.ones((10, p.ny)))
+    st.close()
+    st = Step(dataclasses.replace(p, left=0, omega=0.3))
+    with pytest.raises(LNS3DError):                 # no inflow boundary
+        st.set_inflow(x, 0.0, np.ones((10, p.ny)))
     st.close()
