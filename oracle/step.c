@@ -264,11 +264,15 @@ static int supported(const orc_params *p) {
 /* ---- sponge: the module state src/lrhs3d.f90:222-289 and the LHS routines
  *      read (spg, spg2 of src/sponge.f90:20, wamp, x0, omega) ----------------- */
 static orc_sponge g_spg;          /* ispg = 0: no sponge */
+static struct { const double *x, *vm, *eig; double x0; zdouble lalpha; } g_inf;   /* see orc_set_inflow */
 
 int orc_set_sponge(const orc_sponge *s) {
   if (!s || s->ispg == 0) { memset(&g_spg, 0, sizeof g_spg); return 0; }
-  if (s->ispg < 0 || s->ispg == 4 || s->ispg > 5 || !s->spg) return -1;
+  if (s->ispg < 0 || s->ispg > 5 || !s->spg) return -1;
   if (s->ispg >= 2 && !s->spg2) return -1;
+  /* ispg = 4 sponges towards the eigenfunction inflow continued downstream:
+   * it needs the profiles and lalpha (orc_set_inflow first) and x */
+  if (s->ispg == 4 && (!s->x || !g_inf.eig)) return -1;
   if ((s->ispg == 2 || s->ispg == 3) && (!s->x || !s->vm)) return -1;
   if (s->ispg == 3 && !s->wamp) return -1;
   g_spg = *s;
@@ -308,8 +312,6 @@ static void acoustic_wave(const orc_params *p, const double *vml, double xl, dou
  * :207-246, src/itrbc3d.f90:247-318, omega != 0 so no transient factor) and
  * right = 4 (the same profiles times exp(i lalpha x(nx,j)):
  * src/itrbc3d.f90:352-376) */
-static struct { const double *x, *vm, *eig; double x0; zdouble lalpha; } g_inf;
-
 int orc_set_inflow(const double *x, const double *vm, double x0, const double *eig,
                    const double *lalpha) {
   g_inf.x = x; g_inf.vm = vm; g_inf.x0 = x0; g_inf.eig = eig;
@@ -363,6 +365,16 @@ static void rhs_sponge(const orc_params *p, long q, int i, const zdouble *vq, zd
     for (int m = 0; m < 5; m++) rr[m] = rr[m] + g_spg.spg[q] * vq[m];
   } else if (ispg > 4) {
     for (int m = 0; m < 5; m++) rr[m] = rr[m] + (g_spg.spg[q] + g_spg.spg2[q]) * vq[m];
+  } else if (ispg == 4) {
+    /* src/sponge.f90:536-606 (cspg_it; omega != 0, so no transient factor):
+     * (spg + spg2) (v - cvic), cvic = profile(j) exp(i lalpha x(i,j)) */
+    const int j = (int)(q / p->nx);
+    const zdouble e = cexp(_Complex_I * g_inf.lalpha * g_spg.x[q]);
+    const double s = g_spg.spg[q] + g_spg.spg2[q];
+    for (int m = 0; m < 5; m++) {
+      const zdouble cv = (g_inf.eig[10 * j + 2 * m] + _Complex_I * g_inf.eig[10 * j + 2 * m + 1]) * e;
+      rr[m] = rr[m] + s * (vq[m] - cv);
+    }
   } else if (ispg == 2 || ispg == 3) {
     for (int m = 0; m < 5; m++) rr[m] = rr[m] + g_spg.spg[q] * vq[m];       /* outflow */
     zdouble f[5];                                                           /* inflow */

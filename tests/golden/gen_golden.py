@@ -37,6 +37,8 @@ reference's outputs:
                         (eigenfunction inflow, seeded profiles, complex
                         lalpha) and left = 4 with wall = wallt = 1, top = 1:
                         itrBC3D, lrhs3D, rhsBC3D, both factors, two steps
+  ref_sponge_4.npz      ispg = 4 (cspg_it) with left = 4: seeded profiles,
+                        lalpha; lrhs3D, both factors, two steps
   ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
                         (with wall = 1, wallt = 1) and mse (the boundary set
                         of test/mse-recep/complex.inp: top 1, left 7); the
@@ -45,7 +47,7 @@ reference's outputs:
 
     python tests/golden/gen_golden.py sponge     (the sponge, top = 1 and impl cases only)
     python tests/golden/gen_golden.py sym        (the symmetry cases only)
-    python tests/golden/gen_golden.py inflow     (the left/right inflow cases only)
+    python tests/golden/gen_golden.py inflow     (the left/right inflow and ispg = 4 cases only)
 """
 from __future__ import annotations
 
@@ -337,6 +339,40 @@ def inflow_case(R, name, kw):
     np.savez_compressed(os.path.join(HERE, f"ref_inflow_{name}.npz"), **out)
 
 
+def sponge4_case(R):
+    """ispg = 4 (src/sponge.f90:536-606, cspg_it): the sponge towards the
+    eigenfunction inflow continued downstream, cvic = profile(j) exp(i lalpha
+    x(i,j)); left = 4 so that the reference holds the profiles
+    (src/gendist.f90:17); omega != 0"""
+    p = Params(nx=13, ny=9, omega=0.3, kz=0.4, left=4)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(41)
+    eig = np.asfortranarray(rng.standard_normal((10, p.ny)))
+    lalpha = 0.27 - 0.015j
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    spg, spg2, wamp = sponge_profiles(p, x, 3)
+    x0 = float(x.min()) + 0.5
+    R.setup(p, x, y, met, vm, ispg=4, x0=x0)
+    R.set_inflow(eig, lalpha, 0.0)
+    R.set_sponge(spg, spg2, wamp)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), ispg=np.array(4), x0=np.array(x0), x=x, y=y, met=met, vm=vm,
+               v0=v0, spg=spg, spg2=spg2, eig=eig, lalpha=np.array(lalpha))
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    out["lrhs"] = r
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    out["dv1"] = R.step(v, True)
+    out["dv2"] = R.step(v, False)
+    out["v2"] = v.copy()
+    np.savez_compressed(os.path.join(HERE, "ref_sponge_4.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -359,6 +395,7 @@ def main():
     if sys.argv[1:] == ["inflow"]:
         for name, kw in INFLOW_CASES.items():
             inflow_case(R, name, kw)
+        sponge4_case(R)
         R.lib.ref_flush()
         return
     for name, kw in STEP_CASES.items():
@@ -376,6 +413,7 @@ def main():
         sym_case(R, name, kw)
     for name, kw in INFLOW_CASES.items():
         inflow_case(R, name, kw)
+    sponge4_case(R)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -357,3 +357,33 @@ def test_inflow(orc, case):
     finally:
         orc.set_inflow()
         orc.set_top()
+
+
+def test_sponge4(orc):
+    """ispg = 4 (src/sponge.f90:536-606, cspg_it: (spg + spg2) (v - cvic),
+    cvic = profile(j) exp(i lalpha x(i,j))), with left = 4 holding the
+    profiles, against the reference's lrhs3D, factors and two steps
+    (ref_sponge_4.npz); the step without the sponge ends elsewhere"""
+    z = load("ref_sponge_4.npz")
+    p = params_of(z)
+    met, vm, x = (np.asfortranarray(z[k]) for k in ("met", "vm", "x"))
+    v0 = np.asfortranarray(z["v0"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    try:
+        orc.set_inflow(x, vm, float(z["x0"]), z["eig"], complex(z["lalpha"]))
+        r0, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r0, z["lrhs"]) > 1e-6
+        orc.set_sponge(4, z["spg"], z["spg2"], x, None, vm, float(z["x0"]))
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        for d in (1, 2):
+            assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        v = np.asfortranarray(z["itrbc"].copy())
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+        assert rel(orc.step(p, c, dtl, f, v), z["dv2"]) < 1e-12
+        assert rel(v, z["v2"]) < 1e-12
+    finally:
+        orc.set_sponge(0)
+        orc.set_inflow()
