@@ -31,12 +31,13 @@ const char *lns3d_last_error(void);
  * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
  * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left in
  * {0,1,4,7} and right in {0,4,7} (1: acoustic wave, 4: eigenfunction inflow,
- * both through lns3d_set_inflow; right 4 with left 4 only; 7: symmetry, src/lhsbs1f3d.f90 and the symmetric closures
- * of src/cgrad.f90, cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
+ * both through lns3d_set_inflow; right 4 with left 4 only; 7: symmetry,
+ * src/lhsbs1f3d.f90 and the symmetric closures of src/cgrad.f90,
+ * cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
  * the reference's symmetric rows read point i + ny j); linear = 2; mattyp in
  * {0,2,3}; loctime in {0,1};
  * Navier-Stokes (viscous) boundary rows; eps_e = 0; sponges through
- * lns3d_set_sponge (ispg 1, 2, 3, 5).  Other values are rejected with
+ * lns3d_set_sponge (ispg 1 to 5).  Other values are rejected with
  * LNS3D_EUNSUPPORTED. */
 typedef struct {
   int nx, ny;
@@ -132,8 +133,10 @@ int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm);
  * spg2, x are host (nx,ny) arrays as src/sponge.f90:20 (init_sponge) forms
  * them.  Call after lns3d_setup (the forcing uses the mean flow and the
  * context's omega, Ma, Re, Pr, gamma) and before lns3d_factor; again after
- * lns3d_ctx_set_params.  ispg = 4 (src/sponge.f90:536) returns
- * LNS3D_EUNSUPPORTED. */
+ * lns3d_ctx_set_params.  ispg = 4 (src/sponge.f90:536-606, cspg_it):
+ * (spg + spg2)(v - cvic) toward the eigenfunction inflow continued
+ * downstream, cvic = profile(j) exp(i lalpha x(i,j)); it needs left = 4 with
+ * its profiles and lalpha given first (lns3d_set_inflow), x, and omega != 0. */
 int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const double *spg2, const double *x,
                      const double *wamp, double x0);
 /* top = 1 (src/rhsbc3d.f90:129-161, src/itrbc3d.f90:145-195): the incoming

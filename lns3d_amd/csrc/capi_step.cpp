@@ -37,6 +37,8 @@ struct lns3d_ctx {
   double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
   double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
   double *side_block = nullptr;     // fside (20 ny), x left column staging (ny)
+  std::vector<double> eig;          // left = 4 profiles (10 ny), kept for ispg = 4
+  double lalpha[2] = {0.0, 0.0};
   bool coeffs_ready = false, factored = false;
   bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
   int impl = 1;                     // time scheme (src/impdrv3d.f90:179-199)
@@ -157,6 +159,7 @@ extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
   c->c.sp1 = c->c.sp2 = c->c.spf = c->c.spl = nullptr;
   c->c.ftop = nullptr;
   c->c.fside = nullptr;
+  c->eig.clear();
   return 0;
 }
 
@@ -191,16 +194,16 @@ extern "C" int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const
                                 const double *x, const double *wamp, double x0) {
   if (!c) return fail(LNS3D_EINVAL, "lns3d_set_sponge: null context");
   if (ispg < 0 || ispg > 5) return fail(LNS3D_EINVAL, "lns3d_set_sponge: ispg must be 0..5");
-  if (ispg == 4)
-    return fail(LNS3D_EUNSUPPORTED, "lns3d_set_sponge: ispg = 4 (iterative sponge, src/sponge.f90:536) "
-                                    "is not implemented");
+  if (ispg == 4 && (c->p.left != 4 || c->eig.empty() || c->p.omega == 0.0))
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_set_sponge: ispg = 4 (src/sponge.f90:536, cspg_it) needs left = 4 "
+                                    "with its profiles (lns3d_set_inflow first) and omega != 0");
   c->factored = false;
   if (ispg == 0) {
     c->c.sp1 = c->c.sp2 = c->c.spf = c->c.spl = nullptr;
     return 0;
   }
   const bool inflow = ispg == 2 || ispg == 3;
-  if (!spg || (ispg >= 2 && !spg2) || (inflow && !x) || (ispg == 3 && !wamp))
+  if (!spg || (ispg >= 2 && !spg2) || ((inflow || ispg == 4) && !x) || (ispg == 3 && !wamp))
     return fail(LNS3D_EINVAL, "lns3d_set_sponge: a profile this ispg needs is null");
   if (inflow && !c->vm_ready)
     return fail(LNS3D_EINVAL, "lns3d_set_sponge: the inflow forcing needs the mean flow (lns3d_setup first)");
@@ -219,6 +222,28 @@ extern "C" int lns3d_set_sponge(lns3d_ctx *c, int ispg, const double *spg, const
   c->c.sp1 = sp1;
   c->c.spl = spl;
   c->c.sp2 = c->c.spf = nullptr;
+  std::vector<double> hf;
+  if (ispg == 4) {
+    // cspg_it (src/sponge.f90:536-606): (spg + spg2) (v - cvic), cvic =
+    // profile(j) exp(i lalpha x(i,j)); the sp2 (v - f) form with sp2 = hl
+    const int nx = c->p.nx;
+    const std::complex<double> la(c->lalpha[0], c->lalpha[1]), I(0.0, 1.0);
+    hf.assign((size_t)10 * P, 0.0);
+    for (long q = 0; q < P; q++) {
+      const long j = q / nx;
+      const std::complex<double> e = std::exp(I * la * x[q]);
+      for (int m = 0; m < 5; m++) {
+        const std::complex<double> f = std::complex<double>(c->eig[10 * j + 2 * m], c->eig[10 * j + 2 * m + 1]) * e;
+        hf[2 * ((size_t)m * P + q)] = f.real();
+        hf[2 * ((size_t)m * P + q) + 1] = f.imag();
+      }
+    }
+    LNS3D_CHECK(hipMemcpyAsync(sp2, hl.data(), (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    LNS3D_CHECK(hipMemcpyAsync(spf, hf.data(), (size_t)P * 80, hipMemcpyHostToDevice, c->stream));
+    c->c.sp1 = nullptr;
+    c->c.sp2 = sp2;
+    c->c.spf = spf;
+  }
   if (inflow) {
     LNS3D_CHECK(hipMemcpyAsync(sp2, spg2, (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
     LNS3D_CHECK(hipMemcpyAsync(xd, x, (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
@@ -296,6 +321,11 @@ extern "C" int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const 
   }
   LNS3D_CHECK(hipMemcpyAsync(c->side_block, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, c->stream));
   c->c.fside = c->side_block;
+  if (l4) {                          // kept for the ispg = 4 sponge
+    c->eig.assign(eig, eig + 10 * (size_t)ny);
+    c->lalpha[0] = lalpha ? lalpha[0] : 0.0;
+    c->lalpha[1] = lalpha ? lalpha[1] : 0.0;
+  }
   if (l1) {
     std::vector<double> xl((size_t)ny);
     for (int j = 0; j < ny; j++) xl[j] = x[(long)nx * j];

@@ -9,7 +9,8 @@ sponge module state (tests/golden/ref_sponge_<ispg>.npz, written by
 tests/golden/gen_golden.py from the compiled reference), ispg = 1, 2, 3, 5.
 Tolerance 1e-12 relative, max norm.  The refined step (complex-float factors,
 whose residual operator carries the sponge too) reaches the complex-double
-increment; ispg = 4 is rejected.  top = 1 (lns3d_set_top_forcing: the same
+increment.  ispg = 4 (cspg_it, with left = 4 holding the profiles) against
+ref_sponge_4.npz, and refused without left = 4.  top = 1 (lns3d_set_top_forcing: the same
 incoming wave on the top boundary, src/rhsbc3d.f90:129-161,
 src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz.
 The time schemes impl = 2 and 3 (lns3d_set_time_scheme) and the corrector
@@ -88,7 +89,70 @@ def test_sponge_against_reference(ispg):
     st32.close()
 
 
-def test_sponge_rejects_iterative_form():
+def test_sponge4_against_reference():
+    """ispg = 4 (cspg_it, src/sponge.f90:536-606) with left = 4: refused
+    before the profiles are given; then lrhs3D, both factors and two steps of
+    the reference (tests/golden/ref_sponge_4.npz) at 1e-12, and the refined
+    step"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, "ref_sponge_4.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"sponge ispg=4 left=4 {p.nx}x{p.ny}", flush=True)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    x0, la = float(z["x0"]), complex(z["lalpha"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t, shape):
+        return t.cpu().numpy().reshape(shape, order="F")
+
+    def make(precision=0):
+        s = Step(p, precision=precision)
+        s.setup(met, vm)
+        return s
+
+    st = make()
+    with pytest.raises(LNS3DError):
+        st.set_sponge(4, z["spg"], z["spg2"], z["x"], None, x0)     # no profiles yet
+    st.set_inflow(z["x"], x0, z["eig"], la)
+    st.set_sponge(4, z["spg"], z["spg2"], z["x"], None, x0)
+    v0 = z["v0"]
+    vd = dev(v0)
+    rd = torch.zeros_like(vd)
+    st.lrhs3d(vd, rd)
+    assert rel(host(rd, v0.shape), z["lrhs"]) < TOL
+    for d in (1, 2):
+        m_dev = torch.zeros(125 * p.nx * p.ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        assert rel(host(m_dev, z[f"lhs{d}"].shape), z[f"lhs{d}"]) < TOL, d
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    for k in (1, 2):
+        st.step(vd, dvd)
+        assert rel(host(dvd, v0.shape), z[f"dv{k}"]) < TOL, k
+    assert rel(host(vd, v0.shape), z["v2"]) < TOL
+    st.close()
+    st32 = make(precision=1)
+    st32.set_inflow(z["x"], x0, z["eig"], la)
+    st32.set_sponge(4, z["spg"], z["spg2"], z["x"], None, x0)
+    st32.factor()
+    vb = dev(z["itrbc"])
+    dvb = torch.zeros_like(vb)
+    its, res = st32.step_refined(vb, dvb, tol=1e-12, max_iters=10)
+    e = rel(host(dvb, v0.shape), z["dv1"])
+    print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
+    assert res <= 1e-12 and e < 1e-10
+    st32.close()
+
+
+def test_sponge4_needs_left4():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
@@ -97,7 +161,7 @@ def test_sponge_rejects_iterative_form():
     st = Step(p)
     st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
     with pytest.raises(LNS3DError):
-        st.set_sponge(4, z["spg"], z["spg2"])
+        st.set_sponge(4, z["spg"], z["spg2"], z["x"])
     st.close()
 
 
