@@ -90,6 +90,8 @@ int orc_set_top(const double *x, const double *wamp, const double *vm, double x0
  * tr, ti; lalpha (re, im) for right = 4 (NULL: 0) */
 int orc_set_inflow(const double *x, const double *vm, double x0, const double *eig,
                    const double *lalpha);
+/* smoother eps_e with the damping profile buff (nx,ny) (eps_e 0: off) */
+int orc_set_smoother(double eps_e, const double *buff);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
                zdouble *r, zdouble *work /* 5 arrays of (5,nx,ny) */);
 void orc_rbe3d(const orc_params *p, zdouble *r, const zdouble *v,

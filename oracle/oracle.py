@@ -79,6 +79,13 @@ class Oracle:
         self.lib.orc_set_top(*(_ptr(a) for a in arrs), x0)
         self._top_keep = arrs
 
+    def set_smoother(self, eps_e=0.0, buff=None):
+        """smoother eps_e with the damping profile buff (nx, ny); 0: off"""
+        b = None if buff is None else np.asfortranarray(buff, dtype=np.float64)
+        self.lib.orc_set_smoother.argtypes = [ctypes.c_double, VP]
+        assert self.lib.orc_set_smoother(float(eps_e), _ptr(b)) == 0
+        self._smo_keep = b
+
     def set_inflow(self, x=None, vm=None, x0=0.0, eig=None, lalpha=0j):
         """left = 1 / left = 4 / right = 4 inflow data (None: cleared); eig
         (10, ny): rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti"""
@@ -265,6 +272,12 @@ class Reference:
         la = np.array([complex(lalpha).real, complex(lalpha).imag])
         lo = np.array([complex(lomega).real, complex(lomega).imag])
         self.lib.ref_set_inflow(_ptr(np.asfortranarray(eig, dtype=np.float64)), _ptr(la), _ptr(lo))
+
+    def set_smoother(self, eps_e, buff):
+        """the deck's eps_e and the damping profile buff (nx, ny)"""
+        self.lib.ref_set_smoother.argtypes = [ctypes.POINTER(ctypes.c_double), VP]
+        self._buff_keep = np.asfortranarray(buff, dtype=np.float64)
+        self.lib.ref_set_smoother(ctypes.byref(ctypes.c_double(eps_e)), _ptr(self._buff_keep))
 
     def set_niter(self, n):
         """corrector iterations per step (the deck's niter)"""

@@ -147,6 +147,20 @@ subroutine ref_set_inflow(eig, la, lo) bind(C, name="ref_set_inflow")
   lalpha = cmplx(la(1), la(2)); lomega = cmplx(lo(1), lo(2))
 end subroutine ref_set_inflow
 
+! smoother (src/smoother3d.f90 and the implicit damping terms): eps_e and the
+! damping profile buff that src/buffer.f90 (init_buffer) forms from ibuff
+subroutine ref_set_smoother(eps, buffl) bind(C, name="ref_set_smoother")
+  use iso_c_binding
+  use global
+  use buffer
+  implicit none
+  real(c_double), intent(in) :: eps, buffl(nx,ny)
+  eps_e = eps
+  if (allocated(buff)) deallocate(buff)
+  allocate(buff(nx,ny))
+  buff = buffl
+end subroutine ref_set_smoother
+
 ! coefficient generation (src/genmtrx.f90) and time step (src/temporal.f90)
 subroutine ref_genmtrx() bind(C, name="ref_genmtrx")
   use global

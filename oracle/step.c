@@ -384,7 +384,96 @@ static void rhs_sponge(const orc_params *p, long q, int i, const zdouble *vq, zd
   }
 }
 
-/* ---- src/lrhs3d.f90:45-285 (point operator, eps_e = 0) -------------------- */
+/* ---- smoother (eps_e != 0): src/smoother3d.f90 in the residual and the
+ *      implicit damping of src/lhs1f3d.f90:169-201, lhs2f3d.f90:171-203,
+ *      lhsbt1f3d.f90:284-304, 451-471, lhsbt2f3d.f90:277-297, 431-451 and
+ *      lhsbs1f3d.f90; buff is the damping profile init_buffer forms ------ */
+static struct { double eps; const double *buff; } g_smo;
+
+int orc_set_smoother(double eps_e, const double *buff) {
+  if (eps_e != 0.0 && !buff) return -1;
+  g_smo.eps = eps_e;
+  g_smo.buff = eps_e != 0.0 ? buff : NULL;
+  return 0;
+}
+
+static const double fa[7] = {-1.666666666666667E-01, 2.0, -6.5, 9.333333333333333E+00, -6.5, 2.0,
+                             -1.666666666666667E-01};
+static const double fb[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
+
+/* the dissipation of src/smoother3d.f90 along one line at position p of N
+ * (value x(k) of point k of the line, component e), without eps buff:
+ * order 2 at the second point from an end, 4 at the third, the 7-point
+ * sixth difference inside; next to a symmetric end (sym) the 7-point form
+ * with reflected points; 0 on the end points */
+#define SMOOTH_LINE(X)                                                                       \
+  do {                                                                                       \
+    if (sym_row(sym, p, N)) {                                                                \
+      zdouble acc = 0; int first = 1;                                                        \
+      for (int k = 0; k < 7; k++) {                                                          \
+        int q = p - 3 + k; double sg = reflect(&q, N, e);                                    \
+        zdouble t = (sg * fa[k]) * X(q);                                                     \
+        acc = first ? t : acc + t; first = 0;                                                \
+      }                                                                                      \
+      out = acc;                                                                             \
+    } else if (p == 1 || p == N - 2) {                                                       \
+      out = -1.0 * X(p - 1) + 2.0 * X(p) - 1.0 * X(p + 1);                                   \
+    } else if (p == 2 || p == N - 3) {                                                       \
+      out = X(p - 2) - 4.0 * X(p - 1) + 6.0 * X(p) - 4.0 * X(p + 1) + 1.0 * X(p + 2);         \
+    } else if (p >= 3 && p <= N - 4) {                                                       \
+      out = fa[0] * X(p - 3) + fa[1] * X(p - 2) + fa[2] * X(p - 1) + fa[3] * X(p) +          \
+            fa[4] * X(p + 1) + fa[5] * X(p + 2) + fa[6] * X(p + 3);                          \
+    } else {                                                                                 \
+      out = 0; has = 0;                                                                      \
+    }                                                                                        \
+  } while (0)
+
+static void rhs_smoother(const orc_params *p_, long q, const zdouble *v, zdouble rr[5]) {
+  if (g_smo.eps == 0.0) return;
+  const int nx = p_->nx, ny = p_->ny, i = (int)(q % nx), j = (int)(q / nx);
+  const double eb = g_smo.eps * g_smo.buff[q];
+  for (int e = 0; e < 5; e++) {
+    zdouble out; int has = 1;
+    {                                            /* eta (no symmetry there) */
+      int p = j, N = ny, sym = 0;
+#define XE(k) v[5 * (i + (long)nx * (k)) + e]
+      SMOOTH_LINE(XE);
+#undef XE
+      if (has) rr[e] = rr[e] + eb * out;
+    }
+    has = 1;
+    {                                            /* xi */
+      int p = i, N = nx, sym = xsym(p_);
+#define XX(k) v[5 * ((k) + (long)nx * j) + e]
+      SMOOTH_LINE(XX);
+#undef XX
+      if (has) rr[e] = rr[e] + eb * out;
+    }
+  }
+}
+
+/* implicit damping weight of slot k (point pos + k - 2 in the interior
+ * layout) on the diagonal of component cc, for a row of a line of N points:
+ * fb in the interior, the second difference on the second and second-last
+ * rows, nothing on the end rows; symmetric rows (sym) reflect fb; fb on
+ * every row of a periodic line (src/lhsbt1f3d.f90:30 returns for xper) */
+static double damp_weight(int pos, int N, int per, int sym, int k, int cc) {
+  if (per) return fb[k];                         /* every row of a periodic line */
+  if (sym_row(sym, pos, N) && (pos <= 1 || pos >= N - 2)) {
+    double w = 0.0;
+    for (int o = 0; o < 5; o++) {
+      int q = pos + o - 2;
+      double s = reflect(&q, N, cc);
+      if (q - pos + 2 == k) w += s * fb[o];
+    }
+    return w;
+  }
+  if (pos == 0 || pos == N - 1) return 0.0;
+  if (pos == 1 || pos == N - 2) return k == 1 ? -1.0 : k == 2 ? 2.0 : k == 3 ? -1.0 : 0.0;
+  return fb[k];
+}
+
+/* ---- src/lrhs3d.f90:45-293 (point operator) ------------------------------ */
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
                zdouble *r, zdouble *work) {
   if (!supported(p)) return -1;
@@ -431,6 +520,7 @@ int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
       rr[m] = rr[m] + ci * (Di[m] * vq[0] + Di[m + 5] * vq[1] + Di[m + 10] * vq[2] +
                            Di[m + 15] * vq[3] + Di[m + 20] * vq[4]);
     rhs_sponge(p, q, (int)(q % nx), vq, rr);
+    rhs_smoother(p, q, v, rr);                   /* src/lrhs3d.f90:293 */
     for (int m = 0; m < 5; m++) r[5 * q + m] = rr[m];
   }
   return 0;
@@ -624,6 +714,19 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
                     c->Dh + 25 * pt, c->Dhi + 25 * pt, Vh + 6 * pt, alfa);
       }
     }
+  }
+
+  if (g_smo.eps != 0.0) {                      /* implicit damping, all rows */
+    const int N = dir == 1 ? nx : ny, sym = dir == 1 ? xsym(p) : 0;
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long pt = i + (long)nx * j;
+        const int pos = dir == 1 ? i : j;
+        const double f = alfa * dtl[pt] * g_smo.eps * g_smo.buff[pt];
+        for (int k = 0; k < 5; k++)
+          for (int r = 0; r < 5; r++)
+            MAT(k, r, r) += f * damp_weight(pos, N, dir == 1 ? p->xper : p->yper, sym, k, r);
+      }
   }
 
   /* boundary conditions */

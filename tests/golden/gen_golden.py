@@ -39,6 +39,9 @@ reference's outputs:
                         itrBC3D, lrhs3D, rhsBC3D, both factors, two steps
   ref_sponge_4.npz      ispg = 4 (cspg_it) with left = 4: seeded profiles,
                         lalpha; lrhs3D, both factors, two steps
+  ref_smooth_<case>.npz the smoother eps_e (src/smoother3d.f90, implicit
+                        damping) with a seeded buff: base, symmetric ends,
+                        periodic x, periodic y; lrhs3D, factors, two steps
   ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
                         (with wall = 1, wallt = 1) and mse (the boundary set
                         of test/mse-recep/complex.inp: top 1, left 7); the
@@ -48,6 +51,7 @@ reference's outputs:
     python tests/golden/gen_golden.py sponge     (the sponge, top = 1 and impl cases only)
     python tests/golden/gen_golden.py sym        (the symmetry cases only)
     python tests/golden/gen_golden.py inflow     (the left/right inflow and ispg = 4 cases only)
+    python tests/golden/gen_golden.py smooth     (the smoother cases only)
 """
 from __future__ import annotations
 
@@ -373,6 +377,45 @@ def sponge4_case(R):
     np.savez_compressed(os.path.join(HERE, "ref_sponge_4.npz"), **out)
 
 
+SMOOTH_CASES = {
+    "base": dict(),
+    "sym": dict(left=7, right=7, wall=1, wallt=1),
+    "xper": dict(xper=1),
+    "yper": dict(yper=1),
+}
+
+
+def smooth_case(R, name, kw):
+    """the smoother eps_e != 0 (src/smoother3d.f90 and the implicit damping
+    of the factors) with a seeded damping profile buff in [0.5, 1] (as
+    init_buffer's profiles lie in [0, 1]); the mean flow varies along x"""
+    p = Params(nx=14, ny=11, kz=0.4, omega=0.3, **kw)
+    x, y, met = make_grid(p)
+    xs = x / x.max()
+    vm = np.asfortranarray(make_mean(p, y) * (1.0 + 0.1 * np.sin(3.0 * xs))[None])
+    rng = np.random.default_rng(43)
+    buff = np.asfortranarray(0.5 + 0.5 * rng.random((p.nx, p.ny)))
+    eps = 0.02
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    R.setup(p, x, y, met, vm)
+    R.set_smoother(eps, buff)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), eps_e=np.array(eps), buff=buff, x=x, y=y, met=met, vm=vm, v0=v0)
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    out["lrhs"] = r
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    out["dv1"] = R.step(v, True)
+    out["dv2"] = R.step(v, False)
+    out["v2"] = v.copy()
+    R.set_smoother(0.0, buff)
+    np.savez_compressed(os.path.join(HERE, f"ref_smooth_{name}.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -390,6 +433,11 @@ def main():
     if sys.argv[1:] == ["sym"]:
         for name, kw in SYM_CASES.items():
             sym_case(R, name, kw)
+        R.lib.ref_flush()
+        return
+    if sys.argv[1:] == ["smooth"]:
+        for name, kw in SMOOTH_CASES.items():
+            smooth_case(R, name, kw)
         R.lib.ref_flush()
         return
     if sys.argv[1:] == ["inflow"]:
@@ -414,6 +462,8 @@ def main():
     for name, kw in INFLOW_CASES.items():
         inflow_case(R, name, kw)
     sponge4_case(R)
+    for name, kw in SMOOTH_CASES.items():
+        smooth_case(R, name, kw)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -387,3 +387,35 @@ def test_sponge4(orc):
     finally:
         orc.set_sponge(0)
         orc.set_inflow()
+
+
+@pytest.mark.parametrize("case", ["base", "sym", "xper", "yper"])
+def test_smoother(orc, case):
+    """the smoother eps_e (src/smoother3d.f90 in the residual; the implicit
+    damping of src/lhs1f3d.f90, lhs2f3d.f90, lhsbt1f3d.f90, lhsbt2f3d.f90
+    and lhsbs1f3d.f90 in the factors) with the damping profile buff, against
+    the reference's routines (ref_smooth_<case>.npz); without it both the
+    residual and both factors miss"""
+    z = load(f"ref_smooth_{case}.npz")
+    p = params_of(z)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    v0 = np.asfortranarray(z["v0"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    r0, _ = orc.lrhs3d(p, c, v0)
+    assert rel(r0, z["lrhs"]) > 1e-6
+    for d in (1, 2):
+        assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) > 1e-7, d
+    try:
+        orc.set_smoother(float(z["eps_e"]), z["buff"])
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        for d in (1, 2):
+            assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        v = np.asfortranarray(z["itrbc"].copy())
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+        assert rel(orc.step(p, c, dtl, f, v), z["dv2"]) < 1e-12
+        assert rel(v, z["v2"]) < 1e-12
+    finally:
+        orc.set_smoother(0.0)
