@@ -36,7 +36,8 @@ const char *lns3d_last_error(void);
  * cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
  * the reference's symmetric rows read point i + ny j); linear = 2; mattyp in
  * {0,2,3}; loctime in {0,1};
- * Navier-Stokes (viscous) boundary rows; eps_e = 0; sponges through
+ * Navier-Stokes (viscous) boundary rows; the smoother eps_e through
+ * lns3d_set_smoother; sponges through
  * lns3d_set_sponge (ispg 1 to 5).  Other values are rejected with
  * LNS3D_EUNSUPPORTED. */
 typedef struct {
@@ -166,6 +167,12 @@ int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep);
  * of the inflow namelist.  lns3d_factor refuses these selectors without it. */
 int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const double *eig,
                      const double *lalpha);
+/* smoother, the deck's eps_e (src/smoother3d.f90 in the residual, called at
+ * src/lrhs3d.f90:293; the implicit damping of src/lhs1f3d.f90:169-201,
+ * lhs2f3d.f90:171-203 and the closure and symmetric rows): eps_e and the
+ * damping profile buff (host (nx,ny)) that src/buffer.f90 (init_buffer)
+ * forms from ibuff.  eps_e = 0 turns it off.  Before lns3d_factor. */
+int lns3d_set_smoother(lns3d_ctx *c, double eps_e, const double *buff);
 /* corrector iterations per step, the deck's niter (src/impdrv3d.f90:173-358):
  * the residual with vold held at the start of the step, solved with the
  * factors of the first iteration, niter times.  lns3d_step only (impl 1 or

@@ -37,6 +37,7 @@ struct lns3d_ctx {
   double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
   double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
   double *side_block = nullptr;     // fside (20 ny), x left column staging (ny)
+  double *smooth_block = nullptr;   // eps_e buff (P)
   std::vector<double> eig;          // left = 4 profiles (10 ny), kept for ispg = 4
   double lalpha[2] = {0.0, 0.0};
   bool coeffs_ready = false, factored = false;
@@ -130,6 +131,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->sponge_block) (void)hipFree(c->sponge_block);
   if (c->top_block) (void)hipFree(c->top_block);
   if (c->side_block) (void)hipFree(c->side_block);
+  if (c->smooth_block) (void)hipFree(c->smooth_block);
   if (c->vhist) (void)hipFree(c->vhist);
   if (c->vstart) (void)hipFree(c->vstart);
   delete c;
@@ -159,6 +161,7 @@ extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
   c->c.sp1 = c->c.sp2 = c->c.spf = c->c.spl = nullptr;
   c->c.ftop = nullptr;
   c->c.fside = nullptr;
+  c->c.sbuf = nullptr;
   c->eig.clear();
   return 0;
 }
@@ -277,6 +280,28 @@ extern "C" int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double
   int rc = launch_top_forcing(c->p, c->c, xd, wamp ? wd : nullptr, x0, c->stream);
   if (rc) return rc;
   LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Smoother (the deck's eps_e; src/smoother3d.f90 in the residual, the implicit
+// damping of the factors): eps_e and the damping profile buff (host (nx,ny)),
+// which src/buffer.f90 (init_buffer) forms from ibuff; eps_e = 0 turns it off.
+// Before lns3d_factor; again after lns3d_ctx_set_params.
+extern "C" int lns3d_set_smoother(lns3d_ctx *c, double eps_e, const double *buff) {
+  if (!c) return fail(LNS3D_EINVAL, "lns3d_set_smoother: null context");
+  c->factored = false;
+  if (eps_e == 0.0) {
+    c->c.sbuf = nullptr;
+    return 0;
+  }
+  if (!buff) return fail(LNS3D_EINVAL, "lns3d_set_smoother: null buff");
+  const long P = (long)c->p.nx * c->p.ny;
+  if (!c->smooth_block) LNS3D_CHECK(hipMalloc(&c->smooth_block, (size_t)P * sizeof(double)));
+  std::vector<double> eb((size_t)P);
+  for (long q = 0; q < P; q++) eb[q] = eps_e * buff[q];
+  LNS3D_CHECK(hipMemcpyAsync(c->smooth_block, eb.data(), (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  c->c.sbuf = c->smooth_block;
   return 0;
 }
 

@@ -37,6 +37,37 @@ __device__ __forceinline__ bool sym_near(int sym, int p, int N) {
   return ((sym & 1) && p <= 2) || ((sym & 2) && p >= N - 3);
 }
 
+// dissipation of src/smoother3d.f90 along a line at position p of N (without
+// eps_e buff): second difference at the second point from an end, fourth at
+// the third, the 7-point sixth difference inside, the 7-point form with
+// reflected points next to a symmetric end; false on the end points
+__device__ __forceinline__ bool smooth_at(int N, int p, int sym, double sgn, const double2 *base, long es,
+                                          z &out) {
+  constexpr double f[7] = {-1.666666666666667E-01, 2.0, -6.5, 9.333333333333333E+00, -6.5, 2.0,
+                           -1.666666666666667E-01};
+  auto x = [&](int q) { return ld(base + (long)q * es); };
+  if (sym && sym_near(sym, p, N)) {
+    auto w = [&](int q) { return q < 0 ? sc(sgn, x(-q)) : q > N - 1 ? sc(sgn, x(2 * (N - 1) - q)) : x(q); };
+    out = sc(f[0], w(p - 3)) + sc(f[1], w(p - 2)) + sc(f[2], w(p - 1)) + sc(f[3], w(p)) + sc(f[4], w(p + 1)) +
+          sc(f[5], w(p + 2)) + sc(f[6], w(p + 3));
+    return true;
+  }
+  if (p == 1 || p == N - 2) {
+    out = sc(-1.0, x(p - 1)) + sc(2.0, x(p)) - sc(1.0, x(p + 1));
+    return true;
+  }
+  if (p == 2 || p == N - 3) {
+    out = x(p - 2) - sc(4.0, x(p - 1)) + sc(6.0, x(p)) - sc(4.0, x(p + 1)) + sc(1.0, x(p + 2));
+    return true;
+  }
+  if (p >= 3 && p <= N - 4) {
+    out = sc(f[0], x(p - 3)) + sc(f[1], x(p - 2)) + sc(f[2], x(p - 1)) + sc(f[3], x(p)) + sc(f[4], x(p + 1)) +
+          sc(f[5], x(p + 2)) + sc(f[6], x(p + 3));
+    return true;
+  }
+  return false;
+}
+
 template <typename S, typename V>
 __device__ __forceinline__ V d1(int N, int per, int p, const double *g, double dinv,
                                 const S *base, long es, int sym = 0, double sgn = 1.0) {
@@ -568,6 +599,32 @@ __device__ __forceinline__ void sym_weights(const AsmCtx &A, int pos, int N, int
   }
 }
 
+// implicit damping weight of slot k on the diagonal of component cc
+// (src/lhs1f3d.f90:169-201, lhs2f3d.f90:171-203; closure rows
+// lhsbt1f3d.f90:284-304, 451-471, lhsbt2f3d.f90:277-297, 431-451;
+// symmetric rows lhsbs1f3d.f90): the fourth difference fb inside and on every
+// row of a periodic line, the second difference on the second and
+// second-last rows, none on the end rows, fb reflected next to a symmetric end
+__device__ __forceinline__ double damp_w(int pos, int N, bool per, int sym, int k, int cc) {
+  constexpr double fb[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
+  if (per) return fb[k];
+  if (sym && (((sym & 1) && pos <= 1) || ((sym & 2) && pos >= N - 2))) {
+    const double s = cc == 2 ? -1.0 : 1.0;
+    double w = 0.0;
+    for (int o = 0; o < 5; o++) {
+      int col = pos + o - 2;
+      double sg = 1.0;
+      if (col < 0) { col = -col; sg = s; }
+      else if (col > N - 1) { col = 2 * (N - 1) - col; sg = s; }
+      if (col - pos + 2 == k) w += sg * fb[o];
+    }
+    return w;
+  }
+  if (pos == 0 || pos == N - 1) return 0.0;
+  if (pos == 1 || pos == N - 2) return k == 1 ? -1.0 : k == 2 ? 2.0 : k == 3 ? -1.0 : 0.0;
+  return fb[k];
+}
+
 __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir, int i, int j, int k,
                                        int r, int cc) {
   using namespace st;
@@ -643,6 +700,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
     if (ai >= 0) im = im - aw * dt * ABs;
     if (vi >= 0) re = re - dt * bw * Vv;
     if (k == 2 && r == cc) re = re + 1.0;
+    if (c.sbuf && r == cc) re = re + alfa * dt * c.sbuf[q] * damp_w(pos, N, per, sym, k, cc);
     return z{re, im};
   }
   if (!per && (pos <= 1 || pos >= N - 2)) {
@@ -677,6 +735,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
     if (vi >= 0) re = re - alfa * S * A.dsinv * dt * Vv;
     if (diag && r == cc) re = re + 1.0;
     if (!calcd && diag && r == cc && c.spl) re = re + alfa * dt * c.spl[q];   // lhsbt2f3d.f90:156-174
+    if (c.sbuf && r == cc) re = re + alfa * dt * c.sbuf[q] * damp_w(pos, N, per, 0, k, cc);
     return z{re, im};
   }
   // interior (src/lhs1f3d.f90:59-230)
@@ -689,6 +748,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   if (vi >= 0) re = re - A.b[k] * dt * Vv;
   if (k == 2 && r == cc) re = re + 1.0;
   if (!calcd && k == 2 && r == cc && c.spl) re = re + alfa * dt * c.spl[q];  // lhs2f3d.f90:207-225
+  if (c.sbuf && r == cc) re = re + alfa * dt * c.sbuf[q] * damp_w(pos, N, per, 0, k, cc);
   return z{re, im};
 }
 
@@ -873,6 +933,13 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
     if (c.sp2) {
       const z f{c.spf[2 * ((long)m * P + q)], c.spf[2 * ((long)m * P + q) + 1]};
       a = a + sc(c.sp2[q], vq(m) - f);
+    }
+    // smoother (src/lrhs3d.f90:293, src/smoother3d.f90): eta, then xi
+    if (c.sbuf) {
+      const double eb = c.sbuf[q];
+      z s;
+      if (smooth_at(ny, j, 0, 1.0, v + 5L * i + m, 5L * nx, s)) a = a + sc(eb, s);
+      if (smooth_at(nx, i, st.sym, m == 2 ? -1.0 : 1.0, v + 5L * nx * j + m, 5, s)) a = a + sc(eb, s);
     }
     // backward Euler: r = vold - v - dtl * r with vold = v (the field already
     // satisfies the boundary conditions when the step starts)
@@ -1115,8 +1182,14 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   }
   if (!symrow)
     for (int k = 0; k < 5; k++) { w1m[k] = w1[k]; w2m[k] = w2[k]; }
-  z d1v[5], d2v[5], xc[5];
-  for (int e = 0; e < 5; e++) { d1v[e] = z{0, 0}; d2v[e] = z{0, 0}; }
+  // implicit damping weights (component 3 apart next to a symmetric end)
+  double wd[5], wdm[5];
+  for (int k = 0; k < 5; k++) {
+    wd[k] = c.sbuf ? damp_w(posc, N, per, symrow ? A.sym : 0, k, 0) : 0.0;
+    wdm[k] = c.sbuf ? damp_w(posc, N, per, symrow ? A.sym : 0, k, 2) : 0.0;
+  }
+  z d1v[5], d2v[5], xc[5], d4v[5];
+  for (int e = 0; e < 5; e++) { d1v[e] = z{0, 0}; d2v[e] = z{0, 0}; d4v[e] = z{0, 0}; }
   for (int k = 0; k < 5; k++) {
     // symmetric rows: slot k is point posc + k - 2; the slots beyond the end
     // carry no weight and are skipped
@@ -1127,6 +1200,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
       const z xv = ld(x + 5 * qc + e);
       d1v[e] = d1v[e] + sc(e == 2 ? w1m[k] : w1[k], xv);
       d2v[e] = d2v[e] + sc(e == 2 ? w2m[k] : w2[k], xv);
+      d4v[e] = d4v[e] + sc(e == 2 ? wdm[k] : wd[k], xv);
       if (k == 2) xc[e] = xv;
     }
   }
@@ -1168,6 +1242,10 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   } else if (c.spl) {                  // sponge of the factor without the D terms
     const double f = p.alfa * dt * c.spl[qr];
     for (int r = 0; r < 5; r++) out[r] = out[r] + sc(f, xc[r]);
+  }
+  if (c.sbuf) {                        // implicit damping of the smoother
+    const double f = p.alfa * dt * c.sbuf[qr];
+    for (int r = 0; r < 5; r++) out[r] = out[r] + sc(f, d4v[r]);
   }
   for (int r = 0; r < 5; r++) out[r] = out[r] + xc[r];
   // boundary-condition rows (src/lhsbc1f3d.f90, lhsbc2f3d.f90), as lhs_entry

@@ -37,6 +37,9 @@ struct Coeffs {
   // :219-376): the values the left and right columns are held at, [2][5][ny]
   // complex as (re, im) pairs (side 0 left, side 1 right)
   double *fside;
+  // smoother (src/smoother3d.f90 and the implicit damping of the factors):
+  // eps_e buff per point, [P]; null: eps_e = 0
+  double *sbuf;
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);

@@ -65,6 +65,7 @@ SIGNATURES = {
     "lns3d_set_time_history": ([VP, VP, ctypes.c_long], CI),
     "lns3d_set_corrector_iterations": ([VP, CI], CI),
     "lns3d_set_inflow": ([VP, VP, ctypes.c_double, VP, VP], CI),
+    "lns3d_set_smoother": ([VP, ctypes.c_double, VP], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -194,6 +195,13 @@ class Step:
         la = np.array([complex(lalpha).real, complex(lalpha).imag])
         hp = lambda a: None if a is None else a.ctypes.data
         check(lib().lns3d_set_inflow(self.h, hp(xa), float(x0), hp(ea), hp(la)), "lns3d_set_inflow")
+
+    def set_smoother(self, eps_e, buff=None):
+        """smoother eps_e with the damping profile buff (nx, ny); 0: off"""
+        import numpy as np
+        b = None if buff is None else np.asfortranarray(buff, dtype=np.float64)
+        check(lib().lns3d_set_smoother(self.h, float(eps_e), None if b is None else b.ctypes.data),
+              "lns3d_set_smoother")
 
     def set_corrector_iterations(self, niter):
         check(lib().lns3d_set_corrector_iterations(self.h, niter), "lns3d_set_corrector_iterations")

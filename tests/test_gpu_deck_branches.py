@@ -16,7 +16,8 @@ src/itrbc3d.f90:145-195) against the reference's tests/golden/ref_top1.npz.
 The time schemes impl = 2 and 3 (lns3d_set_time_scheme) and the corrector
 iterations (lns3d_set_corrector_iterations) over three of the reference's
 steps.  The left/right inflow branches (lns3d_set_inflow: left = 1, left = 4,
-right = 4) against tests/golden/ref_inflow_<case>.npz."""
+right = 4) against tests/golden/ref_inflow_<case>.npz.  The smoother
+(lns3d_set_smoother) against tests/golden/ref_smooth_<case>.npz."""
 import os
 
 import numpy as np
@@ -409,3 +410,65 @@ def test_inflow_refusals():
     with pytest.raises(LNS3DError):                 # no inflow boundary
         st.set_inflow(x, 0.0, np.ones((10, p.ny)))
     st.close()
+
+
+@pytest.mark.parametrize("case", ["base", "sym", "xper", "yper"])
+def test_smoother_against_reference(case):
+    """lns3d_set_smoother (eps_e, buff): smoother3D's dissipation in the
+    residual and the implicit damping of both factors (interior, closure,
+    symmetric and periodic rows) against the reference's lrhs3D, assembled
+    factors and two steps (tests/golden/ref_smooth_<case>.npz), 1e-12; the
+    refined step (non-periodic lines) reaches the same increment"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step
+    z = np.load(os.path.join(GOLD, f"ref_smooth_{case}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"smoother {case}: {p.nx}x{p.ny} xper={p.xper} yper={p.yper} left={p.left}", flush=True)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    eps, buff = float(z["eps_e"]), z["buff"]
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t, shape):
+        return t.cpu().numpy().reshape(shape, order="F")
+
+    st = Step(p)
+    st.setup(met, vm)
+    st.set_smoother(eps, buff)
+    v0 = z["v0"]
+    vd = dev(v0)
+    rd = torch.zeros_like(vd)
+    st.lrhs3d(vd, rd)
+    assert rel(host(rd, v0.shape), z["lrhs"]) < TOL
+    for d in (1, 2):
+        m_dev = torch.zeros(125 * p.nx * p.ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        assert rel(host(m_dev, z[f"lhs{d}"].shape), z[f"lhs{d}"]) < TOL, d
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    for k in (1, 2):
+        st.step(vd, dvd)
+        e = rel(host(dvd, v0.shape), z[f"dv{k}"])
+        print(f"  step {k}: dv rel {e:.2e}", flush=True)
+        assert e < TOL, k
+    assert rel(host(vd, v0.shape), z["v2"]) < TOL
+    st.close()
+    if p.xper or p.yper:
+        return
+    st32 = Step(p, precision=1)
+    st32.setup(met, vm)
+    st32.set_smoother(eps, buff)
+    st32.factor()
+    vb = dev(z["itrbc"])
+    dvb = torch.zeros_like(vb)
+    its, res = st32.step_refined(vb, dvb, tol=1e-12, max_iters=10)
+    e = rel(host(dvb, v0.shape), z["dv1"])
+    print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
+    assert res <= 1e-12 and e < 1e-10
+    st32.close()
