@@ -30,7 +30,8 @@ const char *lns3d_last_error(void);
  * that the step routines read (src/input.f90 deck entries of the same names).
  * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
  * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left in
- * {0,1,4,7} and right in {0,4,7} (1: acoustic wave, 4: eigenfunction inflow,
+ * {0,1,4,7} and right in {0,4,7,8} (8: the column held at its initial
+ * values; 1: acoustic wave, 4: eigenfunction inflow,
  * both through lns3d_set_inflow; right 4 with left 4 only; 7: symmetry,
  * src/lhsbs1f3d.f90 and the symmetric closures of src/cgrad.f90,
  * cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since

@@ -57,7 +57,7 @@ static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
   if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 7) ||
-      (p.right != 0 && p.right != 4 && p.right != 7) || p.linear != 2 ||
+      (p.right != 0 && p.right != 4 && p.right != 7 && p.right != 8) || p.linear != 2 ||
       (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) || (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
   // right = 4 reads the eigenfunction profiles, which the reference reads

@@ -643,6 +643,9 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   if (!p.xper && ((i == 0 && (p.left == 0 || p.left == 4)) ||
                   (i == p.nx - 1 && (p.right == 0 || p.right == 4))))
     return one;
+  // right = 8 holds the initial condition: -I in the xi factor only
+  // (src/lhsbc1f3d.f90:439-446; the eta factor keeps its rows)
+  if (!p.xper && dir == 1 && i == p.nx - 1 && p.right == 8) return z{(k == 2 && r == cc) ? -1.0 : 0.0, 0.0};
   if (!p.yper) {
     if (j == p.ny - 1) return one;
     if (j == 0) {
@@ -999,7 +1002,8 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
           const z d = vq(m) - f;
           a = z{-d.re, -d.im};
         }
-        if ((i == 0 && p.left == 0) || (i == nx - 1 && (p.right == 0 || p.right == 4))) a = z{0, 0};
+        if ((i == 0 && p.left == 0) || (i == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 8)))
+          a = z{0, 0};
       }
     }
     sr[5 * l + m] = make_double2(a.re, a.im);
@@ -1255,6 +1259,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   const bool lrrow = !p.xper && ((ir == 0 && (p.left == 0 || p.left == 4)) ||
                                   (ir == nx - 1 && (p.right == 0 || p.right == 4)));
   if (lrrow) for (int r = 0; r < 5; r++) ident[r] = true;
+  const bool holdrow = !p.xper && dir == 1 && ir == nx - 1 && p.right == 8;   // -I
   if (!p.yper) {
     if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;
     if (jr == 0) {
@@ -1281,6 +1286,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   }
   for (int r = 0; r < 5; r++) {
     if (ident[r]) out[r] = xc[r];
+    if (holdrow) out[r] = z{-xc[r].re, -xc[r].im};
     y[5 * q + r] = make_double2(out[r].re, out[r].im);
   }
 }

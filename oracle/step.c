@@ -247,7 +247,7 @@ static int supported(const orc_params *p) {
   if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
   if ((p->top != 0 && p->top != 1) ||
       (p->left != 0 && p->left != 1 && p->left != 4 && p->left != 7) ||
-      (p->right != 0 && p->right != 4 && p->right != 7))
+      (p->right != 0 && p->right != 4 && p->right != 7 && p->right != 8))
     return 0;
   if (p->right == 4 && p->left != 4) return 0;   /* the profiles exist for left = 4 only */
   /* symmetric rows read the spanwise cross entries through a transposed
@@ -770,7 +770,13 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
     for (int j = 0; j < ny; j++) {                        /* left, right = 0 */
       long pts[2] = {(long)nx * j, nx - 1 + (long)nx * j};
       for (int s = 0; s < 2; s++) {
-        const int sel = s == 0 ? p->left : p->right;   /* src/lhsbc1f3d.f90:414-436 */
+        const int sel = s == 0 ? p->left : p->right;   /* src/lhsbc1f3d.f90:414-446 */
+        if (s == 1 && sel == 8 && dir == 1) {      /* hold the initial condition: -I */
+          long pt = pts[s];
+          for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
+          for (int r = 0; r < 5; r++) MAT(2, r, r) = -1.0;
+          continue;
+        }
         if (sel != 0 && sel != 4) continue;
         long pt = pts[s];
         for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
@@ -826,14 +832,14 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
       }
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) R(d, 0, j) = 0;
-        if (p->right == 0 || p->right == 4) R(d, nx - 1, j) = 0;
+        if (p->right == 0 || p->right == 4 || p->right == 8) R(d, nx - 1, j) = 0;
       }
     }
   }
   return 0;
 }
 
-/* src/itrbc3d.f90:34-376 (same selectors) */
+/* src/itrbc3d.f90:34-376 (same selectors; right = 8 holds the column) */
 int orc_itrbc3d(const orc_params *p, zdouble *v) {
   if (!supported(p)) return -1;
   int nx = p->nx, ny = p->ny;
@@ -876,6 +882,7 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
         if (side_values(p, 1, j, f)) return -1;
         for (int d = 0; d < 5; d++) V(d, nx - 1, j) = f[d];
       }
+
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) V(d, 0, j) = 0;
         if (p->right == 0) V(d, nx - 1, j) = 0;

@@ -303,6 +303,7 @@ def sym_case(R, name, kw):
 
 
 INFLOW_CASES = {
+    "right8": dict(right=8, wall=1, wallt=1),     # hold the initial condition on the right
     "left1": dict(left=1),
     "left4": dict(left=4, right=4),
     "left4top": dict(left=4, wall=1, wallt=1, top=1),

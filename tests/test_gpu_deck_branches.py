@@ -316,7 +316,7 @@ def test_corrector_iterations_against_reference(case):
     st.close()
 
 
-@pytest.mark.parametrize("case", ["left1", "left4", "left4top"])
+@pytest.mark.parametrize("case", ["left1", "left4", "left4top", "right8"])
 def test_inflow_against_reference(case):
     """lns3d_set_inflow: left = 1 (the acoustic wave on the left column), left =
     4 with right = 4 (eigenfunction inflow) and left = 4 with wall = wallt =
@@ -349,14 +349,19 @@ def test_inflow_against_reference(case):
             s.set_top_forcing(z["x"], None, x0)
         return s
 
+    inflow = p.left in (1, 4) or p.right == 4
     st = make()
     v0 = z["v0"]
     vd = dev(v0)
-    with pytest.raises(LNS3DError):
-        st.itrbc3d(vd)                       # the left values are not there yet
-    with pytest.raises(LNS3DError):
-        st.factor()
-    st.set_inflow(z["x"], x0, z["eig"], la)
+    if inflow:
+        with pytest.raises(LNS3DError):
+            st.itrbc3d(vd)                   # the left values are not there yet
+        with pytest.raises(LNS3DError):
+            st.factor()
+        st.set_inflow(z["x"], x0, z["eig"], la)
+    else:                                    # right = 8: no values to give
+        with pytest.raises(LNS3DError):
+            st.set_inflow(z["x"], x0, z["eig"], la)
     st.itrbc3d(vd)
     assert rel(host(vd, v0.shape), z["itrbc"]) < TOL
     vd = dev(v0)
@@ -379,7 +384,8 @@ def test_inflow_against_reference(case):
     st.close()
 
     st32 = make(precision=1)
-    st32.set_inflow(z["x"], x0, z["eig"], la)
+    if inflow:
+        st32.set_inflow(z["x"], x0, z["eig"], la)
     st32.factor()
     vb = dev(z["itrbc"])
     dvb = torch.zeros_like(vb)

@@ -319,14 +319,16 @@ def test_symmetry_transposed_read_bounds(orc):
         orc.genmtrx(p, vm, met)
 
 
-@pytest.mark.parametrize("case", ["left1", "left4", "left4top"])
+@pytest.mark.parametrize("case", ["left1", "left4", "left4top", "right8"])
 def test_inflow(orc, case):
     """left = 1 (the acoustic wave on x(1,j): src/rhsbc3d.f90:176-205,
     src/itrbc3d.f90:219-245, the xi factor's closure row kept), left = 4 and
     right = 4 (eigenfunction inflow: src/rhsbc3d.f90:207-246,
     src/itrbc3d.f90:247-318, 352-376, identity rows of src/lhsbc1f3d.f90
-    :414-436) against the reference's routines (ref_inflow_<case>.npz); the
-    boundary values are not those of left = right = 0"""
+    :414-436), and right = 8 (the right column held: a zero residual row,
+    -I in the xi factor, src/lhsbc1f3d.f90:439-446) against the reference's
+    routines (ref_inflow_<case>.npz); the boundary values are not those of
+    left = right = 0"""
     import dataclasses
     z = load(f"ref_inflow_{case}.npz")
     p = params_of(z)
@@ -341,7 +343,10 @@ def test_inflow(orc, case):
         v = np.asfortranarray(v0.copy())
         orc.itrbc3d(p, v)
         assert rel(v, z["itrbc"]) < 1e-14
-        assert np.abs(v[:, 0, 1:-1]).max() > 0.1          # the inflow is on the boundary
+        if p.right == 8:                                    # the right column is held
+            assert np.array_equal(v[:, -1, 1:-1], v0[:, -1, 1:-1])
+        else:
+            assert np.abs(v[:, 0, 1:-1]).max() > 0.1      # the inflow is on the boundary
         r, _ = orc.lrhs3d(p, c, v0)
         assert rel(r, z["lrhs"]) < 1e-14
         orc.rhsbc3d(p, r, v0)
