@@ -30,8 +30,9 @@ const char *lns3d_last_error(void);
  * that the step routines read (src/input.f90 deck entries of the same names).
  * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
  * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left in
- * {0,1,4,7} and right in {0,4,7,8} (8: the column held at its initial
- * values; 1: acoustic wave, 4: eigenfunction inflow,
+ * {0,1,4,5,7} and right in {0,4,5,7,8} (8: the column held at its initial
+ * values; 1: acoustic wave, 5: the undamped wave, right 5 with nx <= ny;
+ * 4: eigenfunction inflow; 1, 4 and 5
  * both through lns3d_set_inflow; right 4 with left 4 only; 7: symmetry,
  * src/lhsbs1f3d.f90 and the symmetric closures of src/cgrad.f90,
  * cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
@@ -159,9 +160,12 @@ int lns3d_set_time_scheme(lns3d_ctx *c, int impl);
  * vold_dev is the field at the start of the last step taken (device), lstep
  * the number of steps taken. */
 int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep);
-/* left = 1, left = 4 and right = 4 boundary values (src/rhsbc3d.f90:176-246,
- * src/itrbc3d.f90:219-376): left = 1 the acoustic wave on the left column
- * (from the mean flow, x (host (nx,ny)) and x0: lns3d_setup first); left = 4
+/* left = 1, 4, 5 and right = 4, 5 boundary values (src/rhsbc3d.f90:176-307,
+ * src/itrbc3d.f90:219-404): left = 1 the acoustic wave on the left column
+ * (from the mean flow, x (host (nx,ny)) and x0: lns3d_setup first); left = 5
+ * and right = 5 the undamped wave exp(i k x) of that column, x read where the
+ * reference reads it (x(j,1), x(j,nx): memory offsets j and j + nx(nx-1));
+ * left = 4
  * the eigenfunction profiles eig (host (10,ny): rhor, rhoi, ur, ui, vr, vi,
  * wr, wi, tr, ti, the columns of src/gendist.f90's inflow file; omega != 0);
  * right = 4 the same profiles times exp(i lalpha x(nx,j)), lalpha = (re, im)

@@ -56,14 +56,18 @@ struct lns3d_ctx {
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
   if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
-      (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 7) ||
-      (p.right != 0 && p.right != 4 && p.right != 7 && p.right != 8) || p.linear != 2 ||
+      (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
+      (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8) || p.linear != 2 ||
       (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) || (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
   // right = 4 reads the eigenfunction profiles, which the reference reads
   // only for left = 4 (src/gendist.f90:17)
   if (!p.xper && p.right == 4 && p.left != 4)
     return fail(LNS3D_EUNSUPPORTED, "right = 4 needs left = 4 (the eigenfunction inflow)");
+  // right = 5 reads x(j,nx) of x(nx,ny) (src/rhsbc3d.f90:296): on the grid
+  // only for nx <= ny
+  if (!p.xper && p.right == 5 && p.ny - 1 + (long)p.nx * (p.nx - 1) >= (long)p.nx * p.ny)
+    return fail(LNS3D_EUNSUPPORTED, "right = 5 needs nx <= ny (the reference reads x(j,nx) off the grid otherwise)");
   // symmetric ends: src/lhsbs1f3d.f90:17 reads the spanwise cross entries of
   // row (i, j) at point i + ny j (its Ahi(6,ny,nx) shape), which lies on the
   // grid only when nx >= ny (as in the reference's own symmetric case)
@@ -316,15 +320,18 @@ extern "C" int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const 
   if (!c) return fail(LNS3D_EINVAL, "lns3d_set_inflow: null context");
   const lns3d_params &p = c->p;
   const bool l1 = !p.xper && p.left == 1, l4 = !p.xper && p.left == 4, r4 = !p.xper && p.right == 4;
-  if (!l1 && !l4 && !r4) return fail(LNS3D_EINVAL, "lns3d_set_inflow: the context has no left = 1, 4 or right = 4");
-  if (((l1 || r4) && !x) || ((l4 || r4) && !eig)) return fail(LNS3D_EINVAL, "lns3d_set_inflow: null argument");
+  const bool l5 = !p.xper && p.left == 5, r5 = !p.xper && p.right == 5;
+  if (!l1 && !l4 && !r4 && !l5 && !r5)
+    return fail(LNS3D_EINVAL, "lns3d_set_inflow: the context has no left = 1, 4, 5 or right = 4, 5");
+  if (((l1 || r4 || l5 || r5) && !x) || ((l4 || r4) && !eig))
+    return fail(LNS3D_EINVAL, "lns3d_set_inflow: null argument");
   if (l4 && p.omega == 0.0)
     return fail(LNS3D_EUNSUPPORTED, "lns3d_set_inflow: left = 4 with omega = 0 varies in time in the reference "
                                     "(its transient eigenfunction), not implemented");
-  if (l1 && !c->vm_ready)
-    return fail(LNS3D_EINVAL, "lns3d_set_inflow: the left wave needs the mean flow (lns3d_setup first)");
+  if ((l1 || l5 || r5) && !c->vm_ready)
+    return fail(LNS3D_EINVAL, "lns3d_set_inflow: the waves need the mean flow (lns3d_setup first)");
   const int nx = p.nx, ny = p.ny;
-  if (!c->side_block) LNS3D_CHECK(hipMalloc(&c->side_block, (size_t)21 * ny * sizeof(double)));
+  if (!c->side_block) LNS3D_CHECK(hipMalloc(&c->side_block, (size_t)22 * ny * sizeof(double)));
   std::vector<double> hs((size_t)20 * ny, 0.0);
   if (l4)
     for (int j = 0; j < ny; j++)
@@ -351,14 +358,21 @@ extern "C" int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const 
     c->lalpha[0] = lalpha ? lalpha[0] : 0.0;
     c->lalpha[1] = lalpha ? lalpha[1] : 0.0;
   }
-  if (l1) {
-    std::vector<double> xl((size_t)ny);
-    for (int j = 0; j < ny; j++) xl[j] = x[(long)nx * j];
-    double *xd = c->side_block + 20 * ny;
-    LNS3D_CHECK(hipMemcpyAsync(xd, xl.data(), (size_t)ny * 8, hipMemcpyHostToDevice, c->stream));
-    int rc = launch_left_wave(p, c->c, xd, x0, c->stream);
-    if (rc) return rc;
+  // the waves: the x value line j reads is x(1,j) for left = 1, and the
+  // reference's transposed x(j,1) and x(j,nx) (memory offsets j and
+  // j + nx (nx - 1) of x(nx,ny)) for left = 5 and right = 5
+  std::vector<double> xl((size_t)2 * ny);
+  for (int j = 0; j < ny; j++) {
+    xl[j] = l1 ? x[(long)nx * j] : l5 ? x[j] : 0.0;
+    xl[ny + j] = r5 ? x[j + (long)nx * (nx - 1)] : 0.0;
   }
+  double *xd = c->side_block + 20 * ny;
+  if (l1 || l5 || r5)
+    LNS3D_CHECK(hipMemcpyAsync(xd, xl.data(), (size_t)2 * ny * 8, hipMemcpyHostToDevice, c->stream));
+  int rc = 0;
+  if (l1 || l5) rc = launch_side_wave(p, c->c, xd, x0, 0, l1 ? 1 : 0, c->stream);
+  if (!rc && r5) rc = launch_side_wave(p, c->c, xd + ny, x0, 1, 0, c->stream);
+  if (rc) return rc;
   LNS3D_CHECK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -427,8 +441,9 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
   if (!c || !c->coeffs_ready) return fail(LNS3D_EINVAL, "lns3d_factor: no coefficients");
   if (c->p.top == 1 && !c->c.ftop)
     return fail(LNS3D_EINVAL, "lns3d_factor: top = 1 needs its incoming wave (lns3d_set_top_forcing)");
-  if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.right == 4) && !c->c.fside)
-    return fail(LNS3D_EINVAL, "lns3d_factor: left = 1, 4 and right = 4 need their values (lns3d_set_inflow)");
+  if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.left == 5 || c->p.right == 4 || c->p.right == 5) &&
+      !c->c.fside)
+    return fail(LNS3D_EINVAL, "lns3d_factor: left = 1, 4, 5 and right = 4, 5 need their values (lns3d_set_inflow)");
   int rc = launch_assemble(c->p, c->c, 1, c->px, c->stream);
   if (!rc) rc = penta_factor(&c->px, c->stream);
   if (!rc) rc = launch_assemble(c->p, c->c, 2, c->py, c->stream);
@@ -481,8 +496,9 @@ extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
   // the boundary values these selectors hold the field at must be formed first
   if (!c->p.yper && c->p.top == 1 && !c->c.ftop)
     return fail(LNS3D_EINVAL, "lns3d_itrbc3d: top = 1 needs its incoming wave (lns3d_set_top_forcing)");
-  if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.right == 4) && !c->c.fside)
-    return fail(LNS3D_EINVAL, "lns3d_itrbc3d: left = 1, 4 and right = 4 need their values (lns3d_set_inflow)");
+  if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.left == 5 || c->p.right == 4 || c->p.right == 5) &&
+      !c->c.fside)
+    return fail(LNS3D_EINVAL, "lns3d_itrbc3d: left = 1, 4, 5 and right = 4, 5 need their values (lns3d_set_inflow)");
   return launch_itrbc(c->p, c->c, v_dev, c->stream);
 }
 

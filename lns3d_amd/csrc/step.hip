@@ -532,25 +532,44 @@ __global__ void k_top_forcing(lns3d_params p, const double *vm, const double *xt
   }
 }
 
-// left = 1: the wave on the left column (src/rhsbc3d.f90:176-205,
-// src/itrbc3d.f90:219-245; the top wave's form with amplitude 1)
-__global__ void k_left_wave(lns3d_params p, const double *vm, const double *xleft, double x0,
-                            double *fside) {
+// left = 5 / right = 5 (src/rhsbc3d.f90:248-270, 286-307): the same wave
+// undamped, exp(i k x), from the mean flow of that column
+__device__ void plain_wave(const lns3d_params &p, const double *vml, double xl, z (&f)[5]) {
+  const double cm = sqrt(vml[4]) / p.Ma, um = vml[1];
+  const double kk = p.omega / (cm + um);
+  double sn, cs;
+  sincos(kk * xl, &sn, &cs);
+  const z c3{cs, sn};
+  const double rho = vml[0], T = vml[4], cm2 = cm * cm, gM2 = p.gamma * p.Ma * p.Ma;
+  f[0] = sc(0.5 / cm2, c3);
+  f[1] = sc(0.5 / (rho * cm), c3);
+  f[2] = z{0, 0};
+  f[3] = z{0, 0};
+  f[4] = sc(1.0 / rho, sc(gM2 * 0.5, c3) - sc(T * 0.5 / cm2, c3));
+}
+
+// the wave on the left (side 0) or right (side 1) column into fside: damped
+// (left = 1, src/rhsbc3d.f90:176-205, src/itrbc3d.f90:219-245: the top wave's
+// form with amplitude 1) or not (left/right = 5); xcol the x value each line
+// j reads
+__global__ void k_side_wave(lns3d_params p, const double *vm, const double *xcol, double x0, int side,
+                            int damped, double *fside) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p.ny) return;
-  const long q = (long)p.nx * j;
+  const long q = (side ? p.nx - 1 : 0) + (long)p.nx * j;
   z f[5];
-  acoustic_wave(p, vm + 5 * q, xleft[j], x0, 1.0, f);
+  if (damped) acoustic_wave(p, vm + 5 * q, xcol[j], x0, 1.0, f);
+  else plain_wave(p, vm + 5 * q, xcol[j], f);
   for (int m = 0; m < 5; m++) {
-    fside[2 * (m * p.ny + j)] = f[m].re;
-    fside[2 * (m * p.ny + j) + 1] = f[m].im;
+    fside[2 * ((5 * side + m) * p.ny + j)] = f[m].re;
+    fside[2 * ((5 * side + m) * p.ny + j) + 1] = f[m].im;
   }
 }
 
-int launch_left_wave(const lns3d_params &p, const Coeffs &c, const double *xleft, double x0,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_left_wave, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (const double *)c.vm,
-                     xleft, x0, c.fside);
+int launch_side_wave(const lns3d_params &p, const Coeffs &c, const double *xcol, double x0, int side,
+                     int damped, hipStream_t s) {
+  hipLaunchKernelGGL(k_side_wave, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (const double *)c.vm,
+                     xcol, x0, side, damped, c.fside);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
@@ -640,8 +659,8 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   const z one{(k == 2 && r == cc) ? 1.0 : 0.0, 0.0};
   // left/right 0 and 4: identity rows (src/lhsbc1f3d.f90:414-436); left 1
   // keeps the closure row, 7 the symmetric row
-  if (!p.xper && ((i == 0 && (p.left == 0 || p.left == 4)) ||
-                  (i == p.nx - 1 && (p.right == 0 || p.right == 4))))
+  if (!p.xper && ((i == 0 && (p.left == 0 || p.left == 4 || p.left == 5)) ||
+                  (i == p.nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5))))
     return one;
   // right = 8 holds the initial condition: -I in the xi factor only
   // (src/lhsbc1f3d.f90:439-446; the eta factor keeps its rows)
@@ -997,8 +1016,13 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
       } else {
         // src/rhsbc3d.f90:172-300: left 0 and right 0, 4 zero; left 1, 4
         // -(v - the held value); left/right 7 the operator's row
-        if (i == 0 && (p.left == 1 || p.left == 4)) {
+        if (i == 0 && (p.left == 1 || p.left == 4 || p.left == 5)) {
           const z f{c.fside[2 * (m * ny + j)], c.fside[2 * (m * ny + j) + 1]};
+          const z d = vq(m) - f;
+          a = z{-d.re, -d.im};
+        }
+        if (i == nx - 1 && p.right == 5) {
+          const z f{c.fside[2 * ((5 + m) * ny + j)], c.fside[2 * ((5 + m) * ny + j) + 1]};
           const z d = vq(m) - f;
           a = z{-d.re, -d.im};
         }
@@ -1079,10 +1103,10 @@ __global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
       row[5L * (nx - 1) + d] = row[d];
     } else {
       if (p.left == 0) row[d] = make_double2(0, 0);
-      if (p.left == 1 || p.left == 4)
+      if (p.left == 1 || p.left == 4 || p.left == 5)
         row[d] = make_double2(fside[2 * (d * ny + j)], fside[2 * (d * ny + j) + 1]);
       if (p.right == 0) row[5L * (nx - 1) + d] = make_double2(0, 0);
-      if (p.right == 4)
+      if (p.right == 4 || p.right == 5)
         row[5L * (nx - 1) + d] =
             make_double2(fside[2 * ((5 + d) * ny + j)], fside[2 * ((5 + d) * ny + j) + 1]);
     }
@@ -1256,8 +1280,8 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   bool ident[5] = {false, false, false, false, false};
   bool wallrow0 = false, wallrow4 = false;
   const int ir = dir == 1 ? posc : i, jr = dir == 1 ? j : posc;
-  const bool lrrow = !p.xper && ((ir == 0 && (p.left == 0 || p.left == 4)) ||
-                                  (ir == nx - 1 && (p.right == 0 || p.right == 4)));
+  const bool lrrow = !p.xper && ((ir == 0 && (p.left == 0 || p.left == 4 || p.left == 5)) ||
+                                  (ir == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5)));
   if (lrrow) for (int r = 0; r < 5; r++) ident[r] = true;
   const bool holdrow = !p.xper && dir == 1 && ir == nx - 1 && p.right == 8;   // -I
   if (!p.yper) {

@@ -50,10 +50,11 @@ int launch_sponge_forcing(const lns3d_params &p, int ispg, const Coeffs &c, cons
                           const double *wamp, double x0, hipStream_t s);
 // the top = 1 wave into c.ftop from the mean flow c.vm, the top row of the
 // physical x (nx, device) and wamp (nx, device; null for 1)
-// the left column's acoustic wave (left = 1) into c.fside side 0, from the
-// mean flow and the physical x of the left column (ny, device)
-int launch_left_wave(const lns3d_params &p, const Coeffs &c, const double *xleft, double x0,
-                     hipStream_t s);
+// the acoustic wave of the left (side 0) or right (side 1) column into
+// c.fside, damped (left = 1) or not (left/right = 5), from the mean flow and
+// the x value each line reads (ny, device)
+int launch_side_wave(const lns3d_params &p, const Coeffs &c, const double *xcol, double x0, int side,
+                     int damped, hipStream_t s);
 int launch_top_forcing(const lns3d_params &p, const Coeffs &c, const double *xtop, const double *wamp,
                        double x0, hipStream_t s);
 // assemble the LHS of direction dir (1: xi, 2: eta) into the line-interleaved

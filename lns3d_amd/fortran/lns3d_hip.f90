@@ -19,6 +19,8 @@
 !    arrays (the time loop of src/impdrv3d.f90:126-406):
 !       call lns3d_hip_init(prm, metric, vm)   ! coefficients, dtl, factors
 !       call lns3d_hip_scheme(impl, niter)     ! optional: the deck's impl, niter
+!       call lns3d_hip_sponge / _top / _inflow / _smoother, then
+!       call lns3d_hip_factor()                ! the deck's sponge, boundary values, eps_e
 !       call lns3d_hip_put(v)                  ! state to the device
 !       call lns3d_hip_advance(nsteps, dv)     ! steps on the device
 !       call lns3d_hip_get(v)                  ! state back
@@ -116,6 +118,27 @@ module lns3d_hip
       type(c_ptr), value :: c
       integer(c_int), value :: niter
     end function
+    integer(c_int) function c_set_sponge(c, ispg, spg, spg2, x, wamp, x0) bind(C, name="lns3d_set_sponge")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c, spg, spg2, x, wamp
+      integer(c_int), value :: ispg
+      real(c_double), value :: x0
+    end function
+    integer(c_int) function c_set_top_forcing(c, x, wamp, x0) bind(C, name="lns3d_set_top_forcing")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c, x, wamp
+      real(c_double), value :: x0
+    end function
+    integer(c_int) function c_set_inflow(c, x, x0, eig, lalpha) bind(C, name="lns3d_set_inflow")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c, x, eig, lalpha
+      real(c_double), value :: x0
+    end function
+    integer(c_int) function c_set_smoother(c, eps_e, buff) bind(C, name="lns3d_set_smoother")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c, buff
+      real(c_double), value :: eps_e
+    end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
@@ -152,16 +175,64 @@ contains
 
   ! metric(nx,ny,10) and vm(5,nx,ny) as src/gengrid.f90 and genmean.f90
   ! hold them; generates the coefficients and dtl on the device and factors
-  ! both directions (src/impdrv3d.f90:248-269, 310-319)
+  ! both directions (src/impdrv3d.f90:248-269, 310-319).  Selectors whose
+  ! boundary values come from the deck (top = 1, left = 1, 4 or 5, right = 4
+  ! or 5) are not factored here: give those values (lns3d_hip_top,
+  ! lns3d_hip_inflow), then call lns3d_hip_factor.
   subroutine lns3d_hip_init(prm, metric, vm)
     type(lns3d_params), intent(in) :: prm
     real(c_double), intent(in), target :: metric(*), vm(*)
     integer(c_int) :: rc
+    logical :: needs_values
     if (c_associated(ctx)) call lns3d_hip_free()
+    needs_values = prm%top == 1 .or. ((prm%left == 1 .or. prm%left == 4 .or. prm%left == 5 .or. &
+                                       prm%right == 4 .or. prm%right == 5) .and. prm%xper == 0)
     rc = c_ctx_create(prm, ctx)
     if (rc == 0) rc = c_setup(ctx, c_loc(metric), c_loc(vm))
-    if (rc == 0) rc = c_factor(ctx)
+    if (rc == 0 .and. .not. needs_values) rc = c_factor(ctx)
     call lns3d_hip_record(rc)
+  end subroutine
+
+  ! forms the factors again (after any of the setters below)
+  subroutine lns3d_hip_factor()
+    call lns3d_hip_record(c_factor(ctx))
+  end subroutine
+
+  ! sponge: the module state src/sponge.f90:20 (init_sponge) leaves, spg and
+  ! spg2 (nx,ny), the physical x (nx,ny), wamp (nx), x0
+  subroutine lns3d_hip_sponge(ispg, spg, spg2, x, wamp, x0)
+    integer, intent(in) :: ispg
+    real(c_double), intent(in), target :: spg(*), spg2(*), x(*), wamp(*)
+    real(c_double), intent(in) :: x0
+    call lns3d_hip_record(c_set_sponge(ctx, int(ispg, c_int), c_loc(spg), c_loc(spg2), c_loc(x), &
+                                       c_loc(wamp), x0))
+  end subroutine
+
+  ! top = 1: the incoming wave from x (nx,ny), wamp (nx) and x0
+  subroutine lns3d_hip_top(x, wamp, x0)
+    real(c_double), intent(in), target :: x(*), wamp(*)
+    real(c_double), intent(in) :: x0
+    call lns3d_hip_record(c_set_top_forcing(ctx, c_loc(x), c_loc(wamp), x0))
+  end subroutine
+
+  ! left = 1, 4, 5 and right = 4, 5: x (nx,ny), x0, the eigenfunction
+  ! profiles eig(10,ny) (rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti) and the
+  ! inflow namelist's lalpha
+  subroutine lns3d_hip_inflow(x, x0, eig, lalpha)
+    real(c_double), intent(in), target :: x(*), eig(*)
+    real(c_double), intent(in) :: x0
+    complex(c_double_complex), intent(in) :: lalpha
+    real(c_double), target :: la(2)
+    la(1) = real(lalpha, c_double)
+    la(2) = aimag(lalpha)
+    call lns3d_hip_record(c_set_inflow(ctx, c_loc(x), x0, c_loc(eig), c_loc(la)))
+  end subroutine
+
+  ! smoother: eps_e and the damping profile buff (nx,ny) of init_buffer
+  subroutine lns3d_hip_smoother(eps_e, buff)
+    real(c_double), intent(in) :: eps_e
+    real(c_double), intent(in), target :: buff(*)
+    call lns3d_hip_record(c_set_smoother(ctx, eps_e, c_loc(buff)))
   end subroutine
 
   ! the deck's time scheme and corrector iterations (src/impdrv3d.f90:173-199):

@@ -246,9 +246,11 @@ static int supported(const orc_params *p) {
   if (p->nx < 7 || p->ny < 7) return 0;
   if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
   if ((p->top != 0 && p->top != 1) ||
-      (p->left != 0 && p->left != 1 && p->left != 4 && p->left != 7) ||
-      (p->right != 0 && p->right != 4 && p->right != 7 && p->right != 8))
+      (p->left != 0 && p->left != 1 && p->left != 4 && p->left != 5 && p->left != 7) ||
+      (p->right != 0 && p->right != 4 && p->right != 5 && p->right != 7 && p->right != 8))
     return 0;
+  if (!p->xper && p->right == 5 && p->ny - 1 + (long)p->nx * (p->nx - 1) >= (long)p->nx * p->ny)
+    return 0;                                    /* x(j,nx) off the grid (nx > ny) */
   if (p->right == 4 && p->left != 4) return 0;   /* the profiles exist for left = 4 only */
   /* symmetric rows read the spanwise cross entries through a transposed
    * shape (src/lhsbs1f3d.f90:17, Ahi(6,ny,nx)): point i + ny j, which must
@@ -289,13 +291,8 @@ static double lhs_sponge(long q) {
 /* the incoming acoustic wave of src/lrhs3d.f90:248-282 and
  * src/rhsbc3d.f90:131-159: amplitude amp, mean state vml at physical x xl;
  * f[m] is the wave's disturbance in component m */
-static void acoustic_wave(const orc_params *p, const double *vml, double xl, double x0, double amp,
-                          zdouble f[5]) {
-  const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
-  const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / p->Re;
-  const double kk = p->omega / (cm + um);
-  const double a = p->omega * p->omega * d / ((cm + um) * (cm + um) * (cm + um));
-  const zdouble c3 = amp * (exp(-a * (xl - x0)) * cexp(_Complex_I * (kk * xl)));
+static void wave_terms(const orc_params *p, const double *vml, zdouble c3, zdouble f[5]) {
+  const double cm = sqrt(vml[4]) / p->Ma;
   const double rho = vml[0], T = vml[4];
   const double gM2 = p->gamma * p->Ma * p->Ma;
   f[0] = 0.5 * c3 / (cm * cm);
@@ -303,6 +300,24 @@ static void acoustic_wave(const orc_params *p, const double *vml, double xl, dou
   f[2] = 0;
   f[3] = 0;
   f[4] = (gM2 * c3 * 0.5 - T * 0.5 * c3 / (cm * cm)) / rho;
+}
+
+static void acoustic_wave(const orc_params *p, const double *vml, double xl, double x0, double amp,
+                          zdouble f[5]) {
+  const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
+  const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / p->Re;
+  const double kk = p->omega / (cm + um);
+  const double a = p->omega * p->omega * d / ((cm + um) * (cm + um) * (cm + um));
+  const zdouble c3 = amp * (exp(-a * (xl - x0)) * cexp(_Complex_I * (kk * xl)));
+  wave_terms(p, vml, c3, f);
+}
+
+/* left = 5 / right = 5 (src/rhsbc3d.f90:248-270, 286-307, itrbc3d.f90:320-340,
+ * 382-404): the undamped wave exp(i k x) of amplitude 1 */
+static void plain_wave(const orc_params *p, const double *vml, double xl, zdouble f[5]) {
+  const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
+  const double kk = p->omega / (cm + um);
+  wave_terms(p, vml, cexp(_Complex_I * (kk * xl)), f);
 }
 
 /* left/right inflow state: left = 1 (the acoustic wave on x(1,j), from the
@@ -327,6 +342,15 @@ static int side_values(const orc_params *p, int side, int j, zdouble f[5]) {
   if (sel == 1) {
     if (!g_inf.x || !g_inf.vm) return -1;
     acoustic_wave(p, g_inf.vm + 5 * q, g_inf.x[q], g_inf.x0, 1.0, f);
+    return 0;
+  }
+  if (sel == 5) {
+    /* the reference reads x(j,1) and x(j,nx): memory offsets j - 1 and
+     * j - 1 + nx (nx - 1) of x(nx,ny), transposed; on the grid for left = 5,
+     * and for right = 5 when nx <= ny (supported() checks) */
+    if (!g_inf.x || !g_inf.vm) return -1;
+    const long xo = side ? j + (long)p->nx * (p->nx - 1) : j;
+    plain_wave(p, g_inf.vm + 5 * q, g_inf.x[xo], f);
     return 0;
   }
   if (sel == 4) {
@@ -777,7 +801,7 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
           for (int r = 0; r < 5; r++) MAT(2, r, r) = -1.0;
           continue;
         }
-        if (sel != 0 && sel != 4) continue;
+        if (sel != 0 && sel != 4 && sel != 5) continue;
         long pt = pts[s];
         for (int e = 0; e < 125; e++) mat[125 * pt + e] = 0;
         for (int r = 0; r < 5; r++) MAT(2, r, r) = 1.0;
@@ -826,9 +850,13 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
   } else {
     for (int j = 0; j < ny; j++) {            /* left/right = 7: the operator's row */
       zdouble f[5];
-      if (p->left == 1 || p->left == 4) {
+      if (p->left == 1 || p->left == 4 || p->left == 5) {
         if (side_values(p, 0, j, f)) return -1;
         for (int d = 0; d < 5; d++) R(d, 0, j) = -(V(d, 0, j) - f[d]);
+      }
+      if (p->right == 5) {
+        if (side_values(p, 1, j, f)) return -1;
+        for (int d = 0; d < 5; d++) R(d, nx - 1, j) = -(V(d, nx - 1, j) - f[d]);
       }
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) R(d, 0, j) = 0;
@@ -874,11 +902,11 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
   } else {
     for (int j = 0; j < ny; j++) {            /* left/right = 7: left as they are */
       zdouble f[5];
-      if (p->left == 1 || p->left == 4) {
+      if (p->left == 1 || p->left == 4 || p->left == 5) {
         if (side_values(p, 0, j, f)) return -1;
         for (int d = 0; d < 5; d++) V(d, 0, j) = f[d];
       }
-      if (p->right == 4) {
+      if (p->right == 4 || p->right == 5) {
         if (side_values(p, 1, j, f)) return -1;
         for (int d = 0; d < 5; d++) V(d, nx - 1, j) = f[d];
       }

@@ -307,6 +307,11 @@ INFLOW_CASES = {
     "left1": dict(left=1),
     "left4": dict(left=4, right=4),
     "left4top": dict(left=4, wall=1, wallt=1, top=1),
+    # left = 5 / right = 5: the undamped wave on x(j,1) / x(j,nx), transposed
+    # reads of x(nx,ny) (right = 5 stays on the grid for nx <= ny)
+    "left5": dict(left=5),
+    "right5": dict(nx=9, ny=13, right=5),
+    "both5": dict(nx=11, ny=11, left=5, right=5, wall=1, wallt=1),
 }
 
 
@@ -314,7 +319,7 @@ def inflow_case(R, name, kw):
     """left = 1 (acoustic wave on the left boundary), left = 4 / right = 4
     (eigenfunction inflow: seeded profiles in the columns of src/gendist.f90's
     inflow file, a complex lalpha; omega != 0)"""
-    p = Params(nx=13, ny=9, kz=0.4, omega=0.3, **kw)
+    p = Params(**{**dict(nx=13, ny=9, kz=0.4, omega=0.3), **kw})
     x, y, met = make_grid(p)
     vm = make_mean(p, y)
     rng = np.random.default_rng(37)

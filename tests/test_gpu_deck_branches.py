@@ -316,7 +316,7 @@ def test_corrector_iterations_against_reference(case):
     st.close()
 
 
-@pytest.mark.parametrize("case", ["left1", "left4", "left4top", "right8"])
+@pytest.mark.parametrize("case", ["left1", "left4", "left4top", "right8", "left5", "right5", "both5"])
 def test_inflow_against_reference(case):
     """lns3d_set_inflow: left = 1 (the acoustic wave on the left column), left =
     4 with right = 4 (eigenfunction inflow) and left = 4 with wall = wallt =
@@ -349,7 +349,7 @@ def test_inflow_against_reference(case):
             s.set_top_forcing(z["x"], None, x0)
         return s
 
-    inflow = p.left in (1, 4) or p.right == 4
+    inflow = p.left in (1, 4, 5) or p.right in (4, 5)
     st = make()
     v0 = z["v0"]
     vd = dev(v0)

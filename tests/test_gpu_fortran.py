@@ -181,3 +181,49 @@ def test_module_driver_scheme(flib, orc):
         v2old = start
     assert relerr(v, vo) < TOL
     assert np.abs(dv - dvo).max() < TOL * np.abs(vo).max()
+
+
+@pytest.mark.parametrize("case", ["sponge_3", "top1", "smooth_base", "inflow_left4"])
+def test_module_driver_deck_extras(flib, case):
+    """the Fortran module driver with a deck's extras (lns3d_hip_sponge,
+    lns3d_hip_top, lns3d_hip_smoother, lns3d_hip_inflow, then
+    lns3d_hip_factor) against one of the reference's steps
+    (tests/golden/ref_<case>.npz)"""
+    import json
+    import os
+    from lns3d_amd.config import Params
+    from tests.penta_cases import relerr
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    z = np.load(os.path.join(gold, f"ref_{case}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"fortran module driver {case} {p.nx}x{p.ny}", flush=True)
+    fp = FParams()
+    for name, _ in FParams._fields_:
+        setattr(fp, name, 0 if name == "precision" else getattr(p, name))
+    fn = lambda s: getattr(flib, "_QMlns3d_hipPlns3d_hip_" + s)
+    D = lambda v: ctypes.byref(ctypes.c_double(v))
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    fn("init")(ctypes.byref(fp), P(met), P(vm))
+    assert status(flib) == 0
+    if case.startswith("sponge"):
+        x = np.asfortranarray(z["x"])
+        fn("sponge")(I(int(z["ispg"])), P(np.asfortranarray(z["spg"])), P(np.asfortranarray(z["spg2"])),
+                     P(x), P(np.ascontiguousarray(z["wamp"])), D(float(z["x0"])))
+    elif case == "top1":
+        fn("top")(P(np.asfortranarray(z["x"])), P(np.ones(p.nx)), D(float(z["x0"])))
+    elif case.startswith("smooth"):
+        fn("smoother")(D(float(z["eps_e"])), P(np.asfortranarray(z["buff"])))
+    else:
+        la = np.array([complex(z["lalpha"]).real, complex(z["lalpha"]).imag])
+        fn("inflow")(P(np.asfortranarray(z["x"])), D(float(z["x0"])), P(np.asfortranarray(z["eig"])), P(la))
+    assert status(flib) == 0
+    fn("factor")()
+    assert status(flib) == 0
+    v = np.asfortranarray(z["itrbc"].copy())
+    fn("put")(P(v))
+    dv = np.zeros_like(v, order="F")
+    fn("advance")(I(1), P(dv))
+    assert status(flib) == 0
+    fn("get")(P(v))
+    fn("free")()
+    assert relerr(dv, z["dv1"]) < TOL
