@@ -178,6 +178,17 @@ int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const double *eig
  * damping profile buff (host (nx,ny)) that src/buffer.f90 (init_buffer)
  * forms from ibuff.  eps_e = 0 turns it off.  Before lns3d_factor. */
 int lns3d_set_smoother(lns3d_ctx *c, double eps_e, const double *buff);
+/* the shift-invert driver's pseudo-time steps (src/si_eigdrv3d_new.f90
+ * :310-440, the operator an ARPACK shift-invert loop applies): one corrector
+ * r = rhsBC3D(vold - v - dtl (lrhs3D(v) - f)) (ebe3D, src/si_eigdrv3d.f90
+ * :319-345), the two line solves with the context's factors, v += r,
+ * itrBC3D.  f, v, vold are device (5,nx,ny) complex; vold NULL means v (the
+ * first corrector of a step); impl = 1. */
+int lns3d_step_forced(lns3d_ctx *c, void *v_dev, const void *vold_dev, const void *f_dev, void *dv_dev);
+/* the same residual r (into r_dev) and its norm sqrt(sum |r|^2 / dtl^2), the
+ * driver's convergence measure (src/si_eigdrv3d_new.f90:359-374) */
+int lns3d_forced_residual(lns3d_ctx *c, const void *v_dev, const void *vold_dev, const void *f_dev,
+                          void *r_dev, double *norm_out);
 /* corrector iterations per step, the deck's niter (src/impdrv3d.f90:173-358):
  * the residual with vold held at the start of the step, solved with the
  * factors of the first iteration, niter times.  lns3d_step only (impl 1 or

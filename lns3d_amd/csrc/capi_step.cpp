@@ -565,6 +565,44 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   return rc;
 }
 
+// One pseudo-time corrector of the shift-invert driver (src/si_eigdrv3d_new
+// .f90:327-358): r = rhsBC3D(vold - v - dtl (lrhs3D(v) - f)), the two line
+// solves with the factors, v += r, itrBC3D.  vold NULL means v (the first
+// corrector of a step); dv_dev NULL keeps the increment in the context.
+extern "C" int lns3d_step_forced(lns3d_ctx *c, void *v_dev, const void *vold_dev, const void *f_dev,
+                                 void *dv_dev) {
+  if (!c || !c->factored || !v_dev || !f_dev)
+    return fail(LNS3D_EINVAL, "lns3d_step_forced: needs a factored context, v and f");
+  if (c->impl != 1)
+    return fail(LNS3D_EUNSUPPORTED, "lns3d_step_forced: the shift-invert steps are backward Euler (impl = 1)");
+  void *r = dv_dev ? dv_dev : c->r;
+  int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 5, c->stream, nullptr,
+                      vold_dev ? vold_dev : v_dev, f_dev);
+  if (!rc) rc = solve_xy(c, c->px, c->py, r);
+  if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
+  return rc;
+}
+
+// The shift-invert driver's residual after a corrector
+// (src/si_eigdrv3d_new.f90:359-374): r = rhsBC3D(vold - v - dtl (lrhs3D(v) -
+// f)) into r_dev and sqrt(sum |r|^2 / dtl^2) into *norm_out (synchronises).
+extern "C" int lns3d_forced_residual(lns3d_ctx *c, const void *v_dev, const void *vold_dev, const void *f_dev,
+                                     void *r_dev, double *norm_out) {
+  if (!c || !c->coeffs_ready || !v_dev || !vold_dev || !f_dev || !r_dev || !norm_out)
+    return fail(LNS3D_EINVAL, "lns3d_forced_residual: bad arguments");
+  int rc = launch_rhs(c->p, c->st, c->c, v_dev, r_dev, c->work, 5, c->stream, nullptr, vold_dev, f_dev);
+  int nb = 0;
+  if (!rc) rc = launch_forced_norm(c->p, c->c, r_dev, c->scal + 16, &nb, c->stream);
+  if (rc) return rc;
+  std::vector<double> part((size_t)nb);
+  LNS3D_CHECK(hipMemcpyAsync(part.data(), c->scal + 16, (size_t)nb * 8, hipMemcpyDeviceToHost, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[b];
+  *norm_out = sqrt(s);
+  return 0;
+}
+
 // corrector iterations per step, the deck's niter (src/impdrv3d.f90:173)
 extern "C" int lns3d_set_corrector_iterations(lns3d_ctx *c, int niter) {
   if (!c || niter < 1) return fail(LNS3D_EINVAL, "lns3d_set_corrector_iterations: niter must be >= 1");

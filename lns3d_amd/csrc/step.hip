@@ -869,7 +869,7 @@ constexpr int NDER = 17;
 __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, Coeffs c,
                                                   const double2 *v, const double2 *c1a, double2 *r,
                                                   int with_rbe, const double2 *v2old,
-                                                  const double2 *vold) {
+                                                  const double2 *vold, const double2 *forcing) {
   using namespace st;
   __shared__ double2 sv[5 * RT], sg1[5 * RT], sd[NDER * RT], sr[5 * RT];
   const int nx = p.nx, ny = p.ny;
@@ -981,6 +981,11 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
       } else if (with_rbe == 4) {
         const z vv = vq(m), vb = ld(vold + 5 * q + m), vo = ld(v2old + 5 * q + m);
         a = sc(onept33, vb) - sc(pt33, vo) - vv - sc(pt66 * dt, a);
+      } else if (with_rbe == 5) {
+        // forced backward Euler of the shift-invert driver (ebe3D,
+        // src/si_eigdrv3d.f90:333-342): r = vold - v - dtl (r - f)
+        const z vv = vq(m), vb = ld(vold + 5 * q + m), f = ld(forcing + 5 * q + m);
+        a = vb - vv - sc(dt, a - f);
       } else {
         a = z{-(dt * a.re), -(dt * a.im)};
       }
@@ -1037,13 +1042,14 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
 }
 
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
-               void *work, int with_rbe, hipStream_t s, const void *v2old, const void *vold) {
+               void *work, int with_rbe, hipStream_t s, const void *v2old, const void *vold,
+               const void *forcing) {
   long P = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(5 * P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
                      (const double2 *)v, (double2 *)work);
   hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, RT)), dim3(5 * 64), 0, s, p, st, c, (const double2 *)v,
                      (const double2 *)work, (double2 *)r, with_rbe, (const double2 *)v2old,
-                     (const double2 *)vold);
+                     (const double2 *)vold, (const double2 *)forcing);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
@@ -1125,6 +1131,39 @@ int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s)
 __global__ void k_copy2(double2 *dst, const double2 *src, long n) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = src[t];
+}
+
+// the shift-invert driver's residual norm terms (src/si_eigdrv3d_new.f90
+// :360-372): sum of |r|^2 / dtl^2 over the points of each block
+__global__ __launch_bounds__(256) void k_forced_norm(const double2 *r, const double *dtl, long P,
+                                                     double *part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < P; q += (long)gridDim.x * 256) {
+    const double d2 = dtl[q] * dtl[q];
+    for (int e = 0; e < 5; e++) {
+      const double2 v = r[5 * q + e];
+      acc += (v.x * v.x + v.y * v.y) / d2;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,
+                       hipStream_t s) {
+  const long P = (long)p.nx * p.ny;
+  const int blocks = (int)std::min<long>(nblocks(P, 256), 1024);
+  hipLaunchKernelGGL(k_forced_norm, dim3(blocks), dim3(256), 0, s, (const double2 *)r, (const double *)c.dtl,
+                     P, part);
+  LNS3D_CHECK(hipGetLastError());
+  *nb = blocks;
+  return 0;
 }
 
 // device copy as a kernel (graph-capture friendly)

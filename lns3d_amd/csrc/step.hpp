@@ -65,10 +65,15 @@ int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *m
 // r = -dtl * L(v) with the boundary rows of rhsBC3D; work holds 5P complex
 // with_rbe: 0 the operator only, 1 backward Euler (rbe3D), 2 two-step
 // backward (rtb3D) with v2old, the field at the start of the previous step,
-// both with vold = v; 3 and 4 the same with vold given (corrector iterations)
+// both with vold = v; 3 and 4 the same with vold given (corrector iterations);
+// 5 the forced form vold - v - dtl (L(v) - f) of the shift-invert driver
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
                void *work, int with_rbe, hipStream_t s, const void *v2old = nullptr,
-               const void *vold = nullptr);
+               const void *vold = nullptr, const void *forcing = nullptr);
+// sum over the points of |r|^2 / dtl^2, per block into part (nblocks(P, 256)
+// doubles); the caller adds them
+int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,
+                       hipStream_t s);
 // ftop: the top = 1 wave (Coeffs::ftop), null for top = 0
 int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, const void *dv, hipStream_t s);
 int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s);

@@ -66,6 +66,8 @@ SIGNATURES = {
     "lns3d_set_corrector_iterations": ([VP, CI], CI),
     "lns3d_set_inflow": ([VP, VP, ctypes.c_double, VP, VP], CI),
     "lns3d_set_smoother": ([VP, ctypes.c_double, VP], CI),
+    "lns3d_step_forced": ([VP, VP, VP, VP, VP], CI),
+    "lns3d_forced_residual": ([VP, VP, VP, VP, VP, ctypes.POINTER(ctypes.c_double)], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -195,6 +197,20 @@ class Step:
         la = np.array([complex(lalpha).real, complex(lalpha).imag])
         hp = lambda a: None if a is None else a.ctypes.data
         check(lib().lns3d_set_inflow(self.h, hp(xa), float(x0), hp(ea), hp(la)), "lns3d_set_inflow")
+
+    def step_forced(self, v_dev, f_dev, vold_dev=None, dv_dev=None):
+        """one corrector of the shift-invert driver's pseudo-time steps
+        (ebe3D forcing f); vold None: v"""
+        check(lib().lns3d_step_forced(self.h, ptr(v_dev), None if vold_dev is None else ptr(vold_dev),
+                                      ptr(f_dev), None if dv_dev is None else ptr(dv_dev)),
+              "lns3d_step_forced")
+
+    def forced_residual(self, v_dev, vold_dev, f_dev, r_dev):
+        """the driver's residual into r_dev; returns sqrt(sum |r|^2 / dtl^2)"""
+        out = ctypes.c_double(0.0)
+        check(lib().lns3d_forced_residual(self.h, ptr(v_dev), ptr(vold_dev), ptr(f_dev), ptr(r_dev),
+                                          ctypes.byref(out)), "lns3d_forced_residual")
+        return out.value
 
     def set_smoother(self, eps_e, buff=None):
         """smoother eps_e with the damping profile buff (nx, ny); 0: off"""

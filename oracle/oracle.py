@@ -279,6 +279,22 @@ class Reference:
         self._buff_keep = np.asfortranarray(buff, dtype=np.float64)
         self.lib.ref_set_smoother(ctypes.byref(ctypes.c_double(eps_e)), _ptr(self._buff_keep))
 
+    def set_forcing(self, f=None):
+        """the shift-invert driver's forcing f (5, nx, ny): steps then use ebe3D
+        (None: rbe3D again)"""
+        if f is None:
+            self.lib.ref_set_forcing(None, ctypes.c_int(0))
+        else:
+            self._f_keep = np.asfortranarray(f, dtype=np.complex128)
+            self.lib.ref_set_forcing(_ptr(self._f_keep), ctypes.c_int(1))
+
+    def forced_residual(self, v, vold):
+        r = np.zeros_like(v, order="F")
+        nrm = ctypes.c_double(0.0)
+        self.lib.ref_forced_residual(_ptr(np.asfortranarray(v)), _ptr(np.asfortranarray(vold)), _ptr(r),
+                                     ctypes.byref(nrm))
+        return r, nrm.value
+
     def set_niter(self, n):
         """corrector iterations per step (the deck's niter)"""
         self.lib.ref_set_niter(ctypes.c_int(n))

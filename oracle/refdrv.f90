@@ -17,6 +17,9 @@ module refdrv_state
   ! impl = 2: the field at the start of the previous step, and the steps taken
   complex, allocatable :: vhist(:,:,:), v2old(:,:,:)
   integer :: lstep_h = 0
+  ! forcing of the shift-invert driver's steps (src/si_eigdrv3d_new.f90:325-
+  ! 358 with ebe3D, src/si_eigdrv3d.f90:319-345); unallocated: none
+  complex, allocatable :: fsi(:,:,:)
 end module refdrv_state
 
 !-----------------------------------------------------------------------------
@@ -352,6 +355,17 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
     call lrhs3D(v, r, vm, x, y)
     call rtb3D(r, v, vold, v2old, dtlh)
     alfa = pt66
+  else if (allocated(fsi)) then
+    ! ebe3D of src/si_eigdrv3d.f90:333-342: r = vold - v - dtl (r - f)
+    call lrhs3D(v, r, vm, x, y)
+    do j = 1, ny
+      do i = 1, nx
+        do idof = 1, ndof
+          r(idof,i,j) = vold(idof,i,j) - v(idof,i,j) - dtlh(i,j) * (r(idof,i,j) - fsi(idof,i,j))
+        end do
+      end do
+    end do
+    alfa = one
   else
     call lrhs3D(v, r, vm, x, y)
     call rbe3D(r, v, vold, dtlh)
@@ -413,6 +427,60 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   lstep_h = lstep_h + 1
   deallocate(vold, r)
 end subroutine ref_step
+
+! the forcing f of the shift-invert driver's pseudo-time steps (set: the
+! steps use ebe3D; clear: rbe3D)
+subroutine ref_set_forcing(f, on) bind(C, name="ref_set_forcing")
+  use iso_c_binding
+  use global
+  use refdrv_state
+  implicit none
+  complex(c_double_complex), intent(in) :: f(ndof,nx,ny)
+  integer(c_int), value :: on
+  if (allocated(fsi)) deallocate(fsi)
+  if (on /= 0) then
+    allocate(fsi(ndof,nx,ny))
+    fsi = f
+  end if
+end subroutine ref_set_forcing
+
+! the shift-invert driver's residual norm after a step
+! (src/si_eigdrv3d_new.f90:359-374): r = rhsBC3D(ebe3D(lrhs3D(v))) with
+! vold, and sqrt(sum |r|^2 / dtl^2)
+subroutine ref_forced_residual(v, vold, r, nrm) bind(C, name="ref_forced_residual")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  use refdrv_state
+  implicit none
+  complex(c_double_complex), intent(in) :: v(ndof,nx,ny), vold(ndof,nx,ny)
+  complex(c_double_complex), intent(out) :: r(ndof,nx,ny)
+  real(c_double), intent(out) :: nrm
+  complex, allocatable :: vl(:,:,:)
+  integer :: i, j, idof
+  allocate(vl(ndof,nx,ny))
+  vl = v
+  call lrhs3D(vl, r, vm, x, y)
+  do j = 1, ny
+    do i = 1, nx
+      do idof = 1, ndof
+        r(idof,i,j) = vold(idof,i,j) - vl(idof,i,j) - dtlh(i,j) * (r(idof,i,j) - fsi(idof,i,j))
+      end do
+    end do
+  end do
+  call rhsBC3D(r, vl, vm)
+  nrm = 0.0
+  do j = 1, ny
+    do i = 1, nx
+      do idof = 1, ndof
+        nrm = nrm + real(r(idof,i,j) * conjg(r(idof,i,j))) / dtlh(i,j)**2
+      end do
+    end do
+  end do
+  nrm = sqrt(nrm)
+  deallocate(vl)
+end subroutine ref_forced_residual
 
 ! corrector iterations per step (the deck's niter, src/impdrv3d.f90:173)
 subroutine ref_set_niter(n) bind(C, name="ref_set_niter")

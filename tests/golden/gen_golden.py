@@ -42,6 +42,9 @@ reference's outputs:
   ref_smooth_<case>.npz the smoother eps_e (src/smoother3d.f90, implicit
                         damping) with a seeded buff: base, symmetric ends,
                         periodic x, periodic y; lrhs3D, factors, two steps
+  ref_forced.npz        the shift-invert driver's pseudo-time steps (ebe3D with
+                        a seeded forcing): two steps of two correctors, the
+                        driver's residual and norm after each
   ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
                         (with wall = 1, wallt = 1) and mse (the boundary set
                         of test/mse-recep/complex.inp: top 1, left 7); the
@@ -52,6 +55,7 @@ reference's outputs:
     python tests/golden/gen_golden.py sym        (the symmetry cases only)
     python tests/golden/gen_golden.py inflow     (the left/right inflow and ispg = 4 cases only)
     python tests/golden/gen_golden.py smooth     (the smoother cases only)
+    python tests/golden/gen_golden.py forced     (the shift-invert steps only)
 """
 from __future__ import annotations
 
@@ -422,6 +426,39 @@ def smooth_case(R, name, kw):
     np.savez_compressed(os.path.join(HERE, f"ref_smooth_{name}.npz"), **out)
 
 
+def forced_case(R):
+    """the shift-invert driver's pseudo-time steps (src/si_eigdrv3d_new.f90
+    :310-374; ebe3D of src/si_eigdrv3d.f90:319-345): a seeded forcing f, two
+    steps of two correctors each (vold held at the step's start), and the
+    driver's residual and norm after each step"""
+    p = Params(nx=13, ny=9, kz=0.4, omega=0.3, wall=1, wallt=1)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(47)
+    f = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    v0 = np.asfortranarray(0.1 * (rng.standard_normal((5, p.nx, p.ny)) +
+                                  1j * rng.standard_normal((5, p.nx, p.ny))))
+    R.setup(p, x, y, met, vm)
+    R.genmtrx()
+    R.set_time_step()
+    R.set_forcing(f)
+    R.set_niter(2)
+    out = dict(params=_params_json(p), met=met, vm=vm, v0=v0, f=f)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    for k in (1, 2):
+        start = v.copy(order="F")
+        out[f"dv{k}"] = R.step(v, k == 1)
+        out[f"v{k}"] = v.copy()
+        r, nrm = R.forced_residual(v, start)
+        out[f"res{k}"] = r
+        out[f"nrm{k}"] = np.array(nrm)
+    R.set_niter(1)
+    R.set_forcing(None)
+    np.savez_compressed(os.path.join(HERE, "ref_forced.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -439,6 +476,10 @@ def main():
     if sys.argv[1:] == ["sym"]:
         for name, kw in SYM_CASES.items():
             sym_case(R, name, kw)
+        R.lib.ref_flush()
+        return
+    if sys.argv[1:] == ["forced"]:
+        forced_case(R)
         R.lib.ref_flush()
         return
     if sys.argv[1:] == ["smooth"]:
@@ -470,6 +511,7 @@ def main():
     sponge4_case(R)
     for name, kw in SMOOTH_CASES.items():
         smooth_case(R, name, kw)
+    forced_case(R)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

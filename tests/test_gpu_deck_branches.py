@@ -17,7 +17,9 @@ The time schemes impl = 2 and 3 (lns3d_set_time_scheme) and the corrector
 iterations (lns3d_set_corrector_iterations) over three of the reference's
 steps.  The left/right inflow branches (lns3d_set_inflow: left = 1, left = 4,
 right = 4) against tests/golden/ref_inflow_<case>.npz.  The smoother
-(lns3d_set_smoother) against tests/golden/ref_smooth_<case>.npz."""
+(lns3d_set_smoother) against tests/golden/ref_smooth_<case>.npz.  The
+shift-invert driver's forced steps (lns3d_step_forced) against
+tests/golden/ref_forced.npz."""
 import os
 
 import numpy as np
@@ -478,3 +480,58 @@ def test_smoother_against_reference(case):
     print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
     assert res <= 1e-12 and e < 1e-10
     st32.close()
+
+
+def test_forced_steps_against_reference():
+    """lns3d_step_forced / lns3d_forced_residual: the shift-invert driver's
+    pseudo-time steps (ebe3D with a forcing f, src/si_eigdrv3d_new.f90
+    :310-374): two steps of two correctors each, vold held at the step's
+    start, against the reference's (tests/golden/ref_forced.npz).  The
+    returned increment is the second corrector's, small against the field, so
+    it is compared at 1e-12 of max|v| (as the corrector iterations are); the
+    fields at 1e-12 relative.  The driver's residual is a small difference of
+    O(|v|) terms (vold - v - dtl (L(v) - f)), so it is compared at 1e-12 of
+    max|v| and its norm at 1e-9 relative; the norm's reduction is checked
+    against the device residual itself at 1e-13"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step
+    z = np.load(os.path.join(GOLD, "ref_forced.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"forced steps {p.nx}x{p.ny}", flush=True)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t):
+        return t.cpu().numpy().reshape(z["v0"].shape, order="F")
+
+    st = Step(p)
+    st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+    st.factor()
+    _, dtl = st.get_coeffs()
+    fd = dev(z["f"])
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    rd = torch.zeros_like(vd)
+    for k in (1, 2):
+        start = vd.clone()
+        st.step_forced(vd, fd, None, dvd)            # first corrector: vold = v
+        st.step_forced(vd, fd, start, dvd)           # second: vold held
+        scale = float(np.abs(host(start)).max())
+        e = float(np.abs(host(dvd) - z[f"dv{k}"]).max()) / scale
+        print(f"  step {k}: dv err/max|v| {e:.2e}", flush=True)
+        assert e < TOL, k
+        assert rel(host(vd), z[f"v{k}"]) < TOL, k
+        nrm = st.forced_residual(vd, start, fd, rd)
+        scale = float(np.abs(host(start)).max())
+        err = float(np.abs(host(rd) - z[f"res{k}"]).max()) / scale
+        print(f"  residual: err/max|v| {err:.2e}, norm {nrm:.6e} vs {float(z[f'nrm{k}']):.6e}", flush=True)
+        assert err < TOL, k
+        assert abs(nrm - float(z[f"nrm{k}"])) < 1e-9 * float(z[f"nrm{k}"]), k
+        own = np.sqrt((np.abs(host(rd)) ** 2 / dtl[None] ** 2).sum())
+        assert abs(nrm - own) < 1e-13 * own, k
+    st.close()
