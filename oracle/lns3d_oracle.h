@@ -90,6 +90,12 @@ int orc_set_top(const double *x, const double *wamp, const double *vm, double x0
  * tr, ti; lalpha (re, im) for right = 4 (NULL: 0) */
 int orc_set_inflow(const double *x, const double *vm, double x0, const double *eig,
                    const double *lalpha);
+/* the shift-invert driver's forcing f (5,nx,ny) (NULL: off): steps then use
+ * ebe3D; orc_forced_residual gives its residual after a step and the norm */
+int orc_set_forcing(const zdouble *f);
+int orc_forced_residual(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                        const zdouble *v, const zdouble *vold, zdouble *r,
+                        zdouble *work /* 5 arrays of (5,nx,ny) */, double *norm);
 /* smoother eps_e with the damping profile buff (nx,ny) (eps_e 0: off) */
 int orc_set_smoother(double eps_e, const double *buff);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,

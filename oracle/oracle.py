@@ -79,6 +79,21 @@ class Oracle:
         self.lib.orc_set_top(*(_ptr(a) for a in arrs), x0)
         self._top_keep = arrs
 
+    def set_forcing(self, f=None):
+        """the shift-invert driver's forcing f (5, nx, ny); None: off"""
+        self._force_keep = None if f is None else np.asfortranarray(f, dtype=np.complex128)
+        self.lib.orc_set_forcing(_ptr(self._force_keep))
+
+    def forced_residual(self, p, c, dtl, v, vold):
+        r = np.zeros_like(v, order="F")
+        work = np.zeros(v.shape + (5,), complex, order="F")
+        nrm = ctypes.c_double(0.0)
+        ps, cs = self.params(p), self._coeffs(c)
+        assert self.lib.orc_forced_residual(ctypes.byref(ps), ctypes.byref(cs), _ptr(dtl), _ptr(v),
+                                            _ptr(np.asfortranarray(vold)), _ptr(r), _ptr(work),
+                                            ctypes.byref(nrm)) == 0
+        return r, nrm.value
+
     def set_smoother(self, eps_e=0.0, buff=None):
         """smoother eps_e with the damping profile buff (nx, ny); 0: off"""
         b = None if buff is None else np.asfortranarray(buff, dtype=np.float64)

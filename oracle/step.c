@@ -1265,6 +1265,46 @@ static int solve_update(const orc_params *p, zdouble *matx, zdouble *maty, zdoub
  * the field at the start of the step); v2old NULL: backward Euler (rbe3D),
  * else the two-step backward residual of impl = 2 (rtb3D,
  * src/impdrv3d.f90:187-190); half: the residual at 0.5 (v + vold) (impl = 3) */
+/* the shift-invert driver's forcing (src/si_eigdrv3d_new.f90:325-374): when
+ * set, the steps use ebe3D, r = vold - v - dtl (r - f) (src/si_eigdrv3d.f90
+ * :333-342), in place of rbe3D */
+static const zdouble *g_force;
+
+int orc_set_forcing(const zdouble *f) {
+  g_force = f;
+  return 0;
+}
+
+static void ebe3d(const orc_params *p, zdouble *r, const zdouble *v, const zdouble *vold,
+                  const double *dtl) {
+  const long P = (long)p->nx * p->ny;
+  for (long q = 0; q < P; q++)
+    for (int m = 0; m < 5; m++)
+      r[5 * q + m] = vold[5 * q + m] - v[5 * q + m] - dtl[q] * (r[5 * q + m] - g_force[5 * q + m]);
+}
+
+/* the driver's residual after a step and its norm sqrt(sum |r|^2 / dtl^2)
+ * (src/si_eigdrv3d_new.f90:359-374) */
+int orc_forced_residual(const orc_params *p, const orc_coeffs *c, const double *dtl,
+                        const zdouble *v, const zdouble *vold, zdouble *r, zdouble *work,
+                        double *norm) {
+  if (!g_force) return -1;
+  const long P = (long)p->nx * p->ny;
+  int rc = orc_lrhs3d(p, c, v, r, work);
+  if (rc) return rc;
+  ebe3d(p, r, v, vold, dtl);
+  rc = orc_rhsbc3d(p, r, v);
+  if (rc) return rc;
+  double s = 0.0;
+  for (long q = 0; q < P; q++)
+    for (int m = 0; m < 5; m++) {
+      const zdouble e = r[5 * q + m];
+      s += (creal(e) * creal(e) + cimag(e) * cimag(e)) / (dtl[q] * dtl[q]);
+    }
+  *norm = sqrt(s);
+  return 0;
+}
+
 static int step_core(const orc_params *p, const orc_coeffs *c, const double *dtl,
                      zdouble *matx, zdouble *maty, zdouble *perx, zdouble *per2x,
                      zdouble *pery, zdouble *per2y, zdouble *v, const zdouble *v2old,
@@ -1285,6 +1325,7 @@ static int step_core(const orc_params *p, const orc_coeffs *c, const double *dtl
     }
     if (rc) break;
     if (v2old) orc_rtb3d(p, r, v, vold, v2old, dtl);
+    else if (g_force) ebe3d(p, r, v, vold, dtl);
     else orc_rbe3d(p, r, v, vold, dtl);
     rc = solve_update(p, matx, maty, perx, per2x, pery, per2y, v, r, dv);
   }

@@ -425,3 +425,36 @@ def test_smoother(orc, case):
         assert rel(v, z["v2"]) < 1e-12
     finally:
         orc.set_smoother(0.0)
+
+
+def test_forced_steps(orc):
+    """the shift-invert driver's pseudo-time steps (ebe3D with a forcing f,
+    src/si_eigdrv3d_new.f90:310-374, src/si_eigdrv3d.f90:319-345): two steps
+    of two correctors each and the driver's residual and norm after each,
+    against the reference's through the harness (ref_forced.npz); the last
+    corrector's increment and the residual are small differences of O(|v|)
+    terms, so they are compared at 1e-12 of max|v|; without the forcing the
+    step ends elsewhere"""
+    z = load("ref_forced.npz")
+    p = params_of(z)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    f = orc.factor(p, c, dtl)
+    v0 = np.asfortranarray(z["itrbc"].copy())
+    orc.step_iter(p, c, dtl, f, v0, 2)
+    assert rel(v0, z["v1"]) > 1e-3
+    try:
+        orc.set_forcing(z["f"])
+        v = np.asfortranarray(z["itrbc"].copy())
+        for k in (1, 2):
+            start = v.copy(order="F")
+            scale = np.abs(start).max()
+            dv = orc.step_iter(p, c, dtl, f, v, 2)
+            assert np.abs(dv - z[f"dv{k}"]).max() < 1e-12 * scale, k
+            assert rel(v, z[f"v{k}"]) < 1e-12, k
+            r, nrm = orc.forced_residual(p, c, dtl, v, start)
+            assert np.abs(r - z[f"res{k}"]).max() < 1e-12 * scale, k
+            assert abs(nrm - float(z[f"nrm{k}"])) < 1e-9 * float(z[f"nrm{k}"]), k
+    finally:
+        orc.set_forcing(None)
