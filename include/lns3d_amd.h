@@ -235,7 +235,11 @@ int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev);
 int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev, int corrections, int check);
 int lns3d_last_residual(lns3d_ctx *c, double *rel);
 /* device bytes a factored context of these parameters holds (admission by
- * memory budget in the serving loop) */
+ * memory budget in the serving loop).  The optional setters add their own
+ * blocks on first use (P = nx ny): lns3d_set_sponge 8 (14 P + nx),
+ * lns3d_set_smoother 8 P, lns3d_set_time_scheme(2) and
+ * lns3d_set_corrector_iterations(> 1) 80 P each, lns3d_set_top_forcing
+ * 96 nx, lns3d_set_inflow 176 ny. */
 int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
 
 /* Graphs of steps: nsteps steps of each of count factored contexts, every
