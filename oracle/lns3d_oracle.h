@@ -96,6 +96,8 @@ int orc_set_forcing(const zdouble *f);
 int orc_forced_residual(const orc_params *p, const orc_coeffs *c, const double *dtl,
                         const zdouble *v, const zdouble *vold, zdouble *r,
                         zdouble *work /* 5 arrays of (5,nx,ny) */, double *norm);
+/* bump wall values for wall = 3: uw (nx,5) complex u1w, u2w, u3w, tw, twp */
+int orc_set_bump(const zdouble *uw);
 /* smoother eps_e with the damping profile buff (nx,ny) (eps_e 0: off) */
 int orc_set_smoother(double eps_e, const double *buff);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,

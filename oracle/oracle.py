@@ -79,6 +79,11 @@ class Oracle:
         self.lib.orc_set_top(*(_ptr(a) for a in arrs), x0)
         self._top_keep = arrs
 
+    def set_bump(self, uw=None):
+        """bump wall values uw (nx, 5) complex (wall = 3); None: cleared"""
+        self._bump_keep = None if uw is None else np.asfortranarray(uw, dtype=np.complex128)
+        self.lib.orc_set_bump(_ptr(self._bump_keep))
+
     def set_forcing(self, f=None):
         """the shift-invert driver's forcing f (5, nx, ny); None: off"""
         self._force_keep = None if f is None else np.asfortranarray(f, dtype=np.complex128)
@@ -309,6 +314,11 @@ class Reference:
         self.lib.ref_forced_residual(_ptr(np.asfortranarray(v)), _ptr(np.asfortranarray(vold)), _ptr(r),
                                      ctypes.byref(nrm))
         return r, nrm.value
+
+    def set_bump(self, uw):
+        """the bump wall values uw (nx, 5) complex: u1w, u2w, u3w, tw, twp"""
+        self._bump_keep = np.asfortranarray(uw, dtype=np.complex128)
+        self.lib.ref_set_bump(_ptr(self._bump_keep))
 
     def set_niter(self, n):
         """corrector iterations per step (the deck's niter)"""

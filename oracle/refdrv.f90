@@ -135,6 +135,19 @@ subroutine ref_set_sponge(spgl, spg2l, wampl) bind(C, name="ref_set_sponge")
   spg = spgl; spg2 = spg2l; wamp = wampl
 end subroutine ref_set_sponge
 
+! the bump wall (wall = 3): the wall values src/genbump.f90:74-131 forms,
+! uw(nx,5) complex: u1w, u2w, u3w, tw, twp
+subroutine ref_set_bump(uw) bind(C, name="ref_set_bump")
+  use iso_c_binding
+  use global
+  use bump
+  implicit none
+  complex(c_double_complex), intent(in) :: uw(nx,5)
+  if (allocated(u1w)) deallocate(u1w, u2w, u3w, tw, twp)
+  allocate(u1w(nx), u2w(nx), u3w(nx), tw(nx), twp(nx))
+  u1w = uw(:,1); u2w = uw(:,2); u3w = uw(:,3); tw = uw(:,4); twp = uw(:,5)
+end subroutine ref_set_bump
+
 ! eigenfunction inflow (left = 4, right = 4): the profiles src/gendist.f90
 ! reads from inflow_file (columns rhor, rhoi, ur, ui, vr, vi, wr, wi, tr, ti)
 ! and the inflow namelist's lalpha, lomega

@@ -244,7 +244,7 @@ void orc_grad2(const orc_params *p, int ndof, const double *v, const double *g1,
 
 static int supported(const orc_params *p) {
   if (p->nx < 7 || p->ny < 7) return 0;
-  if (p->wall < 0 || p->wall > 2 || p->wallt < 0 || p->wallt > 2) return 0;
+  if (p->wall < 0 || p->wall > 3 || p->wallt < 0 || p->wallt > 2) return 0;
   if ((p->top != 0 && p->top != 1) ||
       (p->left != 0 && p->left != 1 && p->left != 4 && p->left != 5 && p->left != 7) ||
       (p->right != 0 && p->right != 4 && p->right != 5 && p->right != 7 && p->right != 8))
@@ -266,6 +266,14 @@ static int supported(const orc_params *p) {
 /* ---- sponge: the module state src/lrhs3d.f90:222-289 and the LHS routines
  *      read (spg, spg2 of src/sponge.f90:20, wamp, x0, omega) ----------------- */
 static orc_sponge g_spg;          /* ispg = 0: no sponge */
+/* the bump wall (wall = 3): the wall values of src/genbump.f90:74-131, uw
+ * (nx,5) complex: u1w, u2w, u3w, tw, twp */
+static const zdouble *g_bump;
+
+int orc_set_bump(const zdouble *uw) {
+  g_bump = uw;
+  return 0;
+}
 static struct { const double *x, *vm, *eig; double x0; zdouble lalpha; } g_inf;   /* see orc_set_inflow */
 
 int orc_set_sponge(const orc_sponge *s) {
@@ -757,7 +765,7 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
   if (!p->yper) {
     for (int i = 0; i < nx; i++) {
       long pt = i;
-      if (dir == 1 && (p->wall == 1 || p->wall == 2)) {
+      if (dir == 1 && (p->wall == 1 || p->wall == 2 || p->wall == 3)) {
         for (int k = 0; k < 5; k++)
           for (int cc = 0; cc < 5; cc++) MAT(k, 0, cc) = 0;
         MAT(2, 0, 0) = 1.0;
@@ -767,6 +775,12 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
           for (int cc = 0; cc < 5; cc++) MAT(k, 0, cc) = 0;
         MAT(2, 0, 0) = -gc1; MAT(3, 0, 0) = -gc2; MAT(4, 0, 0) = -gc3;
         MAT(0, 0, 0) = -gc4; MAT(1, 0, 0) = -gc5;
+      }
+      if (dir == 2 && p->wall == 3) {            /* src/lhsbc2f3d.f90:41-48 */
+        for (int k = 0; k < 5; k++)
+          for (int cc = 0; cc < 5; cc++) MAT(k, 0, cc) = 0;
+        MAT(2, 0, 0) = -1.0; MAT(3, 0, 0) = 4.0; MAT(4, 0, 0) = -6.0;
+        MAT(0, 0, 0) = 4.0; MAT(1, 0, 0) = -1.0;
       }
       for (int r = 1; r <= 3; r++)
         for (int k = 0; k < 5; k++)
@@ -823,16 +837,22 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
     for (int i = 0; i < nx; i++)
       for (int d = 0; d < 5; d++) R(d, i, ny - 1) = 0;
   } else {
+    if (p->wall == 3 && !g_bump) return -1;
     for (int i = 0; i < nx; i++) {
-      for (int d = 1; d <= 3; d++) R(d, i, 0) = -(V(d, i, 0) - 0.0);
+      for (int d = 1; d <= 3; d++)              /* wall = 3: the bump's velocities */
+        R(d, i, 0) = -(V(d, i, 0) - (p->wall == 3 ? g_bump[i + (long)nx * (d - 1)] : 0.0));
       if (p->wall == 1)
         R(0, i, 0) = gc1 * V(0, i, 0) + gc2 * V(0, i, 1) + gc3 * V(0, i, 2) +
                      gc4 * V(0, i, 3) + gc5 * V(0, i, 4);
       if (p->wall == 2) R(0, i, 0) = V(0, i, 0) - 2.0 * V(0, i, 1) + V(0, i, 2);
-      if (p->wallt == 0) R(4, i, 0) = 0;
-      if (p->wallt == 1)
+      if (p->wall == 3)                          /* src/rhsbc3d.f90:74-79 */
+        R(0, i, 0) = V(0, i, 0) - (4.0 * V(0, i, 1) - 6.0 * V(0, i, 2) + 4.0 * V(0, i, 3) - V(0, i, 4));
+      if (p->wallt == 0) R(4, i, 0) = p->wall == 3 ? -(V(4, i, 0) - g_bump[i + 3L * nx]) : 0;
+      if (p->wallt == 1) {
         R(4, i, 0) = gc1 * V(4, i, 0) + gc2 * V(4, i, 1) + gc3 * V(4, i, 2) +
                      gc4 * V(4, i, 3) + gc5 * V(4, i, 4);
+        if (p->wall == 3) R(4, i, 0) = R(4, i, 0) - g_bump[i + 4L * nx];
+      }
     }
     for (int i = 0; i < nx; i++) {
       if (p->top == 1) {                           /* src/rhsbc3d.f90:129-161 */
@@ -875,16 +895,24 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
     for (int i = 0; i < nx; i++)
       for (int d = 0; d < 5; d++) V(d, i, ny - 1) = V(d, i, 0);
   } else {
+    if (p->wall == 3 && !g_bump) return -1;
     for (int i = 0; i < nx; i++) {
-      V(1, i, 0) = 0; V(2, i, 0) = 0; V(3, i, 0) = 0;
+      for (int d = 1; d <= 3; d++) V(d, i, 0) = p->wall == 3 ? g_bump[i + (long)nx * (d - 1)] : 0;
       if (p->wall == 1)
         V(0, i, 0) = -(gc2 * V(0, i, 1) + gc3 * V(0, i, 2) + gc4 * V(0, i, 3) +
                        gc5 * V(0, i, 4)) / gc1;
       if (p->wall == 2) V(0, i, 0) = 2.0 * V(0, i, 1) - V(0, i, 2);
-      if (p->wallt == 0) V(4, i, 0) = 0;
-      if (p->wallt == 1)
-        V(4, i, 0) = -(gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
-                       gc5 * V(4, i, 4)) / gc1;
+      if (p->wall == 3)                          /* src/itrbc3d.f90:76-81 */
+        V(0, i, 0) = 4.0 * V(0, i, 1) - 6.0 * V(0, i, 2) + 4.0 * V(0, i, 3) - V(0, i, 4);
+      if (p->wallt == 0) V(4, i, 0) = p->wall == 3 ? g_bump[i + 3L * nx] : 0;
+      if (p->wallt == 1) {
+        if (p->wall == 3)                        /* src/itrbc3d.f90:93-104 */
+          V(4, i, 0) = (g_bump[i + 4L * nx] - (gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
+                                                gc5 * V(4, i, 4))) / gc1;
+        else
+          V(4, i, 0) = -(gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
+                         gc5 * V(4, i, 4)) / gc1;
+      }
     }
     for (int i = 0; i < nx; i++) {
       if (p->top == 1) {                           /* src/itrbc3d.f90:145-195 */

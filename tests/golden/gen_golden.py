@@ -45,6 +45,8 @@ reference's outputs:
   ref_forced.npz        the shift-invert driver's pseudo-time steps (ebe3D with
                         a seeded forcing): two steps of two correctors, the
                         driver's residual and norm after each
+  ref_bump_t<wallt>.npz the bump wall (wall = 3) with seeded wall values, isothermal
+                        and adiabatic: itrBC3D, lrhs3D, rhsBC3D, both factors, two steps
   ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
                         (with wall = 1, wallt = 1) and mse (the boundary set
                         of test/mse-recep/complex.inp: top 1, left 7); the
@@ -56,6 +58,7 @@ reference's outputs:
     python tests/golden/gen_golden.py inflow     (the left/right inflow and ispg = 4 cases only)
     python tests/golden/gen_golden.py smooth     (the smoother cases only)
     python tests/golden/gen_golden.py forced     (the shift-invert steps only)
+    python tests/golden/gen_golden.py bump       (the bump wall cases only)
 """
 from __future__ import annotations
 
@@ -459,6 +462,37 @@ def forced_case(R):
     np.savez_compressed(os.path.join(HERE, "ref_forced.npz"), **out)
 
 
+def bump_case(R, wallt):
+    """the bump wall, wall = 3 (src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105,
+    src/lhsbc2f3d.f90:41-48): seeded wall values u1w, u2w, u3w, tw, twp in
+    place of the ones src/genbump.f90 forms from a bump shape"""
+    p = Params(nx=13, ny=9, kz=0.4, omega=0.3, wall=3, wallt=wallt)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(53 + wallt)
+    uw = np.asfortranarray(0.1 * (rng.standard_normal((p.nx, 5)) + 1j * rng.standard_normal((p.nx, 5))))
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    R.setup(p, x, y, met, vm)
+    R.set_bump(uw)
+    R.genmtrx()
+    R.set_time_step()
+    out = dict(params=_params_json(p), met=met, vm=vm, v0=v0, uw=uw)
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    rb = np.asfortranarray(r.copy())
+    R.rhsbc3d(rb, np.asfortranarray(v0))
+    out["lrhs"] = r
+    out["rhsbc"] = rb
+    out["lhs1"] = R.lhs(1)
+    out["lhs2"] = R.lhs(2)
+    out["dv1"] = R.step(v, True)
+    out["dv2"] = R.step(v, False)
+    out["v2"] = v.copy()
+    np.savez_compressed(os.path.join(HERE, f"ref_bump_t{wallt}.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -476,6 +510,11 @@ def main():
     if sys.argv[1:] == ["sym"]:
         for name, kw in SYM_CASES.items():
             sym_case(R, name, kw)
+        R.lib.ref_flush()
+        return
+    if sys.argv[1:] == ["bump"]:
+        for wallt in (0, 1):
+            bump_case(R, wallt)
         R.lib.ref_flush()
         return
     if sys.argv[1:] == ["forced"]:
@@ -512,6 +551,8 @@ def main():
     for name, kw in SMOOTH_CASES.items():
         smooth_case(R, name, kw)
     forced_case(R)
+    for wallt in (0, 1):
+        bump_case(R, wallt)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

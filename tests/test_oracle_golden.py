@@ -458,3 +458,37 @@ def test_forced_steps(orc):
             assert abs(nrm - float(z[f"nrm{k}"])) < 1e-9 * float(z[f"nrm{k}"]), k
     finally:
         orc.set_forcing(None)
+
+
+@pytest.mark.parametrize("wallt", [0, 1])
+def test_bump_wall(orc, wallt):
+    """the bump wall, wall = 3 (src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105,
+    src/lhsbc1f3d.f90:30, src/lhsbc2f3d.f90:41-48): the wall held at given
+    velocities and temperature (or temperature gradient), the density by a
+    third-order extrapolation, against the reference's routines
+    (ref_bump_t<wallt>.npz)"""
+    z = load(f"ref_bump_t{wallt}.npz")
+    p = params_of(z)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    v0 = np.asfortranarray(z["v0"])
+    c = orc.genmtrx(p, vm, met)
+    dtl = orc.set_time_step(p, vm, met)
+    try:
+        orc.set_bump(z["uw"])
+        v = np.asfortranarray(v0.copy())
+        orc.itrbc3d(p, v)
+        assert rel(v, z["itrbc"]) < 1e-14
+        # the wall velocities (the corners belong to the left and right columns)
+        assert np.abs(v[1:4, 1:-1, 0] - z["uw"][1:-1, 0:3].T).max() == 0.0
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        orc.rhsbc3d(p, r, v0)
+        assert rel(r, z["rhsbc"]) < 1e-14
+        for d in (1, 2):
+            assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+        assert rel(orc.step(p, c, dtl, f, v), z["dv2"]) < 1e-12
+        assert rel(v, z["v2"]) < 1e-12
+    finally:
+        orc.set_bump(None)
