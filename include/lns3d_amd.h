@@ -28,7 +28,8 @@ const char *lns3d_last_error(void);
 
 /* Scheme, physics and boundary selectors: the module state of src/global.f90
  * that the step routines read (src/input.f90 deck entries of the same names).
- * Supported: optx/opty any; xper/yper; wall in {0,1,2}; wallt in {0,1,2};
+ * Supported: optx/opty any; xper/yper; wall in {0,1,2,3} (3: the bump wall,
+ * lns3d_set_wall); wallt in {0,1,2};
  * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left in
  * {0,1,4,5,7} and right in {0,4,5,7,8} (8: the column held at its initial
  * values; 1: acoustic wave, 5: the undamped wave, right 5 with nx <= ny;
@@ -172,6 +173,11 @@ int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep);
  * of the inflow namelist.  lns3d_factor refuses these selectors without it. */
 int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const double *eig,
                      const double *lalpha);
+/* the bump wall, wall = 3 (src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105,
+ * src/lhsbc2f3d.f90:41-48): the wall values src/genbump.f90:74-131 forms
+ * from the bump shape, uw host (nx,5) complex: u1w, u2w, u3w, tw, twp.
+ * lns3d_factor refuses wall = 3 without them. */
+int lns3d_set_wall(lns3d_ctx *c, const double *uw);
 /* smoother, the deck's eps_e (src/smoother3d.f90 in the residual, called at
  * src/lrhs3d.f90:293; the implicit damping of src/lhs1f3d.f90:169-201,
  * lhs2f3d.f90:171-203 and the closure and symmetric rows): eps_e and the

@@ -38,6 +38,7 @@ struct lns3d_ctx {
   double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
   double *side_block = nullptr;     // fside (20 ny), x left column staging (ny)
   double *smooth_block = nullptr;   // eps_e buff (P)
+  double *wall_block = nullptr;     // fwall (10 nx)
   std::vector<double> eig;          // left = 4 profiles (10 ny), kept for ispg = 4
   double lalpha[2] = {0.0, 0.0};
   bool coeffs_ready = false, factored = false;
@@ -55,7 +56,7 @@ struct lns3d_ctx {
 
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
-  if (p.wall < 0 || p.wall > 2 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
+  if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
       (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8) || p.linear != 2 ||
       (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) || (p.loctime != 0 && p.loctime != 1))
@@ -136,6 +137,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->top_block) (void)hipFree(c->top_block);
   if (c->side_block) (void)hipFree(c->side_block);
   if (c->smooth_block) (void)hipFree(c->smooth_block);
+  if (c->wall_block) (void)hipFree(c->wall_block);
   if (c->vhist) (void)hipFree(c->vhist);
   if (c->vstart) (void)hipFree(c->vstart);
   delete c;
@@ -166,6 +168,7 @@ extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
   c->c.ftop = nullptr;
   c->c.fside = nullptr;
   c->c.sbuf = nullptr;
+  c->c.fwall = nullptr;
   c->eig.clear();
   return 0;
 }
@@ -284,6 +287,27 @@ extern "C" int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double
   int rc = launch_top_forcing(c->p, c->c, xd, wamp ? wd : nullptr, x0, c->stream);
   if (rc) return rc;
   LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// The bump wall (wall = 3): the wall values src/genbump.f90:74-131 forms from
+// the bump shape, uw host (nx,5) complex as (re, im) pairs: u1w, u2w, u3w, tw,
+// twp (src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105).  Before lns3d_factor.
+extern "C" int lns3d_set_wall(lns3d_ctx *c, const double *uw) {
+  if (!c || !uw) return fail(LNS3D_EINVAL, "lns3d_set_wall: null argument");
+  if (c->p.wall != 3 || c->p.yper) return fail(LNS3D_EINVAL, "lns3d_set_wall: the context has no wall = 3");
+  const int nx = c->p.nx;
+  if (!c->wall_block) LNS3D_CHECK(hipMalloc(&c->wall_block, (size_t)10 * nx * sizeof(double)));
+  std::vector<double> h((size_t)10 * nx);
+  for (int k = 0; k < 5; k++)
+    for (int i = 0; i < nx; i++) {              // (nx,5) complex -> [5][nx] (re, im)
+      h[2 * ((size_t)k * nx + i)] = uw[2 * ((size_t)k * nx + i)];
+      h[2 * ((size_t)k * nx + i) + 1] = uw[2 * ((size_t)k * nx + i) + 1];
+    }
+  LNS3D_CHECK(hipMemcpyAsync(c->wall_block, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  c->c.fwall = c->wall_block;
+  c->factored = false;
   return 0;
 }
 
@@ -444,6 +468,8 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
   if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.left == 5 || c->p.right == 4 || c->p.right == 5) &&
       !c->c.fside)
     return fail(LNS3D_EINVAL, "lns3d_factor: left = 1, 4, 5 and right = 4, 5 need their values (lns3d_set_inflow)");
+  if (!c->p.yper && c->p.wall == 3 && !c->c.fwall)
+    return fail(LNS3D_EINVAL, "lns3d_factor: wall = 3 needs its wall values (lns3d_set_wall)");
   int rc = launch_assemble(c->p, c->c, 1, c->px, c->stream);
   if (!rc) rc = penta_factor(&c->px, c->stream);
   if (!rc) rc = launch_assemble(c->p, c->c, 2, c->py, c->stream);
@@ -499,6 +525,8 @@ extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
   if (!c->p.xper && (c->p.left == 1 || c->p.left == 4 || c->p.left == 5 || c->p.right == 4 || c->p.right == 5) &&
       !c->c.fside)
     return fail(LNS3D_EINVAL, "lns3d_itrbc3d: left = 1, 4, 5 and right = 4, 5 need their values (lns3d_set_inflow)");
+  if (!c->p.yper && c->p.wall == 3 && !c->c.fwall)
+    return fail(LNS3D_EINVAL, "lns3d_itrbc3d: wall = 3 needs its wall values (lns3d_set_wall)");
   return launch_itrbc(c->p, c->c, v_dev, c->stream);
 }
 

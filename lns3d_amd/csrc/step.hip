@@ -670,9 +670,13 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
     if (j == 0) {
       if (r >= 1 && r <= 3) return one;
       if (r == 0) {
-        if (dir == 1 && (p.wall == 1 || p.wall == 2)) return one;
+        if (dir == 1 && (p.wall == 1 || p.wall == 2 || p.wall == 3)) return one;
         if (dir == 2 && p.wall == 1) {
           const double w[5] = {-gc4, -gc5, -gc1, -gc2, -gc3};   // slots 1..5
+          return z{cc == 0 ? w[k] : 0.0, 0.0};
+        }
+        if (dir == 2 && p.wall == 3) {   // third-order extrapolation, src/lhsbc2f3d.f90:41-48
+          const double w[5] = {4.0, -1.0, -1.0, 4.0, -6.0};
           return z{cc == 0 ? w[k] : 0.0, 0.0};
         }
       }
@@ -994,17 +998,35 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
         if (j == ny - 1) a = z{0, 0};
       } else {
         if (j == 0) {
-          if (m >= 1 && m <= 3) a = z{-vq(m).re, -vq(m).im};
+          // wall = 3: the bump's wall values u1w, u2w, u3w, tw, twp (src/rhsbc3d.f90:47-100)
+          auto W = [&](int k) { return z{c.fwall[2 * (k * nx + i)], c.fwall[2 * (k * nx + i) + 1]}; };
+          if (m >= 1 && m <= 3) {
+            const z wm = p.wall == 3 ? W(m - 1) : z{0, 0};
+            const z d = vq(m) - wm;
+            a = z{-d.re, -d.im};
+          }
+          if (m == 0 && p.wall == 3)
+            a = ld(v + 5 * q) - (sc(4.0, ld(v + 5 * (q + nx))) - sc(6.0, ld(v + 5 * (q + 2L * nx))) +
+                                 sc(4.0, ld(v + 5 * (q + 3L * nx))) - ld(v + 5 * (q + 4L * nx)));
           if (m == 0 && p.wall == 1)
             a = sc(gc1, ld(v + 5 * q)) + sc(gc2, ld(v + 5 * (q + nx))) + sc(gc3, ld(v + 5 * (q + 2L * nx))) +
                 sc(gc4, ld(v + 5 * (q + 3L * nx))) + sc(gc5, ld(v + 5 * (q + 4L * nx)));
           if (m == 0 && p.wall == 2)
             a = ld(v + 5 * q) - sc(2.0, ld(v + 5 * (q + nx))) + ld(v + 5 * (q + 2L * nx));
-          if (m == 4 && p.wallt == 0) a = z{0, 0};
-          if (m == 4 && p.wallt == 1)
+          if (m == 4 && p.wallt == 0) {
+            if (p.wall == 3) {
+              const z d = vq(4) - W(3);
+              a = z{-d.re, -d.im};
+            } else {
+              a = z{0, 0};
+            }
+          }
+          if (m == 4 && p.wallt == 1) {
             a = sc(gc1, ld(v + 5 * q + 4)) + sc(gc2, ld(v + 5 * (q + nx) + 4)) +
                 sc(gc3, ld(v + 5 * (q + 2L * nx) + 4)) + sc(gc4, ld(v + 5 * (q + 3L * nx) + 4)) +
                 sc(gc5, ld(v + 5 * (q + 4L * nx) + 4));
+            if (p.wall == 3) a = a - W(4);
+          }
         }
         if (j == ny - 1) {
           if (p.top == 1) {        // src/rhsbc3d.f90:152-157: -(v - wave)
@@ -1063,7 +1085,7 @@ __global__ void k_axpy(double2 *x, const double2 *d, long n) {
   x[t] = make_double2(a.x + b.x, a.y + b.y);
 }
 
-__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, double2 *v) {
+__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const double *fwall, double2 *v) {
   using namespace st;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = p.nx, ny = p.ny;
@@ -1073,7 +1095,12 @@ __global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, double2 *v)
     for (int d = 0; d < 5; d++) V(d, ny - 1) = V(d, 0);
     return;
   }
-  V(1, 0) = make_double2(0, 0); V(2, 0) = make_double2(0, 0); V(3, 0) = make_double2(0, 0);
+  // the wall value of component d (wall = 3: the bump's, src/itrbc3d.f90:48-105)
+  auto W = [&](int d) { return z{fwall[2 * (d * nx + i)], fwall[2 * (d * nx + i) + 1]}; };
+  for (int d = 1; d <= 3; d++) {
+    const z w = p.wall == 3 ? W(d - 1) : z{0, 0};
+    V(d, 0) = make_double2(w.re, w.im);
+  }
   auto lin = [&](int d, double a, double b, double c2, double e) {
     z s = sc(a, ld(&V(d, 1))) + sc(b, ld(&V(d, 2))) + sc(c2, ld(&V(d, 3))) + sc(e, ld(&V(d, 4)));
     return s;
@@ -1086,10 +1113,22 @@ __global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, double2 *v)
     z a = ld(&V(0, 1)), b = ld(&V(0, 2));
     V(0, 0) = make_double2(2.0 * a.re - b.re, 2.0 * a.im - b.im);
   }
-  if (p.wallt == 0) V(4, 0) = make_double2(0, 0);
+  if (p.wall == 3) {      // third-order extrapolation of the density
+    z s = sc(4.0, ld(&V(0, 1))) - sc(6.0, ld(&V(0, 2))) + sc(4.0, ld(&V(0, 3))) - ld(&V(0, 4));
+    V(0, 0) = make_double2(s.re, s.im);
+  }
+  if (p.wallt == 0) {
+    const z w = p.wall == 3 ? W(3) : z{0, 0};
+    V(4, 0) = make_double2(w.re, w.im);
+  }
   if (p.wallt == 1) {
     z s = lin(4, gc2, gc3, gc4, gc5);
-    V(4, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+    if (p.wall == 3) {
+      const z t = W(4) - s;
+      V(4, 0) = make_double2(t.re / gc1, t.im / gc1);
+    } else {
+      V(4, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+    }
   }
   if (p.top == 1)        // src/itrbc3d.f90:187-191: the incoming wave
     for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(ftop[2 * (d * nx + i)], ftop[2 * (d * nx + i) + 1]);
@@ -1121,7 +1160,7 @@ __global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
 
 int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s) {
   hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (const double *)c.ftop,
-                     (double2 *)v);
+                     (const double *)c.fwall, (double2 *)v);
   hipLaunchKernelGGL(k_itrbc_lr, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (const double *)c.fside,
                      (double2 *)v);
   LNS3D_CHECK(hipGetLastError());
@@ -1327,8 +1366,8 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
     if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;
     if (jr == 0) {
       ident[1] = ident[2] = ident[3] = true;
-      if (dir == 1 && (p.wall == 1 || p.wall == 2)) ident[0] = true;
-      if (dir == 2 && p.wall == 1) wallrow0 = true;
+      if (dir == 1 && (p.wall == 1 || p.wall == 2 || p.wall == 3)) ident[0] = true;
+      if (dir == 2 && (p.wall == 1 || p.wall == 3)) wallrow0 = true;
       if (p.wallt == 0 || (dir == 1 && p.wallt == 1)) ident[4] = true;
       if (dir == 2 && p.wallt == 1) wallrow4 = true;
     }
@@ -1337,11 +1376,13 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   if (!p.yper && jr == ny - 1) wallrow0 = wallrow4 = false;
   if (wallrow0 || wallrow4) {
     const double w[5] = {-gc4, -gc5, -gc1, -gc2, -gc3};
+    const double w3[5] = {4.0, -1.0, -1.0, 4.0, -6.0};     // wall = 3 density row
+    const double *w0 = p.wall == 3 ? w3 : w;
     z s0{0, 0}, s4{0, 0};
     for (int k = 0; k < 5; k++) {
       const int col = slot_col(N, per, posc, k);
       const long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
-      s0 = s0 + sc(w[k], ld(x + 5 * qc + 0));
+      s0 = s0 + sc(w0[k], ld(x + 5 * qc + 0));
       s4 = s4 + sc(w[k], ld(x + 5 * qc + 4));
     }
     if (wallrow0 && !ident[0]) out[0] = s0;

@@ -40,6 +40,9 @@ struct Coeffs {
   // smoother (src/smoother3d.f90 and the implicit damping of the factors):
   // eps_e buff per point, [P]; null: eps_e = 0
   double *sbuf;
+  // the bump wall (wall = 3, src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105):
+  // u1w, u2w, u3w, tw, twp, [5][nx] complex as (re, im) pairs
+  double *fwall;
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);

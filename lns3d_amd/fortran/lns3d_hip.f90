@@ -134,6 +134,10 @@ module lns3d_hip
       type(c_ptr), value :: c, x, eig, lalpha
       real(c_double), value :: x0
     end function
+    integer(c_int) function c_set_wall(c, uw) bind(C, name="lns3d_set_wall")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, uw
+    end function
     integer(c_int) function c_set_smoother(c, eps_e, buff) bind(C, name="lns3d_set_smoother")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: c, buff
@@ -186,7 +190,8 @@ contains
     logical :: needs_values
     if (c_associated(ctx)) call lns3d_hip_free()
     needs_values = prm%top == 1 .or. ((prm%left == 1 .or. prm%left == 4 .or. prm%left == 5 .or. &
-                                       prm%right == 4 .or. prm%right == 5) .and. prm%xper == 0)
+                                       prm%right == 4 .or. prm%right == 5) .and. prm%xper == 0) .or. &
+                   (prm%wall == 3 .and. prm%yper == 0)
     rc = c_ctx_create(prm, ctx)
     if (rc == 0) rc = c_setup(ctx, c_loc(metric), c_loc(vm))
     if (rc == 0 .and. .not. needs_values) rc = c_factor(ctx)
@@ -226,6 +231,12 @@ contains
     la(1) = real(lalpha, c_double)
     la(2) = aimag(lalpha)
     call lns3d_hip_record(c_set_inflow(ctx, c_loc(x), x0, c_loc(eig), c_loc(la)))
+  end subroutine
+
+  ! wall = 3: the bump's wall values uw(nx,5): u1w, u2w, u3w, tw, twp
+  subroutine lns3d_hip_wall(uw)
+    complex(c_double_complex), intent(in), target :: uw(*)
+    call lns3d_hip_record(c_set_wall(ctx, c_loc(uw)))
   end subroutine
 
   ! smoother: eps_e and the damping profile buff (nx,ny) of init_buffer

@@ -66,6 +66,7 @@ SIGNATURES = {
     "lns3d_set_corrector_iterations": ([VP, CI], CI),
     "lns3d_set_inflow": ([VP, VP, ctypes.c_double, VP, VP], CI),
     "lns3d_set_smoother": ([VP, ctypes.c_double, VP], CI),
+    "lns3d_set_wall": ([VP, VP], CI),
     "lns3d_step_forced": ([VP, VP, VP, VP, VP], CI),
     "lns3d_forced_residual": ([VP, VP, VP, VP, VP, ctypes.POINTER(ctypes.c_double)], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
@@ -211,6 +212,12 @@ class Step:
         check(lib().lns3d_forced_residual(self.h, ptr(v_dev), ptr(vold_dev), ptr(f_dev), ptr(r_dev),
                                           ctypes.byref(out)), "lns3d_forced_residual")
         return out.value
+
+    def set_wall(self, uw):
+        """the bump wall values (wall = 3): uw (nx, 5) complex, u1w, u2w, u3w, tw, twp"""
+        import numpy as np
+        a = np.asfortranarray(uw, dtype=np.complex128)
+        check(lib().lns3d_set_wall(self.h, a.ctypes.data), "lns3d_set_wall")
 
     def set_smoother(self, eps_e, buff=None):
         """smoother eps_e with the damping profile buff (nx, ny); 0: off"""

@@ -535,3 +535,68 @@ def test_forced_steps_against_reference():
         own = np.sqrt((np.abs(host(rd)) ** 2 / dtl[None] ** 2).sum())
         assert abs(nrm - own) < 1e-13 * own, k
     st.close()
+
+
+@pytest.mark.parametrize("wallt", [0, 1])
+def test_bump_wall_against_reference(wallt):
+    """lns3d_set_wall: the bump wall (wall = 3) with given wall velocities and
+    temperature (wallt 0) or temperature gradient (wallt 1), the density by a
+    third-order extrapolation, against the reference's itrBC3D, lrhs3D, both
+    assembled factors and two steps (tests/golden/ref_bump_t<wallt>.npz), 1e-12;
+    the refined step reaches the same increment; refused before the values
+    are given"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, f"ref_bump_t{wallt}.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"bump wall wallt={wallt} {p.nx}x{p.ny}", flush=True)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t, shape):
+        return t.cpu().numpy().reshape(shape, order="F")
+
+    st = Step(p)
+    st.setup(met, vm)
+    v0 = z["v0"]
+    vd = dev(v0)
+    with pytest.raises(LNS3DError):
+        st.itrbc3d(vd)
+    with pytest.raises(LNS3DError):
+        st.factor()
+    st.set_wall(z["uw"])
+    st.itrbc3d(vd)
+    assert rel(host(vd, v0.shape), z["itrbc"]) < TOL
+    vd = dev(v0)
+    rd = torch.zeros_like(vd)
+    st.lrhs3d(vd, rd)
+    assert rel(host(rd, v0.shape), z["lrhs"]) < TOL
+    for d in (1, 2):
+        m_dev = torch.zeros(125 * p.nx * p.ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        assert rel(host(m_dev, z[f"lhs{d}"].shape), z[f"lhs{d}"]) < TOL, d
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    for k in (1, 2):
+        st.step(vd, dvd)
+        assert rel(host(dvd, v0.shape), z[f"dv{k}"]) < TOL, k
+    assert rel(host(vd, v0.shape), z["v2"]) < TOL
+    st.close()
+    st32 = Step(p, precision=1)
+    st32.setup(met, vm)
+    st32.set_wall(z["uw"])
+    st32.factor()
+    vb = dev(z["itrbc"])
+    dvb = torch.zeros_like(vb)
+    its, res = st32.step_refined(vb, dvb, tol=1e-12, max_iters=10)
+    e = rel(host(dvb, v0.shape), z["dv1"])
+    print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
+    assert res <= 1e-12 and e < 1e-10
+    st32.close()
