@@ -98,6 +98,8 @@ int orc_forced_residual(const orc_params *p, const orc_coeffs *c, const double *
                         zdouble *work /* 5 arrays of (5,nx,ny) */, double *norm);
 /* bump wall values for wall = 3: uw (nx,5) complex u1w, u2w, u3w, tw, twp */
 int orc_set_bump(const zdouble *uw);
+/* Sutherland's constant S0 = 110.4 K / Tref for mattyp = 1 */
+int orc_set_sutherland(double S0);
 /* smoother eps_e with the damping profile buff (nx,ny) (eps_e 0: off) */
 int orc_set_smoother(double eps_e, const double *buff);
 int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,

@@ -79,6 +79,11 @@ class Oracle:
         self.lib.orc_set_top(*(_ptr(a) for a in arrs), x0)
         self._top_keep = arrs
 
+    def set_sutherland(self, S0=-1.0):
+        """Sutherland's constant S0 for mattyp = 1 (negative: cleared)"""
+        self.lib.orc_set_sutherland.argtypes = [ctypes.c_double]
+        self.lib.orc_set_sutherland(float(S0))
+
     def set_bump(self, uw=None):
         """bump wall values uw (nx, 5) complex (wall = 3); None: cleared"""
         self._bump_keep = None if uw is None else np.asfortranarray(uw, dtype=np.complex128)
@@ -314,6 +319,10 @@ class Reference:
         self.lib.ref_forced_residual(_ptr(np.asfortranarray(v)), _ptr(np.asfortranarray(vold)), _ptr(r),
                                      ctypes.byref(nrm))
         return r, nrm.value
+
+    def set_sutherland(self, S0):
+        """Sutherland's constant S0 (110.4 K / Tref) for mattyp = 1"""
+        self.lib.ref_set_sutherland(ctypes.c_double(S0))
 
     def set_bump(self, uw):
         """the bump wall values uw (nx, 5) complex: u1w, u2w, u3w, tw, twp"""

@@ -135,6 +135,15 @@ subroutine ref_set_sponge(spgl, spg2l, wampl) bind(C, name="ref_set_sponge")
   spg = spgl; spg2 = spg2l; wamp = wampl
 end subroutine ref_set_sponge
 
+! Sutherland's constant (mattyp = 1): S0 nondimensional, src/input.f90:41-47
+subroutine ref_set_sutherland(s) bind(C, name="ref_set_sutherland")
+  use iso_c_binding
+  use global
+  implicit none
+  real(c_double), value :: s
+  S0 = s
+end subroutine ref_set_sutherland
+
 ! the bump wall (wall = 3): the wall values src/genbump.f90:74-131 forms,
 ! uw(nx,5) complex: u1w, u2w, u3w, tw, twp
 subroutine ref_set_bump(uw) bind(C, name="ref_set_bump")

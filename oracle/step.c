@@ -951,11 +951,25 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
 }
 
 /* ---- src/getmat.f90 (scalar form) --------------------------------------- */
+/* Sutherland's constant S0 (the deck's 110.4 K over Tref, src/input.f90:41-47)
+ * for mattyp = 1; negative: not given */
+static double g_S0 = -1.0;
+
+int orc_set_sutherland(double S0) {
+  g_S0 = S0;
+  return 0;
+}
+
 static void getmat(int mattyp, double t, double *mu, double *lm, double *con, double *dmu,
                    double *d2mu, double *dlm, double *d2lm, double *dcon, double *d2con) {
   if (mattyp == 0) {
     *mu = 1.0; *dmu = 0.0; *d2mu = 0.0;
-  } else {  /* 2, 3: linear law (1, Sutherland, is refused by supported()) */
+  } else if (mattyp == 1) {                      /* Sutherland's law, src/getmat.f90:28-40 */
+    const double S0 = g_S0, ts = S0 + t;
+    *mu = pow(t, 1.5) * (1.0 + S0) / ts;
+    *dmu = sqrt(t) * (1.0 + S0) * (3.0 * S0 + t) * 0.5 / (ts * ts);
+    *d2mu = (1.0 + S0) * (3.0 * (S0 * S0) - 6.0 * S0 * t - t * t) * 0.25 / sqrt(t) / (ts * ts * ts);
+  } else {  /* 2, 3: linear law */
     *mu = t; *dmu = 1.0; *d2mu = 0.0;
   }
   *con = *mu; *dcon = *dmu; *d2con = *d2mu;
@@ -965,7 +979,7 @@ static void getmat(int mattyp, double t, double *mu, double *lm, double *con, do
 
 /* ---- src/genmtrx.f90:72-1000 (general metrics, linear = 2, idamp = 0) ----- */
 int orc_genmtrx(const orc_params *p, const double *vm, const double *metric, orc_coeffs *c) {
-  if (!supported(p) || p->mattyp == 1) return -1;
+  if (!supported(p) || (p->mattyp == 1 && g_S0 < 0.0)) return -1;
   int nx = p->nx, ny = p->ny;
   long P = (long)nx * ny;
   double *g = calloc(25 * P, sizeof(double));
@@ -1198,7 +1212,7 @@ int orc_genmtrx(const orc_params *p, const double *vm, const double *metric, orc
 /* src/temporal.f90:40-160 (acoustic CFL, global or local time step) */
 int orc_set_time_step(const orc_params *p, const double *vm, const double *metric,
                       double *dtl) {
-  if (p->mattyp == 1) return -1;
+  if (p->mattyp == 1 && g_S0 < 0.0) return -1;
   int nx = p->nx, ny = p->ny;
   long P = (long)nx * ny;
   double dxi = 1.0 / (double)(nx - 1), deta = 1.0 / (double)(ny - 1);

@@ -47,6 +47,8 @@ reference's outputs:
                         driver's residual and norm after each
   ref_bump_t<wallt>.npz the bump wall (wall = 3) with seeded wall values, isothermal
                         and adiabatic: itrBC3D, lrhs3D, rhsBC3D, both factors, two steps
+  ref_sutherland.npz    mattyp = 1 (Sutherland's law, S0 = 110.4/300):
+                        coefficients, dtl, two steps
   ref_sym_<case>.npz    symmetry boundaries (left/right = 7): left7, both7
                         (with wall = 1, wallt = 1) and mse (the boundary set
                         of test/mse-recep/complex.inp: top 1, left 7); the
@@ -59,6 +61,7 @@ reference's outputs:
     python tests/golden/gen_golden.py smooth     (the smoother cases only)
     python tests/golden/gen_golden.py forced     (the shift-invert steps only)
     python tests/golden/gen_golden.py bump       (the bump wall cases only)
+    python tests/golden/gen_golden.py sutherland (the Sutherland case only)
 """
 from __future__ import annotations
 
@@ -493,6 +496,31 @@ def bump_case(R, wallt):
     np.savez_compressed(os.path.join(HERE, f"ref_bump_t{wallt}.npz"), **out)
 
 
+def sutherland_case(R):
+    """mattyp = 1, Sutherland's law (src/getmat.f90:28-40) with S0 = 110.4 K /
+    300 K: coefficients, dtl and two steps"""
+    p = Params(nx=13, ny=9, kz=0.4, omega=0.3, mattyp=1, wall=1, wallt=1)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    S0 = 110.4 / 300.0
+    rng = np.random.default_rng(59)
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    R.setup(p, x, y, met, vm)
+    R.set_sutherland(S0)
+    c = R.genmtrx()
+    c["ABhi"][4:] = 0.0                   # never set by genmtrx (unused, not initialised)
+    dtl = R.set_time_step()
+    out = dict(params=_params_json(p), S0=np.array(S0), met=met, vm=vm, v0=v0, dtl=dtl,
+               **{"c_" + k: v for k, v in c.items()})
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    out["dv1"] = R.step(v, True)
+    out["dv2"] = R.step(v, False)
+    out["v2"] = v.copy()
+    np.savez_compressed(os.path.join(HERE, "ref_sutherland.npz"), **out)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libref.so is not built (oracle/build_ref.sh)")
@@ -510,6 +538,10 @@ def main():
     if sys.argv[1:] == ["sym"]:
         for name, kw in SYM_CASES.items():
             sym_case(R, name, kw)
+        R.lib.ref_flush()
+        return
+    if sys.argv[1:] == ["sutherland"]:
+        sutherland_case(R)
         R.lib.ref_flush()
         return
     if sys.argv[1:] == ["bump"]:
@@ -553,6 +585,7 @@ def main():
     forced_case(R)
     for wallt in (0, 1):
         bump_case(R, wallt)
+    sutherland_case(R)
     R.lib.ref_flush()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

@@ -492,3 +492,32 @@ def test_bump_wall(orc, wallt):
         assert rel(v, z["v2"]) < 1e-12
     finally:
         orc.set_bump(None)
+
+
+def test_sutherland(orc):
+    """mattyp = 1, Sutherland's law (src/getmat.f90:28-40, S0 = 110.4 K /
+    Tref as src/input.f90:41-47 forms it), against the reference's genmtrx,
+    set_time_step and two steps (ref_sutherland.npz); refused without S0; the
+    linear law misses"""
+    import dataclasses
+    z = load("ref_sutherland.npz")
+    p = params_of(z)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    with pytest.raises(AssertionError):
+        orc.genmtrx(p, vm, met)
+    try:
+        orc.set_sutherland(float(z["S0"]))
+        c = orc.genmtrx(p, vm, met)
+        for k in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22"):
+            assert rel(c[k], z["c_" + k]) < 1e-14, k
+        assert rel(c["ABhi"][:4], z["c_ABhi"][:4]) < 1e-14
+        assert rel(orc.genmtrx(dataclasses.replace(p, mattyp=2), vm, met)["Dh"], z["c_Dh"]) > 1e-3
+        dtl = orc.set_time_step(p, vm, met)
+        assert rel(dtl, z["dtl"]) < 1e-14
+        v = np.asfortranarray(z["itrbc"].copy())
+        f = orc.factor(p, c, dtl)
+        assert rel(orc.step(p, c, dtl, f, v), z["dv1"]) < 1e-12
+        assert rel(orc.step(p, c, dtl, f, v), z["dv2"]) < 1e-12
+        assert rel(v, z["v2"]) < 1e-12
+    finally:
+        orc.set_sutherland(-1.0)
