@@ -38,7 +38,7 @@ const char *lns3d_last_error(void);
  * src/lhsbs1f3d.f90 and the symmetric closures of src/cgrad.f90,
  * cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
  * the reference's symmetric rows read point i + ny j); linear = 2; mattyp in
- * {0,2,3}; loctime in {0,1};
+ * {0,1,2,3} (1: Sutherland's law, lns3d_set_sutherland); loctime in {0,1};
  * Navier-Stokes (viscous) boundary rows; the smoother eps_e through
  * lns3d_set_smoother; sponges through
  * lns3d_set_sponge (ispg 1 to 5).  Other values are rejected with
@@ -173,6 +173,10 @@ int lns3d_set_time_history(lns3d_ctx *c, const void *vold_dev, long lstep);
  * of the inflow namelist.  lns3d_factor refuses these selectors without it. */
 int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const double *eig,
                      const double *lalpha);
+/* Sutherland's law, mattyp = 1 (src/getmat.f90:28-40): the constant S0 =
+ * 110.4 K / Tref (src/input.f90:41-47).  Before lns3d_setup, which refuses
+ * mattyp = 1 without it. */
+int lns3d_set_sutherland(lns3d_ctx *c, double S0);
 /* the bump wall, wall = 3 (src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105,
  * src/lhsbc2f3d.f90:41-48): the wall values src/genbump.f90:74-131 forms
  * from the bump shape, uw host (nx,5) complex: u1w, u2w, u3w, tw, twp.

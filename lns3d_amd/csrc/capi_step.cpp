@@ -43,6 +43,7 @@ struct lns3d_ctx {
   double lalpha[2] = {0.0, 0.0};
   bool coeffs_ready = false, factored = false;
   bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
+  bool s0_set = false;              // Sutherland's constant given (mattyp = 1)
   int impl = 1;                     // time scheme (src/impdrv3d.f90:179-199)
   long lstep = 0;                   // steps taken with this scheme
   void *vhist = nullptr;            // impl = 2: 5P complex, v at the start of the last step
@@ -59,7 +60,7 @@ static int check_params(const lns3d_params &p) {
   if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
       (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8) || p.linear != 2 ||
-      (p.mattyp != 0 && p.mattyp != 2 && p.mattyp != 3) || (p.loctime != 0 && p.loctime != 1))
+      (p.mattyp < 0 || p.mattyp > 3) || (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");
   // right = 4 reads the eigenfunction profiles, which the reference reads
   // only for left = 4 (src/gendist.f90:17)
@@ -183,6 +184,8 @@ extern "C" int lns3d_ctx_set_stream(lns3d_ctx *c, void *stream) {
 // step are then generated on the device (src/genmtrx.f90, src/temporal.f90)
 extern "C" int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm) {
   if (!c || !metric || !vm) return fail(LNS3D_EINVAL, "lns3d_setup: null argument");
+  if (c->p.mattyp == 1 && !c->s0_set)
+    return fail(LNS3D_EINVAL, "lns3d_setup: mattyp = 1 needs Sutherland's constant (lns3d_set_sutherland)");
   const long P = (long)c->p.nx * c->p.ny;
   LNS3D_CHECK(hipMemcpyAsync(c->c.metric, metric, (size_t)P * 10 * 8, hipMemcpyHostToDevice, c->stream));
   LNS3D_CHECK(hipMemcpyAsync(c->c.vm, vm, (size_t)P * 5 * 8, hipMemcpyHostToDevice, c->stream));
@@ -287,6 +290,18 @@ extern "C" int lns3d_set_top_forcing(lns3d_ctx *c, const double *x, const double
   int rc = launch_top_forcing(c->p, c->c, xd, wamp ? wd : nullptr, x0, c->stream);
   if (rc) return rc;
   LNS3D_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Sutherland's law (mattyp = 1, src/getmat.f90:28-40): the nondimensional
+// constant S0 = 110.4 K / Tref as src/input.f90:41-47 forms it from the deck.
+// Before lns3d_setup (the coefficients use it).
+extern "C" int lns3d_set_sutherland(lns3d_ctx *c, double S0) {
+  if (!c || !(S0 > 0.0)) return fail(LNS3D_EINVAL, "lns3d_set_sutherland: needs a context and S0 > 0");
+  c->c.S0 = S0;
+  c->s0_set = true;
+  c->coeffs_ready = false;
+  c->factored = false;
   return 0;
 }
 

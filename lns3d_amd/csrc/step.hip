@@ -194,10 +194,18 @@ __global__ void k_mean_d1x(int nx, int ny, int xper, Stencil st, const double *v
     g1[5 * q + e] = d1<double, double>(nx, xper, i, st.gx, st.dxinv, base + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
 }
 
-__device__ __forceinline__ void getmat(int mattyp, double t, double &mu, double &lm, double &con,
+// material properties (src/getmat.f90:14-70): constant (0), Sutherland's law
+// with the nondimensional constant S0 (1), linear law (2, 3)
+__device__ __forceinline__ void getmat(int mattyp, double S0, double t, double &mu, double &lm, double &con,
                                        double &dmu, double &d2mu, double &dlm, double &d2lm,
                                        double &dcon, double &d2con) {
   if (mattyp == 0) { mu = 1.0; dmu = 0.0; d2mu = 0.0; }
+  else if (mattyp == 1) {
+    const double ts = S0 + t;
+    mu = pow(t, 1.5) * (1.0 + S0) / ts;
+    dmu = sqrt(t) * (1.0 + S0) * (3.0 * S0 + t) * 0.5 / (ts * ts);
+    d2mu = (1.0 + S0) * (3.0 * (S0 * S0) - 6.0 * S0 * t - t * t) * 0.25 / sqrt(t) / (ts * ts * ts);
+  }
   else { mu = t; dmu = 1.0; d2mu = 0.0; }
   con = mu; dcon = dmu; d2con = d2mu;
   if (mattyp == 3) { lm = 0.0; dlm = 0.0; d2lm = 0.0; }
@@ -264,7 +272,7 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
   S[0][2] = 0.5 * gu[2][0];
   S[1][2] = 0.5 * gu[2][1];
   double mu, lm, con, dmu, d2mu, dlm, d2lm, dcon, d2con;
-  getmat(p.mattyp, t, mu, lm, con, dmu, d2mu, dlm, d2lm, dcon, d2con);
+  getmat(p.mattyp, c.S0, t, mu, lm, con, dmu, d2mu, dlm, d2lm, dcon, d2con);
   double gmu[3], gdmu[3], gcon[3], gdcon[3], glm[3], gdlm[3];
   for (int k = 0; k < 3; k++) {
     gmu[k] = dmu * gt[k]; gdmu[k] = d2mu * gt[k];

@@ -43,6 +43,7 @@ struct Coeffs {
   // the bump wall (wall = 3, src/rhsbc3d.f90:47-100, src/itrbc3d.f90:48-105):
   // u1w, u2w, u3w, tw, twp, [5][nx] complex as (re, im) pairs
   double *fwall;
+  double S0;                    // Sutherland's constant (mattyp = 1), 110.4 K / Tref
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);

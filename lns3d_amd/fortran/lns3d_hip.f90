@@ -41,9 +41,12 @@ module lns3d_hip
 
   integer(c_int), public :: lns3d_hip_last = 0
   type(c_ptr), save :: ctx = c_null_ptr
+  real(c_double), save :: s0_given = -1.0_c_double   ! Sutherland's constant for mattyp = 1
 
   public :: lns3d_hip_status, lns3d_hip_message, lns3d_hip_record
   public :: lns3d_hip_init, lns3d_hip_put, lns3d_hip_get, lns3d_hip_advance, lns3d_hip_free
+  public :: lns3d_hip_factor, lns3d_hip_scheme, lns3d_hip_sponge, lns3d_hip_top, lns3d_hip_inflow
+  public :: lns3d_hip_smoother, lns3d_hip_wall, lns3d_hip_sutherland
   public :: c_cpenta1bc, c_cpenta2bc, c_cpenta1p, c_cpenta2p, c_set_periodic_exact
 
   interface
@@ -134,6 +137,11 @@ module lns3d_hip
       type(c_ptr), value :: c, x, eig, lalpha
       real(c_double), value :: x0
     end function
+    integer(c_int) function c_set_sutherland(c, s0) bind(C, name="lns3d_set_sutherland")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), value :: s0
+    end function
     integer(c_int) function c_set_wall(c, uw) bind(C, name="lns3d_set_wall")
       import :: c_int, c_ptr
       type(c_ptr), value :: c, uw
@@ -193,9 +201,17 @@ contains
                                        prm%right == 4 .or. prm%right == 5) .and. prm%xper == 0) .or. &
                    (prm%wall == 3 .and. prm%yper == 0)
     rc = c_ctx_create(prm, ctx)
+    if (rc == 0 .and. prm%mattyp == 1) rc = c_set_sutherland(ctx, s0_given)
     if (rc == 0) rc = c_setup(ctx, c_loc(metric), c_loc(vm))
     if (rc == 0 .and. .not. needs_values) rc = c_factor(ctx)
     call lns3d_hip_record(rc)
+  end subroutine
+
+  ! Sutherland's constant S0 = 110.4 K / Tref (src/input.f90:41-47) for a
+  ! mattyp = 1 deck; call before lns3d_hip_init
+  subroutine lns3d_hip_sutherland(s0)
+    real(c_double), intent(in) :: s0
+    s0_given = s0
   end subroutine
 
   ! forms the factors again (after any of the setters below)

@@ -67,6 +67,7 @@ SIGNATURES = {
     "lns3d_set_inflow": ([VP, VP, ctypes.c_double, VP, VP], CI),
     "lns3d_set_smoother": ([VP, ctypes.c_double, VP], CI),
     "lns3d_set_wall": ([VP, VP], CI),
+    "lns3d_set_sutherland": ([VP, ctypes.c_double], CI),
     "lns3d_step_forced": ([VP, VP, VP, VP, VP], CI),
     "lns3d_forced_residual": ([VP, VP, VP, VP, VP, ctypes.POINTER(ctypes.c_double)], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
@@ -212,6 +213,10 @@ class Step:
         check(lib().lns3d_forced_residual(self.h, ptr(v_dev), ptr(vold_dev), ptr(f_dev), ptr(r_dev),
                                           ctypes.byref(out)), "lns3d_forced_residual")
         return out.value
+
+    def set_sutherland(self, S0):
+        """Sutherland's constant S0 = 110.4 K / Tref for mattyp = 1; before setup"""
+        check(lib().lns3d_set_sutherland(self.h, float(S0)), "lns3d_set_sutherland")
 
     def set_wall(self, uw):
         """the bump wall values (wall = 3): uw (nx, 5) complex, u1w, u2w, u3w, tw, twp"""

@@ -600,3 +600,45 @@ def test_bump_wall_against_reference(wallt):
     print(f"  refined: iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
     assert res <= 1e-12 and e < 1e-10
     st32.close()
+
+
+def test_sutherland_against_reference():
+    """lns3d_set_sutherland: mattyp = 1 (Sutherland's law, src/getmat.f90:28-40)
+    against the reference's coefficients, dtl and two steps
+    (tests/golden/ref_sutherland.npz), 1e-12; lns3d_setup refuses mattyp = 1
+    without S0"""
+    import json
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, LNS3DError
+    z = np.load(os.path.join(GOLD, "ref_sutherland.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    print(f"sutherland {p.nx}x{p.ny}", flush=True)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    def host(t):
+        return t.cpu().numpy().reshape(z["v0"].shape, order="F")
+
+    st = Step(p)
+    with pytest.raises(LNS3DError):
+        st.setup(met, vm)
+    st.set_sutherland(float(z["S0"]))
+    st.setup(met, vm)
+    cg, dtg = st.get_coeffs()
+    for k in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22"):
+        assert rel(cg[k], z["c_" + k]) < TOL, k
+    assert rel(cg["ABhi"][:4], z["c_ABhi"][:4]) < TOL
+    assert rel(dtg, z["dtl"]) < TOL
+    st.factor()
+    vd = dev(z["itrbc"])
+    dvd = torch.zeros_like(vd)
+    for k in (1, 2):
+        st.step(vd, dvd)
+        assert rel(host(dvd), z[f"dv{k}"]) < TOL, k
+    assert rel(host(vd), z["v2"]) < TOL
+    st.close()

@@ -183,7 +183,8 @@ def test_module_driver_scheme(flib, orc):
     assert np.abs(dv - dvo).max() < TOL * np.abs(vo).max()
 
 
-@pytest.mark.parametrize("case", ["sponge_3", "top1", "smooth_base", "inflow_left4"])
+@pytest.mark.parametrize("case", ["sponge_3", "top1", "smooth_base", "inflow_left4", "bump_t1",
+                                  "sutherland"])
 def test_module_driver_deck_extras(flib, case):
     """the Fortran module driver with a deck's extras (lns3d_hip_sponge,
     lns3d_hip_top, lns3d_hip_smoother, lns3d_hip_inflow, then
@@ -203,9 +204,15 @@ def test_module_driver_deck_extras(flib, case):
     fn = lambda s: getattr(flib, "_QMlns3d_hipPlns3d_hip_" + s)
     D = lambda v: ctypes.byref(ctypes.c_double(v))
     met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    if case == "sutherland":
+        fn("sutherland")(D(float(z["S0"])))
     fn("init")(ctypes.byref(fp), P(met), P(vm))
     assert status(flib) == 0
-    if case.startswith("sponge"):
+    if case == "sutherland":
+        pass
+    elif case.startswith("bump"):
+        fn("wall")(P(np.asfortranarray(z["uw"])))
+    elif case.startswith("sponge"):
         x = np.asfortranarray(z["x"])
         fn("sponge")(I(int(z["ispg"])), P(np.asfortranarray(z["spg"])), P(np.asfortranarray(z["spg2"])),
                      P(x), P(np.ascontiguousarray(z["wamp"])), D(float(z["x0"])))
