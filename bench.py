@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep leg")
+    ap.add_argument("--requests", type=int, default=16, help="requests in the serving sweep")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch 5 xi solves, 5 eta solves, 5 steps and a calibration copy of "
                          "one solve's algorithmic bytes, then exit (for rocprofv3 --pmc runs)")
@@ -230,29 +231,28 @@ def main():
         b = run_batch(count=64, nsteps=20, seed=0)
         out["batch"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()}
 
-    # ---- serving sweep: closed-loop peak, then Poisson arrivals at 0.8 x peak ----
-    # complex-float factors with one complex-double correction per step (the
-    # refined step), and with the outer pseudo-time iteration as the only
-    # refinement (the complex-float step), beside serial complex-double solves
+    # ---- serving sweep: receptivity requests marched to 1e-9 of their first
+    # residual; closed-loop peak, then Poisson arrivals at 0.8 x peak, with
+    # complex-float factors and the complex-double residual each step (the
+    # pseudo-time iteration as the refinement), beside the same requests
+    # solved one at a time with complex-double factors and with complex-float
+    # factors (time to solution of each path) ----
     if not args.no_sweep:
-        from lns3d_amd.native import MODE_FP32, MODE_REFINED
-        from lns3d_amd.serve import sweep, serial_fp64, draw_requests
-        ser = serial_fp64(draw_requests(32, None, 0))
-        sweeps = {}
-        for name, mode in (("refined", MODE_REFINED), ("fp32_outer", MODE_FP32)):
-            sw = sweep(count=32, seed=0, mode=mode)
-            sweeps[name] = {"peak_solves_s": round(sw["peak"]["solves_per_s"], 3),
-                            "arrival_rate_s": round(sw["rate"], 3),
-                            "solves_s": round(sw["open_loop"]["solves_per_s"], 3),
-                            "p50_ms": round(sw["open_loop"]["p50_ms"], 2),
-                            "p99_ms": round(sw["open_loop"]["p99_ms"], 2)}
-        out["sweep"] = {"requests": 32,
-                        "modes": {"refined": "FP32 factors + one FP64 correction per step",
-                                  "fp32_outer": "FP32 factors, FP64 residual each step (outer refinement)"},
-                        **sweeps,
-                        "serial_fp64": {"solves_s": round(ser["solves_per_s"], 3),
-                                        "p50_ms": round(ser["p50_ms"], 2),
-                                        "p99_ms": round(ser["p99_ms"], 2)}}
+        from lns3d_amd.serve import sweep, serial, draw_requests
+        nreq = args.requests
+        ser64, _ = serial(draw_requests(nreq, None, 0), mode="fp64")
+        ser32, _ = serial(draw_requests(nreq, None, 0), mode="fp32_outer")
+        sw = sweep(count=nreq, seed=0, mode="fp32_outer")
+        rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+        out["sweep"] = {"requests": nreq,
+                        "request": "receptivity (ispg 2 sponge forcing, loctime 1, CFL 80) to 1e-9 of "
+                                   "the first residual; grids 129x65..1025x257, Re U[50,150]",
+                        "mode": "FP32 factors, FP64 residual each step (outer refinement)",
+                        "peak": rnd(sw["peak"]), "arrival_rate_s": round(sw["rate"], 3),
+                        "poisson": rnd(sw["open_loop"]),
+                        "serial_fp64": rnd(ser64), "serial_fp32_outer": rnd(ser32),
+                        "time_to_solution_ratio_fp64_over_fp32":
+                            round(ser32["solves_per_s"] / ser64["solves_per_s"], 3)}
 
     # ---- CPU baseline ----
     if not args.no_cpu:

@@ -56,7 +56,21 @@ typedef struct {
   double alfa;      /* time-scheme factor: 1 for backward Euler */
   int periodic_exact;
   int precision;    /* 0: complex-double factors; 1: complex-float factors */
+  /* &extras namelist (src/lns3d.f90:42) and deck entries; zero selects the
+   * reference's default for each */
+  int wall_is, wall_ie; /* wall extent is:ie, 1-based (src/lns3d.f90:73
+                         * default 1:nx); 0, 0 = the whole wall */
+  int idamp;        /* viscous damper (src/input.f90:268, src/damper.f90) */
+  int use_calcd;    /* useCalcd (src/global.f90:34): the D terms alternate
+                     * between the factors (src/lhs1f3d.f90:27-37) */
+  int update_lhs;   /* updateLHS (src/global.f90:33): factors formed again
+                     * every iteration (src/impdrv3d.f90:248,310) */
 } lns3d_params;
+/* Not implemented, rejected with LNS3D_EUNSUPPORTED: a wall extent other
+ * than 1:nx, idamp != 0, use_calcd, update_lhs, and Re = 0 with mattyp = 0
+ * (the reference's inviscid deck, src/input.f90:31-36: Navier = .false.).
+ * Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, and Re = 0 with
+ * mattyp != 0 are rejected with LNS3D_EINVAL. */
 
 /* ======================================================================
  * Block-pentadiagonal line solvers, drop-in form (host arrays).
@@ -101,6 +115,11 @@ int lns3d_release_factors(void);
 typedef struct lns3d_penta lns3d_penta;
 int lns3d_penta_create(int kind, int layout, int nsys, int nrows, int precision, int exact,
                        lns3d_penta **out);
+/* factors, then checks them (synchronises the stream): a line with a zero
+ * pivot or a non-finite entry returns LNS3D_ENONFINITE, naming the first
+ * such line (the reference divides by the pivot unguarded,
+ * src/cpenta1bc.f:80-82).  The drop-ins above report the same status for
+ * code 0 and 2 (code 2 still solves, as the reference does). */
 int lns3d_penta_factor(lns3d_penta *h, const void *mat_dev, void *stream);
 int lns3d_penta_solve(lns3d_penta *h, void *rhs_dev, void *stream);
 /* same result, one lane per line in the reference's operation order */
@@ -199,6 +218,19 @@ int lns3d_step_forced(lns3d_ctx *c, void *v_dev, const void *vold_dev, const voi
  * driver's convergence measure (src/si_eigdrv3d_new.f90:359-374) */
 int lns3d_forced_residual(lns3d_ctx *c, const void *v_dev, const void *vold_dev, const void *f_dev,
                           void *r_dev, double *norm_out);
+/* residual norm sqrt(sum over points and components of |r|^2 / dtl^2) of r
+ * in the step's form (dtl times the residual, with the rhsBC3D rows): the
+ * measure src/resstat3d.f90:61-89 prints (totres, without its 1/N, which
+ * cancels in the ratio to the first step's that it reports).  Written to
+ * *norm_dev (device) with no host synchronisation; the sum's order is fixed. */
+int lns3d_residual_norm(lns3d_ctx *c, const void *r_dev, double *norm_dev);
+/* record the residual norm of every step: lns3d_step (first corrector),
+ * lns3d_step_fp32 and lns3d_step_refined_fixed write the norm of the step's
+ * residual (before the line solves, as src/impdrv3d.f90:207 calls
+ * resstat3D) into norms_dev[k mod count], k counting the steps since this
+ * call; graph-capturable.  NULL stops recording.  No counterpart in the
+ * reference (its norm is printed per step on the host). */
+int lns3d_ctx_set_residual_output(lns3d_ctx *c, double *norms_dev, int count);
 /* corrector iterations per step, the deck's niter (src/impdrv3d.f90:173-358):
  * the residual with vold held at the start of the step, solved with the
  * factors of the first iteration, niter times.  lns3d_step only (impl 1 or
@@ -213,7 +245,9 @@ int lns3d_get_coeffs(lns3d_ctx *c, double *Ah, double *Bh, double *Dh, double *D
                      double *Vh11, double *Vh12, double *Vh22, double *ABhi, double *dtl);
 /* replaces src/lhs1f3d.f90:2 lhs1f3D + src/lhs2f3d.f90:2 lhs2f3D (with
  * lhsbt1f3D/lhsbt2f3d/lhsbc1f3d/lhsbc2f3d) and the factor calls
- * cpenta1bc/cpenta2p(..,0) of src/impdrv3d.f90:252-269,314-319 */
+ * cpenta1bc/cpenta2p(..,0) of src/impdrv3d.f90:252-269,314-319; checks the
+ * factors as lns3d_penta_factor does (LNS3D_ENONFINITE: the context stays
+ * unfactored) */
 int lns3d_factor(lns3d_ctx *c);
 /* assembled LHS of direction dir (1 xi, 2 eta) into mat_dev (5,5,5,nx,ny) */
 int lns3d_assemble(lns3d_ctx *c, int dir, void *mat_dev);
@@ -232,7 +266,10 @@ int lns3d_solve_dir(lns3d_ctx *c, int dir, void *r_dev);
 int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev);
 /* the same step with complex-float factors (precision = 1) used as a
  * defect-correction preconditioner of the complex-double AF system until
- * |res|_max <= tol |r|_max (or max_iters solves) */
+ * |res|_max <= tol |r|_max or max_iters solves (max_iters <= 0: up to 64).
+ * The update is applied either way: *resid_out > tol on return says that
+ * the solve count ran out first.  A non-finite residual returns
+ * LNS3D_ENONFINITE and leaves v unchanged. */
 int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int max_iters,
                        int *iters_out, double *resid_out);
 /* the step with complex-float factors and no inner refinement */

@@ -49,6 +49,11 @@ class Params:
     dtmax: float = 1.0e30
     alfa: float = 1.0
     periodic_exact: int = 0
+    wall_is: int = 0         # &extras is, ie (1-based wall extent); 0: 1..nx
+    wall_ie: int = 0
+    idamp: int = 0
+    use_calcd: int = 0       # &extras useCalcd, updateLHS
+    update_lhs: int = 0
     xmin: float = 0.0
     xmax: float = 100.0
     ymax: float = 40.0

@@ -83,8 +83,19 @@ extern "C" int lns3d_penta_factor(lns3d_penta *h, const void *mat_dev, void *str
   hipStream_t s = (hipStream_t)stream;
   int rc = penta_import(&h->plan, mat_dev, sL, sR, s);
   if (!rc) rc = penta_factor(&h->plan, s);
-  if (!rc) h->factored = 1;
-  return rc;
+  if (!rc) rc = penta_check_launch(&h->plan, s);
+  if (!rc && h->plan.prec) rc = penta_check_records_launch(&h->plan, s);
+  long bad = 0, first = -1;
+  if (!rc) rc = penta_check_result(&h->plan, s, &bad, &first);
+  if (rc) return rc;
+  h->factored = 1;
+  if (bad) {
+    char msg[160];
+    snprintf(msg, sizeof msg, "lns3d_penta_factor: %ld line(s) with a zero pivot or a non-finite factor (first: line %ld)",
+             bad, first + 1);
+    return fail(LNS3D_ENONFINITE, msg);
+  }
+  return 0;
 }
 
 extern "C" int lns3d_penta_solve(lns3d_penta *h, void *rhs_dev, void *stream) {
@@ -137,6 +148,7 @@ static int dropin(int kind, int layout, int nsys, int nrows, int nblk, void *mat
   DropKey key{mat, kind, layout, nsys, nrows};
   size_t nmat = (size_t)nsys * nrows * 125 * 16, nrhs = (size_t)nsys * nrows * 5 * 16;
   auto it = g_drop.find(key);
+  int factor_rc = 0;
   if (code == 0 || code == 2) {
     if (it == g_drop.end()) {
       DropEntry e;
@@ -150,18 +162,22 @@ static int dropin(int kind, int layout, int nsys, int nrows, int nblk, void *mat
     }
     it->second.h->plan.exact = g_periodic_exact;
     LNS3D_CHECK(hipMemcpy(it->second.dmat, mat, nmat, hipMemcpyHostToDevice));
-    int rc = lns3d_penta_factor(it->second.h, it->second.dmat, nullptr);
-    if (rc) return rc;
+    // a zero pivot or a non-finite factor is reported (LNS3D_ENONFINITE) and,
+    // as in the reference, the solve of code 2 still runs and spreads it
+    factor_rc = lns3d_penta_factor(it->second.h, it->second.dmat, nullptr);
+    if (factor_rc && factor_rc != LNS3D_ENONFINITE) return factor_rc;
   } else if (it == g_drop.end()) {
     return fail(LNS3D_ENOTFACTORED, "cpenta: solve (code 1) before factor (code 0) for this mat");
   }
   if (code == 1 || code == 2) {
+    const std::string msg = lns3d_last_error();
     LNS3D_CHECK(hipMemcpy(it->second.drhs, rhs, nrhs, hipMemcpyHostToDevice));
     int rc = lns3d_penta_solve(it->second.h, it->second.drhs, nullptr);
     if (rc) return rc;
     LNS3D_CHECK(hipMemcpy(rhs, it->second.drhs, nrhs, hipMemcpyDeviceToHost));
+    if (factor_rc) g_err = msg;
   }
-  return 0;
+  return factor_rc;
 }
 
 extern "C" int lns3d_cpenta1bc(int nsys, int n, int nblk, void *mat, void *rhs, int code) {

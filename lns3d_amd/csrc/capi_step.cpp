@@ -2,6 +2,8 @@
 // (grid, mean flow, coefficients, factors, work arrays), all in HBM.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <cmath>
+#include <stdio.h>
 #include <string.h>
 #include <string>
 #include <complex>
@@ -32,7 +34,11 @@ struct lns3d_ctx {
   void *work = nullptr;      // 5P complex
   void *r = nullptr;         // 5P complex
   void *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;   // 5P complex each (refinement)
-  double *scal = nullptr;    // device scalars (residual norms at +8, partials after)
+  double *scal = nullptr;    // device scalars: [0,8) time step, [8,2048) refinement norms and
+                             // partials, [2048,3072) step residual-norm partials
+  double *norm_out = nullptr;  // per-step residual norms (lns3d_ctx_set_residual_output)
+  int norm_count = 0;
+  long norm_idx = 0;
   double *coef_block = nullptr;
   double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
   double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
@@ -57,6 +63,24 @@ struct lns3d_ctx {
 
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
+  if (!std::isfinite(p.Ma) || !std::isfinite(p.Re) || !std::isfinite(p.Pr) || !std::isfinite(p.gamma) ||
+      !std::isfinite(p.kz) || !std::isfinite(p.omega))
+    return fail(LNS3D_EINVAL, "Ma, Re, Pr, gamma, kz and omega must be finite");
+  if (p.Re < 0.0) return fail(LNS3D_EINVAL, "Re must be >= 0");
+  // Re = 0 is the reference's inviscid deck for mattyp = 0 (src/input.f90:31-36)
+  if (p.Re == 0.0 && p.mattyp != 0)
+    return fail(LNS3D_EINVAL, "Re = 0 (inviscid) is defined for mattyp = 0 only (src/input.f90:31-36)");
+  if (p.Re == 0.0)
+    return fail(LNS3D_EUNSUPPORTED, "Re = 0: the inviscid wall rows (Navier = .false.) are not implemented");
+  if (p.wall_is < 0 || p.wall_ie < 0 || p.wall_is > p.nx || p.wall_ie > p.nx ||
+      (p.wall_is && p.wall_ie && p.wall_is > p.wall_ie))
+    return fail(LNS3D_EINVAL, "wall extent: 1 <= wall_is <= wall_ie <= nx (0 for the default)");
+  if ((p.wall_is > 1 || (p.wall_ie != 0 && p.wall_ie != p.nx)))
+    return fail(LNS3D_EUNSUPPORTED, "a wall extent other than 1:nx (&extras is, ie) is not implemented");
+  if (p.idamp != 0)
+    return fail(LNS3D_EUNSUPPORTED, "idamp != 0 (the viscous damper, src/damper.f90) is not implemented");
+  if (p.use_calcd || p.update_lhs)
+    return fail(LNS3D_EUNSUPPORTED, "useCalcd and updateLHS are not implemented");
   if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
       (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8) || p.linear != 2 ||
@@ -111,7 +135,7 @@ extern "C" int lns3d_ctx_create(const lns3d_params *p, lns3d_ctx **out) {
   c->c.metric = b;
   if (hipMalloc(&c->work, (size_t)P * 5 * 16) != hipSuccess ||
       hipMalloc(&c->r, (size_t)P * 5 * 16) != hipSuccess ||
-      hipMalloc(&c->scal, 2048 * sizeof(double)) != hipSuccess)
+      hipMalloc(&c->scal, 4096 * sizeof(double)) != hipSuccess)
     return bail(fail(LNS3D_EHIP, "hipMalloc work"));
   rc = penta_plan_create(&c->px, p->xper, p->periodic_exact, p->nx, p->ny, 0);
   if (!rc) rc = penta_plan_create(&c->py, p->yper, p->periodic_exact, p->ny, p->nx, 0);
@@ -492,8 +516,20 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
   if (rc) return rc;
   if (c->p.precision == 1) {
     if (!c->have32) {
-      rc = penta_plan_create(&c->px32, 0, 0, c->p.nx, c->p.ny, 1);
-      if (!rc) rc = penta_plan_create(&c->py32, 0, 0, c->p.ny, c->p.nx, 1);
+      // The complex-float records decode the complex-double LU, whose layout
+      // depends on its family (a twisted LU eliminates the bottom chain
+      // bottom-up): take the tuned complex-float variant when it is of the
+      // same family, else the family's default.
+      auto family_variant = [](const PentaPlan &d, int n, long lines) {
+        const int v = penta_variant_for(n, lines, 1);
+        if (penta_twisted(v) == (d.h >= 0)) return v;
+        return d.h >= 0 ? (int)PENTA_TWISTED_PAIRS : (lines >= 512 ? (int)PENTA_PLAIN3 : (int)PENTA_PLAIN1);
+      };
+      rc = penta_plan_create_variant(&c->px32, 0, 0, c->p.nx, c->p.ny, 1, family_variant(c->px, c->p.nx, c->p.ny));
+      if (!rc)
+        rc = penta_plan_create_variant(&c->py32, 0, 0, c->p.ny, c->p.nx, 1, family_variant(c->py, c->p.ny, c->p.nx));
+      if (!rc && (c->px32.h != c->px.h || c->py32.h != c->py.h))
+        rc = fail(LNS3D_EHIP, "lns3d_factor: complex-float plans split their lines unlike the complex-double LU");
       const long P = (long)c->p.nx * c->p.ny;
       if (!rc && (hipMalloc(&c->t1, (size_t)P * 80) != hipSuccess || hipMalloc(&c->t2, (size_t)P * 80) != hipSuccess ||
                   hipMalloc(&c->t3, (size_t)P * 80) != hipSuccess))
@@ -517,7 +553,28 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
     }
     if (rc) return rc;
   }
+  // factor health, both directions (and the complex-float records)
+  rc = penta_check_launch(&c->px, c->stream);
+  if (!rc) rc = penta_check_launch(&c->py, c->stream);
+  long bad[4] = {0, 0, 0, 0}, first[4] = {-1, -1, -1, -1};
+  if (!rc) rc = penta_check_result(&c->px, c->stream, &bad[0], &first[0]);
+  if (!rc) rc = penta_check_result(&c->py, c->stream, &bad[1], &first[1]);
+  if (!rc && c->p.precision == 1) {
+    rc = penta_check_records_launch(&c->px32, c->stream);
+    if (!rc) rc = penta_check_records_launch(&c->py32, c->stream);
+    if (!rc) rc = penta_check_result(&c->px32, c->stream, &bad[2], &first[2]);
+    if (!rc) rc = penta_check_result(&c->py32, c->stream, &bad[3], &first[3]);
+  }
+  if (rc) return rc;
   c->factored = true;
+  for (int k = 0; k < 4; k++)
+    if (bad[k]) {
+      char msg[200];
+      snprintf(msg, sizeof msg, "lns3d_factor: %s factor: %ld line(s) with a zero pivot or a non-finite entry "
+               "(first: %s line %ld)", k % 2 ? "eta" : "xi", bad[k], k % 2 ? "eta" : "xi", first[k] + 1);
+      c->factored = false;
+      return fail(LNS3D_ENONFINITE, msg);
+    }
   return 0;
 }
 
@@ -543,6 +600,27 @@ extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
   if (!c->p.yper && c->p.wall == 3 && !c->c.fwall)
     return fail(LNS3D_EINVAL, "lns3d_itrbc3d: wall = 3 needs its wall values (lns3d_set_wall)");
   return launch_itrbc(c->p, c->c, v_dev, c->stream);
+}
+
+// the residual norm of a step into the next slot of the context's output
+// (lns3d_ctx_set_residual_output), if one is set
+static int record_norm(lns3d_ctx *c, const void *r) {
+  if (!c->norm_out) return 0;
+  double *out = c->norm_out + (c->norm_idx++ % c->norm_count);
+  return launch_residual_norm(c->c.dtl, r, (long)c->p.nx * c->p.ny, c->scal + 2048, out, c->stream);
+}
+
+extern "C" int lns3d_residual_norm(lns3d_ctx *c, const void *r_dev, double *norm_dev) {
+  if (!c || !c->coeffs_ready || !r_dev || !norm_dev) return fail(LNS3D_EINVAL, "lns3d_residual_norm: bad arguments");
+  return launch_residual_norm(c->c.dtl, r_dev, (long)c->p.nx * c->p.ny, c->scal + 2048, norm_dev, c->stream);
+}
+
+extern "C" int lns3d_ctx_set_residual_output(lns3d_ctx *c, double *norms_dev, int count) {
+  if (!c || (norms_dev && count < 1)) return fail(LNS3D_EINVAL, "lns3d_ctx_set_residual_output: bad arguments");
+  c->norm_out = norms_dev;
+  c->norm_count = norms_dev ? count : 0;
+  c->norm_idx = 0;
+  return 0;
 }
 
 static int solve_xy(lns3d_ctx *c, PentaPlan &px, PentaPlan &py, void *r) {
@@ -589,6 +667,7 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   }
   if (c->niter == 1) {
     rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
+    if (!rc) rc = record_norm(c, r);
     if (!rc && c->impl == 2) rc = launch_copy(c->vhist, v_dev, n, c->stream);
     if (!rc) rc = solve_xy(c, c->px, c->py, r);
     if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
@@ -599,6 +678,7 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     for (int it = 0; it < c->niter && !rc; it++) {
       const int mode = it == 0 ? (two ? 2 : 1) : (two ? 4 : 3);
       rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, mode, c->stream, c->vhist, c->vstart);
+      if (!rc && it == 0) rc = record_norm(c, r);
       if (!rc) rc = solve_xy(c, c->px, c->py, r);
       if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
     }
@@ -701,15 +781,17 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
   if (rc) return rc;
   rc = solve_xy(c, c->px32, c->py32, dv);
   if (rc) return rc;
+  // max_iters <= 0: no caller limit; the loop still stops after 64 solves
+  const int cap = max_iters > 0 ? max_iters : 64;
   int it = 1;
   double nr[2] = {0, 0};
-  for (; it < max_iters || max_iters <= 0; it++) {
+  for (;; it++) {
     rc = launch_af_matvec(c->p, c->c, dv, c->t2, c->t1, c->stream);       // t2 = M dv
     if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);   // t2 = rhs - M dv
     if (rc) return rc;
     LNS3D_CHECK(hipMemcpyAsync(nr, c->scal + 8, 16, hipMemcpyDeviceToHost, c->stream));
     LNS3D_CHECK(hipStreamSynchronize(c->stream));
-    if (nr[0] <= tol * nr[1] || it >= max_iters) break;
+    if (nr[0] <= tol * nr[1] || it >= cap || !(nr[0] == nr[0])) break;
     rc = solve_xy(c, c->px32, c->py32, c->t2);
     if (!rc) rc = launch_axpy(dv, c->t2, n, c->stream);
     if (rc) return rc;
@@ -731,6 +813,7 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_fp32: needs precision = 1 and lns3d_factor");
   void *r = dv_dev ? dv_dev : c->r;
   int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
+  if (!rc) rc = record_norm(c, r);
   if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
   if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
   return rc;
@@ -750,6 +833,7 @@ extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev,
   void *dv = dv_dev ? dv_dev : c->r;
   void *rhs = c->t3;
   int rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
+  if (!rc) rc = record_norm(c, rhs);
   if (!rc) rc = launch_copy(dv, rhs, n, c->stream);
   if (!rc) rc = solve_xy(c, c->px32, c->py32, dv);
   for (int k = 0; k < corrections && !rc; k++) {

@@ -33,9 +33,11 @@
 //  with the border sums formed by the loader waves.  k_solve_lines (one lane
 //  per line, the reference's substitution order on the LU working copy) is
 //  kept as the order-exact solve behind lns3d_penta_solve_reference_order.
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "common.hpp"
 #include "penta.hpp"
@@ -1987,9 +1989,86 @@ __global__ void k_import_mat(const double2 *ref, cx<double> *lu, int n, long L, 
   lu[line_major ? line * n * 125 + e : idx] = cx<double>{v.x, v.y};
 }
 
+// Factor health (the reference has no pivoting and no check: a zero pivot
+// spreads Inf/NaN through its line, src/cpenta1bc.f:80-82 fact = one/pivot).
+// One flag per line: set when an entry of the line's factors is not finite
+// or a pivot (diagonal of the slot-3 block) is zero.  a holds total complex
+// values; line-major arrays hold per_line values per line, interleaved ones
+// put the line index fastest (stride L).
+template <typename T>
+__global__ void k_check_factor(const cx<T> *a, long total, long L, int line_major, long per_line,
+                               int pivots, int *flag) {
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const cx<T> x = a[idx];
+    const long line = line_major ? idx / per_line : idx % L;
+    const long e = line_major ? idx % per_line : idx / L;     // ((row*5 + slot)*25 + 5r + c)
+    bool bad = !(isfinite(x.re) && isfinite(x.im));
+    if (pivots && (e / 25) % 5 == 2 && (e % 25) % 6 == 0 && x.re == T(0) && x.im == T(0)) bad = true;
+    if (bad) flag[line] = 1;
+  }
+}
+__global__ void k_zero_i32(int *x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 static int grid_for(long n, int b) { return (int)((n + b - 1) / b); }
+
+static int check_launch_grid(long total) { return (int)std::min<long>(grid_for(total, 256), 4096); }
+
+int penta_check_launch(PentaPlan *pl, hipStream_t s) {
+  if (!pl->flags) LNS3D_CHECK(hipMalloc(&pl->flags, (size_t)pl->L * sizeof(int)));
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(pl->L, 256)), dim3(256), 0, s, (int *)pl->flags, pl->L);
+  const long n = pl->n, L = pl->L;
+  if (pl->periodic) {
+    const long tl = (long)pl->np * 125 * L, tp = (long)pl->np * 50 * L;
+    hipLaunchKernelGGL(k_check_factor<double>, dim3(check_launch_grid(tl)), dim3(256), 0, s,
+                       (const cx<double> *)pl->lu, tl, L, 0, 0L, 1, (int *)pl->flags);
+    hipLaunchKernelGGL(k_check_factor<double>, dim3(check_launch_grid(tp)), dim3(256), 0, s,
+                       (const cx<double> *)pl->per, tp, L, 0, 0L, 0, (int *)pl->flags);
+    hipLaunchKernelGGL(k_check_factor<double>, dim3(check_launch_grid(tp)), dim3(256), 0, s,
+                       (const cx<double> *)pl->per2, tp, L, 0, 0L, 0, (int *)pl->flags);
+  } else {
+    const long tl = n * 125 * L;
+    hipLaunchKernelGGL(k_check_factor<double>, dim3(check_launch_grid(tl)), dim3(256), 0, s,
+                       (const cx<double> *)pl->lu, tl, L, 1, n * 125, 1, (int *)pl->flags);
+  }
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// records of a plan whose LU another plan holds (the complex-float records
+// of the refined step): their entries, per line
+int penta_check_records_launch(PentaPlan *pl, hipStream_t s) {
+  if (pl->periodic || !pl->prec) return 0;
+  if (!pl->flags) LNS3D_CHECK(hipMalloc(&pl->flags, (size_t)pl->L * sizeof(int)));
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(pl->L, 256)), dim3(256), 0, s, (int *)pl->flags, pl->L);
+  const long n = pl->n, L = pl->L;
+  const long tf = n * 60 * L, tb = n * 66 * L;
+  hipLaunchKernelGGL(k_check_factor<float>, dim3(check_launch_grid(tf)), dim3(256), 0, s,
+                     (const cx<float> *)pl->fwd, tf, L, 1, n * 60, 0, (int *)pl->flags);
+  hipLaunchKernelGGL(k_check_factor<float>, dim3(check_launch_grid(tb)), dim3(256), 0, s,
+                     (const cx<float> *)pl->bwd, tb, L, 1, n * 66, 0, (int *)pl->flags);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int penta_check_result(PentaPlan *pl, hipStream_t s, long *bad_lines, long *first_bad) {
+  *bad_lines = 0;
+  *first_bad = -1;
+  if (!pl->flags) return 0;
+  std::vector<int> f((size_t)pl->L);
+  LNS3D_CHECK(hipMemcpyAsync(f.data(), pl->flags, f.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  LNS3D_CHECK(hipStreamSynchronize(s));
+  for (long l = 0; l < pl->L; l++)
+    if (f[l]) {
+      if (*first_bad < 0) *first_bad = l;
+      ++*bad_lines;
+    }
+  return 0;
+}
 
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, long, int>, int> g_tuned;
@@ -2034,7 +2113,7 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
   size_t ne = (size_t)pl->np * 125 * lines;
   LNS3D_CHECK(hipMalloc(&pl->lu, ne * sizeof(cx<double>)));
   pl->per = pl->per2 = nullptr;
-  pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = pl->xf = pl->xb = nullptr;
+  pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = pl->xf = pl->xb = pl->flags = nullptr;
   if (periodic) {
     const size_t z = sizeof(cx<double>);
     LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * z));
@@ -2072,8 +2151,9 @@ void penta_plan_destroy(PentaPlan *pl) {
   if (pl->qp) (void)hipFree(pl->qp);
   if (pl->xf) (void)hipFree(pl->xf);
   if (pl->xb) (void)hipFree(pl->xb);
+  if (pl->flags) (void)hipFree(pl->flags);
   pl->lu = pl->per = pl->per2 = pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = nullptr;
-  pl->xf = pl->xb = nullptr;
+  pl->xf = pl->xb = pl->flags = nullptr;
 }
 
 int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStream_t s) {

@@ -23,6 +23,7 @@ struct PentaPlan {
   void *p2 = nullptr, *qp = nullptr;   // periodic lines: border records (complex double)
   void *xf = nullptr, *xb = nullptr;   // twisted lines: per step pair, the second row's
                                        // original N2 (forward) / Q4 (backward) block
+  void *flags = nullptr;   // per line: 1 when its factors hold a non-finite entry or a zero pivot
 };
 
 // solve-kernel variants of non-periodic lines: one line per workgroup with a
@@ -58,5 +59,12 @@ int penta_factor_records_only(PentaPlan *pl, hipStream_t s);
 // complex offset (l*sL + i*sR)*5 + d
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);
 int penta_solve_reference_order(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);
+// factor health: flag the lines whose LU (and border fill) holds a
+// non-finite entry or a zero pivot (launch after penta_factor), or whose
+// complex-float records hold a non-finite entry (after the records); then
+// count them (synchronises s)
+int penta_check_launch(PentaPlan *pl, hipStream_t s);
+int penta_check_records_launch(PentaPlan *pl, hipStream_t s);
+int penta_check_result(PentaPlan *pl, hipStream_t s, long *bad_lines, long *first_bad);
 
 }  // namespace lns3d

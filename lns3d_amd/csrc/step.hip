@@ -1202,10 +1202,33 @@ __global__ __launch_bounds__(256) void k_forced_norm(const double2 *r, const dou
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// sqrt of the sum of nb partials, in a fixed order (one block): the same
+// bits on every run
+__global__ __launch_bounds__(256) void k_norm_final(const double *part, int nb, double *out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 256) acc += part[k];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(red[0]);
+}
+
+int launch_residual_norm(const double *dtl, const void *r, long P, double *part, double *out, hipStream_t s) {
+  const int blocks = (int)std::min<long>(nblocks(P, 256), NORM_BLOCKS);
+  hipLaunchKernelGGL(k_forced_norm, dim3(blocks), dim3(256), 0, s, (const double2 *)r, dtl, P, part);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, s, (const double *)part, blocks, out);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
 int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,
                        hipStream_t s) {
   const long P = (long)p.nx * p.ny;
-  const int blocks = (int)std::min<long>(nblocks(P, 256), 1024);
+  const int blocks = (int)std::min<long>(nblocks(P, 256), NORM_BLOCKS);
   hipLaunchKernelGGL(k_forced_norm, dim3(blocks), dim3(256), 0, s, (const double2 *)r, (const double *)c.dtl,
                      P, part);
   LNS3D_CHECK(hipGetLastError());

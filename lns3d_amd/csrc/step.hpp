@@ -74,8 +74,12 @@ int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *m
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
                void *work, int with_rbe, hipStream_t s, const void *v2old = nullptr,
                const void *vold = nullptr, const void *forcing = nullptr);
-// sum over the points of |r|^2 / dtl^2, per block into part (nblocks(P, 256)
-// doubles); the caller adds them
+// the residual norm sqrt(sum |r|^2 / dtl^2) into *out (device), its
+// per-block partials in part (NORM_BLOCKS doubles); no host synchronisation
+constexpr int NORM_BLOCKS = 1024;
+int launch_residual_norm(const double *dtl, const void *r, long P, double *part, double *out, hipStream_t s);
+// sum over the points of |r|^2 / dtl^2, per block into part (at most
+// NORM_BLOCKS doubles); the caller adds them
 int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,
                        hipStream_t s);
 // ftop: the top = 1 wave (Coeffs::ftop), null for top = 0

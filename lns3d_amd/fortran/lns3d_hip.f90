@@ -37,11 +37,15 @@ module lns3d_hip
     integer(c_int) :: linear, mattyp, loctime
     real(c_double) :: Ma, Re, Pr, gamma, kz, omega, cflmax, dtmax, alfa
     integer(c_int) :: periodic_exact, precision
+    integer(c_int) :: wall_is = 0, wall_ie = 0, idamp = 0, use_calcd = 0, update_lhs = 0
   end type lns3d_params
 
   integer(c_int), public :: lns3d_hip_last = 0
   type(c_ptr), save :: ctx = c_null_ptr
   real(c_double), save :: s0_given = -1.0_c_double   ! Sutherland's constant for mattyp = 1
+  ! the deck's boundary values (top = 1, left 1/4/5, right 4/5, wall = 3) are
+  ! still to come: lns3d_hip_init did not factor, and lns3d_hip_scheme does not
+  logical, save :: factor_deferred = .false.
 
   public :: lns3d_hip_status, lns3d_hip_message, lns3d_hip_record
   public :: lns3d_hip_init, lns3d_hip_put, lns3d_hip_get, lns3d_hip_advance, lns3d_hip_free
@@ -204,6 +208,7 @@ contains
     if (rc == 0 .and. prm%mattyp == 1) rc = c_set_sutherland(ctx, s0_given)
     if (rc == 0) rc = c_setup(ctx, c_loc(metric), c_loc(vm))
     if (rc == 0 .and. .not. needs_values) rc = c_factor(ctx)
+    factor_deferred = needs_values
     call lns3d_hip_record(rc)
   end subroutine
 
@@ -216,7 +221,10 @@ contains
 
   ! forms the factors again (after any of the setters below)
   subroutine lns3d_hip_factor()
-    call lns3d_hip_record(c_factor(ctx))
+    integer(c_int) :: rc
+    rc = c_factor(ctx)
+    if (rc == 0) factor_deferred = .false.
+    call lns3d_hip_record(rc)
   end subroutine
 
   ! sponge: the module state src/sponge.f90:20 (init_sponge) leaves, spg and
@@ -264,13 +272,15 @@ contains
 
   ! the deck's time scheme and corrector iterations (src/impdrv3d.f90:173-199):
   ! impl 1 backward Euler, 2 two-step backward, 3 alfa = 1/2; niter >= 1
-  ! (impl 1 or 2).  Forms the factors again; a new run starts at lstep 0.
+  ! (impl 1 or 2).  Forms the factors again (unless the deck's boundary
+  ! values are still to come: then lns3d_hip_factor after their setters); a
+  ! new run starts at lstep 0.
   subroutine lns3d_hip_scheme(impl, niter)
     integer, intent(in) :: impl, niter
     integer(c_int) :: rc
     rc = c_set_time_scheme(ctx, int(impl, c_int))
     if (rc == 0) rc = c_set_corrector_iterations(ctx, int(niter, c_int))
-    if (rc == 0) rc = c_factor(ctx)
+    if (rc == 0 .and. .not. factor_deferred) rc = c_factor(ctx)
     call lns3d_hip_record(rc)
   end subroutine
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "lns3d_set_sutherland": ([VP, ctypes.c_double], CI),
     "lns3d_step_forced": ([VP, VP, VP, VP, VP], CI),
     "lns3d_forced_residual": ([VP, VP, VP, VP, VP, ctypes.POINTER(ctypes.c_double)], CI),
+    "lns3d_residual_norm": ([VP, VP, VP], CI),
+    "lns3d_ctx_set_residual_output": ([VP, VP, CI], CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),
@@ -85,7 +87,14 @@ _lib = None
 
 
 class LNS3DError(RuntimeError):
-    pass
+    """a non-zero status of the library; .code is the LNS3D_E* value"""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
+
+
+EINVAL, EHIP, ENOTFACTORED, EUNSUPPORTED, ENONFINITE = 1, 2, 3, 4, 5
 
 
 def lib():
@@ -107,7 +116,7 @@ def lib():
 def check(rc: int, what: str = ""):
     if rc != 0:
         msg = lib().lns3d_last_error().decode(errors="replace")
-        raise LNS3DError(f"{what}: error {rc}: {msg}")
+        raise LNS3DError(f"{what}: error {rc}: {msg}", rc)
 
 
 def ptr(t) -> int:
@@ -155,7 +164,8 @@ class CParams(ctypes.Structure):
                                   "top", "left", "right", "linear", "mattyp", "loctime")] + \
                [(n, CD) for n in ("Ma", "Re", "Pr", "gamma", "kz", "omega", "cflmax", "dtmax",
                                   "alfa")] + \
-               [("periodic_exact", CI), ("precision", CI)]
+               [("periodic_exact", CI), ("precision", CI)] + \
+               [(n, CI) for n in ("wall_is", "wall_ie", "idamp", "use_calcd", "update_lhs")]
 
     @classmethod
     def from_params(cls, p, precision=0):
@@ -164,7 +174,7 @@ class CParams(ctypes.Structure):
             if name == "precision":
                 s.precision = precision
             else:
-                setattr(s, name, getattr(p, name))
+                setattr(s, name, getattr(p, name, 0))
         return s
 
 
@@ -213,6 +223,17 @@ class Step:
         check(lib().lns3d_forced_residual(self.h, ptr(v_dev), ptr(vold_dev), ptr(f_dev), ptr(r_dev),
                                           ctypes.byref(out)), "lns3d_forced_residual")
         return out.value
+
+    def residual_norm(self, r_dev, norm_dev):
+        """sqrt(sum |r|^2 / dtl^2) of r_dev into the device double norm_dev"""
+        check(lib().lns3d_residual_norm(self.h, ptr(r_dev), ptr(norm_dev)), "lns3d_residual_norm")
+
+    def set_residual_output(self, norms_dev=None, count=0):
+        """every step writes its residual norm to norms_dev[k mod count] (device
+        float64 tensor); None stops"""
+        check(lib().lns3d_ctx_set_residual_output(self.h, None if norms_dev is None else ptr(norms_dev),
+                                                  int(count)), "lns3d_ctx_set_residual_output")
+        self._norms_keep = norms_dev
 
     def set_sutherland(self, S0):
         """Sutherland's constant S0 = 110.4 K / Tref for mattyp = 1; before setup"""

@@ -53,7 +53,7 @@ def test_error_query_without_gpu(lib):
 
 
 def test_params_struct_layout():
-    """CParams mirrors lns3d_params: 14 ints, 9 doubles, 2 ints"""
+    """CParams mirrors lns3d_params: 14 ints, 9 doubles, 7 ints"""
     from lns3d_amd.native import CParams
     names = [f[0] for f in CParams._fields_]
     src = open(HEADER).read()
@@ -144,10 +144,42 @@ def test_request_and_instance_draws():
     assert [r.arrival for r in a] == [r.arrival for r in b]
     gaps = np.diff([0.0] + [r.arrival for r in a])
     assert abs(gaps.mean() - 1 / 50.0) < 0.15 / 50.0
-    assert all(0 <= r.beta <= 1 and 0.01 <= r.omega <= 0.2 and r.steps % 10 == 0 for r in a)
+    assert all(0 <= r.beta <= 1 and 0.01 <= r.omega <= 0.2 and 50 <= r.Re <= 150 for r in a)
     assert all(r.arrival == 0.0 for r in draw_requests(5, None, seed=1))
     inst = draw_instances(64, seed=0)
     assert len(inst) == 64
     assert all(65 <= p.nx <= 513 and 33 <= p.ny <= 257 for p in inst)
     assert all(0.3 <= p.Ma <= 0.8 and 500 <= p.Re <= 3000 for p in inst)
     assert len({(p.nx, p.ny) for p in inst}) > 32
+
+
+@pytest.mark.parametrize("change,code,text", [
+    (dict(Re=-1.0), 1, b"Re must be >= 0"),
+    (dict(Ma=float("nan")), 1, b"must be finite"),
+    (dict(kz=float("inf")), 1, b"must be finite"),
+    (dict(Re=0.0, mattyp=1), 1, b"mattyp = 0 only"),
+    (dict(Re=0.0), 4, b"inviscid"),
+    (dict(wall_is=5, wall_ie=3), 1, b"wall extent"),
+    (dict(wall_is=2, wall_ie=33), 4, b"wall extent"),
+    (dict(idamp=1), 4, b"idamp"),
+    (dict(use_calcd=1), 4, b"useCalcd"),
+    (dict(update_lhs=1), 4, b"updateLHS"),
+])
+def test_params_refused_before_any_device_call(lib, change, code, text):
+    """lns3d_ctx_create checks the deck before it touches the device: values
+    the reference defines differently, or that the library does not
+    implement, come back as LNS3D_EINVAL (1) or LNS3D_EUNSUPPORTED (4) with
+    a message, never as a context that computes something else (Re = 0 with
+    constant viscosity is the reference's inviscid deck, src/input.f90:31-36)"""
+    import dataclasses
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import CParams
+    p = dataclasses.replace(Params(nx=33, ny=17), **change)
+    cp = CParams.from_params(p, 0)
+    out = ctypes.c_void_p()
+    lib.lns3d_ctx_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    lib.lns3d_last_error.restype = ctypes.c_char_p
+    rc = lib.lns3d_ctx_create(ctypes.byref(cp), ctypes.byref(out))
+    assert rc == code
+    assert not out.value
+    assert text in lib.lns3d_last_error()

@@ -8,9 +8,11 @@ autotuner and the serving loop, on the GPU.
 * The refined step with one correction agrees with the complex-double step
   to 1e-10 relative, and its residual check reports <= 1e-10.
 * The autotuner returns a variant whose plans still solve like the oracle.
-* Serving: a request gives the same state alone and inside a mix of
-  overlapping requests (bit for bit), and a memory budget that holds one
-  context at a time still completes every request."""
+* Serving: receptivity requests converge to 1e-9 of their first residual;
+  a request gives the same state alone and inside a mix of overlapping
+  requests (bit for bit); the complex-float mode reaches the complex-double
+  march's steady state; a refused request fails alone; a memory budget that
+  holds one context at a time still completes every request."""
 import numpy as np
 import pytest
 
@@ -108,47 +110,83 @@ def test_tuner_choice_solves(torch_cuda, orc):
     h.close()
 
 
-GRIDS = ((33, 17), (65, 33))
-
-
-def _reqs(count, rate, seed):
+def _reqs(count, rate, seed, grids, weights, **kw):
     from lns3d_amd.serve import draw_requests
-    return draw_requests(count, rate, seed, steps=(10, 30), chunk=10, grids=GRIDS, grid_weights=(1, 1))
+    return draw_requests(count, rate, seed, grids=grids, grid_weights=weights, **kw)
 
 
 def test_serve_alone_and_in_mix(torch_cuda):
-    from lns3d_amd.serve import Server
-    print("serve alone vs mix", flush=True)
-    mix = _reqs(8, 400.0, 11)
-    target = mix[3]
-    srv = Server(8 * 2**30, keep_states=True)
-    alone = srv.run([dataclass_replace(target, arrival=0.0)])[0]["state"]
+    """receptivity requests over 129x65 .. 1025x257 arriving while others
+    run: each converges to 1e-9 of its first residual, and the one on the
+    largest grid gives the same state, bit for bit, alone and in the mix"""
+    from lns3d_amd.serve import Server, GRIDS
+    print("serve: Poisson mix over 129x65 .. 1025x257", flush=True)
+    mix = _reqs(6, 40.0, 11, GRIDS, (1, 1, 1, 1))
+    mix[-1] = dataclass_replace(mix[-1], nx=1025, ny=257)
+    srv = Server(64 * 2**30, keep_states=True)
     recs = srv.run(mix)
+    for r in recs:
+        print(f"  rid {r['rid']} {r['nx']}x{r['ny']}: {r['steps']} steps, ratio {r['ratio']:.2e}, "
+              f"latency {r['latency']*1e3:.0f} ms", flush=True)
+    assert len(recs) == len(mix) and all(r["error"] is None and r["converged"] for r in recs)
+    target = mix[-1]
+    alone = srv.run([dataclass_replace(target, arrival=0.0)])[0]
     srv.close()
-    got = [r for r in recs if r["rid"] == target.rid][0]["state"]
-    assert len(recs) == len(mix)
-    assert np.array_equal(alone, got)
-    assert np.all(np.isfinite(got))
+    got = [r for r in recs if r["rid"] == target.rid][0]
+    assert alone["steps"] == got["steps"]
+    assert np.array_equal(alone["state"], got["state"])
+    assert np.all(np.isfinite(got["state"]))
+
+
+def test_serve_matches_complex_double_march(torch_cuda):
+    """per grid, a request served with complex-float factors (the default
+    mode) converges to the steady state of the complex-double march (the
+    complex-double step is the oracle-pinned one)"""
+    from lns3d_amd.serve import Server, GRIDS
+    reqs = [dataclass_replace(r, rid=k, nx=g[0], ny=g[1], arrival=0.0)
+            for k, (r, g) in enumerate(zip(_reqs(4, None, 5, GRIDS, (1, 1, 1, 1)), GRIDS))]
+    out = {}
+    for mode in ("fp32_outer", "fp64"):
+        print(f"serve: one request per grid, {mode}", flush=True)
+        srv = Server(64 * 2**30, mode=mode, keep_states=True)
+        out[mode] = srv.run(reqs)
+        srv.close()
+    for a, b in zip(out["fp32_outer"], out["fp64"]):
+        assert a["converged"] and b["converged"]
+        e = float(np.abs(a["state"] - b["state"]).max() / np.abs(b["state"]).max())
+        print(f"  {a['nx']}x{a['ny']}: steps {a['steps']} (fp64 {b['steps']}), state rel diff {e:.2e}",
+              flush=True)
+        assert e < 1e-6
+
+
+def test_serve_isolates_a_failing_request(torch_cuda):
+    """a request the library refuses fails alone: the others complete with
+    the states they reach without it"""
+    from lns3d_amd.serve import Server
+    grids, w = ((129, 65), (257, 129)), (1, 1)
+    reqs = _reqs(4, None, 7, grids, w, max_steps=400)
+    bad = dataclass_replace(reqs[1], Re=-1.0)            # refused by lns3d_ctx_set_params
+    srv = Server(64 * 2**30, keep_states=True)
+    recs = srv.run([reqs[0], bad, reqs[2], reqs[3]])
+    assert recs[1]["error"] and "Re must be" in recs[1]["error"]
+    solo = {r.rid: srv.run([r])[0] for r in (reqs[0], reqs[2], reqs[3])}
+    srv.close()
+    for r in (recs[0], recs[2], recs[3]):
+        assert r["error"] is None
+        assert np.array_equal(r["state"], solo[r["rid"]]["state"])
 
 
 def test_serve_tight_budget(torch_cuda):
     """the pool never holds more than the budget; with room for one context
     of the single grid in use, requests run one after another"""
-    import dataclasses
-    from lns3d_amd.native import ctx_bytes
     from lns3d_amd.serve import Server
-    reqs = _reqs(5, None, 4)
-    one = max(ctx_bytes(r.params(), 1) + 160 * r.nx * r.ny for r in reqs)
-    srv = Server(one + 1024, keep_states=False)
+    reqs = _reqs(4, None, 4, ((129, 65),), (1,), max_steps=200)
+    srv = Server(0, keep_states=False)
+    one = srv._need(reqs[0])
+    srv.budget = one + 1024
     recs = srv.run(reqs)
-    assert len(recs) == 5
-    assert srv._held() <= srv.budget
-    srv.close()
-    same = [dataclasses.replace(r, nx=65, ny=33) for r in reqs]
-    one = ctx_bytes(same[0].params(), 1) + 160 * 65 * 33
-    srv = Server(one + 1024, keep_states=False)
-    recs = srv.run(same)
-    assert len(recs) == 5 and len(srv.pool) == 1
+    assert len(recs) == 4 and all(r["error"] is None for r in recs)
+    assert srv._held() <= srv.budget and len(srv.pool) == 1
     assert all(b["admitted"] >= a["done"] - 1e-3 for a, b in zip(recs, recs[1:]))
     srv.close()
 

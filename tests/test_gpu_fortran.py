@@ -99,7 +99,8 @@ class FParams(ctypes.Structure):
                  "right", "linear", "mattyp", "loctime")] + \
                [(n, ctypes.c_double) for n in
                 ("Ma", "Re", "Pr", "gamma", "kz", "omega", "cflmax", "dtmax", "alfa")] + \
-               [("periodic_exact", ctypes.c_int), ("precision", ctypes.c_int)]
+               [("periodic_exact", ctypes.c_int), ("precision", ctypes.c_int)] + \
+               [(n, ctypes.c_int) for n in ("wall_is", "wall_ie", "idamp", "use_calcd", "update_lhs")]
 
 
 def test_module_driver_steps(flib, orc):
@@ -113,7 +114,7 @@ def test_module_driver_steps(flib, orc):
     v0 = make_disturbance(p, x, seed=5, amp=1.0)
     fp = FParams()
     for name, _ in FParams._fields_:
-        setattr(fp, name, 0 if name == "precision" else getattr(p, name))
+        setattr(fp, name, 0 if name == "precision" else getattr(p, name, 0))
     fn = lambda s: getattr(flib, "_QMlns3d_hipPlns3d_hip_" + s)
     print("fortran module driver 33x17", flush=True)
     fn("init")(ctypes.byref(fp), P(met), P(vm))
@@ -148,7 +149,7 @@ def test_module_driver_scheme(flib, orc):
     v0 = make_disturbance(p, x, seed=6, amp=1.0)
     fp = FParams()
     for name, _ in FParams._fields_:
-        setattr(fp, name, 0 if name == "precision" else getattr(p, name))
+        setattr(fp, name, 0 if name == "precision" else getattr(p, name, 0))
     fn = lambda s: getattr(flib, "_QMlns3d_hipPlns3d_hip_" + s)
     print("fortran module driver 33x17 impl 2 niter 2", flush=True)
     fn("init")(ctypes.byref(fp), P(met), P(vm))
@@ -200,7 +201,7 @@ def test_module_driver_deck_extras(flib, case):
     print(f"fortran module driver {case} {p.nx}x{p.ny}", flush=True)
     fp = FParams()
     for name, _ in FParams._fields_:
-        setattr(fp, name, 0 if name == "precision" else getattr(p, name))
+        setattr(fp, name, 0 if name == "precision" else getattr(p, name, 0))
     fn = lambda s: getattr(flib, "_QMlns3d_hipPlns3d_hip_" + s)
     D = lambda v: ctypes.byref(ctypes.c_double(v))
     met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
