@@ -105,6 +105,68 @@ int lns3d_set_periodic_exact(int exact);
 int lns3d_release_factors(void);
 
 /* ======================================================================
+ * Drop-ins for the RHS, the LHS assembly and the derivative routines, with
+ * the reference's argument lists (host arrays in its layouts).  The routines
+ * that read module state in the reference (lrhs3D, lhs1f3D, lhs2f3D) read it
+ * here from the state the caller forwards once through
+ * lns3d_set_module_state: the variables of src/global.f90, local2d.f90 and
+ * local3d.f90 the step reads.  Arrays are copied over PCIe on every call
+ * (the device-resident context below avoids that).
+ * ====================================================================== */
+typedef struct {
+  int nx, ny, ndof;                       /* sizes; ndof must be 5 */
+  double alfa, dxi, deta;                 /* scheme (dxi = 1/(nx-1), deta = 1/(ny-1)) */
+  int optx, opty, carp;                   /* carp (Carpenter closure, impl = 0 only) must be 0 */
+  int xper, yper, lsym, rsym, bsym, tsym; /* lsym/rsym follow left/right = 7; bsym = tsym = 0 */
+  int wall, wallt, top, left, right;      /* boundary selectors, as lns3d_params */
+  int is, ie, navier;                     /* wall extent (1-based) and Navier */
+  int linear, mattyp, loctime, idamp;
+  double Ma, Re, Pr, gamma, gamma1, kz, omega, x0, S0, cflmax, dtmax;   /* physics */
+  int ispg;                               /* sponge (src/global.f90:132-141) */
+  const double *spg, *spg2;               /* (nx,ny); NULL when ispg does not use them */
+  double eps_e;                           /* smoother and its damping profile */
+  const double *buff;                     /* (nx,ny); NULL when eps_e = 0 */
+  int use_calcd, update_lhs, istep, iter; /* factor control */
+  const double *wamp;                     /* (nx) sponge / top wave amplitude; NULL: 1 */
+  const double *vm;                       /* (5,nx,ny) mean flow */
+  const double *x;                        /* (nx,ny) physical x */
+  const double *bnb;                      /* (nx,2) wall normals (src/gengrid.f90:133-137) */
+  const double *eig;                      /* (10,ny) left = 4 inflow profiles; NULL otherwise */
+  double lalpha[2];                       /* inflow namelist's lalpha (re, im) */
+  const double *bump;                     /* (nx,5) complex wall values for wall = 3; NULL otherwise */
+  const double *Ah, *Bh, *Dh, *Dhi;       /* (5,5,nx,ny) src/local2d.f90:34-36 */
+  const double *Vh11, *Vh12, *Vh22, *ABhi;/* (6,nx,ny) src/local3d.f90:14 */
+  const double *dtl;                      /* (nx,ny) */
+} lns3d_module_state;
+/* forward the module state; creates (or reuses) the context the drop-ins
+ * below compute with.  Options the device path does not implement are
+ * refused as lns3d_ctx_create refuses them. */
+int lns3d_set_module_state(const lns3d_module_state *s);
+/* replaces src/lrhs3d.f90:2 lrhs3D(vl,rl,vml,xl,yl): rl = the linearized
+ * operator of vl with the forwarded coefficients, sponge and smoother terms
+ * (vml, xl, yl are the module arrays the state already holds; unused) */
+int lns3d_drop_lrhs3d(const void *vl, void *rl, const double *vml, const double *xl, const double *yl);
+/* replaces src/lhs1f3d.f90:2 lhs1f3D(mat,Ah,Dh,Dhi,Vh11,ABhi,dtl,vl) with
+ * lhsbt1f3D/lhsbs1f3D/lhsbc1f3d: mat(5,5,5,nx,ny) assembled from the given
+ * arrays and the forwarded state (vl unused, as in the reference) */
+int lns3d_drop_lhs1f3d(void *mat, const double *Ah, const double *Dh, const double *Dhi, const double *Vh11,
+                       const double *ABhi, const double *dtl, const void *vl);
+/* replaces src/lhs2f3d.f90:2 lhs2f3D(mat,Bh,Dh,Dhi,Vh22,ABhi,dtl,vl) with
+ * lhsbt2f3d/lhsbc2f3d */
+int lns3d_drop_lhs2f3d(void *mat, const double *Bh, const double *Dh, const double *Dhi, const double *Vh22,
+                       const double *ABhi, const double *dtl, const void *vl);
+/* replaces src/cgrad.f90:2-3 cgrad(ndof,nx,ny,v,g1v,g2v,dx,dy,optx,opty,
+ * xper,yper,lsym,rsym,bsym,tsym,carp); logicals as 0/1; ndof = 5, bsym =
+ * tsym = carp = 0 */
+int lns3d_drop_cgrad(int ndof, int nx, int ny, const void *v, void *g1v, void *g2v, double dx, double dy,
+                     int optx, int opty, int xper, int yper, int lsym, int rsym, int bsym, int tsym, int carp);
+/* replaces src/cgrad2.f90:2-3 cgrad2(ndof,nx,ny,v,g1v,g11v,g12v,g22v,dx,dy,
+ * optx,opty,xper,yper,lsym,rsym,bsym,tsym,carp); g12v = d/deta of g1v */
+int lns3d_drop_cgrad2(int ndof, int nx, int ny, const void *v, const void *g1v, void *g11v, void *g12v,
+                      void *g22v, double dx, double dy, int optx, int opty, int xper, int yper, int lsym,
+                      int rsym, int bsym, int tsym, int carp);
+
+/* ======================================================================
  * Device-resident solver handles: factor once from a device matrix in the
  * reference's layout, then solve device right-hand sides in place.
  * kind 0 = non-periodic (cpenta*bc), 1 = periodic (cpenta*p);
@@ -243,17 +305,19 @@ int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const dou
                      const double *Vh22, const double *ABhi, const double *dtl);
 int lns3d_get_coeffs(lns3d_ctx *c, double *Ah, double *Bh, double *Dh, double *Dhi,
                      double *Vh11, double *Vh12, double *Vh22, double *ABhi, double *dtl);
-/* replaces src/lhs1f3d.f90:2 lhs1f3D + src/lhs2f3d.f90:2 lhs2f3D (with
- * lhsbt1f3D/lhsbt2f3d/lhsbc1f3d/lhsbc2f3d) and the factor calls
+/* the device form of src/lhs1f3d.f90:2 lhs1f3D + src/lhs2f3d.f90:2 lhs2f3D
+ * (with lhsbt1f3D/lhsbt2f3d/lhsbc1f3d/lhsbc2f3d) and the factor calls
  * cpenta1bc/cpenta2p(..,0) of src/impdrv3d.f90:252-269,314-319; checks the
  * factors as lns3d_penta_factor does (LNS3D_ENONFINITE: the context stays
  * unfactored) */
 int lns3d_factor(lns3d_ctx *c);
 /* assembled LHS of direction dir (1 xi, 2 eta) into mat_dev (5,5,5,nx,ny) */
 int lns3d_assemble(lns3d_ctx *c, int dir, void *mat_dev);
-/* replaces src/lrhs3d.f90:2 lrhs3D(vl,rl,vml,xl,yl): r = L(v) */
+/* the device form of src/lrhs3d.f90:2 lrhs3D: r = L(v) on device arrays
+ * with the context's coefficients (the drop-in with the reference's
+ * argument list is lns3d_drop_lrhs3d) */
 int lns3d_lrhs3d(lns3d_ctx *c, const void *v_dev, void *r_dev);
-/* replaces src/itrbc3d.f90:2 itrBC3D(vl,vml) */
+/* the device form of src/itrbc3d.f90:2 itrBC3D(vl,vml) */
 int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev);
 /* xi then eta line solves of r in place (src/impdrv3d.f90:284-327) */
 int lns3d_solve(lns3d_ctx *c, void *r_dev);

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "liblns3d_amd.so")
 FLIB = os.path.join(PKG, "liblns3d_fortran.so")
 
-HIP_SOURCES = ["penta.hip", "step.hip", "capi_penta.cpp", "capi_step.cpp", "fileio.cpp"]
+HIP_SOURCES = ["penta.hip", "step.hip", "capi_penta.cpp", "capi_step.cpp", "capi_dropin.cpp", "fileio.cpp"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument", "-Wno-unused-result"]

@@ -12,6 +12,7 @@
 
 #include "../../include/lns3d_amd.h"
 #include "common.hpp"
+#include "ctx.hpp"
 #include "step.hpp"
 
 using namespace lns3d;
@@ -23,43 +24,6 @@ int fail(int code, const char *msg) {
 }
 }  // namespace
 
-struct lns3d_ctx {
-  lns3d_params p{};
-  Stencil st{};
-  Coeffs c{};
-  hipStream_t stream = nullptr;
-  PentaPlan px, py;          // xi lines (ny of length nx), eta lines (nx of length ny)
-  PentaPlan px32, py32;      // complex-float factors for the refined step
-  bool have32 = false;
-  void *work = nullptr;      // 5P complex
-  void *r = nullptr;         // 5P complex
-  void *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;   // 5P complex each (refinement)
-  double *scal = nullptr;    // device scalars: [0,8) time step, [8,2048) refinement norms and
-                             // partials, [2048,3072) step residual-norm partials
-  double *norm_out = nullptr;  // per-step residual norms (lns3d_ctx_set_residual_output)
-  int norm_count = 0;
-  long norm_idx = 0;
-  double *coef_block = nullptr;
-  double *sponge_block = nullptr;   // sp1, sp2, spl (P each), spf (10P), x and wamp staging
-  double *top_block = nullptr;      // ftop (10 nx), x top row and wamp staging (nx each)
-  double *side_block = nullptr;     // fside (20 ny), x left column staging (ny)
-  double *smooth_block = nullptr;   // eps_e buff (P)
-  double *wall_block = nullptr;     // fwall (10 nx)
-  std::vector<double> eig;          // left = 4 profiles (10 ny), kept for ispg = 4
-  double lalpha[2] = {0.0, 0.0};
-  bool coeffs_ready = false, factored = false;
-  bool vm_ready = false;            // the mean flow is on the device (lns3d_setup)
-  bool s0_set = false;              // Sutherland's constant given (mattyp = 1)
-  int impl = 1;                     // time scheme (src/impdrv3d.f90:179-199)
-  long lstep = 0;                   // steps taken with this scheme
-  void *vhist = nullptr;            // impl = 2: 5P complex, v at the start of the last step
-  int niter = 1;                    // corrector iterations per step (src/impdrv3d.f90:173)
-  void *vstart = nullptr;           // niter > 1: 5P complex, v at the start of this step
-  void *vstate = nullptr;    // 5P complex: the state of the host-array driver
-  bool state_fresh = false;  // uploaded, boundary conditions not yet applied
-  int last_iters = 0;
-  double last_resid = 0.0;
-};
 
 static int check_params(const lns3d_params &p) {
   if (p.nx < 7 || p.ny < 7) return fail(LNS3D_EINVAL, "nx and ny must be >= 7");
@@ -443,29 +407,30 @@ extern "C" int lns3d_set_inflow(lns3d_ctx *c, const double *x, double x0, const 
 // Coefficients in the reference's layouts (host): Ah,Bh,Dh,Dhi (5,5,nx,ny),
 // Vh11,Vh12,Vh22,ABhi (6,nx,ny), dtl (nx,ny) -- the arrays src/genmtrx.f90
 // and src/temporal.f90 leave in modules local2d/local3d.
+int lns3d::ctx_put_soa(lns3d_ctx *c, double *dst, const double *src, int ne, int stride) {
+  const long P = (long)c->p.nx * c->p.ny;
+  std::vector<double> soa((size_t)P * ne);
+  for (long q = 0; q < P; q++)
+    for (int e = 0; e < ne; e++) soa[(size_t)e * P + q] = src[q * stride + e];
+  LNS3D_CHECK(hipMemcpyAsync(dst, soa.data(), (size_t)ne * P * 8, hipMemcpyHostToDevice, c->stream));
+  LNS3D_CHECK(hipStreamSynchronize(c->stream));     // the host staging vector ends here
+  return 0;
+}
+
 extern "C" int lns3d_set_coeffs(lns3d_ctx *c, const double *Ah, const double *Bh, const double *Dh,
                                 const double *Dhi, const double *Vh11, const double *Vh12,
                                 const double *Vh22, const double *ABhi, const double *dtl) {
   if (!c || !Ah || !Bh || !Dh || !Dhi || !Vh11 || !Vh12 || !Vh22 || !ABhi || !dtl)
     return fail(LNS3D_EINVAL, "lns3d_set_coeffs: null argument");
-  const long P = (long)c->p.nx * c->p.ny;
-  std::vector<double> soa((size_t)P * 25);
-  auto put = [&](const double *src, int ne, int stride, double *dst) -> int {
-    for (long q = 0; q < P; q++)
-      for (int e = 0; e < ne; e++) soa[(size_t)e * P + q] = src[q * stride + e];
-    if (hipMemcpy(dst, soa.data(), (size_t)ne * P * 8, hipMemcpyHostToDevice) != hipSuccess)
-      return fail(LNS3D_EHIP, "hipMemcpy coefficients");
-    return 0;
-  };
-  int rc = put(Ah, 25, 25, c->c.Ah);
-  if (!rc) rc = put(Bh, 25, 25, c->c.Bh);
-  if (!rc) rc = put(Dh, 25, 25, c->c.Dh);
-  if (!rc) rc = put(Dhi, 25, 25, c->c.Dhi);
-  if (!rc) rc = put(Vh11, 6, 6, c->c.V11);
-  if (!rc) rc = put(Vh12, 6, 6, c->c.V12);
-  if (!rc) rc = put(Vh22, 6, 6, c->c.V22);
-  if (!rc) rc = put(ABhi, 4, 6, c->c.ABhi);
-  if (!rc) rc = put(dtl, 1, 1, c->c.dtl);
+  int rc = ctx_put_soa(c, c->c.Ah, Ah, 25, 25);
+  if (!rc) rc = ctx_put_soa(c, c->c.Bh, Bh, 25, 25);
+  if (!rc) rc = ctx_put_soa(c, c->c.Dh, Dh, 25, 25);
+  if (!rc) rc = ctx_put_soa(c, c->c.Dhi, Dhi, 25, 25);
+  if (!rc) rc = ctx_put_soa(c, c->c.V11, Vh11, 6, 6);
+  if (!rc) rc = ctx_put_soa(c, c->c.V12, Vh12, 6, 6);
+  if (!rc) rc = ctx_put_soa(c, c->c.V22, Vh22, 6, 6);
+  if (!rc) rc = ctx_put_soa(c, c->c.ABhi, ABhi, 4, 6);
+  if (!rc) rc = ctx_put_soa(c, c->c.dtl, dtl, 1, 1);
   if (rc) return rc;
   c->coeffs_ready = true;
   c->factored = false;

@@ -135,14 +135,21 @@ __device__ __forceinline__ V d2(int N, int per, int p, const double *d, double d
 }
 
 Stencil make_stencil(const lns3d_params &p) {
+  // src/gengrid.f90:36,44 (dxi = 1/(nx-1), deta = 1/(ny-1)); symmetric ends
+  // from left/right = 7 (src/input.f90:235-243, off for periodic x)
+  return make_stencil_spacing(p.optx, p.opty, 1.0 / (double)(p.nx - 1), 1.0 / (double)(p.ny - 1),
+                              p.xper ? 0 : ((p.left == 7 ? 1 : 0) | (p.right == 7 ? 2 : 0)));
+}
+
+Stencil make_stencil_spacing(int optx, int opty, double dxi, double deta, int sym) {
   Stencil s;
-  s.dxi = 1.0 / (double)(p.nx - 1);
-  s.deta = 1.0 / (double)(p.ny - 1);
+  s.dxi = dxi;
+  s.deta = deta;
   s.dxinv = 1.0 / s.dxi;
   s.dyinv = 1.0 / s.deta;
   s.dxsinv = 1.0 / (s.dxi * s.dxi);
   s.dysinv = 1.0 / (s.deta * s.deta);
-  s.sym = p.xper ? 0 : ((p.left == 7 ? 1 : 0) | (p.right == 7 ? 2 : 0));   // src/input.f90:235-243
+  s.sym = sym;
   auto first = [](int opt, double dinv, double *g) {
     double c;
     if (opt == 0) c = 1.0 / 60.0;
@@ -171,10 +178,10 @@ Stencil make_stencil(const lns3d_params &p) {
     d[3] = -2.0 * (a + b + c) * dsinv;
     d[4] = a * dsinv; d[5] = b * dsinv; d[6] = c * dsinv;
   };
-  first(p.optx, s.dxinv, s.gx);
-  first(p.opty, s.dyinv, s.gy);
-  second(p.optx, s.dxsinv, s.dx);
-  second(p.opty, s.dysinv, s.dy);
+  first(optx, s.dxinv, s.gx);
+  first(opty, s.dyinv, s.gy);
+  second(optx, s.dxsinv, s.dx);
+  second(opty, s.dysinv, s.dy);
   return s;
 }
 
@@ -1200,6 +1207,54 @@ __global__ __launch_bounds__(256) void k_forced_norm(const double2 *r, const dou
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// src/cgrad.f90 on a complex field v(5,nx,ny): g1 = d/dxi v, g2 = d/deta v,
+// one lane per (point, component), with the stencils of the residual
+__global__ void k_cgrad(int nx, int ny, int xper, int yper, Stencil st, const double2 *v, double2 *g1,
+                        double2 *g2) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 5L * nx * ny) return;
+  const int e = (int)(t % 5);
+  const long q = t / 5;
+  const int i = (int)(q % nx), j = (int)(q / nx);
+  const z a = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, v + 5L * nx * j + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
+  const z b = d1<double2, z>(ny, yper, j, st.gy, st.dyinv, v + 5L * i + e, 5L * nx);
+  g1[t] = make_double2(a.re, a.im);
+  g2[t] = make_double2(b.re, b.im);
+}
+
+// src/cgrad2.f90: g11 = d2/dxi2 v, g22 = d2/deta2 v and the cross
+// derivative g12 = d/deta of the caller's g1
+__global__ void k_cgrad2(int nx, int ny, int xper, int yper, Stencil st, const double2 *v, const double2 *g1,
+                         double2 *g11, double2 *g12, double2 *g22) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 5L * nx * ny) return;
+  const int e = (int)(t % 5);
+  const long q = t / 5;
+  const int i = (int)(q % nx), j = (int)(q / nx);
+  const z a = d2<double2, z>(nx, xper, i, st.dx, st.dxsinv, v + 5L * nx * j + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
+  const z b = d1<double2, z>(ny, yper, j, st.gy, st.dyinv, g1 + 5L * i + e, 5L * nx);
+  const z c = d2<double2, z>(ny, yper, j, st.dy, st.dysinv, v + 5L * i + e, 5L * nx);
+  g11[t] = make_double2(a.re, a.im);
+  g12[t] = make_double2(b.re, b.im);
+  g22[t] = make_double2(c.re, c.im);
+}
+
+int launch_cgrad(int nx, int ny, int xper, int yper, const Stencil &st, const void *v, void *g1, void *g2,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_cgrad, dim3(nblocks(5L * nx * ny, 256)), dim3(256), 0, s, nx, ny, xper, yper, st,
+                     (const double2 *)v, (double2 *)g1, (double2 *)g2);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_cgrad2(int nx, int ny, int xper, int yper, const Stencil &st, const void *v, const void *g1,
+                  void *g11, void *g12, void *g22, hipStream_t s) {
+  hipLaunchKernelGGL(k_cgrad2, dim3(nblocks(5L * nx * ny, 256)), dim3(256), 0, s, nx, ny, xper, yper, st,
+                     (const double2 *)v, (const double2 *)g1, (double2 *)g11, (double2 *)g12, (double2 *)g22);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
 }
 
 // sqrt of the sum of nb partials, in a fixed order (one block): the same

@@ -16,6 +16,14 @@ struct Stencil {
   int sym;               // symmetric xi ends (left/right = 7): bit 0 left, bit 1 right
 };
 Stencil make_stencil(const lns3d_params &p);
+// the same from explicit spacings and symmetric-end bits (bit 0 left, 1 right)
+Stencil make_stencil_spacing(int optx, int opty, double dxi, double deta, int sym);
+// src/cgrad.f90 and src/cgrad2.f90 on a device field v (5,nx,ny) complex:
+// g1, g2 first derivatives; g11, g12 (= d/deta of g1), g22 second
+int launch_cgrad(int nx, int ny, int xper, int yper, const Stencil &st, const void *v, void *g1, void *g2,
+                 hipStream_t s);
+int launch_cgrad2(int nx, int ny, int xper, int yper, const Stencil &st, const void *v, const void *g1,
+                  void *g11, void *g12, void *g22, hipStream_t s);
 
 // coefficients on the device, structure of arrays over the P = nx*ny points
 struct Coeffs {

@@ -4,7 +4,17 @@
 !
 ! 1. Drop-in externals with the reference's names and argument lists, so a
 !    Fortran driver links them in place of src/cpenta1bc.f, cpenta2bc.f,
-!    cpenta1p.f and cpenta2p.f:
+!    cpenta1p.f, cpenta2p.f, lrhs3d.f90, lhs1f3d.f90, lhs2f3d.f90, cgrad.f90
+!    and cgrad2.f90:
+!       call lrhs3D(vl, rl, vml, xl, yl)                   ! lrhs3d.f90:2
+!       call lhs1f3D(mat, Ah, Dh, Dhi, Vh11, ABhi, dtl, vl) ! lhs1f3d.f90:2
+!       call lhs2f3D(mat, Bh, Dh, Dhi, Vh22, ABhi, dtl, vl) ! lhs2f3d.f90:2
+!       call cgrad(ndof, nx, ny, v, g1v, g2v, dx, dy, optx, opty, xper, yper,
+!                  lsym, rsym, bsym, tsym, carp)           ! cgrad.f90:2-3
+!       call cgrad2(ndof, nx, ny, v, g1v, g11v, g12v, g22v, dx, dy, ...)
+!    The first three read the module state the host forwards once, after
+!    genmtrx and set_time_step, with c_set_module_state(st) (type
+!    lns3d_module_state below, one bind(C) struct).
 !       call cpenta1bc(nsys, n, nblk, mat, rhs, code)      ! cpenta1bc.f:2
 !       call cpenta2bc(n, nsys, nblk, mat, rhs, code)      ! cpenta2bc.f:2
 !       call cpenta1p (nsys, np, nblk, mat, rhs, per, per2, code)
@@ -40,6 +50,32 @@ module lns3d_hip
     integer(c_int) :: wall_is = 0, wall_ie = 0, idamp = 0, use_calcd = 0, update_lhs = 0
   end type lns3d_params
 
+  ! mirror of lns3d_module_state (include/lns3d_amd.h): the module state of
+  ! src/global.f90, local2d.f90 and local3d.f90 that lrhs3D, lhs1f3D and
+  ! lhs2f3D read, forwarded once by the host through lns3d_set_module_state
+  type, bind(C), public :: lns3d_module_state
+    integer(c_int) :: nx = 0, ny = 0, ndof = 5
+    real(c_double) :: alfa = 1.0_c_double, dxi = 0, deta = 0
+    integer(c_int) :: optx = 0, opty = 0, carp = 0
+    integer(c_int) :: xper = 0, yper = 0, lsym = 0, rsym = 0, bsym = 0, tsym = 0
+    integer(c_int) :: wall = 0, wallt = 0, top = 0, left = 0, right = 0
+    integer(c_int) :: is = 0, ie = 0, navier = 1
+    integer(c_int) :: linear = 2, mattyp = 0, loctime = 0, idamp = 0
+    real(c_double) :: Ma = 0, Re = 0, Pr = 0, gamma = 0, gamma1 = 0, kz = 0, omega = 0, x0 = 0, S0 = 0
+    real(c_double) :: cflmax = 0, dtmax = 0
+    integer(c_int) :: ispg = 0
+    type(c_ptr) :: spg = c_null_ptr, spg2 = c_null_ptr
+    real(c_double) :: eps_e = 0
+    type(c_ptr) :: buff = c_null_ptr
+    integer(c_int) :: use_calcd = 0, update_lhs = 0, istep = 0, iter = 1
+    type(c_ptr) :: wamp = c_null_ptr, vm = c_null_ptr, x = c_null_ptr, bnb = c_null_ptr, eig = c_null_ptr
+    real(c_double) :: lalpha(2) = 0
+    type(c_ptr) :: bump = c_null_ptr
+    type(c_ptr) :: Ah = c_null_ptr, Bh = c_null_ptr, Dh = c_null_ptr, Dhi = c_null_ptr
+    type(c_ptr) :: Vh11 = c_null_ptr, Vh12 = c_null_ptr, Vh22 = c_null_ptr, ABhi = c_null_ptr
+    type(c_ptr) :: dtl = c_null_ptr
+  end type lns3d_module_state
+
   integer(c_int), public :: lns3d_hip_last = 0
   type(c_ptr), save :: ctx = c_null_ptr
   real(c_double), save :: s0_given = -1.0_c_double   ! Sutherland's constant for mattyp = 1
@@ -52,6 +88,8 @@ module lns3d_hip
   public :: lns3d_hip_factor, lns3d_hip_scheme, lns3d_hip_sponge, lns3d_hip_top, lns3d_hip_inflow
   public :: lns3d_hip_smoother, lns3d_hip_wall, lns3d_hip_sutherland
   public :: c_cpenta1bc, c_cpenta2bc, c_cpenta1p, c_cpenta2p, c_set_periodic_exact
+  public :: c_set_module_state, c_drop_lrhs3d, c_drop_lhs1f3d, c_drop_lhs2f3d, c_drop_cgrad, c_drop_cgrad2
+  public :: lns3d_hip_flag
 
   interface
     function c_last_error() bind(C, name="lns3d_last_error") result(p)
@@ -155,6 +193,36 @@ module lns3d_hip
       type(c_ptr), value :: c, buff
       real(c_double), value :: eps_e
     end function
+    integer(c_int) function c_set_module_state(st) bind(C, name="lns3d_set_module_state")
+      import :: c_int, lns3d_module_state
+      type(lns3d_module_state), intent(in) :: st
+    end function
+    integer(c_int) function c_drop_lrhs3d(vl, rl, vml, xl, yl) bind(C, name="lns3d_drop_lrhs3d")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: vl, rl, vml, xl, yl
+    end function
+    integer(c_int) function c_drop_lhs1f3d(mat, ah, dh, dhi, vh11, abhi, dtl, vl) bind(C, name="lns3d_drop_lhs1f3d")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mat, ah, dh, dhi, vh11, abhi, dtl, vl
+    end function
+    integer(c_int) function c_drop_lhs2f3d(mat, bh, dh, dhi, vh22, abhi, dtl, vl) bind(C, name="lns3d_drop_lhs2f3d")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mat, bh, dh, dhi, vh22, abhi, dtl, vl
+    end function
+    integer(c_int) function c_drop_cgrad(ndof, nx, ny, v, g1v, g2v, dx, dy, optx, opty, xper, yper, &
+        lsym, rsym, bsym, tsym, carp) bind(C, name="lns3d_drop_cgrad")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: ndof, nx, ny, optx, opty, xper, yper, lsym, rsym, bsym, tsym, carp
+      type(c_ptr), value :: v, g1v, g2v
+      real(c_double), value :: dx, dy
+    end function
+    integer(c_int) function c_drop_cgrad2(ndof, nx, ny, v, g1v, g11v, g12v, g22v, dx, dy, optx, opty, &
+        xper, yper, lsym, rsym, bsym, tsym, carp) bind(C, name="lns3d_drop_cgrad2")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: ndof, nx, ny, optx, opty, xper, yper, lsym, rsym, bsym, tsym, carp
+      type(c_ptr), value :: v, g1v, g11v, g12v, g22v
+      real(c_double), value :: dx, dy
+    end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
@@ -163,6 +231,12 @@ module lns3d_hip
   end interface
 
 contains
+
+  ! a Fortran logical as the C ABI's 0/1
+  integer(c_int) function lns3d_hip_flag(l)
+    logical, intent(in) :: l
+    lns3d_hip_flag = merge(1_c_int, 0_c_int, l)
+  end function
 
   subroutine lns3d_hip_record(rc)
     integer(c_int), intent(in) :: rc
@@ -353,6 +427,72 @@ subroutine cpenta2p(np, nsys, nblk, mat, rhs, per, per2, code)
   call lns3d_hip_record(c_cpenta2p(int(np, c_int), int(nsys, c_int), int(nblk, c_int), &
                                    c_loc(mat), c_loc(rhs), c_loc(per), c_loc(per2), &
                                    int(code, c_int)))
+end subroutine
+
+!-----------------------------------------------------------------------------
+! RHS, LHS assembly and derivative drop-ins with the reference's argument
+! lists (lrhs3d.f90:2, lhs1f3d.f90:2, lhs2f3d.f90:2, cgrad.f90:2-3,
+! cgrad2.f90:2-3).  lrhs3D, lhs1f3D and lhs2f3D read the module state the
+! host forwarded with lns3d_set_module_state (c_set_module_state).
+!-----------------------------------------------------------------------------
+subroutine lrhs3D(vl, rl, vml, xl, yl)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  complex(c_double_complex), target :: vl(*), rl(*)
+  real(c_double), target :: vml(*), xl(*), yl(*)
+  call lns3d_hip_record(c_drop_lrhs3d(c_loc(vl), c_loc(rl), c_loc(vml), c_loc(xl), c_loc(yl)))
+end subroutine
+
+subroutine lhs1f3D(mat, Ah, Dh, Dhi, Vh11, ABhi, dtl, vl)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  complex(c_double_complex), target :: mat(*), vl(*)
+  real(c_double), target :: Ah(*), Dh(*), Dhi(*), Vh11(*), ABhi(*), dtl(*)
+  call lns3d_hip_record(c_drop_lhs1f3d(c_loc(mat), c_loc(Ah), c_loc(Dh), c_loc(Dhi), c_loc(Vh11), &
+                                       c_loc(ABhi), c_loc(dtl), c_loc(vl)))
+end subroutine
+
+subroutine lhs2f3D(mat, Bh, Dh, Dhi, Vh22, ABhi, dtl, vl)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  complex(c_double_complex), target :: mat(*), vl(*)
+  real(c_double), target :: Bh(*), Dh(*), Dhi(*), Vh22(*), ABhi(*), dtl(*)
+  call lns3d_hip_record(c_drop_lhs2f3d(c_loc(mat), c_loc(Bh), c_loc(Dh), c_loc(Dhi), c_loc(Vh22), &
+                                       c_loc(ABhi), c_loc(dtl), c_loc(vl)))
+end subroutine
+
+subroutine cgrad(ndof, nx, ny, v, g1v, g2v, dx, dy, optx, opty, xper, yper, lsym, rsym, bsym, tsym, carp)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  integer :: ndof, nx, ny, optx, opty
+  logical :: xper, yper, lsym, rsym, bsym, tsym, carp
+  complex(c_double_complex), target :: v(*), g1v(*), g2v(*)
+  real(c_double) :: dx, dy
+  call lns3d_hip_record(c_drop_cgrad(int(ndof, c_int), int(nx, c_int), int(ny, c_int), c_loc(v), c_loc(g1v), &
+                                     c_loc(g2v), dx, dy, int(optx, c_int), int(opty, c_int), &
+                                     lns3d_hip_flag(xper), lns3d_hip_flag(yper), lns3d_hip_flag(lsym), &
+                                     lns3d_hip_flag(rsym), lns3d_hip_flag(bsym), lns3d_hip_flag(tsym), &
+                                     lns3d_hip_flag(carp)))
+end subroutine
+
+subroutine cgrad2(ndof, nx, ny, v, g1v, g11v, g12v, g22v, dx, dy, optx, opty, xper, yper, lsym, rsym, &
+                  bsym, tsym, carp)
+  use lns3d_hip
+  use iso_c_binding
+  implicit none
+  integer :: ndof, nx, ny, optx, opty
+  logical :: xper, yper, lsym, rsym, bsym, tsym, carp
+  complex(c_double_complex), target :: v(*), g1v(*), g11v(*), g12v(*), g22v(*)
+  real(c_double) :: dx, dy
+  call lns3d_hip_record(c_drop_cgrad2(int(ndof, c_int), int(nx, c_int), int(ny, c_int), c_loc(v), c_loc(g1v), &
+                                      c_loc(g11v), c_loc(g12v), c_loc(g22v), dx, dy, int(optx, c_int), &
+                                      int(opty, c_int), lns3d_hip_flag(xper), lns3d_hip_flag(yper), &
+                                      lns3d_hip_flag(lsym), lns3d_hip_flag(rsym), lns3d_hip_flag(bsym), &
+                                      lns3d_hip_flag(tsym), lns3d_hip_flag(carp)))
 end subroutine
 
 ! status of the last drop-in call, callable from fixed-form code

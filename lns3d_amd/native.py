@@ -72,6 +72,12 @@ SIGNATURES = {
     "lns3d_forced_residual": ([VP, VP, VP, VP, VP, ctypes.POINTER(ctypes.c_double)], CI),
     "lns3d_residual_norm": ([VP, VP, VP], CI),
     "lns3d_ctx_set_residual_output": ([VP, VP, CI], CI),
+    "lns3d_set_module_state": ([VP], CI),
+    "lns3d_drop_lrhs3d": ([VP] * 5, CI),
+    "lns3d_drop_lhs1f3d": ([VP] * 8, CI),
+    "lns3d_drop_lhs2f3d": ([VP] * 8, CI),
+    "lns3d_drop_cgrad": ([CI, CI, CI, VP, VP, VP, CD, CD] + [CI] * 9, CI),
+    "lns3d_drop_cgrad2": ([CI, CI, CI, VP, VP, VP, VP, VP, CD, CD] + [CI] * 9, CI),
     "lns3d_read_header": ([ctypes.c_char_p, VP], CI),
     "lns3d_read_restart": ([ctypes.c_char_p, VP, VP, VP, ctypes.POINTER(CI)], CI),
     "lns3d_write_restart": ([ctypes.c_char_p, VP, VP, VP], CI),

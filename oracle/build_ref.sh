@@ -17,6 +17,8 @@
 #   to cpenta1bc_shipped_ / cpenta2bc_shipped_, so tests can show the defect.
 # * The main program (src/lns3d.f90) is left out; the harness calls routines
 #   directly.
+# * Second output, oracle/_ref/libref_hip.so: the harness relinked against the
+#   HIP library's Fortran drop-ins (below).
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 REF="${LNS3D_REFERENCE:-/root/reference}"
@@ -83,3 +85,26 @@ mkdir -p "$OUT"
 "$FC" -shared -fopenmp -o "$OUT/libref.so.tmp" *.o
 mv "$OUT/libref.so.tmp" "$OUT/libref.so"
 echo "built $OUT/libref.so"
+
+# The same harness with the reference's lrhs3D, lhs1f3D, lhs2f3D, cgrad,
+# cgrad2 and cpenta* objects left out, so that those names resolve to the
+# HIP library's Fortran drop-ins (lns3d_amd/liblns3d_fortran.so, whose module
+# file lns3d_hip.mod is in lns3d_amd/_build), plus the glue that forwards the
+# module state (refdrv_hip.f90): oracle/_ref/libref_hip.so.  Built only where
+# the HIP library has been built.
+PKG="$HERE/../lns3d_amd"
+if [ -f "$PKG/liblns3d_fortran.so" ] && [ -f "$PKG/_build/lns3d_hip.mod" ]; then
+  mkdir hip
+  "$FC" $F90FLAGS -I. -I"$PKG/_build" -c "$HERE/refdrv_hip.f90" -o hip/refdrv_hip.o
+  KEEP=""
+  for o in *.o; do
+    case "$o" in
+      lrhs3d.o|lhs1f3d.o|lhs2f3d.o|cgrad.o|cgrad2.o|cpenta1bc.o|cpenta2bc.o|cpenta1p.o|cpenta2p.o) ;;
+      *) KEEP="$KEEP $o" ;;
+    esac
+  done
+  "$FC" -shared -fopenmp -o "$OUT/libref_hip.so.tmp" $KEEP hip/refdrv_hip.o \
+      -L"$PKG" -llns3d_fortran -llns3d_amd -Wl,-rpath,'$ORIGIN/../../lns3d_amd'
+  mv "$OUT/libref_hip.so.tmp" "$OUT/libref_hip.so"
+  echo "built $OUT/libref_hip.so"
+fi

@@ -1,0 +1,78 @@
+!=============================================================================!
+! Glue of the relinked harness (oracle/_ref/libref_hip.so): the reference's
+! own routines with lrhs3D, lhs1f3D, lhs2f3D, cgrad, cgrad2 and cpenta*
+! resolved to the HIP library's drop-ins (lns3d_amd/liblns3d_fortran.so)
+! instead of the reference's objects.  ref_bind_hip forwards the reference's
+! module state (src/global.f90, local2d.f90, local3d.f90, buffer.f90,
+! bump.f90) through the one bind(C) setter, as a Fortran host linking the
+! drop-ins would after genmtrx and set_time_step.  Test infrastructure only.
+!=============================================================================!
+module refdrv_hip_addr
+  use iso_c_binding
+  implicit none
+contains
+  type(c_ptr) function raddr(a)
+    real(c_double), target, intent(in) :: a(*)
+    raddr = c_loc(a(1))
+  end function
+  type(c_ptr) function caddr(a)
+    complex(c_double_complex), target, intent(in) :: a(*)
+    caddr = c_loc(a(1))
+  end function
+end module refdrv_hip_addr
+
+subroutine ref_bind_hip(rc) bind(C, name="ref_bind_hip")
+  use iso_c_binding
+  use global
+  use local2d
+  use local3d
+  use buffer, only: buff
+  use bump, only: u1w, u2w, u3w, tw, twp
+  use refdrv_state, only: dtlh
+  use refdrv_hip_addr
+  use lns3d_hip, only: lns3d_module_state, c_set_module_state, lns3d_hip_flag
+  implicit none
+  integer(c_int), intent(out) :: rc
+  type(lns3d_module_state) :: st
+  real(c_double), allocatable, target, save :: eig(:,:)
+  complex(c_double_complex), allocatable, target, save :: uw(:,:)
+
+  st%nx = nx; st%ny = ny; st%ndof = ndof
+  st%alfa = alfa; st%dxi = dxi; st%deta = deta
+  st%optx = optx; st%opty = opty; st%carp = lns3d_hip_flag(carp)
+  st%xper = lns3d_hip_flag(xper); st%yper = lns3d_hip_flag(yper)
+  st%lsym = lns3d_hip_flag(lsym); st%rsym = lns3d_hip_flag(rsym)
+  st%bsym = lns3d_hip_flag(bsym); st%tsym = lns3d_hip_flag(tsym)
+  st%wall = wall; st%wallt = wallt; st%top = top; st%left = left; st%right = right
+  st%is = is; st%ie = ie; st%navier = lns3d_hip_flag(Navier)
+  st%linear = linear; st%mattyp = mattyp; st%loctime = loctime; st%idamp = idamp
+  st%Ma = Ma; st%Re = Re; st%Pr = Pr; st%gamma = gamma; st%gamma1 = gamma1
+  st%kz = kz; st%omega = omega; st%x0 = x0; st%S0 = S0; st%cflmax = cflmax; st%dtmax = dtmax
+  st%ispg = ispg
+  if (allocated(spg)) st%spg = raddr(spg)
+  if (allocated(spg2)) st%spg2 = raddr(spg2)
+  st%eps_e = eps_e
+  if (allocated(buff)) st%buff = raddr(buff)
+  st%use_calcd = lns3d_hip_flag(useCalcd); st%update_lhs = lns3d_hip_flag(updateLHS)
+  st%istep = istep; st%iter = iter
+  if (allocated(wamp)) st%wamp = raddr(wamp)
+  st%vm = raddr(vm); st%x = raddr(x); st%bnb = raddr(bnb)
+  if (allocated(rhor)) then
+    if (allocated(eig)) deallocate(eig)
+    allocate(eig(10, ny))
+    eig(1,:) = rhor; eig(2,:) = rhoi; eig(3,:) = ur; eig(4,:) = ui; eig(5,:) = vr
+    eig(6,:) = vi; eig(7,:) = wr; eig(8,:) = wi; eig(9,:) = tr; eig(10,:) = ti
+    st%eig = raddr(eig)
+  end if
+  st%lalpha(1) = real(lalpha); st%lalpha(2) = aimag(lalpha)
+  if (allocated(u1w)) then
+    if (allocated(uw)) deallocate(uw)
+    allocate(uw(nx, 5))
+    uw(:,1) = u1w; uw(:,2) = u2w; uw(:,3) = u3w; uw(:,4) = tw; uw(:,5) = twp
+    st%bump = caddr(uw)
+  end if
+  st%Ah = raddr(Ah); st%Bh = raddr(Bh); st%Dh = raddr(Dh); st%Dhi = raddr(Dhi)
+  st%Vh11 = raddr(Vh11); st%Vh12 = raddr(Vh12); st%Vh22 = raddr(Vh22); st%ABhi = raddr(ABhi)
+  st%dtl = raddr(dtlh)
+  rc = c_set_module_state(st)
+end subroutine ref_bind_hip

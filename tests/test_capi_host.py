@@ -183,3 +183,22 @@ def test_params_refused_before_any_device_call(lib, change, code, text):
     assert rc == code
     assert not out.value
     assert text in lib.lns3d_last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref_hip.so")),
+                    reason="relinked harness not built (reference sources absent)")
+def test_relinked_harness_takes_the_dropins():
+    """oracle/_ref/libref_hip.so defines none of the routines the drop-ins
+    replace: it imports those it calls, and liblns3d_fortran.so defines
+    every one"""
+    names = ["lrhs3d_", "lhs1f3d_", "lhs2f3d_", "cgrad_", "cgrad2_", "cpenta1bc_", "cpenta2bc_",
+             "cpenta1p_", "cpenta2p_"]
+    nm = lambda lib, opt: set(l.split()[-1] for l in subprocess.run(
+        ["nm", "-D", opt, lib], capture_output=True, text=True, check=True).stdout.splitlines())
+    hip = os.path.join(ROOT, "oracle", "_ref", "libref_hip.so")
+    flib = os.path.join(ROOT, "lns3d_amd", "liblns3d_fortran.so")
+    assert not set(names) & nm(hip, "--defined-only")
+    # every one the reference's driver calls (cpenta2bc is not among them:
+    # its ImpDrv3D calls cpenta1bc for both sweeps, src/impdrv3d.f90:35-43)
+    assert set(names) - {"cpenta2bc_"} <= nm(hip, "--undefined-only")
+    assert set(names) <= nm(flib, "--defined-only")
