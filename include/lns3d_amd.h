@@ -286,6 +286,19 @@ int lns3d_forced_residual(lns3d_ctx *c, const void *v_dev, const void *vold_dev,
  * cancels in the ratio to the first step's that it reports).  Written to
  * *norm_dev (device) with no host synchronisation; the sum's order is fixed. */
 int lns3d_residual_norm(lns3d_ctx *c, const void *r_dev, double *norm_dev);
+/* lns3d_residual_norm without a context and without allocating: for
+ * callers that cannot call hipMalloc (code under HIP graph capture, serving
+ * loops that forbid allocation).  No counterpart in the reference.  r_dev
+ * holds npts x 5 complex (the step's residual), dtl_dev npts doubles, the
+ * norm goes to *norm_dev (device); the current device's stream `stream`.
+ * The per-block partial sums live in a static device array of 1024 doubles
+ * (8 KiB) that each GPU holds until the process ends; every call on one GPU
+ * shares it, so calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller.  Sizes: 1 <= npts <=
+ * LNS3D_NOALLOC_MAX_POINTS, else LNS3D_EINVAL. */
+#define LNS3D_NOALLOC_MAX_POINTS (1L << 28)
+int lns3d_residual_norm_noalloc(const void *r_dev, const double *dtl_dev, long npts, double *norm_dev,
+                                void *stream);
 /* record the residual norm of every step: lns3d_step (first corrector),
  * lns3d_step_fp32 and lns3d_step_refined_fixed write the norm of the step's
  * residual (before the line solves, as src/impdrv3d.f90:207 calls

@@ -580,6 +580,14 @@ extern "C" int lns3d_residual_norm(lns3d_ctx *c, const void *r_dev, double *norm
   return launch_residual_norm(c->c.dtl, r_dev, (long)c->p.nx * c->p.ny, c->scal + 2048, norm_dev, c->stream);
 }
 
+extern "C" int lns3d_residual_norm_noalloc(const void *r_dev, const double *dtl_dev, long npts, double *norm_dev,
+                                           void *stream) {
+  if (!r_dev || !dtl_dev || !norm_dev) return fail(LNS3D_EINVAL, "lns3d_residual_norm_noalloc: null argument");
+  if (npts < 1 || npts > LNS3D_NOALLOC_MAX_POINTS)
+    return fail(LNS3D_EINVAL, "lns3d_residual_norm_noalloc: npts must be 1..LNS3D_NOALLOC_MAX_POINTS (2^28)");
+  return launch_residual_norm_noalloc(dtl_dev, r_dev, npts, norm_dev, (hipStream_t)stream);
+}
+
 extern "C" int lns3d_ctx_set_residual_output(lns3d_ctx *c, double *norms_dev, int count) {
   if (!c || (norms_dev && count < 1)) return fail(LNS3D_EINVAL, "lns3d_ctx_set_residual_output: bad arguments");
   c->norm_out = norms_dev;

@@ -11,6 +11,7 @@
 // One thread per grid point.  Coefficients are structure-of-arrays ([e][P])
 // so a wave's loads of one coefficient are contiguous.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 #include "step.hpp"
@@ -1278,6 +1279,35 @@ int launch_residual_norm(const double *dtl, const void *r, long P, double *part,
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, s, (const double *)part, blocks, out);
   LNS3D_CHECK(hipGetLastError());
   return 0;
+}
+
+// The per-block partials of the allocation-free residual norm: one static
+// array per GPU (NORM_BLOCKS doubles, 8 KiB), held until the process ends.
+// Host code never passes this name to a kernel (on the host it is only a
+// stand-in for the symbol): its device address is looked up once per device
+// with hipGetSymbolAddress and kept in scratch_dev.
+static __device__ __align__(256) double scratch[NORM_BLOCKS];
+static std::mutex scratch_mu;
+static double *scratch_dev[MAX_DEVICES] = {};
+
+int launch_residual_norm_noalloc(const double *dtl, const void *r, long P, double *out, hipStream_t s) {
+  int dev = -1;
+  LNS3D_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return set_error(__FILE__, __LINE__, "device index beyond MAX_DEVICES");
+  double *part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+      void *a = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !a) return set_error(__FILE__, __LINE__, "hipGetSymbolAddress(scratch) failed");
+      scratch_dev[dev] = (double *)a;
+    }
+    part = scratch_dev[dev];
+  }
+  // every partial the final sum reads is written first by k_forced_norm:
+  // nothing needs clearing between calls
+  return launch_residual_norm(dtl, r, P, part, out, s);
 }
 
 int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,

@@ -86,6 +86,10 @@ int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const 
 // per-block partials in part (NORM_BLOCKS doubles); no host synchronisation
 constexpr int NORM_BLOCKS = 1024;
 int launch_residual_norm(const double *dtl, const void *r, long P, double *part, double *out, hipStream_t s);
+// the same with the partials in the library's static device array (no
+// allocation, calls on one GPU share it); on the current device
+constexpr int MAX_DEVICES = 64;
+int launch_residual_norm_noalloc(const double *dtl, const void *r, long P, double *out, hipStream_t s);
 // sum over the points of |r|^2 / dtl^2, per block into part (at most
 // NORM_BLOCKS doubles); the caller adds them
 int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,

@@ -72,6 +72,7 @@ SIGNATURES = {
     "lns3d_forced_residual": ([VP, VP, VP, VP, VP, ctypes.POINTER(ctypes.c_double)], CI),
     "lns3d_residual_norm": ([VP, VP, VP], CI),
     "lns3d_ctx_set_residual_output": ([VP, VP, CI], CI),
+    "lns3d_residual_norm_noalloc": ([VP, VP, ctypes.c_long, VP, VP], CI),
     "lns3d_set_module_state": ([VP], CI),
     "lns3d_drop_lrhs3d": ([VP] * 5, CI),
     "lns3d_drop_lhs1f3d": ([VP] * 8, CI),
