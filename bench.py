@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep leg")
@@ -297,11 +297,43 @@ def cpu_baseline(p, met, vm, v0, nsteps):
     return out
 
 
-def _cpu_baseline(p, met, vm, v0, nsteps):
-    """The reference's Fortran step (oracle/_ref/libref.so, OpenMP threads on
-    the host) or, where that library was not built, the C restatement (one
-    thread).  The LU is formed once outside the timed region, as in the
-    reference; the sample is nsteps full steps at the benchmark size."""
+def host_description():
+    """CPU model, logical CPUs and the NUMA layout of the host"""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    numa = {}
+    base = "/sys/devices/system/node"
+    try:
+        for d in sorted(os.listdir(base)):
+            if d.startswith("node") and d[4:].isdigit():
+                with open(os.path.join(base, d, "cpulist")) as f:
+                    numa[d] = f.read().strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "numa_nodes": numa,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def _cpu_baseline(p, met, vm, v0, nsteps, warm=3):
+    """The reference's Fortran step (oracle/_ref/libref.so: its own lrhs3D,
+    rhsBC3D, lhs*, cpenta* and itrBC3D compiled with flang -O2 -fopenmp,
+    the transposes of its driver in parallel as src/impdrv3d.f90:257-304) on
+    the host: OMP_NUM_THREADS threads (OMP_PROC_BIND=close, OMP_PLACES=cores
+    unless set) and one thread.  The first step forms and factors the LHS
+    (timed apart); then `warm` steps, then `nsteps` steps timed one by one:
+    median and p90, and the median of each phase.  Where the library was not
+    built, the C restatement on one thread."""
+    import ctypes
+    # thread placement of the reference's OpenMP runtime, read when it starts
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle.oracle import Oracle, Reference
     from lns3d_amd.config import make_grid
     x, y, met2 = make_grid(p)
@@ -311,29 +343,58 @@ def _cpu_baseline(p, met, vm, v0, nsteps):
         R.setup(p, x, y, met2, vm)
         R.genmtrx()
         R.set_time_step()
-        v = np.asfortranarray(v0.copy())
-        R.itrbc3d(v)
-        R.step(v, True)                        # forms and factors the LHS
-        t0 = time.perf_counter()
-        for _ in range(nsteps):
-            R.step(v, False)
-        dt = time.perf_counter() - t0
-        return {"value": round(nsteps / dt, 4), "unit": "steps/s", "cores": cores,
-                "kind": "reference",
-                "sample": f"{nsteps} implicit steps at {p.nx}x{p.ny} after one factoring step "
-                          "(flang -O2 -fopenmp build of the reference sources)"}
+        ph = (ctypes.c_double * 5)()
+        res = {}
+        for nthr in (cores, 1):
+            R.lib.ref_set_threads(ctypes.c_int(nthr))
+            v = np.asfortranarray(v0.copy())
+            R.itrbc3d(v)
+            t0 = time.perf_counter()
+            R.step(v, True)                    # forms and factors the LHS
+            t_first = time.perf_counter() - t0
+            R.lib.ref_phase_times(ph)
+            factor_s = ph[4]
+            for _ in range(warm):
+                R.step(v, False)
+            ts, phases = [], []
+            n = nsteps if nthr == cores else max(5, nsteps // 4)
+            for _ in range(n):
+                t0 = time.perf_counter()
+                R.step(v, False)
+                ts.append(time.perf_counter() - t0)
+                R.lib.ref_phase_times(ph)
+                phases.append(list(ph)[:4])
+            med = float(np.median(ts))
+            pm = np.median(np.array(phases), axis=0)
+            res[nthr] = {"steps_s": round(1.0 / med, 4), "median_ms": round(1e3 * med, 3),
+                         "p90_ms": round(1e3 * float(np.percentile(ts, 90)), 3), "steps_timed": n,
+                         "phases_ms": {"rhs": round(1e3 * pm[0], 3), "xi_solve": round(1e3 * pm[1], 3),
+                                       "eta_solve": round(1e3 * pm[2], 3), "update": round(1e3 * pm[3], 3)},
+                         "factor_ms": round(1e3 * factor_s, 2), "first_step_ms": round(1e3 * t_first, 2)}
+        R.lib.ref_set_threads(ctypes.c_int(cores))
+        top = res[cores]
+        return {"value": top["steps_s"], "unit": "steps/s", "cores": cores, "kind": "reference",
+                "sample": f"{warm} warm-up then {top['steps_timed']} implicit steps at {p.nx}x{p.ny}, each "
+                          "timed, median (flang -O2 -fopenmp build of the reference sources)",
+                "median_ms": top["median_ms"], "p90_ms": top["p90_ms"], "phases_ms": top["phases_ms"],
+                "factor_ms": top["factor_ms"], "one_thread": res[1],
+                "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+                "host": host_description()}
     O = Oracle()
     c = O.genmtrx(p, vm, met)
     dtl = O.set_time_step(p, vm, met)
     v = np.asfortranarray(v0.copy())
     O.itrbc3d(p, v)
     f = O.factor(p, c, dtl)
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
+    ts = []
+    for _ in range(max(3, nsteps // 4)):
+        t0 = time.perf_counter()
         O.step(p, c, dtl, f, v)
-    dt = time.perf_counter() - t0
-    return {"value": round(nsteps / dt, 4), "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{nsteps} implicit steps at {p.nx}x{p.ny} (C restatement, one thread)"}
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return {"value": round(1.0 / med, 4), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{len(ts)} implicit steps at {p.nx}x{p.ny} (C restatement, one thread), median",
+            "host": host_description()}
 
 
 if __name__ == "__main__":

@@ -20,6 +20,10 @@ module refdrv_state
   ! forcing of the shift-invert driver's steps (src/si_eigdrv3d_new.f90:325-
   ! 358 with ebe3D, src/si_eigdrv3d.f90:319-345); unallocated: none
   complex, allocatable :: fsi(:,:,:)
+  ! wall-clock seconds of the phases of the last ref_step: 1 residual
+  ! (lrhs3D, rbe3D, rhsBC3D), 2 xi solve with its transposes, 3 eta solve,
+  ! 4 update and itrBC3D, 5 forming and factoring the LHS (first step)
+  real(c_double) :: phase_t(5) = 0
 end module refdrv_state
 
 !-----------------------------------------------------------------------------
@@ -324,6 +328,7 @@ end subroutine ref_lhs
 ! solution increment r that was added.
 subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   use iso_c_binding
+  use omp_lib, only: omp_get_wtime
   use global
   use local2d
   use local3d
@@ -335,7 +340,9 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   complex, allocatable :: vold(:,:,:), r(:,:,:)
   integer :: i, j, idof, jdof, it
   logical :: form
+  real(c_double) :: tc0, tc1
 
+  phase_t = 0
   allocate(vold(ndof,nx,ny), r(ndof,nx,ny))
   if (first /= 0) lstep_h = 0
   ! src/impdrv3d.f90:134-153: the time levels of impl = 2
@@ -368,6 +375,7 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   do it = 1, niter
   iter = it
   if (it > 1) form = .false.
+  tc0 = omp_get_wtime()
   ! src/impdrv3d.f90:179-199
   if (impl == 3) then
     call lrhs3D(pt5*(v+vold), r, vm, x, y)
@@ -394,12 +402,15 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
     alfa = one
   end if
   call rhsBC3D(r, v, vm)
+  tc1 = omp_get_wtime(); phase_t(1) = phase_t(1) + (tc1 - tc0); tc0 = tc1
 
   if (form) then
     call lhs1f3D(matx, Ah, Dh, Dhi, Vh11, ABhi, dtlh, v)
     if (xper) then
       call cpenta2p(nx, ny, ndof, matx, r, perx, per2x, 0)
     else
+      ! the transposes run in parallel, as src/impdrv3d.f90:257-304 does
+      !$omp parallel do private(i, idof, jdof)
       do j = 1, ny
         do i = 1, nx
           do idof = 1, ndof
@@ -411,22 +422,26 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
       end do
       call cpenta1bc(ny, nx, ndof, lmatx, lr, 0)
     end if
+    tc1 = omp_get_wtime(); phase_t(5) = phase_t(5) + (tc1 - tc0); tc0 = tc1
   end if
   if (xper) then
     call cpenta2p(nx, ny, ndof, matx, r, perx, per2x, 1)
   else
+    !$omp parallel do private(i)
     do j = 1, ny
       do i = 1, nx
         lr(:,j,i) = r(:,i,j)
       end do
     end do
     call cpenta1bc(ny, nx, ndof, lmatx, lr, 1)
+    !$omp parallel do private(i)
     do j = 1, ny
       do i = 1, nx
         r(:,i,j) = lr(:,j,i)
       end do
     end do
   end if
+  tc1 = omp_get_wtime(); phase_t(2) = phase_t(2) + (tc1 - tc0); tc0 = tc1
 
   if (form) then
     call lhs2f3D(maty, Bh, Dh, Dhi, Vh22, ABhi, dtlh, v)
@@ -435,20 +450,47 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
     else
       call cpenta1bc(nx, ny, ndof, maty, r, 0)
     end if
+    tc1 = omp_get_wtime(); phase_t(5) = phase_t(5) + (tc1 - tc0); tc0 = tc1
   end if
   if (yper) then
     call cpenta1p(nx, ny, ndof, maty, r, pery, per2y, 1)
   else
     call cpenta1bc(nx, ny, ndof, maty, r, 1)
   end if
+  tc1 = omp_get_wtime(); phase_t(3) = phase_t(3) + (tc1 - tc0); tc0 = tc1
 
-  v = v + r
-  dv = r
+  ! the update in parallel, as src/impdrv3d.f90:340-348
+  !$omp parallel do private(i)
+  do j = 1, ny
+    do i = 1, nx
+      v(:,i,j) = v(:,i,j) + r(:,i,j)
+      dv(:,i,j) = r(:,i,j)
+    end do
+  end do
   call itrBC3D(v, vm)
+  tc1 = omp_get_wtime(); phase_t(4) = phase_t(4) + (tc1 - tc0); tc0 = tc1
   end do
   lstep_h = lstep_h + 1
   deallocate(vold, r)
 end subroutine ref_step
+
+! the phase times of the last ref_step (seconds; see phase_t)
+subroutine ref_phase_times(t) bind(C, name="ref_phase_times")
+  use iso_c_binding
+  use refdrv_state
+  implicit none
+  real(c_double), intent(out) :: t(5)
+  t = phase_t
+end subroutine ref_phase_times
+
+! OpenMP threads of the harness's parallel regions
+subroutine ref_set_threads(n) bind(C, name="ref_set_threads")
+  use iso_c_binding
+  use omp_lib, only: omp_set_num_threads
+  implicit none
+  integer(c_int), value :: n
+  call omp_set_num_threads(n)
+end subroutine ref_set_threads
 
 ! the forcing f of the shift-invert driver's pseudo-time steps (set: the
 ! steps use ebe3D; clear: rbe3D)
