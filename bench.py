@@ -245,7 +245,7 @@ def main():
         sw = sweep(count=nreq, seed=0, mode="fp32_outer")
         rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
         out["sweep"] = {"requests": nreq,
-                        "request": "receptivity (ispg 2 sponge forcing, loctime 1, CFL 80) to 1e-9 of "
+                        "request": "receptivity (ispg 2 sponge forcing, loctime 1, CFL 20 per 128 cells in x) to 1e-9 of "
                                    "the first residual; grids 129x65..1025x257, Re U[50,150]",
                         "mode": "FP32 factors, FP64 residual each step (outer refinement)",
                         "peak": rnd(sw["peak"]), "arrival_rate_s": round(sw["rate"], 3),

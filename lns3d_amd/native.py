@@ -108,6 +108,14 @@ def lib():
     """Load the native library (once)."""
     global _lib
     if _lib is None:
+        # torch carries its own HIP runtime beside ROCm's libamdhip64.so.7,
+        # which this library links; a process that loaded this library first
+        # and torch's runtime after it saw the library's hipMalloc fail, while
+        # the other order works.  Load torch first where it is installed.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise LNS3DError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
                              "or python lns3d_amd/build.py")

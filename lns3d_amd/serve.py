@@ -6,7 +6,7 @@ omega enters through the inflow sponge (ispg = 2: the outflow sponge spg and
 the inflow sponge spg2 toward the incoming wave, src/lrhs3d.f90:222-282,
 profiles as src/sponge.f90:20 forms them), and the linearized equations are
 marched in pseudo-time with local time steps (loctime = 1,
-src/temporal.f90:145-160, CFL 80) until the residual norm the reference prints per
+src/temporal.f90:145-160, CFL 20 per 128 cells along x) until the residual norm the reference prints per
 step (src/resstat3d.f90:61-89, totres) has fallen to `tol` (1e-9) times its
 value at the first step, or `max_steps` have been taken.  The reference has
 no stopping test; this one is the ratio it reports as jtotrs.
@@ -51,11 +51,15 @@ from .config import Params, disturbance_envelope, make_disturbance_device, make_
 from .native import Graph, Step, ctx_bytes, LNS3DError, MODE_FP64, MODE_FP32, MODE_REFINED
 
 GRIDS = ((129, 65), (257, 129), (513, 257), (1025, 257))
-# pseudo-time CFL of a request (local time steps): measured steps to 1e-9 at
-# beta 0.5, omega 0.1, Ma 0.5, Re 100 -- 257x129: 4559 (CFL 20), 4919 (80),
-# over 20000 (320); 513x257: over 20000, 9027, 15193; 1025x257: over 20000,
-# 9212, 8479
-CFL = 80.0
+# Pseudo-time CFL of a request (local time steps): 20 per 128 cells along x.
+# Steps to 1e-9 measured for three drawn requests per grid (sum over the
+# three): 129x65 -- CFL 10: 6999, 20: 5322, 40: 9081; 257x129 -- 20: 21433,
+# 40: 12886, 80: 14034; 513x257 -- 40: 50995, 80: 30285, 160: 25464;
+# 1025x257 -- 80: 33284, 160: 21923, 320: 31435.
+def request_cfl(nx: int) -> float:
+    return 20.0 * (nx - 1) / 128.0
+
+
 # parameters that reach kernels by value rather than through the device
 # coefficients (the physics: beta, omega, Mach, Re, ... does not)
 STRUCTURAL = ("nx", "ny", "optx", "opty", "xper", "yper", "wall", "wallt", "top", "left", "right",
@@ -79,7 +83,7 @@ class Request:
 
     def params(self) -> Params:
         return Params(nx=self.nx, ny=self.ny, kz=self.beta, omega=self.omega, Ma=self.Ma, Re=self.Re,
-                      loctime=1, cflmax=CFL)
+                      loctime=1, cflmax=request_cfl(self.nx))
 
 
 def sponge_profiles(x):
