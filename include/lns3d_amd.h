@@ -66,11 +66,20 @@ typedef struct {
   int update_lhs;   /* updateLHS (src/global.f90:33): factors formed again
                      * every iteration (src/impdrv3d.f90:248,310) */
 } lns3d_params;
-/* Not implemented, rejected with LNS3D_EUNSUPPORTED: a wall extent other
- * than 1:nx, idamp != 0, use_calcd, update_lhs, and Re = 0 with mattyp = 0
- * (the reference's inviscid deck, src/input.f90:31-36: Navier = .false.).
- * Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, and Re = 0 with
- * mattyp != 0 are rejected with LNS3D_EINVAL. */
+/* Re = 0 with mattyp = 0 is the reference's inviscid deck (src/input.f90
+ * :31-36: Navier = .false., Re = 1e99): the wall row holds the rotated
+ * momentum rows of src/rhsbc3d.f90, src/lhsbt1f3d.f90 and src/lhsbt2f3d.f90
+ * with the wall normals lns3d_setup forms from the metric (src/gengrid.f90
+ * :133-137), itrBC3D's tangency condition, and bcfix after the eta solve
+ * (src/impdrv3d.f90:333-336).  update_lhs forms both factors again each
+ * iteration, as the build with USE_CPENTA1BC does (src/impdrv3d.f90:248,310);
+ * the factors change only with useCalcd's D-term placement, so they are
+ * formed again only then.
+ * Not implemented, rejected with LNS3D_EUNSUPPORTED: idamp != 0, use_calcd
+ * with left or right = 7, and precision = 1 with the inviscid deck or
+ * update_lhs.  Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, Re = 0
+ * with mattyp != 0 and a wall extent outside 1 <= is <= ie <= nx are
+ * rejected with LNS3D_EINVAL. */
 
 /* ======================================================================
  * Block-pentadiagonal line solvers, drop-in form (host arrays).
@@ -126,7 +135,11 @@ typedef struct {
   const double *spg, *spg2;               /* (nx,ny); NULL when ispg does not use them */
   double eps_e;                           /* smoother and its damping profile */
   const double *buff;                     /* (nx,ny); NULL when eps_e = 0 */
-  int use_calcd, update_lhs, istep, iter; /* factor control */
+  int use_calcd, update_lhs;              /* factor control */
+  const int *istep, *iter;                /* the caller's step and iteration
+                                           * counters (src/global.f90), read at
+                                           * each lhs1f3D / lhs2f3D call for
+                                           * useCalcd; NULL reads as 1 */
   const double *wamp;                     /* (nx) sponge / top wave amplitude; NULL: 1 */
   const double *vm;                       /* (5,nx,ny) mean flow */
   const double *x;                        /* (nx,ny) physical x */

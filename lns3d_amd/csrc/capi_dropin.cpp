@@ -29,6 +29,8 @@ void *g_mat = nullptr;             // assembled LHS staging (125 P complex)
 size_t g_mat_bytes = 0;
 void *g_buf = nullptr;             // field staging for cgrad / cgrad2 (5 arrays of 5 P complex)
 size_t g_buf_bytes = 0;
+const int *g_istep = nullptr;      // the caller's step and iteration counters (useCalcd)
+const int *g_iter = nullptr;
 
 int grow(void **p, size_t *have, size_t need) {
   if (*have >= need) return 0;
@@ -54,6 +56,8 @@ extern "C" int lns3d_set_module_state(const lns3d_module_state *m) {
   if (m->lsym != (!m->xper && m->left == 7) || m->rsym != (!m->xper && m->right == 7))
     return fail(LNS3D_EINVAL, "lns3d_set_module_state: lsym/rsym disagree with left/right = 7 (src/input.f90:235-243)");
   if (!close_to(m->gamma1, m->gamma - 1.0)) return fail(LNS3D_EINVAL, "lns3d_set_module_state: gamma1 != gamma - 1");
+  if (!m->navier && (m->mattyp != 0 || !m->bnb))
+    return fail(LNS3D_EINVAL, "lns3d_set_module_state: Navier = .false. comes with mattyp = 0 and the wall normals bnb");
   if (!m->vm || !m->Ah || !m->Bh || !m->Dh || !m->Dhi || !m->Vh11 || !m->Vh12 || !m->Vh22 || !m->ABhi || !m->dtl)
     return fail(LNS3D_EINVAL, "lns3d_set_module_state: the mean flow and the coefficient arrays are required");
   lns3d_params p{};
@@ -94,6 +98,14 @@ extern "C" int lns3d_set_module_state(const lns3d_module_state *m) {
       (rc = lns3d_set_inflow(c, m->x, m->x0, m->eig, m->lalpha)))
     return rc;
   if (!p.yper && p.wall == 3 && (rc = lns3d_set_wall(c, m->bump))) return rc;
+  if (!c->navier) {
+    // bnb(nx,2) is the [2][nx] layout the inviscid rows read
+    LNS3D_CHECK(hipMemcpyAsync(c->bnb_block, m->bnb, (size_t)2 * p.nx * 8, hipMemcpyHostToDevice, c->stream));
+    LNS3D_CHECK(hipStreamSynchronize(c->stream));
+    c->bnb_ready = true;
+  }
+  g_istep = m->istep;
+  g_iter = m->iter;
   return 0;
 }
 
@@ -140,6 +152,10 @@ static int drop_lhs(int dir, void *mat, const double *AB, const double *Dh, cons
   if (!rc) rc = ctx_put_soa(c, c->c.dtl, dtl, 1, 1);
   if (rc) return rc;
   c->factored = false;
+  // useCalcd (src/lhs1f3d.f90:27-37, lhs2f3d.f90:27-37): the xi factor
+  // holds the D terms when istep - 1 + iter is even, the eta factor otherwise
+  const int istep = g_istep ? *g_istep : 1, iter = g_iter ? *g_iter : 1;
+  c->c.calcd_xi = c->p.use_calcd ? (int)((istep - 1 + iter) % 2 == 0) : 1;
   const size_t bytes = 125ul * c->p.nx * c->p.ny * 16;
   if ((rc = grow(&g_mat, &g_mat_bytes, bytes))) return rc;
   rc = lns3d_assemble(c, dir, g_mat);

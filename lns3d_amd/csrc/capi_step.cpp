@@ -34,17 +34,17 @@ static int check_params(const lns3d_params &p) {
   // Re = 0 is the reference's inviscid deck for mattyp = 0 (src/input.f90:31-36)
   if (p.Re == 0.0 && p.mattyp != 0)
     return fail(LNS3D_EINVAL, "Re = 0 (inviscid) is defined for mattyp = 0 only (src/input.f90:31-36)");
-  if (p.Re == 0.0)
-    return fail(LNS3D_EUNSUPPORTED, "Re = 0: the inviscid wall rows (Navier = .false.) are not implemented");
   if (p.wall_is < 0 || p.wall_ie < 0 || p.wall_is > p.nx || p.wall_ie > p.nx ||
-      (p.wall_is && p.wall_ie && p.wall_is > p.wall_ie))
+      ((p.wall_is ? p.wall_is : 1) > (p.wall_ie ? p.wall_ie : p.nx)))
     return fail(LNS3D_EINVAL, "wall extent: 1 <= wall_is <= wall_ie <= nx (0 for the default)");
-  if ((p.wall_is > 1 || (p.wall_ie != 0 && p.wall_ie != p.nx)))
-    return fail(LNS3D_EUNSUPPORTED, "a wall extent other than 1:nx (&extras is, ie) is not implemented");
   if (p.idamp != 0)
     return fail(LNS3D_EUNSUPPORTED, "idamp != 0 (the viscous damper, src/damper.f90) is not implemented");
-  if (p.use_calcd || p.update_lhs)
-    return fail(LNS3D_EUNSUPPORTED, "useCalcd and updateLHS are not implemented");
+  if ((p.use_calcd != 0 && p.use_calcd != 1) || (p.update_lhs != 0 && p.update_lhs != 1))
+    return fail(LNS3D_EINVAL, "use_calcd and update_lhs are 0 or 1");
+  if (p.use_calcd && !p.xper && (p.left == 7 || p.right == 7))
+    return fail(LNS3D_EUNSUPPORTED, "useCalcd with symmetric ends (left/right = 7) is not implemented");
+  if (p.precision == 1 && (p.Re == 0.0 || p.update_lhs))
+    return fail(LNS3D_EUNSUPPORTED, "complex-float factors with the inviscid deck (Re = 0) or updateLHS are not implemented");
   if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
       (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8) || p.linear != 2 ||
@@ -71,12 +71,31 @@ static int check_params(const lns3d_params &p) {
   return 0;
 }
 
+// the deck into the context: the inviscid deck holds Re = 1e99 (src/input.f90
+// :31-36); the wall extent and the wall normals of its rows
+static int apply_params(lns3d_ctx *c, const lns3d_params &p) {
+  c->p = p;
+  c->navier = !(p.Re == 0.0 && p.mattyp == 0);
+  if (!c->navier) c->p.Re = 1.0e99;
+  c->c.wlo = p.wall_is ? p.wall_is - 1 : 0;
+  c->c.whi = p.wall_ie ? p.wall_ie - 1 : p.nx - 1;
+  if (!c->navier && !c->bnb_block) LNS3D_CHECK(hipMalloc(&c->bnb_block, (size_t)2 * p.nx * sizeof(double)));
+  c->c.bnb = c->navier ? nullptr : c->bnb_block;
+  c->bnb_ready = false;
+  c->c.calcd_xi = 1;
+  c->cur_iter = 1;
+  return 0;
+}
+
 extern "C" int lns3d_ctx_create(const lns3d_params *p, lns3d_ctx **out) {
   if (!p || !out) return fail(LNS3D_EINVAL, "lns3d_ctx_create: null argument");
   int rc = check_params(*p);
   if (rc) return rc;
   auto *c = new lns3d_ctx;
-  c->p = *p;
+  if ((rc = apply_params(c, *p))) {
+    delete c;
+    return rc;
+  }
   c->st = make_stencil(*p);
   const long P = (long)p->nx * p->ny;
   size_t ncoef = (size_t)P * (25 * 4 + 6 * 3 + 4 + 1 + 5 + 10);
@@ -127,6 +146,7 @@ extern "C" int lns3d_ctx_destroy(lns3d_ctx *c) {
   if (c->side_block) (void)hipFree(c->side_block);
   if (c->smooth_block) (void)hipFree(c->smooth_block);
   if (c->wall_block) (void)hipFree(c->wall_block);
+  if (c->bnb_block) (void)hipFree(c->bnb_block);
   if (c->vhist) (void)hipFree(c->vhist);
   if (c->vstart) (void)hipFree(c->vstart);
   delete c;
@@ -143,7 +163,7 @@ extern "C" int lns3d_ctx_set_params(lns3d_ctx *c, const lns3d_params *p) {
     return fail(LNS3D_EINVAL, "lns3d_ctx_set_params: grid size, periodicity and precision are fixed");
   int rc = check_params(*p);
   if (rc) return rc;
-  c->p = *p;
+  if ((rc = apply_params(c, *p))) return rc;
   c->st = make_stencil(*p);
   c->px.exact = c->py.exact = p->periodic_exact;
   c->coeffs_ready = false;
@@ -177,7 +197,13 @@ extern "C" int lns3d_setup(lns3d_ctx *c, const double *metric, const double *vm)
   const long P = (long)c->p.nx * c->p.ny;
   LNS3D_CHECK(hipMemcpyAsync(c->c.metric, metric, (size_t)P * 10 * 8, hipMemcpyHostToDevice, c->stream));
   LNS3D_CHECK(hipMemcpyAsync(c->c.vm, vm, (size_t)P * 5 * 8, hipMemcpyHostToDevice, c->stream));
-  int rc = launch_genmtrx(c->p, c->st, c->c, (double *)c->work, c->stream);
+  int rc = 0;
+  if (!c->navier) {
+    rc = launch_wall_normals(c->p, c->c, c->bnb_block, c->stream);
+    if (rc) return rc;
+    c->bnb_ready = true;
+  }
+  rc = launch_genmtrx(c->p, c->st, c->c, (double *)c->work, c->stream);
   if (!rc) rc = launch_timestep(c->p, c->c, c->scal, c->stream);
   if (rc) return rc;
   c->coeffs_ready = true;
@@ -474,6 +500,11 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
     return fail(LNS3D_EINVAL, "lns3d_factor: left = 1, 4, 5 and right = 4, 5 need their values (lns3d_set_inflow)");
   if (!c->p.yper && c->p.wall == 3 && !c->c.fwall)
     return fail(LNS3D_EINVAL, "lns3d_factor: wall = 3 needs its wall values (lns3d_set_wall)");
+  if (!c->navier && !c->bnb_ready)
+    return fail(LNS3D_EINVAL, "lns3d_factor: the inviscid rows need the wall normals (lns3d_setup with the metric)");
+  // useCalcd (src/lhs1f3d.f90:27-37): the D terms in the xi factor when
+  // istep - 1 + iter is even, istep counting this run's steps from 1
+  c->c.calcd_xi = c->p.use_calcd ? (int)((c->lstep + c->cur_iter) % 2 == 0) : 1;
   int rc = launch_assemble(c->p, c->c, 1, c->px, c->stream);
   if (!rc) rc = penta_factor(&c->px, c->stream);
   if (!rc) rc = launch_assemble(c->p, c->c, 2, c->py, c->stream);
@@ -532,6 +563,7 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
   }
   if (rc) return rc;
   c->factored = true;
+  c->factored_calcd = c->c.calcd_xi;
   for (int k = 0; k < 4; k++)
     if (bad[k]) {
       char msg[200];
@@ -564,6 +596,8 @@ extern "C" int lns3d_itrbc3d(lns3d_ctx *c, void *v_dev) {
     return fail(LNS3D_EINVAL, "lns3d_itrbc3d: left = 1, 4, 5 and right = 4, 5 need their values (lns3d_set_inflow)");
   if (!c->p.yper && c->p.wall == 3 && !c->c.fwall)
     return fail(LNS3D_EINVAL, "lns3d_itrbc3d: wall = 3 needs its wall values (lns3d_set_wall)");
+  if (!c->navier && !c->bnb_ready)
+    return fail(LNS3D_EINVAL, "lns3d_itrbc3d: the inviscid rows need the wall normals (lns3d_setup with the metric)");
   return launch_itrbc(c->p, c->c, v_dev, c->stream);
 }
 
@@ -638,8 +672,21 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     rc = lns3d_factor(c);
     if (rc) return rc;
   }
+  // updateLHS: the factors formed again at every iteration (src/impdrv3d.f90
+  // :248,310).  They depend on the step and iteration only through useCalcd's
+  // D-term placement, so they are formed again when that placement changes.
+  auto refactor = [&](int iter) {
+    if (!c->p.update_lhs) return 0;
+    const int want = c->p.use_calcd ? (int)((c->lstep + iter) % 2 == 0) : 1;
+    if (want == c->factored_calcd) return 0;
+    c->cur_iter = iter;
+    int e = lns3d_factor(c);
+    c->cur_iter = 1;
+    return e;
+  };
   if (c->niter == 1) {
-    rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
+    rc = refactor(1);
+    if (!rc) rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
     if (!rc) rc = record_norm(c, r);
     if (!rc && c->impl == 2) rc = launch_copy(c->vhist, v_dev, n, c->stream);
     if (!rc) rc = solve_xy(c, c->px, c->py, r);
@@ -650,7 +697,8 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     rc = launch_copy(c->vstart, v_dev, n, c->stream);
     for (int it = 0; it < c->niter && !rc; it++) {
       const int mode = it == 0 ? (two ? 2 : 1) : (two ? 4 : 3);
-      rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, mode, c->stream, c->vhist, c->vstart);
+      rc = refactor(it + 1);
+      if (!rc) rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, mode, c->stream, c->vhist, c->vstart);
       if (!rc && it == 0) rc = record_norm(c, r);
       if (!rc) rc = solve_xy(c, c->px, c->py, r);
       if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
@@ -868,8 +916,8 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
   for (int i = 0; i < count; i++) {
     if (!ctxs[i] || !ctxs[i]->factored || !v_dev[i] || (mode > 0 && !ctxs[i]->have32))
       return fail(LNS3D_EINVAL, "lns3d_graph_create: every context must be factored (precision 1 for modes 1, 2)");
-    if (ctxs[i]->impl == 2 || ctxs[i]->niter > 1)
-      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2 and corrector iterations take lns3d_step");
+    if (ctxs[i]->impl == 2 || ctxs[i]->niter > 1 || (ctxs[i]->p.update_lhs && ctxs[i]->p.use_calcd))
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2, corrector iterations and updateLHS with useCalcd take lns3d_step");
   }
   auto *gr = new lns3d_graph;
   std::vector<hipStream_t> st(count, nullptr);

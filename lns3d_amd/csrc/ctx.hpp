@@ -49,6 +49,11 @@ struct lns3d_ctx {
   bool state_fresh = false;  // uploaded, boundary conditions not yet applied
   int last_iters = 0;
   double last_resid = 0.0;
+  bool navier = true;               // false: the inviscid deck (Re = 0, src/input.f90:31-36)
+  double *bnb_block = nullptr;      // wall normals [2][nx] of the inviscid rows
+  bool bnb_ready = false;
+  int cur_iter = 1;                 // corrector iteration being taken (useCalcd's parity)
+  int factored_calcd = -1;          // the D-term placement the factors hold
 };
 
 namespace lns3d {

@@ -240,7 +240,7 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
     g22[e] = d2<double, double>(ny, p.yper, j, st.dy, st.dysinv, colv + e, 5L * nx);
     g12[e] = d1<double, double>(ny, p.yper, j, st.gy, st.dyinv, colg + e, 5L * nx);
   }
-  if ((p.wallt == 1 || p.wallt == 2) && j == 0) g2[4] = 0.0;   // src/gradbc.f90:15
+  if (!c.bnb && (p.wallt == 1 || p.wallt == 2) && j == 0) g2[4] = 0.0;   // src/gradbc.f90:15 (Navier)
   const double M1 = met[q], M2 = met[P + q], N1 = met[2 * P + q], N2 = met[3 * P + q];
   const double m11 = met[4 * P + q], m12 = met[5 * P + q], m22 = met[6 * P + q];
   const double n11 = met[7 * P + q], n12 = met[8 * P + q], n22 = met[9 * P + q];
@@ -614,6 +614,7 @@ struct AsmCtx {
   double d, dinv, dsinv;        // spacing along the line
   double a[5], b[5];            // interior first/second derivative weights x alfa
   int sym;                      // symmetric ends of the xi lines (bit 0 left, bit 1 right)
+  int calcd;                    // this factor carries the D terms (else the sponge)
 };
 
 // weights of slot k (point pos + k - 2) for column component cc in a row next
@@ -683,7 +684,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   if (!p.xper && dir == 1 && i == p.nx - 1 && p.right == 8) return z{(k == 2 && r == cc) ? -1.0 : 0.0, 0.0};
   if (!p.yper) {
     if (j == p.ny - 1) return one;
-    if (j == 0) {
+    if (j == 0 && !c.bnb && i >= c.wlo && i <= c.whi) {   // the viscous wall rows of is:ie
       if (r >= 1 && r <= 3) return one;
       if (r == 0) {
         if (dir == 1 && (p.wall == 1 || p.wall == 2 || p.wall == 3)) return one;
@@ -709,7 +710,7 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   const double X = (dir == 1 ? c.Ah : c.Bh)[(long)(r + 5 * cc) * P + q];
   const double *Vh = dir == 1 ? c.V11 : c.V22;
   const int ab = dir == 1 ? 0 : 2;
-  const bool calcd = dir == 1;
+  const bool calcd = A.calcd != 0;
   // which viscous / spanwise entry (r, cc) is
   int vi = -1;
   if (r == 1 && cc == 1) vi = 0;
@@ -794,10 +795,50 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
   return z{re, im};
 }
 
-static AsmCtx make_asm(const lns3d_params &p, int dir) {
+// The inviscid wall rows (Navier = .false.; src/lhsbc1f3d.f90:55-393,
+// src/lhsbc2f3d.f90:75-120) on rows i of is:ie: the blocks of the wall
+// row's unknowns rotated to the body normal coordinates -- in the xi factor
+// every block of the wall line, with the normal of the block's column (the
+// non-periodic slot map, as the reference indexes it); in the eta factor the
+// blocks of column j = 0 in rows 0, 1, 2 -- then, on the wall row, the u row
+// rotated and the v row (normal velocity) held at zero.  Elsewhere lhs_entry.
+__device__ __forceinline__ z lhs_entry_bc(const AsmCtx &A, const Coeffs &c, int dir, int i, int j, int k,
+                                          int r, int cc) {
+  const lns3d_params &p = A.p;
+  const int nx = p.nx;
+  const bool lr = !p.xper && ((i == 0 && (p.left == 0 || p.left == 4 || p.left == 5)) ||
+                              (i == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5 ||
+                                               (p.right == 8 && dir == 1))));
+  const bool zone = c.bnb && !p.yper && !lr && i >= c.wlo && i <= c.whi &&
+                    ((dir == 1 && j == 0) || (dir == 2 && j <= 2));
+  if (!zone) return lhs_entry(A, c, dir, i, j, k, r, cc);
+  if (j == 0 && r == 2) return z{(k == 2 && cc == 2) ? 1.0 : 0.0, 0.0};
+  int col = -1;                                   // the wall point whose normal rotates block k
+  if (dir == 1) {
+    const int t0[5] = {3, 4, 0, 1, 2}, t1[5] = {4, 0, 1, 2, 3};
+    const int tm[5] = {nx - 4, nx - 3, nx - 2, nx - 1, nx - 5}, tn[5] = {nx - 3, nx - 2, nx - 1, nx - 5, nx - 4};
+    col = i == 0 ? t0[k] : i == 1 ? t1[k] : i == nx - 2 ? tm[k] : i == nx - 1 ? tn[k] : i - 2 + k;
+  } else if (k == 2 - j) {
+    col = i;
+  }
+  auto cr = [&](int rr) -> z {                    // the column-rotated entry of row rr
+    if (col < 0 || (cc != 1 && cc != 2)) return lhs_entry(A, c, dir, i, j, k, rr, cc);
+    const double b1 = c.bnb[col], b2 = c.bnb[nx + col];
+    const z u = lhs_entry(A, c, dir, i, j, k, rr, 1), w = lhs_entry(A, c, dir, i, j, k, rr, 2);
+    return cc == 1 ? sc(b2, u) - sc(b1, w) : sc(b1, u) + sc(b2, w);
+  };
+  if (j == 0 && r == 1) {
+    const double b1 = c.bnb[i], b2 = c.bnb[nx + i];
+    return sc(b2, cr(1)) - sc(b1, cr(2));
+  }
+  return cr(r);
+}
+
+static AsmCtx make_asm(const lns3d_params &p, const Coeffs &c, int dir) {
   using namespace st;
   AsmCtx A;
   A.p = p;
+  A.calcd = dir == 1 ? c.calcd_xi : !c.calcd_xi;
   A.d = dir == 1 ? 1.0 / (double)(p.nx - 1) : 1.0 / (double)(p.ny - 1);
   A.dinv = 1.0 / A.d;
   A.dsinv = 1.0 / (A.d * A.d);
@@ -826,7 +867,7 @@ __global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, int line_major, z 
       for (int cc = 0; cc < 5; cc++) {
         const long e = line_major ? ((line * n + row) * 5 + k) * 25 + r * 5 + cc
                                   : ((row * 5 + k) * 25 + r * 5 + cc) * L + line;
-        lu[e] = lhs_entry(A, c, dir, i, j, k, r, cc);
+        lu[e] = lhs_entry_bc(A, c, dir, i, j, k, r, cc);
       }
 }
 
@@ -838,13 +879,13 @@ __global__ void k_assemble_ref(AsmCtx A, Coeffs c, int dir, double2 *mat) {
   for (int k = 0; k < 5; k++)
     for (int r = 0; r < 5; r++)
       for (int cc = 0; cc < 5; cc++) {
-        z v = lhs_entry(A, c, dir, i, j, k, r, cc);
+        z v = lhs_entry_bc(A, c, dir, i, j, k, r, cc);
         mat[q * 125 + k + 5 * r + 25 * cc] = make_double2(v.re, v.im);
       }
 }
 
 int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &plan, hipStream_t s) {
-  AsmCtx A = make_asm(p, dir);
+  AsmCtx A = make_asm(p, c, dir);
   long tot = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_assemble_lines, dim3(nblocks(tot, 128)), dim3(128), 0, s, A, c, dir,
                      plan.periodic ? 0 : 1, (z *)plan.lu);
@@ -853,7 +894,7 @@ int launch_assemble(const lns3d_params &p, const Coeffs &c, int dir, PentaPlan &
 }
 
 int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *mat, hipStream_t s) {
-  AsmCtx A = make_asm(p, dir);
+  AsmCtx A = make_asm(p, c, dir);
   long tot = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_assemble_ref, dim3(nblocks(tot, 128)), dim3(128), 0, s, A, c, dir,
                      (double2 *)mat);
@@ -915,7 +956,7 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
     z h;
     if (kind == 0) {
       h = d1<double2, z>(ny, p.yper, jj, st.gy, st.dyinv, v + 5L * ii + e, 5L * nx);
-      if (e == 4 && (p.wallt == 1 || p.wallt == 2) && jj == 0) h = z{0, 0};   // src/cgradbc.f90:15
+      if (e == 4 && !c.bnb && (p.wallt == 1 || p.wallt == 2) && jj == 0) h = z{0, 0};   // src/cgradbc.f90:15 (Navier)
     } else if (kind == 1) {
       h = d2<double2, z>(nx, p.xper, ii, st.dx, st.dxsinv, v + 5L * nx * jj + e, 5, st.sym,
                          e == 2 ? -1.0 : 1.0);
@@ -1013,9 +1054,11 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
       if (p.yper) {
         if (j == ny - 1) a = z{0, 0};
       } else {
-        if (j == 0) {
-          // wall = 3: the bump's wall values u1w, u2w, u3w, tw, twp (src/rhsbc3d.f90:47-100)
-          auto W = [&](int k) { return z{c.fwall[2 * (k * nx + i)], c.fwall[2 * (k * nx + i) + 1]}; };
+        if (j == 0 && !c.bnb && i >= c.wlo && i <= c.whi) {
+          // wall = 3: the bump's wall values u1w, u2w, u3w, tw, twp (src/rhsbc3d.f90:47-100),
+          // whole arrays against the is:ie section: position i takes element i - is
+          const int ib = i - c.wlo;
+          auto W = [&](int k) { return z{c.fwall[2 * (k * nx + ib)], c.fwall[2 * (k * nx + ib) + 1]}; };
           if (m >= 1 && m <= 3) {
             const z wm = p.wall == 3 ? W(m - 1) : z{0, 0};
             const z d = vq(m) - wm;
@@ -1076,7 +1119,31 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
     sr[5 * l + m] = make_double2(a.re, a.im);
   }
   __syncthreads();
-  if (tid < 5 * npt) r[5 * q0 + tid] = sr[tid];
+  if (tid < 5 * npt) {
+    double2 o = sr[tid];
+    // the inviscid wall rows (src/rhsbc3d.f90:108-118) on is:ie, rows 2 and 3
+    // of the residual: the tangential momentum row rotated, the normal
+    // velocity held at zero; the left and right columns keep their rows
+    const int pt = tid / 5, m = tid - 5 * pt;
+    const long qq = q0 + pt;
+    const int ii = (int)(qq % nx), jj = (int)(qq / nx);
+    if (c.bnb && with_rbe && !p.yper && jj == 0 && (m == 1 || m == 2) && ii >= c.wlo && ii <= c.whi) {
+      const bool lr = p.xper ? ii == nx - 1
+                             : ((ii == 0 && (p.left == 0 || p.left == 1 || p.left == 4 || p.left == 5)) ||
+                                (ii == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5 || p.right == 8)));
+      if (!lr) {
+        const double b1 = c.bnb[ii], b2 = c.bnb[nx + ii];
+        if (m == 1) {
+          const z t = sc(b2, z{sr[5 * pt + 1].x, sr[5 * pt + 1].y}) - sc(b1, z{sr[5 * pt + 2].x, sr[5 * pt + 2].y});
+          o = make_double2(t.re, t.im);
+        } else {
+          const z t = sc(b1, z{sv[5 * pt + 1].x, sv[5 * pt + 1].y}) + sc(b2, z{sv[5 * pt + 2].x, sv[5 * pt + 2].y});
+          o = make_double2(-t.re, -t.im);
+        }
+      }
+    }
+    r[5 * q0 + tid] = o;
+  }
 }
 
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
@@ -1101,7 +1168,8 @@ __global__ void k_axpy(double2 *x, const double2 *d, long n) {
   x[t] = make_double2(a.x + b.x, a.y + b.y);
 }
 
-__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const double *fwall, double2 *v) {
+__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const double *fwall, int wlo, int whi,
+                                 const double *bnb, double2 *v) {
   using namespace st;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = p.nx, ny = p.ny;
@@ -1111,45 +1179,95 @@ __global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const doubl
     for (int d = 0; d < 5; d++) V(d, ny - 1) = V(d, 0);
     return;
   }
-  // the wall value of component d (wall = 3: the bump's, src/itrbc3d.f90:48-105)
-  auto W = [&](int d) { return z{fwall[2 * (d * nx + i)], fwall[2 * (d * nx + i) + 1]}; };
-  for (int d = 1; d <= 3; d++) {
-    const z w = p.wall == 3 ? W(d - 1) : z{0, 0};
-    V(d, 0) = make_double2(w.re, w.im);
-  }
-  auto lin = [&](int d, double a, double b, double c2, double e) {
-    z s = sc(a, ld(&V(d, 1))) + sc(b, ld(&V(d, 2))) + sc(c2, ld(&V(d, 3))) + sc(e, ld(&V(d, 4)));
-    return s;
-  };
-  if (p.wall == 1) {
-    z s = lin(0, gc2, gc3, gc4, gc5);
-    V(0, 0) = make_double2(-s.re / gc1, -s.im / gc1);
-  }
-  if (p.wall == 2) {
-    z a = ld(&V(0, 1)), b = ld(&V(0, 2));
-    V(0, 0) = make_double2(2.0 * a.re - b.re, 2.0 * a.im - b.im);
-  }
-  if (p.wall == 3) {      // third-order extrapolation of the density
-    z s = sc(4.0, ld(&V(0, 1))) - sc(6.0, ld(&V(0, 2))) + sc(4.0, ld(&V(0, 3))) - ld(&V(0, 4));
-    V(0, 0) = make_double2(s.re, s.im);
-  }
-  if (p.wallt == 0) {
-    const z w = p.wall == 3 ? W(3) : z{0, 0};
-    V(4, 0) = make_double2(w.re, w.im);
-  }
-  if (p.wallt == 1) {
-    z s = lin(4, gc2, gc3, gc4, gc5);
-    if (p.wall == 3) {
-      const z t = W(4) - s;
-      V(4, 0) = make_double2(t.re / gc1, t.im / gc1);
-    } else {
-      V(4, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+  const bool on = i >= wlo && i <= whi;           // the wall extent is:ie
+  if (bnb) {
+    // inviscid (src/itrbc3d.f90:120-132): the normal velocity zeroed, the
+    // tangential one kept, on is:ie
+    if (on) {
+      const double b1 = bnb[i], b2 = bnb[nx + i];
+      const z ub = sc(b2, ld(&V(1, 0))) - sc(b1, ld(&V(2, 0)));
+      const z u = sc(b2, ub), w = sc(-b1, ub);
+      V(1, 0) = make_double2(u.re, u.im);
+      V(2, 0) = make_double2(w.re, w.im);
+    }
+  } else {
+    // the wall value of component d (wall = 3: the bump's, src/itrbc3d.f90:48-105):
+    // its velocities on the whole row, the rest on is:ie, where the whole
+    // arrays tw, twp meet the is:ie section (position i takes element i - is)
+    auto W = [&](int d, int at) { return z{fwall[2 * (d * nx + at)], fwall[2 * (d * nx + at) + 1]}; };
+    if (p.wall == 3)
+      for (int d = 1; d <= 3; d++) {
+        const z w = W(d - 1, i);
+        V(d, 0) = make_double2(w.re, w.im);
+      }
+    if (on) {
+      const int ib = i - wlo;
+      if (p.wall != 3)
+        for (int d = 1; d <= 3; d++) V(d, 0) = make_double2(0, 0);
+      auto lin = [&](int d, double a, double b, double c2, double e) {
+        z s = sc(a, ld(&V(d, 1))) + sc(b, ld(&V(d, 2))) + sc(c2, ld(&V(d, 3))) + sc(e, ld(&V(d, 4)));
+        return s;
+      };
+      if (p.wall == 1) {
+        z s = lin(0, gc2, gc3, gc4, gc5);
+        V(0, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+      }
+      if (p.wall == 2) {
+        z a = ld(&V(0, 1)), b = ld(&V(0, 2));
+        V(0, 0) = make_double2(2.0 * a.re - b.re, 2.0 * a.im - b.im);
+      }
+      if (p.wall == 3) {      // third-order extrapolation of the density
+        z s = sc(4.0, ld(&V(0, 1))) - sc(6.0, ld(&V(0, 2))) + sc(4.0, ld(&V(0, 3))) - ld(&V(0, 4));
+        V(0, 0) = make_double2(s.re, s.im);
+      }
+      if (p.wallt == 0) {
+        const z w = p.wall == 3 ? W(3, ib) : z{0, 0};
+        V(4, 0) = make_double2(w.re, w.im);
+      }
+      if (p.wallt == 1) {
+        z s = lin(4, gc2, gc3, gc4, gc5);
+        if (p.wall == 3) {
+          const z t = W(4, ib) - s;
+          V(4, 0) = make_double2(t.re / gc1, t.im / gc1);
+        } else {
+          V(4, 0) = make_double2(-s.re / gc1, -s.im / gc1);
+        }
+      }
     }
   }
   if (p.top == 1)        // src/itrbc3d.f90:187-191: the incoming wave
     for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(ftop[2 * (d * nx + i)], ftop[2 * (d * nx + i) + 1]);
   else
     for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(0, 0);
+}
+
+// bcfix (src/impdrv3d.f90:333-336, src/bcfix.f90): the wall row of the
+// increment back from the body normal coordinates, on every wall point
+__global__ void k_bcfix(int nx, const double *bnb, double2 *r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  const double b1 = bnb[i], b2 = bnb[nx + i];
+  const z a = ld(r + 5 * i + 1), b = ld(r + 5 * i + 2);
+  const z t = sc(b2, a) + sc(b1, b), u = sc(-b1, a) + sc(b2, b);
+  r[5 * i + 1] = make_double2(t.re, t.im);
+  r[5 * i + 2] = make_double2(u.re, u.im);
+}
+
+// the wall normals (src/gengrid.f90:133-137) from the metric's n1, n2 on j = 0
+__global__ void k_wall_normals(int nx, long P, const double *met, double *bnb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  const double n1 = met[2 * P + i], n2 = met[3 * P + i];
+  const double s = sqrt(n1 * n1 + n2 * n2);
+  bnb[i] = -n1 / s;
+  bnb[nx + i] = -n2 / s;
+}
+
+int launch_wall_normals(const lns3d_params &p, const Coeffs &c, double *bnb, hipStream_t s) {
+  hipLaunchKernelGGL(k_wall_normals, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p.nx, (long)p.nx * p.ny,
+                     (const double *)c.metric, bnb);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
 }
 
 // left and right columns (src/itrbc3d.f90:200-376): periodic copy; 0 zero;
@@ -1176,7 +1294,7 @@ __global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
 
 int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s) {
   hipLaunchKernelGGL(k_itrbc_wall_top, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p, (const double *)c.ftop,
-                     (const double *)c.fwall, (double2 *)v);
+                     (const double *)c.fwall, c.wlo, c.whi, c.bnb, (double2 *)v);
   hipLaunchKernelGGL(k_itrbc_lr, dim3(nblocks(p.ny, 128)), dim3(128), 0, s, p, (const double *)c.fside,
                      (double2 *)v);
   LNS3D_CHECK(hipGetLastError());
@@ -1335,7 +1453,11 @@ int launch_axpy(void *x, const void *d, long n, hipStream_t s) {
   return 0;
 }
 
-int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, const void *dv, hipStream_t s) {
+int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, void *dv, hipStream_t s) {
+  if (c.bnb && !p.yper) {      // inviscid: the increment's wall row back to x, y components
+    hipLaunchKernelGGL(k_bcfix, dim3(nblocks(p.nx, 128)), dim3(128), 0, s, p.nx, c.bnb, (double2 *)dv);
+    LNS3D_CHECK(hipGetLastError());
+  }
   int rc = launch_axpy(v, dv, 5L * p.nx * p.ny, s);
   if (rc) return rc;
   return launch_itrbc(p, c, v, s);
@@ -1451,7 +1573,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
     out[3] = out[3] - z{-t3.im, t3.re};
   }
   for (int r = 0; r < 5; r++) out[r] = sc(dt, out[r]);
-  if (dir == 1) {
+  if (A.calcd) {
     for (int r = 0; r < 5; r++) {
       z a{0, 0}, b{0, 0};
       for (int cc = 0; cc < 5; cc++) {
@@ -1461,7 +1583,8 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
       const double f = p.alfa * dt;
       out[r] = out[r] + sc(f, a) + z{-f * b.im, f * b.re};
     }
-  } else if (c.spl) {                  // sponge of the factor without the D terms
+  } else if (c.spl) {                  // sponge of the factor without the D terms (complex-float steps:
+                                       // the viscous wall rows only, lns3d_ctx_create refuses Re = 0)
     const double f = p.alfa * dt * c.spl[qr];
     for (int r = 0; r < 5; r++) out[r] = out[r] + sc(f, xc[r]);
   }
@@ -1480,7 +1603,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
   const bool holdrow = !p.xper && dir == 1 && ir == nx - 1 && p.right == 8;   // -I
   if (!p.yper) {
     if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;
-    if (jr == 0) {
+    if (jr == 0 && ir >= c.wlo && ir <= c.whi) {
       ident[1] = ident[2] = ident[3] = true;
       if (dir == 1 && (p.wall == 1 || p.wall == 2 || p.wall == 3)) ident[0] = true;
       if (dir == 2 && (p.wall == 1 || p.wall == 3)) wallrow0 = true;
@@ -1514,7 +1637,7 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,
                      hipStream_t s) {
   long P = (long)p.nx * p.ny;
-  AsmCtx Ax = make_asm(p, 1), Ay = make_asm(p, 2);
+  AsmCtx Ax = make_asm(p, c, 1), Ay = make_asm(p, c, 2);
   hipLaunchKernelGGL(k_line_op, dim3(nblocks(P, 256)), dim3(256), 0, s, Ay, c, 2,
                      (const double2 *)x, (double2 *)tmp);
   hipLaunchKernelGGL(k_line_op, dim3(nblocks(P, 256)), dim3(256), 0, s, Ax, c, 1,

@@ -52,6 +52,16 @@ struct Coeffs {
   // u1w, u2w, u3w, tw, twp, [5][nx] complex as (re, im) pairs
   double *fwall;
   double S0;                    // Sutherland's constant (mattyp = 1), 110.4 K / Tref
+  // the wall rows of the factors, the residual and the field (src/lhsbc1f3d.f90,
+  // lhsbc2f3d.f90, rhsbc3d.f90, itrbc3d.f90) cover points wlo..whi of the wall
+  // (the &extras is:ie, 0-based inclusive)
+  int wlo = 0, whi = 0;
+  // the inviscid deck (Navier = .false.): the wall normals bnb, [2][nx]
+  // (src/gengrid.f90:133-137); null for the viscous rows
+  const double *bnb = nullptr;
+  // the xi factor carries the D terms (1; the eta factor the sponge), or the
+  // eta factor does (0: useCalcd, src/lhs1f3d.f90:27-37)
+  int calcd_xi = 1;
 };
 
 int launch_genmtrx(const lns3d_params &p, const Stencil &st, Coeffs &c, double *work, hipStream_t s);
@@ -95,8 +105,12 @@ int launch_residual_norm_noalloc(const double *dtl, const void *r, long P, doubl
 int launch_forced_norm(const lns3d_params &p, const Coeffs &c, const void *r, double *part, int *nb,
                        hipStream_t s);
 // ftop: the top = 1 wave (Coeffs::ftop), null for top = 0
-int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, const void *dv, hipStream_t s);
+// v += dv and itrBC3D; for the inviscid deck bcfix turns dv's wall row back
+// to x, y components first (src/impdrv3d.f90:333-336)
+int launch_update_bc(const lns3d_params &p, const Coeffs &c, void *v, void *dv, hipStream_t s);
 int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s);
+// the wall normals bnb [2][nx] from the metric's n1, n2 on the wall row
+int launch_wall_normals(const lns3d_params &p, const Coeffs &c, double *bnb, hipStream_t s);
 // y = M x with M = Lx * Ly the approximate-factorization operator (FP64,
 // blocks regenerated from the coefficients); tmp holds 5P complex
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,

@@ -67,7 +67,8 @@ module lns3d_hip
     type(c_ptr) :: spg = c_null_ptr, spg2 = c_null_ptr
     real(c_double) :: eps_e = 0
     type(c_ptr) :: buff = c_null_ptr
-    integer(c_int) :: use_calcd = 0, update_lhs = 0, istep = 0, iter = 1
+    integer(c_int) :: use_calcd = 0, update_lhs = 0
+    type(c_ptr) :: istep = c_null_ptr, iter = c_null_ptr   ! the caller's counters, read per call
     type(c_ptr) :: wamp = c_null_ptr, vm = c_null_ptr, x = c_null_ptr, bnb = c_null_ptr, eig = c_null_ptr
     real(c_double) :: lalpha(2) = 0
     type(c_ptr) :: bump = c_null_ptr

@@ -36,7 +36,14 @@ typedef struct {
   double cflmax, dtmax;
   double alfa;
   int periodic_exact; /* 1: exact LU for periodic lines; 0: shipped slot order */
+  int wall_is, wall_ie; /* wall extent is:ie, 1-based (src/lns3d.f90:42,73); 0: 1, nx */
+  int d_in_eta;       /* 1: the D terms in the eta factor and the sponge in the xi
+                       * factor (useCalcd's first factors, src/lhs1f3d.f90:27-37) */
 } orc_params;
+/* Re = 0 with mattyp = 0 is the inviscid deck (src/input.f90:31-36:
+ * Navier = .false., Re = 1e99): rotated wall rows with the wall normals of
+ * orc_set_wall_normals */
+int orc_set_wall_normals(const double *bnb);   /* (nx,2) as src/gengrid.f90:133-137; NULL clears */
 
 /* ---- block-pentadiagonal line solvers (src/cpenta1bc.f, cpenta2bc.f,
  *      cpenta1p.f, cpenta2p.f) ---------------------------------------------- */

@@ -28,7 +28,7 @@ class _OrcParams(ctypes.Structure):
                  "left", "right", "linear", "mattyp", "loctime")] + \
                [(n, ctypes.c_double) for n in
                 ("Ma", "Re", "Pr", "gamma", "kz", "omega", "cflmax", "dtmax", "alfa")] + \
-               [("periodic_exact", ctypes.c_int)]
+               [(n, ctypes.c_int) for n in ("periodic_exact", "wall_is", "wall_ie", "d_in_eta")]
 
 
 class _OrcSponge(ctypes.Structure):
@@ -59,8 +59,13 @@ class Oracle:
     def params(self, p):
         s = _OrcParams()
         for name, _ in _OrcParams._fields_:
-            setattr(s, name, getattr(p, name))
+            setattr(s, name, getattr(p, name, 0))
         return s
+
+    def set_wall_normals(self, bnb=None):
+        """wall normals bnb (nx, 2) of the inviscid wall rows; None clears"""
+        self._bnb_keep = None if bnb is None else np.asfortranarray(bnb, dtype=np.float64)
+        self.lib.orc_set_wall_normals(_ptr(self._bnb_keep))
 
     def set_sponge(self, ispg, spg=None, spg2=None, x=None, wamp=None, vm=None, x0=0.0):
         """sponge state for later lrhs3d / lhs / step calls (ispg 0: off); the
@@ -281,12 +286,19 @@ class Reference:
     def setup(self, p, x, y, met, vm, usecalcd=0, ispg=0, x0=0.0, impl=1):
         lsym = int(p.left == 7 and not p.xper)           # src/input.f90:235-243
         rsym = int(p.right == 7 and not p.xper)
+        # Re = 0 with constant viscosity: Navier = .false., Re = 1e99
+        # (src/input.f90:31-36)
+        inviscid = p.Re == 0.0 and p.mattyp == 0
         ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, lsym, rsym, p.wall,
                        p.wallt, p.top, p.left, p.right, p.linear, impl, p.loctime, p.mattyp,
-                       ispg, 1, usecalcd], dtype=np.int32)
-        rp = np.array([p.Ma, p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax, p.dtmax,
-                       0.0, x0])
+                       ispg, 0 if inviscid else 1, usecalcd or getattr(p, "use_calcd", 0)],
+                      dtype=np.int32)
+        rp = np.array([p.Ma, 1.0e99 if inviscid else p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax,
+                       p.dtmax, 0.0, x0])
         self.lib.ref_init(ip.ctypes.data_as(VP), rp.ctypes.data_as(VP))
+        wis, wie = getattr(p, "wall_is", 0), getattr(p, "wall_ie", 0)
+        self.lib.ref_set_extras(ctypes.c_int(wis or 1), ctypes.c_int(wie or p.nx),
+                                ctypes.c_int(getattr(p, "update_lhs", 0)))
         self.lib.ref_set_grid(_ptr(x), _ptr(y), _ptr(met))
         self.lib.ref_set_mean(_ptr(vm))
         self.p = p

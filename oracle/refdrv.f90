@@ -56,7 +56,7 @@ subroutine ref_init(ip, rp) bind(C, name="ref_init")
   x0 = rp(10)
   iLHS = 1; niter = 1; ires = 0; idamp = 0; general = .true.
   complex_analysis = (kz /= zero) .or. (linear == 2)
-  is = 1; ie = nx; js = 1; je = ny
+  is = 1; ie = nx; js = 1; je = ny; updateLHS = .false.
   istep = 0; iter = 1; lstep = 0; alfa = one; time = zero
 
   dxi = one / float(nx-1)
@@ -357,7 +357,7 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   end if
   ! src/impdrv3d.f90:229-231: factors formed at the first step, and again
   ! at the second for impl = 2 (alfa 2/3)
-  form = first /= 0 .or. (impl == 2 .and. lstep_h == 1)
+  form = first /= 0 .or. (impl == 2 .and. lstep_h == 1) .or. updateLHS
   if (first /= 0) then
     if (allocated(matx)) deallocate(matx, maty, lmatx, lr)
     allocate(matx(5,ndof,ndof,nx,ny), maty(5,ndof,ndof,nx,ny))
@@ -368,13 +368,15 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
       allocate(pery(2,ndof,ndof,nx,ny), per2y(2,ndof,ndof,nx,ny))
     end if
   end if
-  istep = 1; iter = 1
+  ! the step count of this run (src/impdrv3d.f90:130 do istep = 1, nstep):
+  ! useCalcd alternates the D terms with istep - 1 + iter
+  istep = lstep_h + 1; iter = 1
   vold = v
   call itrBC3D(v, vm)
   ! the corrector iterations, src/impdrv3d.f90:173-358 (factors at iter 1)
   do it = 1, niter
   iter = it
-  if (it > 1) form = .false.
+  if (it > 1 .and. .not. updateLHS) form = .false.
   tc0 = omp_get_wtime()
   ! src/impdrv3d.f90:179-199
   if (impl == 3) then
@@ -457,6 +459,9 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   else
     call cpenta1bc(nx, ny, ndof, maty, r, 1)
   end if
+  ! inviscid: rotate back to global coordinates on the body
+  ! (src/impdrv3d.f90:333-336)
+  if ((.not. Navier) .and. (.not. yper)) call bcfix(nx, ny, ndof, bnb, r)
   tc1 = omp_get_wtime(); phase_t(3) = phase_t(3) + (tc1 - tc0); tc0 = tc1
 
   ! the update in parallel, as src/impdrv3d.f90:340-348
@@ -473,6 +478,17 @@ subroutine ref_step(v, dv, first) bind(C, name="ref_step")
   lstep_h = lstep_h + 1
   deallocate(vold, r)
 end subroutine ref_step
+
+! the &extras namelist's wall extent is:ie (src/lns3d.f90:42, 73) and
+! updateLHS (src/global.f90:33)
+subroutine ref_set_extras(is_l, ie_l, update) bind(C, name="ref_set_extras")
+  use iso_c_binding
+  use global
+  implicit none
+  integer(c_int), value :: is_l, ie_l, update
+  is = is_l; ie = ie_l
+  updateLHS = update /= 0
+end subroutine ref_set_extras
 
 ! the phase times of the last ref_step (seconds; see phase_t)
 subroutine ref_phase_times(t) bind(C, name="ref_phase_times")

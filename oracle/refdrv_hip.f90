@@ -15,6 +15,10 @@ contains
     real(c_double), target, intent(in) :: a(*)
     raddr = c_loc(a(1))
   end function
+  type(c_ptr) function iaddr(a)
+    integer(c_int), target, intent(in) :: a
+    iaddr = c_loc(a)
+  end function
   type(c_ptr) function caddr(a)
     complex(c_double_complex), target, intent(in) :: a(*)
     caddr = c_loc(a(1))
@@ -54,7 +58,7 @@ subroutine ref_bind_hip(rc) bind(C, name="ref_bind_hip")
   st%eps_e = eps_e
   if (allocated(buff)) st%buff = raddr(buff)
   st%use_calcd = lns3d_hip_flag(useCalcd); st%update_lhs = lns3d_hip_flag(updateLHS)
-  st%istep = istep; st%iter = iter
+  st%istep = iaddr(istep); st%iter = iaddr(iter)
   if (allocated(wamp)) st%wamp = raddr(wamp)
   st%vm = raddr(vm); st%x = raddr(x); st%bnb = raddr(bnb)
   if (allocated(rhor)) then

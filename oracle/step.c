@@ -263,6 +263,22 @@ static int supported(const orc_params *p) {
   return 1;
 }
 
+/* Navier (src/input.f90:31-36): Re = 0 with constant viscosity is the
+ * inviscid deck, which holds Re = 1e99 */
+static int navier(const orc_params *p) { return !(p->Re == 0.0 && p->mattyp == 0); }
+static double re_eff(const orc_params *p) { return navier(p) ? p->Re : 1.0e99; }
+/* the wall extent is:ie (0-based, inclusive) */
+static int wall_lo(const orc_params *p) { return p->wall_is ? p->wall_is - 1 : 0; }
+static int wall_hi(const orc_params *p) { return p->wall_ie ? p->wall_ie - 1 : p->nx - 1; }
+static int on_wall(const orc_params *p, int i) { return i >= wall_lo(p) && i <= wall_hi(p); }
+
+/* wall normals bnb(nx,2) (src/gengrid.f90:133-137) for the inviscid rows */
+static const double *g_bnb;
+int orc_set_wall_normals(const double *bnb) {
+  g_bnb = bnb;
+  return 0;
+}
+
 /* ---- sponge: the module state src/lrhs3d.f90:222-289 and the LHS routines
  *      read (spg, spg2 of src/sponge.f90:20, wamp, x0, omega) ----------------- */
 static orc_sponge g_spg;          /* ispg = 0: no sponge */
@@ -313,7 +329,7 @@ static void wave_terms(const orc_params *p, const double *vml, zdouble c3, zdoub
 static void acoustic_wave(const orc_params *p, const double *vml, double xl, double x0, double amp,
                           zdouble f[5]) {
   const double cm = sqrt(vml[4]) / p->Ma, um = vml[1];
-  const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / p->Re;
+  const double d = 0.5 * (1.3333333333333333333 + (p->gamma - 1.0) / p->Pr) / re_eff(p);
   const double kk = p->omega / (cm + um);
   const double a = p->omega * p->omega * d / ((cm + um) * (cm + um) * (cm + um));
   const zdouble c3 = amp * (exp(-a * (xl - x0)) * cexp(_Complex_I * (kk * xl)));
@@ -515,7 +531,7 @@ int orc_lrhs3d(const orc_params *p, const orc_coeffs *c, const zdouble *v,
           *c22 = work + 20 * P;
   orc_cgrad(p, v, c1, c2);
   orc_cgrad2(p, v, c1, c11, c12, c22);
-  if (p->wallt == 1 || p->wallt == 2)                  /* src/cgradbc.f90:15 */
+  if (navier(p) && (p->wallt == 1 || p->wallt == 2))   /* src/cgradbc.f90:15, src/lrhs3d.f90:43 */
     for (int i = 0; i < nx; i++) c2[4 + 5 * i] = 0;
   const zdouble ci = _Complex_I;
   for (long q = 0; q < P; q++) {
@@ -665,7 +681,9 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
   const double *X = dir == 1 ? c->Ah : c->Bh;
   const double *Vh = dir == 1 ? c->Vh11 : c->Vh22;
   int off = dir == 1 ? 0 : 2;                  /* ABhi(1:2) or ABhi(3:4) */
-  int calcd = dir == 1;                        /* D terms ride in the xi factor */
+  /* the D terms ride in the xi factor, or in the eta factor for useCalcd's
+   * first factors (src/lhs1f3d.f90:27-37, lhs2f3d.f90:27-37) */
+  int calcd = (dir == 1) != (p->d_in_eta != 0);
   double a[5] = {alfa * ga1 / d, alfa * ga2 / d, alfa * 0.0 / d, alfa * ga3 / d, alfa * ga4 / d};
   double b[5] = {alfa * da1 / (d * d), alfa * da2 / (d * d), alfa * da3 / (d * d),
                  alfa * da4 / (d * d), alfa * da5 / (d * d)};
@@ -762,9 +780,48 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
   }
 
   /* boundary conditions */
+  if (!p->yper && !navier(p)) {
+    /* inviscid wall rows (src/lhsbc1f3d.f90:55-393, src/lhsbc2f3d.f90:75-120),
+     * rows i of is:ie: the blocks of the wall's unknowns rotated to the body
+     * normal coordinates (u, v components), then the u and v rows rotated and
+     * the normal velocity held at zero */
+    if (!g_bnb) return -1;
+    const int cols0[5] = {3, 4, 0, 1, 2}, cols1[5] = {4, 0, 1, 2, 3};
+    for (int i = wall_lo(p); i <= wall_hi(p); i++) {
+      for (int k = 0; k < 5; k++) {
+        long pt;
+        int col;
+        if (dir == 1) {                 /* every block of the wall line: its column's normal */
+          pt = i;
+          col = i == 0 ? cols0[k] : i == 1 ? cols1[k] :
+                i == nx - 2 ? (int[5]){nx - 4, nx - 3, nx - 2, nx - 1, nx - 5}[k] :
+                i == nx - 1 ? (int[5]){nx - 3, nx - 2, nx - 1, nx - 5, nx - 4}[k] : i - 2 + k;
+        } else {                        /* the blocks of column j = 0: rows 0, 1, 2 */
+          if (k > 2) continue;
+          pt = i + (long)nx * (2 - k);
+          col = i;
+        }
+        const double b1 = g_bnb[col], b2 = g_bnb[col + nx];
+        for (int r = 0; r < 5; r++) {
+          const zdouble u = MAT(k, r, 1), w = MAT(k, r, 2);
+          MAT(k, r, 1) = u * b2 - w * b1;
+          MAT(k, r, 2) = u * b1 + w * b2;
+        }
+      }
+      long pt = i;
+      const double b1 = g_bnb[i], b2 = g_bnb[i + nx];
+      for (int k = 0; k < 5; k++)
+        for (int cc = 0; cc < 5; cc++) {
+          MAT(k, 1, cc) = b2 * MAT(k, 1, cc) - b1 * MAT(k, 2, cc);
+          MAT(k, 2, cc) = 0;
+        }
+      MAT(2, 2, 2) = 1.0;
+    }
+  }
   if (!p->yper) {
     for (int i = 0; i < nx; i++) {
       long pt = i;
+      if (!navier(p) || !on_wall(p, i)) continue;    /* the viscous rows of is:ie */
       if (dir == 1 && (p->wall == 1 || p->wall == 2 || p->wall == 3)) {
         for (int k = 0; k < 5; k++)
           for (int cc = 0; cc < 5; cc++) MAT(k, 0, cc) = 0;
@@ -838,20 +895,33 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
       for (int d = 0; d < 5; d++) R(d, i, ny - 1) = 0;
   } else {
     if (p->wall == 3 && !g_bump) return -1;
+    if (!navier(p)) {                            /* src/rhsbc3d.f90:108-118 */
+      if (!g_bnb) return -1;
+      for (int i = wall_lo(p); i <= wall_hi(p); i++) {
+        const double b1 = g_bnb[i], b2 = g_bnb[i + nx];
+        R(1, i, 0) = b2 * R(1, i, 0) - b1 * R(2, i, 0);
+        R(2, i, 0) = -(b1 * V(1, i, 0) + b2 * V(2, i, 0));
+      }
+    }
     for (int i = 0; i < nx; i++) {
+      if (!navier(p) || !on_wall(p, i)) continue;   /* the viscous rows of is:ie */
+      /* the bump's wall values enter these rows as whole arrays against the
+       * is:ie section (vl(2,is:ie,1) - u1w, src/rhsbc3d.f90:47-50): position
+       * i of the section takes element i - is of the array */
+      const long ib = i - wall_lo(p);
       for (int d = 1; d <= 3; d++)              /* wall = 3: the bump's velocities */
-        R(d, i, 0) = -(V(d, i, 0) - (p->wall == 3 ? g_bump[i + (long)nx * (d - 1)] : 0.0));
+        R(d, i, 0) = -(V(d, i, 0) - (p->wall == 3 ? g_bump[ib + (long)nx * (d - 1)] : 0.0));
       if (p->wall == 1)
         R(0, i, 0) = gc1 * V(0, i, 0) + gc2 * V(0, i, 1) + gc3 * V(0, i, 2) +
                      gc4 * V(0, i, 3) + gc5 * V(0, i, 4);
       if (p->wall == 2) R(0, i, 0) = V(0, i, 0) - 2.0 * V(0, i, 1) + V(0, i, 2);
       if (p->wall == 3)                          /* src/rhsbc3d.f90:74-79 */
         R(0, i, 0) = V(0, i, 0) - (4.0 * V(0, i, 1) - 6.0 * V(0, i, 2) + 4.0 * V(0, i, 3) - V(0, i, 4));
-      if (p->wallt == 0) R(4, i, 0) = p->wall == 3 ? -(V(4, i, 0) - g_bump[i + 3L * nx]) : 0;
+      if (p->wallt == 0) R(4, i, 0) = p->wall == 3 ? -(V(4, i, 0) - g_bump[ib + 3L * nx]) : 0;
       if (p->wallt == 1) {
         R(4, i, 0) = gc1 * V(4, i, 0) + gc2 * V(4, i, 1) + gc3 * V(4, i, 2) +
                      gc4 * V(4, i, 3) + gc5 * V(4, i, 4);
-        if (p->wall == 3) R(4, i, 0) = R(4, i, 0) - g_bump[i + 4L * nx];
+        if (p->wall == 3) R(4, i, 0) = R(4, i, 0) - g_bump[ib + 4L * nx];
       }
     }
     for (int i = 0; i < nx; i++) {
@@ -896,18 +966,37 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
       for (int d = 0; d < 5; d++) V(d, i, ny - 1) = V(d, i, 0);
   } else {
     if (p->wall == 3 && !g_bump) return -1;
+    if (!navier(p)) {                            /* src/itrbc3d.f90:120-132 */
+      if (!g_bnb) return -1;
+      for (int i = wall_lo(p); i <= wall_hi(p); i++) {
+        const double b1 = g_bnb[i], b2 = g_bnb[i + nx];
+        const zdouble ub = b2 * V(1, i, 0) - b1 * V(2, i, 0);
+        V(1, i, 0) = b2 * ub;
+        V(2, i, 0) = -b1 * ub;
+      }
+    }
     for (int i = 0; i < nx; i++) {
-      for (int d = 1; d <= 3; d++) V(d, i, 0) = p->wall == 3 ? g_bump[i + (long)nx * (d - 1)] : 0;
+      if (!navier(p)) break;
+      /* wall = 3 holds the velocities on the whole row, the other wall rows
+       * on is:ie only (src/itrbc3d.f90:48-58) */
+      if (p->wall == 3)
+        for (int d = 1; d <= 3; d++) V(d, i, 0) = g_bump[i + (long)nx * (d - 1)];
+      if (!on_wall(p, i)) continue;
+      if (p->wall != 3)
+        for (int d = 1; d <= 3; d++) V(d, i, 0) = 0;
+      /* the is:ie section against the whole arrays tw, twp
+       * (src/itrbc3d.f90:86-104): element i - is of the array */
+      const long ib = i - wall_lo(p);
       if (p->wall == 1)
         V(0, i, 0) = -(gc2 * V(0, i, 1) + gc3 * V(0, i, 2) + gc4 * V(0, i, 3) +
                        gc5 * V(0, i, 4)) / gc1;
       if (p->wall == 2) V(0, i, 0) = 2.0 * V(0, i, 1) - V(0, i, 2);
       if (p->wall == 3)                          /* src/itrbc3d.f90:76-81 */
         V(0, i, 0) = 4.0 * V(0, i, 1) - 6.0 * V(0, i, 2) + 4.0 * V(0, i, 3) - V(0, i, 4);
-      if (p->wallt == 0) V(4, i, 0) = p->wall == 3 ? g_bump[i + 3L * nx] : 0;
+      if (p->wallt == 0) V(4, i, 0) = p->wall == 3 ? g_bump[ib + 3L * nx] : 0;
       if (p->wallt == 1) {
         if (p->wall == 3)                        /* src/itrbc3d.f90:93-104 */
-          V(4, i, 0) = (g_bump[i + 4L * nx] - (gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
+          V(4, i, 0) = (g_bump[ib + 4L * nx] - (gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
                                                 gc5 * V(4, i, 4))) / gc1;
         else
           V(4, i, 0) = -(gc2 * V(4, i, 1) + gc3 * V(4, i, 2) + gc4 * V(4, i, 3) +
@@ -987,12 +1076,12 @@ int orc_genmtrx(const orc_params *p, const double *vm, const double *metric, orc
   double *g1 = g, *g2 = g + 5 * P, *g11 = g + 10 * P, *g12 = g + 15 * P, *g22 = g + 20 * P;
   orc_grad(p, 5, vm, g1, g2);
   orc_grad2(p, 5, vm, g1, g11, g12, g22);
-  if (p->wallt == 1 || p->wallt == 2)                    /* src/gradbc.f90:15 */
+  if (navier(p) && (p->wallt == 1 || p->wallt == 2))     /* src/gradbc.f90:15, src/genmtrx.f90:82 */
     for (int i = 0; i < nx; i++) g2[4 + 5 * i] = 0;
   const double *m1 = metric, *m2 = metric + P, *n1 = metric + 2 * P, *n2 = metric + 3 * P,
                *m11 = metric + 4 * P, *m12 = metric + 5 * P, *m22 = metric + 6 * P,
                *n11 = metric + 7 * P, *n12 = metric + 8 * P, *n22 = metric + 9 * P;
-  double gamma = p->gamma, gamma1 = gamma - 1.0, Ma = p->Ma, Re = p->Re, Pr = p->Pr;
+  double gamma = p->gamma, gamma1 = gamma - 1.0, Ma = p->Ma, Re = re_eff(p), Pr = p->Pr;
   double kz = p->kz;
   for (long q = 0; q < P; q++) {
     double M1 = m1[q], M2 = m2[q], N1 = n1[q], N2 = n2[q];
@@ -1299,6 +1388,15 @@ static int solve_update(const orc_params *p, zdouble *matx, zdouble *maty, zdoub
   if (p->yper) {
     if (orc_cpenta_p(1, nx, ny, maty, r, pery, per2y, 1, p->periodic_exact)) return -1;
   } else if (orc_cpenta_bc(1, nx, ny, maty, r, 1)) return -1;
+  if (!navier(p) && !p->yper) {                /* bcfix, src/impdrv3d.f90:333-336, src/bcfix.f90 */
+    if (!g_bnb) return -1;
+    for (int i = 0; i < nx; i++) {
+      const double b1 = g_bnb[i], b2 = g_bnb[i + nx];
+      const zdouble t = b2 * r[1 + 5 * i] + b1 * r[2 + 5 * i];
+      r[2 + 5 * i] = -b1 * r[1 + 5 * i] + b2 * r[2 + 5 * i];
+      r[1 + 5 * i] = t;
+    }
+  }
   for (long e = 0; e < 5 * P; e++) { v[e] += r[e]; dv[e] = r[e]; }
   return orc_itrbc3d(p, v);
 }

@@ -62,6 +62,7 @@ reference's outputs:
     python tests/golden/gen_golden.py forced     (the shift-invert steps only)
     python tests/golden/gen_golden.py bump       (the bump wall cases only)
     python tests/golden/gen_golden.py sutherland (the Sutherland case only)
+    python tests/golden/gen_golden.py extras     (Re = 0, the wall extent, useCalcd, updateLHS)
 """
 from __future__ import annotations
 
@@ -496,6 +497,74 @@ def bump_case(R, wallt):
     np.savez_compressed(os.path.join(HERE, f"ref_bump_t{wallt}.npz"), **out)
 
 
+EXTRA_CASES = {
+    # Re = 0: the inviscid deck (Navier = .false.), the whole wall and a part of it
+    "inviscid": dict(Re=0.0),
+    "inviscid_isie": dict(Re=0.0, wall_is=3, wall_ie=10),
+    # the viscous wall rows over a part of the wall
+    "wall_isie": dict(wall=1, wallt=1, wall_is=4, wall_ie=10),
+    "bump_isie": dict(wall=3, wallt=0, wall_is=3, wall_ie=11),
+    # useCalcd: the D terms in the eta factor, the sponge in the xi factor;
+    # with updateLHS the factors are formed again at every step, alternating
+    "calcd": dict(use_calcd=1),
+    "calcd_update": dict(use_calcd=1, update_lhs=1),
+}
+
+
+def twisted_metric(met, x):
+    """the analytic metric with an x-dependent n1 (n1 = 0.3 n2 sin(2 pi x /
+    100)): boundary normals that are not along y, for the inviscid wall rows"""
+    m = np.array(met, order="F")
+    m[:, :, 2] = 0.3 * m[:, :, 3] * np.sin(2.0 * np.pi * x / 100.0)
+    return np.asfortranarray(m)
+
+
+def extras_case(R, name, kw):
+    """deck options of the &extras namelist and Re = 0 (src/lns3d.f90:42,
+    src/input.f90:31-36): lrhs3D + rhsBC3D, itrBC3D, the factors (where they
+    do not depend on the step) and two or three steps"""
+    p = Params(nx=13, ny=9, kz=0.4, omega=0.2, **kw)
+    x, y, met = make_grid(p)
+    met = twisted_metric(met, x)
+    vm = make_mean(p, y)
+    rng = np.random.default_rng(61 + len(name))
+    v0 = np.asfortranarray(rng.standard_normal((5, p.nx, p.ny)) + 1j * rng.standard_normal((5, p.nx, p.ny)))
+    sponge = p.use_calcd == 1
+    R.setup(p, x, y, met, vm, ispg=1 if sponge else 0)
+    out = dict(params=_params_json(p), met=met, vm=vm, v0=v0, x=x, y=y)
+    if sponge:
+        spg = np.asfortranarray(np.where(x >= 60.0, ((x - 60.0) / 40.0) ** 2, 0.0))
+        R.set_sponge(spg, np.zeros_like(spg), np.ones(p.nx))
+        out["spg"] = spg
+    if p.wall == 3:
+        uw = np.asfortranarray(0.1 * (rng.standard_normal((p.nx, 5)) + 1j * rng.standard_normal((p.nx, 5))))
+        R.set_bump(uw)
+        out["uw"] = uw
+    c = R.genmtrx()
+    c["ABhi"][4:] = 0.0                   # never set by genmtrx (unused, not initialised)
+    dtl = R.set_time_step()
+    out.update(dtl=dtl, **{"c_" + k: v for k, v in c.items()})
+    r, _ = R.lrhs3d(np.asfortranarray(v0))
+    rb = np.asfortranarray(r.copy())
+    R.rhsbc3d(rb, np.asfortranarray(v0))
+    out["lrhs"] = r
+    out["rhsbc"] = rb
+    v = np.asfortranarray(v0.copy())
+    R.itrbc3d(v)
+    out["itrbc"] = v.copy()
+    if not p.use_calcd:
+        out["lhs1"] = R.lhs(1)
+        out["lhs2"] = R.lhs(2)
+    n = 3 if p.update_lhs else 2
+    for k in range(n):
+        out[f"dv{k + 1}"] = R.step(v, k == 0)
+        out[f"v{k + 1}"] = v.copy()
+    if p.wall == 3:
+        R.set_bump(np.zeros((p.nx, 5), complex))
+    R.setup(Params(nx=p.nx, ny=p.ny), x, y, make_grid(Params(nx=p.nx, ny=p.ny))[2], vm)   # defaults again
+    np.savez_compressed(os.path.join(HERE, f"ref_extra_{name}.npz"), **out)
+
+
 def sutherland_case(R):
     """mattyp = 1, Sutherland's law (src/getmat.f90:28-40) with S0 = 110.4 K /
     300 K: coefficients, dtl and two steps"""
@@ -538,6 +607,11 @@ def main():
     if sys.argv[1:] == ["sym"]:
         for name, kw in SYM_CASES.items():
             sym_case(R, name, kw)
+        R.lib.ref_flush()
+        return
+    if sys.argv[1:] == ["extras"]:
+        for name, kw in EXTRA_CASES.items():
+            extras_case(R, name, kw)
         R.lib.ref_flush()
         return
     if sys.argv[1:] == ["sutherland"]:

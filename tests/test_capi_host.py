@@ -158,12 +158,14 @@ def test_request_and_instance_draws():
     (dict(Ma=float("nan")), 1, b"must be finite"),
     (dict(kz=float("inf")), 1, b"must be finite"),
     (dict(Re=0.0, mattyp=1), 1, b"mattyp = 0 only"),
-    (dict(Re=0.0), 4, b"inviscid"),
     (dict(wall_is=5, wall_ie=3), 1, b"wall extent"),
-    (dict(wall_is=2, wall_ie=33), 4, b"wall extent"),
+    (dict(wall_is=2, wall_ie=34), 1, b"wall extent"),
+    (dict(wall_is=-1), 1, b"wall extent"),
     (dict(idamp=1), 4, b"idamp"),
-    (dict(use_calcd=1), 4, b"useCalcd"),
-    (dict(update_lhs=1), 4, b"updateLHS"),
+    (dict(use_calcd=2), 1, b"use_calcd"),
+    (dict(use_calcd=1, left=7), 4, b"useCalcd"),
+    (dict(Re=0.0, precision=1), 4, b"inviscid"),
+    (dict(update_lhs=1, precision=1), 4, b"updateLHS"),
 ])
 def test_params_refused_before_any_device_call(lib, change, code, text):
     """lns3d_ctx_create checks the deck before it touches the device: values
@@ -174,8 +176,10 @@ def test_params_refused_before_any_device_call(lib, change, code, text):
     import dataclasses
     from lns3d_amd.config import Params
     from lns3d_amd.native import CParams
+    change = dict(change)
+    precision = change.pop("precision", 0)
     p = dataclasses.replace(Params(nx=33, ny=17), **change)
-    cp = CParams.from_params(p, 0)
+    cp = CParams.from_params(p, precision)
     out = ctypes.c_void_p()
     lib.lns3d_ctx_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
     lib.lns3d_last_error.restype = ctypes.c_char_p

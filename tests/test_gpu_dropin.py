@@ -8,7 +8,7 @@ ref_bind_hip forwards the reference's module state through the one bind(C)
 setter (lns3d_set_module_state).  Every step of the harness then runs the
 HIP path at each of those call sites and must reproduce the vectors the
 unmodified reference wrote (ref_step_*, ref_sponge_*, ref_smooth_base,
-ref_top1) to 1e-12.  The cgrad and cgrad2 externals are checked against the
+ref_top1, ref_extra_*) to 1e-12.  The cgrad and cgrad2 externals are checked against the
 oracle's restatement (pinned by closed forms) on the same fields."""
 import ctypes
 import json
@@ -139,6 +139,33 @@ def test_harness_top_wave(R):
     assert rel(R.lhs(1), z["lhs1"]) < TOL and rel(R.lhs(2), z["lhs2"]) < TOL
     assert rel(R.step(v, True), z["dv1"]) < TOL
     lib().lns3d_release_factors()
+
+
+@pytest.mark.parametrize("name", ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd",
+                                  "calcd_update"])
+def test_harness_extras(R, name):
+    """the inviscid deck (its wall rows in lhs1f3D / lhs2f3D through the
+    forwarded wall normals bnb), the wall extent is:ie, and useCalcd with and
+    without updateLHS: lhs1f3D / lhs2f3D read the harness's own istep and
+    iter through the forwarded counters at every call"""
+    from lns3d_amd.native import lib
+    print(f"relinked harness, ref_extra_{name}", flush=True)
+    z, p = load(f"ref_extra_{name}.npz")
+    sponge = "spg" in z.files
+    R.setup(p, z["x"], z["y"], z["met"], z["vm"], ispg=1 if sponge else 0)
+    if sponge:
+        R.set_sponge(z["spg"], np.zeros_like(z["spg"]), np.ones(p.nx))
+    if "uw" in z.files:
+        R.set_bump(z["uw"])
+    R.genmtrx()
+    R.set_time_step()
+    bind(R)
+    try:
+        check_case(R, z, z["v0"], 3 if p.update_lhs else 2)
+    finally:
+        if "uw" in z.files:
+            R.set_bump(np.zeros((p.nx, 5), complex))
+        lib().lns3d_release_factors()
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(optx=0, opty=0), dict(optx=1, opty=1), dict(xper=1),

@@ -521,3 +521,70 @@ def test_sutherland(orc):
         assert rel(v, z["v2"]) < 1e-12
     finally:
         orc.set_sutherland(-1.0)
+
+
+def wall_normals(met):
+    """bnb (nx, 2) of src/gengrid.f90:133-137 from the metric's n1, n2 at j = 1"""
+    n1, n2 = met[:, 0, 2], met[:, 0, 3]
+    s = np.sqrt(n1 ** 2 + n2 ** 2)
+    return np.asfortranarray(np.stack([-n1 / s, -n2 / s], axis=1))
+
+
+EXTRA_CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update"]
+
+
+@pytest.mark.parametrize("name", EXTRA_CASES)
+def test_extras(orc, name):
+    """Re = 0 (the inviscid deck: Navier = .false., Re = 1e99, the rotated wall
+    rows of src/lhsbc1f3d.f90:55-393, lhsbc2f3d.f90:75-120, rhsbc3d.f90
+    :108-118, itrbc3d.f90:120-132 and bcfix), the wall extent is:ie of the
+    &extras namelist, and useCalcd / updateLHS (src/lhs1f3d.f90:27-37,
+    impdrv3d.f90:248,310), against the reference's routines
+    (ref_extra_<case>.npz); without the option the oracle misses"""
+    import dataclasses
+    import types
+    z = load(f"ref_extra_{name}.npz")
+    p = params_of(z)
+    met, vm = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"])
+    v0 = np.asfortranarray(z["v0"])
+    orc.set_wall_normals(wall_normals(met) if p.Re == 0.0 else None)
+    if "spg" in z.files:
+        orc.set_sponge(1, spg=z["spg"])
+    if "uw" in z.files:
+        orc.set_bump(z["uw"])
+    try:
+        c = orc.genmtrx(p, vm, met)
+        for k in ("Ah", "Bh", "Dh", "Dhi", "Vh11", "Vh12", "Vh22", "ABhi"):
+            assert rel(c[k], z["c_" + k]) < 1e-14, k
+        dtl = orc.set_time_step(p, vm, met)
+        assert rel(dtl, z["dtl"]) < 1e-14
+        r, _ = orc.lrhs3d(p, c, v0)
+        assert rel(r, z["lrhs"]) < 1e-14
+        orc.rhsbc3d(p, r, v0)
+        assert rel(r, z["rhsbc"]) < 1e-14
+        v = np.asfortranarray(v0.copy())
+        orc.itrbc3d(p, v)
+        assert rel(v, z["itrbc"]) < 1e-14
+        for d in (1, 2):
+            if f"lhs{d}" in z.files:
+                assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
+        nsteps = 3 if p.update_lhs else 2
+        f = None
+        for k in range(1, nsteps + 1):
+            # useCalcd: the D terms in the xi factor when istep - 1 + iter is
+            # even; the factors of step 1 (or of every step with updateLHS)
+            if f is None or p.update_lhs:
+                pd = types.SimpleNamespace(**dataclasses.asdict(p), d_in_eta=int(p.use_calcd and k % 2 == 1))
+                f = orc.factor(pd, c, dtl)
+            assert rel(orc.step(p, c, dtl, f, v), z[f"dv{k}"]) < 1e-12, k
+            assert rel(v, z[f"v{k}"]) < 1e-12, k
+        # the same deck without the option misses
+        plain = dataclasses.replace(p, Re=1000.0 if p.Re == 0.0 else p.Re, wall_is=0, wall_ie=0)
+        v = np.asfortranarray(v0.copy())
+        orc.itrbc3d(plain, v)
+        fp = orc.factor(plain, c, dtl)
+        assert rel(orc.step(plain, c, dtl, fp, v), z["dv1"]) > 1e-6
+    finally:
+        orc.set_wall_normals(None)
+        orc.set_sponge(0)
+        orc.set_bump(None)
