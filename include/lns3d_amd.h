@@ -379,12 +379,17 @@ int lns3d_last_residual(lns3d_ctx *c, double *rel);
  * 96 nx, lns3d_set_inflow 176 ny. */
 int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
 
-/* Graphs of steps: nsteps steps of each of count factored contexts, every
- * context on its own branch, captured once and replayed with one launch
- * (batches of uneven instances).  mode 0 complex double (lns3d_step), 1
- * complex-float factors (lns3d_step_fp32), 2 complex-float factors with
- * `corrections` defect corrections (lns3d_step_refined_fixed).  dv_dev may
- * be NULL.  Launch on `stream` (NULL: the graph's own stream). */
+/* Graphs of steps: nsteps steps of each of count factored contexts,
+ * captured once and replayed with one launch (batches of uneven
+ * instances).  mode 0 complex double (lns3d_step), 1 complex-float factors
+ * (lns3d_step_fp32), 2 complex-float factors with `corrections` defect
+ * corrections (lns3d_step_refined_fixed): every context on its own branch.
+ * mode 3: the steps of mode 2 with one launch per stage over all contexts
+ * (residual, line solves grouped by solve kernel, corrections, update),
+ * the same arithmetic; non-periodic lines, viscous decks, no recorded
+ * residual norms (LNS3D_EUNSUPPORTED otherwise).  dv_dev may be NULL.
+ * Launch on `stream` (NULL: the graph's own stream).  No counterpart in the
+ * reference. */
 typedef struct lns3d_graph lns3d_graph;
 int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps, int mode,
                        int corrections, lns3d_graph **out);

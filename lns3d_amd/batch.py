@@ -6,8 +6,10 @@ HBM).  Before the contexts are built, the line-solve kernel of every line
 shape in the batch (xi lines: ny lines of nx rows; eta lines: nx lines of ny
 rows) is chosen by the autotuner (lns3d_tune_lines: every variant timed on
 synthetic lines of that shape).  The steps of all instances are then captured
-into one graph, each instance on its own branch, so one launch runs
-nsteps x count steps with the instances overlapping on the device.
+into one graph, so one launch runs nsteps x count steps: with
+MODE_BATCHED every stage of a step (residual, line solves grouped by solve
+kernel, corrections, update) is one kernel launch over all instances; with
+the other modes each instance runs on its own branch.
 """
 from __future__ import annotations
 
@@ -17,7 +19,7 @@ import time
 import numpy as np
 
 from .config import Params, make_grid, make_mean, make_disturbance
-from .native import Graph, Step, tune_lines, MODE_FP64, MODE_FP32, MODE_REFINED
+from .native import Graph, Step, tune_lines, set_line_variant, MODE_FP64, MODE_FP32, MODE_REFINED, MODE_BATCHED
 
 
 def draw_instances(count: int = 64, seed: int = 0, nx_range=(65, 513), ny_range=(33, 257)):
@@ -44,6 +46,54 @@ def tune_shapes(params_list, precisions=(0, 1)):
                 if key not in done:
                     done[key] = tune_lines(n, lines, prec)
     return done
+
+
+# line-solve variants (lns3d_tune_lines numbering) the batch-level tuner
+# tries for every line shape of a staged batch: one chain per line with
+# three or four lines per workgroup.  A shape alone is solved fastest by
+# the variant that shortens the chain (twisted lines); a staged launch over
+# thousands of lines is bound by throughput, where more lines per workgroup
+# share the issue of one wave.
+BATCH_CANDIDATES = (("plain3", 4), ("plain4", 1))
+
+
+def pin_variants(params_list, variant, tuned=None, precisions=(0, 1)):
+    """use one solve variant for every line shape of the batch; None: the
+    per-shape choice again (the records of `tuned`, a tune_shapes result,
+    else the built-in rule)"""
+    for p in params_list:
+        for n, lines in ((p.nx, p.ny), (p.ny, p.nx)):
+            for prec in precisions:
+                v = variant
+                if v is None:
+                    v = tuned[(n, lines, prec)][0] if tuned and (n, lines, prec) in tuned else -1
+                set_line_variant(n, lines, prec, v)
+
+
+def tune_batch(params_list, precision=1, nsteps=2, corrections=1, seed=0, tuned=None):
+    """the batch-level tuner of a staged batch: times a short staged graph
+    with the per-shape variants (`tuned`, a tune_shapes result) and with
+    each candidate of BATCH_CANDIDATES for all lines; pins the fastest and
+    returns (name, {name: seconds})"""
+    import torch
+    times = {}
+    for name, variant in (("per-shape", None),) + BATCH_CANDIDATES:
+        pin_variants(params_list, variant, tuned)
+        b = Batch(params_list, precision=precision, seed=seed)
+        g = b.capture(nsteps, MODE_BATCHED, corrections)
+        g.launch()
+        g.synchronize()
+        b.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.launch()
+        g.synchronize()
+        times[name] = time.perf_counter() - t0
+        g.close()
+        b.close()
+    best = min(times, key=times.get)
+    pin_variants(params_list, dict(BATCH_CANDIDATES).get(best), tuned)
+    return best, times
 
 
 class Batch:
@@ -99,7 +149,7 @@ class Batch:
         self.steps = []
 
 
-def run_batch(count=64, nsteps=20, seed=0, precision=1, mode=MODE_REFINED, corrections=1,
+def run_batch(count=64, nsteps=20, seed=0, precision=1, mode=MODE_BATCHED, corrections=1,
               repeats=3, tune=True):
     """the batch benchmark: tune, build, capture, replay; returns a dict of
     timings and throughput (instance-steps/s and grid-point-steps/s)"""
@@ -107,6 +157,9 @@ def run_batch(count=64, nsteps=20, seed=0, precision=1, mode=MODE_REFINED, corre
     params = draw_instances(count, seed)
     t0 = time.perf_counter()
     tuned = tune_shapes(params, (0, 1) if precision else (0,)) if tune else {}
+    choice, choice_times = None, {}
+    if tune and mode == MODE_BATCHED:
+        choice, choice_times = tune_batch(params, precision, corrections=corrections, seed=seed, tuned=tuned)
     t_tune = time.perf_counter() - t0
     t0 = time.perf_counter()
     b = Batch(params, precision=precision, seed=seed)
@@ -128,6 +181,10 @@ def run_batch(count=64, nsteps=20, seed=0, precision=1, mode=MODE_REFINED, corre
                point_steps_per_s=b.points * nsteps / t, tune_seconds=t_tune,
                build_seconds=t_build,
                variants={f"{k[0]}x{k[1]}p{k[2]}": v[0] for k, v in list(tuned.items())[:8]})
+    if choice is not None:
+        out["batch_variant"] = choice
+        out["batch_variant_seconds"] = {k: round(v, 5) for k, v in choice_times.items()}
+        pin_variants(params, None, tuned)
     g.close()
     b.close()
     return out

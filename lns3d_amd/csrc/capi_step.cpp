@@ -907,11 +907,100 @@ struct lns3d_graph {
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
   hipStream_t s = nullptr;
+  // mode 3: the device tables of the batched stages
+  StepBatch sb;
+  std::vector<PentaBatchGroup> gx, gy;
 };
+
+static void graph_free(lns3d_graph *gr) {
+  if (gr->x) (void)hipGraphExecDestroy(gr->x);
+  if (gr->g) (void)hipGraphDestroy(gr->g);
+  if (gr->s) (void)hipStreamDestroy(gr->s);
+  step_batch_free(gr->sb);
+  penta_batch_free(gr->gx);
+  penta_batch_free(gr->gy);
+  delete gr;
+}
+
+// mode 3: the refined steps of every member with one launch per stage: the
+// residual, the two line solves, the defect corrections and the update each
+// run over all members at once (the arithmetic of lns3d_step_refined_fixed)
+static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps,
+                                int corrections, lns3d_graph **out) {
+  for (int i = 0; i < count; i++) {
+    lns3d_ctx *c = ctxs[i];
+    if (c->px32.periodic || c->py32.periodic || !c->navier || c->norm_out)
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create mode 3: periodic lines, the inviscid deck and "
+                                      "recorded residual norms take mode 2");
+  }
+  auto *gr = new lns3d_graph;
+  std::vector<StepBatchMember> m((size_t)count);
+  std::vector<const PentaPlan *> px((size_t)count), py((size_t)count);
+  std::vector<double2 *> r0((size_t)count), r1((size_t)count);
+  std::vector<long> sx1((size_t)count), sx2((size_t)count), sy1((size_t)count), sy2((size_t)count);
+  for (int i = 0; i < count; i++) {
+    lns3d_ctx *c = ctxs[i];
+    StepBatchMember &a = m[(size_t)i];
+    a.p = c->p;
+    a.st = c->st;
+    a.c = c->c;
+    a.v = (double2 *)v_dev[i];
+    a.dv = (double2 *)(dv_dev && dv_dev[i] ? dv_dev[i] : c->r);
+    a.rhs = (double2 *)c->t3;
+    a.t1 = (double2 *)c->t1;
+    a.t2 = (double2 *)c->t2;
+    a.work = (double2 *)c->work;
+    px[(size_t)i] = &c->px32;
+    py[(size_t)i] = &c->py32;
+    r0[(size_t)i] = a.dv;
+    r1[(size_t)i] = a.t2;
+    sx1[(size_t)i] = c->p.nx;   // xi lines: line j, row i at (i + nx j)
+    sx2[(size_t)i] = 1;
+    sy1[(size_t)i] = 1;         // eta lines: line i, row j
+    sy2[(size_t)i] = c->p.nx;
+  }
+  int rc = step_batch_build(m.data(), count, gr->sb);
+  if (!rc) rc = penta_batch_build(px.data(), r0.data(), r1.data(), sx1.data(), sx2.data(), count, gr->gx);
+  if (!rc) rc = penta_batch_build(py.data(), r0.data(), r1.data(), sy1.data(), sy2.data(), count, gr->gy);
+  if (!rc && hipStreamCreateWithFlags(&gr->s, hipStreamNonBlocking) != hipSuccess) rc = fail(LNS3D_EHIP, "stream");
+  if (!rc && hipStreamBeginCapture(gr->s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    rc = fail(LNS3D_EHIP, "hipStreamBeginCapture");
+  const bool capturing = !rc;
+  const hipStream_t s = gr->s;
+  for (int k = 0; k < nsteps && !rc; k++) {
+    rc = step_batch_rhs(gr->sb, s);
+    if (!rc) rc = step_batch_vec(gr->sb, SB_COPY, s);
+    if (!rc) rc = penta_batch_solve(gr->gx, 0, s);
+    if (!rc) rc = penta_batch_solve(gr->gy, 0, s);
+    for (int q = 0; q < corrections && !rc; q++) {
+      rc = step_batch_matvec(gr->sb, s);
+      if (!rc) rc = step_batch_vec(gr->sb, SB_SUB, s);
+      if (!rc) rc = penta_batch_solve(gr->gx, 1, s);
+      if (!rc) rc = penta_batch_solve(gr->gy, 1, s);
+      if (!rc) rc = step_batch_vec(gr->sb, q == corrections - 1 ? SB_UPDATE : SB_ADD_DV, s);
+    }
+    if (!rc && corrections == 0) rc = step_batch_vec(gr->sb, SB_ADD_V, s);
+    if (!rc) rc = step_batch_itrbc(gr->sb, s);
+  }
+  if (capturing) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (!rc && e != hipSuccess) rc = fail(LNS3D_EHIP, "hipStreamEndCapture");
+    gr->g = g;
+  }
+  if (!rc && hipGraphInstantiate(&gr->x, gr->g, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(LNS3D_EHIP, "hipGraphInstantiate");
+  if (rc) {
+    graph_free(gr);
+    return rc;
+  }
+  *out = gr;
+  return 0;
+}
 
 extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps,
                                   int mode, int corrections, lns3d_graph **out) {
-  if (!ctxs || !v_dev || count < 1 || nsteps < 1 || mode < 0 || mode > 2 || !out)
+  if (!ctxs || !v_dev || count < 1 || nsteps < 1 || mode < 0 || mode > 3 || corrections < 0 || !out)
     return fail(LNS3D_EINVAL, "lns3d_graph_create: bad arguments");
   for (int i = 0; i < count; i++) {
     if (!ctxs[i] || !ctxs[i]->factored || !v_dev[i] || (mode > 0 && !ctxs[i]->have32))
@@ -919,6 +1008,7 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
     if (ctxs[i]->impl == 2 || ctxs[i]->niter > 1 || (ctxs[i]->p.update_lhs && ctxs[i]->p.use_calcd))
       return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2, corrector iterations and updateLHS with useCalcd take lns3d_step");
   }
+  if (mode == 3) return graph_create_batched(ctxs, v_dev, dv_dev, count, nsteps, corrections, out);
   auto *gr = new lns3d_graph;
   std::vector<hipStream_t> st(count, nullptr);
   std::vector<hipEvent_t> ev(count + 1, nullptr);
@@ -962,10 +1052,7 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
     rc = fail(LNS3D_EHIP, "hipGraphInstantiate");
   cleanup();
   if (rc) {
-    if (gr->x) (void)hipGraphExecDestroy(gr->x);
-    if (gr->g) (void)hipGraphDestroy(gr->g);
-    if (gr->s) (void)hipStreamDestroy(gr->s);
-    delete gr;
+    graph_free(gr);
     return rc;
   }
   *out = gr;
@@ -987,10 +1074,7 @@ extern "C" int lns3d_graph_synchronize(lns3d_graph *gr) {
 extern "C" int lns3d_graph_destroy(lns3d_graph *gr) {
   if (!gr) return 0;
   (void)hipStreamSynchronize(gr->s);
-  if (gr->x) (void)hipGraphExecDestroy(gr->x);
-  if (gr->g) (void)hipGraphDestroy(gr->g);
-  if (gr->s) (void)hipStreamDestroy(gr->s);
-  delete gr;
+  graph_free(gr);
   return 0;
 }
 

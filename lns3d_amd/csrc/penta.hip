@@ -977,16 +977,16 @@ __device__ __forceinline__ void compute_pass(int nchunk, Body body) {
   }
 }
 
+// the solve of workgroup bid (lines bid G .. bid G + G - 1)
 template <typename T, int R, int G>
-__global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__restrict__ fwd,
-                                                 const cx<T> *__restrict__ bwd,
-                                                 const cx<T> *__restrict__ tail, double2 *rhs,
-                                                 long sL, long sR) {
+__device__ __forceinline__ void solve_bc(long bid, int n, long L, const cx<T> *__restrict__ fwd,
+                                         const cx<T> *__restrict__ bwd, const cx<T> *__restrict__ tail,
+                                         double2 *rhs, long sL, long sR) {
   constexpr int NBUF = 4, BUFE = R * 66;                        // LDS record rows of one line
   static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
   __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][G][BUFE];
   __shared__ double2 bst[NBUF][G][R][5];    // rhs rows, then c (or d) in place
-  const long line0 = (long)blockIdx.x * G;
+  const long line0 = bid * G;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   // G lines per workgroup; DPP row r (16 lanes) of the compute wave runs the
@@ -1200,6 +1200,14 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
   });
 }
 
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__restrict__ fwd,
+                                                 const cx<T> *__restrict__ bwd,
+                                                 const cx<T> *__restrict__ tail, double2 *rhs,
+                                                 long sL, long sR) {
+  solve_bc<T, R, G>(blockIdx.x, n, L, fwd, bwd, tail, rhs, sL, sR);
+}
+
 // ---------------------------------------------------------------------------
 // stage 3, twisted lines: the factor eliminated rows n-1..h+2 bottom-up and
 // then rows 0..h+1 top-down, so each line is two recurrences of about n/2
@@ -1211,13 +1219,11 @@ __global__ __launch_bounds__(256) void k_solve_bc(int n, long L, const cx<T> *__
 // later from chain 0's x_{h+1}, x_h.
 // HALF (one line per workgroup, single steps): the DPP rows that would
 // repeat the two chains instead take half of each row's columns (below).
-template <typename T, int R, int G, bool PAIRS, int NBUF, bool HALF = false>
-__global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
-                                                 const cx<T> *__restrict__ bwd,
-                                                 const cx<T> *__restrict__ tail,
-                                                 const cx<T> *__restrict__ xf,
-                                                 const cx<T> *__restrict__ xb, double2 *rhs,
-                                                 long sL, long sR) {
+template <typename T, int R, int G, bool PAIRS, int NBUF, bool HALF>
+__device__ __forceinline__ void solve_tw(long bid, int n, int h, long L, const cx<T> *__restrict__ fwd,
+                                         const cx<T> *__restrict__ bwd, const cx<T> *__restrict__ tail,
+                                         const cx<T> *__restrict__ xf, const cx<T> *__restrict__ xb,
+                                         double2 *rhs, long sL, long sR) {
   constexpr int C = 2 * G;
   constexpr int BUFE = R * 66;
   static_assert(R % 6 == 0, "a chunk is whole groups of six rows");
@@ -1230,7 +1236,7 @@ __global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const
   __shared__ cx<T> kept[C][2][5];    // x at chain rows 3, 4 (backward boundary blocks)
   // zero rows standing in for the staged c rows in the column half without c
   __shared__ double2 zrows[HALF ? R : 1][5];
-  const long line0 = (long)blockIdx.x * G;
+  const long line0 = bid * G;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int ch = (lane >> 4) % C;    // chain of this DPP row (rows past 2G repeat)
@@ -1625,6 +1631,16 @@ __global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const
   };
   run_pass(true);
   run_pass(false);
+}
+
+template <typename T, int R, int G, bool PAIRS, int NBUF, bool HALF = false>
+__global__ __launch_bounds__(256, 2) void k_solve_tw(int n, int h, long L, const cx<T> *__restrict__ fwd,
+                                                 const cx<T> *__restrict__ bwd,
+                                                 const cx<T> *__restrict__ tail,
+                                                 const cx<T> *__restrict__ xf,
+                                                 const cx<T> *__restrict__ xb, double2 *rhs,
+                                                 long sL, long sR) {
+  solve_tw<T, R, G, PAIRS, NBUF, HALF>(blockIdx.x, n, h, L, fwd, bwd, tail, xf, xb, rhs, sL, sR);
 }
 
 // ---------------------------------------------------------------------------
@@ -2272,6 +2288,142 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
     else launch_solve_bc<double>(pl, rhs, sL, sR, s);
   }
   LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batches: one launch solves the lines of many plans of one solve kernel.
+// Workgroup b belongs to member k with off[k] <= b < off[k + 1]; the member's
+// item holds its plan's records and its two right-hand sides.
+__device__ __forceinline__ int batch_member(const int *off, int count) {
+  int lo = 0, hi = count;
+  const int b = (int)blockIdx.x;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void kb_solve_bc(const PentaBatchItem *items, const int *off, int count,
+                                                  int which) {
+  const int k = batch_member(off, count);
+  const PentaBatchItem &b = items[k];
+  solve_bc<T, R, G>((long)blockIdx.x - off[k], b.n, b.L, (const cx<T> *)b.fwd, (const cx<T> *)b.bwd,
+                    (const cx<T> *)b.tail, b.rhs[which], b.sL, b.sR);
+}
+
+template <typename T, int R, int G, bool PAIRS, int NBUF, bool HALF = false>
+__global__ __launch_bounds__(256, 2) void kb_solve_tw(const PentaBatchItem *items, const int *off, int count,
+                                                     int which) {
+  const int k = batch_member(off, count);
+  const PentaBatchItem &b = items[k];
+  solve_tw<T, R, G, PAIRS, NBUF, HALF>((long)blockIdx.x - off[k], b.n, b.h, b.L, (const cx<T> *)b.fwd,
+                                       (const cx<T> *)b.bwd, (const cx<T> *)b.tail, (const cx<T> *)b.xf,
+                                       (const cx<T> *)b.xb, b.rhs[which], b.sL, b.sR);
+}
+
+// lines per workgroup of a non-periodic solve variant
+static int variant_lines(int v) {
+  return v == PENTA_PLAIN2 ? 2 : v == PENTA_PLAIN3 ? 3 : v == PENTA_PLAIN4 ? 4 : 1;
+}
+
+int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, double2 *const *rhs1, const long *sL,
+                      const long *sR, int count, std::vector<PentaBatchGroup> &out) {
+  out.clear();
+  std::map<std::pair<int, int>, std::vector<int>> by;   // (variant, precision) -> members
+  for (int k = 0; k < count; k++) {
+    const PentaPlan *pl = plans[k];
+    if (pl->periodic) return set_error(__FILE__, __LINE__, "batched line solves take non-periodic lines");
+    by[{pl->variant, pl->prec}].push_back(k);
+  }
+  for (auto &kv : by) {
+    // longest lines first: their workgroups start in the first wave and the
+    // short ones fill in behind them
+    std::stable_sort(kv.second.begin(), kv.second.end(),
+                     [&](int a, int b) { return plans[a]->n > plans[b]->n; });
+    PentaBatchGroup g;
+    g.variant = kv.first.first;
+    g.prec = kv.first.second;
+    g.count = (int)kv.second.size();
+    std::vector<PentaBatchItem> items;
+    std::vector<int> off(1, 0);
+    for (int k : kv.second) {
+      const PentaPlan *pl = plans[k];
+      PentaBatchItem it;
+      it.n = pl->n;
+      it.h = pl->h;
+      it.L = pl->L;
+      it.fwd = pl->fwd;
+      it.bwd = pl->bwd;
+      it.tail = pl->tail;
+      it.xf = pl->xf;
+      it.xb = pl->xb;
+      it.rhs[0] = rhs0[k];
+      it.rhs[1] = rhs1[k];
+      it.sL = sL[k];
+      it.sR = sR[k];
+      items.push_back(it);
+      const long wg = (pl->L + variant_lines(g.variant) - 1) / variant_lines(g.variant);
+      if ((long)off.back() + wg > 0x7fffffffL) return set_error(__FILE__, __LINE__, "batch too large");
+      off.push_back(off.back() + (int)wg);
+    }
+    g.blocks = off.back();
+    LNS3D_CHECK(hipMalloc(&g.items, items.size() * sizeof(PentaBatchItem)));
+    LNS3D_CHECK(hipMalloc(&g.off, off.size() * sizeof(int)));
+    LNS3D_CHECK(hipMemcpy(g.items, items.data(), items.size() * sizeof(PentaBatchItem), hipMemcpyHostToDevice));
+    LNS3D_CHECK(hipMemcpy(g.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    out.push_back(g);
+  }
+  return 0;
+}
+
+void penta_batch_free(std::vector<PentaBatchGroup> &groups) {
+  for (auto &g : groups) {
+    if (g.items) (void)hipFree(g.items);
+    if (g.off) (void)hipFree(g.off);
+  }
+  groups.clear();
+}
+
+template <typename T>
+static void launch_batch_group(const PentaBatchGroup &g, int which, hipStream_t s) {
+  constexpr bool D = sizeof(T) == 8;
+  const auto *it = (const PentaBatchItem *)g.items;
+  const dim3 grid((unsigned)g.blocks), blk(256);
+  switch (g.variant) {
+    case PENTA_TWISTED:
+      hipLaunchKernelGGL((kb_solve_tw<T, D ? 6 : 12, 1, false, 4>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    case PENTA_TWISTED_HALF:
+      hipLaunchKernelGGL((kb_solve_tw<T, D ? 6 : 12, 1, false, 4, true>), grid, blk, 0, s, it, g.off, g.count,
+                         which);
+      break;
+    case PENTA_TWISTED_PAIRS:
+      hipLaunchKernelGGL((kb_solve_tw<T, D ? 6 : 12, 1, true, 4>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    case PENTA_PLAIN2:
+      hipLaunchKernelGGL((kb_solve_bc<T, D ? 6 : 12, 2>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    case PENTA_PLAIN3:
+      hipLaunchKernelGGL((kb_solve_bc<T, 6, 3>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    case PENTA_PLAIN4:
+      hipLaunchKernelGGL((kb_solve_bc<T, 6, 4>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    default:
+      hipLaunchKernelGGL((kb_solve_bc<T, D ? 12 : 24, 1>), grid, blk, 0, s, it, g.off, g.count, which);
+  }
+}
+
+int penta_batch_solve(const std::vector<PentaBatchGroup> &groups, int which, hipStream_t s) {
+  for (const auto &g : groups) {
+    if (g.prec) launch_batch_group<float>(g, which, s);
+    else launch_batch_group<double>(g, which, s);
+    LNS3D_CHECK(hipGetLastError());
+  }
   return 0;
 }
 

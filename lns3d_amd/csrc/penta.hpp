@@ -1,5 +1,6 @@
 // Host interface of the block-pentadiagonal line solvers (penta.hip).
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 
 namespace lns3d {
@@ -24,6 +25,23 @@ struct PentaPlan {
   void *xf = nullptr, *xb = nullptr;   // twisted lines: per step pair, the second row's
                                        // original N2 (forward) / Q4 (backward) block
   void *flags = nullptr;   // per line: 1 when its factors hold a non-finite entry or a zero pivot
+};
+
+// one member of a batched line solve: a plan's records (non-periodic lines)
+// and two right-hand sides, chosen per launch
+struct PentaBatchItem {
+  int n, h;
+  long L;
+  const void *fwd, *bwd, *tail, *xf, *xb;
+  double2 *rhs[2];
+  long sL, sR;
+};
+// the members of a batch that share a solve kernel; device tables
+struct PentaBatchGroup {
+  int variant = 0, prec = 0, count = 0;
+  long blocks = 0;
+  void *items = nullptr;   // PentaBatchItem[count]
+  int *off = nullptr;      // first workgroup of each member, [count + 1]
 };
 
 // solve-kernel variants of non-periodic lines: one line per workgroup with a
@@ -59,6 +77,13 @@ int penta_factor_records_only(PentaPlan *pl, hipStream_t s);
 // complex offset (l*sL + i*sR)*5 + d
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);
 int penta_solve_reference_order(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);
+// batched solves: the plans grouped by solve kernel, with right-hand sides
+// rhs0[k], rhs1[k] of strides sL[k], sR[k]; penta_batch_solve launches one
+// kernel per group on right-hand side `which` (0 or 1) of every member
+int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, double2 *const *rhs1, const long *sL,
+                      const long *sR, int count, std::vector<PentaBatchGroup> &out);
+int penta_batch_solve(const std::vector<PentaBatchGroup> &groups, int which, hipStream_t s);
+void penta_batch_free(std::vector<PentaBatchGroup> &groups);
 // factor health: flag the lines whose LU (and border fill) holds a
 // non-finite entry or a zero pivot (launch after penta_factor), or whose
 // complex-float records hold a non-finite entry (after the records); then

@@ -905,14 +905,18 @@ int launch_assemble_ref(const lns3d_params &p, const Coeffs &c, int dir, void *m
 // ==========================================================================
 // RHS: c1 = d/dxi v first (stored), then the point operator
 // one lane per (point, component), so a wave's stencil loads are contiguous
-__global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v, double2 *c1) {
-  long t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rhs_d1x(long bid, int nx, int ny, int xper, const Stencil &st, const double2 *v,
+                                        double2 *c1) {
+  long t = bid * blockDim.x + threadIdx.x;
   if (t >= 5L * nx * ny) return;
   const long q = t / 5;
   const int e = (int)(t - 5 * q);
   const int i = q % nx, j = q / nx;
   z g = d1<double2, z>(nx, xper, i, st.gx, st.dxinv, v + 5L * nx * j + e, 5, st.sym, e == 2 ? -1.0 : 1.0);
   c1[t] = make_double2(g.re, g.im);
+}
+__global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v, double2 *c1) {
+  rhs_d1x(xcd_tile(blockIdx.x, gridDim.x), nx, ny, xper, st, v, c1);
 }
 
 // The point operator works on tiles of RT consecutive grid points with one
@@ -927,15 +931,14 @@ __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v
 constexpr int RT = 64;
 constexpr int NDER = 17;
 
-__global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, Coeffs c,
-                                                  const double2 *v, const double2 *c1a, double2 *r,
-                                                  int with_rbe, const double2 *v2old,
-                                                  const double2 *vold, const double2 *forcing) {
+__device__ __forceinline__ void rhs_point(long bid, const lns3d_params &p, const Stencil &st, const Coeffs &c,
+                                          const double2 *v, const double2 *c1a, double2 *r, int with_rbe,
+                                          const double2 *v2old, const double2 *vold, const double2 *forcing) {
   using namespace st;
   __shared__ double2 sv[5 * RT], sg1[5 * RT], sd[NDER * RT], sr[5 * RT];
   const int nx = p.nx, ny = p.ny;
   const long P = (long)nx * ny;
-  const long q0 = xcd_tile(blockIdx.x, gridDim.x) * RT;   // tiles of one XCD: consecutive rows
+  const long q0 = bid * RT;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int npt = (int)min((long)RT, P - q0);
   if (tid < 5 * npt) {
@@ -1145,6 +1148,13 @@ __global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, C
     r[5 * q0 + tid] = o;
   }
 }
+__global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, Coeffs c,
+                                                  const double2 *v, const double2 *c1a, double2 *r,
+                                                  int with_rbe, const double2 *v2old,
+                                                  const double2 *vold, const double2 *forcing) {
+  // tiles of one XCD: consecutive rows
+  rhs_point(xcd_tile(blockIdx.x, gridDim.x), p, st, c, v, c1a, r, with_rbe, v2old, vold, forcing);
+}
 
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
                void *work, int with_rbe, hipStream_t s, const void *v2old, const void *vold,
@@ -1168,10 +1178,11 @@ __global__ void k_axpy(double2 *x, const double2 *d, long n) {
   x[t] = make_double2(a.x + b.x, a.y + b.y);
 }
 
-__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const double *fwall, int wlo, int whi,
-                                 const double *bnb, double2 *v) {
+__device__ __forceinline__ void itrbc_wall_top(long bid, const lns3d_params &p, const double *ftop,
+                                               const double *fwall, int wlo, int whi, const double *bnb,
+                                               double2 *v) {
   using namespace st;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = (int)(bid * blockDim.x + threadIdx.x);
   const int nx = p.nx, ny = p.ny;
   if (i >= nx) return;
   auto V = [&](int d, int jj) -> double2 & { return v[5 * (i + (long)nx * jj) + d]; };
@@ -1240,6 +1251,10 @@ __global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const doubl
   else
     for (int d = 0; d < 5; d++) V(d, ny - 1) = make_double2(0, 0);
 }
+__global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const double *fwall, int wlo, int whi,
+                                 const double *bnb, double2 *v) {
+  itrbc_wall_top(blockIdx.x, p, ftop, fwall, wlo, whi, bnb, v);
+}
 
 // bcfix (src/impdrv3d.f90:333-336, src/bcfix.f90): the wall row of the
 // increment back from the body normal coordinates, on every wall point
@@ -1272,8 +1287,8 @@ int launch_wall_normals(const lns3d_params &p, const Coeffs &c, double *bnb, hip
 
 // left and right columns (src/itrbc3d.f90:200-376): periodic copy; 0 zero;
 // left 1 and 4, right 4 the values in fside; 7 (symmetry) left alone
-__global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void itrbc_lr(long bid, const lns3d_params &p, const double *fside, double2 *v) {
+  int j = (int)(bid * blockDim.x + threadIdx.x);
   const int nx = p.nx, ny = p.ny;
   if (j >= ny) return;
   double2 *row = v + 5L * nx * j;
@@ -1290,6 +1305,9 @@ __global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
             make_double2(fside[2 * ((5 + d) * ny + j)], fside[2 * ((5 + d) * ny + j) + 1]);
     }
   }
+}
+__global__ void k_itrbc_lr(lns3d_params p, const double *fside, double2 *v) {
+  itrbc_lr(blockIdx.x, p, fside, v);
 }
 
 int launch_itrbc(const lns3d_params &p, const Coeffs &c, void *v, hipStream_t s) {
@@ -1486,12 +1504,13 @@ __device__ __forceinline__ int slot_col(int N, int per, int pos, int k) {
 // V the compact viscous entries and AB the spanwise cross entries of the
 // direction; boundary-condition rows as lhs_entry.  Equal to the assembled
 // block rows up to rounding.
-__global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
+__device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs &c, int dir, const double2 *x,
+                                        double2 *y) {
   using namespace st;
   const lns3d_params &p = A.p;
   const int nx = p.nx, ny = p.ny;
   const long P = (long)nx * ny;
-  long q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  long q = bid * blockDim.x + threadIdx.x;
   if (q >= P) return;
   const int i = q % nx, j = q / nx;
   const int N = dir == 1 ? nx : ny, pos = dir == 1 ? i : j;
@@ -1633,6 +1652,9 @@ __global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2
     y[5 * q + r] = make_double2(out[r].re, out[r].im);
   }
 }
+__global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
+  line_op(xcd_tile(blockIdx.x, gridDim.x), A, c, dir, x, y);
+}
 
 int launch_af_matvec(const lns3d_params &p, const Coeffs &c, const void *x, void *y, void *tmp,
                      hipStream_t s) {
@@ -1706,6 +1728,185 @@ int launch_residual_norms(const void *b, void *y, long n, double *norms, hipStre
                      norms + 2);
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, s, (const double *)(norms + 2), nb,
                      norms);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ==========================================================================
+// batches: each stage of the step is one launch over the members of a batch.
+// Workgroup b of a stage belongs to member k with off[k] <= b < off[k + 1]
+// (the stage's own table); the member's item holds its deck, coefficients and
+// buffers.  The arithmetic is that of the single-member kernels.
+struct StepBatchItem {
+  lns3d_params p;
+  Stencil st;
+  Coeffs c;
+  AsmCtx ax, ay;
+  double2 *buf[6];     // v, dv, rhs, t1, t2, work
+  long P;
+};
+
+__device__ __forceinline__ int step_member(const int *off, int count) {
+  int lo = 0, hi = count;
+  const int b = (int)blockIdx.x;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void kb_rhs_d1x(const StepBatchItem *items, const int *off, int count) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  rhs_d1x((long)blockIdx.x - off[k], b.p.nx, b.p.ny, b.p.xper, b.st, b.buf[SB_V], b.buf[SB_WORK]);
+}
+
+__global__ __launch_bounds__(320) void kb_rhs_point(const StepBatchItem *items, const int *off, int count) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  rhs_point((long)blockIdx.x - off[k], b.p, b.st, b.c, b.buf[SB_V], b.buf[SB_WORK], b.buf[SB_RHS], 1, nullptr,
+            nullptr, nullptr);
+}
+
+// y = L_dir x with x, y chosen among the member's buffers
+__global__ void kb_line_op(const StepBatchItem *items, const int *off, int count, int dir, int xs, int ys) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  line_op((long)blockIdx.x - off[k], dir == 1 ? b.ax : b.ay, b.c, dir, b.buf[xs], b.buf[ys]);
+}
+
+// elementwise stages over the 5P values of a member
+__global__ void kb_vec(const StepBatchItem *items, const int *off, int count, int op) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  const long t = ((long)blockIdx.x - off[k]) * blockDim.x + threadIdx.x;
+  if (t >= 5 * b.P) return;
+  double2 *v = b.buf[SB_V], *dv = b.buf[SB_DV], *rhs = b.buf[SB_RHS], *t2 = b.buf[SB_T2];
+  if (op == SB_COPY) {
+    dv[t] = rhs[t];
+  } else if (op == SB_SUB) {
+    const double2 bb = rhs[t], yy = t2[t];
+    t2[t] = make_double2(bb.x - yy.x, bb.y - yy.y);
+  } else if (op == SB_ADD_DV || op == SB_UPDATE) {
+    const double2 a = dv[t], c = t2[t];
+    const double2 d = make_double2(a.x + c.x, a.y + c.y);
+    dv[t] = d;
+    if (op == SB_UPDATE) {
+      const double2 x = v[t];
+      v[t] = make_double2(x.x + d.x, x.y + d.y);
+    }
+  } else {   // SB_ADD_V
+    const double2 x = v[t], d = dv[t];
+    v[t] = make_double2(x.x + d.x, x.y + d.y);
+  }
+}
+
+__global__ void kb_itrbc_wall_top(const StepBatchItem *items, const int *off, int count) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  itrbc_wall_top((long)blockIdx.x - off[k], b.p, b.c.ftop, b.c.fwall, b.c.wlo, b.c.whi, b.c.bnb, b.buf[SB_V]);
+}
+
+__global__ void kb_itrbc_lr(const StepBatchItem *items, const int *off, int count) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  itrbc_lr((long)blockIdx.x - off[k], b.p, b.c.fside, b.buf[SB_V]);
+}
+
+// workgroups of each stage for a member
+static long stage_blocks(int stage, const lns3d_params &p) {
+  const long P = (long)p.nx * p.ny;
+  switch (stage) {
+    case SB_STAGE_D1X: return nblocks(5 * P, 256);
+    case SB_STAGE_POINT: return nblocks(P, RT);
+    case SB_STAGE_LINE: return nblocks(P, 256);
+    case SB_STAGE_VEC: return nblocks(5 * P, 256);
+    case SB_STAGE_WALL: return nblocks(p.nx, 128);
+    default: return nblocks(p.ny, 128);
+  }
+}
+
+int step_batch_build(const StepBatchMember *m, int count, StepBatch &out) {
+  step_batch_free(out);
+  std::vector<StepBatchItem> items((size_t)count);
+  std::vector<std::vector<int>> off(SB_NSTAGE, std::vector<int>(1, 0));
+  for (int k = 0; k < count; k++) {
+    const StepBatchMember &a = m[k];
+    if (a.c.bnb) return set_error(__FILE__, __LINE__, "batched steps: the inviscid wall rows take the per-member steps");
+    StepBatchItem &it = items[(size_t)k];
+    it.p = a.p;
+    it.st = a.st;
+    it.c = a.c;
+    it.ax = make_asm(a.p, a.c, 1);
+    it.ay = make_asm(a.p, a.c, 2);
+    it.buf[SB_V] = a.v;
+    it.buf[SB_DV] = a.dv;
+    it.buf[SB_RHS] = a.rhs;
+    it.buf[SB_T1] = a.t1;
+    it.buf[SB_T2] = a.t2;
+    it.buf[SB_WORK] = a.work;
+    it.P = (long)a.p.nx * a.p.ny;
+    for (int sg = 0; sg < SB_NSTAGE; sg++) {
+      const long nb = stage_blocks(sg, a.p);
+      if ((long)off[sg].back() + nb > 0x7fffffffL) return set_error(__FILE__, __LINE__, "batch too large");
+      off[sg].push_back(off[sg].back() + (int)nb);
+    }
+  }
+  out.count = count;
+  LNS3D_CHECK(hipMalloc(&out.items, items.size() * sizeof(StepBatchItem)));
+  LNS3D_CHECK(hipMemcpy(out.items, items.data(), items.size() * sizeof(StepBatchItem), hipMemcpyHostToDevice));
+  for (int sg = 0; sg < SB_NSTAGE; sg++) {
+    out.blocks[sg] = off[sg].back();
+    LNS3D_CHECK(hipMalloc(&out.off[sg], off[sg].size() * sizeof(int)));
+    LNS3D_CHECK(hipMemcpy(out.off[sg], off[sg].data(), off[sg].size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+void step_batch_free(StepBatch &b) {
+  if (b.items) (void)hipFree(b.items);
+  for (int sg = 0; sg < SB_NSTAGE; sg++) {
+    if (b.off[sg]) (void)hipFree(b.off[sg]);
+    b.off[sg] = nullptr;
+  }
+  b.items = nullptr;
+  b.count = 0;
+}
+
+int step_batch_rhs(const StepBatch &b, hipStream_t s) {
+  const auto *it = (const StepBatchItem *)b.items;
+  hipLaunchKernelGGL(kb_rhs_d1x, dim3((unsigned)b.blocks[SB_STAGE_D1X]), dim3(256), 0, s, it, b.off[SB_STAGE_D1X],
+                     b.count);
+  hipLaunchKernelGGL(kb_rhs_point, dim3((unsigned)b.blocks[SB_STAGE_POINT]), dim3(5 * 64), 0, s, it,
+                     b.off[SB_STAGE_POINT], b.count);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int step_batch_matvec(const StepBatch &b, hipStream_t s) {
+  const auto *it = (const StepBatchItem *)b.items;
+  const dim3 g((unsigned)b.blocks[SB_STAGE_LINE]);
+  hipLaunchKernelGGL(kb_line_op, g, dim3(256), 0, s, it, b.off[SB_STAGE_LINE], b.count, 2, (int)SB_DV, (int)SB_T1);
+  hipLaunchKernelGGL(kb_line_op, g, dim3(256), 0, s, it, b.off[SB_STAGE_LINE], b.count, 1, (int)SB_T1, (int)SB_T2);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int step_batch_vec(const StepBatch &b, int op, hipStream_t s) {
+  hipLaunchKernelGGL(kb_vec, dim3((unsigned)b.blocks[SB_STAGE_VEC]), dim3(256), 0, s,
+                     (const StepBatchItem *)b.items, b.off[SB_STAGE_VEC], b.count, op);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int step_batch_itrbc(const StepBatch &b, hipStream_t s) {
+  const auto *it = (const StepBatchItem *)b.items;
+  hipLaunchKernelGGL(kb_itrbc_wall_top, dim3((unsigned)b.blocks[SB_STAGE_WALL]), dim3(128), 0, s, it,
+                     b.off[SB_STAGE_WALL], b.count);
+  hipLaunchKernelGGL(kb_itrbc_lr, dim3((unsigned)b.blocks[SB_STAGE_LR]), dim3(128), 0, s, it, b.off[SB_STAGE_LR],
+                     b.count);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }

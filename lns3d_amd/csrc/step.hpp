@@ -121,4 +121,34 @@ int launch_residual_norms(const void *b, void *y, long n, double *norms, hipStre
 int launch_axpy(void *x, const void *d, long n, hipStream_t s);   // x += d
 int launch_copy(void *dst, const void *src, long n, hipStream_t s);   // complex n values
 
+// batched steps: one launch per stage over the members of a batch (device
+// tables built once).  Buffers of a member: v, dv, rhs (the step's
+// residual), t1, t2 (the correction), work (U_xi)
+enum { SB_V = 0, SB_DV, SB_RHS, SB_T1, SB_T2, SB_WORK };
+// elementwise stages: dv = rhs; t2 = rhs - t2; dv += t2; dv += t2 then
+// v += dv; v += dv
+enum { SB_COPY = 0, SB_SUB, SB_ADD_DV, SB_UPDATE, SB_ADD_V };
+enum { SB_STAGE_D1X = 0, SB_STAGE_POINT, SB_STAGE_LINE, SB_STAGE_VEC, SB_STAGE_WALL, SB_STAGE_LR, SB_NSTAGE };
+struct StepBatchMember {
+  lns3d_params p;
+  Stencil st;
+  Coeffs c;
+  double2 *v, *dv, *rhs, *t1, *t2, *work;
+};
+struct StepBatch {
+  int count = 0;
+  void *items = nullptr;
+  int *off[SB_NSTAGE] = {};
+  long blocks[SB_NSTAGE] = {};
+};
+int step_batch_build(const StepBatchMember *m, int count, StepBatch &out);
+void step_batch_free(StepBatch &b);
+// rhs = the backward-Euler residual with the rhsBC3D rows (launch_rhs mode 1)
+int step_batch_rhs(const StepBatch &b, hipStream_t s);
+// t2 = Lx (Ly dv) through t1 (launch_af_matvec)
+int step_batch_matvec(const StepBatch &b, hipStream_t s);
+int step_batch_vec(const StepBatch &b, int op, hipStream_t s);
+// itrBC3D of v (launch_itrbc)
+int step_batch_itrbc(const StepBatch &b, hipStream_t s);
+
 }  // namespace lns3d

@@ -408,6 +408,9 @@ def set_line_variant(n: int, lines: int, precision: int, variant: int) -> None:
 
 
 MODE_FP64, MODE_FP32, MODE_REFINED = 0, 1, 2
+# the refined steps of MODE_REFINED with one launch per stage over all
+# contexts of the graph (instead of one branch per context)
+MODE_BATCHED = 3
 
 
 class Graph:

@@ -1,10 +1,12 @@
 """Batches, graphs, the refined step without host synchronisation, the
 autotuner and the serving loop, on the GPU.
 
-* A graph of several uneven instances (each on its own branch) gives the
-  same states, bit for bit, as the same steps launched one by one; it is
-  replayed several times with eager work and synchronisation between the
-  replays, and every replay is checked.
+* A graph of several uneven instances (each on its own branch, or every
+  stage one launch over all instances) gives the same states, bit for bit,
+  as the same steps launched one by one; it is replayed several times with
+  eager work and synchronisation between the replays, and every replay is
+  checked.  The staged graph groups the line solves by solve kernel: a batch
+  whose shapes are pinned to different kernels is checked too.
 * The refined step with one correction agrees with the complex-double step
   to 1e-10 relative, and its residual check reports <= 1e-10.
 * The autotuner returns a variant whose plans still solve like the oracle.
@@ -35,14 +37,27 @@ SHAPES = [dict(nx=33, ny=17, kz=0.3), dict(nx=47, ny=21, kz=0.8, Ma=0.7),
           dict(nx=65, ny=33, kz=0.1, Re=2500.0)]
 
 
-@pytest.mark.parametrize("mode", [0, 2])
-def test_graph_matches_eager(torch_cuda, mode):
+@pytest.mark.parametrize("mode,pin", [(0, None), (2, None), (3, None), (3, "mixed")])
+def test_graph_matches_eager(torch_cuda, mode, pin):
     from lns3d_amd.batch import Batch
     from lns3d_amd.config import Params
+    from lns3d_amd.native import lib
     torch = torch_cuda
-    print(f"graph vs eager mode={mode}", flush=True)
+    print(f"graph vs eager mode={mode} pin={pin}", flush=True)
     params = [Params(**kw) for kw in SHAPES]
+    pinned = []
+    if pin:
+        # xi lines: twisted pairs, twisted halves, plain; eta lines: three,
+        # two and one lines per workgroup (both precisions of a shape in
+        # the same family)
+        for p, (vx, vy) in zip(params, [(5, 4), (6, 3), (0, 0)]):
+            for prec in (0, 1):
+                pinned += [(p.nx, p.ny, prec), (p.ny, p.nx, prec)]
+                lib().lns3d_tune_set(p.nx, p.ny, prec, vx if prec else (6 if vx == 5 else vx))
+                lib().lns3d_tune_set(p.ny, p.nx, prec, vy)
     b = Batch(params, precision=1, seed=3)
+    for key in pinned:
+        lib().lns3d_tune_set(*key, -1)
     nsteps = 3
     # eager reference
     for k in range(len(params)):

@@ -7,9 +7,11 @@
   three steps; the refinement count of each step is printed.
 * 64 uneven instances (65x33 to 513x257, beta, Mach, Re drawn from the
   seed), line-solve kernels chosen by the autotuner, 20 refined steps of
-  every instance in one graph launch: every instance against 20
-  complex-double steps of the same instance launched one by one (1e-10),
-  and the three smallest instances against 20 of the oracle's steps."""
+  every instance in one graph launch with one launch per stage over all
+  instances: the same states, bit for bit, as the graph with one branch per
+  instance; every instance against 20 complex-double steps of the same
+  instance launched one by one (1e-10), and the three smallest instances
+  against 20 of the oracle's steps."""
 import numpy as np
 import pytest
 
@@ -76,7 +78,7 @@ def test_refined_1025x257_per_variable(torch_cuda, orc):
 def test_uneven_batch_64_instances(torch_cuda, orc):
     from lns3d_amd.batch import Batch, draw_instances, tune_shapes
     from lns3d_amd.config import make_grid, make_mean, make_disturbance
-    from lns3d_amd.native import MODE_REFINED
+    from lns3d_amd.native import MODE_REFINED, MODE_BATCHED
     torch = torch_cuda
     params = draw_instances(64, 0)
     print("batch of 64: tuning", flush=True)
@@ -84,10 +86,19 @@ def test_uneven_batch_64_instances(torch_cuda, orc):
     b = Batch(params, precision=1, seed=0)
     print(f"batch of 64: {b.points} points, capture", flush=True)
     nsteps = 20
-    g = b.capture(nsteps, mode=MODE_REFINED, corrections=1)
+    # one launch per stage (the benchmark's graph), then one branch per
+    # instance: the same states bit for bit
+    g = b.capture(nsteps, mode=MODE_BATCHED, corrections=1)
     g.launch()
     g.synchronize()
     got = [v.clone() for v in b.v]
+    g.close()
+    b.reset()
+    g = b.capture(nsteps, mode=MODE_REFINED, corrections=1)
+    g.launch()
+    g.synchronize()
+    for k in range(len(params)):
+        assert torch.equal(b.v[k], got[k]), (k, params[k])
     # the same instances, 20 complex-double steps each, one by one
     worst = 0.0
     for k, st in enumerate(b.steps):
