@@ -23,6 +23,8 @@ import time
 
 import numpy as np
 
+AFFINITY_AT_START = len(os.sched_getaffinity(0))
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -238,19 +240,30 @@ def main():
     # solved one at a time with complex-double factors and with complex-float
     # factors (time to solution of each path) ----
     if not args.no_sweep:
-        from lns3d_amd.serve import sweep, serial, draw_requests
+        from lns3d_amd.serve import sweep, serial, draw_requests, fifo_latencies
         nreq = args.requests
         ser64, _ = serial(draw_requests(nreq, None, 0), mode="fp64")
         ser32, _ = serial(draw_requests(nreq, None, 0), mode="fp32_outer")
         sw = sweep(count=nreq, seed=0, mode="fp32_outer")
-        rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+        rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()
+                         if k != "service_s"}
+
+        def fifo(ser):
+            # the same requests served one at a time in arrival order under
+            # the server's Poisson arrivals (measured service times)
+            lat = fifo_latencies(np.array(ser["service_s"]), np.array(sw["arrivals"]))
+            return {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+
         out["sweep"] = {"requests": nreq,
                         "request": "receptivity (ispg 2 sponge forcing, loctime 1, CFL 20 per 128 cells in x) to 1e-9 of "
-                                   "the first residual; grids 129x65..1025x257, Re U[50,150]",
+                                   "the first residual; grids 129x65..1025x257, Re U[50,150]; one request set for "
+                                   "every leg",
                         "mode": "FP32 factors, FP64 residual each step (outer refinement)",
                         "peak": rnd(sw["peak"]), "arrival_rate_s": round(sw["rate"], 3),
                         "poisson": rnd(sw["open_loop"]),
-                        "serial_fp64": rnd(ser64), "serial_fp32_outer": rnd(ser32),
+                        "serial_fp64": dict(rnd(ser64), same_arrivals=fifo(ser64)),
+                        "serial_fp32_outer": dict(rnd(ser32), same_arrivals=fifo(ser32)),
                         "time_to_solution_ratio_fp64_over_fp32":
                             round(ser32["solves_per_s"] / ser64["solves_per_s"], 3)}
 
@@ -317,8 +330,10 @@ def host_description():
                     numa[d] = f.read().strip()
     except OSError:
         pass
+    # the process's CPUs as it started (the OpenMP runtime binds the calling
+    # thread to one place when it starts, with OMP_PROC_BIND=close)
     return {"cpu_model": model, "logical_cpus": os.cpu_count(), "numa_nodes": numa,
-            "affinity_cpus": len(os.sched_getaffinity(0))}
+            "affinity_cpus": AFFINITY_AT_START}
 
 
 def _cpu_baseline(p, met, vm, v0, nsteps, warm=3):

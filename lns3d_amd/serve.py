@@ -117,6 +117,31 @@ def draw_requests(count: int, rate: Optional[float], seed: int = 0, grids=GRIDS,
     return out
 
 
+def with_arrivals(requests: List[Request], rate: float, seed: int = 0) -> List[Request]:
+    """the same requests with Poisson arrival times at `rate` per second,
+    drawn from their own seeded stream (so a request set keeps its
+    parameters whatever the arrival process)"""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for r in requests:
+        t += float(rng.exponential(1.0 / rate))
+        out.append(dataclasses.replace(r, arrival=t))
+    return out
+
+
+def fifo_latencies(service_s, arrivals):
+    """latencies of one-at-a-time service in arrival order: request k, with
+    service time service_s[k], arrives at arrivals[k] and starts when it has
+    arrived and the one before it is done"""
+    order = np.argsort(arrivals, kind="stable")
+    free, lat = 0.0, np.zeros(len(order))
+    for k in order:
+        start = max(arrivals[k], free)
+        free = start + service_s[k]
+        lat[k] = free - arrivals[k]
+    return lat
+
+
 class _Slot:
     """a pooled context with its stream, state buffers, per-step residual
     norms and chunk graph"""
@@ -377,13 +402,15 @@ def serial(requests: List[Request], mode: str = "fp64", chunk: int = 50):
     return dict(requests=len(times), solves_per_s=len(times) / t.sum(),
                 p50_ms=float(np.percentile(t, 50) * 1e3), p99_ms=float(np.percentile(t, 99) * 1e3),
                 converged=sum(1 for r in ok if r["converged"]),
-                mean_steps=float(np.mean([r["steps"] for r in ok])) if ok else 0.0), recs
+                mean_steps=float(np.mean([r["steps"] for r in ok])) if ok else 0.0,
+                service_s=[float(x) for x in t]), recs
 
 
 def sweep(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", load=0.8, chunk=50):
     """the serving benchmark: peak throughput with every request waiting at
-    t = 0 (closed loop), then Poisson arrivals at `load` x peak; returns both
-    summaries"""
+    t = 0 (closed loop), then the same requests arriving as a Poisson
+    process at `load` x peak; returns both summaries and the arrival times
+    (in request order)"""
     budget = int(budget_gb * 2**30)
     srv = Server(budget, mode=mode, chunk=chunk)
     # warm the pool and the graphs, then the closed-loop peak
@@ -393,9 +420,9 @@ def sweep(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", load=0.8, chunk=5
     recs = srv.run(reqs)
     peak = summarize(recs, time.perf_counter() - t0)
     rate = load * peak["solves_per_s"]
-    reqs = draw_requests(count, rate, seed + 2)
+    reqs = with_arrivals(reqs, rate, seed + 2)
     t0 = time.perf_counter()
     recs = srv.run(reqs)
     open_loop = summarize(recs, time.perf_counter() - t0)
     srv.close()
-    return dict(peak=peak, rate=rate, open_loop=open_loop)
+    return dict(peak=peak, rate=rate, open_loop=open_loop, arrivals=[r.arrival for r in reqs])

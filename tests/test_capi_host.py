@@ -153,6 +153,24 @@ def test_request_and_instance_draws():
     assert len({(p.nx, p.ny) for p in inst}) > 32
 
 
+def test_serving_arrivals_and_fifo():
+    """Poisson arrivals on a fixed request set keep its parameters, and the
+    one-at-a-time baseline queues requests in arrival order"""
+    from lns3d_amd.serve import draw_requests, with_arrivals, fifo_latencies
+    reqs = draw_requests(50, None, seed=4)
+    a = with_arrivals(reqs, 20.0, seed=9)
+    assert [(r.nx, r.ny, r.beta, r.Re) for r in a] == [(r.nx, r.ny, r.beta, r.Re) for r in reqs]
+    t = np.array([r.arrival for r in a])
+    assert np.all(np.diff(t) > 0) and abs(np.diff(np.r_[0.0, t]).mean() - 1 / 20.0) < 0.3 / 20.0
+    # arrivals 0, 1, 1.5 with service 2, 1, 0.25: the second waits until 2,
+    # the third until 3
+    lat = fifo_latencies(np.array([2.0, 1.0, 0.25]), np.array([0.0, 1.0, 1.5]))
+    assert np.allclose(lat, [2.0, 2.0, 1.75])
+    # out of order arrivals are served in arrival order
+    lat = fifo_latencies(np.array([1.0, 1.0]), np.array([5.0, 0.0]))
+    assert np.allclose(lat, [1.0, 1.0])
+
+
 @pytest.mark.parametrize("change,code,text", [
     (dict(Re=-1.0), 1, b"Re must be >= 0"),
     (dict(Ma=float("nan")), 1, b"must be finite"),
