@@ -931,14 +931,32 @@ __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v
 constexpr int RT = 64;
 constexpr int NDER = 17;
 
-__device__ __forceinline__ void rhs_point(long bid, const lns3d_params &p, const Stencil &st, const Coeffs &c,
-                                          const double2 *v, const double2 *c1a, double2 *r, int with_rbe,
-                                          const double2 *v2old, const double2 *vold, const double2 *forcing) {
+struct StepBatchItem;
+__device__ __forceinline__ int step_member(const int *off, int count);
+__device__ __forceinline__ void batch_point_args(const StepBatchItem *items, const int *off, int count,
+                                                 lns3d_params &p, Stencil &st, Coeffs &c, const double2 *&v,
+                                                 const double2 *&c1a, double2 *&r, long &tile);
+
+// BATCH: the kernel of a staged batch (the member's deck, coefficients and
+// buffers replace the arguments, from items; its workgroups are numbered
+// from off[member]); otherwise one context with its tiles numbered per XCD.
+// The single-context kernel keeps the arguments as kernel arguments: its
+// body moved into a device function was measured 11% slower (78 to 90
+// VGPRs).
+template <bool BATCH>
+__global__ __launch_bounds__(320) void k_rhs_point_t(lns3d_params p, Stencil st, Coeffs c, const double2 *v,
+                                                    const double2 *c1a, double2 *r, int with_rbe,
+                                                    const double2 *v2old, const double2 *vold,
+                                                    const double2 *forcing, const StepBatchItem *items,
+                                                    const int *off, int count) {
   using namespace st;
+  long tile;
+  if constexpr (BATCH) batch_point_args(items, off, count, p, st, c, v, c1a, r, tile);
+  else tile = xcd_tile(blockIdx.x, gridDim.x);   // tiles of one XCD: consecutive rows
   __shared__ double2 sv[5 * RT], sg1[5 * RT], sd[NDER * RT], sr[5 * RT];
   const int nx = p.nx, ny = p.ny;
   const long P = (long)nx * ny;
-  const long q0 = bid * RT;
+  const long q0 = tile * RT;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int npt = (int)min((long)RT, P - q0);
   if (tid < 5 * npt) {
@@ -1148,13 +1166,6 @@ __device__ __forceinline__ void rhs_point(long bid, const lns3d_params &p, const
     r[5 * q0 + tid] = o;
   }
 }
-__global__ __launch_bounds__(320) void k_rhs_point(lns3d_params p, Stencil st, Coeffs c,
-                                                  const double2 *v, const double2 *c1a, double2 *r,
-                                                  int with_rbe, const double2 *v2old,
-                                                  const double2 *vold, const double2 *forcing) {
-  // tiles of one XCD: consecutive rows
-  rhs_point(xcd_tile(blockIdx.x, gridDim.x), p, st, c, v, c1a, r, with_rbe, v2old, vold, forcing);
-}
 
 int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const void *v, void *r,
                void *work, int with_rbe, hipStream_t s, const void *v2old, const void *vold,
@@ -1162,9 +1173,9 @@ int launch_rhs(const lns3d_params &p, const Stencil &st, const Coeffs &c, const 
   long P = (long)p.nx * p.ny;
   hipLaunchKernelGGL(k_rhs_d1x, dim3(nblocks(5 * P, 256)), dim3(256), 0, s, p.nx, p.ny, p.xper, st,
                      (const double2 *)v, (double2 *)work);
-  hipLaunchKernelGGL(k_rhs_point, dim3(nblocks(P, RT)), dim3(5 * 64), 0, s, p, st, c, (const double2 *)v,
+  hipLaunchKernelGGL(k_rhs_point_t<false>, dim3(nblocks(P, RT)), dim3(5 * 64), 0, s, p, st, c, (const double2 *)v,
                      (const double2 *)work, (double2 *)r, with_rbe, (const double2 *)v2old,
-                     (const double2 *)vold, (const double2 *)forcing);
+                     (const double2 *)vold, (const double2 *)forcing, nullptr, nullptr, 0);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
@@ -1763,11 +1774,18 @@ __global__ void kb_rhs_d1x(const StepBatchItem *items, const int *off, int count
   rhs_d1x((long)blockIdx.x - off[k], b.p.nx, b.p.ny, b.p.xper, b.st, b.buf[SB_V], b.buf[SB_WORK]);
 }
 
-__global__ __launch_bounds__(320) void kb_rhs_point(const StepBatchItem *items, const int *off, int count) {
+__device__ __forceinline__ void batch_point_args(const StepBatchItem *items, const int *off, int count,
+                                                 lns3d_params &p, Stencil &st, Coeffs &c, const double2 *&v,
+                                                 const double2 *&c1a, double2 *&r, long &tile) {
   const int k = step_member(off, count);
   const StepBatchItem &b = items[k];
-  rhs_point((long)blockIdx.x - off[k], b.p, b.st, b.c, b.buf[SB_V], b.buf[SB_WORK], b.buf[SB_RHS], 1, nullptr,
-            nullptr, nullptr);
+  p = b.p;
+  st = b.st;
+  c = b.c;
+  v = b.buf[SB_V];
+  c1a = b.buf[SB_WORK];
+  r = b.buf[SB_RHS];
+  tile = (long)blockIdx.x - off[k];
 }
 
 // y = L_dir x with x, y chosen among the member's buffers
@@ -1879,8 +1897,9 @@ int step_batch_rhs(const StepBatch &b, hipStream_t s) {
   const auto *it = (const StepBatchItem *)b.items;
   hipLaunchKernelGGL(kb_rhs_d1x, dim3((unsigned)b.blocks[SB_STAGE_D1X]), dim3(256), 0, s, it, b.off[SB_STAGE_D1X],
                      b.count);
-  hipLaunchKernelGGL(kb_rhs_point, dim3((unsigned)b.blocks[SB_STAGE_POINT]), dim3(5 * 64), 0, s, it,
-                     b.off[SB_STAGE_POINT], b.count);
+  hipLaunchKernelGGL(k_rhs_point_t<true>, dim3((unsigned)b.blocks[SB_STAGE_POINT]), dim3(5 * 64), 0, s,
+                     lns3d_params{}, Stencil{}, Coeffs{}, nullptr, nullptr, nullptr, 1, nullptr, nullptr, nullptr,
+                     it, b.off[SB_STAGE_POINT], b.count);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
