@@ -37,7 +37,10 @@ const char *lns3d_last_error(void);
  * both through lns3d_set_inflow; right 4 with left 4 only; 7: symmetry,
  * src/lhsbs1f3d.f90 and the symmetric closures of src/cgrad.f90,
  * cgrad2.f90, grad.f90, grad2.f90; needs nx >= ny, since
- * the reference's symmetric rows read point i + ny j); linear = 2; mattyp in
+ * the reference's symmetric rows read point i + ny j); linear 1 or 2 (the
+ * deck's 2, complex analysis, is 1 once read, src/input.f90:367-371, and the
+ * 3D driver runs only with 1: the coefficients take the linear = 1 forms of
+ * src/genmtrx.f90:719-820); mattyp in
  * {0,1,2,3} (1: Sutherland's law, lns3d_set_sutherland); loctime in {0,1};
  * Navier-Stokes (viscous) boundary rows; the smoother eps_e through
  * lns3d_set_smoother; sponges through

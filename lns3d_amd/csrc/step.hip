@@ -268,8 +268,6 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
                          0.5 * (g11l[2] + g12l[1] + g22l[2] + g22l[2]), 0.5 * (g11l[3] + g22l[3])};
   const double Lapt = g11l[4] + g22l[4];
   const double divu = gu[0][0] + gu[1][1] + gu[2][2];
-  double gp[3];
-  for (int k = 0; k < 3; k++) gp[k] = fact * (grho[k] * t + rho * gt[k]);
   double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   S[0][0] = gu[0][0];
   S[1][0] = 0.5 * (gu[1][0] + gu[0][1]);
@@ -359,13 +357,13 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
     for (int r = 0; r < 5; r++)
       for (int cc = 0; cc < 5; cc++) dst[(long)(r + 5 * cc) * P + q] = H[r][cc];
   }
-  // D (src/genmtrx.f90:700-820)
+  // D (src/genmtrx.f90:700-820) in its linear = 1 forms (:719-723, 743-747,
+  // 767-771, 794-797, 817-820): the deck's linear = 2 is linear = 1 once read
+  // (src/input.f90:367-371), the only value the 3D driver runs with
   f1 = rhoinv / Re;
   D[0][0] = divu; D[0][1] = grho[0]; D[0][2] = grho[1]; D[0][3] = grho[2]; D[0][4] = 0;
   for (int k = 0; k < 3; k++) {
-    D[1 + k][0] = rhoinv * (gmsinv * gt[k] - rhoinv * gp[k] + f1 * (glm[k] * divu + lm * gdivu[k]) +
-                            f1 * 2.0 * (gmu[0] * S[k][0] + gmu[1] * S[k][1] + gmu[2] * S[k][2] +
-                                        mu * Sjj[k]));
+    D[1 + k][0] = rhoinv * (u1 * gu[k][0] + u2 * gu[k][1] + u3 * gu[k][2] + gmsinv * gt[k]);
     D[1 + k][1] = gu[k][0]; D[1 + k][2] = gu[k][1]; D[1 + k][3] = gu[k][2];
     D[1 + k][4] = rhoinv * gmsinv * grho[k] - f1 * (gdlm[k] * divu + dlm * gdivu[k]) -
                   f1 * 2.0 * (gdmu[0] * S[k][0] + gdmu[1] * S[k][1] + gdmu[2] * S[k][2] + dmu * Sjj[k]);
@@ -375,10 +373,9 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
   const double SS = S[0][0] * S[0][0] + S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][0] * S[1][0] +
                     S[1][1] * S[1][1] + S[1][2] * S[1][2] + S[2][0] * S[2][0] + S[2][1] * S[2][1] +
                     S[2][2] * S[2][2];
-  D[4][0] = f1 * rhoinv * (gcon[0] * gt[0] + gcon[1] * gt[1] + gcon[2] * gt[2] + con * Lapt) +
-            f2 * rhoinv * lm * (divu * divu) + 2.0 * f2 * rhoinv * mu * SS;
+  D[4][0] = rhoinv * (u1 * gt[0] + u2 * gt[1] + u3 * gt[2] + gamma1 * t * divu);
   D[4][1] = gt[0]; D[4][2] = gt[1]; D[4][3] = gt[2];
-  D[4][4] = gamma1 * divu;
+  D[4][4] = -gamma1 * rhoinv * (u1 * grho[0] + u2 * grho[1] + u3 * grho[2]);
   D[4][4] = D[4][4] - f1 * (gdcon[0] * gt[0] + gdcon[1] * gt[1] + gdcon[2] * gt[2] + dcon * Lapt) -
             f2 * dlm * (divu * divu) - 2.0 * f2 * dmu * SS;
   if (kz != 0.0) {

@@ -55,7 +55,11 @@ subroutine ref_init(ip, rp) bind(C, name="ref_init")
   kz = rp(5); omega = rp(6); cflmax = rp(7); dtmax = rp(8); eps_e = rp(9)
   x0 = rp(10)
   iLHS = 1; niter = 1; ires = 0; idamp = 0; general = .true.
+  ! as src/input.f90:367-371 does after reading the deck: the complex
+  ! analysis runs with linear = 1 (the only value src/lns3d.f90:153-167 lets
+  ! reach impdrv3d)
   complex_analysis = (kz /= zero) .or. (linear == 2)
+  if (linear == 2) linear = 1
   is = 1; ie = nx; js = 1; je = ny; updateLHS = .false.
   istep = 0; iter = 1; lstep = 0; alfa = one; time = zero
 
@@ -191,13 +195,21 @@ subroutine ref_set_smoother(eps, buffl) bind(C, name="ref_set_smoother")
 end subroutine ref_set_smoother
 
 ! coefficient generation (src/genmtrx.f90) and time step (src/temporal.f90)
+! genmtrx with linear = 1 and wall = 3 calls genbump (src/genmtrx.f90:100),
+! which forms the bump's wall values from a namelist file; here those values
+! are given through ref_set_bump, as the library takes them, so genmtrx runs
+! with the bump switched off (nothing else in it reads wall = 3)
 subroutine ref_genmtrx() bind(C, name="ref_genmtrx")
   use global
   implicit none
   real, allocatable :: rl(:,:,:)
+  integer :: wall_deck
   allocate(rl(ndof,nx,ny))
   rl = zero
+  wall_deck = wall
+  if (wall == 3) wall = 1
   call genmtrx(vm, kz, rl)
+  wall = wall_deck
   deallocate(rl)
 end subroutine ref_genmtrx
 
