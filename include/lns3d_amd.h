@@ -63,7 +63,10 @@ typedef struct {
    * reference's default for each */
   int wall_is, wall_ie; /* wall extent is:ie, 1-based (src/lns3d.f90:73
                          * default 1:nx); 0, 0 = the whole wall */
-  int idamp;        /* viscous damper (src/input.f90:268, src/damper.f90) */
+  int idamp;        /* viscous damper (src/input.f90:268, src/damper.f90): 1
+                     * zeroes the outflow column's mean xi second derivatives
+                     * and Vh11 (src/genmtrx.f90:87-96, 877-881); 2 has no
+                     * effect on this path */
   int use_calcd;    /* useCalcd (src/global.f90:34): the D terms alternate
                      * between the factors (src/lhs1f3d.f90:27-37) */
   int update_lhs;   /* updateLHS (src/global.f90:33): factors formed again
@@ -78,11 +81,11 @@ typedef struct {
  * iteration, as the build with USE_CPENTA1BC does (src/impdrv3d.f90:248,310);
  * the factors change only with useCalcd's D-term placement, so they are
  * formed again only then.
- * Not implemented, rejected with LNS3D_EUNSUPPORTED: idamp != 0, use_calcd
- * with left or right = 7, and precision = 1 with the inviscid deck or
- * update_lhs.  Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, Re = 0
- * with mattyp != 0 and a wall extent outside 1 <= is <= ie <= nx are
- * rejected with LNS3D_EINVAL. */
+ * Not implemented, rejected with LNS3D_EUNSUPPORTED: use_calcd with left
+ * or right = 7, and precision = 1 with the inviscid deck or update_lhs.
+ * Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, Re = 0 with
+ * mattyp != 0, idamp outside 0..2 and a wall extent outside
+ * 1 <= is <= ie <= nx are rejected with LNS3D_EINVAL. */
 
 /* ======================================================================
  * Block-pentadiagonal line solvers, drop-in form (host arrays).

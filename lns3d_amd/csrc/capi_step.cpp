@@ -37,13 +37,10 @@ static int check_params(const lns3d_params &p) {
   if (p.wall_is < 0 || p.wall_ie < 0 || p.wall_is > p.nx || p.wall_ie > p.nx ||
       ((p.wall_is ? p.wall_is : 1) > (p.wall_ie ? p.wall_ie : p.nx)))
     return fail(LNS3D_EINVAL, "wall extent: 1 <= wall_is <= wall_ie <= nx (0 for the default)");
-  // idamp = 1 replaces the pressure gradient of the outflow column by calcp's
-  // gpr (src/genmtrx.f90:97,273-275), whose second component src/calcp.f90
-  // :23-27 never sets (both assignments write gpr(1,i,j)): the reference's
-  // result is undefined there
-  if (p.idamp != 0)
-    return fail(LNS3D_EUNSUPPORTED, "idamp != 0: the reference's damper reads gpr(2,:,:), which src/calcp.f90 "
-                                    "never sets (genmtrx.f90:273-275); not implemented");
+  // idamp (src/input.f90:268, src/damper.f90): 1 damps the outflow column's
+  // mean xi second derivatives and Vh11 (src/genmtrx.f90:87-96, 877-881);
+  // 2 only forms damp, which this path never reads
+  if (p.idamp < 0 || p.idamp > 2) return fail(LNS3D_EINVAL, "idamp must be 0, 1 or 2 (src/damper.f90:26-47)");
   if ((p.use_calcd != 0 && p.use_calcd != 1) || (p.update_lhs != 0 && p.update_lhs != 1))
     return fail(LNS3D_EINVAL, "use_calcd and update_lhs are 0 or 1");
   if (p.use_calcd && !p.xper && (p.left == 7 || p.right == 7))

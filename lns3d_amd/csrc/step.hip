@@ -241,6 +241,10 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
     g12[e] = d1<double, double>(ny, p.yper, j, st.gy, st.dyinv, colg + e, 5L * nx);
   }
   if (!c.bnb && (p.wallt == 1 || p.wallt == 2) && j == 0) g2[4] = 0.0;   // src/gradbc.f90:15 (Navier)
+  // idamp = 1: the damper's zero on the last column times the xi second
+  // derivatives of the mean (src/damper.f90:26-29, src/genmtrx.f90:87-96)
+  if (p.idamp == 1 && i == nx - 1)
+    for (int e = 0; e < 5; e++) g11[e] = 0.0 * g11[e];
   const double M1 = met[q], M2 = met[P + q], N1 = met[2 * P + q], N2 = met[3 * P + q];
   const double m11 = met[4 * P + q], m12 = met[5 * P + q], m22 = met[6 * P + q];
   const double n11 = met[7 * P + q], n12 = met[8 * P + q], n22 = met[9 * P + q];
@@ -396,6 +400,8 @@ __global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const do
   double V22[6] = {Vij[0] * n1n1 + Vij[1] * n2n2, Vij[1] * n1n1 + Vij[0] * n2n2,
                    Vij[1] * n1n1 + Vij[1] * n2n2, Vij[2] * n1n1 + Vij[2] * n2n2,
                    Vij[3] * n1n2, Vij[3] * n1n2};
+  if (p.idamp == 1 && i == nx - 1)        // src/genmtrx.f90:877-881
+    for (int e = 0; e < 6; e++) V11[e] = 0.0 * V11[e];
   for (int e = 0; e < 6; e++) {
     c.V11[e * P + q] = V11[e];
     c.V12[e * P + q] = V12[e];

@@ -39,6 +39,8 @@ typedef struct {
   int wall_is, wall_ie; /* wall extent is:ie, 1-based (src/lns3d.f90:42,73); 0: 1, nx */
   int d_in_eta;       /* 1: the D terms in the eta factor and the sponge in the xi
                        * factor (useCalcd's first factors, src/lhs1f3d.f90:27-37) */
+  int idamp;          /* 1: the outflow column's damper (src/damper.f90:26-29,
+                       * src/genmtrx.f90:87-99,877-881); 0, 2: none on this path */
 } orc_params;
 /* Re = 0 with mattyp = 0 is the inviscid deck (src/input.f90:31-36:
  * Navier = .false., Re = 1e99): rotated wall rows with the wall normals of

@@ -28,7 +28,7 @@ class _OrcParams(ctypes.Structure):
                  "left", "right", "linear", "mattyp", "loctime")] + \
                [(n, ctypes.c_double) for n in
                 ("Ma", "Re", "Pr", "gamma", "kz", "omega", "cflmax", "dtmax", "alfa")] + \
-               [(n, ctypes.c_int) for n in ("periodic_exact", "wall_is", "wall_ie", "d_in_eta")]
+               [(n, ctypes.c_int) for n in ("periodic_exact", "wall_is", "wall_ie", "d_in_eta", "idamp")]
 
 
 class _OrcSponge(ctypes.Structure):
@@ -291,7 +291,8 @@ class Reference:
         inviscid = p.Re == 0.0 and p.mattyp == 0
         ip = np.array([p.nx, p.ny, p.optx, p.opty, 0, p.xper, p.yper, lsym, rsym, p.wall,
                        p.wallt, p.top, p.left, p.right, p.linear, impl, p.loctime, p.mattyp,
-                       ispg, 0 if inviscid else 1, usecalcd or getattr(p, "use_calcd", 0)],
+                       ispg, 0 if inviscid else 1, usecalcd or getattr(p, "use_calcd", 0),
+                       getattr(p, "idamp", 0)],
                       dtype=np.int32)
         rp = np.array([p.Ma, 1.0e99 if inviscid else p.Re, p.Pr, p.gamma, p.kz, p.omega, p.cflmax,
                        p.dtmax, 0.0, x0])

@@ -29,7 +29,7 @@ end module refdrv_state
 !-----------------------------------------------------------------------------
 ! ip: 1 nx, 2 ny, 3 optx, 4 opty, 5 carp, 6 xper, 7 yper, 8 lsym, 9 rsym,
 !     10 wall, 11 wallt, 12 top, 13 left, 14 right, 15 linear, 16 impl,
-!     17 loctime, 18 mattyp, 19 ispg, 20 navier, 21 usecalcd
+!     17 loctime, 18 mattyp, 19 ispg, 20 navier, 21 usecalcd, 22 idamp
 ! rp: 1 Ma, 2 Re, 3 Pr, 4 gamma, 5 kz, 6 omega, 7 cflmax, 8 dtmax, 9 eps_e,
 !     10 x0
 !-----------------------------------------------------------------------------
@@ -39,8 +39,9 @@ subroutine ref_init(ip, rp) bind(C, name="ref_init")
   use local2d
   use local3d
   use refdrv_state
+  use pot, only: g1p
   implicit none
-  integer(c_int), intent(in) :: ip(21)
+  integer(c_int), intent(in) :: ip(22)
   real(c_double), intent(in) :: rp(10)
   integer :: i, j
 
@@ -54,7 +55,7 @@ subroutine ref_init(ip, rp) bind(C, name="ref_init")
   Ma = rp(1); Re = rp(2); Pr = rp(3); gamma = rp(4); gamma1 = gamma - one
   kz = rp(5); omega = rp(6); cflmax = rp(7); dtmax = rp(8); eps_e = rp(9)
   x0 = rp(10)
-  iLHS = 1; niter = 1; ires = 0; idamp = 0; general = .true.
+  iLHS = 1; niter = 1; ires = 0; idamp = ip(22); general = .true.
   ! as src/input.f90:367-371 does after reading the deck: the complex
   ! analysis runs with linear = 1 (the only value src/lns3d.f90:153-167 lets
   ! reach impdrv3d)
@@ -65,6 +66,19 @@ subroutine ref_init(ip, rp) bind(C, name="ref_init")
 
   dxi = one / float(nx-1)
   deta = one / float(ny-1)
+
+  ! the damper (src/lns3d.f90:140 calls init_damper after the deck): damp is
+  ! zero on the outflow column for idamp = 1.  genmtrx then calls calcp,
+  ! whose gpr only the nonlinear forms read (src/genmtrx.f90:273, 646) but
+  ! which reads the potential-flow gradient g1p, allocated only when that
+  ! input was read (src/potential.f90:78): zeros stand in for it here
+  if (idamp == 1) then
+    if (allocated(damp)) deallocate(damp)
+    call init_damper
+    if (allocated(g1p)) deallocate(g1p)
+    allocate(g1p(max(nx,ny), max(nx,ny)))
+    g1p = zero
+  end if
 
   if (allocated(x)) then
     deallocate(x, y, m1, m2, n1, n2, m11, m12, m22, n11, n12, n22)

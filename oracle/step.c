@@ -262,6 +262,7 @@ static int supported(const orc_params *p) {
    * (src/input.f90:367-371), and the 3D driver runs only with linear = 1
    * (src/lns3d.f90:153-167): both decks take the linear = 1 forms */
   if (p->linear != 1 && p->linear != 2) return 0;
+  if (p->idamp < 0 || p->idamp > 2) return 0;
   if (p->mattyp < 0 || p->mattyp > 3) return 0;
   return 1;
 }
@@ -1081,6 +1082,12 @@ int orc_genmtrx(const orc_params *p, const double *vm, const double *metric, orc
   orc_grad2(p, 5, vm, g1, g11, g12, g22);
   if (navier(p) && (p->wallt == 1 || p->wallt == 2))     /* src/gradbc.f90:15, src/genmtrx.f90:82 */
     for (int i = 0; i < nx; i++) g2[4 + 5 * i] = 0;
+  /* idamp = 1: damp = 0 on the last column (src/damper.f90:26-29) times the
+   * xi second derivatives of the mean (src/genmtrx.f90:87-96); calcp's gpr
+   * enters only the nonlinear forms (:273, 646) */
+  if (p->idamp == 1)
+    for (int j = 0; j < ny; j++)
+      for (int d = 0; d < 5; d++) g11[5 * (nx - 1 + (long)nx * j) + d] = 0.0 * g11[5 * (nx - 1 + (long)nx * j) + d];
   const double *m1 = metric, *m2 = metric + P, *n1 = metric + 2 * P, *n2 = metric + 3 * P,
                *m11 = metric + 4 * P, *m12 = metric + 5 * P, *m22 = metric + 6 * P,
                *n11 = metric + 7 * P, *n12 = metric + 8 * P, *n22 = metric + 9 * P;
@@ -1252,6 +1259,8 @@ int orc_genmtrx(const orc_params *p, const double *vm, const double *metric, orc
     V11[3] = Vij[2] * m1m1 + Vij[2] * m2m2;
     V11[4] = Vij[3] * m1m2;
     V11[5] = Vij[3] * m1m2;
+    if (p->idamp == 1 && q % nx == nx - 1)      /* src/genmtrx.f90:877-881 */
+      for (int k = 0; k < 6; k++) V11[k] = 0.0 * V11[k];
     V12[0] = 2.0 * (Vij[0] * m1n1 + Vij[1] * m2n2);
     V12[1] = 2.0 * (Vij[1] * m1n1 + Vij[0] * m2n2);
     V12[2] = 2.0 * (Vij[1] * m1n1 + Vij[1] * m2n2);

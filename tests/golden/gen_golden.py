@@ -62,7 +62,7 @@ reference's outputs:
     python tests/golden/gen_golden.py forced     (the shift-invert steps only)
     python tests/golden/gen_golden.py bump       (the bump wall cases only)
     python tests/golden/gen_golden.py sutherland (the Sutherland case only)
-    python tests/golden/gen_golden.py extras     (Re = 0, the wall extent, useCalcd, updateLHS)
+    python tests/golden/gen_golden.py extras     (Re = 0, the wall extent, useCalcd, updateLHS, idamp)
 """
 from __future__ import annotations
 
@@ -508,6 +508,9 @@ EXTRA_CASES = {
     # with updateLHS the factors are formed again at every step, alternating
     "calcd": dict(use_calcd=1),
     "calcd_update": dict(use_calcd=1, update_lhs=1),
+    # idamp = 1: the outflow column's mean xi second derivatives and Vh11
+    # damped; its rows are the operator's only with the symmetric end
+    "idamp": dict(idamp=1, wall=1, wallt=1, right=7),
 }
 
 

@@ -179,7 +179,7 @@ def test_serving_arrivals_and_fifo():
     (dict(wall_is=5, wall_ie=3), 1, b"wall extent"),
     (dict(wall_is=2, wall_ie=34), 1, b"wall extent"),
     (dict(wall_is=-1), 1, b"wall extent"),
-    (dict(idamp=1), 4, b"idamp"),
+    (dict(idamp=3), 1, b"idamp"),
     (dict(use_calcd=2), 1, b"use_calcd"),
     (dict(use_calcd=1, left=7), 4, b"useCalcd"),
     (dict(Re=0.0, precision=1), 4, b"inviscid"),

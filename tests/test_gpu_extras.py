@@ -10,7 +10,9 @@ wall normals are not along y):
 * the wall extent is:ie (src/lns3d.f90:73) of the viscous and the bump wall;
 * useCalcd (src/lhs1f3d.f90:27-37: the D terms in the eta factor, the sponge
   in the xi factor, alternating with istep - 1 + iter) and updateLHS
-  (src/impdrv3d.f90:248,310: the factors formed again each step).
+  (src/impdrv3d.f90:248,310: the factors formed again each step);
+* idamp = 1 (src/damper.f90:26-29, src/genmtrx.f90:87-96, 877-881: the
+  outflow column's mean xi second derivatives and Vh11 zeroed).
 
 lrhs3D, itrBC3D, the assembled factors (where they do not depend on the step)
 and two or three implicit steps, 1e-12 relative in the max norm."""
@@ -24,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update"]
+CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp"]
 
 
 def rel(a, b):
