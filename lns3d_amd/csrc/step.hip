@@ -220,7 +220,7 @@ __device__ __forceinline__ void getmat(int mattyp, double S0, double t, double &
   else { lm = -st::pt66 * mu; dlm = -st::pt66 * dmu; d2lm = -st::pt66 * d2mu; }
 }
 
-__global__ void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const double *met,
+__global__ __launch_bounds__(128) void k_genmtrx(lns3d_params p, Stencil st, const double *vm, const double *met,
                           const double *g1a, Coeffs c) {
   // set-up kernel: rounded operation by operation (no fused multiply-add) so
   // the coefficients follow the reference's arithmetic; several of them are
@@ -858,7 +858,7 @@ static AsmCtx make_asm(const lns3d_params &p, const Coeffs &c, int dir) {
 // into the working copy of the line solver: line-interleaved, element
 // ((row*5 + k)*25 + r*5 + cc)*L + line (periodic lines), or line-major,
 // element ((line*n + row)*5 + k)*25 + r*5 + cc
-__global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, int line_major, z *lu) {
+__global__ __launch_bounds__(128) void k_assemble_lines(AsmCtx A, Coeffs c, int dir, int line_major, z *lu) {
   const int nx = A.p.nx, ny = A.p.ny;
   const long L = dir == 1 ? ny : nx, n = dir == 1 ? nx : ny;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -874,7 +874,7 @@ __global__ void k_assemble_lines(AsmCtx A, Coeffs c, int dir, int line_major, z 
       }
 }
 
-__global__ void k_assemble_ref(AsmCtx A, Coeffs c, int dir, double2 *mat) {
+__global__ __launch_bounds__(128) void k_assemble_ref(AsmCtx A, Coeffs c, int dir, double2 *mat) {
   const int nx = A.p.nx, ny = A.p.ny;
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long)nx * ny) return;
@@ -1666,7 +1666,7 @@ __device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs 
     y[5 * q + r] = make_double2(out[r].re, out[r].im);
   }
 }
-__global__ void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
+__global__ __launch_bounds__(256) void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {
   line_op(xcd_tile(blockIdx.x, gridDim.x), A, c, dir, x, y);
 }
 
@@ -1792,7 +1792,8 @@ __device__ __forceinline__ void batch_point_args(const StepBatchItem *items, con
 }
 
 // y = L_dir x with x, y chosen among the member's buffers
-__global__ void kb_line_op(const StepBatchItem *items, const int *off, int count, int dir, int xs, int ys) {
+__global__ __launch_bounds__(256) void kb_line_op(const StepBatchItem *items, const int *off, int count, int dir,
+                                                  int xs, int ys) {
   const int k = step_member(off, count);
   const StepBatchItem &b = items[k];
   line_op((long)blockIdx.x - off[k], dir == 1 ? b.ax : b.ay, b.c, dir, b.buf[xs], b.buf[ys]);
