@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep leg")
     ap.add_argument("--requests", type=int, default=16, help="requests in the serving sweep")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the host-array drop-in timings")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch 5 xi solves, 5 eta solves, 5 steps and a calibration copy of "
                          "one solve's algorithmic bytes, then exit (for rocprofv3 --pmc runs)")
@@ -225,6 +226,11 @@ def main():
                               "one_correction_resid": res_fix,
                               "fp32_no_refine_steps_s": round(args.steps / t_f32, 3),
                               "ratio_to_fp64": round((args.steps / t_fix) / steps_s, 3)}
+
+    # ---- the drop-in line solves with host arrays (the reference's call
+    # sites): the assembled blocks and the right-hand side cross PCIe ----
+    if not args.no_dropin:
+        out["dropin"] = dropin_rates(st, p)
     st.close()
 
     # ---- uneven batch: 64 instances x 20 refined steps in one graph launch ----
@@ -271,6 +277,41 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, met, vm, v0, args.cpu_steps)
     print(json.dumps(out), flush=True)
+
+
+def dropin_rates(st, p, nsolve=5):
+    """lns3d_cpenta2bc (xi lines) and lns3d_cpenta1bc (eta lines) called as
+    the reference calls cpenta2bc / cpenta1bc, with host arrays: code 0 copies
+    the assembled blocks (2000 B per point) and factors, code 1 copies the
+    right-hand side in and the solution out (80 B per point each way) around
+    the device solve.  Times per call (median of nsolve code-1 calls)"""
+    import torch
+    from lns3d_amd.native import lib, check
+    L = lib()
+    nx, ny = p.nx, p.ny
+    out = {}
+    rng = np.random.default_rng(5)
+    for name, d, fn in (("cpenta2bc_xi", 1, L.lns3d_cpenta2bc), ("cpenta1bc_eta", 2, L.lns3d_cpenta1bc)):
+        m_dev = torch.zeros(125 * nx * ny, dtype=torch.complex128, device="cuda")
+        st.assemble(d, m_dev)
+        torch.cuda.synchronize()
+        mat = m_dev.cpu().numpy()
+        del m_dev
+        rhs = (rng.standard_normal(5 * nx * ny) + 1j * rng.standard_normal(5 * nx * ny)).astype(np.complex128)
+        t0 = time.perf_counter()
+        check(fn(nx, ny, 5, mat.ctypes.data, rhs.ctypes.data, 0), name)
+        t_factor = time.perf_counter() - t0
+        ts = []
+        for _ in range(nsolve):
+            t0 = time.perf_counter()
+            check(fn(nx, ny, 5, mat.ctypes.data, rhs.ctypes.data, 1), name)
+            ts.append(time.perf_counter() - t0)
+        check(L.lns3d_release_factors(), "lns3d_release_factors")
+        out[name] = {"factor_call_ms": round(t_factor * 1e3, 3),
+                     "solve_call_ms": round(float(np.median(ts)) * 1e3, 3),
+                     "host_bytes_factor": 2000 * nx * ny, "host_bytes_solve": 160 * nx * ny}
+        del mat
+    return out
 
 
 def _dir_solver(st, which):

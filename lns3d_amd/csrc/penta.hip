@@ -1721,7 +1721,7 @@ __device__ __noinline__ void records_per_row(LineGeom g, cx<double> *mat, cx<dou
   lmul(Lb, n - 1, i < n - 3 ? i : -1, P2 + 25, 5);
 }
 
-__global__ void k_records_per(LineGeom g, cx<double> *mat, cx<double> *per, cx<double> *per2,
+__global__ __launch_bounds__(128) void k_records_per(LineGeom g, cx<double> *mat, cx<double> *per, cx<double> *per2,
                               cx<double> *fwd, cx<double> *bwd, cx<double> *p2, cx<double> *qp) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < g.L * g.n) records_per_row(g, mat, per, per2, fwd, bwd, p2, qp, idx);
