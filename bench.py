@@ -246,11 +246,12 @@ def main():
     # solved one at a time with complex-double factors and with complex-float
     # factors (time to solution of each path) ----
     if not args.no_sweep:
-        from lns3d_amd.serve import sweep, serial, draw_requests, fifo_latencies
+        from lns3d_amd.serve import sweep, peak, serial, draw_requests, fifo_latencies
         nreq = args.requests
         ser64, _ = serial(draw_requests(nreq, None, 0), mode="fp64")
         ser32, _ = serial(draw_requests(nreq, None, 0), mode="fp32_outer")
         sw = sweep(count=nreq, seed=0, mode="fp32_outer")
+        pk_b = peak(count=nreq, seed=0, mode="fp32_outer", batched=True)
         rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()
                          if k != "service_s"}
 
@@ -267,6 +268,7 @@ def main():
                                    "every leg",
                         "mode": "FP32 factors, FP64 residual each step (outer refinement)",
                         "peak": rnd(sw["peak"]), "arrival_rate_s": round(sw["rate"], 3),
+                        "peak_batched_per_grid": rnd(pk_b),
                         "poisson": rnd(sw["open_loop"]),
                         "serial_fp64": dict(rnd(ser64), same_arrivals=fifo(ser64)),
                         "serial_fp32_outer": dict(rnd(ser32), same_arrivals=fifo(ser32)),

@@ -391,9 +391,12 @@ int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
  * (lns3d_step_fp32), 2 complex-float factors with `corrections` defect
  * corrections (lns3d_step_refined_fixed): every context on its own branch.
  * mode 3: the steps of mode 2 with one launch per stage over all contexts
- * (residual, line solves grouped by solve kernel, corrections, update),
- * the same arithmetic; non-periodic lines, viscous decks, no recorded
- * residual norms (LNS3D_EUNSUPPORTED otherwise).  dv_dev may be NULL.
+ * (residual, residual norms, line solves grouped by solve kernel,
+ * corrections, update), the same arithmetic; with corrections = 0 the steps
+ * of mode 1.  Non-periodic lines and viscous decks (LNS3D_EUNSUPPORTED
+ * otherwise).  Contexts with a residual output
+ * (lns3d_ctx_set_residual_output) record each step's norm in their next
+ * slot, with the bits of the per-context steps.  dv_dev may be NULL.
  * Launch on `stream` (NULL: the graph's own stream).  No counterpart in the
  * reference. */
 typedef struct lns3d_graph lns3d_graph;

@@ -932,9 +932,8 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
                                 int corrections, lns3d_graph **out) {
   for (int i = 0; i < count; i++) {
     lns3d_ctx *c = ctxs[i];
-    if (c->px32.periodic || c->py32.periodic || !c->navier || c->norm_out)
-      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create mode 3: periodic lines, the inviscid deck and "
-                                      "recorded residual norms take mode 2");
+    if (c->px32.periodic || c->py32.periodic || !c->navier)
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create mode 3: periodic lines and the inviscid deck take mode 2");
   }
   auto *gr = new lns3d_graph;
   std::vector<StepBatchMember> m((size_t)count);
@@ -953,6 +952,9 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
     a.t1 = (double2 *)c->t1;
     a.t2 = (double2 *)c->t2;
     a.work = (double2 *)c->work;
+    a.norm = c->norm_out;
+    a.norm_count = c->norm_count;
+    a.norm_idx = c->norm_idx;
     px[(size_t)i] = &c->px32;
     py[(size_t)i] = &c->py32;
     r0[(size_t)i] = a.dv;
@@ -972,6 +974,7 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
   const hipStream_t s = gr->s;
   for (int k = 0; k < nsteps && !rc; k++) {
     rc = step_batch_rhs(gr->sb, s);
+    if (!rc) rc = step_batch_norm(gr->sb, k, s);
     if (!rc) rc = step_batch_vec(gr->sb, SB_COPY, s);
     if (!rc) rc = penta_batch_solve(gr->gx, 0, s);
     if (!rc) rc = penta_batch_solve(gr->gy, 0, s);
@@ -997,6 +1000,9 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
     graph_free(gr);
     return rc;
   }
+  // the graph's steps took the next nsteps norm slots, as mode 2's do
+  for (int i = 0; i < count; i++)
+    if (ctxs[i]->norm_out) ctxs[i]->norm_idx += nsteps;
   *out = gr;
   return 0;
 }

@@ -977,7 +977,7 @@ __device__ __forceinline__ void compute_pass(int nchunk, Body body) {
   }
 }
 
-// the solve of workgroup bid (lines bid G .. bid G + G - 1)
+// the solve of workgroup bid (lines bid G .. bid G + cnt - 1, cnt <= G)
 template <typename T, int R, int G>
 __device__ __forceinline__ void solve_bc(long bid, int n, long L, const cx<T> *__restrict__ fwd,
                                          const cx<T> *__restrict__ bwd, const cx<T> *__restrict__ tail,
@@ -987,15 +987,16 @@ __device__ __forceinline__ void solve_bc(long bid, int n, long L, const cx<T> *_
   __shared__ __attribute__((aligned(16))) cx<T> rec[NBUF][G][BUFE];
   __shared__ double2 bst[NBUF][G][R][5];    // rhs rows, then c (or d) in place
   const long line0 = bid * G;
+  const int cnt = (int)min((long)G, L - line0);
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  // G lines per workgroup; DPP row r (16 lanes) of the compute wave runs the
-  // recurrence of line g = r mod G (rows past the last line repeat it and
-  // store the same values).  Lane m of a row owns component m (lanes past 4
-  // duplicate component 4).
-  const int g = (lane >> 4) % G;
+  // cnt lines in this workgroup; DPP row r (16 lanes) of the compute wave
+  // runs the recurrence of line g = r mod cnt (rows past the last line repeat
+  // one and store the same values).  Lane m of a row owns component m (lanes
+  // past 4 duplicate component 4).  LDS slots past cnt are not loaded.
+  const int g = (lane >> 4) % cnt;
   const int m = min(lane & 15, 4);
-  auto line_of = [&](int gg) { return min(line0 + gg, L - 1); };
+  auto line_of = [&](int gg) { return line0 + min(gg, cnt - 1); };
   const long line = line_of(g);
   const int nchunk = (n + R - 1) / R;
   double2 *X = rhs + line * sL * 5;
@@ -1022,6 +1023,7 @@ __device__ __forceinline__ void solve_bc(long bid, int n, long L, const cx<T> *_
       chunk(k, fw, lo, nr);
 #pragma unroll
       for (int gg = 0; gg < G; gg++) {
+        if (gg >= cnt) break;
         const long r0 = line_of(gg) * n + lo;
         if (fw) dma_rows(fwd + r0 * 60, rec[buf][gg], nr * 60, lane);
         else dma_rows(bwd + r0 * 66, rec[buf][gg], nr * 66, lane);
@@ -1030,7 +1032,7 @@ __device__ __forceinline__ void solve_bc(long bid, int n, long L, const cx<T> *_
 #pragma unroll
       for (int b = 0; b * 64 < G * R * 5; b++) {
         const int e = b * 64 + lane, gg = e / (R * 5), j = (e / 5) % R, q = e % 5;
-        if (e < G * R * 5 && j < nr)
+        if (e < G * R * 5 && gg < cnt && j < nr)
           __builtin_amdgcn_global_load_lds(
               (const void *)(rhs + line_of(gg) * sL * 5 + (long)(lo + j) * rs + q),
               (lds_void *)(dst + b * 64), 16, 0, 0);
@@ -1042,7 +1044,7 @@ __device__ __forceinline__ void solve_bc(long bid, int n, long L, const cx<T> *_
       chunk(k, fw, lo, nr);
       for (int r0 = 0; r0 < G * R; r0 += 12) {     // 12 rows (60 lanes) at a time
         const int rr = r0 + lane / 5, gg = rr / R, j = rr % R;
-        if (lane < 60 && rr < G * R && j < nr)
+        if (lane < 60 && rr < G * R && gg < cnt && j < nr)
           parallel_part(bst[buf][gg][j], rec[buf][gg] + j * (fw ? 60 : 66), lane % 5, fw);
       }
     };

@@ -1340,11 +1340,13 @@ __global__ void k_copy2(double2 *dst, const double2 *src, long n) {
 
 // the shift-invert driver's residual norm terms (src/si_eigdrv3d_new.f90
 // :360-372): sum of |r|^2 / dtl^2 over the points of each block
-__global__ __launch_bounds__(256) void k_forced_norm(const double2 *r, const double *dtl, long P,
-                                                     double *part) {
+// block blk of nblk (256 threads) of sum |r|^2 / dtl^2 over P points: points
+// blk 256 + t, stepping by nblk 256, then a tree over the block; every
+// thread returns the block's sum
+__device__ __forceinline__ double norm_block(long blk, long nblk, const double2 *r, const double *dtl, long P) {
   __shared__ double red[256];
   double acc = 0.0;
-  for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < P; q += (long)gridDim.x * 256) {
+  for (long q = blk * 256 + threadIdx.x; q < P; q += nblk * 256) {
     const double d2 = dtl[q] * dtl[q];
     for (int e = 0; e < 5; e++) {
       const double2 v = r[5 * q + e];
@@ -1357,7 +1359,13 @@ __global__ __launch_bounds__(256) void k_forced_norm(const double2 *r, const dou
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  return red[0];
+}
+
+__global__ __launch_bounds__(256) void k_forced_norm(const double2 *r, const double *dtl, long P,
+                                                     double *part) {
+  const double t = norm_block(blockIdx.x, gridDim.x, r, dtl, P);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // src/cgrad.f90 on a complex field v(5,nx,ny): g1 = d/dxi v, g2 = d/deta v,
@@ -1410,7 +1418,7 @@ int launch_cgrad2(int nx, int ny, int xper, int yper, const Stencil &st, const v
 
 // sqrt of the sum of nb partials, in a fixed order (one block): the same
 // bits on every run
-__global__ __launch_bounds__(256) void k_norm_final(const double *part, int nb, double *out) {
+__device__ __forceinline__ double norm_sum(const double *part, int nb) {
   __shared__ double red[256];
   double acc = 0.0;
   for (int k = threadIdx.x; k < nb; k += 256) acc += part[k];
@@ -1420,7 +1428,12 @@ __global__ __launch_bounds__(256) void k_norm_final(const double *part, int nb, 
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sqrt(red[0]);
+  return red[0];
+}
+
+__global__ __launch_bounds__(256) void k_norm_final(const double *part, int nb, double *out) {
+  const double t = norm_sum(part, nb);
+  if (threadIdx.x == 0) *out = sqrt(t);
 }
 
 int launch_residual_norm(const double *dtl, const void *r, long P, double *part, double *out, hipStream_t s) {
@@ -1758,6 +1771,9 @@ struct StepBatchItem {
   AsmCtx ax, ay;
   double2 *buf[6];     // v, dv, rhs, t1, t2, work
   long P;
+  double *norm;        // residual-norm slots (or null), as StepBatchMember
+  int norm_count;
+  long norm_idx;
 };
 
 __device__ __forceinline__ int step_member(const int *off, int count) {
@@ -1837,6 +1853,25 @@ __global__ void kb_itrbc_lr(const StepBatchItem *items, const int *off, int coun
   itrbc_lr((long)blockIdx.x - off[k], b.p, b.c.fside, b.buf[SB_V]);
 }
 
+// the residual norm of the members with a norm output: the blocks of a
+// member are those of launch_residual_norm for its P, then one block per
+// member sums its partials in the order k_norm_final does
+__global__ __launch_bounds__(256) void kb_norm_part(const StepBatchItem *items, const int *off, int count,
+                                                    double *part) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  const double t = norm_block((long)blockIdx.x - off[k], off[k + 1] - off[k], b.buf[SB_RHS], b.c.dtl, b.P);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void kb_norm_final(const StepBatchItem *items, const int *off, const double *part,
+                                                     long step) {
+  const StepBatchItem &b = items[blockIdx.x];
+  if (!b.norm) return;
+  const double t = norm_sum(part + off[blockIdx.x], off[blockIdx.x + 1] - off[blockIdx.x]);
+  if (threadIdx.x == 0) b.norm[(b.norm_idx + step) % b.norm_count] = sqrt(t);
+}
+
 // workgroups of each stage for a member
 static long stage_blocks(int stage, const lns3d_params &p) {
   const long P = (long)p.nx * p.ny;
@@ -1870,8 +1905,13 @@ int step_batch_build(const StepBatchMember *m, int count, StepBatch &out) {
     it.buf[SB_T2] = a.t2;
     it.buf[SB_WORK] = a.work;
     it.P = (long)a.p.nx * a.p.ny;
+    it.norm = a.norm;
+    it.norm_count = a.norm ? a.norm_count : 0;
+    it.norm_idx = a.norm_idx;
+    if (a.norm && a.norm_count < 1) return set_error(__FILE__, __LINE__, "batched steps: a norm output of no slots");
     for (int sg = 0; sg < SB_NSTAGE; sg++) {
-      const long nb = stage_blocks(sg, a.p);
+      const long nb = sg != SB_STAGE_NORM ? stage_blocks(sg, a.p)
+                                          : (a.norm ? std::min<long>(nblocks(it.P, 256), NORM_BLOCKS) : 0);
       if ((long)off[sg].back() + nb > 0x7fffffffL) return set_error(__FILE__, __LINE__, "batch too large");
       off[sg].push_back(off[sg].back() + (int)nb);
     }
@@ -1884,6 +1924,8 @@ int step_batch_build(const StepBatchMember *m, int count, StepBatch &out) {
     LNS3D_CHECK(hipMalloc(&out.off[sg], off[sg].size() * sizeof(int)));
     LNS3D_CHECK(hipMemcpy(out.off[sg], off[sg].data(), off[sg].size() * sizeof(int), hipMemcpyHostToDevice));
   }
+  if (out.blocks[SB_STAGE_NORM] > 0)
+    LNS3D_CHECK(hipMalloc(&out.part, out.blocks[SB_STAGE_NORM] * sizeof(double)));
   return 0;
 }
 
@@ -1893,7 +1935,9 @@ void step_batch_free(StepBatch &b) {
     if (b.off[sg]) (void)hipFree(b.off[sg]);
     b.off[sg] = nullptr;
   }
+  if (b.part) (void)hipFree(b.part);
   b.items = nullptr;
+  b.part = nullptr;
   b.count = 0;
 }
 
@@ -1920,6 +1964,17 @@ int step_batch_matvec(const StepBatch &b, hipStream_t s) {
 int step_batch_vec(const StepBatch &b, int op, hipStream_t s) {
   hipLaunchKernelGGL(kb_vec, dim3((unsigned)b.blocks[SB_STAGE_VEC]), dim3(256), 0, s,
                      (const StepBatchItem *)b.items, b.off[SB_STAGE_VEC], b.count, op);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
+int step_batch_norm(const StepBatch &b, long k, hipStream_t s) {
+  if (b.blocks[SB_STAGE_NORM] == 0) return 0;
+  const auto *it = (const StepBatchItem *)b.items;
+  hipLaunchKernelGGL(kb_norm_part, dim3((unsigned)b.blocks[SB_STAGE_NORM]), dim3(256), 0, s, it,
+                     b.off[SB_STAGE_NORM], b.count, b.part);
+  hipLaunchKernelGGL(kb_norm_final, dim3((unsigned)b.count), dim3(256), 0, s, it, b.off[SB_STAGE_NORM],
+                     (const double *)b.part, k);
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }

@@ -128,18 +128,25 @@ enum { SB_V = 0, SB_DV, SB_RHS, SB_T1, SB_T2, SB_WORK };
 // elementwise stages: dv = rhs; t2 = rhs - t2; dv += t2; dv += t2 then
 // v += dv; v += dv
 enum { SB_COPY = 0, SB_SUB, SB_ADD_DV, SB_UPDATE, SB_ADD_V };
-enum { SB_STAGE_D1X = 0, SB_STAGE_POINT, SB_STAGE_LINE, SB_STAGE_VEC, SB_STAGE_WALL, SB_STAGE_LR, SB_NSTAGE };
+enum { SB_STAGE_D1X = 0, SB_STAGE_POINT, SB_STAGE_LINE, SB_STAGE_VEC, SB_STAGE_WALL, SB_STAGE_LR, SB_STAGE_NORM,
+       SB_NSTAGE };
+// norm (optional): the member's residual-norm output of norm_count slots;
+// step k of a captured sequence writes slot (norm_idx + k) mod norm_count
 struct StepBatchMember {
   lns3d_params p;
   Stencil st;
   Coeffs c;
   double2 *v, *dv, *rhs, *t1, *t2, *work;
+  double *norm = nullptr;
+  int norm_count = 0;
+  long norm_idx = 0;
 };
 struct StepBatch {
   int count = 0;
   void *items = nullptr;
   int *off[SB_NSTAGE] = {};
   long blocks[SB_NSTAGE] = {};
+  double *part = nullptr;    // per-block partial sums of the norm stage
 };
 int step_batch_build(const StepBatchMember *m, int count, StepBatch &out);
 void step_batch_free(StepBatch &b);
@@ -150,5 +157,8 @@ int step_batch_matvec(const StepBatch &b, hipStream_t s);
 int step_batch_vec(const StepBatch &b, int op, hipStream_t s);
 // itrBC3D of v (launch_itrbc)
 int step_batch_itrbc(const StepBatch &b, hipStream_t s);
+// the residual norm sqrt(sum |rhs|^2 / dtl^2) of every member with a norm
+// output, into its slot for step k (the bits of launch_residual_norm)
+int step_batch_norm(const StepBatch &b, long k, hipStream_t s);
 
 }  // namespace lns3d

@@ -33,6 +33,19 @@ for the host's convergence test.  A request whose set-up fails (a refused
 parameter, a singular factor) or whose residual stops being finite is
 recorded as failed and its slot freed; the others continue.
 
+Batched per grid (Server(batched=True)): the requests of one grid share a
+staged graph, one launch per stage over all of them (lns3d_graph_create
+mode 3, residual norms recorded per request), captured again when a request
+joins or leaves (about 1-2 ms of host time); the grids' cohorts run on
+streams of their own.  The chunk discipline is the same, so a request takes
+the same steps and reaches the same state either way.  Measured on the
+serving mixes here, the two choices serve at the same rate: with a graph
+per request, four hardware queues already overlap the small requests'
+latency-bound launches, and both are then held by the line solves' two
+workgroups per CU.  The staged graph makes one launch per stage for all of
+a grid's requests instead of one per request (2.5x faster under a kernel
+tracer, which adds a cost per launch).
+
 Results do not depend on what else runs: every request's arithmetic is its
 own (own buffers, deterministic kernels and reductions), which the GPU test
 checks bit for bit.
@@ -48,7 +61,7 @@ from typing import List, Optional
 import numpy as np
 
 from .config import Params, disturbance_envelope, make_disturbance_device, make_grid, make_mean
-from .native import Graph, Step, ctx_bytes, LNS3DError, MODE_FP64, MODE_FP32, MODE_REFINED
+from .native import Graph, Step, ctx_bytes, LNS3DError, MODE_FP64, MODE_FP32, MODE_REFINED, MODE_BATCHED
 
 GRIDS = ((129, 65), (257, 129), (513, 257), (1025, 257))
 # Pseudo-time CFL of a request (local time steps): 20 per 128 cells along x.
@@ -165,8 +178,9 @@ class _Slot:
         self.req = None
         self.admitted = 0.0
 
-    def start(self, req: Request, grid_cache):
-        """set-up and the first two chunks (the caller handles failures)"""
+    def start(self, req: Request, grid_cache, own_graph: bool = True):
+        """set-up and, with its own graph, the first two chunks (the caller
+        handles failures); without, a cohort launches its chunks"""
         import torch
         p = req.params()
         x, y, met, envelope = grid_cache(p)
@@ -180,6 +194,17 @@ class _Slot:
         with torch.cuda.stream(self.stream):
             self.v.copy_(make_disturbance_device(envelope, req.seed))
         self.st.itrbc3d(self.v)
+        self.r0 = None
+        self.steps = 0            # steps whose norms were examined
+        self.launched = 0         # chunks launched
+        self.converged_at = None
+        self.final_ratio = None
+        self.failed = None
+        self.inflight = collections.deque()
+        self.queued = 0           # chunks of a cohort's launches not yet examined
+        self.done = False
+        if not own_graph:
+            return
         # kernel arguments passed by value (grid, scheme and boundary
         # selectors) are frozen in the graph: capture again when they change
         key = tuple(getattr(p, f) for f in STRUCTURAL)
@@ -191,13 +216,6 @@ class _Slot:
             self.graph = Graph([self.st], [self.v], [self.dv], nsteps=self.chunk, mode=self.mode,
                                corrections=self.corrections)
             self.graph_key = key
-        self.r0 = None
-        self.steps = 0            # steps whose norms were examined
-        self.launched = 0         # chunks launched
-        self.converged_at = None
-        self.final_ratio = None
-        self.failed = None
-        self.inflight = collections.deque()
         self._launch()
         self._launch()
 
@@ -212,27 +230,33 @@ class _Slot:
         self.inflight.append((ev, buf))
         self.launched += 1
 
+    def examine(self, nrm) -> bool:
+        """the convergence test over one chunk's norms; True once done"""
+        if self.r0 is None:
+            self.r0 = float(nrm[0])
+        for k in range(self.chunk):
+            ratio = float(nrm[k]) / self.r0 if self.r0 > 0 else 0.0
+            if not math.isfinite(ratio):
+                self.failed = "non-finite residual"
+                return True
+            if ratio <= self.req.tol:
+                self.converged_at = self.steps + k + 1
+                self.final_ratio = ratio
+                return True
+        self.steps += self.chunk
+        self.final_ratio = float(nrm[-1]) / self.r0 if self.r0 > 0 else 0.0
+        return self.steps >= self.req.max_steps
+
+    def wants_chunk(self) -> bool:
+        return not self.done and self.launched * self.chunk < self.req.max_steps
+
     def poll(self) -> bool:
         """examine the chunks that have finished; True once the request is done"""
         while self.inflight and self.inflight[0][0].query():
             _, buf = self.inflight.popleft()
-            nrm = buf.numpy()
-            if self.r0 is None:
-                self.r0 = float(nrm[0])
-            for k in range(self.chunk):
-                ratio = float(nrm[k]) / self.r0 if self.r0 > 0 else 0.0
-                if not math.isfinite(ratio):
-                    self.failed = "non-finite residual"
-                    return True
-                if ratio <= self.req.tol:
-                    self.converged_at = self.steps + k + 1
-                    self.final_ratio = ratio
-                    return True
-            self.steps += self.chunk
-            self.final_ratio = float(nrm[-1]) / self.r0 if self.r0 > 0 else 0.0
-            if self.steps >= self.req.max_steps:
+            if self.examine(buf.numpy()):
                 return True
-            if self.launched * self.chunk < self.req.max_steps:
+            if self.wants_chunk():
                 self._launch()
         return False
 
@@ -240,7 +264,7 @@ class _Slot:
         """every chunk still queued for a finished request has run"""
         while self.inflight and self.inflight[0][0].query():
             self.inflight.popleft()
-        return not self.inflight
+        return not self.inflight and self.queued == 0
 
     def drain(self):
         self.stream.synchronize()
@@ -252,16 +276,117 @@ class _Slot:
         self.st.close()
 
 
+class _Cohort:
+    """the requests of one grid and precision stepped together: a staged
+    graph (one launch per stage over every member, lns3d_graph_create mode
+    3) of `chunk` steps, captured again whenever the members change.
+
+    Two launches stay queued.  A member takes part in every launch from the
+    first after it joins until the one queued when its norms showed it done,
+    which is the chunk count its own graph would give: its chunk k + 1 is
+    queued before chunk k is examined, and no later chunk follows.  Replaced
+    graphs are destroyed when the serving run ends (freeing their tables
+    waits for the whole device)."""
+
+    def __init__(self, key, mode: int, chunk: int, corrections: int):
+        import torch
+        self.key = key
+        self.mode, self.chunk, self.corrections = mode, chunk, corrections
+        self.stream = torch.cuda.Stream()
+        self.members: List[_Slot] = []      # in the next launch
+        self.graph = None
+        self.graph_members = ()
+        self.old = []                       # replaced graphs
+        self.inflight = collections.deque() # (event, [(slot, host buffer)])
+        self.rebuilds = 0
+        self.rebuild_s = 0.0                # host time capturing graphs
+
+    def join(self, s: _Slot):
+        self.members.append(s)
+        if not self.inflight:
+            self._fill()
+
+    def _launch(self):
+        import torch
+        members = tuple(self.members)
+        if members != self.graph_members:
+            t0 = time.perf_counter()
+            self.retire_graph()
+            for m in members:
+                m.st.set_residual_output(m.norms, self.chunk)
+            self.graph = Graph([m.st for m in members], [m.v for m in members], [m.dv for m in members],
+                               nsteps=self.chunk, mode=MODE_BATCHED, corrections=self.corrections)
+            self.graph_members = members
+            self.rebuilds += 1
+            self.rebuild_s += time.perf_counter() - t0
+        for m in members:
+            self.stream.wait_stream(m.stream)    # a new member's set-up
+        self.graph.launch(self.stream)
+        entries = []
+        with torch.cuda.stream(self.stream):
+            for m in members:
+                buf = m.hnorms[m.launched % 2]
+                buf.copy_(m.norms, non_blocking=True)
+                m.launched += 1
+                m.queued += 1
+                entries.append((m, buf))
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        self.inflight.append((ev, entries))
+
+    def _fill(self):
+        self.members = [m for m in self.members if m.wants_chunk()]
+        while self.members and len(self.inflight) < 2:
+            self._launch()
+
+    def poll(self) -> List[_Slot]:
+        """examine the launches that have finished; returns the members
+        found done"""
+        out = []
+        while self.inflight and self.inflight[0][0].query():
+            _, entries = self.inflight.popleft()
+            for m, buf in entries:
+                m.queued -= 1
+                if not m.done and m.examine(buf.numpy()):
+                    m.done = True
+                    out.append(m)
+            self._fill()
+        return out
+
+    def retire_graph(self):
+        """the current graph is not launched again"""
+        if self.graph is not None:
+            self.old.append(self.graph)
+        self.graph, self.graph_members = None, ()
+
+    def release(self):
+        """destroy replaced graphs once nothing of the cohort runs (their
+        tables are freed, which waits for the device)"""
+        self.stream.synchronize()
+        for g in self.old:
+            g.close()
+        self.old = []
+
+    def close(self):
+        self.retire_graph()
+        self.release()
+
+
 class Server:
-    """budget_bytes: device memory the pooled contexts may hold"""
+    """budget_bytes: device memory the pooled contexts may hold.  batched:
+    the requests of one grid share a staged graph (one launch per stage over
+    all of them); otherwise each request has its own graph and stream.  The
+    complex-double mode is never batched."""
 
     def __init__(self, budget_bytes: int, mode: str = "fp32_outer", chunk: int = 50,
-                 corrections: int = 1, keep_states: bool = False):
+                 corrections: int = 1, keep_states: bool = False, batched: bool = False):
         self.budget = budget_bytes
         self.precision, self.gmode = MODES[mode]
         self.mode_name, self.chunk, self.corrections = mode, chunk, corrections
         self.keep_states = keep_states
+        self.batched = batched and self.gmode != MODE_FP64
         self.pool: List[_Slot] = []
+        self.cohorts = {}
         self._grids = {}
 
     def _grid(self, p):
@@ -293,11 +418,20 @@ class Server:
             if not idle:
                 return None
             victim = max(idle, key=lambda s: s.bytes)
-            victim.close()
-            self.pool.remove(victim)
+            self._drop(victim)
         s = _Slot(req.params(), self.precision, self.gmode, self.chunk, self.corrections)
         self.pool.append(s)
         return s
+
+    def _drop(self, s: _Slot):
+        """close a pooled context.  A cohort graph that holds it is never
+        launched again (its members differ from the cohort's from now on) and
+        is destroyed with the cohort's other replaced graphs."""
+        c = self.cohorts.get(s.key)
+        if c is not None and s in c.graph_members:
+            c.retire_graph()
+        s.close()
+        self.pool.remove(s)
 
     def _retire(self, s: _Slot, now: float, out: dict):
         r = s.req
@@ -321,10 +455,16 @@ class Server:
         retiring: List[_Slot] = []    # done; a queued chunk may still run on its stream
         while pending or active or retiring:
             now = time.perf_counter() - t0
-            for s in list(active):
-                if s.poll():
-                    active.remove(s)
-                    retiring.append(s)
+            if self.batched:
+                for c in self.cohorts.values():
+                    for s in c.poll():
+                        active.remove(s)
+                        retiring.append(s)
+            else:
+                for s in list(active):
+                    if s.poll():
+                        active.remove(s)
+                        retiring.append(s)
             for s in list(retiring):
                 if s.drained():
                     self._retire(s, time.perf_counter() - t0, out)
@@ -343,7 +483,7 @@ class Server:
                 r = pending.pop(0)
                 s.admitted = time.perf_counter() - t0
                 try:
-                    s.start(r, self._grid)
+                    s.start(r, self._grid, own_graph=not self.batched)
                 except LNS3DError as e:
                     # this request alone fails: record it, drop its slot (its
                     # context may be half set up), keep serving the others
@@ -351,17 +491,26 @@ class Server:
                     out[r.rid] = dict(rid=r.rid, nx=r.nx, ny=r.ny, arrival=r.arrival, error=str(e),
                                       done=time.perf_counter() - t0, converged=False)
                     s.req = None
-                    s.close()
-                    self.pool.remove(s)
+                    self._drop(s)
                     continue
+                if self.batched:
+                    if s.key not in self.cohorts:
+                        self.cohorts[s.key] = _Cohort(s.key, self.gmode, self.chunk,
+                                                      0 if self.gmode == MODE_FP32 else self.corrections)
+                    self.cohorts[s.key].join(s)
                 active.append(s)
             if not active and not retiring and pending:
                 time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
             else:
                 time.sleep(poll_s)
+        for c in self.cohorts.values():
+            c.release()
         return [out[k] for k in sorted(out)]
 
     def close(self):
+        for c in self.cohorts.values():
+            c.close()
+        self.cohorts = {}
         for s in self.pool:
             s.close()
         self.pool = []
@@ -406,13 +555,28 @@ def serial(requests: List[Request], mode: str = "fp64", chunk: int = 50):
                 service_s=[float(x) for x in t]), recs
 
 
-def sweep(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", load=0.8, chunk=50):
+def peak(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", chunk=50, batched=True):
+    """closed-loop peak throughput of the requests of `sweep` (after a
+    warm-up run), with the server's batching choice; adds the batched
+    server's graph captures (count and host seconds)"""
+    srv = Server(int(budget_gb * 2**30), mode=mode, chunk=chunk, batched=batched)
+    srv.run(draw_requests(min(count, 4), None, seed + 1, max_steps=200))
+    t0 = time.perf_counter()
+    recs = srv.run(draw_requests(count, None, seed))
+    out = summarize(recs, time.perf_counter() - t0)
+    out["graph_captures"] = sum(c.rebuilds for c in srv.cohorts.values())
+    out["capture_s"] = sum(c.rebuild_s for c in srv.cohorts.values())
+    srv.close()
+    return out
+
+
+def sweep(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", load=0.8, chunk=50, batched=False):
     """the serving benchmark: peak throughput with every request waiting at
     t = 0 (closed loop), then the same requests arriving as a Poisson
     process at `load` x peak; returns both summaries and the arrival times
     (in request order)"""
     budget = int(budget_gb * 2**30)
-    srv = Server(budget, mode=mode, chunk=chunk)
+    srv = Server(budget, mode=mode, chunk=chunk, batched=batched)
     # warm the pool and the graphs, then the closed-loop peak
     srv.run(draw_requests(min(count, 4), None, seed + 1, max_steps=200))
     reqs = draw_requests(count, None, seed)

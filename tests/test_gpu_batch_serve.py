@@ -12,9 +12,13 @@ autotuner and the serving loop, on the GPU.
 * The autotuner returns a variant whose plans still solve like the oracle.
 * Serving: receptivity requests converge to 1e-9 of their first residual;
   a request gives the same state alone and inside a mix of overlapping
-  requests (bit for bit); the complex-float mode reaches the complex-double
-  march's steady state; a refused request fails alone; a memory budget that
-  holds one context at a time still completes every request."""
+  requests (bit for bit); requests batched per grid in staged graphs give
+  the states of a graph per request (bit for bit); the complex-float mode
+  reaches the complex-double march's steady state; a refused request fails
+  alone; a memory budget that holds one context at a time still completes
+  every request.
+* The staged graph records each instance's residual norms with the bits of
+  the per-instance steps."""
 import numpy as np
 import pytest
 
@@ -75,6 +79,44 @@ def test_graph_matches_eager(torch_cuda, mode, pin):
             assert torch.equal(b.v[k], ref[k]), (rep, k)
         del junk
     g.close()
+    b.close()
+
+
+def test_staged_norms_match_branches(torch_cuda):
+    """the staged graph's complex-float steps (corrections = 0) with a
+    residual output per instance: the states and every recorded norm equal,
+    bit for bit, those of the same steps on one branch per instance, on two
+    consecutive launches (the second continues the slots)"""
+    from lns3d_amd.batch import Batch
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Graph, MODE_FP32, MODE_BATCHED
+    torch = torch_cuda
+    params = [Params(**kw) for kw in SHAPES]
+    b = Batch(params, precision=1, seed=5)
+    nsteps, slots = 3, 8
+    out = {}
+    for mode in (MODE_FP32, MODE_BATCHED):
+        print(f"norms mode={mode}", flush=True)
+        b.reset()
+        norms = [torch.full((slots,), -1.0, dtype=torch.float64, device="cuda") for _ in params]
+        for st, n in zip(b.steps, norms):
+            st.set_residual_output(n, slots)
+        g = Graph(b.steps, b.v, b.dv, nsteps=nsteps, mode=mode, corrections=0)
+        g.launch()
+        g.synchronize()
+        g2 = Graph(b.steps, b.v, b.dv, nsteps=nsteps, mode=mode, corrections=0)   # slots 3..5
+        g2.launch()
+        g2.synchronize()
+        out[mode] = ([v.clone() for v in b.v], [n.clone() for n in norms])
+        for st in b.steps:
+            st.set_residual_output(None, 0)
+        g.close()
+        g2.close()
+    (va, na), (vb, nb) = out[MODE_FP32], out[MODE_BATCHED]
+    for k in range(len(params)):
+        assert torch.equal(va[k], vb[k]), k
+        assert torch.equal(na[k], nb[k]), (k, na[k], nb[k])
+        assert bool((na[k][:2 * nsteps] > 0).all()) and bool((na[k][2 * nsteps:] == -1.0).all()), na[k]
     b.close()
 
 
@@ -153,6 +195,30 @@ def test_serve_alone_and_in_mix(torch_cuda):
     assert np.all(np.isfinite(got["state"]))
 
 
+def test_serve_batched_matches_unbatched(torch_cuda):
+    """requests of two grids arriving while others run: served with one
+    staged graph per grid (rebuilt as requests come and go) they reach the
+    same states, bit for bit, in the same number of steps, as with a graph
+    and stream per request"""
+    from lns3d_amd.serve import Server
+    grids = ((129, 65), (257, 129))
+    reqs = _reqs(6, 60.0, 13, grids, (1, 1), max_steps=3000)
+    out = {}
+    for batched in (False, True):
+        print(f"serve: batched={batched}", flush=True)
+        srv = Server(64 * 2**30, keep_states=True, batched=batched)
+        out[batched] = srv.run(reqs)
+        if batched:
+            reb = {k: (c.rebuilds, round(c.rebuild_s, 3)) for k, c in srv.cohorts.items()}
+            print(f"  graph rebuilds (count, s) per grid: {reb}", flush=True)
+        srv.close()
+    for a, b in zip(out[False], out[True]):
+        print(f"  rid {a['rid']} {a['nx']}x{a['ny']}: steps {a['steps']} / {b['steps']}", flush=True)
+        assert a["error"] is None and b["error"] is None
+        assert a["steps"] == b["steps"] and a["converged"] == b["converged"]
+        assert np.array_equal(a["state"], b["state"]), a["rid"]
+
+
 def test_serve_matches_complex_double_march(torch_cuda):
     """per grid, a request served with complex-float factors (the default
     mode) converges to the steady state of the complex-double march (the
@@ -174,14 +240,15 @@ def test_serve_matches_complex_double_march(torch_cuda):
         assert e < 1e-6
 
 
-def test_serve_isolates_a_failing_request(torch_cuda):
+@pytest.mark.parametrize("batched", [False, True])
+def test_serve_isolates_a_failing_request(torch_cuda, batched):
     """a request the library refuses fails alone: the others complete with
     the states they reach without it"""
     from lns3d_amd.serve import Server
     grids, w = ((129, 65), (257, 129)), (1, 1)
     reqs = _reqs(4, None, 7, grids, w, max_steps=400)
     bad = dataclass_replace(reqs[1], Re=-1.0)            # refused by lns3d_ctx_set_params
-    srv = Server(64 * 2**30, keep_states=True)
+    srv = Server(64 * 2**30, keep_states=True, batched=batched)
     recs = srv.run([reqs[0], bad, reqs[2], reqs[3]])
     assert recs[1]["error"] and "Re must be" in recs[1]["error"]
     solo = {r.rid: srv.run([r])[0] for r in (reqs[0], reqs[2], reqs[3])}
