@@ -252,6 +252,12 @@ def main():
         ser32, _ = serial(draw_requests(nreq, None, 0), mode="fp32_outer")
         sw = sweep(count=nreq, seed=0, mode="fp32_outer")
         pk_b = peak(count=nreq, seed=0, mode="fp32_outer", batched=True)
+        # many small requests: 2 nreq on the two smaller grids, with a graph
+        # per request and batched per grid with three lines per workgroup
+        small = dict(count=2 * nreq, seed=0, mode="fp32_outer", grids=((129, 65), (257, 129)),
+                     grid_weights=(0.6, 0.4))
+        pk_s0 = peak(batched=False, **small)
+        pk_s1 = peak(batched=True, small_line_variant=4, **small)
         rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()
                          if k != "service_s"}
 
@@ -269,6 +275,9 @@ def main():
                         "mode": "FP32 factors, FP64 residual each step (outer refinement)",
                         "peak": rnd(sw["peak"]), "arrival_rate_s": round(sw["rate"], 3),
                         "peak_batched_per_grid": rnd(pk_b),
+                        "small_grids_peak": {"requests": 2 * nreq, "grids": "129x65 (0.6), 257x129 (0.4)",
+                                             "graph_per_request": rnd(pk_s0),
+                                             "batched_per_grid_three_lines_per_workgroup": rnd(pk_s1)},
                         "poisson": rnd(sw["open_loop"]),
                         "serial_fp64": dict(rnd(ser64), same_arrivals=fifo(ser64)),
                         "serial_fp32_outer": dict(rnd(ser32), same_arrivals=fifo(ser32)),

@@ -61,7 +61,8 @@ from typing import List, Optional
 import numpy as np
 
 from .config import Params, disturbance_envelope, make_disturbance_device, make_grid, make_mean
-from .native import Graph, Step, ctx_bytes, LNS3DError, MODE_FP64, MODE_FP32, MODE_REFINED, MODE_BATCHED
+from .native import (Graph, Step, ctx_bytes, set_line_variant, LNS3DError, MODE_FP64, MODE_FP32, MODE_REFINED,
+                     MODE_BATCHED)
 
 GRIDS = ((129, 65), (257, 129), (513, 257), (1025, 257))
 # Pseudo-time CFL of a request (local time steps): 20 per 128 cells along x.
@@ -379,8 +380,16 @@ class Server:
     complex-double mode is never batched."""
 
     def __init__(self, budget_bytes: int, mode: str = "fp32_outer", chunk: int = 50,
-                 corrections: int = 1, keep_states: bool = False, batched: bool = False):
+                 corrections: int = 1, keep_states: bool = False, batched: bool = False,
+                 small_line_variant: Optional[int] = None):
         self.budget = budget_bytes
+        # small_line_variant: the line-solve variant (lns3d_tune_set) for
+        # the contexts of grids of at most 257 rows in either direction, in
+        # both precisions, pinned for the server's life and dropped at
+        # close (e.g. 4, three lines per workgroup: more lines in flight
+        # when many small requests run together, longer chains per line)
+        self.small_line_variant = small_line_variant
+        self._pinned = set()
         self.precision, self.gmode = MODES[mode]
         self.mode_name, self.chunk, self.corrections = mode, chunk, corrections
         self.keep_states = keep_states
@@ -419,6 +428,13 @@ class Server:
                 return None
             victim = max(idle, key=lambda s: s.bytes)
             self._drop(victim)
+        if self.small_line_variant is not None and max(req.nx, req.ny) <= 257:
+            for shape in ((req.nx, req.ny), (req.ny, req.nx)):
+                for prec in (0, 1):
+                    key3 = shape + (prec,)
+                    if key3 not in self._pinned:
+                        set_line_variant(shape[0], shape[1], prec, self.small_line_variant)
+                        self._pinned.add(key3)
         s = _Slot(req.params(), self.precision, self.gmode, self.chunk, self.corrections)
         self.pool.append(s)
         return s
@@ -514,6 +530,9 @@ class Server:
         for s in self.pool:
             s.close()
         self.pool = []
+        for n, lines, prec in self._pinned:
+            set_line_variant(n, lines, prec, -1)
+        self._pinned = set()
 
 
 def summarize(records, t_total=None):
@@ -555,14 +574,16 @@ def serial(requests: List[Request], mode: str = "fp64", chunk: int = 50):
                 service_s=[float(x) for x in t]), recs
 
 
-def peak(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", chunk=50, batched=True):
+def peak(count=24, seed=0, budget_gb=64.0, mode="fp32_outer", chunk=50, batched=True, small_line_variant=None,
+         grids=GRIDS, grid_weights=(0.4, 0.3, 0.2, 0.1)):
     """closed-loop peak throughput of the requests of `sweep` (after a
-    warm-up run), with the server's batching choice; adds the batched
-    server's graph captures (count and host seconds)"""
-    srv = Server(int(budget_gb * 2**30), mode=mode, chunk=chunk, batched=batched)
-    srv.run(draw_requests(min(count, 4), None, seed + 1, max_steps=200))
+    warm-up run), with the server's batching and variant choices; adds the
+    batched server's graph captures (count and host seconds)"""
+    srv = Server(int(budget_gb * 2**30), mode=mode, chunk=chunk, batched=batched,
+                 small_line_variant=small_line_variant)
+    srv.run(draw_requests(min(count, 4), None, seed + 1, grids=grids, grid_weights=grid_weights, max_steps=200))
     t0 = time.perf_counter()
-    recs = srv.run(draw_requests(count, None, seed))
+    recs = srv.run(draw_requests(count, None, seed, grids=grids, grid_weights=grid_weights))
     out = summarize(recs, time.perf_counter() - t0)
     out["graph_captures"] = sum(c.rebuilds for c in srv.cohorts.values())
     out["capture_s"] = sum(c.rebuild_s for c in srv.cohorts.values())

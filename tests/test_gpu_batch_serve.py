@@ -219,6 +219,23 @@ def test_serve_batched_matches_unbatched(torch_cuda):
         assert np.array_equal(a["state"], b["state"]), a["rid"]
 
 
+def test_serve_small_line_variant(torch_cuda):
+    """three lines per workgroup on the small grids: batched and per request
+    agree bit for bit, and the server drops its variant records at close"""
+    from lns3d_amd.serve import Server
+    reqs = _reqs(4, None, 17, ((129, 65), (257, 129)), (1, 1), max_steps=500)
+    out = {}
+    for batched in (False, True):
+        srv = Server(64 * 2**30, keep_states=True, batched=batched, small_line_variant=4)
+        out[batched] = srv.run(reqs)
+        pinned = set(srv._pinned)
+        srv.close()
+        assert pinned and not srv._pinned
+    for a, b in zip(out[False], out[True]):
+        assert a["error"] is None and a["steps"] == b["steps"]
+        assert np.array_equal(a["state"], b["state"]), a["rid"]
+
+
 def test_serve_matches_complex_double_march(torch_cuda):
     """per grid, a request served with complex-float factors (the default
     mode) converges to the steady state of the complex-double march (the
