@@ -396,7 +396,9 @@ int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
  * of mode 1.  Non-periodic lines and viscous decks (LNS3D_EUNSUPPORTED
  * otherwise).  Contexts with a residual output
  * (lns3d_ctx_set_residual_output) record each step's norm in their next
- * slot, with the bits of the per-context steps.  dv_dev may be NULL.
+ * slot, with the bits of the per-context steps.  A mode-3 graph's tables are
+ * copied to the device on the stream of its first launch, ahead of it
+ * (creating the graph enqueues no device work).  dv_dev may be NULL.
  * Launch on `stream` (NULL: the graph's own stream).  No counterpart in the
  * reference. */
 typedef struct lns3d_graph lns3d_graph;

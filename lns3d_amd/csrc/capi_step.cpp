@@ -910,12 +910,21 @@ struct lns3d_graph {
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
   hipStream_t s = nullptr;
-  // mode 3: the device tables of the batched stages
+  // mode 3: the device tables of the batched stages, copied from their
+  // pinned staging on the first launch's stream (up_ev marks the copies)
   StepBatch sb;
   std::vector<PentaBatchGroup> gx, gy;
+  std::vector<TableUpload> ups;
+  bool uploaded = false;
+  hipEvent_t up_ev = nullptr;
 };
 
 static void graph_free(lns3d_graph *gr) {
+  if (gr->up_ev) {
+    (void)hipEventSynchronize(gr->up_ev);
+    (void)hipEventDestroy(gr->up_ev);
+  }
+  for (auto &u : gr->ups) (void)hipHostFree(u.host);
   if (gr->x) (void)hipGraphExecDestroy(gr->x);
   if (gr->g) (void)hipGraphDestroy(gr->g);
   if (gr->s) (void)hipStreamDestroy(gr->s);
@@ -964,10 +973,11 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
     sy1[(size_t)i] = 1;         // eta lines: line i, row j
     sy2[(size_t)i] = c->p.nx;
   }
-  int rc = step_batch_build(m.data(), count, gr->sb);
-  if (!rc) rc = penta_batch_build(px.data(), r0.data(), r1.data(), sx1.data(), sx2.data(), count, gr->gx);
-  if (!rc) rc = penta_batch_build(py.data(), r0.data(), r1.data(), sy1.data(), sy2.data(), count, gr->gy);
-  if (!rc && hipStreamCreateWithFlags(&gr->s, hipStreamNonBlocking) != hipSuccess) rc = fail(LNS3D_EHIP, "stream");
+  int rc = hipStreamCreateWithFlags(&gr->s, hipStreamNonBlocking) != hipSuccess ? fail(LNS3D_EHIP, "stream") : 0;
+  if (!rc) rc = step_batch_build(m.data(), count, gr->sb, gr->ups);
+  if (!rc) rc = penta_batch_build(px.data(), r0.data(), r1.data(), sx1.data(), sx2.data(), count, gr->gx, gr->ups);
+  if (!rc) rc = penta_batch_build(py.data(), r0.data(), r1.data(), sy1.data(), sy2.data(), count, gr->gy, gr->ups);
+  if (!rc && hipEventCreateWithFlags(&gr->up_ev, hipEventDisableTiming) != hipSuccess) rc = fail(LNS3D_EHIP, "event");
   if (!rc && hipStreamBeginCapture(gr->s, hipStreamCaptureModeThreadLocal) != hipSuccess)
     rc = fail(LNS3D_EHIP, "hipStreamBeginCapture");
   const bool capturing = !rc;
@@ -1070,7 +1080,13 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
 
 extern "C" int lns3d_graph_launch(lns3d_graph *gr, void *stream) {
   if (!gr) return fail(LNS3D_EINVAL, "lns3d_graph_launch: null graph");
-  LNS3D_CHECK(hipGraphLaunch(gr->x, stream ? (hipStream_t)stream : gr->s));
+  const hipStream_t s = stream ? (hipStream_t)stream : gr->s;
+  if (!gr->uploaded && !gr->ups.empty()) {
+    for (auto &u : gr->ups) LNS3D_CHECK(hipMemcpyAsync(u.dev, u.host, u.bytes, hipMemcpyHostToDevice, s));
+    LNS3D_CHECK(hipEventRecord(gr->up_ev, s));
+    gr->uploaded = true;
+  }
+  LNS3D_CHECK(hipGraphLaunch(gr->x, s));
   return 0;
 }
 

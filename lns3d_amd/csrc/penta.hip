@@ -34,6 +34,7 @@
 //  per line, the reference's substitution order on the LU working copy) is
 //  kept as the order-exact solve behind lns3d_penta_solve_reference_order.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -2337,8 +2338,17 @@ static int variant_lines(int v) {
   return v == PENTA_PLAIN2 ? 2 : v == PENTA_PLAIN3 ? 3 : v == PENTA_PLAIN4 ? 4 : 1;
 }
 
+int stage_upload(void *dev, const void *src, size_t bytes, std::vector<TableUpload> &ups) {
+  void *h = nullptr;
+  LNS3D_CHECK(hipHostMalloc(&h, bytes, hipHostMallocDefault));
+  memcpy(h, src, bytes);
+  ups.push_back(TableUpload{dev, h, bytes});
+  return 0;
+}
+
 int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, double2 *const *rhs1, const long *sL,
-                      const long *sR, int count, std::vector<PentaBatchGroup> &out) {
+                      const long *sR, int count, std::vector<PentaBatchGroup> &out,
+                      std::vector<TableUpload> &ups) {
   out.clear();
   std::map<std::pair<int, int>, std::vector<int>> by;   // (variant, precision) -> members
   for (int k = 0; k < count; k++) {
@@ -2379,10 +2389,11 @@ int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, doubl
     }
     g.blocks = off.back();
     LNS3D_CHECK(hipMalloc(&g.items, items.size() * sizeof(PentaBatchItem)));
-    LNS3D_CHECK(hipMalloc(&g.off, off.size() * sizeof(int)));
-    LNS3D_CHECK(hipMemcpy(g.items, items.data(), items.size() * sizeof(PentaBatchItem), hipMemcpyHostToDevice));
-    LNS3D_CHECK(hipMemcpy(g.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-    out.push_back(g);
+    out.push_back(g);             // freed with the groups from here on
+    LNS3D_CHECK(hipMalloc(&out.back().off, off.size() * sizeof(int)));
+    int rc = stage_upload(g.items, items.data(), items.size() * sizeof(PentaBatchItem), ups);
+    if (!rc) rc = stage_upload(out.back().off, off.data(), off.size() * sizeof(int), ups);
+    if (rc) return rc;
   }
   return 0;
 }

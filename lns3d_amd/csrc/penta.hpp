@@ -5,6 +5,17 @@
 
 namespace lns3d {
 
+// A device table of a batch graph, held in pinned host memory until the
+// graph's first launch copies it on the launch stream (building a graph then
+// puts no work on a device queue, where it could wait behind other streams'
+// kernels).  stage_upload allocates the pinned copy; the graph frees it.
+struct TableUpload {
+  void *dev = nullptr;
+  void *host = nullptr;
+  size_t bytes = 0;
+};
+int stage_upload(void *dev, const void *src, size_t bytes, std::vector<TableUpload> &ups);
+
 // One set of independent lines of equal length, factored once, solved many
 // times.  Device buffers:
 //   lu, per, per2 : LU working copy (np rows): line-interleaved for periodic
@@ -81,7 +92,8 @@ int penta_solve_reference_order(PentaPlan *pl, void *rhs, long sL, long sR, hipS
 // rhs0[k], rhs1[k] of strides sL[k], sR[k]; penta_batch_solve launches one
 // kernel per group on right-hand side `which` (0 or 1) of every member
 int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, double2 *const *rhs1, const long *sL,
-                      const long *sR, int count, std::vector<PentaBatchGroup> &out);
+                      const long *sR, int count, std::vector<PentaBatchGroup> &out,
+                      std::vector<TableUpload> &ups);
 int penta_batch_solve(const std::vector<PentaBatchGroup> &groups, int which, hipStream_t s);
 void penta_batch_free(std::vector<PentaBatchGroup> &groups);
 // factor health: flag the lines whose LU (and border fill) holds a

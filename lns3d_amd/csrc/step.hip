@@ -11,6 +11,7 @@
 // One thread per grid point.  Coefficients are structure-of-arrays ([e][P])
 // so a wave's loads of one coefficient are contiguous.
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 
 #include "common.hpp"
@@ -1885,7 +1886,7 @@ static long stage_blocks(int stage, const lns3d_params &p) {
   }
 }
 
-int step_batch_build(const StepBatchMember *m, int count, StepBatch &out) {
+int step_batch_build(const StepBatchMember *m, int count, StepBatch &out, std::vector<TableUpload> &ups) {
   step_batch_free(out);
   std::vector<StepBatchItem> items((size_t)count);
   std::vector<std::vector<int>> off(SB_NSTAGE, std::vector<int>(1, 0));
@@ -1916,26 +1917,35 @@ int step_batch_build(const StepBatchMember *m, int count, StepBatch &out) {
       off[sg].push_back(off[sg].back() + (int)nb);
     }
   }
+  // one device block: the items, each stage's table, the norm partials
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t bytes = up(items.size() * sizeof(StepBatchItem)), at_off[SB_NSTAGE];
+  for (int sg = 0; sg < SB_NSTAGE; sg++) {
+    at_off[sg] = bytes;
+    bytes += up(off[sg].size() * sizeof(int));
+  }
+  const size_t at_part = bytes;
+  bytes += off[SB_STAGE_NORM].back() * sizeof(double);
+  std::vector<char> host(at_part);
+  memcpy(host.data(), items.data(), items.size() * sizeof(StepBatchItem));
+  for (int sg = 0; sg < SB_NSTAGE; sg++) memcpy(host.data() + at_off[sg], off[sg].data(), off[sg].size() * sizeof(int));
+  char *dev = nullptr;
+  LNS3D_CHECK(hipMalloc(&dev, bytes));
+  out.mem = dev;
   out.count = count;
-  LNS3D_CHECK(hipMalloc(&out.items, items.size() * sizeof(StepBatchItem)));
-  LNS3D_CHECK(hipMemcpy(out.items, items.data(), items.size() * sizeof(StepBatchItem), hipMemcpyHostToDevice));
+  out.items = dev;
   for (int sg = 0; sg < SB_NSTAGE; sg++) {
     out.blocks[sg] = off[sg].back();
-    LNS3D_CHECK(hipMalloc(&out.off[sg], off[sg].size() * sizeof(int)));
-    LNS3D_CHECK(hipMemcpy(out.off[sg], off[sg].data(), off[sg].size() * sizeof(int), hipMemcpyHostToDevice));
+    out.off[sg] = (int *)(dev + at_off[sg]);
   }
-  if (out.blocks[SB_STAGE_NORM] > 0)
-    LNS3D_CHECK(hipMalloc(&out.part, out.blocks[SB_STAGE_NORM] * sizeof(double)));
-  return 0;
+  out.part = out.blocks[SB_STAGE_NORM] > 0 ? (double *)(dev + at_part) : nullptr;
+  return stage_upload(dev, host.data(), at_part, ups);
 }
 
 void step_batch_free(StepBatch &b) {
-  if (b.items) (void)hipFree(b.items);
-  for (int sg = 0; sg < SB_NSTAGE; sg++) {
-    if (b.off[sg]) (void)hipFree(b.off[sg]);
-    b.off[sg] = nullptr;
-  }
-  if (b.part) (void)hipFree(b.part);
+  if (b.mem) (void)hipFree(b.mem);
+  for (int sg = 0; sg < SB_NSTAGE; sg++) b.off[sg] = nullptr;
+  b.mem = nullptr;
   b.items = nullptr;
   b.part = nullptr;
   b.count = 0;

@@ -143,12 +143,14 @@ struct StepBatchMember {
 };
 struct StepBatch {
   int count = 0;
+  void *mem = nullptr;       // the one device block the tables live in
   void *items = nullptr;
   int *off[SB_NSTAGE] = {};
   long blocks[SB_NSTAGE] = {};
   double *part = nullptr;    // per-block partial sums of the norm stage
 };
-int step_batch_build(const StepBatchMember *m, int count, StepBatch &out);
+// the device tables (their contents staged in ups for the first launch)
+int step_batch_build(const StepBatchMember *m, int count, StepBatch &out, std::vector<TableUpload> &ups);
 void step_batch_free(StepBatch &b);
 // rhs = the backward-Euler residual with the rhsBC3D rows (launch_rhs mode 1)
 int step_batch_rhs(const StepBatch &b, hipStream_t s);
