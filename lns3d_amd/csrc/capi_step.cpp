@@ -917,6 +917,7 @@ struct lns3d_graph {
   std::vector<TableUpload> ups;
   bool uploaded = false;
   hipEvent_t up_ev = nullptr;
+  hipStream_t up_stream = nullptr;   // the stream the copies were made on
 };
 
 static void graph_free(lns3d_graph *gr) {
@@ -1085,6 +1086,9 @@ extern "C" int lns3d_graph_launch(lns3d_graph *gr, void *stream) {
     for (auto &u : gr->ups) LNS3D_CHECK(hipMemcpyAsync(u.dev, u.host, u.bytes, hipMemcpyHostToDevice, s));
     LNS3D_CHECK(hipEventRecord(gr->up_ev, s));
     gr->uploaded = true;
+    gr->up_stream = s;
+  } else if (gr->uploaded && s != gr->up_stream) {
+    LNS3D_CHECK(hipStreamWaitEvent(s, gr->up_ev, 0));   // launched on another stream than the copies
   }
   LNS3D_CHECK(hipGraphLaunch(gr->x, s));
   return 0;

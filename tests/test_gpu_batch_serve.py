@@ -69,12 +69,19 @@ def test_graph_matches_eager(torch_cuda, mode, pin):
     torch.cuda.synchronize()
     ref = [v.clone() for v in b.v]
     g = b.capture(nsteps, mode=mode)
+    # replays on two caller streams, then the graph's own (a staged graph
+    # copies its tables on its first launch's stream; later launches on
+    # another stream wait for that copy)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream(), None]
     for rep in range(3):
         b.reset()
         junk = torch.randn(1000, device="cuda").sum()      # eager work between replays
         torch.cuda.synchronize()
-        g.launch()
-        g.synchronize()
+        g.launch(streams[rep])
+        if streams[rep] is None:
+            g.synchronize()
+        else:
+            streams[rep].synchronize()
         for k in range(len(params)):
             assert torch.equal(b.v[k], ref[k]), (rep, k)
         del junk
