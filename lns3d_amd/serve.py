@@ -289,10 +289,10 @@ class _Cohort:
     graphs are destroyed when the serving run ends (freeing their tables
     waits for the whole device)."""
 
-    def __init__(self, key, mode: int, chunk: int, corrections: int):
+    def __init__(self, key, chunk: int, corrections: int):
         import torch
         self.key = key
-        self.mode, self.chunk, self.corrections = mode, chunk, corrections
+        self.chunk, self.corrections = chunk, corrections
         self.stream = torch.cuda.Stream()
         self.members: List[_Slot] = []      # in the next launch
         self.graph = None
@@ -301,6 +301,7 @@ class _Cohort:
         self.inflight = collections.deque() # (event, [(slot, host buffer)])
         self.rebuilds = 0
         self.rebuild_s = 0.0                # host time capturing graphs
+        self.failed: List[_Slot] = []       # members whose graph could not be made
 
     def join(self, s: _Slot):
         self.members.append(s)
@@ -338,12 +339,19 @@ class _Cohort:
     def _fill(self):
         self.members = [m for m in self.members if m.wants_chunk()]
         while self.members and len(self.inflight) < 2:
-            self._launch()
+            try:
+                self._launch()
+            except LNS3DError as e:
+                # no graph for these members: they fail, the other grids go on
+                for m in self.members:
+                    m.failed, m.done = f"batched graph: {e}", True
+                    self.failed.append(m)
+                self.members = []
 
     def poll(self) -> List[_Slot]:
         """examine the launches that have finished; returns the members
         found done"""
-        out = []
+        out, self.failed = self.failed, []
         while self.inflight and self.inflight[0][0].query():
             _, entries = self.inflight.popleft()
             for m, buf in entries:
@@ -511,7 +519,7 @@ class Server:
                     continue
                 if self.batched:
                     if s.key not in self.cohorts:
-                        self.cohorts[s.key] = _Cohort(s.key, self.gmode, self.chunk,
+                        self.cohorts[s.key] = _Cohort(s.key, self.chunk,
                                                       0 if self.gmode == MODE_FP32 else self.corrections)
                     self.cohorts[s.key].join(s)
                 active.append(s)

@@ -282,6 +282,38 @@ def test_serve_isolates_a_failing_request(torch_cuda, batched):
         assert np.array_equal(r["state"], solo[r["rid"]]["state"])
 
 
+def test_serve_batched_graph_failure(torch_cuda, monkeypatch):
+    """a staged graph that cannot be made (forced here on the first capture
+    for the 257x129 grid) fails that cohort's requests with the error; the
+    other grid's requests complete with the states they reach without them"""
+    from lns3d_amd import serve
+    from lns3d_amd.native import LNS3DError
+    grids, w = ((129, 65), (257, 129)), (1, 1)
+    reqs = _reqs(4, None, 19, grids, w, max_steps=300)
+    real = serve.Graph
+    tripped = []
+
+    def flaky(steps, *a, **k):
+        if not tripped and steps[0].params.nx == 257:
+            tripped.append(1)
+            raise LNS3DError("forced failure", 2)
+        return real(steps, *a, **k)
+
+    monkeypatch.setattr(serve, "Graph", flaky)
+    srv = serve.Server(64 * 2**30, keep_states=True, batched=True)
+    recs = srv.run(reqs)
+    monkeypatch.setattr(serve, "Graph", real)
+    solo = {r.rid: srv.run([r])[0] for r in reqs if r.nx == 129}
+    srv.close()
+    assert tripped
+    for r in recs:
+        if r["nx"] == 257:
+            assert r["error"] and "forced failure" in r["error"], r
+        else:
+            assert r["error"] is None
+            assert np.array_equal(r["state"], solo[r["rid"]]["state"])
+
+
 def test_serve_tight_budget(torch_cuda):
     """the pool never holds more than the budget; with room for one context
     of the single grid in use, requests run one after another"""
