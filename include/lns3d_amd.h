@@ -31,8 +31,11 @@ const char *lns3d_last_error(void);
  * Supported: optx/opty any; xper/yper; wall in {0,1,2,3} (3: the bump wall,
  * lns3d_set_wall); wallt in {0,1,2};
  * top in {0,1} (1: the incoming wave of lns3d_set_top_forcing); left in
- * {0,1,4,5,7} and right in {0,4,5,7,8} (8: the column held at its initial
- * values; 1: acoustic wave, 5: the undamped wave, right 5 with nx <= ny;
+ * {0,1,4,5,7} and right in {0,4,5,7,8,9} (8: the column held at its initial
+ * values; 9: extrapolated from the two columns inside, exp(2 log v(nx-1) -
+ * log v(nx-2)), src/rhsbc3d.f90:317-320, src/itrbc3d.f90:408-410, which is
+ * not finite where both are zero, as at a no-slip wall's corner, in the
+ * reference as here; 1: acoustic wave, 5: the undamped wave, right 5 with nx <= ny;
  * 4: eigenfunction inflow; 1, 4 and 5
  * both through lns3d_set_inflow; right 4 with left 4 only; 7: symmetry,
  * src/lhsbs1f3d.f90 and the symmetric closures of src/cgrad.f90,

@@ -49,7 +49,7 @@ static int check_params(const lns3d_params &p) {
     return fail(LNS3D_EUNSUPPORTED, "complex-float factors with the inviscid deck (Re = 0) or updateLHS are not implemented");
   if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
-      (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8) ||
+      (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8 && p.right != 9) ||
       (p.linear != 1 && p.linear != 2) ||
       (p.mattyp < 0 || p.mattyp > 3) || (p.loctime != 0 && p.loctime != 1))
     return fail(LNS3D_EUNSUPPORTED, "unsupported boundary/physics selector (see lns3d_params)");

@@ -27,6 +27,14 @@ __device__ __forceinline__ double sc(double a, double x) { return a * x; }
 __device__ __forceinline__ z sc(double a, z x) { return {a * x.re, a * x.im}; }
 __device__ __forceinline__ double ld(const double *p) { return *p; }
 __device__ __forceinline__ z ld(const double2 *p) { double2 v = *p; return {v.x, v.y}; }
+// right = 9 (src/rhsbc3d.f90:317-320, src/itrbc3d.f90:408-410): the value
+// exp(2 log a - log b) of the column from the two columns inside it, formed
+// as a^2 / b (the same value, zero for a = 0, not finite for b = 0)
+__device__ __forceinline__ z extrap_log(z a, z b) {
+  const z a2{a.re * a.re - a.im * a.im, 2.0 * a.re * a.im};
+  const double d = b.re * b.re + b.im * b.im;
+  return {(a2.re * b.re + a2.im * b.im) / d, (a2.im * b.re - a2.re * b.im) / d};
+}
 
 // first derivative along a line at position p (N points), value stride es
 // between consecutive points; component e selected by base offset.
@@ -1137,6 +1145,10 @@ __global__ __launch_bounds__(320) void k_rhs_point_t(lns3d_params p, Stencil st,
           const z d = vq(m) - f;
           a = z{-d.re, -d.im};
         }
+        if (i == nx - 1 && p.right == 9) {      // src/rhsbc3d.f90:317-320
+          const z d = vq(m) - extrap_log(ld(v + 5 * (q - 1) + m), ld(v + 5 * (q - 2) + m));
+          a = z{-d.re, -d.im};
+        }
         if ((i == 0 && p.left == 0) || (i == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 8)))
           a = z{0, 0};
       }
@@ -1155,7 +1167,8 @@ __global__ __launch_bounds__(320) void k_rhs_point_t(lns3d_params p, Stencil st,
     if (c.bnb && with_rbe && !p.yper && jj == 0 && (m == 1 || m == 2) && ii >= c.wlo && ii <= c.whi) {
       const bool lr = p.xper ? ii == nx - 1
                              : ((ii == 0 && (p.left == 0 || p.left == 1 || p.left == 4 || p.left == 5)) ||
-                                (ii == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5 || p.right == 8)));
+                                (ii == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5 || p.right == 8 ||
+                                                  p.right == 9)));
       if (!lr) {
         const double b1 = c.bnb[ii], b2 = c.bnb[nx + ii];
         if (m == 1) {
@@ -1300,8 +1313,9 @@ int launch_wall_normals(const lns3d_params &p, const Coeffs &c, double *bnb, hip
   return 0;
 }
 
-// left and right columns (src/itrbc3d.f90:200-376): periodic copy; 0 zero;
-// left 1 and 4, right 4 the values in fside; 7 (symmetry) left alone
+// left and right columns (src/itrbc3d.f90:200-410): periodic copy; 0 zero;
+// left 1 and 4, right 4 the values in fside; right 9 the extrapolation from
+// the two columns inside; 7 (symmetry) left alone
 __device__ __forceinline__ void itrbc_lr(long bid, const lns3d_params &p, const double *fside, double2 *v) {
   int j = (int)(bid * blockDim.x + threadIdx.x);
   const int nx = p.nx, ny = p.ny;
@@ -1318,6 +1332,10 @@ __device__ __forceinline__ void itrbc_lr(long bid, const lns3d_params &p, const 
       if (p.right == 4 || p.right == 5)
         row[5L * (nx - 1) + d] =
             make_double2(fside[2 * ((5 + d) * ny + j)], fside[2 * ((5 + d) * ny + j) + 1]);
+      if (p.right == 9) {
+        const z e = extrap_log(ld(row + 5L * (nx - 2) + d), ld(row + 5L * (nx - 3) + d));
+        row[5L * (nx - 1) + d] = make_double2(e.re, e.im);
+      }
     }
   }
 }

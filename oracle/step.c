@@ -3,12 +3,10 @@
  * boundary closures and boundary conditions, coefficient generation, time
  * step, and the implicit step driver.  TEST INFRASTRUCTURE ONLY (see
  * lns3d_oracle.h).  Supported options are the ones the benchmark and test
- * configurations use; anything else returns -1 rather than guessing:
- *   optx/opty any value; xper/yper; carp = no; lsym/rsym (left/right = 7);
- *   wall in {0,1,2}; wallt in {0,1,2}; top in {0,1}; left/right in {0,7};
- *   Navier = yes; ispg = 0; eps_e = 0; idamp = 0; useCalcd = no;
- *   linear = 2 (complex analysis); mattyp 0..3.
+ * configurations use (supported() below lists them); anything else returns
+ * -1 rather than guessing.
  */
+#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -247,7 +245,7 @@ static int supported(const orc_params *p) {
   if (p->wall < 0 || p->wall > 3 || p->wallt < 0 || p->wallt > 2) return 0;
   if ((p->top != 0 && p->top != 1) ||
       (p->left != 0 && p->left != 1 && p->left != 4 && p->left != 5 && p->left != 7) ||
-      (p->right != 0 && p->right != 4 && p->right != 5 && p->right != 7 && p->right != 8))
+      (p->right != 0 && p->right != 4 && p->right != 5 && p->right != 7 && p->right != 8 && p->right != 9))
     return 0;
   if (!p->xper && p->right == 5 && p->ny - 1 + (long)p->nx * (p->nx - 1) >= (long)p->nx * p->ny)
     return 0;                                    /* x(j,nx) off the grid (nx > ny) */
@@ -955,6 +953,8 @@ int orc_rhsbc3d(const orc_params *p, zdouble *r, const zdouble *v) {
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) R(d, 0, j) = 0;
         if (p->right == 0 || p->right == 4 || p->right == 8) R(d, nx - 1, j) = 0;
+        if (p->right == 9)                       /* src/rhsbc3d.f90:317-320 */
+          R(d, nx - 1, j) = -(V(d, nx - 1, j) - cexp(2.0 * clog(V(d, nx - 2, j)) - clog(V(d, nx - 3, j))));
       }
     }
   }
@@ -1035,6 +1035,8 @@ int orc_itrbc3d(const orc_params *p, zdouble *v) {
       for (int d = 0; d < 5; d++) {
         if (p->left == 0) V(d, 0, j) = 0;
         if (p->right == 0) V(d, nx - 1, j) = 0;
+        if (p->right == 9)                       /* src/itrbc3d.f90:408-410 */
+          V(d, nx - 1, j) = cexp(2.0 * clog(V(d, nx - 2, j)) - clog(V(d, nx - 3, j)));
       }
     }
   }

@@ -511,6 +511,10 @@ EXTRA_CASES = {
     # idamp = 1: the outflow column's mean xi second derivatives and Vh11
     # damped; its rows are the operator's only with the symmetric end
     "idamp": dict(idamp=1, wall=1, wallt=1, right=7),
+    # right = 9: the right column extrapolated from the two inside it
+    # (src/rhsbc3d.f90:317-320, itrbc3d.f90:408-410); periodic in y, since a
+    # wall's zero velocities make its corner the log of zero
+    "right9": dict(yper=1, right=9),
 }
 
 
@@ -612,9 +616,10 @@ def main():
             sym_case(R, name, kw)
         R.lib.ref_flush()
         return
-    if sys.argv[1:] == ["extras"]:
+    if sys.argv[1:2] == ["extras"]:
         for name, kw in EXTRA_CASES.items():
-            extras_case(R, name, kw)
+            if len(sys.argv) < 3 or name in sys.argv[2:]:
+                extras_case(R, name, kw)
         R.lib.ref_flush()
         return
     if sys.argv[1:] == ["sutherland"]:

@@ -12,7 +12,9 @@ wall normals are not along y):
   in the xi factor, alternating with istep - 1 + iter) and updateLHS
   (src/impdrv3d.f90:248,310: the factors formed again each step);
 * idamp = 1 (src/damper.f90:26-29, src/genmtrx.f90:87-96, 877-881: the
-  outflow column's mean xi second derivatives and Vh11 zeroed).
+  outflow column's mean xi second derivatives and Vh11 zeroed);
+* right = 9 (src/rhsbc3d.f90:317-320, src/itrbc3d.f90:408-410: the right
+  column extrapolated from the two inside it), periodic in y.
 
 lrhs3D, itrBC3D, the assembled factors (where they do not depend on the step)
 and two or three implicit steps, 1e-12 relative in the max norm."""
@@ -26,7 +28,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp"]
+CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp", "right9"]
 
 
 def rel(a, b):

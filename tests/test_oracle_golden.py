@@ -530,7 +530,7 @@ def wall_normals(met):
     return np.asfortranarray(np.stack([-n1 / s, -n2 / s], axis=1))
 
 
-EXTRA_CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp"]
+EXTRA_CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp", "right9"]
 
 
 @pytest.mark.parametrize("name", EXTRA_CASES)
@@ -539,8 +539,8 @@ def test_extras(orc, name):
     rows of src/lhsbc1f3d.f90:55-393, lhsbc2f3d.f90:75-120, rhsbc3d.f90
     :108-118, itrbc3d.f90:120-132 and bcfix), the wall extent is:ie of the
     &extras namelist, and useCalcd / updateLHS (src/lhs1f3d.f90:27-37,
-    impdrv3d.f90:248,310), against the reference's routines
-    (ref_extra_<case>.npz); without the option the oracle misses"""
+    impdrv3d.f90:248,310), idamp = 1 and right = 9, against the reference's
+    routines (ref_extra_<case>.npz); without the option the oracle misses"""
     import dataclasses
     import types
     z = load(f"ref_extra_{name}.npz")
@@ -565,6 +565,12 @@ def test_extras(orc, name):
         v = np.asfortranarray(v0.copy())
         orc.itrbc3d(p, v)
         assert rel(v, z["itrbc"]) < 1e-14
+        if p.right == 9:
+            # exp(2 log a - log b) = a^2 / b: the reference's right column
+            # (rows below the periodic copy) from the two columns inside
+            a, b = v0[:, -2, :-1], v0[:, -3, :-1]
+            assert rel(z["itrbc"][:, -1, :-1], a * a / b) < 1e-13
+            assert rel(z["rhsbc"][:, -1, :-1], -(v0[:, -1, :-1] - a * a / b)) < 1e-13
         for d in (1, 2):
             if f"lhs{d}" in z.files:
                 assert rel(orc.lhs(p, d, c, dtl), z[f"lhs{d}"]) < 1e-14, d
@@ -579,7 +585,8 @@ def test_extras(orc, name):
             assert rel(orc.step(p, c, dtl, f, v), z[f"dv{k}"]) < 1e-12, k
             assert rel(v, z[f"v{k}"]) < 1e-12, k
         # the same deck without the option misses
-        plain = dataclasses.replace(p, Re=1000.0 if p.Re == 0.0 else p.Re, wall_is=0, wall_ie=0, idamp=0)
+        plain = dataclasses.replace(p, Re=1000.0 if p.Re == 0.0 else p.Re, wall_is=0, wall_ie=0, idamp=0,
+                                    right=0 if p.right == 9 else p.right)
         if p.idamp:
             c = orc.genmtrx(plain, vm, met)
             assert rel(c["Vh11"], z["c_Vh11"]) > 1e-6
