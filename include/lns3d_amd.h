@@ -84,8 +84,8 @@ typedef struct {
  * iteration, as the build with USE_CPENTA1BC does (src/impdrv3d.f90:248,310);
  * the factors change only with useCalcd's D-term placement, so they are
  * formed again only then.
- * Not implemented, rejected with LNS3D_EUNSUPPORTED: use_calcd with left
- * or right = 7, and precision = 1 with the inviscid deck or update_lhs.
+ * Not implemented, rejected with LNS3D_EUNSUPPORTED: precision = 1 with
+ * the inviscid deck or update_lhs.
  * Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, Re = 0 with
  * mattyp != 0, idamp outside 0..2 and a wall extent outside
  * 1 <= is <= ie <= nx are rejected with LNS3D_EINVAL. */

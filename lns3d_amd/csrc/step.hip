@@ -739,22 +739,25 @@ __device__ __forceinline__ z lhs_entry(const AsmCtx &A, const Coeffs &c, int dir
 
   const int sym = dir == 1 ? A.sym : 0;
   if (sym && (((sym & 1) && pos <= 1) || ((sym & 2) && pos >= N - 2))) {
-    // rows next to a symmetric end (src/lhsbs1f3d.f90, xi factor, calcd):
-    // the interior row with the weight of each point beyond the end added to
-    // the point it reflects onto, negated for column component 3; the
-    // spanwise cross entries are read through that routine's Ahi(6,ny,nx)
-    // shape (point i + ny j; check_params keeps it on the grid)
+    // rows next to a symmetric end (src/lhsbs1f3d.f90, xi factor): the
+    // interior row with the weight of each point beyond the end added to
+    // the point it reflects onto, negated for column component 3; the D
+    // terms times rcalcd, the sponge without them (:154-170); the spanwise
+    // cross entries are read through that routine's Ahi(6,ny,nx) shape
+    // (point i + ny j; check_params keeps it on the grid)
     double aw, bw;
     sym_weights(A, pos, N, k, cc, aw, bw);
     const double ABs = ai >= 0 ? c.ABhi[(long)(ab + ai) * P + pos + (long)p.ny * j] : 0.0;
     double re = dt * aw * X, im = 0.0;
     if (k == 2) {
-      re = re + alfa * dt * c.Dh[(long)(r + 5 * cc) * P + q];
-      im = alfa * dt * c.Dhi[(long)(r + 5 * cc) * P + q];
+      const double rc = calcd ? 1.0 : 0.0;
+      re = re + rc * alfa * dt * c.Dh[(long)(r + 5 * cc) * P + q];
+      im = rc * alfa * dt * c.Dhi[(long)(r + 5 * cc) * P + q];
     }
     if (ai >= 0) im = im - aw * dt * ABs;
     if (vi >= 0) re = re - dt * bw * Vv;
     if (k == 2 && r == cc) re = re + 1.0;
+    if (!calcd && k == 2 && r == cc && c.spl) re = re + alfa * dt * c.spl[q];
     if (c.sbuf && r == cc) re = re + alfa * dt * c.sbuf[q] * damp_w(pos, N, per, sym, k, cc);
     return z{re, im};
   }

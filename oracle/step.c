@@ -629,8 +629,9 @@ static void closure_row(zdouble *mat, long pt, const double F[5], const double S
 }
 
 /* a row of the xi factor next to a symmetric end (src/lhsbs1f3d.f90:36-329
- * left, :331-624 right; calcd, so no sponge; Xi2 read where that routine
- * reads it, see orc_lhs): the interior row of
+ * left, :331-624 right: the D terms times rcalcd, the sponge spgd on the
+ * diagonal without them (:154-170); Xi2 read where that routine reads it,
+ * see orc_lhs): the interior row of
  * src/lhs1f3d.f90 with the weight of each point beyond the end added to the
  * point it reflects onto, negated for column component 3 (index 2); slots
  * keep their interior meaning (slot k: point i + k - 2), the slots that then
@@ -638,7 +639,7 @@ static void closure_row(zdouble *mat, long pt, const double F[5], const double S
  * times alfa. */
 static void sym_lhs_row(zdouble *mat, long pt, int i, int N, const double a[5], const double b[5],
                         double dt, const double *X, const double *Xi2, const double *D,
-                        const double *Di, const double *Vh, double alfa) {
+                        const double *Di, const double *Vh, double alfa, double rc, double spgd) {
   double aw[5][5], bw[5][5];                   /* [slot][column component] */
   for (int k = 0; k < 5; k++)
     for (int cc = 0; cc < 5; cc++) aw[k][cc] = bw[k][cc] = 0.0;
@@ -658,7 +659,7 @@ static void sym_lhs_row(zdouble *mat, long pt, int i, int N, const double a[5], 
       for (int cc = 0; cc < 5; cc++) MAT(k, r, cc) = dt * aw[k][cc] * X[r + 5 * cc];
   for (int r = 0; r < 5; r++)
     for (int cc = 0; cc < 5; cc++)
-      MAT(2, r, cc) += alfa * dt * D[r + 5 * cc] + _Complex_I * (alfa * dt * Di[r + 5 * cc]);
+      MAT(2, r, cc) += rc * alfa * dt * D[r + 5 * cc] + _Complex_I * (rc * alfa * dt * Di[r + 5 * cc]);
   for (int k = 0; k < 5; k++) {
     MAT(k, 1, 3) -= _Complex_I * (aw[k][3] * dt * Xi2[0]);
     MAT(k, 2, 3) -= _Complex_I * (aw[k][3] * dt * Xi2[1]);
@@ -672,6 +673,7 @@ static void sym_lhs_row(zdouble *mat, long pt, int i, int N, const double a[5], 
     MAT(k, 4, 4) -= dt * bw[k][4] * Vh[3];
   }
   for (int r = 0; r < 5; r++) MAT(2, r, r) += 1.0;
+  for (int r = 0; r < 5; r++) MAT(2, r, r) += spgd;
 }
 
 int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl,
@@ -763,7 +765,8 @@ int orc_lhs(const orc_params *p, int dir, const orc_coeffs *c, const double *dtl
         long pt = rows[w] + (long)nx * j;
         const long pa = rows[w] + (long)ny * j;     /* Ahi(6,ny,nx) of src/lhsbs1f3d.f90:17 */
         sym_lhs_row(mat, pt, rows[w], nx, a, b, dtl[pt], X + 25 * pt, c->ABhi + 6 * pa,
-                    c->Dh + 25 * pt, c->Dhi + 25 * pt, Vh + 6 * pt, alfa);
+                    c->Dh + 25 * pt, c->Dhi + 25 * pt, Vh + 6 * pt, alfa, calcd ? 1.0 : 0.0,
+                    (!calcd && g_spg.ispg) ? alfa * dtl[pt] * lhs_sponge(pt) : 0.0);
       }
     }
   }

@@ -508,6 +508,9 @@ EXTRA_CASES = {
     # with updateLHS the factors are formed again at every step, alternating
     "calcd": dict(use_calcd=1),
     "calcd_update": dict(use_calcd=1, update_lhs=1),
+    # the same with symmetric ends: the D terms and the sponge of
+    # src/lhsbs1f3d.f90's rows follow calcd too
+    "calcd_sym": dict(use_calcd=1, update_lhs=1, left=7, right=7),
     # idamp = 1: the outflow column's mean xi second derivatives and Vh11
     # damped; its rows are the operator's only with the symmetric end
     "idamp": dict(idamp=1, wall=1, wallt=1, right=7),

@@ -181,7 +181,6 @@ def test_serving_arrivals_and_fifo():
     (dict(wall_is=-1), 1, b"wall extent"),
     (dict(idamp=3), 1, b"idamp"),
     (dict(use_calcd=2), 1, b"use_calcd"),
-    (dict(use_calcd=1, left=7), 4, b"useCalcd"),
     (dict(Re=0.0, precision=1), 4, b"inviscid"),
     (dict(update_lhs=1, precision=1), 4, b"updateLHS"),
 ])

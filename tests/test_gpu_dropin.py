@@ -142,7 +142,7 @@ def test_harness_top_wave(R):
 
 
 @pytest.mark.parametrize("name", ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd",
-                                  "calcd_update", "idamp", "right9"])
+                                  "calcd_update", "calcd_sym", "idamp", "right9"])
 def test_harness_extras(R, name):
     """the inviscid deck (its wall rows in lhs1f3D / lhs2f3D through the
     forwarded wall normals bnb), the wall extent is:ie, and useCalcd with and

@@ -28,7 +28,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp", "right9"]
+CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "calcd_sym", "idamp",
+         "right9"]
 
 
 def rel(a, b):

@@ -530,7 +530,8 @@ def wall_normals(met):
     return np.asfortranarray(np.stack([-n1 / s, -n2 / s], axis=1))
 
 
-EXTRA_CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "idamp", "right9"]
+EXTRA_CASES = ["inviscid", "inviscid_isie", "wall_isie", "bump_isie", "calcd", "calcd_update", "calcd_sym", "idamp",
+               "right9"]
 
 
 @pytest.mark.parametrize("name", EXTRA_CASES)
