@@ -43,8 +43,8 @@ static int check_params(const lns3d_params &p) {
   if (p.idamp < 0 || p.idamp > 2) return fail(LNS3D_EINVAL, "idamp must be 0, 1 or 2 (src/damper.f90:26-47)");
   if ((p.use_calcd != 0 && p.use_calcd != 1) || (p.update_lhs != 0 && p.update_lhs != 1))
     return fail(LNS3D_EINVAL, "use_calcd and update_lhs are 0 or 1");
-  if (p.precision == 1 && (p.Re == 0.0 || p.update_lhs))
-    return fail(LNS3D_EUNSUPPORTED, "complex-float factors with the inviscid deck (Re = 0) or updateLHS are not implemented");
+  if (p.precision == 1 && p.Re == 0.0)
+    return fail(LNS3D_EUNSUPPORTED, "complex-float factors with the inviscid deck (Re = 0) are not implemented");
   if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
       (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8 && p.right != 9) ||
@@ -650,6 +650,19 @@ extern "C" int lns3d_solve_dir(lns3d_ctx *c, int dir, void *r_dev) {
   return penta_solve(&c->py, r_dev, 1, c->p.nx, c->stream);
 }
 
+// updateLHS: the factors formed again at every iteration (src/impdrv3d.f90
+// :248,310).  They depend on the step and iteration only through useCalcd's
+// D-term placement, so they are formed again when that placement changes.
+static int refactor_for(lns3d_ctx *c, int iter) {
+  if (!c->p.update_lhs) return 0;
+  const int want = c->p.use_calcd ? (int)((c->lstep + iter) % 2 == 0) : 1;
+  if (want == c->factored_calcd) return 0;
+  c->cur_iter = iter;
+  const int e = lns3d_factor(c);
+  c->cur_iter = 1;
+  return e;
+}
+
 // One implicit step (src/impdrv3d.f90:146-356, impl = 1, niter = 1):
 // r = -dtl * lrhs3D(v) with rhsBC3D rows, xi solve, eta solve, v += r,
 // itrBC3D.  v must already satisfy the boundary conditions (call
@@ -673,18 +686,7 @@ extern "C" int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev) {
     rc = lns3d_factor(c);
     if (rc) return rc;
   }
-  // updateLHS: the factors formed again at every iteration (src/impdrv3d.f90
-  // :248,310).  They depend on the step and iteration only through useCalcd's
-  // D-term placement, so they are formed again when that placement changes.
-  auto refactor = [&](int iter) {
-    if (!c->p.update_lhs) return 0;
-    const int want = c->p.use_calcd ? (int)((c->lstep + iter) % 2 == 0) : 1;
-    if (want == c->factored_calcd) return 0;
-    c->cur_iter = iter;
-    int e = lns3d_factor(c);
-    c->cur_iter = 1;
-    return e;
-  };
+  auto refactor = [&](int iter) { return refactor_for(c, iter); };
   if (c->niter == 1) {
     rc = refactor(1);
     if (!rc) rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, two ? 2 : 1, c->stream, c->vhist);
@@ -797,7 +799,8 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
   const long n = 5L * c->p.nx * c->p.ny;
   void *dv = dv_dev ? dv_dev : c->r;
   void *rhs = c->t3;            // the residual r of the step
-  int rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
+  int rc = refactor_for(c, 1);
+  if (!rc) rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
   if (rc) return rc;
   rc = launch_copy(dv, rhs, n, c->stream);
   if (rc) return rc;
@@ -823,7 +826,9 @@ extern "C" int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, doubl
   if (iters_out) *iters_out = it;
   if (resid_out) *resid_out = c->last_resid;
   if (!(c->last_resid == c->last_resid)) return fail(LNS3D_ENONFINITE, "refinement produced NaN");
-  return launch_update_bc(c->p, c->c, v_dev, dv, c->stream);
+  rc = launch_update_bc(c->p, c->c, v_dev, dv, c->stream);
+  if (!rc) c->lstep++;
+  return rc;
 }
 
 // the same step, but only the complex-float solve (no refinement): the
@@ -834,10 +839,12 @@ extern "C" int lns3d_step_fp32(lns3d_ctx *c, void *v_dev, void *dv_dev) {
   if (!c || !c->factored || !c->have32 || !v_dev)
     return fail(LNS3D_ENOTFACTORED, "lns3d_step_fp32: needs precision = 1 and lns3d_factor");
   void *r = dv_dev ? dv_dev : c->r;
-  int rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
+  int rc = refactor_for(c, 1);
+  if (!rc) rc = launch_rhs(c->p, c->st, c->c, v_dev, r, c->work, 1, c->stream);
   if (!rc) rc = record_norm(c, r);
   if (!rc) rc = solve_xy(c, c->px32, c->py32, r);
   if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, r, c->stream);
+  if (!rc) c->lstep++;
   return rc;
 }
 
@@ -854,7 +861,8 @@ extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev,
   const long n = 5L * c->p.nx * c->p.ny;
   void *dv = dv_dev ? dv_dev : c->r;
   void *rhs = c->t3;
-  int rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
+  int rc = refactor_for(c, 1);
+  if (!rc) rc = launch_rhs(c->p, c->st, c->c, v_dev, rhs, c->work, 1, c->stream);
   if (!rc) rc = record_norm(c, rhs);
   if (!rc) rc = launch_copy(dv, rhs, n, c->stream);
   if (!rc) rc = solve_xy(c, c->px32, c->py32, dv);
@@ -869,6 +877,7 @@ extern "C" int lns3d_step_refined_fixed(lns3d_ctx *c, void *v_dev, void *dv_dev,
     if (!rc) rc = launch_residual_norms(rhs, c->t2, n, c->scal + 8, c->stream);
   }
   if (!rc) rc = launch_update_bc(c->p, c->c, v_dev, dv, c->stream);
+  if (!rc) c->lstep++;
   return rc;
 }
 

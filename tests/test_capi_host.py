@@ -182,7 +182,6 @@ def test_serving_arrivals_and_fifo():
     (dict(idamp=3), 1, b"idamp"),
     (dict(use_calcd=2), 1, b"use_calcd"),
     (dict(Re=0.0, precision=1), 4, b"inviscid"),
-    (dict(update_lhs=1, precision=1), 4, b"updateLHS"),
 ])
 def test_params_refused_before_any_device_call(lib, change, code, text):
     """lns3d_ctx_create checks the deck before it touches the device: values

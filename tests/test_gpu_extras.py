@@ -84,6 +84,39 @@ def test_extras_against_reference(name):
     st.close()
 
 
+def test_calcd_update_complex_float():
+    """useCalcd with updateLHS and complex-float factors: the refined step
+    (defect correction to 1e-10 of the complex-double AF system) forms both
+    precisions' factors again as the D terms change factor, and follows the
+    reference's three steps to 1e-9"""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step
+    z = np.load(os.path.join(GOLD, "ref_extra_calcd_update.npz"), allow_pickle=False)
+    p = Params(**json.loads(str(z["params"])))
+    met, vm, v0 = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]), z["v0"]
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    st = Step(p, precision=1)
+    st.setup(met, vm)
+    st.set_sponge(1, z["spg"])
+    vd = dev(v0)
+    st.itrbc3d(vd)
+    st.factor()
+    dvd = torch.zeros_like(vd)
+    for k in range(1, 4):
+        iters, res = st.step_refined(vd, dvd, tol=1e-10)
+        dv = dvd.cpu().numpy().reshape(v0.shape, order="F")
+        e = rel(dv, z[f"dv{k}"])
+        print(f"  step {k}: {iters} solves, residual {res:.1e}, dv {e:.2e}", flush=True)
+        assert e < 1e-9, k
+    st.close()
+
+
 def test_inviscid_refusals():
     """the inviscid deck needs the wall normals (lns3d_setup with the metric)
     before its factors and its boundary rows; the complex-float factors refuse it"""
