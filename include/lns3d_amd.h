@@ -84,9 +84,10 @@ typedef struct {
  * iteration, as the build with USE_CPENTA1BC does (src/impdrv3d.f90:248,310);
  * the factors change only with useCalcd's D-term placement, so they are
  * formed again only then.
- * Not implemented, rejected with LNS3D_EUNSUPPORTED: precision = 1 with
- * the inviscid deck.  With precision = 1 and update_lhs the complex-float
- * steps form both precisions' factors again as lns3d_step does.
+ * With precision = 1 the refined steps' operator applies the inviscid
+ * deck's rotated wall rows as the assembled factors hold them, and with
+ * update_lhs the complex-float steps form both precisions' factors again
+ * as lns3d_step does.
  * Re < 0, a non-finite Ma, Re, Pr, gamma, kz or omega, Re = 0 with
  * mattyp != 0, idamp outside 0..2 and a wall extent outside
  * 1 <= is <= ie <= nx are rejected with LNS3D_EINVAL. */

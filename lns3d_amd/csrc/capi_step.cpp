@@ -43,8 +43,6 @@ static int check_params(const lns3d_params &p) {
   if (p.idamp < 0 || p.idamp > 2) return fail(LNS3D_EINVAL, "idamp must be 0, 1 or 2 (src/damper.f90:26-47)");
   if ((p.use_calcd != 0 && p.use_calcd != 1) || (p.update_lhs != 0 && p.update_lhs != 1))
     return fail(LNS3D_EINVAL, "use_calcd and update_lhs are 0 or 1");
-  if (p.precision == 1 && p.Re == 0.0)
-    return fail(LNS3D_EUNSUPPORTED, "complex-float factors with the inviscid deck (Re = 0) are not implemented");
   if (p.wall < 0 || p.wall > 3 || p.wallt < 0 || p.wallt > 2 || (p.top != 0 && p.top != 1) ||
       (p.left != 0 && p.left != 1 && p.left != 4 && p.left != 5 && p.left != 7) ||
       (p.right != 0 && p.right != 4 && p.right != 5 && p.right != 7 && p.right != 8 && p.right != 9) ||

@@ -1600,6 +1600,17 @@ __device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs 
     wd[k] = c.sbuf ? damp_w(posc, N, per, symrow ? A.sym : 0, k, 0) : 0.0;
     wdm[k] = c.sbuf ? damp_w(posc, N, per, symrow ? A.sym : 0, k, 2) : 0.0;
   }
+  const int ir = dir == 1 ? posc : i, jr = dir == 1 ? j : posc;   // the row point
+  // the inviscid wall zone (as lhs_entry_bc): rows of is:ie on the wall line
+  // (xi factor) or within two rows of it (eta factor).  A block whose column
+  // is rotated to the wall normal (b1, b2) acts on the column's values as the
+  // unrotated block on (b2 x1 + b1 x2, -b1 x1 + b2 x2); the normal is that of
+  // the non-periodic slot map's point (xi) or of the row's wall point (eta)
+  const bool lrz = !p.xper && ((ir == 0 && (p.left == 0 || p.left == 4 || p.left == 5)) ||
+                               (ir == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5 ||
+                                                 (p.right == 8 && dir == 1))));
+  const bool zone = c.bnb && !p.yper && !lrz && ir >= c.wlo && ir <= c.whi &&
+                    ((dir == 1 && jr == 0) || (dir == 2 && jr <= 2));
   z d1v[5], d2v[5], xc[5], d4v[5];
   for (int e = 0; e < 5; e++) { d1v[e] = z{0, 0}; d2v[e] = z{0, 0}; d4v[e] = z{0, 0}; }
   for (int k = 0; k < 5; k++) {
@@ -1608,8 +1619,26 @@ __device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs 
     const int col = symrow ? posc - 2 + k : slot_col(N, per, posc, k);
     if (symrow && (col < 0 || col > N - 1)) continue;
     const long qc = dir == 1 ? (col + (long)nx * j) : (i + (long)nx * col);
+    z xs[5];
+    for (int e = 0; e < 5; e++) xs[e] = ld(x + 5 * qc + e);
+    if (zone) {
+      int ncol = -1;
+      if (dir == 1) {
+        const int t0[5] = {3, 4, 0, 1, 2}, t1[5] = {4, 0, 1, 2, 3};
+        const int tm[5] = {nx - 4, nx - 3, nx - 2, nx - 1, nx - 5}, tn[5] = {nx - 3, nx - 2, nx - 1, nx - 5, nx - 4};
+        ncol = ir == 0 ? t0[k] : ir == 1 ? t1[k] : ir == nx - 2 ? tm[k] : ir == nx - 1 ? tn[k] : ir - 2 + k;
+      } else if (k == 2 - jr) {
+        ncol = ir;
+      }
+      if (ncol >= 0) {
+        const double b1 = c.bnb[ncol], b2 = c.bnb[nx + ncol];
+        const z u = xs[1], w = xs[2];
+        xs[1] = sc(b2, u) + sc(b1, w);
+        xs[2] = sc(-b1, u) + sc(b2, w);
+      }
+    }
     for (int e = 0; e < 5; e++) {
-      const z xv = ld(x + 5 * qc + e);
+      const z xv = xs[e];
       d1v[e] = d1v[e] + sc(e == 2 ? w1m[k] : w1[k], xv);
       d2v[e] = d2v[e] + sc(e == 2 ? w2m[k] : w2[k], xv);
       d4v[e] = d4v[e] + sc(e == 2 ? wdm[k] : wd[k], xv);
@@ -1651,8 +1680,7 @@ __device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs 
       const double f = p.alfa * dt;
       out[r] = out[r] + sc(f, a) + z{-f * b.im, f * b.re};
     }
-  } else if (c.spl) {                  // sponge of the factor without the D terms (complex-float steps:
-                                       // the viscous wall rows only, lns3d_ctx_create refuses Re = 0)
+  } else if (c.spl) {                  // sponge of the factor without the D terms
     const double f = p.alfa * dt * c.spl[qr];
     for (int r = 0; r < 5; r++) out[r] = out[r] + sc(f, xc[r]);
   }
@@ -1661,17 +1689,23 @@ __device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs 
     for (int r = 0; r < 5; r++) out[r] = out[r] + sc(f, d4v[r]);
   }
   for (int r = 0; r < 5; r++) out[r] = out[r] + xc[r];
+  if (zone && jr == 0) {               // the wall row: u rotated, the normal velocity held
+    const double b1 = c.bnb[ir], b2 = c.bnb[nx + ir];
+    out[1] = sc(b2, out[1]) - sc(b1, out[2]);
+    out[2] = ld(x + 5 * qr + 2);
+  }
   // boundary-condition rows (src/lhsbc1f3d.f90, lhsbc2f3d.f90), as lhs_entry
+  // (the identity rows take the unrotated values: none lies in the zone)
+  for (int e = 0; e < 5; e++) xc[e] = ld(x + 5 * qr + e);
   bool ident[5] = {false, false, false, false, false};
   bool wallrow0 = false, wallrow4 = false;
-  const int ir = dir == 1 ? posc : i, jr = dir == 1 ? j : posc;
   const bool lrrow = !p.xper && ((ir == 0 && (p.left == 0 || p.left == 4 || p.left == 5)) ||
                                   (ir == nx - 1 && (p.right == 0 || p.right == 4 || p.right == 5)));
   if (lrrow) for (int r = 0; r < 5; r++) ident[r] = true;
   const bool holdrow = !p.xper && dir == 1 && ir == nx - 1 && p.right == 8;   // -I
   if (!p.yper) {
     if (jr == ny - 1) for (int r = 0; r < 5; r++) ident[r] = true;
-    if (jr == 0 && ir >= c.wlo && ir <= c.whi) {
+    if (jr == 0 && !c.bnb && ir >= c.wlo && ir <= c.whi) {   // the viscous wall rows
       ident[1] = ident[2] = ident[3] = true;
       if (dir == 1 && (p.wall == 1 || p.wall == 2 || p.wall == 3)) ident[0] = true;
       if (dir == 2 && (p.wall == 1 || p.wall == 3)) wallrow0 = true;

@@ -181,7 +181,6 @@ def test_serving_arrivals_and_fifo():
     (dict(wall_is=-1), 1, b"wall extent"),
     (dict(idamp=3), 1, b"idamp"),
     (dict(use_calcd=2), 1, b"use_calcd"),
-    (dict(Re=0.0, precision=1), 4, b"inviscid"),
 ])
 def test_params_refused_before_any_device_call(lib, change, code, text):
     """lns3d_ctx_create checks the deck before it touches the device: values

@@ -84,31 +84,39 @@ def test_extras_against_reference(name):
     st.close()
 
 
-def test_calcd_update_complex_float():
-    """useCalcd with updateLHS and complex-float factors: the refined step
-    (defect correction to 1e-10 of the complex-double AF system) forms both
-    precisions' factors again as the D terms change factor, and follows the
-    reference's three steps to 1e-9"""
+@pytest.mark.parametrize("name", ["calcd_update", "inviscid", "inviscid_isie"])
+def test_complex_float_refined_against_reference(name):
+    """complex-float factors under the refined step (defect correction to
+    1e-10 of the complex-double AF system, whose operator regenerates the
+    blocks in double): useCalcd with updateLHS forms both precisions' factors
+    again as the D terms change factor; the inviscid deck's operator applies
+    the rotated wall rows of the assembled factors (src/lhsbc1f3d.f90,
+    src/lhsbc2f3d.f90) and bcfix follows the solve.  The reference's steps
+    to 1e-9"""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from lns3d_amd.config import Params
     from lns3d_amd.native import Step
-    z = np.load(os.path.join(GOLD, "ref_extra_calcd_update.npz"), allow_pickle=False)
+    z = np.load(os.path.join(GOLD, f"ref_extra_{name}.npz"), allow_pickle=False)
     p = Params(**json.loads(str(z["params"])))
     met, vm, v0 = np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]), z["v0"]
+    print(f"complex float {name}: {p.nx}x{p.ny} Re={p.Re} is:ie={p.wall_is}:{p.wall_ie}", flush=True)
 
     def dev(a):
         return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
 
     st = Step(p, precision=1)
     st.setup(met, vm)
-    st.set_sponge(1, z["spg"])
+    if "spg" in z.files:
+        st.set_sponge(1, z["spg"])
+    if "uw" in z.files:
+        st.set_wall(z["uw"])
     vd = dev(v0)
     st.itrbc3d(vd)
     st.factor()
     dvd = torch.zeros_like(vd)
-    for k in range(1, 4):
+    for k in range(1, (4 if p.update_lhs else 3)):
         iters, res = st.step_refined(vd, dvd, tol=1e-10)
         dv = dvd.cpu().numpy().reshape(v0.shape, order="F")
         e = rel(dv, z[f"dv{k}"])
@@ -119,7 +127,7 @@ def test_calcd_update_complex_float():
 
 def test_inviscid_refusals():
     """the inviscid deck needs the wall normals (lns3d_setup with the metric)
-    before its factors and its boundary rows; the complex-float factors refuse it"""
+    before its factors and its boundary rows"""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
@@ -136,5 +144,3 @@ def test_inviscid_refusals():
     with pytest.raises(LNS3DError):
         st.itrbc3d(vd)
     st.close()
-    with pytest.raises(LNS3DError):
-        Step(p, precision=1)
