@@ -943,7 +943,8 @@ __global__ void k_rhs_d1x(int nx, int ny, int xper, Stencil st, const double2 *v
 // of src/lrhs3d.f90:45-220 (U,xi and U,eta; U; U,xixi; U,xieta; U,etaeta;
 // A_i/B_i; D_i), then rbe3D and rhsBC3D (impdrv3d.f90:444-470,
 // src/rhsbc3d.f90:34-160, 200-215).
-constexpr int RT = 64;
+constexpr int RT = 64;   // the interior path's task split (320 threads, 512 tasks) assumes 64
+static_assert(RT == 64, "k_rhs_point_t's interior path assumes tiles of 64 points");
 constexpr int NDER = 17;
 
 struct StepBatchItem;
@@ -979,9 +980,54 @@ __global__ __launch_bounds__(320) void k_rhs_point_t(lns3d_params p, Stencil st,
     sg1[tid] = c1a[5 * q0 + tid];
   }
   // stencil derivatives into LDS, one lane per (point, component): slots
-  // 0-4 U_eta, 5-8 U_xixi, 9-12 U_xieta, 13-16 U_etaeta (components 1-4);
-  // the task ranges of the four kinds are multiples of 64, so each wave
-  // works on one kind at a time
+  // 0-4 U_eta, 5-8 U_xixi, 9-12 U_xieta, 13-16 U_etaeta (components 1-4).
+  // A tile within one row whose points all lie inside both directions'
+  // closures (no periodic direction) takes the interior stencils without
+  // branches: U_eta and U_etaeta from one 7-row gather of v, then U_xixi or
+  // U_xieta, then U_xieta again (threads below 192), each summed in the
+  // interior formula's order (the bits of d1, d2).  Other tiles loop over
+  // the tasks, whose four kinds span multiples of 64, so each wave works on
+  // one kind at a time.
+  bool inner = false;
+  if (!p.xper && !p.yper && npt == RT) {
+    const int i0 = (int)(q0 % nx), j0 = (int)(q0 / nx);
+    const int i1 = (int)((q0 + RT - 1) % nx), j1 = (int)((q0 + RT - 1) / nx);
+    inner = j0 == j1 && j0 >= 3 && j0 <= ny - 4 && i0 >= 3 && i1 <= nx - 4;
+  }
+  if (inner) {
+    const long es = 5L * nx;
+    {
+      const int pt = tid / 5, e = tid - 5 * pt;
+      const double2 *b = v + 5 * (q0 + pt) + e;
+      z x[7];
+#pragma unroll
+      for (int k = 0; k < 7; k++) x[k] = ld(b + (k - 3) * es);
+      const z h1 = sc(st.gy[0], x[0]) + sc(st.gy[1], x[1]) + sc(st.gy[2], x[2]) + sc(st.gy[3], x[4]) +
+                   sc(st.gy[4], x[5]) + sc(st.gy[5], x[6]);
+      const z h3 = sc(st.dy[0], x[0]) + sc(st.dy[1], x[1]) + sc(st.dy[2], x[2]) + sc(st.dy[3], x[3]) +
+                   sc(st.dy[4], x[4]) + sc(st.dy[5], x[5]) + sc(st.dy[6], x[6]);
+      sd[e * RT + pt] = make_double2(h1.re, h1.im);
+      if (e >= 1) sd[(12 + e) * RT + pt] = make_double2(h3.re, h3.im);
+    }
+#pragma unroll
+    for (int sl = 0; sl < 2; sl++) {
+      // task u of 512: U_xixi (u < 256), then U_xieta; components 1-4
+      const int u0 = tid + 5 * 64 * sl, u = min(u0, 511);
+      const bool xx = u < 256;
+      const int w4 = u & 255, pt = w4 >> 2, e = 1 + (w4 & 3);
+      const double2 *b = (xx ? v : c1a) + 5 * (q0 + pt) + e;
+      const long s1 = xx ? 5L : es;
+      z x[7];
+#pragma unroll
+      for (int k = 0; k < 7; k++) x[k] = ld(b + (k - 3) * s1);
+      const double *wv = xx ? st.dx : st.gy;
+      z h = sc(wv[0], x[0]) + sc(wv[1], x[1]) + sc(wv[2], x[2]);
+      if (xx) h = h + sc(wv[3], x[3]);
+      const int o = xx ? 4 : 3;
+      h = h + sc(wv[o], x[4]) + sc(wv[o + 1], x[5]) + sc(wv[o + 2], x[6]);
+      if (u0 < 512) sd[((xx ? 4 : 8) + e) * RT + pt] = make_double2(h.re, h.im);
+    }
+  } else
   for (int tsk = tid; tsk < 5 * RT + 3 * 4 * RT; tsk += 5 * 64) {
     int kind, pt, e;
     if (tsk < 5 * RT) { kind = 0; pt = tsk / 5; e = tsk - 5 * pt; }
