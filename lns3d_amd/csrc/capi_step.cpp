@@ -947,8 +947,8 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
                                 int corrections, lns3d_graph **out) {
   for (int i = 0; i < count; i++) {
     lns3d_ctx *c = ctxs[i];
-    if (c->px32.periodic || c->py32.periodic || !c->navier)
-      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create mode 3: periodic lines and the inviscid deck take mode 2");
+    if (c->px32.periodic || c->py32.periodic)
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create mode 3: periodic lines take mode 2");
   }
   auto *gr = new lns3d_graph;
   std::vector<StepBatchMember> m((size_t)count);
@@ -988,6 +988,9 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
     rc = fail(LNS3D_EHIP, "hipStreamBeginCapture");
   const bool capturing = !rc;
   const hipStream_t s = gr->s;
+  // members with the inviscid wall rows take bcfix on the final increment
+  // (as launch_update_bc): dv += t2, bcfix, v += dv instead of the fused update
+  const bool fix = !rc && gr->sb.blocks[SB_STAGE_BCFIX] > 0;
   for (int k = 0; k < nsteps && !rc; k++) {
     rc = step_batch_rhs(gr->sb, s);
     if (!rc) rc = step_batch_norm(gr->sb, k, s);
@@ -999,9 +1002,10 @@ static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, i
       if (!rc) rc = step_batch_vec(gr->sb, SB_SUB, s);
       if (!rc) rc = penta_batch_solve(gr->gx, 1, s);
       if (!rc) rc = penta_batch_solve(gr->gy, 1, s);
-      if (!rc) rc = step_batch_vec(gr->sb, q == corrections - 1 ? SB_UPDATE : SB_ADD_DV, s);
+      if (!rc) rc = step_batch_vec(gr->sb, q == corrections - 1 && !fix ? SB_UPDATE : SB_ADD_DV, s);
     }
-    if (!rc && corrections == 0) rc = step_batch_vec(gr->sb, SB_ADD_V, s);
+    if (!rc && fix) rc = step_batch_bcfix(gr->sb, s);
+    if (!rc && (corrections == 0 || fix)) rc = step_batch_vec(gr->sb, SB_ADD_V, s);
     if (!rc) rc = step_batch_itrbc(gr->sb, s);
   }
   if (capturing) {

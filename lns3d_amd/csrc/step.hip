@@ -1335,14 +1335,17 @@ __global__ void k_itrbc_wall_top(lns3d_params p, const double *ftop, const doubl
 
 // bcfix (src/impdrv3d.f90:333-336, src/bcfix.f90): the wall row of the
 // increment back from the body normal coordinates, on every wall point
-__global__ void k_bcfix(int nx, const double *bnb, double2 *r) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nx) return;
+__device__ __forceinline__ void bcfix_point(int i, int nx, const double *bnb, double2 *r) {
   const double b1 = bnb[i], b2 = bnb[nx + i];
   const z a = ld(r + 5 * i + 1), b = ld(r + 5 * i + 2);
   const z t = sc(b2, a) + sc(b1, b), u = sc(-b1, a) + sc(b2, b);
   r[5 * i + 1] = make_double2(t.re, t.im);
   r[5 * i + 2] = make_double2(u.re, u.im);
+}
+__global__ void k_bcfix(int nx, const double *bnb, double2 *r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  bcfix_point(i, nx, bnb, r);
 }
 
 // the wall normals (src/gengrid.f90:133-137) from the metric's n1, n2 on j = 0
@@ -1993,7 +1996,6 @@ int step_batch_build(const StepBatchMember *m, int count, StepBatch &out, std::v
   std::vector<std::vector<int>> off(SB_NSTAGE, std::vector<int>(1, 0));
   for (int k = 0; k < count; k++) {
     const StepBatchMember &a = m[k];
-    if (a.c.bnb) return set_error(__FILE__, __LINE__, "batched steps: the inviscid wall rows take the per-member steps");
     StepBatchItem &it = items[(size_t)k];
     it.p = a.p;
     it.st = a.st;
@@ -2012,8 +2014,9 @@ int step_batch_build(const StepBatchMember *m, int count, StepBatch &out, std::v
     it.norm_idx = a.norm_idx;
     if (a.norm && a.norm_count < 1) return set_error(__FILE__, __LINE__, "batched steps: a norm output of no slots");
     for (int sg = 0; sg < SB_NSTAGE; sg++) {
-      const long nb = sg != SB_STAGE_NORM ? stage_blocks(sg, a.p)
-                                          : (a.norm ? std::min<long>(nblocks(it.P, 256), NORM_BLOCKS) : 0);
+      const long nb = sg == SB_STAGE_NORM    ? (a.norm ? std::min<long>(nblocks(it.P, 256), NORM_BLOCKS) : 0)
+                      : sg == SB_STAGE_BCFIX ? ((a.c.bnb && !a.p.yper) ? nblocks(a.p.nx, 128) : 0)
+                                             : stage_blocks(sg, a.p);
       if ((long)off[sg].back() + nb > 0x7fffffffL) return set_error(__FILE__, __LINE__, "batch too large");
       off[sg].push_back(off[sg].back() + (int)nb);
     }
@@ -2072,6 +2075,21 @@ int step_batch_matvec(const StepBatch &b, hipStream_t s) {
   return 0;
 }
 
+// bcfix of the members with the inviscid wall rows (their wall points)
+__global__ void kb_bcfix(const StepBatchItem *items, const int *off, int count) {
+  const int k = step_member(off, count);
+  const StepBatchItem &b = items[k];
+  const int i = (int)(((long)blockIdx.x - off[k]) * blockDim.x + threadIdx.x);
+  if (i >= b.p.nx) return;
+  bcfix_point(i, b.p.nx, b.c.bnb, b.buf[SB_DV]);
+}
+int step_batch_bcfix(const StepBatch &b, hipStream_t s) {
+  if (b.blocks[SB_STAGE_BCFIX] == 0) return 0;
+  hipLaunchKernelGGL(kb_bcfix, dim3((unsigned)b.blocks[SB_STAGE_BCFIX]), dim3(128), 0, s,
+                     (const StepBatchItem *)b.items, b.off[SB_STAGE_BCFIX], b.count);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
 int step_batch_vec(const StepBatch &b, int op, hipStream_t s) {
   hipLaunchKernelGGL(kb_vec, dim3((unsigned)b.blocks[SB_STAGE_VEC]), dim3(256), 0, s,
                      (const StepBatchItem *)b.items, b.off[SB_STAGE_VEC], b.count, op);

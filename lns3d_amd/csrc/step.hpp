@@ -129,7 +129,7 @@ enum { SB_V = 0, SB_DV, SB_RHS, SB_T1, SB_T2, SB_WORK };
 // v += dv; v += dv
 enum { SB_COPY = 0, SB_SUB, SB_ADD_DV, SB_UPDATE, SB_ADD_V };
 enum { SB_STAGE_D1X = 0, SB_STAGE_POINT, SB_STAGE_LINE, SB_STAGE_VEC, SB_STAGE_WALL, SB_STAGE_LR, SB_STAGE_NORM,
-       SB_NSTAGE };
+       SB_STAGE_BCFIX, SB_NSTAGE };
 // norm (optional): the member's residual-norm output of norm_count slots;
 // step k of a captured sequence writes slot (norm_idx + k) mod norm_count
 struct StepBatchMember {
@@ -162,5 +162,8 @@ int step_batch_itrbc(const StepBatch &b, hipStream_t s);
 // the residual norm sqrt(sum |rhs|^2 / dtl^2) of every member with a norm
 // output, into its slot for step k (the bits of launch_residual_norm)
 int step_batch_norm(const StepBatch &b, long k, hipStream_t s);
+// bcfix on the increments of the members with the inviscid wall rows (no
+// launch when none has them); between dv += t2 and v += dv
+int step_batch_bcfix(const StepBatch &b, hipStream_t s);
 
 }  // namespace lns3d

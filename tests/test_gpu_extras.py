@@ -125,6 +125,57 @@ def test_complex_float_refined_against_reference(name):
     st.close()
 
 
+@pytest.mark.parametrize("corrections", [1, 0])
+def test_staged_inviscid_matches_branches(corrections):
+    """a staged graph (one launch per stage over all contexts) of two
+    inviscid contexts (whole wall, is:ie) and one viscous one (is:ie): the
+    inviscid members' rotated wall rows in the residual, the operator and
+    itrBC3D, and bcfix between the last correction and the update; three
+    steps give the states of the same steps on one branch per context, bit
+    for bit"""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Step, Graph, MODE_FP32, MODE_REFINED, MODE_BATCHED
+    names = ["inviscid", "inviscid_isie", "wall_isie"]
+    print(f"staged inviscid corrections={corrections}", flush=True)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+
+    steps, v0s = [], []
+    for name in names:
+        z = np.load(os.path.join(GOLD, f"ref_extra_{name}.npz"), allow_pickle=False)
+        p = Params(**json.loads(str(z["params"])))
+        st = Step(p, precision=1)
+        st.setup(np.asfortranarray(z["met"]), np.asfortranarray(z["vm"]))
+        if "spg" in z.files:
+            st.set_sponge(1, z["spg"])
+        if "uw" in z.files:
+            st.set_wall(z["uw"])
+        v = dev(z["v0"])
+        st.itrbc3d(v)
+        st.factor()
+        steps.append(st)
+        v0s.append(v)
+    out = {}
+    for mode in (MODE_REFINED if corrections else MODE_FP32, MODE_BATCHED):
+        vs = [v.clone() for v in v0s]
+        dvs = [torch.zeros_like(v) for v in v0s]
+        g = Graph(steps, vs, dvs, nsteps=3, mode=mode, corrections=corrections)
+        g.launch()
+        g.synchronize()
+        out[mode] = vs
+        g.close()
+    a, b = out.values()
+    for k, name in enumerate(names):
+        assert bool(torch.isfinite(b[k]).all()), name
+        assert torch.equal(a[k], b[k]), name
+    for st in steps:
+        st.close()
+
+
 def test_inviscid_refusals():
     """the inviscid deck needs the wall normals (lns3d_setup with the metric)
     before its factors and its boundary rows"""
