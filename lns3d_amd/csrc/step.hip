@@ -988,8 +988,11 @@ __global__ __launch_bounds__(320) void k_rhs_point_t(lns3d_params p, Stencil st,
   // interior formula's order (the bits of d1, d2).  Other tiles loop over
   // the tasks, whose four kinds span multiples of 64, so each wave works on
   // one kind at a time.
+  // (the staged batch's kernel keeps the task loop: the interior path raises
+  // it from 78 to 97 VGPRs, six to four waves per SIMD, and its small grids
+  // have few such tiles)
   bool inner = false;
-  if (!p.xper && !p.yper && npt == RT) {
+  if (!BATCH && !p.xper && !p.yper && npt == RT) {
     const int i0 = (int)(q0 % nx), j0 = (int)(q0 / nx);
     const int i1 = (int)((q0 + RT - 1) % nx), j1 = (int)((q0 + RT - 1) / nx);
     inner = j0 == j1 && j0 >= 3 && j0 <= ny - 4 && i0 >= 3 && i1 <= nx - 4;
