@@ -370,7 +370,11 @@ int lns3d_step(lns3d_ctx *c, void *v_dev, void *dv_dev);
  * |res|_max <= tol |r|_max or max_iters solves (max_iters <= 0: up to 64).
  * The update is applied either way: *resid_out > tol on return says that
  * the solve count ran out first.  A non-finite residual returns
- * LNS3D_ENONFINITE and leaves v unchanged. */
+ * LNS3D_ENONFINITE and leaves v unchanged.  Periodic lines: complex-float
+ * records rounded from the complex-double ones; the refinement converges to
+ * the cyclic system's solution (with periodic_exact = 0 the shipped factor
+ * misses it on short lines), and the increment's duplicate periodic column
+ * (xper, which itrBC3D overwrites in v) keeps complex-float accuracy. */
 int lns3d_step_refined(lns3d_ctx *c, void *v_dev, void *dv_dev, double tol, int max_iters,
                        int *iters_out, double *resid_out);
 /* the step with complex-float factors and no inner refinement */

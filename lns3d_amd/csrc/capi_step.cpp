@@ -65,8 +65,6 @@ static int check_params(const lns3d_params &p) {
                   (p.right == 7 && p.nx - 1 + (long)p.ny * (p.ny - 1) >= P)))
     return fail(LNS3D_EUNSUPPORTED, "left/right = 7 needs nx >= ny (the reference reads off the grid otherwise)");
   if (p.precision != 0 && p.precision != 1) return fail(LNS3D_EINVAL, "precision must be 0 or 1");
-  if ((p.xper || p.yper) && p.precision == 1)
-    return fail(LNS3D_EUNSUPPORTED, "complex-float factors are for non-periodic lines");
   return 0;
 }
 
@@ -520,9 +518,12 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
         if (penta_twisted(v) == (d.h >= 0)) return v;
         return d.h >= 0 ? (int)PENTA_TWISTED_PAIRS : (lines >= 512 ? (int)PENTA_PLAIN3 : (int)PENTA_PLAIN1);
       };
-      rc = penta_plan_create_variant(&c->px32, 0, 0, c->p.nx, c->p.ny, 1, family_variant(c->px, c->p.nx, c->p.ny));
+      // periodic lines: complex-float records rounded from the complex-double ones
+      rc = penta_plan_create_variant(&c->px32, c->p.xper, 0, c->p.nx, c->p.ny, 1,
+                                     c->p.xper ? 0 : family_variant(c->px, c->p.nx, c->p.ny));
       if (!rc)
-        rc = penta_plan_create_variant(&c->py32, 0, 0, c->p.ny, c->p.nx, 1, family_variant(c->py, c->p.ny, c->p.nx));
+        rc = penta_plan_create_variant(&c->py32, c->p.yper, 0, c->p.ny, c->p.nx, 1,
+                                       c->p.yper ? 0 : family_variant(c->py, c->p.ny, c->p.nx));
       if (!rc && (c->px32.h != c->px.h || c->py32.h != c->py.h))
         rc = fail(LNS3D_EHIP, "lns3d_factor: complex-float plans split their lines unlike the complex-double LU");
       const long P = (long)c->p.nx * c->p.ny;
@@ -537,15 +538,17 @@ extern "C" int lns3d_factor(lns3d_ctx *c) {
       if (rc) return rc;
       c->have32 = true;
     }
-    // the complex-float records come from the same complex-double LU
-    std::swap(c->px32.lu, c->px.lu);
-    rc = penta_factor_records_only(&c->px32, c->stream);
-    std::swap(c->px32.lu, c->px.lu);
-    if (!rc) {
-      std::swap(c->py32.lu, c->py.lu);
-      rc = penta_factor_records_only(&c->py32, c->stream);
-      std::swap(c->py32.lu, c->py.lu);
-    }
+    // the complex-float records come from the same complex-double LU (or,
+    // periodic lines, from its records)
+    auto records32 = [&](PentaPlan &d32, PentaPlan &d64) {
+      if (d32.periodic) return penta_records_narrow(&d32, &d64, c->stream);
+      std::swap(d32.lu, d64.lu);
+      const int r = penta_factor_records_only(&d32, c->stream);
+      std::swap(d32.lu, d64.lu);
+      return r;
+    };
+    rc = records32(c->px32, c->px);
+    if (!rc) rc = records32(c->py32, c->py);
     if (rc) return rc;
   }
   // factor health, both directions (and the complex-float records)

@@ -1741,13 +1741,14 @@ __global__ __launch_bounds__(128) void k_records_per(LineGeom g, cx<double> *mat
 //            the loaders formed from p2 and the computed y;
 //   backward: rows n-1, n-2 first, then rows n-3..0 with d_i including
 //            qp_i (x_{n-2}, x_{n-1}); finally x(np) = x(1) (cpenta1p.f:1037).
-template <int R, int G>
-__global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<double> *__restrict__ fwd,
-                                                  const cx<double> *__restrict__ bwd,
-                                                  const cx<double> *__restrict__ p2,
-                                                  const cx<double> *__restrict__ qp, double2 *rhs,
+// T: the records' precision (complex float records are the complex-double
+// ones rounded; the right-hand side and solution stay complex double)
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<T> *__restrict__ fwd,
+                                                  const cx<T> *__restrict__ bwd,
+                                                  const cx<T> *__restrict__ p2,
+                                                  const cx<T> *__restrict__ qp, double2 *rhs,
                                                   long sL, long sR) {
-  using T = double;
   constexpr int RECW = 66;
   constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
   __shared__ __attribute__((aligned(1024))) cx<T> rec[2][G][BUFE];
@@ -1766,7 +1767,7 @@ __global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<doubl
   double2 *X = rhs + line * sL * 5;
   const int nreg = n - 2;                         // rows 0..n-3: band recurrence only
   const int nck = (nreg + R - 1) / R;
-  auto zx = [](double2 a) { return cx<T>{a.x, a.y}; };
+  auto zx = [](double2 a) { return cx<T>{(T)a.x, (T)a.y}; };
 
   // ---- loaders: record rows [r0, r0+nr) of pass fw into buffer kk, with
   // c = Linv b (forward) or d = Uinv y + qp (x_{n-2}, x_{n-1}) (backward)
@@ -2066,7 +2067,7 @@ int penta_check_launch(PentaPlan *pl, hipStream_t s) {
 // records of a plan whose LU another plan holds (the complex-float records
 // of the refined step): their entries, per line
 int penta_check_records_launch(PentaPlan *pl, hipStream_t s) {
-  if (pl->periodic || !pl->prec) return 0;
+  if (!pl->prec) return 0;
   if (!pl->flags) LNS3D_CHECK(hipMalloc(&pl->flags, (size_t)pl->L * sizeof(int)));
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(pl->L, 256)), dim3(256), 0, s, (int *)pl->flags, pl->L);
   const long n = pl->n, L = pl->L;
@@ -2075,6 +2076,13 @@ int penta_check_records_launch(PentaPlan *pl, hipStream_t s) {
                      (const cx<float> *)pl->fwd, tf, L, 1, n * 60, 0, (int *)pl->flags);
   hipLaunchKernelGGL(k_check_factor<float>, dim3(check_launch_grid(tb)), dim3(256), 0, s,
                      (const cx<float> *)pl->bwd, tb, L, 1, n * 66, 0, (int *)pl->flags);
+  if (pl->periodic) {         // the border records, rounded too
+    const long tq = n * 50 * L;
+    hipLaunchKernelGGL(k_check_factor<float>, dim3(check_launch_grid(tq)), dim3(256), 0, s,
+                       (const cx<float> *)pl->p2, tq, L, 1, n * 50, 0, (int *)pl->flags);
+    hipLaunchKernelGGL(k_check_factor<float>, dim3(check_launch_grid(tq)), dim3(256), 0, s,
+                       (const cx<float> *)pl->qp, tq, L, 1, n * 50, 0, (int *)pl->flags);
+  }
   LNS3D_CHECK(hipGetLastError());
   return 0;
 }
@@ -2139,9 +2147,13 @@ int penta_plan_create_variant(PentaPlan *pl, int periodic, int exact, int n_rows
   pl->per = pl->per2 = nullptr;
   pl->fwd = pl->bwd = pl->tail = pl->mid = pl->p2 = pl->qp = pl->xf = pl->xb = pl->flags = nullptr;
   if (periodic) {
-    const size_t z = sizeof(cx<double>);
-    LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * z));
-    LNS3D_CHECK(hipMalloc(&pl->per2, (size_t)pl->np * 50 * lines * z));
+    // complex-float plans take their records from a complex-double plan
+    // (penta_records_narrow): no bordered LU of their own
+    const size_t z = prec ? sizeof(cx<float>) : sizeof(cx<double>);
+    if (!prec) {
+      LNS3D_CHECK(hipMalloc(&pl->per, (size_t)pl->np * 50 * lines * z));
+      LNS3D_CHECK(hipMalloc(&pl->per2, (size_t)pl->np * 50 * lines * z));
+    }
     LNS3D_CHECK(hipMalloc(&pl->fwd, (size_t)pl->n * 60 * lines * z));
     LNS3D_CHECK(hipMalloc(&pl->bwd, (size_t)pl->n * 66 * lines * z));
     LNS3D_CHECK(hipMalloc(&pl->p2, (size_t)pl->n * 50 * lines * z));
@@ -2217,8 +2229,31 @@ int penta_factor_records_only(PentaPlan *pl, hipStream_t s) {
   return 0;
 }
 
+__global__ void k_narrow(const cx<double> *src, cx<float> *dst, long count) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const cx<double> a = src[t];
+  dst[t] = cx<float>{(float)a.re, (float)a.im};
+}
+
+int penta_records_narrow(PentaPlan *dst, const PentaPlan *src, hipStream_t s) {
+  if (!dst->periodic || !src->periodic || !dst->prec || src->prec || dst->n != src->n || dst->L != src->L)
+    return set_error(__FILE__, __LINE__, "penta_records_narrow: a complex-float and a complex-double periodic plan of one shape");
+  const long n = src->n, L = src->L;
+  const void *from[4] = {src->fwd, src->bwd, src->p2, src->qp};
+  void *to[4] = {dst->fwd, dst->bwd, dst->p2, dst->qp};
+  const long count[4] = {n * 60 * L, n * 66 * L, n * 50 * L, n * 50 * L};
+  for (int k = 0; k < 4; k++)
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(count[k], 256)), dim3(256), 0, s, (const cx<double> *)from[k],
+                       (cx<float> *)to[k], count[k]);
+  LNS3D_CHECK(hipGetLastError());
+  return 0;
+}
+
 int penta_factor(PentaPlan *pl, hipStream_t s) {
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L, pl->h};
+  if (pl->periodic && pl->prec)
+    return set_error(__FILE__, __LINE__, "periodic complex-float plans take their records from a complex-double plan");
   if (pl->periodic) {
     hipLaunchKernelGGL(k_lu_per, dim3((unsigned)pl->L), dim3(64), 0, s, g,
                        (cx<double> *)pl->lu, (cx<double> *)pl->per, (cx<double> *)pl->per2);
@@ -2284,13 +2319,20 @@ int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s) {
   LineGeom g{pl->n, pl->periodic, pl->exact, pl->L};
   if (pl->periodic) {
     (void)g;
-    auto *f = (const cx<double> *)pl->fwd; auto *b = (const cx<double> *)pl->bwd;
-    auto *p2 = (const cx<double> *)pl->p2; auto *qp = (const cx<double> *)pl->qp;
-    // one line per workgroup while two workgroups per CU hold every line
-    if (pl->L > 512)
-      hipLaunchKernelGGL((k_solve_per<6, 4>), dim3((unsigned)((pl->L + 3) / 4)), dim3(256), 0, s, pl->n, pl->L, f, b, p2, qp, (double2 *)rhs, sL, sR);
-    else
-      hipLaunchKernelGGL((k_solve_per<30, 1>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, pl->L, f, b, p2, qp, (double2 *)rhs, sL, sR);
+    auto per = [&](auto tag) {
+      using T = decltype(tag);
+      auto *f = (const cx<T> *)pl->fwd; auto *b = (const cx<T> *)pl->bwd;
+      auto *p2 = (const cx<T> *)pl->p2; auto *qp = (const cx<T> *)pl->qp;
+      // one line per workgroup while two workgroups per CU hold every line
+      if (pl->L > 512)
+        hipLaunchKernelGGL((k_solve_per<T, 6, 4>), dim3((unsigned)((pl->L + 3) / 4)), dim3(256), 0, s, pl->n, pl->L,
+                           f, b, p2, qp, (double2 *)rhs, sL, sR);
+      else
+        hipLaunchKernelGGL((k_solve_per<T, 30, 1>), dim3((unsigned)pl->L), dim3(256), 0, s, pl->n, pl->L, f, b, p2,
+                           qp, (double2 *)rhs, sL, sR);
+    };
+    if (pl->prec) per(float{});
+    else per(double{});
   } else {
     if (pl->prec) launch_solve_bc<float>(pl, rhs, sL, sR, s);
     else launch_solve_bc<double>(pl, rhs, sL, sR, s);

@@ -84,6 +84,9 @@ int penta_import(PentaPlan *pl, const void *ref_mat, long sLm, long sRm, hipStre
 int penta_factor(PentaPlan *pl, hipStream_t s);
 // records only, from an LU already in pl->lu (non-periodic lines)
 int penta_factor_records_only(PentaPlan *pl, hipStream_t s);
+// periodic lines: the complex-float records of dst, rounded from the
+// complex-double records of src (same shape, factored)
+int penta_records_narrow(PentaPlan *dst, const PentaPlan *src, hipStream_t s);
 // in-place solve of rhs (complex double); component d of (line l, row i) at
 // complex offset (l*sL + i*sR)*5 + d
 int penta_solve(PentaPlan *pl, void *rhs, long sL, long sR, hipStream_t s);

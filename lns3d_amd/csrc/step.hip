@@ -1781,10 +1781,13 @@ __device__ __forceinline__ void line_op(long bid, const AsmCtx &A, const Coeffs 
     if (wallrow0 && !ident[0]) out[0] = s0;
     if (wallrow4 && !ident[4]) out[4] = s4;
   }
+  // the duplicate row of a periodic line is no unknown of the factor (its
+  // solve copies row 0 there, its residual is zero): no operator row
+  const bool dup = per && pos == N - 1;
   for (int r = 0; r < 5; r++) {
     if (ident[r]) out[r] = xc[r];
     if (holdrow) out[r] = z{-xc[r].re, -xc[r].im};
-    y[5 * q + r] = make_double2(out[r].re, out[r].im);
+    y[5 * q + r] = dup ? make_double2(0.0, 0.0) : make_double2(out[r].re, out[r].im);
   }
 }
 __global__ __launch_bounds__(256) void k_line_op(AsmCtx A, Coeffs c, int dir, const double2 *x, double2 *y) {

@@ -65,3 +65,43 @@ def test_refined_step_matches_fp64(torch_cuda, nx, ny):
     e = float((dvb - dva).abs().max() / dva.abs().max())
     print(f"  iters {its} resid {res:.2e} dv rel diff {e:.2e}", flush=True)
     assert res <= 1e-12 and e < 1e-10
+
+
+@pytest.mark.parametrize("nx,ny,xper,yper", [(129, 65, 1, 0), (600, 40, 0, 1), (65, 600, 1, 0)])
+def test_refined_step_periodic_matches_fp64(torch_cuda, nx, ny, xper, yper):
+    """periodic lines: complex-float records rounded from the complex-double
+    bordered LU (one line per workgroup up to 512 lines, four above), the
+    operator without a row on the duplicate periodic row; the refined step's
+    increment equals the complex-double step's to 1e-10 (both with the exact
+    cyclic factor: the refinement converges to the cyclic system's solution,
+    which the shipped factor misses on short lines)"""
+    import dataclasses
+    from lns3d_amd.config import Params, make_grid, make_mean, make_disturbance
+    from lns3d_amd.native import Step
+    print(f"refined step {nx}x{ny} xper={xper} yper={yper}", flush=True)
+    torch = torch_cuda
+    p = Params(nx=nx, ny=ny, kz=0.3)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    v = make_disturbance(p, x, seed=2)
+    p = dataclasses.replace(p, xper=xper, yper=yper, periodic_exact=1)
+    st = Step(p, precision=1)
+    st.setup(met, vm)
+    st.factor()
+    va = _dev(torch, v)
+    st.itrbc3d(va)
+    vb = va.clone()
+    dva, dvb = torch.zeros_like(va), torch.zeros_like(va)
+    st.step(va, dva)
+    its, res = st.step_refined(vb, dvb, tol=1e-12, max_iters=10)
+    # unique points: the duplicate periodic column of the increment (xper)
+    # comes from its own eta line solved on a copied right-hand side, where
+    # the refinement has no residual row; v takes column 0 there (itrBC3D)
+    d, d0 = (dvb - dva).reshape(ny, nx, 5), dva.reshape(ny, nx, 5)
+    if xper:
+        d, d0 = d[:, :-1], d0[:, :-1]
+    e = float(d.abs().max() / d0.abs().max())
+    ev = float((vb - va).abs().max() / va.abs().max())
+    print(f"  iters {its} resid {res:.2e} dv rel diff {e:.2e} v rel diff {ev:.2e}", flush=True)
+    assert res <= 1e-12 and e < 1e-10 and ev < 1e-12
+    st.close()
