@@ -403,8 +403,8 @@ int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
  * (residual, residual norms, line solves grouped by solve kernel,
  * corrections, update), the same arithmetic; with corrections = 0 the steps
  * of mode 1; contexts of the inviscid deck take bcfix between the last
- * correction and the update, as their own steps do.  Non-periodic lines
- * (LNS3D_EUNSUPPORTED otherwise).  Contexts with a residual output
+ * correction and the update, as their own steps do, and periodic lines
+ * their periodic solve kernels.  Contexts with a residual output
  * (lns3d_ctx_set_residual_output) record each step's norm in their next
  * slot, with the bits of the per-context steps.  A mode-3 graph's tables are
  * copied to the device on the stream of its first launch, ahead of it

@@ -948,11 +948,6 @@ static void graph_free(lns3d_graph *gr) {
 // run over all members at once (the arithmetic of lns3d_step_refined_fixed)
 static int graph_create_batched(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps,
                                 int corrections, lns3d_graph **out) {
-  for (int i = 0; i < count; i++) {
-    lns3d_ctx *c = ctxs[i];
-    if (c->px32.periodic || c->py32.periodic)
-      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create mode 3: periodic lines take mode 2");
-  }
   auto *gr = new lns3d_graph;
   std::vector<StepBatchMember> m((size_t)count);
   std::vector<const PentaPlan *> px((size_t)count), py((size_t)count);

@@ -1742,13 +1742,12 @@ __global__ __launch_bounds__(128) void k_records_per(LineGeom g, cx<double> *mat
 //   backward: rows n-1, n-2 first, then rows n-3..0 with d_i including
 //            qp_i (x_{n-2}, x_{n-1}); finally x(np) = x(1) (cpenta1p.f:1037).
 // T: the records' precision (complex float records are the complex-double
-// ones rounded; the right-hand side and solution stay complex double)
+// ones rounded; the right-hand side and solution stay complex double).
+// Workgroup bid takes lines bid*G .. bid*G + G - 1.
 template <typename T, int R, int G>
-__global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<T> *__restrict__ fwd,
-                                                  const cx<T> *__restrict__ bwd,
-                                                  const cx<T> *__restrict__ p2,
-                                                  const cx<T> *__restrict__ qp, double2 *rhs,
-                                                  long sL, long sR) {
+__device__ __forceinline__ void solve_per(long bid, int n, long L, const cx<T> *__restrict__ fwd,
+                                          const cx<T> *__restrict__ bwd, const cx<T> *__restrict__ p2,
+                                          const cx<T> *__restrict__ qp, double2 *rhs, long sL, long sR) {
   constexpr int RECW = 66;
   constexpr int BUFE = ((R * RECW * (int)sizeof(cx<T>) + 1023) / 1024) * 1024 / (int)sizeof(cx<T>);
   __shared__ __attribute__((aligned(1024))) cx<T> rec[2][G][BUFE];
@@ -1757,7 +1756,7 @@ __global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<T> *_
   constexpr int NSUB = 192 / (10 * G);  // loader threads per border sum
   __shared__ cx<T> bpart[NSUB][10 * G];
   __shared__ cx<T> xb[G][2][5];       // x_{n-2}, x_{n-1}
-  const long line0 = (long)blockIdx.x * G;
+  const long line0 = bid * G;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = (lane >> 4) % G;
   const int m = min(lane & 15, 4);
@@ -1995,6 +1994,14 @@ __global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<T> *_
   }
   // x(np) = x(1)
   if (wave == 0 && (lane & 15) < 5) X[(long)n * rs + m] = X[m];
+}
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void k_solve_per(int n, long L, const cx<T> *__restrict__ fwd,
+                                                  const cx<T> *__restrict__ bwd,
+                                                  const cx<T> *__restrict__ p2,
+                                                  const cx<T> *__restrict__ qp, double2 *rhs,
+                                                  long sL, long sR) {
+  solve_per<T, R, G>(blockIdx.x, n, L, fwd, bwd, p2, qp, rhs, sL, sR);
 }
 
 // ---------------------------------------------------------------------------
@@ -2375,9 +2382,22 @@ __global__ __launch_bounds__(256, 2) void kb_solve_tw(const PentaBatchItem *item
                                        (const cx<T> *)b.xb, b.rhs[which], b.sL, b.sR);
 }
 
-// lines per workgroup of a non-periodic solve variant
+template <typename T, int R, int G>
+__global__ __launch_bounds__(256) void kb_solve_per(const PentaBatchItem *items, const int *off, int count,
+                                                   int which) {
+  const int k = batch_member(off, count);
+  const PentaBatchItem &b = items[k];
+  solve_per<T, R, G>((long)blockIdx.x - off[k], b.n, b.L, (const cx<T> *)b.fwd, (const cx<T> *)b.bwd,
+                     (const cx<T> *)b.p2, (const cx<T> *)b.qp, b.rhs[which], b.sL, b.sR);
+}
+
+// batch groups of periodic lines, by their solve kernel (as penta_solve
+// chooses it: four lines per workgroup above 512 lines, else one)
+constexpr int BATCH_PERIODIC1 = 1001, BATCH_PERIODIC4 = 1004;
+
+// lines per workgroup of a batch group's solve kernel
 static int variant_lines(int v) {
-  return v == PENTA_PLAIN2 ? 2 : v == PENTA_PLAIN3 ? 3 : v == PENTA_PLAIN4 ? 4 : 1;
+  return v == PENTA_PLAIN2 ? 2 : v == PENTA_PLAIN3 ? 3 : (v == PENTA_PLAIN4 || v == BATCH_PERIODIC4) ? 4 : 1;
 }
 
 int stage_upload(void *dev, const void *src, size_t bytes, std::vector<TableUpload> &ups) {
@@ -2395,8 +2415,8 @@ int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, doubl
   std::map<std::pair<int, int>, std::vector<int>> by;   // (variant, precision) -> members
   for (int k = 0; k < count; k++) {
     const PentaPlan *pl = plans[k];
-    if (pl->periodic) return set_error(__FILE__, __LINE__, "batched line solves take non-periodic lines");
-    by[{pl->variant, pl->prec}].push_back(k);
+    const int v = pl->periodic ? (pl->L > 512 ? BATCH_PERIODIC4 : BATCH_PERIODIC1) : pl->variant;
+    by[{v, pl->prec}].push_back(k);
   }
   for (auto &kv : by) {
     // longest lines first: their workgroups start in the first wave and the
@@ -2420,6 +2440,8 @@ int penta_batch_build(const PentaPlan *const *plans, double2 *const *rhs0, doubl
       it.tail = pl->tail;
       it.xf = pl->xf;
       it.xb = pl->xb;
+      it.p2 = pl->p2;
+      it.qp = pl->qp;
       it.rhs[0] = rhs0[k];
       it.rhs[1] = rhs1[k];
       it.sL = sL[k];
@@ -2472,6 +2494,12 @@ static void launch_batch_group(const PentaBatchGroup &g, int which, hipStream_t 
       break;
     case PENTA_PLAIN4:
       hipLaunchKernelGGL((kb_solve_bc<T, 6, 4>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    case BATCH_PERIODIC1:
+      hipLaunchKernelGGL((kb_solve_per<T, 30, 1>), grid, blk, 0, s, it, g.off, g.count, which);
+      break;
+    case BATCH_PERIODIC4:
+      hipLaunchKernelGGL((kb_solve_per<T, 6, 4>), grid, blk, 0, s, it, g.off, g.count, which);
       break;
     default:
       hipLaunchKernelGGL((kb_solve_bc<T, D ? 12 : 24, 1>), grid, blk, 0, s, it, g.off, g.count, which);

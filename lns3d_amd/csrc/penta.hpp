@@ -44,6 +44,7 @@ struct PentaBatchItem {
   int n, h;
   long L;
   const void *fwd, *bwd, *tail, *xf, *xb;
+  const void *p2, *qp;     // periodic lines: the border records
   double2 *rhs[2];
   long sL, sR;
 };

@@ -127,6 +127,35 @@ def test_staged_norms_match_branches(torch_cuda):
     b.close()
 
 
+@pytest.mark.parametrize("corrections", [1, 0])
+def test_staged_periodic_matches_branches(torch_cuda, corrections):
+    """a staged graph with periodic members (xper; yper with one and with
+    four lines per workgroup of the batched periodic solve, the latter above
+    512 lines) and a non-periodic one: three steps give the states of the
+    same steps on one branch per instance, bit for bit"""
+    from lns3d_amd.batch import Batch
+    from lns3d_amd.config import Params
+    from lns3d_amd.native import Graph, MODE_FP32, MODE_REFINED, MODE_BATCHED
+    torch = torch_cuda
+    params = [Params(nx=33, ny=17, kz=0.3, xper=1, periodic_exact=1), Params(nx=47, ny=21, kz=0.8, yper=1),
+              Params(nx=600, ny=24, kz=0.5, yper=1, periodic_exact=1), Params(nx=65, ny=33, kz=0.1)]
+    print(f"staged periodic corrections={corrections}", flush=True)
+    b = Batch(params, precision=1, seed=9)
+    out = {}
+    for mode in (MODE_REFINED if corrections else MODE_FP32, MODE_BATCHED):
+        b.reset()
+        g = Graph(b.steps, b.v, b.dv, nsteps=3, mode=mode, corrections=corrections)
+        g.launch()
+        g.synchronize()
+        out[mode] = [v.clone() for v in b.v]
+        g.close()
+    va, vb = out.values()
+    for k in range(len(params)):
+        assert bool(torch.isfinite(vb[k]).all()), k
+        assert torch.equal(va[k], vb[k]), k
+    b.close()
+
+
 def test_refined_fixed_matches_fp64(torch_cuda):
     from lns3d_amd.config import Params, make_grid, make_mean, make_disturbance
     from lns3d_amd.native import Step
