@@ -196,7 +196,8 @@ int lns3d_drop_cgrad2(int ndof, int nx, int ny, const void *v, const void *g1v, 
  * kind 0 = non-periodic (cpenta*bc), 1 = periodic (cpenta*p);
  * layout 1 = system index before row index (cpenta1*), 2 = after (cpenta2*);
  * nrows = n (kind 0) or np (kind 1); precision 0 = complex double factors,
- * 1 = complex float factors (kind 0 only); exact as above.
+ * 1 = complex float factors (kind 1: records rounded from the complex-
+ * double bordered LU, which the handle also holds); exact as above.
  * ====================================================================== */
 typedef struct lns3d_penta lns3d_penta;
 int lns3d_penta_create(int kind, int layout, int nsys, int nrows, int precision, int exact,

@@ -6,6 +6,7 @@
 #include <string.h>
 #include <map>
 #include <mutex>
+#include <algorithm>
 #include <string>
 #include <tuple>
 
@@ -35,6 +36,10 @@ using namespace lns3d;
 
 struct lns3d_penta {
   PentaPlan plan;
+  // periodic lines in complex float: the complex-double bordered LU, whose
+  // records are rounded into plan
+  PentaPlan plan64;
+  bool has64 = false;
   int layout = 1;
   int nsys = 0;
   int nrows = 0;   // n (bc) or np (periodic), as the caller passes it
@@ -46,7 +51,7 @@ extern "C" const char *lns3d_last_error(void) { return g_err.c_str(); }
 extern "C" int lns3d_penta_create(int kind, int layout, int nsys, int nrows, int precision,
                                   int exact, lns3d_penta **out) {
   if (!out || (kind != 0 && kind != 1) || (layout != 1 && layout != 2) || nsys < 1 ||
-      (precision != 0 && precision != 1) || (kind == 1 && precision != 0))
+      (precision != 0 && precision != 1))
     return fail(LNS3D_EINVAL, "lns3d_penta_create: bad arguments");
   int n_unknown = kind == 1 ? nrows - 1 : nrows;
   if (n_unknown < 7) return fail(LNS3D_EINVAL, "lns3d_penta_create: lines need >= 7 unknowns");
@@ -55,8 +60,13 @@ extern "C" int lns3d_penta_create(int kind, int layout, int nsys, int nrows, int
   h->nsys = nsys;
   h->nrows = nrows;
   int rc = penta_plan_create(&h->plan, kind, exact, nrows, nsys, precision);
+  if (!rc && kind == 1 && precision == 1) {
+    h->has64 = true;
+    rc = penta_plan_create(&h->plan64, kind, exact, nrows, nsys, 0);
+  }
   if (rc) {
     penta_plan_destroy(&h->plan);
+    if (h->has64) penta_plan_destroy(&h->plan64);
     delete h;
     return rc;
   }
@@ -67,6 +77,7 @@ extern "C" int lns3d_penta_create(int kind, int layout, int nsys, int nrows, int
 extern "C" int lns3d_penta_destroy(lns3d_penta *h) {
   if (!h) return 0;
   penta_plan_destroy(&h->plan);
+  if (h->has64) penta_plan_destroy(&h->plan64);
   delete h;
   return 0;
 }
@@ -81,12 +92,20 @@ extern "C" int lns3d_penta_factor(lns3d_penta *h, const void *mat_dev, void *str
   long sL, sR;
   ref_strides(h, &sL, &sR);
   hipStream_t s = (hipStream_t)stream;
-  int rc = penta_import(&h->plan, mat_dev, sL, sR, s);
-  if (!rc) rc = penta_factor(&h->plan, s);
-  if (!rc) rc = penta_check_launch(&h->plan, s);
+  PentaPlan &lu = h->has64 ? h->plan64 : h->plan;   // the plan holding the complex-double LU
+  int rc = penta_import(&lu, mat_dev, sL, sR, s);
+  if (!rc) rc = penta_factor(&lu, s);
+  if (!rc) rc = penta_check_launch(&lu, s);
+  if (!rc && h->has64) rc = penta_records_narrow(&h->plan, &lu, s);
   if (!rc && h->plan.prec) rc = penta_check_records_launch(&h->plan, s);
   long bad = 0, first = -1;
   if (!rc) rc = penta_check_result(&h->plan, s, &bad, &first);
+  if (!rc && h->has64) {
+    long bad64 = 0, first64 = -1;
+    rc = penta_check_result(&lu, s, &bad64, &first64);
+    if (bad64 && (first < 0 || first64 < first)) first = first64;
+    bad = std::max(bad, bad64);
+  }
   if (rc) return rc;
   h->factored = 1;
   if (bad) {
@@ -111,7 +130,7 @@ extern "C" int lns3d_penta_solve_reference_order(lns3d_penta *h, void *rhs_dev, 
   if (!h->factored) return fail(LNS3D_ENOTFACTORED, "not factored");
   long sL, sR;
   ref_strides(h, &sL, &sR);
-  return penta_solve_reference_order(&h->plan, rhs_dev, sL, sR, (hipStream_t)stream);
+  return penta_solve_reference_order(h->has64 ? &h->plan64 : &h->plan, rhs_dev, sL, sR, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------

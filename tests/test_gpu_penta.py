@@ -93,6 +93,32 @@ def test_periodic_random(torch_cuda, orc, layout, nsys, np_, exact):
     h.close()
 
 
+@pytest.mark.parametrize("exact", [0, 1])
+@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("nsys,np_", [(4, 17), (33, 65), (520, 24)])
+def test_periodic_complex_float(torch_cuda, orc, layout, nsys, np_, exact):
+    """periodic lines with complex-float records (rounded from the handle's
+    complex-double bordered LU; one line per workgroup, four above 512
+    lines): one solve agrees with the oracle's cpenta1p/cpenta2p of the same
+    factor (shipped or exact) to single-precision accuracy"""
+    from lns3d_amd.native import Penta
+    from tests.penta_cases import random_lines, relerr
+    print(f"periodic complex float layout={layout} nsys={nsys} np={np_} exact={exact}", flush=True)
+    mat, rhs = random_lines(nsys, np_, layout, seed=11 * nsys + np_)
+    m_ref, x_ref = mat.copy(order="F"), rhs.copy(order="F")
+    per = np.zeros((2, 5, 5, nsys, np_) if layout == 1 else (2, 5, 5, np_, nsys), complex, order="F")
+    per2 = per.copy(order="F")
+    orc.cpenta_p(layout, m_ref, x_ref, per, per2, 2, exact)
+    h = Penta(1, layout, nsys, np_, 1, exact)
+    dm, dr = _dev(torch_cuda, mat), _dev(torch_cuda, rhs)
+    h.factor(dm)
+    h.solve(dr)
+    e = relerr(_host(dr, rhs.shape), x_ref)
+    print(f"  rel diff {e:.2e}", flush=True)
+    assert e < 1e-5
+    h.close()
+
+
 # complex-float twisted lines take chain steps in pairs (12-row chunks): the
 # sizes include the shortest twisted lines (no pairs), pair regions ending
 # inside a chunk, several chunks, and many lines (no pairs: plain variants)
