@@ -410,8 +410,11 @@ int lns3d_ctx_bytes(const lns3d_params *p, size_t *bytes);
  * slot, with the bits of the per-context steps.  A mode-3 graph's tables are
  * copied to the device on the stream of its first launch, ahead of it
  * (creating the graph enqueues no device work).  dv_dev may be NULL.
- * Launch on `stream` (NULL: the graph's own stream).  No counterpart in the
- * reference. */
+ * Launch on `stream` (NULL: the graph's own stream).  Every captured step
+ * must be alike: updateLHS with useCalcd is refused, impl = 2 and corrector
+ * iterations take mode 0 (impl = 2 from its third step, with corrector
+ * iterations an even nsteps: the history buffers trade places each step).
+ * No counterpart in the reference. */
 typedef struct lns3d_graph lns3d_graph;
 int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev, int count, int nsteps, int mode,
                        int corrections, lns3d_graph **out);

@@ -1032,8 +1032,20 @@ extern "C" int lns3d_graph_create(lns3d_ctx **ctxs, void **v_dev, void **dv_dev,
   for (int i = 0; i < count; i++) {
     if (!ctxs[i] || !ctxs[i]->factored || !v_dev[i] || (mode > 0 && !ctxs[i]->have32))
       return fail(LNS3D_EINVAL, "lns3d_graph_create: every context must be factored (precision 1 for modes 1, 2)");
-    if (ctxs[i]->impl == 2 || ctxs[i]->niter > 1 || (ctxs[i]->p.update_lhs && ctxs[i]->p.use_calcd))
-      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2, corrector iterations and updateLHS with useCalcd take lns3d_step");
+    const lns3d_ctx *c = ctxs[i];
+    // the captured steps must all be alike: updateLHS with useCalcd forms
+    // factors on the host between steps; the complex-float steps refuse
+    // impl = 2 and corrector iterations
+    if ((c->p.update_lhs && c->p.use_calcd) || (mode > 0 && (c->impl == 2 || c->niter > 1)))
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: updateLHS with useCalcd, and impl = 2 or corrector "
+                                      "iterations outside mode 0, take lns3d_step");
+    // impl = 2: from the step after the first two (the history exists and
+    // the factors hold alfa = 2/3); with corrector iterations the history
+    // buffers trade places each step, so the graph holds an even count
+    if (c->impl == 2 && (c->lstep < 2 || c->p.alfa != 6.6666666666666666666e-1))
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2 graphs start after the first two steps");
+    if (c->impl == 2 && c->niter > 1 && nsteps % 2)
+      return fail(LNS3D_EUNSUPPORTED, "lns3d_graph_create: impl = 2 with corrector iterations takes an even step count");
   }
   if (mode == 3) return graph_create_batched(ctxs, v_dev, dv_dev, count, nsteps, corrections, out);
   auto *gr = new lns3d_graph;

@@ -156,6 +156,51 @@ def test_staged_periodic_matches_branches(torch_cuda, corrections):
     b.close()
 
 
+@pytest.mark.parametrize("impl,niter", [(1, 2), (2, 1), (2, 2), (3, 1)])
+def test_graph_time_schemes_match_eager(torch_cuda, impl, niter):
+    """complex-double graphs (mode 0) of steps with corrector iterations,
+    impl = 2 (after its first two steps: the history exists and the factors
+    hold alfa = 2/3) and impl = 3: two replays in a row equal the same steps
+    launched one by one, bit for bit"""
+    from lns3d_amd.config import Params, make_grid, make_mean, make_disturbance
+    from lns3d_amd.native import Step, Graph, MODE_FP64
+    torch = torch_cuda
+    print(f"graph impl={impl} niter={niter}", flush=True)
+    p = Params(nx=33, ny=17, kz=0.3)
+    x, y, met = make_grid(p)
+    vm = make_mean(p, y)
+    v0 = make_disturbance(p, x, seed=4)
+
+    def make():
+        st = Step(p)
+        st.setup(met, vm)
+        st.set_time_scheme(impl)
+        st.set_corrector_iterations(niter)
+        st.factor()
+        v = torch.from_numpy(np.ascontiguousarray(v0.reshape(-1, order="F"))).cuda()
+        st.itrbc3d(v)
+        dv = torch.zeros_like(v)
+        for _ in range(2):
+            st.step(v, dv)
+        return st, v, dv
+
+    sa, va, dva = make()
+    sb, vb, dvb = make()
+    assert torch.equal(va, vb)
+    nsteps = 2
+    g = Graph([sb], [vb], [dvb], nsteps=nsteps, mode=MODE_FP64)
+    for rep in range(2):
+        for _ in range(nsteps):
+            sa.step(va, dva)
+        g.launch()
+        g.synchronize()
+        assert bool(torch.isfinite(vb).all())
+        assert torch.equal(va, vb), rep
+    g.close()
+    sa.close()
+    sb.close()
+
+
 def test_refined_fixed_matches_fp64(torch_cuda):
     from lns3d_amd.config import Params, make_grid, make_mean, make_disturbance
     from lns3d_amd.native import Step
